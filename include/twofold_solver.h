@@ -1,0 +1,226 @@
+/*
+ * twofold_solver.h - C ABI of the MI355X soft-step contact solver.
+ *
+ * The solver replaces the solve block of the reference engine's step (src/world.cpp:1591-1691:
+ * step state, active/fast lists, SolverContext, prepare, impact marking, colouring, row packing,
+ * the substep loop with warm start, biased solve, position integration and relax, restitution and
+ * the final speed limit) and the island/sleep pass that follows it (src/world.cpp:1174-1247).
+ * Everything on that path runs as gfx950 HIP kernels on state that stays resident in HBM; the
+ * host calls in through the functions below with plain pointers and sizes.
+ *
+ * Conventions (the reference has no FFI; these follow its C++ conventions where it has one):
+ *  - Every function returns 0 on success or a negative TFS_E_* code. tfs_last_error() gives the
+ *    message of the last failure on a context. No C++ exception crosses this boundary.
+ *  - A context is one world. It is not thread safe. All work is ordered on the context's stream.
+ *  - The library owns device memory; the caller owns host arrays.
+ *  - A non-finite or non-positive dt is ignored, as World::step ignores it (src/world.cpp:1565).
+ */
+#ifndef TWOFOLD_SOLVER_H
+#define TWOFOLD_SOLVER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TFS_ABI_VERSION 1
+
+enum {
+    TFS_OK = 0,
+    TFS_E_ARG = -1,      /* bad argument: null pointer, size out of range, body index out of range */
+    TFS_E_DEVICE = -2,   /* a HIP call failed */
+    TFS_E_STATE = -3,    /* call out of order (e.g. step before upload) */
+    TFS_E_CALLBACK = -4, /* the refresh callback returned an error */
+    TFS_E_KERNEL = -5    /* a kernel reported an internal error (e.g. a bounded wait ran out) */
+};
+
+/* Motion types, as twofold::MotionType (include/twofold/world.h:37). */
+enum { TFS_STATIC = 0, TFS_KINEMATIC = 1, TFS_DYNAMIC = 2 };
+
+/* Lock flags, as twofold::LockFlags (include/twofold/world.h:40-50). */
+enum {
+    TFS_LOCK_TX = 1, TFS_LOCK_TY = 2, TFS_LOCK_TZ = 4,
+    TFS_LOCK_RX = 8, TFS_LOCK_RY = 16, TFS_LOCK_RZ = 32,
+    TFS_LOCK_PLANAR = 4 | 8 | 16
+};
+
+/* Colour count; contacts that get none take TFS_COLORS (src/internal.h:24, 620). */
+#define TFS_COLORS 32
+
+/*
+ * A body as the solver sees it: the fields of twofold::Body (src/internal.h:257-325) that the solve
+ * block, the step-state set-up and the sleep pass read or write. Quaternions are (x, y, z, w);
+ * matrices are row-major. Fields marked "out" are recomputed by tfs_step at step start and need not
+ * be valid on upload.
+ */
+typedef struct tfs_body {
+    float p[3];                 /* xf.p */
+    float q[4];                 /* xf.q */
+    float com[3];
+    float local_center[3];
+    float v[3];
+    float w[3];
+    float force[3];
+    float torque[3];
+    float inv_mass;
+    float inertia[9];           /* local inertia about the centre of mass */
+    float lock_frame[4];
+    float linear_damping;
+    float angular_damping;
+    float gravity_scale;
+    float extent;               /* reach of the shapes from the centre of mass */
+    float sleep_time;
+    float kept;
+    float inv_mass_world[9];    /* out */
+    float inv_inertia_world[9]; /* out */
+    float linear_proj[9];       /* out */
+    float angular_proj[9];      /* out */
+    float plane_inv_mass;       /* out */
+    float plane_inv_inertia;    /* out */
+    float dp[3];                /* out */
+    float dq[4];                /* out */
+    float q_base[4];            /* out */
+    float q0[4];                /* out */
+    float com0[3];              /* out */
+    float expected_travel;      /* out */
+    uint32_t island;            /* out: lowest body index of the island, UINT32_MAX when not awake dynamic */
+    uint8_t type;               /* TFS_STATIC / TFS_KINEMATIC / TFS_DYNAMIC */
+    uint8_t locks;              /* TFS_LOCK_* */
+    uint8_t awake;
+    uint8_t can_sleep;
+    uint8_t alive;
+    uint8_t pad_[3];
+} tfs_body;
+
+/* A contact point, as twofold::ContactPoint (src/internal.h:428-440) less the surface points the
+ * narrowphase keeps for itself. */
+typedef struct tfs_point {
+    float local_a[3];
+    float anchor_a[3];
+    float anchor_b[3];
+    float separation;
+    float adjusted_separation;  /* out */
+    float normal_impulse;
+    float tangent_impulse[3];
+    float max_normal_impulse;   /* out */
+    float relative_velocity;    /* out */
+} tfs_point;
+
+/* A contact manifold, as twofold::Contact (src/internal.h:463-503) less the narrowphase's own
+ * reuse data. Contacts are in key order, as the reference keeps them (src/world.cpp:985-988). */
+typedef struct tfs_contact {
+    uint64_t key;
+    uint32_t body_a;
+    uint32_t body_b;
+    float normal[3];
+    float tangent1[3];
+    float tangent2[3];
+    float friction;
+    float restitution;
+    float rolling_resistance;
+    float rolling_radius;       /* out */
+    float rolling_impulse[3];
+    float plane[3];             /* out */
+    int32_t count;
+    uint8_t planar;             /* out */
+    uint8_t laned;              /* out */
+    uint8_t pad_[2];
+    tfs_point points[4];
+} tfs_contact;
+
+/* The subset of twofold::WorldSettings (include/twofold/world.h:216-241) and Tolerances
+ * (src/internal.h:348-351) the path reads. tfs_default_settings() fills the reference defaults. */
+typedef struct tfs_settings {
+    float gravity[3];
+    float length_unit;
+    int32_t substeps;
+    float contact_hertz;
+    float contact_damping_ratio;
+    float max_pushout_speed;
+    float max_linear_speed;
+    float max_angular_speed;
+    float restitution_threshold;
+    float sleep_threshold;
+    float time_to_sleep;
+    float speculative;          /* Tolerances::speculative */
+} tfs_settings;
+
+/*
+ * The refresh callback. Before substep i > 0 of a step, when the list `again` (fast contacts at
+ * i == substeps/2, else fast contacts between bodies that are not planar) is not empty, the
+ * reference collides those pairs again (src/world.cpp:1657-1668, 1057-1139). The solver does the
+ * solver side of that (inverse mass update, rebase of the other active contacts, reset of dp/dq,
+ * then prepare, nearest-point carry and impulse carry of the new points) and asks the host for the
+ * new manifolds through this callback.
+ *
+ * On entry, bodies[0..n_bodies) holds the current body state and contacts[again[k]] the contact
+ * before the refresh. For each k the callback writes new[k]: normal, tangent1, tangent2, count and
+ * per point local_a, separation, anchor_a, anchor_b (other fields are ignored); count 0 drops the
+ * points. `remaining` is the time left in the step, (substeps - i) * h. Returns 0 or an error.
+ */
+typedef int (*tfs_refresh_fn)(void* user, int substep, float remaining, const uint32_t* again,
+                              uint32_t n_again, const tfs_body* bodies, uint32_t n_bodies,
+                              const tfs_contact* contacts, uint32_t n_contacts,
+                              tfs_contact* fresh);
+
+typedef struct tfs_ctx tfs_ctx;
+
+/* Version of this ABI (TFS_ABI_VERSION) and struct sizes, for bindings to check their layouts. */
+int tfs_abi_version(void);
+size_t tfs_sizeof_body(void);
+size_t tfs_sizeof_contact(void);
+size_t tfs_sizeof_settings(void);
+
+/* The reference's default settings (include/twofold/world.h:216-241; src/internal.h:348-351). */
+void tfs_default_settings(tfs_settings* out);
+
+/* Creates a context on HIP device `device` with room for the given counts (they can grow later by
+ * uploading more). Replaces the construction of World::Impl's solver state (src/world.cpp:145-165). */
+int tfs_create(int device, uint32_t max_bodies, uint32_t max_contacts, tfs_ctx** out);
+void tfs_destroy(tfs_ctx* ctx);
+const char* tfs_last_error(const tfs_ctx* ctx);
+
+int tfs_set_settings(tfs_ctx* ctx, const tfs_settings* settings);
+
+/* Host arrays in, resident device state out. Bodies are indexed as the reference's bodies vector;
+ * contacts in key order as the reference's contacts vector after its narrowphase
+ * (src/world.cpp:752-993). A contact's body indices must be < n_bodies. */
+int tfs_upload_bodies(tfs_ctx* ctx, const tfs_body* bodies, uint32_t n);
+int tfs_upload_contacts(tfs_ctx* ctx, const tfs_contact* contacts, uint32_t n);
+
+/*
+ * One step of the path: src/world.cpp:1591-1691 followed by updateSleep (src/world.cpp:1174-1247)
+ * over the resident bodies and contacts. `refresh` may be NULL when no refresh is needed (then a
+ * step that would need one fails with TFS_E_CALLBACK before changing any state).
+ */
+int tfs_step(tfs_ctx* ctx, float dt, tfs_refresh_fn refresh, void* user);
+
+int tfs_download_bodies(tfs_ctx* ctx, tfs_body* out, uint32_t n);
+int tfs_download_contacts(tfs_ctx* ctx, tfs_contact* out, uint32_t n);
+
+/*
+ * The integer results of the last step, for parity checks against the reference:
+ *   kind 0: active contacts (src/world.cpp:1608-1621)            kind 1: fast contacts
+ *   kind 2: fast contacts without a planar body                 kind 3: impact contacts (1029-1049)
+ *   kind 4: active contacts in row order, colour by colour, then the rest (lanes.cpp:376-405)
+ *   kind 5: impact contacts in row order
+ *   kind 6: colour of each contact, one byte per contact index (TFS_COLORS: none; lanes.cpp:356-374)
+ * Writes up to `cap` entries (bytes for kind 6, uint32 otherwise) and the full count to *count.
+ */
+int tfs_download_list(tfs_ctx* ctx, int kind, void* out, uint32_t cap, uint32_t* count);
+
+/* Solve-stage timing of the last step in milliseconds, measured with HIP events on the stream,
+ * as the reference's StepProfile::solver brackets it (src/world.cpp:1644-1692). */
+int tfs_last_step_ms(tfs_ctx* ctx, float* solve_ms, float* sleep_ms);
+
+/* Counts of the last step: active contacts, their points, impact contacts and their points,
+ * colours used, rest contacts. Constraint-iterations per step follow from these. */
+int tfs_last_step_counts(tfs_ctx* ctx, uint32_t counts[6]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,495 @@
+// Test infrastructure: records the inputs and outputs of the reference engine's solve block, step by
+// step, so the oracle and the HIP path can be checked against the reference itself.
+//
+// This file is compiled only by oracle/Makefile, from the reference sources where they lie. It
+// includes the reference's src/world.cpp into this translation unit with the solver's free
+// functions renamed (one macro each), so World::Impl::step calls the recorders below, which record
+// and then call the reference's own functions. No reference source is copied or changed.
+//
+// Hooked calls in World::Impl::step (src/world.cpp): prepareContacts 1649 (solve-block input),
+// colorContacts 1651, batchContacts 1656-1657 (colours, row order, impact list), warmStartBatches
+// 1672 (substep counter), prepareContact 1126 (the fresh manifold of a refreshed fast contact),
+// prepareLanes 1666 (end of a refresh), applyRestitution 1683 + limitSpeed 1157 (end of the solve
+// block, after the final limitSpeeds at 1684).
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <iterator>
+#include <limits>
+#include <map>
+#include <memory>
+#include <utility>
+#include <variant>
+#include <vector>
+
+// World keeps its state behind a private pimpl; the recorder reads it directly.
+#define private public
+#include "twofold/world.h"
+#undef private
+
+#define prepareContacts tfcap_prepareContacts
+#define colorContacts tfcap_colorContacts
+#define batchContacts tfcap_batchContacts
+#define prepareLanes tfcap_prepareLanes
+#define prepareContact tfcap_prepareContact
+#define warmStartBatches tfcap_warmStartBatches
+#define applyRestitution tfcap_applyRestitution
+#define limitSpeed tfcap_limitSpeed
+#include "world.cpp"
+#undef prepareContacts
+#undef colorContacts
+#undef batchContacts
+#undef prepareLanes
+#undef prepareContact
+#undef warmStartBatches
+#undef applyRestitution
+#undef limitSpeed
+
+#include "../include/twofold_solver.h"
+
+namespace twofold {
+// The reference's solver entry points under their own names (src/internal.h:627-647); the macros
+// above renamed the declarations world.cpp saw.
+void prepareContacts(SolverData& s, const std::vector<uint32_t>& active);
+void colorContacts(SolverData& s, const std::vector<uint32_t>& list, ContactColors& colors);
+void batchContacts(SolverData& s, const std::vector<uint32_t>& list, const ContactColors& colors,
+                   const SolverContext& ctx, ContactBatches& out);
+void prepareLanes(SolverData& s, const SolverContext& ctx, ContactBatches& batches);
+void prepareContact(const Body& a, const Body& b, Contact& c, PlanarAxes* planar);
+void warmStartBatches(SolverData& s, const ContactBatches& batches);
+void applyRestitution(SolverData& s, const std::vector<uint32_t>& active, const SolverContext& ctx);
+void limitSpeed(Body& b, float maxSpeed, float maxSpin);
+}
+
+using namespace twofold;
+
+namespace {
+
+void put3(float* d, const Vec3& v) { d[0] = v.x; d[1] = v.y; d[2] = v.z; }
+void put4(float* d, const Quat& q) { d[0] = q.x; d[1] = q.y; d[2] = q.z; d[3] = q.w; }
+void put9(float* d, const Mat3& m) {
+    for (int r = 0; r < 3; ++r) { put3(d + 3 * r, m.r[r]); }
+}
+
+tfs_body toBody(const Body& b) {
+    tfs_body o;
+    std::memset(&o, 0, sizeof o);
+    put3(o.p, b.xf.p);
+    put4(o.q, b.xf.q);
+    put3(o.com, b.com);
+    put3(o.local_center, b.localCenter);
+    put3(o.v, b.v);
+    put3(o.w, b.w);
+    put3(o.force, b.force);
+    put3(o.torque, b.torque);
+    o.inv_mass = b.invMass;
+    put9(o.inertia, b.inertia);
+    put4(o.lock_frame, b.lockFrame);
+    o.linear_damping = b.linearDamping;
+    o.angular_damping = b.angularDamping;
+    o.gravity_scale = b.gravityScale;
+    o.extent = b.extent;
+    o.sleep_time = b.sleepTime;
+    o.kept = b.kept;
+    put9(o.inv_mass_world, b.invMassWorld);
+    put9(o.inv_inertia_world, b.invInertiaWorld);
+    put9(o.linear_proj, b.linearProj);
+    put9(o.angular_proj, b.angularProj);
+    o.plane_inv_mass = b.planeInvMass;
+    o.plane_inv_inertia = b.planeInvInertia;
+    put3(o.dp, b.dp);
+    put4(o.dq, b.dq);
+    put4(o.q_base, b.qBase);
+    put4(o.q0, b.q0);
+    put3(o.com0, b.com0);
+    o.expected_travel = b.expectedTravel;
+    o.island = b.island;
+    o.type = (uint8_t)b.type;
+    o.locks = b.locks;
+    o.awake = b.awake ? 1 : 0;
+    o.can_sleep = b.canSleep ? 1 : 0;
+    o.alive = b.alive ? 1 : 0;
+    return o;
+}
+
+tfs_contact toContact(const Contact& c) {
+    tfs_contact o;
+    std::memset(&o, 0, sizeof o);
+    o.key = c.key;
+    o.body_a = c.bodyA;
+    o.body_b = c.bodyB;
+    put3(o.normal, c.normal);
+    put3(o.tangent1, c.tangent1);
+    put3(o.tangent2, c.tangent2);
+    o.friction = c.friction;
+    o.restitution = c.restitution;
+    o.rolling_resistance = c.rollingResistance;
+    o.rolling_radius = c.rollingRadius;
+    put3(o.rolling_impulse, c.rollingImpulse);
+    put3(o.plane, c.plane);
+    o.count = c.count;
+    o.planar = c.planar ? 1 : 0;
+    o.laned = c.laned ? 1 : 0;
+    for (int i = 0; i < 4; ++i) {
+        const ContactPoint& p = c.points[i];
+        tfs_point& q = o.points[i];
+        put3(q.local_a, p.localA);
+        put3(q.anchor_a, p.anchorA);
+        put3(q.anchor_b, p.anchorB);
+        q.separation = p.separation;
+        q.adjusted_separation = p.adjustedSeparation;
+        q.normal_impulse = p.normalImpulse;
+        put3(q.tangent_impulse, p.tangentImpulse);
+        q.max_normal_impulse = p.maxNormalImpulse;
+        q.relative_velocity = p.relativeVelocity;
+    }
+    return o;
+}
+
+struct Refresh {
+    int substep = 0;
+    float remaining = 0.0f;
+    std::vector<uint32_t> again;
+    std::vector<tfs_contact> fresh;
+    std::map<uint32_t, tfs_contact> seen;
+};
+
+struct Capture {
+    bool solved = false;  // the step reached the solve block
+    std::vector<tfs_body> preBodies, postBodies, stepBodies;
+    std::vector<tfs_contact> preContacts, postContacts;
+    std::vector<uint32_t> active, fast, fastSolid, impact;
+    std::vector<uint32_t> byColor, rest, impactByColor, impactRest;
+    std::vector<uint8_t> colors;
+    std::vector<Refresh> refreshes;
+    float ctx[12] = {};
+};
+
+struct Handle {
+    World world;
+    Capture cap;
+    explicit Handle(const WorldSettings& s) : world(s) {}
+};
+
+Capture* g_cap = nullptr;
+World::Impl* g_impl = nullptr;
+const Contact* g_contactBase = nullptr;
+int g_substep = 0;
+int g_batchCalls = 0;
+int g_lanesCalls = 0;
+bool g_final = false;
+size_t g_limitCount = 0, g_alive = 0;
+
+}
+
+namespace twofold {
+
+void tfcap_prepareContacts(SolverData& s, const std::vector<uint32_t>& active) {
+    if (g_cap) {
+        Capture& c = *g_cap;
+        c.solved = true;
+        c.preBodies.clear();
+        for (const Body& b : s.bodies) { c.preBodies.push_back(toBody(b)); }
+        c.preContacts.clear();
+        for (const Contact& x : s.contacts) { c.preContacts.push_back(toContact(x)); }
+        c.active = active;
+        c.fast = g_impl->fastContacts;
+        c.fastSolid = g_impl->fastSolidContacts;
+        g_contactBase = s.contacts.data();
+        g_alive = 0;
+        for (const Body& b : s.bodies) { g_alive += b.alive ? 1 : 0; }
+    }
+    prepareContacts(s, active);
+}
+
+void tfcap_colorContacts(SolverData& s, const std::vector<uint32_t>& list, ContactColors& colors) {
+    colorContacts(s, list, colors);
+    if (g_cap) { g_cap->colors = colors.of; }
+}
+
+void tfcap_batchContacts(SolverData& s, const std::vector<uint32_t>& list, const ContactColors& colors,
+                         const SolverContext& ctx, ContactBatches& out) {
+    batchContacts(s, list, colors, ctx, out);
+    if (!g_cap) { return; }
+    Capture& c = *g_cap;
+    if (g_batchCalls == 0) {
+        c.byColor = out.byColor;
+        c.rest = out.rest;
+        const float v[12] = { ctx.h, ctx.invH, ctx.contactSoft.biasRate, ctx.contactSoft.massScale,
+                              ctx.contactSoft.impulseScale, ctx.staticSoft.biasRate, ctx.staticSoft.massScale,
+                              ctx.staticSoft.impulseScale, ctx.maxPushout, ctx.restitutionThreshold, 0.0f, 0.0f };
+        std::memcpy(c.ctx, v, sizeof v);
+    } else {
+        c.impact = list;
+        c.impactByColor = out.byColor;
+        c.impactRest = out.rest;
+    }
+    ++g_batchCalls;
+}
+
+void tfcap_warmStartBatches(SolverData& s, const ContactBatches& batches) {
+    warmStartBatches(s, batches);
+    ++g_substep;
+}
+
+void tfcap_prepareContact(const Body& a, const Body& b, Contact& c, PlanarAxes* planar) {
+    if (g_cap) {
+        Capture& cap = *g_cap;
+        if (cap.refreshes.empty() || cap.refreshes.back().substep != g_substep) {
+            Refresh r;
+            r.substep = g_substep;
+            cap.refreshes.push_back(r);
+        }
+        const uint32_t ci = (uint32_t)(&c - g_contactBase);
+        cap.refreshes.back().seen[ci] = toContact(c);
+    }
+    prepareContact(a, b, c, planar);
+}
+
+void tfcap_prepareLanes(SolverData& s, const SolverContext& ctx, ContactBatches& batches) {
+    prepareLanes(s, ctx, batches);
+    if (!g_cap) { return; }
+    if (g_lanesCalls % 2 == 0) {
+        Capture& cap = *g_cap;
+        if (cap.refreshes.empty() || cap.refreshes.back().substep != g_substep) {
+            Refresh r;
+            r.substep = g_substep;
+            cap.refreshes.push_back(r);
+        }
+        Refresh& r = cap.refreshes.back();
+        const int n = std::max(1, g_impl->settings.substeps);
+        r.again = g_substep == n / 2 ? g_impl->fastContacts : g_impl->fastSolidContacts;
+        r.remaining = (float)(n - g_substep) * ctx.h;
+        for (uint32_t ci : r.again) {
+            auto it = r.seen.find(ci);
+            if (it != r.seen.end()) {
+                r.fresh.push_back(it->second);
+            } else {
+                // No manifold was found: the contact keeps its fields and loses its points.
+                tfs_contact x = toContact(s.contacts[ci]);
+                x.count = 0;
+                r.fresh.push_back(x);
+            }
+        }
+        r.seen.clear();
+    }
+    ++g_lanesCalls;
+}
+
+void tfcap_applyRestitution(SolverData& s, const std::vector<uint32_t>& active, const SolverContext& ctx) {
+    applyRestitution(s, active, ctx);
+    g_final = true;
+    g_limitCount = 0;
+}
+
+void tfcap_limitSpeed(Body& b, float maxSpeed, float maxSpin) {
+    limitSpeed(b, maxSpeed, maxSpin);
+    if (!g_final) { return; }
+    if (++g_limitCount < g_alive) { return; }
+    g_final = false;
+    if (!g_cap) { return; }
+    Capture& c = *g_cap;
+    c.postBodies.clear();
+    for (const Body& x : g_impl->bodies) { c.postBodies.push_back(toBody(x)); }
+    c.postContacts.clear();
+    for (const Contact& x : g_impl->contacts) { c.postContacts.push_back(toContact(x)); }
+}
+
+}
+
+namespace {
+
+Quat quat4(const float* q) { return q ? Quat{ q[0], q[1], q[2], q[3] } : Quat{}; }
+Vec3 vec3(const float* v) { return v ? Vec3{ v[0], v[1], v[2] } : Vec3{}; }
+
+template <class T>
+uint32_t copyOut(const std::vector<T>& v, T* out, uint32_t cap) {
+    if (out) { std::memcpy(out, v.data(), sizeof(T) * std::min<size_t>(cap, v.size())); }
+    return (uint32_t)v.size();
+}
+
+}
+
+extern "C" {
+
+// Creates a world with the reference defaults overridden by `s` (NULL keeps them).
+void* tfr_create(const tfs_settings* s) {
+    WorldSettings ws;
+    if (s) {
+        ws.gravity = vec3(s->gravity);
+        ws.lengthUnit = s->length_unit;
+        ws.substeps = s->substeps;
+        ws.contactHertz = s->contact_hertz;
+        ws.contactDampingRatio = s->contact_damping_ratio;
+        ws.maxPushoutSpeed = s->max_pushout_speed;
+        ws.maxLinearSpeed = s->max_linear_speed;
+        ws.maxAngularSpeed = s->max_angular_speed;
+        ws.restitutionThreshold = s->restitution_threshold;
+        ws.sleepThreshold = s->sleep_threshold;
+        ws.timeToSleep = s->time_to_sleep;
+    }
+    return new Handle(ws);
+}
+
+void tfr_destroy(void* h) { delete (Handle*)h; }
+
+// Makes `h` a copy of `from` (World's copy is a full snapshot, src/world.cpp:1732-1737).
+void tfr_copy(void* h, const void* from) { ((Handle*)h)->world = ((const Handle*)from)->world; }
+
+void tfr_set_continuous(void* h, int on) {
+    WorldSettings s = ((Handle*)h)->world.settings();
+    s.continuous = on != 0;
+    ((Handle*)h)->world.setSettings(s);
+}
+
+// type: 0 static, 1 kinematic, 2 dynamic. Returns the body index.
+int tfr_add_body(void* h, int type, const float* pos, const float* rot, const float* v, const float* w,
+                 int locks, const float* lockFrame, int awake, float linearDamping, float angularDamping,
+                 float gravityScale) {
+    BodyDesc d;
+    d.type = (MotionType)type;
+    d.position = vec3(pos);
+    d.rotation = quat4(rot);
+    d.linearVelocity = vec3(v);
+    d.angularVelocity = vec3(w);
+    d.locks = (uint8_t)locks;
+    d.lockFrame = quat4(lockFrame);
+    d.awake = awake != 0;
+    d.linearDamping = linearDamping;
+    d.angularDamping = angularDamping;
+    d.gravityScale = gravityScale;
+    return (int)((Handle*)h)->world.createBody(d).index;
+}
+
+// kind: 0 sphere {r}, 1 box {hx, hy, hz}, 2 cylinder {r, half height}, 3 prism {zMin, zMax, x0, y0, ...},
+// 4 half-space {}. Returns the shape index or -1.
+int tfr_add_shape(void* h, int body, int kind, const float* dims, int ndims, const float* pos, const float* rot,
+                  float density, float friction, float restitution, float rolling) {
+    World& wd = ((Handle*)h)->world;
+    ShapeDesc sd;
+    switch (kind) {
+        case 0: sd.geometry = Sphere{ dims[0] }; break;
+        case 1: sd.geometry = Box{ { dims[0], dims[1], dims[2] } }; break;
+        case 2: sd.geometry = Cylinder{ dims[0], dims[1] }; break;
+        case 3: {
+            Prism p;
+            p.zMin = dims[0];
+            p.zMax = dims[1];
+            for (int i = 2; i + 1 < ndims; i += 2) { p.polygon.push_back({ dims[i], dims[i + 1] }); }
+            sd.geometry = p;
+            break;
+        }
+        case 4: sd.geometry = HalfSpace{}; break;
+        default: return -1;
+    }
+    sd.position = vec3(pos);
+    sd.rotation = quat4(rot);
+    sd.density = density;
+    sd.friction = friction;
+    sd.restitution = restitution;
+    sd.rollingResistance = rolling;
+    if (body < 0 || (size_t)body >= wd.impl_->bodies.size()) { return -1; }
+    const BodyId id{ (uint32_t)body, wd.impl_->bodies[(size_t)body].generation };
+    return (int)wd.addShape(id, sd).index;
+}
+
+// Steps the world, recording the solve block when `record` is set. Returns 1 when the step reached
+// the solve block, 0 otherwise.
+int tfr_step(void* h, float dt, int record) {
+    Handle& hd = *(Handle*)h;
+    hd.cap = Capture{};
+    g_cap = record ? &hd.cap : nullptr;
+    g_impl = hd.world.impl_.get();
+    g_substep = 0;
+    g_batchCalls = 0;
+    g_lanesCalls = 0;
+    g_final = false;
+    hd.world.step(dt);
+    if (record) {
+        for (const Body& b : g_impl->bodies) { hd.cap.stepBodies.push_back(toBody(b)); }
+    }
+    g_cap = nullptr;
+    g_impl = nullptr;
+    return hd.cap.solved ? 1 : 0;
+}
+
+// StepProfile of the last step: total, broadphase, narrowphase, solver, continuous, sleep (ms).
+void tfr_profile(void* h, float* out) {
+    const StepProfile& p = ((Handle*)h)->world.profile();
+    out[0] = p.total;
+    out[1] = p.broadphase;
+    out[2] = p.narrowphase;
+    out[3] = p.solver;
+    out[4] = p.continuous;
+    out[5] = p.sleep;
+}
+
+uint32_t tfr_num_bodies(void* h) { return (uint32_t)((Handle*)h)->world.impl_->bodies.size(); }
+uint32_t tfr_num_contacts(void* h) { return (uint32_t)((Handle*)h)->world.impl_->contacts.size(); }
+
+// Current state, outside any step.
+uint32_t tfr_bodies(void* h, tfs_body* out, uint32_t cap) {
+    std::vector<tfs_body> v;
+    for (const Body& b : ((Handle*)h)->world.impl_->bodies) { v.push_back(toBody(b)); }
+    return copyOut(v, out, cap);
+}
+uint32_t tfr_contacts(void* h, tfs_contact* out, uint32_t cap) {
+    std::vector<tfs_contact> v;
+    for (const Contact& c : ((Handle*)h)->world.impl_->contacts) { v.push_back(toContact(c)); }
+    return copyOut(v, out, cap);
+}
+
+// Recorded bodies: which 0 = at the start of the solve block, 1 = after its final limitSpeeds,
+// 2 = after the whole step (sleep included).
+uint32_t tfr_cap_bodies(void* h, int which, tfs_body* out, uint32_t cap) {
+    const Capture& c = ((Handle*)h)->cap;
+    return copyOut(which == 0 ? c.preBodies : which == 1 ? c.postBodies : c.stepBodies, out, cap);
+}
+
+// Recorded contacts: which 0 = at the start of the solve block, 1 = after it.
+uint32_t tfr_cap_contacts(void* h, int which, tfs_contact* out, uint32_t cap) {
+    const Capture& c = ((Handle*)h)->cap;
+    return copyOut(which == 0 ? c.preContacts : c.postContacts, out, cap);
+}
+
+// Recorded lists: 0 active, 1 fast, 2 fastSolid, 3 impact, 4 byColor, 5 rest, 6 impact byColor,
+// 7 impact rest.
+uint32_t tfr_cap_list(void* h, int which, uint32_t* out, uint32_t cap) {
+    const Capture& c = ((Handle*)h)->cap;
+    const std::vector<uint32_t>* lists[8] = { &c.active, &c.fast, &c.fastSolid, &c.impact,
+                                              &c.byColor, &c.rest, &c.impactByColor, &c.impactRest };
+    if (which < 0 || which > 7) { return 0; }
+    return copyOut(*lists[which], out, cap);
+}
+
+// Colour of each contact index (kColors: none). Entries of inactive contacts are 255.
+uint32_t tfr_cap_colors(void* h, uint8_t* out, uint32_t cap) {
+    Capture& c = ((Handle*)h)->cap;
+    std::vector<uint8_t> v(c.preContacts.size(), 255);
+    for (uint32_t ci : c.active) { v[ci] = c.colors[ci]; }
+    return copyOut(v, out, cap);
+}
+
+// The SolverContext of the step: h, invH, contact soft (3), static soft (3), maxPushout, restitution threshold.
+void tfr_cap_ctx(void* h, float* out) { std::memcpy(out, ((Handle*)h)->cap.ctx, sizeof(float) * 10); }
+
+uint32_t tfr_cap_num_refreshes(void* h) { return (uint32_t)((Handle*)h)->cap.refreshes.size(); }
+
+// Refresh k: its substep and remaining time, and the again list with the fresh manifold of each entry.
+uint32_t tfr_cap_refresh(void* h, uint32_t k, int* substep, float* remaining, uint32_t* again,
+                         tfs_contact* fresh, uint32_t cap) {
+    const Refresh& r = ((Handle*)h)->cap.refreshes[k];
+    if (substep) { *substep = r.substep; }
+    if (remaining) { *remaining = r.remaining; }
+    copyOut(r.again, again, cap);
+    return copyOut(r.fresh, fresh, cap);
+}
+
+size_t tfr_sizeof_body(void) { return sizeof(tfs_body); }
+size_t tfr_sizeof_contact(void) { return sizeof(tfs_contact); }
+
+}

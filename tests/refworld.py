@@ -1,0 +1,256 @@
+"""ctypes wrappers for the two checkers under oracle/ (test infrastructure).
+
+RefWorld drives the reference engine built from its own sources (oracle/_ref/libtwofold_ref.so) and
+reads back what its solve block saw and did each step. Oracle runs the CPU restatement
+(oracle/liboracle.so) on the same POD arrays the HIP path takes.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from twofold_amd import _abi
+from twofold_amd._abi import BODY_DTYPE, CONTACT_DTYPE
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libtwofold_ref.so")
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+
+_fp = ctypes.POINTER(ctypes.c_float)
+_vp = ctypes.c_void_p
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_vp) if a is not None else None
+
+
+def _f(v):
+    return np.ascontiguousarray(v, np.float32).ctypes.data_as(_fp)
+
+
+_ref = None
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def _load_ref():
+    global _ref
+    if _ref is None:
+        lib = ctypes.CDLL(REF_SO)
+        lib.tfr_create.restype = _vp
+        lib.tfr_create.argtypes = [_vp]
+        lib.tfr_destroy.argtypes = [_vp]
+        lib.tfr_copy.argtypes = [_vp, _vp]
+        lib.tfr_set_continuous.argtypes = [_vp, ctypes.c_int]
+        lib.tfr_add_body.restype = ctypes.c_int
+        lib.tfr_add_body.argtypes = [_vp, ctypes.c_int, _fp, _fp, _fp, _fp, ctypes.c_int, _fp, ctypes.c_int,
+                                     ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        lib.tfr_add_shape.restype = ctypes.c_int
+        lib.tfr_add_shape.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _fp, ctypes.c_int, _fp, _fp,
+                                      ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        lib.tfr_step.restype = ctypes.c_int
+        lib.tfr_step.argtypes = [_vp, ctypes.c_float, ctypes.c_int]
+        lib.tfr_profile.argtypes = [_vp, _fp]
+        for name in ("tfr_num_bodies", "tfr_num_contacts", "tfr_cap_num_refreshes"):
+            getattr(lib, name).restype = ctypes.c_uint32
+            getattr(lib, name).argtypes = [_vp]
+        for name in ("tfr_bodies", "tfr_contacts"):
+            getattr(lib, name).restype = ctypes.c_uint32
+            getattr(lib, name).argtypes = [_vp, _vp, ctypes.c_uint32]
+        for name in ("tfr_cap_bodies", "tfr_cap_contacts", "tfr_cap_list"):
+            getattr(lib, name).restype = ctypes.c_uint32
+            getattr(lib, name).argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_uint32]
+        lib.tfr_cap_colors.restype = ctypes.c_uint32
+        lib.tfr_cap_colors.argtypes = [_vp, _vp, ctypes.c_uint32]
+        lib.tfr_cap_ctx.argtypes = [_vp, _fp]
+        lib.tfr_cap_refresh.restype = ctypes.c_uint32
+        lib.tfr_cap_refresh.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int), _fp, _vp, _vp,
+                                        ctypes.c_uint32]
+        _abi.check_sizes(lib, "tfr")
+        _ref = lib
+    return _ref
+
+
+class StepRecord:
+    """What the reference's solve block saw and did in one step."""
+
+    __slots__ = ("pre_bodies", "pre_contacts", "post_bodies", "post_contacts", "step_bodies", "lists",
+                 "colors", "ctx", "refreshes", "dt")
+
+
+class RefWorld:
+    """The reference engine, one world, built from a scene recipe."""
+
+    def __init__(self, scene, settings=None, continuous=True):
+        self.lib = _load_ref()
+        s = settings if settings is not None else _abi.default_settings(substeps=scene.substeps)
+        self.settings = s
+        self.h = self.lib.tfr_create(s.ctypes.data_as(_vp))
+        self.dt = scene.dt
+        if not continuous:
+            self.lib.tfr_set_continuous(self.h, 0)
+        for b in scene.bodies:
+            self.lib.tfr_add_body(self.h, b["type"], _f(b["pos"]), _f(b["rot"]), _f(b["v"]), _f(b["w"]),
+                                  b["locks"], _f(b["lock_frame"]), 1 if b["awake"] else 0,
+                                  b["linear_damping"], b["angular_damping"], b["gravity_scale"])
+        for sh in scene.shapes:
+            dims = sh["dims"] if len(sh["dims"]) else np.zeros(1, np.float32)
+            r = self.lib.tfr_add_shape(self.h, sh["body"], sh["kind"], _f(dims), len(sh["dims"]), _f(sh["pos"]),
+                                       _f(sh["rot"]), sh["density"], sh["friction"], sh["restitution"],
+                                       sh["rolling"])
+            if r < 0:
+                raise ValueError("shape rejected")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.tfr_destroy(self.h)
+            self.h = None
+
+    def step(self, dt=None, record=True):
+        dt = self.dt if dt is None else np.float32(dt)
+        solved = self.lib.tfr_step(self.h, dt, 1 if record else 0)
+        if not record:
+            return None
+        if not solved:
+            return None
+        return self._record(dt)
+
+    def profile(self):
+        out = np.zeros(6, np.float32)
+        self.lib.tfr_profile(self.h, out.ctypes.data_as(_fp))
+        return dict(zip(("total", "broadphase", "narrowphase", "solver", "continuous", "sleep"), out.tolist()))
+
+    def bodies(self):
+        n = self.lib.tfr_num_bodies(self.h)
+        out = np.zeros(n, BODY_DTYPE)
+        self.lib.tfr_bodies(self.h, _ptr(out), n)
+        return out
+
+    def contacts(self):
+        n = self.lib.tfr_num_contacts(self.h)
+        out = np.zeros(n, CONTACT_DTYPE)
+        self.lib.tfr_contacts(self.h, _ptr(out), n)
+        return out
+
+    def _record(self, dt):
+        lib, h = self.lib, self.h
+        r = StepRecord()
+        r.dt = dt
+        nb = lib.tfr_cap_bodies(h, 0, None, 0)
+        nc = lib.tfr_cap_contacts(h, 0, None, 0)
+        def bodies(which):
+            a = np.zeros(nb, BODY_DTYPE)
+            lib.tfr_cap_bodies(h, which, _ptr(a), nb)
+            return a
+        def contacts(which):
+            a = np.zeros(nc, CONTACT_DTYPE)
+            lib.tfr_cap_contacts(h, which, _ptr(a), nc)
+            return a
+        r.pre_bodies, r.post_bodies, r.step_bodies = bodies(0), bodies(1), bodies(2)
+        r.pre_contacts, r.post_contacts = contacts(0), contacts(1)
+        names = ("active", "fast", "fast_solid", "impact", "rows", "rest", "impact_rows", "impact_rest")
+        r.lists = {}
+        for k, name in enumerate(names):
+            n = lib.tfr_cap_list(h, k, None, 0)
+            a = np.zeros(n, np.uint32)
+            lib.tfr_cap_list(h, k, _ptr(a), n)
+            r.lists[name] = a
+        r.colors = np.zeros(nc, np.uint8)
+        lib.tfr_cap_colors(h, _ptr(r.colors), nc)
+        r.ctx = np.zeros(10, np.float32)
+        lib.tfr_cap_ctx(h, r.ctx.ctypes.data_as(_fp))
+        r.refreshes = []
+        for k in range(lib.tfr_cap_num_refreshes(h)):
+            sub = ctypes.c_int()
+            rem = np.zeros(1, np.float32)
+            n = lib.tfr_cap_refresh(h, k, ctypes.byref(sub), rem.ctypes.data_as(_fp), None, None, 0)
+            again = np.zeros(n, np.uint32)
+            fresh = np.zeros(n, CONTACT_DTYPE)
+            lib.tfr_cap_refresh(h, k, None, None, _ptr(again), _ptr(fresh), n)
+            r.refreshes.append(dict(substep=sub.value, remaining=rem[0], again=again, fresh=fresh))
+        return r
+
+
+# ---- the oracle -------------------------------------------------------------------------------
+
+REFRESH_FN = ctypes.CFUNCTYPE(ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, ctypes.POINTER(ctypes.c_uint32),
+                              ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp)
+
+
+class _Lists(ctypes.Structure):
+    _fields_ = [(n, ctypes.POINTER(ctypes.c_uint32)) for n in
+                ("active", "fast", "fast_solid", "impact", "by_color", "impact_by_color")] + [
+        ("colors", ctypes.POINTER(ctypes.c_uint8))] + [
+        (n, ctypes.c_uint32) for n in ("n_active", "n_fast", "n_fast_solid", "n_impact", "n_rest", "n_impact_rest",
+                                       "n_colors", "points_active", "points_impact")]
+
+
+_oracle = None
+
+
+def _load_oracle():
+    global _oracle
+    if _oracle is None:
+        lib = ctypes.CDLL(ORACLE_SO)
+        lib.tfo_step.restype = ctypes.c_int
+        lib.tfo_step.argtypes = [_vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, ctypes.c_float, REFRESH_FN, _vp,
+                                 ctypes.POINTER(_Lists)]
+        lib.tfo_update_inverse_mass.argtypes = [_vp]
+        lib.tfo_make_soft.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_float, _fp]
+        lib.tfo_maxf.restype = ctypes.c_float
+        lib.tfo_maxf.argtypes = [ctypes.c_float, ctypes.c_float]
+        lib.tfo_minf.restype = ctypes.c_float
+        lib.tfo_minf.argtypes = [ctypes.c_float, ctypes.c_float]
+        lib.tfo_color.argtypes = [_vp, ctypes.c_uint32, _vp, _vp, ctypes.c_uint32, _vp]
+        _oracle = lib
+    return _oracle
+
+
+def recorded_refresh(refreshes):
+    """A refresh callback that hands back the manifolds the reference's narrowphase found."""
+    by_sub = {r["substep"]: r for r in refreshes}
+
+    def cb(user, substep, remaining, again, n_again, bodies, n_bodies, contacts, n_contacts, fresh):
+        r = by_sub.get(substep)
+        if r is None or len(r["again"]) != n_again:
+            return -1
+        got = np.ctypeslib.as_array(again, shape=(n_again,))
+        if not np.array_equal(got, r["again"]):
+            return -1
+        out = np.ctypeslib.as_array(ctypes.cast(fresh, ctypes.POINTER(ctypes.c_uint8)),
+                                    shape=(n_again * CONTACT_DTYPE.itemsize,))
+        out[:] = r["fresh"].view(np.uint8)
+        return 0
+
+    return cb
+
+
+def oracle_step(bodies, contacts, settings, dt, refresh=None):
+    """Runs tfo_step in place on copies; returns (bodies, contacts, lists dict)."""
+    lib = _load_oracle()
+    b = np.ascontiguousarray(bodies.copy())
+    c = np.ascontiguousarray(contacts.copy())
+    n = max(len(c), 1)
+    arrays = {k: np.zeros(n, np.uint32) for k in ("active", "fast", "fast_solid", "impact", "by_color",
+                                                  "impact_by_color")}
+    colors = np.zeros(n, np.uint8)
+    L = _Lists()
+    for k, a in arrays.items():
+        setattr(L, k, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    L.colors = colors.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    cb = REFRESH_FN(refresh) if refresh is not None else REFRESH_FN()
+    rc = lib.tfo_step(_ptr(b), len(b), _ptr(c), len(c), settings.ctypes.data_as(_vp), np.float32(dt), cb, None,
+                      ctypes.byref(L))
+    if rc != 0:
+        raise RuntimeError(f"tfo_step failed: {rc}")
+    lists = dict(
+        active=arrays["active"][:L.n_active], fast=arrays["fast"][:L.n_fast],
+        fast_solid=arrays["fast_solid"][:L.n_fast_solid], impact=arrays["impact"][:L.n_impact],
+        rows=arrays["by_color"][:L.n_active], rest=arrays["by_color"][L.n_active - L.n_rest:L.n_active],
+        impact_rows=arrays["impact_by_color"][:L.n_impact],
+        impact_rest=arrays["impact_by_color"][L.n_impact - L.n_impact_rest:L.n_impact],
+        colors=colors[:len(contacts)], n_colors=L.n_colors, points_active=L.points_active,
+        points_impact=L.points_impact)
+    return b, c, lists

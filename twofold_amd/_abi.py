@@ -1,0 +1,98 @@
+"""numpy mirrors of the POD structs in include/twofold_solver.h.
+
+Layouts are checked at load time against the library's own ``tfs_sizeof_*`` exports, so a change
+to the header that is not mirrored here fails loudly instead of corrupting memory.
+"""
+import numpy as np
+
+F = np.float32
+U8 = np.uint8
+U32 = np.uint32
+
+BODY_DTYPE = np.dtype(
+    [
+        ("p", F, 3), ("q", F, 4), ("com", F, 3), ("local_center", F, 3),
+        ("v", F, 3), ("w", F, 3), ("force", F, 3), ("torque", F, 3),
+        ("inv_mass", F), ("inertia", F, 9), ("lock_frame", F, 4),
+        ("linear_damping", F), ("angular_damping", F), ("gravity_scale", F),
+        ("extent", F), ("sleep_time", F), ("kept", F),
+        ("inv_mass_world", F, 9), ("inv_inertia_world", F, 9),
+        ("linear_proj", F, 9), ("angular_proj", F, 9),
+        ("plane_inv_mass", F), ("plane_inv_inertia", F),
+        ("dp", F, 3), ("dq", F, 4), ("q_base", F, 4), ("q0", F, 4), ("com0", F, 3),
+        ("expected_travel", F), ("island", U32),
+        ("type", U8), ("locks", U8), ("awake", U8), ("can_sleep", U8), ("alive", U8),
+        ("pad_", U8, 3),
+    ],
+    align=True,
+)
+
+POINT_DTYPE = np.dtype(
+    [
+        ("local_a", F, 3), ("anchor_a", F, 3), ("anchor_b", F, 3),
+        ("separation", F), ("adjusted_separation", F), ("normal_impulse", F),
+        ("tangent_impulse", F, 3), ("max_normal_impulse", F), ("relative_velocity", F),
+    ],
+    align=True,
+)
+
+CONTACT_DTYPE = np.dtype(
+    [
+        ("key", np.uint64), ("body_a", U32), ("body_b", U32),
+        ("normal", F, 3), ("tangent1", F, 3), ("tangent2", F, 3),
+        ("friction", F), ("restitution", F), ("rolling_resistance", F), ("rolling_radius", F),
+        ("rolling_impulse", F, 3), ("plane", F, 3), ("count", np.int32),
+        ("planar", U8), ("laned", U8), ("pad_", U8, 2),
+        ("points", POINT_DTYPE, 4),
+    ],
+    align=True,
+)
+
+SETTINGS_DTYPE = np.dtype(
+    [
+        ("gravity", F, 3), ("length_unit", F), ("substeps", np.int32),
+        ("contact_hertz", F), ("contact_damping_ratio", F), ("max_pushout_speed", F),
+        ("max_linear_speed", F), ("max_angular_speed", F), ("restitution_threshold", F),
+        ("sleep_threshold", F), ("time_to_sleep", F), ("speculative", F),
+    ],
+    align=True,
+)
+
+STATIC, KINEMATIC, DYNAMIC = 0, 1, 2
+LOCK_PLANAR = 4 | 8 | 16
+COLORS = 32
+
+# tfs_download_list kinds
+LIST_ACTIVE, LIST_FAST, LIST_FAST_SOLID, LIST_IMPACT, LIST_ROWS, LIST_IMPACT_ROWS, LIST_COLORS = range(7)
+
+
+def default_settings(**overrides):
+    """The reference defaults (include/twofold/world.h:216-241, src/internal.h:348-351)."""
+    s = np.zeros((), SETTINGS_DTYPE)
+    s["gravity"] = (0.0, -9.81, 0.0)
+    s["length_unit"] = 1.0
+    s["substeps"] = 4
+    s["contact_hertz"] = 30.0
+    s["contact_damping_ratio"] = 10.0
+    s["max_pushout_speed"] = 3.0
+    s["max_linear_speed"] = 1000.0
+    s["max_angular_speed"] = 1000.0
+    s["restitution_threshold"] = 1.0
+    s["sleep_threshold"] = 0.05
+    s["time_to_sleep"] = 0.5
+    s["speculative"] = 0.02
+    for k, v in overrides.items():
+        s[k] = v
+    return s
+
+
+def check_sizes(lib, prefix="tfs"):
+    """Compare the numpy layouts with the C sizes a library reports."""
+    import ctypes
+
+    for name, dt in (("body", BODY_DTYPE), ("contact", CONTACT_DTYPE)):
+        fn = getattr(lib, f"{prefix}_sizeof_{name}")
+        fn.restype = ctypes.c_size_t
+        size = fn()
+        if size != dt.itemsize:
+            raise RuntimeError(f"{prefix}_{name} is {size} bytes in C but {dt.itemsize} in {__name__}")

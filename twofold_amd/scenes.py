@@ -1,0 +1,245 @@
+"""Scene recipes for the workloads the solver is measured and checked on.
+
+A recipe is plain data: bodies (motion type, pose, velocities, locks) and shapes (kind, dimensions,
+material). The same recipe can be loaded into any world that accepts that data. All arithmetic is
+float32, in the reference's operation order, so a recipe gives the same bits everywhere.
+
+Random draws use the reference viewer's xorshift32 generator (viewer/scenes.cpp:22-32) with its
+seed, drawn in the order each recipe documents.
+"""
+import ctypes
+import ctypes.util
+import math
+
+import numpy as np
+
+from ._abi import DYNAMIC, KINEMATIC, LOCK_PLANAR, STATIC  # noqa: F401
+
+f32 = np.float32
+_libm = ctypes.CDLL(ctypes.util.find_library("m") or "libm.so.6")
+_libm.sinf.restype = ctypes.c_float
+_libm.sinf.argtypes = [ctypes.c_float]
+_libm.cosf.restype = ctypes.c_float
+_libm.cosf.argtypes = [ctypes.c_float]
+_libm.sqrtf.restype = ctypes.c_float
+_libm.sqrtf.argtypes = [ctypes.c_float]
+
+PI = f32(3.14159265358979)
+
+SPHERE, BOX, CYLINDER, PRISM, HALF_SPACE = 0, 1, 2, 3, 4
+
+
+class Rng:
+    """xorshift32 as viewer/scenes.cpp:22-32."""
+
+    def __init__(self, seed=0x9E3779B9):
+        self.s = seed & 0xFFFFFFFF
+
+    def next(self):
+        s = self.s
+        s ^= (s << 13) & 0xFFFFFFFF
+        s ^= s >> 17
+        s ^= (s << 5) & 0xFFFFFFFF
+        self.s = s
+        return f32(s & 0xFFFFFF) / f32(16777216.0)
+
+    def range(self, lo, hi):
+        lo, hi = f32(lo), f32(hi)
+        return lo + (hi - lo) * self.next()
+
+    def normal(self, sigma=1.0):
+        """One N(0, sigma) draw by Box-Muller on two uniform draws."""
+        u1 = max(float(self.next()), 1e-7)
+        u2 = float(self.next())
+        return f32(sigma * math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2))
+
+
+def normalize3(v):
+    v = np.asarray(v, f32)
+    length = _libm.sqrtf(f32(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]))
+    length = f32(length)
+    if length > 0:
+        inv = f32(1.0) / length
+        return np.array([v[0] * inv, v[1] * inv, v[2] * inv], f32)
+    return np.zeros(3, f32)
+
+
+def axis_angle(axis, angle):
+    """Quat::axisAngle (include/twofold/math.h:146-149) in float32 with libm's sinf/cosf."""
+    half = f32(0.5) * f32(angle)
+    a = normalize3(axis) * f32(_libm.sinf(half))
+    return np.array([a[0], a[1], a[2], f32(_libm.cosf(half))], f32)
+
+
+def from_to(a, b):
+    """Quat::fromTo (include/twofold/math.h:152-158) for the unit vectors used here."""
+    a = np.asarray(a, f32)
+    b = np.asarray(b, f32)
+    d = f32(a[0] * b[0] + a[1] * b[1] + a[2] * b[2])
+    if d < f32(-0.999999):
+        raise ValueError("opposite vectors are not used by these recipes")
+    c = np.array([a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]], f32)
+    s = f32(_libm.sqrtf(f32((f32(1.0) + d) * f32(2.0))))
+    return np.array([c[0] / s, c[1] / s, c[2] / s, f32(0.5) * s], f32)
+
+
+IDENTITY = np.array([0, 0, 0, 1], f32)
+UP = from_to((0, 0, 1), (0, 1, 0))  # the ground's rotation in the reference tests (tests/test_world.cpp:14)
+
+
+class Scene:
+    """Bodies and shapes of one world, in creation order."""
+
+    def __init__(self, name, substeps=4, dt=1.0 / 60.0, steps=1):
+        self.name = name
+        self.substeps = substeps
+        self.dt = f32(dt)
+        self.steps = steps
+        self.bodies = []
+        self.shapes = []
+
+    def body(self, type=DYNAMIC, pos=(0, 0, 0), rot=IDENTITY, v=(0, 0, 0), w=(0, 0, 0), locks=0,
+             lock_frame=IDENTITY, awake=True, linear_damping=0.0, angular_damping=0.0, gravity_scale=1.0):
+        self.bodies.append(dict(
+            type=type, pos=np.asarray(pos, f32), rot=np.asarray(rot, f32), v=np.asarray(v, f32),
+            w=np.asarray(w, f32), locks=locks, lock_frame=np.asarray(lock_frame, f32), awake=awake,
+            linear_damping=f32(linear_damping), angular_damping=f32(angular_damping),
+            gravity_scale=f32(gravity_scale)))
+        return len(self.bodies) - 1
+
+    def shape(self, body, kind, dims=(), pos=(0, 0, 0), rot=IDENTITY, density=1.0, friction=0.5,
+              restitution=0.0, rolling=0.0):
+        self.shapes.append(dict(
+            body=body, kind=kind, dims=np.asarray(dims, f32), pos=np.asarray(pos, f32),
+            rot=np.asarray(rot, f32), density=f32(density), friction=f32(friction),
+            restitution=f32(restitution), rolling=f32(rolling)))
+        return len(self.shapes) - 1
+
+    def ground(self, friction=0.6, restitution=0.0, rolling=0.0):
+        g = self.body(STATIC, rot=UP)
+        self.shape(g, HALF_SPACE, friction=friction, restitution=restitution, rolling=rolling)
+        return g
+
+
+def pyramid(rows=20, density=1000.0, friction=0.6, spacing=1.02, steps=600):
+    """Boxes of half-extent 0.5 stacked in rows of rows, rows-1, ... on a half-space; row r at
+    y = 0.5 + r, box i at x = (i - (n-1)/2) * spacing (the viewer's pyramid, viewer/scenes.cpp:76-87)."""
+    s = Scene(f"pyramid-{rows}", substeps=4, steps=steps)
+    s.ground(friction)
+    for row in range(rows):
+        n = rows - row
+        for i in range(n):
+            x = (f32(i) - f32(0.5) * f32(n - 1)) * f32(spacing)
+            b = s.body(DYNAMIC, pos=(x, f32(0.5) + f32(row), 0))
+            s.shape(b, BOX, (0.5, 0.5, 0.5), density=density, friction=friction)
+    return s
+
+
+def pile(side=32, spacing=1.2, jitter=0.1, walls=True, seed=0x9E3779B9, density=1000.0,
+         friction_range=(0.3, 0.8), restitution_range=None, steps=300, half_room=20.0):
+    """The drop-and-settle pile: side^3 bodies on a jittered grid above a half-space inside four
+    static walls. Bodies are created y-major (for y, z, x). Per body the draws are: jitter x, y, z;
+    then a box's three half-extents U[0.2, 0.5] (even creation index) or a sphere's radius
+    U[0.2, 0.5] (odd); friction; rotation axis x, y, z in U[-1, 1]; angle in U[0, pi]; then the
+    restitution when a range is given."""
+    s = Scene(f"pile-{side ** 3}", substeps=4, steps=steps)
+    rng = Rng(seed)
+    s.ground(0.6)
+    if walls:
+        h = f32(half_room)
+        t = f32(0.5)
+        for x in (-1, 1):
+            wb = s.body(STATIC, pos=(f32(x) * (h + t), h, 0))
+            s.shape(wb, BOX, (t, h, h + t), friction=0.6)
+        for z in (-1, 1):
+            wb = s.body(STATIC, pos=(0, h, f32(z) * (h + t)))
+            s.shape(wb, BOX, (h + t, h, t), friction=0.6)
+    k = 0
+    half = f32(0.5) * f32(side - 1)
+    for y in range(side):
+        for z in range(side):
+            for x in range(side):
+                jx, jy, jz = rng.range(-jitter, jitter), rng.range(-jitter, jitter), rng.range(-jitter, jitter)
+                pos = ((f32(x) - half) * f32(spacing) + jx, f32(1.0) + f32(spacing) * f32(y) + jy,
+                       (f32(z) - half) * f32(spacing) + jz)
+                if k % 2 == 0:
+                    kind, dims = BOX, (rng.range(0.2, 0.5), rng.range(0.2, 0.5), rng.range(0.2, 0.5))
+                else:
+                    kind, dims = SPHERE, (rng.range(0.2, 0.5),)
+                fr = rng.range(*friction_range)
+                axis = (rng.range(-1, 1), rng.range(-1, 1), rng.range(-1, 1))
+                angle = rng.range(0, PI)
+                e = rng.range(*restitution_range) if restitution_range else f32(0)
+                b = s.body(DYNAMIC, pos=pos, rot=axis_angle(axis, angle))
+                s.shape(b, kind, dims, density=density, friction=fr, restitution=e)
+                k += 1
+    return s
+
+
+def stack(n=10, steps=240):
+    """n unit boxes in a column on a half-space (tests/test_world.cpp:235-244)."""
+    s = Scene(f"stack-{n}", steps=steps)
+    s.ground(0.6)
+    for i in range(n):
+        b = s.body(DYNAMIC, pos=(0, f32(0.5) + f32(i), 0))
+        s.shape(b, BOX, (0.5, 0.5, 0.5), friction=0.6)
+    return s
+
+
+def bounce(restitution=1.0, height=3.0, radius=0.5, steps=120):
+    """A sphere dropped on a half-space with restitution (tests/test_world.cpp:246-267)."""
+    s = Scene("bounce", steps=steps)
+    s.ground(0.6, restitution=restitution)
+    b = s.body(DYNAMIC, pos=(0, height, 0))
+    s.shape(b, SPHERE, (radius,), restitution=restitution)
+    return s
+
+
+def rolling(speed=4.0, resistance=0.2, steps=240):
+    """A rolling ball and box pushed along a half-space with rolling resistance
+    (tests/test_world.cpp:1023-1059)."""
+    s = Scene("rolling", steps=steps)
+    s.ground(0.6, rolling=resistance)
+    b = s.body(DYNAMIC, pos=(0, 0.5, 0), v=(speed, 0, 0), w=(0, 0, -speed / 0.5))
+    s.shape(b, SPHERE, (0.5,), friction=0.6, rolling=resistance)
+    c = s.body(DYNAMIC, pos=(0, 0.5, 3), v=(speed, 0, 0))
+    s.shape(c, BOX, (0.5, 0.5, 0.5), friction=0.6, rolling=resistance)
+    return s
+
+
+def planar_mix(steps=240):
+    """Planar boxes and a disc locked to the xy plane, a planar rolling disc, and a free sphere
+    landing on them (tests/test_world.cpp:410-480, tests/test_planar.cpp:75-117)."""
+    s = Scene("planar-mix", steps=steps)
+    s.ground(0.6)
+    for i in range(4):
+        b = s.body(DYNAMIC, pos=(f32(-2.0) + f32(1.1) * f32(i), f32(0.5), 0), locks=LOCK_PLANAR)
+        s.shape(b, BOX, (0.5, 0.5, 0.5), friction=0.6)
+    top = s.body(DYNAMIC, pos=(-1.45, 1.6, 0), locks=LOCK_PLANAR, w=(0, 0, 1.0))
+    s.shape(top, BOX, (0.6, 0.25, 0.5), friction=0.6)
+    disc = s.body(DYNAMIC, pos=(4, 0.5, 0), locks=LOCK_PLANAR, w=(0, 0, -6.0))
+    s.shape(disc, CYLINDER, (0.5, 0.25), friction=0.6, rolling=0.05)
+    ball = s.body(DYNAMIC, pos=(-1.0, 3.5, 0.2))
+    s.shape(ball, SPHERE, (0.4,), friction=0.6, restitution=0.3)
+    return s
+
+
+def tumble(n=24, seed=7, steps=180):
+    """Mixed boxes and spheres thrown fast at each other and a half-space: fast pairs, impacts,
+    restitution and friction all in play."""
+    s = Scene("tumble", steps=steps)
+    rng = Rng(seed)
+    s.ground(0.6, restitution=0.2)
+    for i in range(n):
+        pos = (rng.range(-3, 3), rng.range(1, 6), rng.range(-3, 3))
+        v = (rng.range(-8, 8), rng.range(-12, 2), rng.range(-8, 8))
+        w = (rng.range(-6, 6), rng.range(-6, 6), rng.range(-6, 6))
+        rot = axis_angle((rng.range(-1, 1), rng.range(-1, 1), rng.range(-1, 1)), rng.range(0, PI))
+        b = s.body(DYNAMIC, pos=pos, rot=rot, v=v, w=w)
+        if i % 2 == 0:
+            s.shape(b, BOX, (rng.range(0.2, 0.5), rng.range(0.2, 0.5), rng.range(0.2, 0.5)),
+                    friction=rng.range(0.2, 1.0), restitution=rng.range(0, 0.4))
+        else:
+            s.shape(b, SPHERE, (rng.range(0.2, 0.5),), friction=rng.range(0.2, 1.0),
+                    restitution=rng.range(0, 0.4))
+    return s
