@@ -1,0 +1,70 @@
+"""Bit-level comparison of one solve step against what the reference recorded.
+
+`want` holds the reference's record of the step (post_bodies: after the final speed limit;
+step_bodies: after the whole step, sleep included; post_contacts; lists; colors). `got_*` is the
+result of the path under test after one tfs_step / tfo_step on want's inputs: bodies after the
+solve block and the sleep pass, contacts after the solve block.
+"""
+import numpy as np
+
+LISTS = ("active", "fast", "fast_solid", "impact", "rows", "rest", "impact_rows", "impact_rest")
+POSE = ("p", "q", "com", "dp", "dq", "kept")
+MASS = ("inv_mass_world", "inv_inertia_world", "plane_inv_mass", "plane_inv_inertia")
+STEP = ("v", "w", "island", "awake", "sleep_time")
+CONTACT = ("normal", "tangent1", "tangent2", "rolling_impulse", "rolling_radius", "count", "planar", "laned")
+POINT = ("anchor_a", "anchor_b", "adjusted_separation", "normal_impulse", "tangent_impulse", "max_normal_impulse",
+         "relative_velocity", "local_a", "separation")
+
+
+def _bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint32) if a.dtype == np.float32 else a
+
+
+def _rows_differ(a, b):
+    a, b = _bits(a), _bits(b)
+    d = (a != b).reshape(len(a), -1).any(1) if len(a) else np.zeros(0, bool)
+    return np.nonzero(d)[0]
+
+
+def compare(want, got_bodies, got_contacts, got_lists):
+    """Returns a list of mismatch descriptions (empty when everything is bit-identical)."""
+    bad = []
+    for k in LISTS:
+        if not np.array_equal(np.asarray(want["lists"][k]), np.asarray(got_lists[k])):
+            bad.append(f"list {k}: want {len(want['lists'][k])} entries, got {len(got_lists[k])}")
+    act = want["lists"]["active"]
+    if not np.array_equal(want["colors"][act], np.asarray(got_lists["colors"])[act]):
+        bad.append("colours of active contacts")
+    post, stepb = want["post_bodies"], want["step_bodies"]
+    slept = (post["awake"] == 1) & (stepb["awake"] == 0)
+    for f in POSE + MASS:
+        ref = post[f].copy()
+        if f in MASS:
+            ref[slept] = stepb[f][slept]
+        d = _rows_differ(ref, got_bodies[f])
+        if len(d):
+            bad.append(f"body {f} at {d[:5].tolist()}")
+    for f in STEP:
+        d = _rows_differ(stepb[f], got_bodies[f])
+        if len(d):
+            bad.append(f"body {f} after sleep at {d[:5].tolist()}")
+    wc = want["post_contacts"]
+    for f in CONTACT:
+        d = _rows_differ(wc[f], got_contacts[f])
+        if len(d):
+            bad.append(f"contact {f} at {d[:5].tolist()}")
+    for i in range(4):
+        m = wc["count"] > i
+        for f in POINT:
+            d = _rows_differ(wc["points"][f][:, i][m], got_contacts["points"][f][:, i][m])
+            if len(d):
+                bad.append(f"point {i} {f} at {np.nonzero(m)[0][d][:5].tolist()}")
+    return bad
+
+
+def record_dict(rec):
+    """A refworld.StepRecord as the dict compare() takes."""
+    return dict(lists=rec.lists, colors=rec.colors, post_bodies=rec.post_bodies, step_bodies=rec.step_bodies,
+                post_contacts=rec.post_contacts, pre_bodies=rec.pre_bodies, pre_contacts=rec.pre_contacts,
+                refreshes=rec.refreshes, dt=rec.dt)

@@ -1,0 +1,1345 @@
+// solver.hip - gfx950 kernels and host driver of the soft-step contact solver, behind the C ABI of
+// include/twofold_solver.h.
+//
+// One tfs_step runs src/world.cpp:1591-1691 and updateSleep (1174-1247) on state that stays in HBM:
+//
+//   step state, inverse mass (per body)            k_step_init
+//   active / fast / fast-solid lists (per contact)  k_classify + stream compaction
+//   prepare + impact marking (per active contact)   k_prepare, k_impact_flag + compaction
+//   greedy colouring in active order                k_ep_* (per-body endpoint ranks) + k_color
+//   row order: colour, then point count             k_row_hist + scan + k_row_scatter
+//   row packing (SoA rows, coalesced)               k_pack
+//   per substep: [refresh] integrate velocities, warm start, biased solve (+impact), integrate
+//   positions, relax (+impact) - one launch per colour; contacts outside the colours run in order
+//   restitution, final speed limit, impulses back to the contacts, sleep and islands.
+//
+// Bodies live in three device arrays: the full tfs_body records (cold fields), a 64-byte motion
+// record (v, w, dp, dq) that the row kernels gather and scatter, and a 96-byte mass record (world
+// inverse mass and inertia) that they gather. Coloured contacts are packed once per step into
+// structure-of-arrays rows in row order, so every pass reads them with coalesced 16-byte loads.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "stages.h"
+
+using namespace tf;
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kTile = 1024;            // scan tile: 256 threads x 4 items
+constexpr int kKeys = 5 * TFS_COLORS + 1; // row-order buckets: colour x point count, then the rest
+constexpr uint32_t kSpinLimit = 1u << 26;
+
+struct DevCounts {
+    uint32_t nActive, nFast, nFastSolid, nImpact;
+    uint32_t pointsActive, pointsImpact;
+    uint32_t scanTotal;
+    uint32_t error;
+    uint32_t changed;
+    uint32_t nAgain;
+    uint32_t colorStart[TFS_COLORS + 1];
+    uint32_t impactColorStart[TFS_COLORS + 1];
+};
+
+// ---- body records ----------------------------------------------------------------------------
+
+__device__ __forceinline__ Motion loadMotion(const float4* m, uint32_t i) {
+    const float4 v = m[4 * i], w = m[4 * i + 1], dp = m[4 * i + 2], dq = m[4 * i + 3];
+    return Motion{ vec(v.x, v.y, v.z), vec(w.x, w.y, w.z), vec(dp.x, dp.y, dp.z), quat(dq.x, dq.y, dq.z, dq.w) };
+}
+__device__ __forceinline__ void storeVelocity(float4* m, uint32_t i, const Motion& x) {
+    m[4 * i] = make_float4(x.v.x, x.v.y, x.v.z, 0.0f);
+    m[4 * i + 1] = make_float4(x.w.x, x.w.y, x.w.z, 0.0f);
+}
+__device__ __forceinline__ void storeMotion(float4* m, uint32_t i, const Motion& x) {
+    storeVelocity(m, i, x);
+    m[4 * i + 2] = make_float4(x.dp.x, x.dp.y, x.dp.z, 0.0f);
+    m[4 * i + 3] = make_float4(x.dq.x, x.dq.y, x.dq.z, x.dq.w);
+}
+// 24 floats: world inverse mass (9), world inverse inertia (9), inverse mass, plane inverse mass,
+// plane inverse inertia, 3 spare.
+__device__ __forceinline__ Mass loadMass(const float4* m, uint32_t i) {
+    const float4 a = m[6 * i], b = m[6 * i + 1], c = m[6 * i + 2], d = m[6 * i + 3], e = m[6 * i + 4], f = m[6 * i + 5];
+    Mass x;
+    x.imw = M3{ { vec(a.x, a.y, a.z), vec(a.w, b.x, b.y), vec(b.z, b.w, c.x) } };
+    x.iiw = M3{ { vec(c.y, c.z, c.w), vec(d.x, d.y, d.z), vec(d.w, e.x, e.y) } };
+    x.invMass = e.z;
+    x.pim = e.w;
+    x.pii = f.x;
+    return x;
+}
+__device__ __forceinline__ void storeMass(float4* m, uint32_t i, const tfs_body& b) {
+    const float* w = b.inv_mass_world;
+    const float* n = b.inv_inertia_world;
+    m[6 * i] = make_float4(w[0], w[1], w[2], w[3]);
+    m[6 * i + 1] = make_float4(w[4], w[5], w[6], w[7]);
+    m[6 * i + 2] = make_float4(w[8], n[0], n[1], n[2]);
+    m[6 * i + 3] = make_float4(n[3], n[4], n[5], n[6]);
+    m[6 * i + 4] = make_float4(n[7], n[8], b.inv_mass, b.plane_inv_mass);
+    m[6 * i + 5] = make_float4(b.plane_inv_inertia, 0.0f, 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ uint32_t gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+__device__ __forceinline__ uint32_t gstride() { return gridDim.x * blockDim.x; }
+
+// ---- scans and compaction ---------------------------------------------------------------------
+
+__device__ uint32_t blockExclusive(uint32_t v, uint32_t* lds, uint32_t& total) {
+    const uint32_t t = threadIdx.x;
+    lds[t] = v;
+    __syncthreads();
+    for (uint32_t off = 1; off < kThreads; off <<= 1) {
+        const uint32_t x = t >= off ? lds[t - off] : 0u;
+        __syncthreads();
+        lds[t] += x;
+        __syncthreads();
+    }
+    total = lds[kThreads - 1];
+    const uint32_t incl = lds[t];
+    __syncthreads();
+    return incl - v;
+}
+
+__device__ __forceinline__ uint32_t countOf(const uint32_t* nPtr, uint32_t n) { return nPtr ? *nPtr : n; }
+
+// Sum of each 1024-item tile.
+__global__ void k_scan_tiles(const uint32_t* in, const uint32_t* nPtr, uint32_t nConst, uint32_t* tileSums) {
+    __shared__ uint32_t lds[kThreads];
+    const uint32_t n = countOf(nPtr, nConst);
+    const uint32_t base = blockIdx.x * kTile + threadIdx.x * 4;
+    uint32_t s = 0;
+    for (int k = 0; k < 4; ++k) { s += base + k < n ? in[base + k] : 0u; }
+    uint32_t total;
+    blockExclusive(s, lds, total);
+    if (threadIdx.x == 0) { tileSums[blockIdx.x] = total; }
+}
+
+// Exclusive scan of the tile sums in place, one block; the grand total goes to *total.
+__global__ void k_scan_sums(uint32_t* tileSums, uint32_t nTiles, uint32_t* total) {
+    __shared__ uint32_t lds[kThreads];
+    uint32_t carry = 0;
+    for (uint32_t base = 0; base < nTiles; base += kThreads) {
+        const uint32_t i = base + threadIdx.x;
+        const uint32_t v = i < nTiles ? tileSums[i] : 0u;
+        uint32_t sum;
+        const uint32_t ex = blockExclusive(v, lds, sum);
+        if (i < nTiles) { tileSums[i] = carry + ex; }
+        carry += sum;
+    }
+    if (threadIdx.x == 0 && total) { *total = carry; }
+}
+
+// out[i] = exclusive prefix of in; with `list` set, writes i to list[prefix] where in[i] != 0.
+__global__ void k_scan_apply(const uint32_t* in, const uint32_t* nPtr, uint32_t nConst, const uint32_t* tileOffsets,
+                             uint32_t* out, uint32_t* list) {
+    __shared__ uint32_t lds[kThreads];
+    const uint32_t n = countOf(nPtr, nConst);
+    const uint32_t base = blockIdx.x * kTile + threadIdx.x * 4;
+    uint32_t v[4], s = 0;
+    for (int k = 0; k < 4; ++k) {
+        v[k] = base + k < n ? in[base + k] : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t run = blockExclusive(s, lds, total) + tileOffsets[blockIdx.x];
+    for (int k = 0; k < 4; ++k) {
+        if (base + k < n) {
+            if (out) { out[base + k] = run; }
+            if (list && v[k]) { list[run] = base + k; }
+        }
+        run += v[k];
+    }
+}
+
+// ---- step state and lists ------------------------------------------------------------------------
+
+// src/world.cpp:1591-1601: pose at step start, travel bound, dp and dq reset, inverse mass.
+__global__ void k_step_init(tfs_body* bodies, uint32_t n, float dt, float4* motion, float4* mass, uint8_t* moving) {
+    for (uint32_t i = gid(); i < n; i += gstride()) {
+        tfs_body b = bodies[i];
+        if (b.alive) {
+            for (int k = 0; k < 4; ++k) {
+                b.q0[k] = b.q[k];
+                b.q_base[k] = b.q[k];
+            }
+            for (int k = 0; k < 3; ++k) { b.com0[k] = b.com[k]; }
+            b.expected_travel = (length(ld3(b.v)) + length(ld3(b.w)) * b.extent) * dt;
+            b.dp[0] = b.dp[1] = b.dp[2] = 0.0f;
+            b.dq[0] = b.dq[1] = b.dq[2] = 0.0f;
+            b.dq[3] = 1.0f;
+            updateInverseMass(b);
+            bodies[i] = b;
+        }
+        storeMotion(motion, i, Motion{ ld3(b.v), ld3(b.w), ld3(b.dp), ld4(b.dq) });
+        storeMass(mass, i, b);
+        moving[i] = b.alive && movingBody(b) ? 1 : 0;
+    }
+}
+
+// src/world.cpp:1604-1621: flags for the active, fast and fast-solid lists.
+__global__ void k_classify(const tfs_contact* contacts, uint32_t nc, const tfs_body* bodies, float dt, float speculative,
+                           uint32_t* active, uint32_t* fast, uint32_t* fastSolid) {
+    for (uint32_t i = gid(); i < nc; i += gstride()) {
+        const tfs_contact& c = contacts[i];
+        const tfs_body& a = bodies[c.body_a];
+        const tfs_body& b = bodies[c.body_b];
+        uint32_t on = 0, f = 0, fs = 0;
+        if (movingBody(a) || movingBody(b)) {
+            on = 1;
+            if (relativeTravel(a, b, dt) > speculative) {
+                f = 1;
+                fs = !isPlanarLocks(a.locks) && !isPlanarLocks(b.locks) ? 1u : 0u;
+            }
+        }
+        active[i] = on;
+        fast[i] = f;
+        fastSolid[i] = fs;
+    }
+}
+
+// prepareContacts (src/solver.cpp:308-313) and the marking half of impactContacts
+// (src/world.cpp:1029-1041).
+__global__ void k_prepare(const uint32_t* active, const DevCounts* cnt, tfs_contact* contacts, const tfs_body* bodies,
+                          const float4* motion, const float4* mass, float threshold, uint8_t* struck,
+                          uint32_t* points) {
+    const uint32_t n = cnt->nActive;
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        const uint32_t ci = active[k];
+        tfs_contact c = contacts[ci];
+        const tfs_body& ta = bodies[c.body_a];
+        const tfs_body& tb = bodies[c.body_b];
+        prepareContact(viewOf(ta), viewOf(tb), ta, tb, loadMotion(motion, c.body_a), loadMotion(motion, c.body_b),
+                       loadMass(mass, c.body_a), loadMass(mass, c.body_b), c);
+        contacts[ci] = c;
+        atomicAdd(points, (uint32_t)c.count);
+        for (int i = 0; i < c.count; ++i) {
+            if (c.points[i].relative_velocity >= -threshold) { continue; }
+            if (ta.type == TFS_DYNAMIC) { struck[c.body_a] = 1; }
+            if (tb.type == TFS_DYNAMIC) { struck[c.body_b] = 1; }
+            break;
+        }
+    }
+}
+
+// impactContacts' list (src/world.cpp:1042-1048): active contacts that touch a struck body.
+__global__ void k_impact_flag(const uint32_t* active, const DevCounts* cnt, const tfs_contact* contacts,
+                              const uint8_t* struck, uint32_t* flags) {
+    const uint32_t n = cnt->nActive;
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        const tfs_contact& c = contacts[active[k]];
+        flags[k] = (struck[c.body_a] || struck[c.body_b]) ? 1u : 0u;
+    }
+}
+
+// Maps positions in the active list to contact indices.
+__global__ void k_gather_list(const uint32_t* positions, const uint32_t* nPtr, const uint32_t* active, uint32_t* out) {
+    const uint32_t n = *nPtr;
+    for (uint32_t k = gid(); k < n; k += gstride()) { out[k] = active[positions[k]]; }
+}
+
+__global__ void k_points_of(const uint32_t* list, const uint32_t* nPtr, const tfs_contact* contacts, uint32_t* points) {
+    const uint32_t n = *nPtr;
+    for (uint32_t k = gid(); k < n; k += gstride()) { atomicAdd(points, (uint32_t)contacts[list[k]].count); }
+}
+
+// ---- colouring ----------------------------------------------------------------------------------
+//
+// colorContacts (src/lanes.cpp:356-374) gives each contact, in active order, the lowest colour
+// free on both of its moving bodies. A contact's colour depends only on the earlier contacts that
+// share a moving body with it, so the kernel runs every contact as soon as its bodies have seen all
+// their earlier contacts: each moving body keeps one 64-bit word, (contacts seen << 32) | colours
+// taken, and a contact waits until both its words count exactly its rank on that body. The result
+// is the sequential greedy colouring, bit for bit.
+
+__device__ __forceinline__ bool colourable(const tfs_contact& c) { return !c.planar && !(c.rolling_resistance > 0.0f); }
+
+__global__ void k_ep_count(const uint32_t* active, const DevCounts* cnt, const tfs_contact* contacts,
+                           const uint8_t* moving, uint32_t* deg) {
+    const uint32_t n = cnt->nActive;
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        const tfs_contact& c = contacts[active[k]];
+        if (!colourable(c)) { continue; }
+        if (moving[c.body_a]) { atomicAdd(&deg[c.body_a], 1u); }
+        if (moving[c.body_b]) { atomicAdd(&deg[c.body_b], 1u); }
+    }
+}
+
+__global__ void k_ep_fill(const uint32_t* active, const DevCounts* cnt, const tfs_contact* contacts,
+                          const uint8_t* moving, const uint32_t* start, uint32_t* cursor, uint32_t* ep) {
+    const uint32_t n = cnt->nActive;
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        const tfs_contact& c = contacts[active[k]];
+        if (!colourable(c)) { continue; }
+        if (moving[c.body_a]) { ep[start[c.body_a] + atomicAdd(&cursor[c.body_a], 1u)] = k << 1; }
+        if (moving[c.body_b]) { ep[start[c.body_b] + atomicAdd(&cursor[c.body_b], 1u)] = (k << 1) | 1u; }
+    }
+}
+
+// Sorts each body's endpoints by active position and writes each endpoint's rank on its body.
+__global__ void k_ep_rank(uint32_t nb, const uint32_t* deg, const uint32_t* start, uint32_t* ep, uint32_t* rank) {
+    for (uint32_t x = gid(); x < nb; x += gstride()) {
+        const uint32_t d = deg[x];
+        uint32_t* e = ep + start[x];
+        for (uint32_t i = 1; i < d; ++i) {
+            const uint32_t v = e[i];
+            uint32_t j = i;
+            while (j > 0 && e[j - 1] > v) {
+                e[j] = e[j - 1];
+                --j;
+            }
+            e[j] = v;
+        }
+        for (uint32_t i = 0; i < d; ++i) { rank[e[i]] = i; }
+    }
+}
+
+__device__ __forceinline__ unsigned long long loadWord(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void storeWord(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Blocks take chunks of the active list in order from a ticket, so every chunk a contact can wait
+// on belongs to a block that is already running; the first unfinished contact can always proceed.
+__global__ void k_color(const uint32_t* active, const DevCounts* cntIn, tfs_contact* contacts, const uint8_t* moving,
+                        const uint32_t* rank, unsigned long long* word, uint8_t* colors, uint32_t* ticket,
+                        DevCounts* cnt) {
+    __shared__ uint32_t chunk;
+    const uint32_t n = cntIn->nActive;
+    for (;;) {
+        if (threadIdx.x == 0) { chunk = atomicAdd(ticket, 1u); }
+        __syncthreads();
+        const uint32_t k = chunk * blockDim.x + threadIdx.x;
+        if (chunk * blockDim.x >= n) { break; }
+        if (k < n) {
+            const uint32_t ci = active[k];
+            const uint32_t a = contacts[ci].body_a, b = contacts[ci].body_b;
+            uint8_t col = TFS_COLORS;
+            if (colourable(contacts[ci])) {
+                const bool ma = moving[a], mb = moving[b];
+                const uint32_t ra = rank[k << 1], rb = rank[(k << 1) | 1u];
+                // The work happens inside the loop: a lane that left the loop would wait for the
+                // rest of its wave, which may be waiting for it.
+                bool done = false;
+                uint32_t spins = 0;
+                while (!done) {
+                    const unsigned long long wa = ma ? loadWord(&word[a]) : 0ull;
+                    const unsigned long long wb = mb ? loadWord(&word[b]) : 0ull;
+                    if ((!ma || (uint32_t)(wa >> 32) == ra) && (!mb || (uint32_t)(wb >> 32) == rb)) {
+                        const uint32_t taken = (uint32_t)wa | (uint32_t)wb;
+                        uint32_t add = 0;
+                        if (taken != 0xffffffffu) {
+                            const uint32_t bit = (uint32_t)__builtin_ctz(~taken);
+                            col = (uint8_t)bit;
+                            add = 1u << bit;
+                        }
+                        if (ma) { storeWord(&word[a], ((unsigned long long)(ra + 1) << 32) | ((uint32_t)wa | add)); }
+                        if (mb) { storeWord(&word[b], ((unsigned long long)(rb + 1) << 32) | ((uint32_t)wb | add)); }
+                        done = true;
+                    } else if (++spins > kSpinLimit) {
+                        atomicOr(&cnt->error, 1u);
+                        done = true;
+                    } else {
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+            }
+            colors[ci] = col;
+            contacts[ci].laned = col < TFS_COLORS ? 1 : 0;
+        }
+        __syncthreads();
+    }
+}
+
+// ---- row order: batchContacts (src/lanes.cpp:376-405) -------------------------------------------
+//
+// Colour by colour; within a colour, contacts with more points first; ties and the uncoloured rest
+// in list order. A stable counting sort over 5 x 32 + 1 keys.
+
+__device__ __forceinline__ uint32_t rowKey(const tfs_contact& c, uint8_t col) {
+    return col >= TFS_COLORS ? 5u * TFS_COLORS : 5u * col + (4u - (uint32_t)c.count);
+}
+
+__global__ void k_row_hist(const uint32_t* list, const uint32_t* nPtr, const tfs_contact* contacts, const uint8_t* colors,
+                           uint32_t* hist, uint32_t nBlocks) {
+    __shared__ uint32_t h[kKeys];
+    const uint32_t n = *nPtr;
+    for (uint32_t k = threadIdx.x; k < kKeys; k += blockDim.x) { h[k] = 0; }
+    __syncthreads();
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) {
+        const uint32_t ci = list[k];
+        atomicAdd(&h[rowKey(contacts[ci], colors[ci])], 1u);
+    }
+    __syncthreads();
+    for (uint32_t key = threadIdx.x; key < kKeys; key += blockDim.x) { hist[key * nBlocks + blockIdx.x] = h[key]; }
+}
+
+__global__ void k_row_scatter(const uint32_t* list, const uint32_t* nPtr, const tfs_contact* contacts,
+                              const uint8_t* colors, const uint32_t* offsets, uint32_t nBlocks, uint32_t* out) {
+    __shared__ uint32_t keys[kThreads];
+    const uint32_t n = *nPtr;
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t key = 0xffffffffu, ci = 0;
+    if (k < n) {
+        ci = list[k];
+        key = rowKey(contacts[ci], colors[ci]);
+    }
+    keys[threadIdx.x] = key;
+    __syncthreads();
+    if (k < n) {
+        uint32_t before = 0;
+        for (uint32_t t = 0; t < threadIdx.x; ++t) { before += keys[t] == key ? 1u : 0u; }
+        out[offsets[key * nBlocks + blockIdx.x] + before] = ci;
+    }
+}
+
+// Start of each colour's rows; entry 32 is the start of the rest.
+__global__ void k_row_starts(const uint32_t* offsets, uint32_t nBlocks, uint32_t* starts) {
+    const uint32_t c = threadIdx.x;
+    if (c <= TFS_COLORS) { starts[c] = offsets[(5u * c) * nBlocks]; }
+}
+
+// ---- rows --------------------------------------------------------------------------------------
+
+// Structure-of-arrays rows: entry [r] or [k * cap + r] for point k.
+struct Rows {
+    uint4* hdr;     // body a, body b, flags, contact index
+    float4* nrm;    // normal, friction
+    float4* tg1;    // tangent1, restitution
+    float4* tg2;    // tangent2, 0
+    float4* pa;     // anchor a, adjusted separation
+    float4* pb;     // anchor b, relative velocity
+    float4* imp;    // normal impulse, tangent impulse
+    float* peak;    // largest normal impulse
+    uint32_t* at;   // impact rows: the active row holding the impulses
+    uint32_t cap;
+};
+
+__global__ void k_pack(const uint32_t* list, uint32_t n, const tfs_contact* contacts, const uint8_t* moving, Rows rows,
+                       uint32_t* rowOf, const uint32_t* activeRowOf) {
+    for (uint32_t r = gid(); r < n; r += gstride()) {
+        const uint32_t ci = list[r];
+        const tfs_contact& c = contacts[ci];
+        const uint32_t flags = (uint32_t)c.count | (moving[c.body_a] ? kMoveA : 0u) | (moving[c.body_b] ? kMoveB : 0u);
+        rows.hdr[r] = make_uint4(c.body_a, c.body_b, flags, ci);
+        rows.nrm[r] = make_float4(c.normal[0], c.normal[1], c.normal[2], c.friction);
+        rows.tg1[r] = make_float4(c.tangent1[0], c.tangent1[1], c.tangent1[2], c.restitution);
+        rows.tg2[r] = make_float4(c.tangent2[0], c.tangent2[1], c.tangent2[2], 0.0f);
+        for (int k = 0; k < c.count; ++k) {
+            const tfs_point& p = c.points[k];
+            const uint32_t j = (uint32_t)k * rows.cap + r;
+            rows.pa[j] = make_float4(p.anchor_a[0], p.anchor_a[1], p.anchor_a[2], p.adjusted_separation);
+            rows.pb[j] = make_float4(p.anchor_b[0], p.anchor_b[1], p.anchor_b[2], p.relative_velocity);
+            if (!activeRowOf) {
+                rows.imp[j] = make_float4(p.normal_impulse, p.tangent_impulse[0], p.tangent_impulse[1], p.tangent_impulse[2]);
+                rows.peak[j] = p.max_normal_impulse;
+            }
+        }
+        if (rowOf) { rowOf[ci] = r; }
+        if (activeRowOf) { rows.at[r] = activeRowOf[ci]; }
+    }
+}
+
+// Impulses of the coloured rows back into the contacts.
+__global__ void k_unpack(uint32_t n, Rows rows, tfs_contact* contacts) {
+    for (uint32_t r = gid(); r < n; r += gstride()) {
+        const uint4 h = rows.hdr[r];
+        tfs_contact& c = contacts[h.w];
+        const int count = (int)(h.z & kCountMask);
+        for (int k = 0; k < count; ++k) {
+            const uint32_t j = (uint32_t)k * rows.cap + r;
+            const float4 im = rows.imp[j];
+            c.points[k].normal_impulse = im.x;
+            c.points[k].tangent_impulse[0] = im.y;
+            c.points[k].tangent_impulse[1] = im.z;
+            c.points[k].tangent_impulse[2] = im.w;
+            c.points[k].max_normal_impulse = rows.peak[j];
+        }
+    }
+}
+
+enum RowMode { kWarm = 0, kBias = 1, kRelax = 2, kRestitute = 3 };
+
+// One colour of rows. Rows of one colour move no body in common (src/internal.h:607-609), so each
+// thread owns its row's two moving bodies for the whole launch. Impact rows keep their impulses in
+// the active row of the same contact.
+template <int MODE>
+__global__ void __launch_bounds__(kThreads) k_rows(Rows rows, uint32_t begin, uint32_t end, Rows imp, float4* motion,
+                                                   const float4* mass, Ctx ctx) {
+    const uint32_t r = begin + gid();
+    if (r >= end) { return; }
+    const uint4 h = rows.hdr[r];
+    const float4 n = rows.nrm[r], t1 = rows.tg1[r], t2 = rows.tg2[r];
+    const Row row{ h.z, vec(n.x, n.y, n.z), vec(t1.x, t1.y, t1.z), vec(t2.x, t2.y, t2.z), n.w, t1.w };
+    if (MODE == kRestitute && row.restitution == 0.0f) { return; }
+    const int count = (int)(h.z & kCountMask);
+    if (count == 0) { return; }
+    const uint32_t ir = rows.at ? rows.at[r] : r;
+    RowPoint p[4];
+    for (int k = 0; k < count; ++k) {
+        const float4 a = rows.pa[(uint32_t)k * rows.cap + r];
+        const float4 b = rows.pb[(uint32_t)k * rows.cap + r];
+        const float4 im = imp.imp[(uint32_t)k * imp.cap + ir];
+        p[k].ra = vec(a.x, a.y, a.z);
+        p[k].adj = a.w;
+        p[k].rb = vec(b.x, b.y, b.z);
+        p[k].rv = b.w;
+        p[k].ni = im.x;
+        p[k].ti = vec(im.y, im.z, im.w);
+        p[k].nimax = imp.peak[(uint32_t)k * imp.cap + ir];
+    }
+    Motion A = loadMotion(motion, h.x), B = loadMotion(motion, h.y);
+    const Mass MA = loadMass(mass, h.x), MB = loadMass(mass, h.y);
+    if (MODE == kWarm) {
+        warmRow(row, p, MA, MB, A, B);
+    } else if (MODE == kRestitute) {
+        restituteRow(row, p, MA, MB, A, B, ctx);
+    } else {
+        solveRow(row, p, MA, MB, A, B, ctx, MODE == kBias);
+    }
+    if (MODE != kWarm) {
+        for (int k = 0; k < count; ++k) {
+            imp.imp[(uint32_t)k * imp.cap + ir] = make_float4(p[k].ni, p[k].ti.x, p[k].ti.y, p[k].ti.z);
+            imp.peak[(uint32_t)k * imp.cap + ir] = p[k].nimax;
+        }
+    }
+    if (h.z & kMoveA) { storeVelocity(motion, h.x, A); }
+    if (h.z & kMoveB) { storeVelocity(motion, h.y, B); }
+}
+
+// Contacts outside the colours, one at a time in list order, as solveContacts, warmStart and
+// applyRestitution take them (src/solver.cpp:319-460).
+__global__ void k_rest(const uint32_t* list, uint32_t n, int mode, tfs_contact* contacts, const tfs_body* bodies,
+                       float4* motion, const float4* mass, Ctx ctx) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) { return; }
+    for (uint32_t k = 0; k < n; ++k) {
+        tfs_contact& c = contacts[list[k]];
+        const BodyView va = viewOf(bodies[c.body_a]), vb = viewOf(bodies[c.body_b]);
+        Motion a = loadMotion(motion, c.body_a), b = loadMotion(motion, c.body_b);
+        const Mass ma = loadMass(mass, c.body_a), mb = loadMass(mass, c.body_b);
+        if (mode == kWarm) {
+            warmRest(c, va, vb, ma, mb, a, b);
+        } else if (mode == kRestitute) {
+            restituteRest(c, va, vb, ma, mb, a, b, ctx);
+        } else {
+            solveRest(c, va, vb, ma, mb, a, b, ctx, mode == kBias);
+        }
+        storeVelocity(motion, c.body_a, a);
+        storeVelocity(motion, c.body_b, b);
+    }
+}
+
+// ---- bodies per substep ------------------------------------------------------------------------
+
+__global__ void k_integrate_velocities(tfs_body* bodies, uint32_t n, float4* motion, Ctx ctx) {
+    for (uint32_t i = gid(); i < n; i += gstride()) {
+        tfs_body& b = bodies[i];
+        if (!b.alive || b.type != TFS_DYNAMIC || !b.awake) { continue; }
+        Motion m = loadMotion(motion, i);
+        float kept = b.kept;
+        integrateVelocity(b, ctx, m.v, m.w, kept);
+        b.kept = kept;
+        storeVelocity(motion, i, m);
+    }
+}
+
+// integratePositions: limitSpeeds, then each moving body's pose (src/world.cpp:1161-1172).
+__global__ void k_integrate_positions(tfs_body* bodies, uint32_t n, float4* motion, Ctx ctx) {
+    for (uint32_t i = gid(); i < n; i += gstride()) {
+        tfs_body& b = bodies[i];
+        if (!b.alive) { continue; }
+        Motion m = loadMotion(motion, i);
+        float kept = b.kept;
+        limitSpeed(b, m.v, m.w, kept, ctx.maxSpeed, ctx.maxSpin);
+        b.kept = kept;
+        if (b.type != TFS_STATIC && !(b.type == TFS_DYNAMIC && !b.awake)) { integratePosition(b, m, ctx.h); }
+        storeMotion(motion, i, m);
+    }
+}
+
+__global__ void k_limit_speeds(tfs_body* bodies, uint32_t n, float4* motion, Ctx ctx) {
+    for (uint32_t i = gid(); i < n; i += gstride()) {
+        tfs_body& b = bodies[i];
+        if (!b.alive) { continue; }
+        Motion m = loadMotion(motion, i);
+        float kept = b.kept;
+        limitSpeed(b, m.v, m.w, kept, ctx.maxSpeed, ctx.maxSpin);
+        b.kept = kept;
+        storeVelocity(motion, i, m);
+    }
+}
+
+__global__ void k_writeback(tfs_body* bodies, uint32_t n, const float4* motion) {
+    for (uint32_t i = gid(); i < n; i += gstride()) {
+        const Motion m = loadMotion(motion, i);
+        tfs_body& b = bodies[i];
+        st3(b.v, m.v);
+        st3(b.w, m.w);
+        st3(b.dp, m.dp);
+        st4(b.dq, m.dq);
+    }
+}
+
+// ---- refresh: src/world.cpp:1660-1668, 1057-1139 --------------------------------------------------
+
+__global__ void k_update_mass_awake(tfs_body* bodies, uint32_t n, float4* mass) {
+    for (uint32_t i = gid(); i < n; i += gstride()) {
+        tfs_body& b = bodies[i];
+        if (!(b.alive && b.type == TFS_DYNAMIC && b.awake)) { continue; }
+        updateInverseMass(b);
+        storeMass(mass, i, b);
+    }
+}
+
+__global__ void k_mark(const uint32_t* list, uint32_t n, uint32_t* stamp, uint32_t serial) {
+    for (uint32_t k = gid(); k < n; k += gstride()) { stamp[list[k]] = serial; }
+}
+
+__global__ void k_rebase(const uint32_t* active, const DevCounts* cnt, const uint32_t* stamp, uint32_t serial,
+                         tfs_contact* contacts, const float4* motion) {
+    const uint32_t n = cnt->nActive;
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        const uint32_t ci = active[k];
+        if (stamp[ci] == serial) { continue; }
+        tfs_contact& c = contacts[ci];
+        rebaseContact(loadMotion(motion, c.body_a), loadMotion(motion, c.body_b), c);
+    }
+}
+
+__global__ void k_reset_pose(tfs_body* bodies, uint32_t n, float4* motion) {
+    for (uint32_t i = gid(); i < n; i += gstride()) {
+        tfs_body& b = bodies[i];
+        for (int k = 0; k < 4; ++k) { b.q_base[k] = b.q[k]; }
+        motion[4 * i + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        motion[4 * i + 3] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+    }
+}
+
+// The solver side of each refreshed fast contact (src/world.cpp:1108-1138).
+__global__ void k_refresh_apply(const uint32_t* again, uint32_t n, const tfs_contact* fresh, tfs_contact* contacts,
+                                const tfs_body* bodies, const float4* motion, const float4* mass, float matchSq) {
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        tfs_contact& c = contacts[again[k]];
+        const tfs_contact& f = fresh[k];
+        if (f.count <= 0) {
+            c.count = 0;
+            continue;
+        }
+        const tfs_contact old = c;
+        tfs_contact x = c;
+        for (int j = 0; j < 3; ++j) {
+            x.normal[j] = f.normal[j];
+            x.tangent1[j] = f.tangent1[j];
+            x.tangent2[j] = f.tangent2[j];
+        }
+        x.count = f.count > 4 ? 4 : f.count;
+        for (int i = 0; i < x.count; ++i) {
+            tfs_point& p = x.points[i];
+            p = tfs_point{};
+            for (int j = 0; j < 3; ++j) {
+                p.local_a[j] = f.points[i].local_a[j];
+                p.anchor_a[j] = f.points[i].anchor_a[j];
+                p.anchor_b[j] = f.points[i].anchor_b[j];
+            }
+            p.separation = f.points[i].separation;
+        }
+        const tfs_body& ta = bodies[x.body_a];
+        const tfs_body& tb = bodies[x.body_b];
+        prepareContact(viewOf(ta), viewOf(tb), ta, tb, loadMotion(motion, x.body_a), loadMotion(motion, x.body_b),
+                       loadMass(mass, x.body_a), loadMass(mass, x.body_b), x);
+        for (int i = 0; i < x.count; ++i) {
+            const int near = nearestPoint(old, ld3(x.points[i].local_a));
+            if (near < 0) { continue; }
+            x.points[i].relative_velocity = old.points[near].relative_velocity;
+            x.points[i].max_normal_impulse = old.points[near].max_normal_impulse;
+        }
+        carryImpulses(old, x, matchSq);
+        c = x;
+    }
+}
+
+// ---- updateSleep: src/world.cpp:1174-1247 ---------------------------------------------------------
+
+__global__ void k_sleep_time(tfs_body* bodies, uint32_t n, float dt, float threshold, uint32_t* parent, uint32_t* islandTime) {
+    for (uint32_t i = gid(); i < n; i += gstride()) {
+        parent[i] = i;
+        islandTime[i] = 0x7f7fffffu;  // FLT_MAX
+        tfs_body& b = bodies[i];
+        if (!b.alive || b.type != TFS_DYNAMIC || !b.awake) { continue; }
+        const float speed = maxf(length(ld3(b.v)), length(ld3(b.w)) * b.extent);
+        if (!b.can_sleep || speed > threshold) {
+            b.sleep_time = 0.0f;
+        } else {
+            b.sleep_time += dt;
+        }
+    }
+}
+
+__device__ __forceinline__ bool awakeDynamic(const tfs_body& b) { return b.type == TFS_DYNAMIC && b.awake; }
+
+__global__ void k_sleep_kinematic(const tfs_contact* contacts, uint32_t nc, tfs_body* bodies) {
+    for (uint32_t k = gid(); k < nc; k += gstride()) {
+        const tfs_contact& c = contacts[k];
+        if (c.count == 0) { continue; }
+        tfs_body& a = bodies[c.body_a];
+        tfs_body& b = bodies[c.body_b];
+        const bool da = awakeDynamic(a), db = awakeDynamic(b);
+        if (da && db) { continue; }
+        if (da && b.type == TFS_KINEMATIC && movingAwake(b, ld3(b.v), ld3(b.w))) {
+            a.sleep_time = 0.0f;
+        } else if (db && a.type == TFS_KINEMATIC && movingAwake(a, ld3(a.v), ld3(a.w))) {
+            b.sleep_time = 0.0f;
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t rootOf(const uint32_t* parent, uint32_t x) {
+    uint32_t p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (p != x) {
+        x = p;
+        p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return x;
+}
+
+// Links the islands of two awake dynamic bodies in contact. Every link points from a larger index
+// to a smaller one, so each island ends with its lowest body index as root, as the reference's
+// union by the smaller root gives (src/world.cpp:1193-1194, include/twofold/world.h:464-465).
+__global__ void k_sleep_hook(const tfs_contact* contacts, uint32_t nc, const tfs_body* bodies, uint32_t* parent,
+                             uint32_t* changed) {
+    for (uint32_t k = gid(); k < nc; k += gstride()) {
+        const tfs_contact& c = contacts[k];
+        if (c.count == 0) { continue; }
+        if (!awakeDynamic(bodies[c.body_a]) || !awakeDynamic(bodies[c.body_b])) { continue; }
+        const uint32_t ra = rootOf(parent, c.body_a), rb = rootOf(parent, c.body_b);
+        if (ra == rb) { continue; }
+        atomicMin(&parent[ra > rb ? ra : rb], ra < rb ? ra : rb);
+        *changed = 1;
+    }
+}
+
+__global__ void k_sleep_jump(uint32_t* parent, uint32_t n) {
+    for (uint32_t i = gid(); i < n; i += gstride()) { parent[i] = rootOf(parent, i); }
+}
+
+__global__ void k_island_time(tfs_body* bodies, uint32_t n, const uint32_t* parent, uint32_t* islandTime) {
+    for (uint32_t i = gid(); i < n; i += gstride()) {
+        tfs_body& b = bodies[i];
+        if (!b.alive || !awakeDynamic(b)) { continue; }
+        const uint32_t r = parent[i];
+        b.island = r;
+        uint32_t t;
+        memcpy(&t, &b.sleep_time, 4);
+        atomicMin(&islandTime[r], t);
+    }
+}
+
+__global__ void k_sleep_apply(tfs_body* bodies, uint32_t n, const uint32_t* parent, const uint32_t* islandTime,
+                              float timeToSleep, float4* motion, float4* mass) {
+    for (uint32_t i = gid(); i < n; i += gstride()) {
+        tfs_body& b = bodies[i];
+        if (!b.alive || !awakeDynamic(b)) { continue; }
+        float t;
+        memcpy(&t, &islandTime[parent[i]], 4);
+        if (t < timeToSleep) { continue; }
+        b.awake = 0;
+        b.v[0] = b.v[1] = b.v[2] = 0.0f;
+        b.w[0] = b.w[1] = b.w[2] = 0.0f;
+        updateInverseMass(b);
+        storeMotion(motion, i, Motion{ vec(0, 0, 0), vec(0, 0, 0), ld3(b.dp), ld4(b.dq) });
+        storeMass(mass, i, b);
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
+    for (uint32_t i = gid(); i < n; i += gstride()) { p[i] = v; }
+}
+__global__ void k_fill_u8(uint8_t* p, uint32_t n, uint8_t v) {
+    for (uint32_t i = gid(); i < n; i += gstride()) { p[i] = v; }
+}
+__global__ void k_fill_u64(unsigned long long* p, uint32_t n) {
+    for (uint32_t i = gid(); i < n; i += gstride()) { p[i] = 0ull; }
+}
+
+// ---- host side -----------------------------------------------------------------------------------
+
+uint32_t blocksFor(uint32_t n, uint32_t perBlock = kThreads, uint32_t cap = 8192) {
+    const uint32_t b = (n + perBlock - 1) / perBlock;
+    return b < 1 ? 1 : (b > cap ? cap : b);
+}
+
+template <class T>
+struct Buf {
+    T* p = nullptr;
+    size_t n = 0;
+};
+
+}  // namespace
+
+struct tfs_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {};
+    std::string error;
+    tfs_settings settings{};
+    uint32_t nb = 0, nc = 0, capB = 0, capC = 0;
+    bool haveBodies = false, haveContacts = false;
+    uint32_t serial = 0;
+    float solveMs = 0.0f, sleepMs = 0.0f;
+    uint32_t lastCounts[6] = {};
+
+    // bodies
+    tfs_body* bodies = nullptr;
+    float4* motion = nullptr;
+    float4* mass = nullptr;
+    uint8_t* moving = nullptr;
+    uint8_t* struck = nullptr;
+    uint32_t* deg = nullptr;
+    uint32_t* epStart = nullptr;
+    uint32_t* cursor = nullptr;
+    unsigned long long* word = nullptr;
+    uint32_t* parent = nullptr;
+    uint32_t* islandTime = nullptr;
+    // contacts
+    tfs_contact* contacts = nullptr;
+    tfs_contact* fresh = nullptr;
+    uint32_t* fActive = nullptr;
+    uint32_t* fFast = nullptr;
+    uint32_t* fFastSolid = nullptr;
+    uint32_t* active = nullptr;
+    uint32_t* fast = nullptr;
+    uint32_t* fastSolid = nullptr;
+    uint32_t* impactFlag = nullptr;
+    uint32_t* impactPos = nullptr;
+    uint32_t* impact = nullptr;
+    uint32_t* rowsList = nullptr;
+    uint32_t* impactRowsList = nullptr;
+    uint8_t* colors = nullptr;
+    uint32_t* ep = nullptr;
+    uint32_t* rank = nullptr;
+    uint32_t* rowOf = nullptr;
+    uint32_t* stamp = nullptr;
+    uint32_t* hist = nullptr;
+    size_t histCap = 0;
+    uint32_t* tileSums = nullptr;
+    size_t tileCap = 0;
+    uint32_t* ticket = nullptr;
+    Rows rows{}, impactRows{};
+    DevCounts* cnt = nullptr;
+    DevCounts* hostCnt = nullptr;
+    // host staging for the refresh callback
+    std::vector<tfs_body> hBodies;
+    std::vector<tfs_contact> hContacts, hFresh;
+    std::vector<uint32_t> hAgain;
+};
+
+namespace {
+
+int fail(tfs_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) { c->error = buf; }
+    return code;
+}
+
+#define HIP_TRY(call)                                                                          \
+    do {                                                                                       \
+        const hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) { return fail(ctx, TFS_E_DEVICE, "%s: %s", #call, hipGetErrorString(e_)); } \
+    } while (0)
+
+#define LAUNCH(kernel, grid, block, ...)                                                        \
+    do {                                                                                       \
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, ctx->stream, __VA_ARGS__);      \
+        const hipError_t e_ = hipGetLastError();                                               \
+        if (e_ != hipSuccess) { return fail(ctx, TFS_E_DEVICE, "%s: %s", #kernel, hipGetErrorString(e_)); } \
+    } while (0)
+
+template <class T>
+int grow(tfs_ctx* ctx, T*& p, size_t n) {
+    if (p) { HIP_TRY(hipFree(p)); }
+    p = nullptr;
+    HIP_TRY(hipMalloc((void**)&p, sizeof(T) * (n ? n : 1)));
+    return 0;
+}
+
+int allocBodies(tfs_ctx* ctx, uint32_t cap) {
+    int r;
+    if ((r = grow(ctx, ctx->bodies, cap)) || (r = grow(ctx, ctx->motion, 4 * (size_t)cap)) ||
+        (r = grow(ctx, ctx->mass, 6 * (size_t)cap)) || (r = grow(ctx, ctx->moving, cap)) ||
+        (r = grow(ctx, ctx->struck, cap)) || (r = grow(ctx, ctx->deg, cap)) || (r = grow(ctx, ctx->epStart, cap)) ||
+        (r = grow(ctx, ctx->cursor, cap)) || (r = grow(ctx, ctx->word, cap)) || (r = grow(ctx, ctx->parent, cap)) ||
+        (r = grow(ctx, ctx->islandTime, cap))) {
+        return r;
+    }
+    ctx->capB = cap;
+    return 0;
+}
+
+int allocRows(tfs_ctx* ctx, Rows& r, uint32_t cap, bool impact) {
+    int e;
+    if ((e = grow(ctx, r.hdr, cap)) || (e = grow(ctx, r.nrm, cap)) || (e = grow(ctx, r.tg1, cap)) ||
+        (e = grow(ctx, r.tg2, cap)) || (e = grow(ctx, r.pa, 4 * (size_t)cap)) || (e = grow(ctx, r.pb, 4 * (size_t)cap))) {
+        return e;
+    }
+    if (impact) {
+        if ((e = grow(ctx, r.at, cap))) { return e; }
+    } else if ((e = grow(ctx, r.imp, 4 * (size_t)cap)) || (e = grow(ctx, r.peak, 4 * (size_t)cap))) {
+        return e;
+    }
+    r.cap = cap;
+    return 0;
+}
+
+int allocContacts(tfs_ctx* ctx, uint32_t cap) {
+    int r;
+    if ((r = grow(ctx, ctx->contacts, cap)) || (r = grow(ctx, ctx->fresh, cap)) || (r = grow(ctx, ctx->fActive, cap)) ||
+        (r = grow(ctx, ctx->fFast, cap)) || (r = grow(ctx, ctx->fFastSolid, cap)) || (r = grow(ctx, ctx->active, cap)) ||
+        (r = grow(ctx, ctx->fast, cap)) || (r = grow(ctx, ctx->fastSolid, cap)) ||
+        (r = grow(ctx, ctx->impactFlag, cap)) || (r = grow(ctx, ctx->impactPos, cap)) ||
+        (r = grow(ctx, ctx->impact, cap)) || (r = grow(ctx, ctx->rowsList, cap)) ||
+        (r = grow(ctx, ctx->impactRowsList, cap)) || (r = grow(ctx, ctx->colors, cap)) ||
+        (r = grow(ctx, ctx->ep, 2 * (size_t)cap)) || (r = grow(ctx, ctx->rank, 2 * (size_t)cap)) ||
+        (r = grow(ctx, ctx->rowOf, cap)) || (r = grow(ctx, ctx->stamp, cap))) {
+        return r;
+    }
+    if ((r = allocRows(ctx, ctx->rows, cap, false)) || (r = allocRows(ctx, ctx->impactRows, cap, true))) { return r; }
+    ctx->rows.at = nullptr;
+    ctx->impactRows.imp = ctx->rows.imp;
+    ctx->impactRows.peak = ctx->rows.peak;
+    ctx->capC = cap;
+    return 0;
+}
+
+int ensureScratch(tfs_ctx* ctx, size_t histEntries, size_t tiles) {
+    int r;
+    if (histEntries > ctx->histCap) {
+        if ((r = grow(ctx, ctx->hist, histEntries))) { return r; }
+        ctx->histCap = histEntries;
+    }
+    if (tiles > ctx->tileCap) {
+        if ((r = grow(ctx, ctx->tileSums, tiles))) { return r; }
+        ctx->tileCap = tiles;
+    }
+    return 0;
+}
+
+// Exclusive scan of in[0..n) (n read from *nPtr on the device when given, else nMax); optionally
+// the prefix array and/or the compacted index list; the total goes to *total.
+int scan(tfs_ctx* ctx, const uint32_t* in, const uint32_t* nPtr, uint32_t nMax, uint32_t* out, uint32_t* list,
+         uint32_t* total) {
+    const uint32_t tiles = (nMax + kTile - 1) / kTile ? (nMax + kTile - 1) / kTile : 1;
+    int r;
+    if ((r = ensureScratch(ctx, 0, tiles))) { return r; }
+    LAUNCH(k_scan_tiles, tiles, kThreads, in, nPtr, nMax, ctx->tileSums);
+    LAUNCH(k_scan_sums, 1, kThreads, ctx->tileSums, tiles, total);
+    LAUNCH(k_scan_apply, tiles, kThreads, in, nPtr, nMax, ctx->tileSums, out, list);
+    return 0;
+}
+
+// Stable row order of `list` (length *nPtr <= nMax) into out; colour starts into starts[0..32].
+int rowOrder(tfs_ctx* ctx, const uint32_t* list, const uint32_t* nPtr, uint32_t nMax, uint32_t* out, uint32_t* starts) {
+    const uint32_t nBlocks = blocksFor(nMax, kThreads, 1u << 30);
+    int r;
+    if ((r = ensureScratch(ctx, (size_t)kKeys * nBlocks, 0))) { return r; }
+    LAUNCH(k_row_hist, nBlocks, kThreads, list, nPtr, ctx->contacts, ctx->colors, ctx->hist, nBlocks);
+    if ((r = scan(ctx, ctx->hist, nullptr, kKeys * nBlocks, ctx->hist, nullptr, &ctx->cnt->scanTotal))) { return r; }
+    LAUNCH(k_row_scatter, nBlocks, kThreads, list, nPtr, ctx->contacts, ctx->colors, ctx->hist, nBlocks, out);
+    LAUNCH(k_row_starts, 1, 64, ctx->hist, nBlocks, starts);
+    return 0;
+}
+
+int runRows(tfs_ctx* ctx, int mode, const Rows& rows, const uint32_t* starts, const Ctx& c) {
+    for (int col = 0; col < TFS_COLORS; ++col) {
+        const uint32_t b = starts[col], e = starts[col + 1];
+        if (e <= b) { continue; }
+        const uint32_t grid = (e - b + kThreads - 1) / kThreads;
+        switch (mode) {
+            case kWarm: LAUNCH(k_rows<kWarm>, grid, kThreads, rows, b, e, ctx->rows, ctx->motion, ctx->mass, c); break;
+            case kBias: LAUNCH(k_rows<kBias>, grid, kThreads, rows, b, e, ctx->rows, ctx->motion, ctx->mass, c); break;
+            case kRelax: LAUNCH(k_rows<kRelax>, grid, kThreads, rows, b, e, ctx->rows, ctx->motion, ctx->mass, c); break;
+            default: LAUNCH(k_rows<kRestitute>, grid, kThreads, rows, b, e, ctx->rows, ctx->motion, ctx->mass, c); break;
+        }
+    }
+    return 0;
+}
+
+int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32_t end, const Ctx& c) {
+    if (end <= begin) { return 0; }
+    LAUNCH(k_rest, 1, 64, list + begin, end - begin, mode, ctx->contacts, ctx->bodies, ctx->motion, ctx->mass, c);
+    return 0;
+}
+
+int syncCounts(tfs_ctx* ctx) {
+    HIP_TRY(hipMemcpyAsync(ctx->hostCnt, ctx->cnt, sizeof(DevCounts), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->hostCnt->error) { return fail(ctx, TFS_E_KERNEL, "a kernel's bounded wait ran out (error %u)", ctx->hostCnt->error); }
+    return 0;
+}
+
+int refresh(tfs_ctx* ctx, int substep, const uint32_t* again, uint32_t nAgain, float remaining, tfs_refresh_fn cb,
+            void* user, const DevCounts& hc) {
+    const uint32_t nb = ctx->nb, nc = ctx->nc;
+    const uint32_t laned = hc.colorStart[TFS_COLORS], impactLaned = hc.impactColorStart[TFS_COLORS];
+    LAUNCH(k_update_mass_awake, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->mass);
+    LAUNCH(k_unpack, blocksFor(laned), kThreads, laned, ctx->rows, ctx->contacts);
+    const uint32_t serial = ++ctx->serial;
+    LAUNCH(k_mark, blocksFor(nAgain), kThreads, again, nAgain, ctx->stamp, serial);
+    LAUNCH(k_rebase, blocksFor(hc.nActive), kThreads, ctx->active, ctx->cnt, ctx->stamp, serial, ctx->contacts, ctx->motion);
+    LAUNCH(k_reset_pose, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion);
+    LAUNCH(k_writeback, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion);
+    ctx->hBodies.resize(nb);
+    ctx->hContacts.resize(nc);
+    ctx->hAgain.resize(nAgain);
+    ctx->hFresh.assign(nAgain, tfs_contact{});
+    HIP_TRY(hipMemcpyAsync(ctx->hBodies.data(), ctx->bodies, sizeof(tfs_body) * nb, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->hContacts.data(), ctx->contacts, sizeof(tfs_contact) * nc, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->hAgain.data(), again, sizeof(uint32_t) * nAgain, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (cb(user, substep, remaining, ctx->hAgain.data(), nAgain, ctx->hBodies.data(), nb, ctx->hContacts.data(), nc,
+           ctx->hFresh.data()) != 0) {
+        return fail(ctx, TFS_E_CALLBACK, "the refresh callback failed at substep %d", substep);
+    }
+    HIP_TRY(hipMemcpyAsync(ctx->fresh, ctx->hFresh.data(), sizeof(tfs_contact) * nAgain, hipMemcpyHostToDevice, ctx->stream));
+    const float matchSq = ctx->settings.speculative * ctx->settings.speculative;
+    LAUNCH(k_refresh_apply, blocksFor(nAgain), kThreads, again, nAgain, ctx->fresh, ctx->contacts, ctx->bodies,
+           ctx->motion, ctx->mass, matchSq);
+    LAUNCH(k_pack, blocksFor(laned), kThreads, ctx->rowsList, laned, ctx->contacts, ctx->moving, ctx->rows, ctx->rowOf,
+           (const uint32_t*)nullptr);
+    LAUNCH(k_pack, blocksFor(impactLaned), kThreads, ctx->impactRowsList, impactLaned, ctx->contacts, ctx->moving,
+           ctx->impactRows, (uint32_t*)nullptr, ctx->rowOf);
+    // The host waits for the copy of `fresh` before reusing its staging buffer next time.
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
+    const uint32_t nb = ctx->nb, nc = ctx->nc;
+    const tfs_settings& s = ctx->settings;
+    const Ctx c = makeCtx(s, dt);
+    const int n = s.substeps > 1 ? s.substeps : 1;
+    HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->cnt, 0, sizeof(DevCounts), ctx->stream));
+
+    // step state and lists
+    LAUNCH(k_step_init, blocksFor(nb), kThreads, ctx->bodies, nb, dt, ctx->motion, ctx->mass, ctx->moving);
+    LAUNCH(k_classify, blocksFor(nc), kThreads, ctx->contacts, nc, ctx->bodies, dt, s.speculative, ctx->fActive,
+           ctx->fFast, ctx->fFastSolid);
+    int r;
+    if ((r = scan(ctx, ctx->fActive, nullptr, nc, nullptr, ctx->active, &ctx->cnt->nActive)) ||
+        (r = scan(ctx, ctx->fFast, nullptr, nc, nullptr, ctx->fast, &ctx->cnt->nFast)) ||
+        (r = scan(ctx, ctx->fFastSolid, nullptr, nc, nullptr, ctx->fastSolid, &ctx->cnt->nFastSolid))) {
+        return r;
+    }
+
+    // prepare, impact
+    HIP_TRY(hipMemsetAsync(ctx->struck, 0, nb ? nb : 1, ctx->stream));
+    LAUNCH(k_prepare, blocksFor(nc), kThreads, ctx->active, ctx->cnt, ctx->contacts, ctx->bodies, ctx->motion, ctx->mass,
+           c.restitutionThreshold, ctx->struck, &ctx->cnt->pointsActive);
+    LAUNCH(k_impact_flag, blocksFor(nc), kThreads, ctx->active, ctx->cnt, ctx->contacts, ctx->struck, ctx->impactFlag);
+    if ((r = scan(ctx, ctx->impactFlag, &ctx->cnt->nActive, nc, nullptr, ctx->impactPos, &ctx->cnt->nImpact))) { return r; }
+    LAUNCH(k_gather_list, blocksFor(nc), kThreads, ctx->impactPos, &ctx->cnt->nImpact, ctx->active, ctx->impact);
+    LAUNCH(k_points_of, blocksFor(nc), kThreads, ctx->impact, &ctx->cnt->nImpact, ctx->contacts, &ctx->cnt->pointsImpact);
+
+    // colouring
+    HIP_TRY(hipMemsetAsync(ctx->deg, 0, sizeof(uint32_t) * (nb ? nb : 1), ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->cursor, 0, sizeof(uint32_t) * (nb ? nb : 1), ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->word, 0, sizeof(unsigned long long) * (nb ? nb : 1), ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->ticket, 0, sizeof(uint32_t), ctx->stream));
+    HIP_TRY(hipMemsetAsync(ctx->colors, 0xff, nc ? nc : 1, ctx->stream));
+    LAUNCH(k_ep_count, blocksFor(nc), kThreads, ctx->active, ctx->cnt, ctx->contacts, ctx->moving, ctx->deg);
+    if ((r = scan(ctx, ctx->deg, nullptr, nb, ctx->epStart, nullptr, &ctx->cnt->scanTotal))) { return r; }
+    LAUNCH(k_ep_fill, blocksFor(nc), kThreads, ctx->active, ctx->cnt, ctx->contacts, ctx->moving, ctx->epStart,
+           ctx->cursor, ctx->ep);
+    LAUNCH(k_ep_rank, blocksFor(nb), kThreads, nb, ctx->deg, ctx->epStart, ctx->ep, ctx->rank);
+    LAUNCH(k_color, blocksFor(nc, kThreads, 2048), kThreads, ctx->active, ctx->cnt, ctx->contacts, ctx->moving,
+           ctx->rank, ctx->word, ctx->colors, ctx->ticket, ctx->cnt);
+
+    // row order and packing
+    if ((r = rowOrder(ctx, ctx->active, &ctx->cnt->nActive, nc, ctx->rowsList, ctx->cnt->colorStart)) ||
+        (r = rowOrder(ctx, ctx->impact, &ctx->cnt->nImpact, nc, ctx->impactRowsList, ctx->cnt->impactColorStart))) {
+        return r;
+    }
+    if ((r = syncCounts(ctx))) { return r; }
+    const DevCounts hc = *ctx->hostCnt;
+    const uint32_t laned = hc.colorStart[TFS_COLORS], impactLaned = hc.impactColorStart[TFS_COLORS];
+    if (n > 1 && !cb) {
+        bool needs = false;
+        for (int i = 1; i < n; ++i) { needs = needs || (i == n / 2 ? hc.nFast : hc.nFastSolid) > 0; }
+        if (needs) { return fail(ctx, TFS_E_CALLBACK, "fast contacts need a refresh callback"); }
+    }
+    LAUNCH(k_pack, blocksFor(laned), kThreads, ctx->rowsList, laned, ctx->contacts, ctx->moving, ctx->rows, ctx->rowOf,
+           (const uint32_t*)nullptr);
+    LAUNCH(k_pack, blocksFor(impactLaned), kThreads, ctx->impactRowsList, impactLaned, ctx->contacts, ctx->moving,
+           ctx->impactRows, (uint32_t*)nullptr, ctx->rowOf);
+
+    // substeps: src/world.cpp:1658-1681
+    for (int i = 0; i < n; ++i) {
+        const bool half = i == n / 2;
+        const uint32_t nAgain = half ? hc.nFast : hc.nFastSolid;
+        if (i > 0 && nAgain > 0) {
+            if ((r = refresh(ctx, i, half ? ctx->fast : ctx->fastSolid, nAgain, (float)(n - i) * c.h, cb, user, hc))) {
+                return r;
+            }
+        }
+        LAUNCH(k_integrate_velocities, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c);
+        if ((r = runRows(ctx, kWarm, ctx->rows, hc.colorStart, c)) ||
+            (r = runRest(ctx, kWarm, ctx->rowsList, laned, hc.nActive, c)) ||
+            (r = runRows(ctx, kBias, ctx->rows, hc.colorStart, c)) ||
+            (r = runRest(ctx, kBias, ctx->rowsList, laned, hc.nActive, c))) {
+            return r;
+        }
+        if (hc.nImpact) {
+            if ((r = runRows(ctx, kBias, ctx->impactRows, hc.impactColorStart, c)) ||
+                (r = runRest(ctx, kBias, ctx->impactRowsList, impactLaned, hc.nImpact, c))) {
+                return r;
+            }
+        }
+        LAUNCH(k_integrate_positions, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c);
+        if ((r = runRows(ctx, kRelax, ctx->rows, hc.colorStart, c)) ||
+            (r = runRest(ctx, kRelax, ctx->rowsList, laned, hc.nActive, c))) {
+            return r;
+        }
+        if (hc.nImpact) {
+            if ((r = runRows(ctx, kRelax, ctx->impactRows, hc.impactColorStart, c)) ||
+                (r = runRest(ctx, kRelax, ctx->impactRowsList, impactLaned, hc.nImpact, c))) {
+                return r;
+            }
+        }
+    }
+    if ((r = runRows(ctx, kRestitute, ctx->rows, hc.colorStart, c)) ||
+        (r = runRest(ctx, kRestitute, ctx->rowsList, laned, hc.nActive, c))) {
+        return r;
+    }
+    LAUNCH(k_limit_speeds, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c);
+    LAUNCH(k_unpack, blocksFor(laned), kThreads, laned, ctx->rows, ctx->contacts);
+    LAUNCH(k_writeback, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion);
+    HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
+
+    // sleep and islands
+    LAUNCH(k_sleep_time, blocksFor(nb), kThreads, ctx->bodies, nb, dt, s.sleep_threshold * s.length_unit, ctx->parent,
+           ctx->islandTime);
+    LAUNCH(k_sleep_kinematic, blocksFor(nc), kThreads, ctx->contacts, nc, ctx->bodies);
+    for (int it = 0; it < 64; ++it) {
+        HIP_TRY(hipMemsetAsync(&ctx->cnt->changed, 0, sizeof(uint32_t), ctx->stream));
+        LAUNCH(k_sleep_hook, blocksFor(nc), kThreads, ctx->contacts, nc, ctx->bodies, ctx->parent, &ctx->cnt->changed);
+        LAUNCH(k_sleep_jump, blocksFor(nb), kThreads, ctx->parent, nb);
+        uint32_t changed = 0;
+        HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->changed, &ctx->cnt->changed, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        changed = ctx->hostCnt->changed;
+        if (!changed) { break; }
+    }
+    LAUNCH(k_island_time, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime);
+    LAUNCH(k_sleep_apply, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime, s.time_to_sleep,
+           ctx->motion, ctx->mass);
+    HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->ev[2]));
+    HIP_TRY(hipEventElapsedTime(&ctx->solveMs, ctx->ev[0], ctx->ev[1]));
+    HIP_TRY(hipEventElapsedTime(&ctx->sleepMs, ctx->ev[1], ctx->ev[2]));
+    uint32_t nColors = 0;
+    for (int col = 0; col < TFS_COLORS; ++col) {
+        if (hc.colorStart[col + 1] > hc.colorStart[col]) { nColors = (uint32_t)col + 1; }
+    }
+    const uint32_t counts[6] = { hc.nActive, hc.pointsActive, hc.nImpact, hc.pointsImpact, nColors, hc.nActive - laned };
+    memcpy(ctx->lastCounts, counts, sizeof counts);
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tfs_abi_version(void) { return TFS_ABI_VERSION; }
+size_t tfs_sizeof_body(void) { return sizeof(tfs_body); }
+size_t tfs_sizeof_contact(void) { return sizeof(tfs_contact); }
+size_t tfs_sizeof_settings(void) { return sizeof(tfs_settings); }
+
+void tfs_default_settings(tfs_settings* s) {
+    if (!s) { return; }
+    *s = tfs_settings{};
+    s->gravity[1] = -9.81f;
+    s->length_unit = 1.0f;
+    s->substeps = 4;
+    s->contact_hertz = 30.0f;
+    s->contact_damping_ratio = 10.0f;
+    s->max_pushout_speed = 3.0f;
+    s->max_linear_speed = 1000.0f;
+    s->max_angular_speed = 1000.0f;
+    s->restitution_threshold = 1.0f;
+    s->sleep_threshold = 0.05f;
+    s->time_to_sleep = 0.5f;
+    s->speculative = 0.02f;
+}
+
+int tfs_create(int device, uint32_t maxBodies, uint32_t maxContacts, tfs_ctx** out) {
+    if (!out) { return TFS_E_ARG; }
+    *out = nullptr;
+    tfs_ctx* ctx = new tfs_ctx();
+    tfs_default_settings(&ctx->settings);
+    ctx->device = device;
+    int r = 0;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return TFS_E_DEVICE;
+    }
+    for (hipEvent_t& e : ctx->ev) {
+        if (hipEventCreate(&e) != hipSuccess) { r = TFS_E_DEVICE; }
+    }
+    if (!r && hipMalloc((void**)&ctx->cnt, sizeof(DevCounts)) != hipSuccess) { r = TFS_E_DEVICE; }
+    if (!r && hipHostMalloc((void**)&ctx->hostCnt, sizeof(DevCounts), hipHostMallocDefault) != hipSuccess) { r = TFS_E_DEVICE; }
+    if (!r && hipMalloc((void**)&ctx->ticket, sizeof(uint32_t)) != hipSuccess) { r = TFS_E_DEVICE; }
+    if (!r) { r = allocBodies(ctx, maxBodies ? maxBodies : 1); }
+    if (!r) { r = allocContacts(ctx, maxContacts ? maxContacts : 1); }
+    if (r) {
+        tfs_destroy(ctx);
+        return r;
+    }
+    *out = ctx;
+    return 0;
+}
+
+void tfs_destroy(tfs_ctx* ctx) {
+    if (!ctx) { return; }
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) { (void)hipStreamSynchronize(ctx->stream); }
+    void* ptrs[] = { ctx->bodies, ctx->motion, ctx->mass, ctx->moving, ctx->struck, ctx->deg, ctx->epStart, ctx->cursor,
+                     ctx->word, ctx->parent, ctx->islandTime, ctx->contacts, ctx->fresh, ctx->fActive, ctx->fFast,
+                     ctx->fFastSolid, ctx->active, ctx->fast, ctx->fastSolid, ctx->impactFlag, ctx->impactPos,
+                     ctx->impact, ctx->rowsList, ctx->impactRowsList, ctx->colors, ctx->ep, ctx->rank, ctx->rowOf,
+                     ctx->stamp, ctx->hist, ctx->tileSums, ctx->ticket, ctx->cnt, ctx->rows.hdr, ctx->rows.nrm,
+                     ctx->rows.tg1, ctx->rows.tg2, ctx->rows.pa, ctx->rows.pb, ctx->rows.imp, ctx->rows.peak,
+                     ctx->impactRows.hdr, ctx->impactRows.nrm, ctx->impactRows.tg1, ctx->impactRows.tg2,
+                     ctx->impactRows.pa, ctx->impactRows.pb, ctx->impactRows.at };
+    for (void* p : ptrs) {
+        if (p) { (void)hipFree(p); }
+    }
+    if (ctx->hostCnt) { (void)hipHostFree(ctx->hostCnt); }
+    for (hipEvent_t e : ctx->ev) {
+        if (e) { (void)hipEventDestroy(e); }
+    }
+    if (ctx->stream) { (void)hipStreamDestroy(ctx->stream); }
+    delete ctx;
+}
+
+const char* tfs_last_error(const tfs_ctx* ctx) { return ctx ? ctx->error.c_str() : "no context"; }
+
+int tfs_set_settings(tfs_ctx* ctx, const tfs_settings* s) {
+    if (!ctx || !s) { return fail(ctx, TFS_E_ARG, "null argument"); }
+    if (s->substeps < 1 || s->substeps > 64) { return fail(ctx, TFS_E_ARG, "substeps %d out of range", s->substeps); }
+    ctx->settings = *s;
+    return 0;
+}
+
+int tfs_upload_bodies(tfs_ctx* ctx, const tfs_body* bodies, uint32_t n) {
+    if (!ctx || (!bodies && n)) { return fail(ctx, TFS_E_ARG, "null argument"); }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (n > ctx->capB) {
+        int r = allocBodies(ctx, n);
+        if (r) { return r; }
+    }
+    if (n) { HIP_TRY(hipMemcpy(ctx->bodies, bodies, sizeof(tfs_body) * n, hipMemcpyHostToDevice)); }
+    ctx->nb = n;
+    ctx->haveBodies = true;
+    return 0;
+}
+
+int tfs_upload_contacts(tfs_ctx* ctx, const tfs_contact* contacts, uint32_t n) {
+    if (!ctx || (!contacts && n)) { return fail(ctx, TFS_E_ARG, "null argument"); }
+    if (!ctx->haveBodies) { return fail(ctx, TFS_E_STATE, "upload bodies before contacts"); }
+    for (uint32_t i = 0; i < n; ++i) {
+        const tfs_contact& c = contacts[i];
+        if (c.body_a >= ctx->nb || c.body_b >= ctx->nb) {
+            return fail(ctx, TFS_E_ARG, "contact %u names body %u/%u of %u", i, c.body_a, c.body_b, ctx->nb);
+        }
+        if (c.count < 0 || c.count > 4) { return fail(ctx, TFS_E_ARG, "contact %u has %d points", i, c.count); }
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (n > ctx->capC) {
+        int r = allocContacts(ctx, n);
+        if (r) { return r; }
+    }
+    if (n) { HIP_TRY(hipMemcpy(ctx->contacts, contacts, sizeof(tfs_contact) * n, hipMemcpyHostToDevice)); }
+    ctx->nc = n;
+    ctx->haveContacts = true;
+    return 0;
+}
+
+int tfs_step(tfs_ctx* ctx, float dt, tfs_refresh_fn refresh, void* user) {
+    if (!ctx) { return TFS_E_ARG; }
+    if (!ctx->haveBodies || !ctx->haveContacts) { return fail(ctx, TFS_E_STATE, "upload bodies and contacts first"); }
+    if (!(dt > 0.0f) || !(dt <= 3.4e38f)) { return 0; }
+    if (hipSetDevice(ctx->device) != hipSuccess) { return fail(ctx, TFS_E_DEVICE, "hipSetDevice failed"); }
+    return step(ctx, dt, refresh, user);
+}
+
+int tfs_download_bodies(tfs_ctx* ctx, tfs_body* out, uint32_t n) {
+    if (!ctx || (!out && n)) { return fail(ctx, TFS_E_ARG, "null argument"); }
+    if (n > ctx->nb) { return fail(ctx, TFS_E_ARG, "asked for %u bodies of %u", n, ctx->nb); }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (n) { HIP_TRY(hipMemcpy(out, ctx->bodies, sizeof(tfs_body) * n, hipMemcpyDeviceToHost)); }
+    return 0;
+}
+
+int tfs_download_contacts(tfs_ctx* ctx, tfs_contact* out, uint32_t n) {
+    if (!ctx || (!out && n)) { return fail(ctx, TFS_E_ARG, "null argument"); }
+    if (n > ctx->nc) { return fail(ctx, TFS_E_ARG, "asked for %u contacts of %u", n, ctx->nc); }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (n) { HIP_TRY(hipMemcpy(out, ctx->contacts, sizeof(tfs_contact) * n, hipMemcpyDeviceToHost)); }
+    return 0;
+}
+
+int tfs_download_list(tfs_ctx* ctx, int kind, void* out, uint32_t cap, uint32_t* count) {
+    if (!ctx || !count) { return fail(ctx, TFS_E_ARG, "null argument"); }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const DevCounts& h = *ctx->hostCnt;
+    const void* src = nullptr;
+    uint32_t n = 0;
+    size_t size = sizeof(uint32_t);
+    switch (kind) {
+        case 0: src = ctx->active; n = h.nActive; break;
+        case 1: src = ctx->fast; n = h.nFast; break;
+        case 2: src = ctx->fastSolid; n = h.nFastSolid; break;
+        case 3: src = ctx->impact; n = h.nImpact; break;
+        case 4: src = ctx->rowsList; n = h.nActive; break;
+        case 5: src = ctx->impactRowsList; n = h.nImpact; break;
+        case 6: src = ctx->colors; n = ctx->nc; size = 1; break;
+        default: return fail(ctx, TFS_E_ARG, "unknown list kind %d", kind);
+    }
+    *count = n;
+    const uint32_t m = n < cap ? n : cap;
+    if (m && out) { HIP_TRY(hipMemcpy(out, src, size * m, hipMemcpyDeviceToHost)); }
+    return 0;
+}
+
+int tfs_last_step_ms(tfs_ctx* ctx, float* solveMs, float* sleepMs) {
+    if (!ctx) { return TFS_E_ARG; }
+    if (solveMs) { *solveMs = ctx->solveMs; }
+    if (sleepMs) { *sleepMs = ctx->sleepMs; }
+    return 0;
+}
+
+int tfs_last_step_counts(tfs_ctx* ctx, uint32_t counts[6]) {
+    if (!ctx || !counts) { return TFS_E_ARG; }
+    memcpy(counts, ctx->lastCounts, sizeof ctx->lastCounts);
+    return 0;
+}
+
+}  // extern "C"

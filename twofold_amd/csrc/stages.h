@@ -1,0 +1,714 @@
+// stages.h - the numerics of each stage of the solve path, per body or per contact.
+//
+// The kernels in solver.hip load a body's or a row's data into the structs below, call one of these
+// functions and store the result. Each function takes the float operations of the reference
+// function it cites, in the same order. Point axes (src/solver.cpp:5-17) are rebuilt where they are
+// used from the same body matrices and anchors the reference builds them from once per step; that
+// gives the same bits and reads far fewer bytes than storing them.
+#pragma once
+
+#include "../../include/twofold_solver.h"
+#include "fmath.h"
+
+namespace tf {
+
+// Row flags: point count, which bodies move, whether the contact is solved with the static
+// softness (src/lanes.cpp:77-84).
+enum : uint32_t { kCountMask = 7u, kMoveA = 8u, kMoveB = 16u };
+
+// A body's motion during the step, as the solve passes read and write it (src/lanes.cpp:112-131).
+struct Motion {
+    V3 v, w, dp;
+    Q4 dq;
+};
+
+// A body's inverse mass data, fixed between inverse-mass updates (src/world.cpp:62-92).
+struct Mass {
+    M3 imw, iiw;
+    float invMass, pim, pii;
+};
+
+struct Soft {
+    float biasRate, massScale, impulseScale;
+};
+
+struct Ctx {
+    float h, invH;
+    Soft contactSoft, staticSoft;
+    float maxPushout, restitutionThreshold, maxSpeed, maxSpin;
+    V3 gravity;
+};
+
+TF_HD bool isPlanarLocks(uint32_t locks) { return (locks & TFS_LOCK_PLANAR) == TFS_LOCK_PLANAR; }
+TF_HD bool movingBody(const tfs_body& b) { return b.type == TFS_DYNAMIC && b.awake; }
+
+// makeSoft: src/solver.cpp:255-262.
+TF_HD Soft makeSoft(float hertz, float zeta, float h) {
+    if (hertz == 0.0f) { return Soft{ 0.0f, 1.0f, 0.0f }; }
+    const float omega = 2.0f * 3.14159265358979f * hertz;
+    const float a1 = 2.0f * zeta + h * omega;
+    const float a2 = h * omega * a1;
+    const float a3 = 1.0f / (1.0f + a2);
+    return Soft{ omega / a1, a2 * a3, a3 };
+}
+
+// SolverContext: src/world.cpp:1634-1642, plus the speed limits of src/world.cpp:1142, 1154-1155.
+inline Ctx makeCtx(const tfs_settings& s, float dt) {
+    Ctx c;
+    const int n = s.substeps > 1 ? s.substeps : 1;
+    c.h = dt / (float)n;
+    c.invH = 1.0f / c.h;
+    const float hertz = minf(s.contact_hertz, 0.25f * c.invH);
+    c.contactSoft = makeSoft(hertz, s.contact_damping_ratio, c.h);
+    c.staticSoft = makeSoft(2.0f * hertz, s.contact_damping_ratio, c.h);
+    c.maxPushout = s.max_pushout_speed * s.length_unit;
+    c.restitutionThreshold = s.restitution_threshold * s.length_unit;
+    c.maxSpeed = s.max_linear_speed * s.length_unit;
+    c.maxSpin = s.max_angular_speed;
+    c.gravity = ld3(s.gravity);
+    return c;
+}
+
+// ---- bodies --------------------------------------------------------------------------------------
+
+// invertSubspace: src/world.cpp:44-60.
+TF_HD M3 invertSubspace(const M3& m, const V3* axes, int k) {
+    if (k == 0) { return mzero(); }
+    if (k == 3) { return inverse(m); }
+    if (k == 1) {
+        const float d = dot(axes[0], m * axes[0]);
+        return d > 0.0f ? outer(axes[0], axes[0]) * (1.0f / d) : mzero();
+    }
+    const float a = dot(axes[0], m * axes[0]);
+    const float b = dot(axes[0], m * axes[1]);
+    const float c = dot(axes[1], m * axes[0]);
+    const float d = dot(axes[1], m * axes[1]);
+    float det = a * d - b * c;
+    if (det == 0.0f) { return mzero(); }
+    det = 1.0f / det;
+    return outer(axes[0], axes[0]) * (d * det) - outer(axes[0], axes[1]) * (b * det) -
+           outer(axes[1], axes[0]) * (c * det) + outer(axes[1], axes[1]) * (a * det);
+}
+
+// updateInverseMass: src/world.cpp:62-92.
+TF_HD void updateInverseMass(tfs_body& b) {
+    const M3 lf = toMat3(ld4(b.lock_frame));
+    V3 ang[3];
+    int na = 0;
+    M3 lp = mzero(), ap = mzero();
+    for (int i = 0; i < 3; ++i) {
+        const V3 axis = column(lf, i);
+        if (!(b.locks & (TFS_LOCK_TX << i))) { lp = lp + outer(axis, axis); }
+        if (!(b.locks & (TFS_LOCK_RX << i))) {
+            ang[na++] = axis;
+            ap = ap + outer(axis, axis);
+        }
+    }
+    st9(b.linear_proj, lp);
+    st9(b.angular_proj, ap);
+    if (b.type != TFS_DYNAMIC || !b.awake) {
+        st9(b.inv_mass_world, mzero());
+        st9(b.inv_inertia_world, mzero());
+        b.plane_inv_mass = 0.0f;
+        b.plane_inv_inertia = 0.0f;
+        return;
+    }
+    st9(b.inv_mass_world, lp * b.inv_mass);
+    const M3 r = toMat3(ld4(b.q));
+    const M3 iw = mul(mul(r, ld9(b.inertia)), transpose(r));
+    const M3 inv = invertSubspace(iw, ang, na);
+    st9(b.inv_inertia_world, inv);
+    if (isPlanarLocks(b.locks)) {
+        const V3 n = rotate(ld4(b.lock_frame), vec(0, 0, 1));
+        b.plane_inv_mass = b.inv_mass;
+        b.plane_inv_inertia = dot(n, inv * n);
+    }
+}
+
+TF_HD Mass massOf(const tfs_body& b) {
+    return Mass{ ld9(b.inv_mass_world), ld9(b.inv_inertia_world), b.inv_mass, b.plane_inv_mass, b.plane_inv_inertia };
+}
+
+// limitSpeed: src/solver.cpp:30-41.
+TF_HD void limitSpeed(const tfs_body& b, V3& v, V3& w, float& kept, float maxSpeed, float maxSpin) {
+    if (b.type != TFS_DYNAMIC || !b.awake) { return; }
+    const float speed = length(v), spin = length(w);
+    if (speed > maxSpeed) {
+        v = v * (maxSpeed / speed);
+        kept = minf(kept, maxSpeed / speed);
+    }
+    if (spin > maxSpin) {
+        w = w * (maxSpin / spin);
+        kept = minf(kept, maxSpin / spin);
+    }
+}
+
+// integrateVelocities, one body: src/world.cpp:1141-1151.
+TF_HD void integrateVelocity(const tfs_body& b, const Ctx& c, V3& v, V3& w, float& kept) {
+    v = v + (c.gravity * b.gravity_scale + ld3(b.force) * b.inv_mass) * c.h;
+    v = ld9(b.linear_proj) * (v * (1.0f / (1.0f + c.h * b.linear_damping)));
+    w = w + (ld9(b.inv_inertia_world) * ld3(b.torque)) * c.h;
+    w = ld9(b.angular_proj) * (w * (1.0f / (1.0f + c.h * b.angular_damping)));
+    limitSpeed(b, v, w, kept, c.maxSpeed, c.maxSpin);
+}
+
+// integratePositions, one body, after limitSpeeds: src/world.cpp:1161-1172.
+TF_HD void integratePosition(tfs_body& b, Motion& m, float h) {
+    const V3 vh = m.v * h;
+    const V3 com = ld3(b.com) + vh;
+    st3(b.com, com);
+    m.dp = m.dp + vh;
+    const Q4 q = integrate(ld4(b.q), m.w, h);
+    st4(b.q, q);
+    m.dq = q * conjugate(ld4(b.q_base));
+    st3(b.p, com - rotate(q, ld3(b.local_center)));
+}
+
+// relativeTravel: src/world.cpp:484-486.
+TF_HD float relativeTravel(const tfs_body& a, const tfs_body& b, float dt) {
+    return (length(ld3(b.v) - ld3(a.v)) + length(ld3(a.w)) * a.extent + length(ld3(b.w)) * b.extent) * dt;
+}
+
+// movingAwake: src/world.cpp:94-98.
+TF_HD bool movingAwake(const tfs_body& b, V3 v, V3 w) {
+    if (b.type == TFS_DYNAMIC) { return b.awake; }
+    if (b.type == TFS_KINEMATIC) { return dot(v, v) > 0.0f || dot(w, w) > 0.0f; }
+    return false;
+}
+
+// ---- contact axes -------------------------------------------------------------------------------
+
+struct Axis {
+    V3 armA, armB, linearA, angularA, linearB, angularB;
+    float mass;
+};
+
+// makeAxis: src/solver.cpp:5-17 (makeDirection, src/lanes.cpp:38-49, is the same per lane).
+TF_HD Axis makeAxis(const Mass& a, const Mass& b, V3 ra, V3 rb, V3 d) {
+    Axis x;
+    x.armA = cross(ra, d);
+    x.armB = cross(rb, d);
+    x.linearA = a.imw * d;
+    x.angularA = a.iiw * x.armA;
+    x.linearB = b.imw * d;
+    x.angularB = b.iiw * x.armB;
+    const float k = dot(d, x.linearA) + dot(d, x.linearB) + dot(x.armA, x.angularA) + dot(x.armB, x.angularB);
+    x.mass = k > 1e-6f * (a.invMass + b.invMass) ? 1.0f / k : 0.0f;
+    return x;
+}
+
+// speedAlong: src/solver.cpp:19-21.
+TF_HD float speedAlong(const Motion& a, const Motion& b, V3 d, const Axis& x) {
+    return dot(d, b.v - a.v) + dot(x.armB, b.w) - dot(x.armA, a.w);
+}
+
+// push / pushMoving / the masked lane push: src/solver.cpp:23-28, 212-221; src/lanes.cpp:161-166.
+TF_HD void push(Motion& a, Motion& b, const Axis& x, float impulse, bool moveA, bool moveB) {
+    if (moveA) {
+        a.v = a.v - x.linearA * impulse;
+        a.w = a.w - x.angularA * impulse;
+    }
+    if (moveB) {
+        b.v = b.v + x.linearB * impulse;
+        b.w = b.w + x.angularB * impulse;
+    }
+}
+
+// turned: src/solver.cpp:57-59 (src/lanes.cpp:168).
+TF_HD V3 turned(const Motion& m, V3 r) { return m.dq.x == 0.0f && m.dq.y == 0.0f && m.dq.z == 0.0f ? r : rotate(m.dq, r); }
+
+// ---- a coloured ("laned") contact row -----------------------------------------------------------
+
+struct RowPoint {
+    V3 ra, rb;
+    float adj;       // adjusted separation
+    float rv;        // relative velocity at prepare
+    float ni;        // normal impulse
+    V3 ti;           // tangent impulse
+    float nimax;     // largest normal impulse this step
+};
+
+struct Row {
+    uint32_t flags;
+    V3 n, t1, t2;
+    float friction, restitution;
+};
+
+// warmStartLanes, one lane: src/lanes.cpp:195-217.
+TF_HD void warmRow(const Row& r, const RowPoint* p, const Mass& ma, const Mass& mb, Motion& a, Motion& b) {
+    const bool moveA = r.flags & kMoveA, moveB = r.flags & kMoveB;
+    const int count = (int)(r.flags & kCountMask);
+    for (int i = 0; i < count; ++i) {
+        push(a, b, makeAxis(ma, mb, p[i].ra, p[i].rb, r.n), p[i].ni, moveA, moveB);
+        push(a, b, makeAxis(ma, mb, p[i].ra, p[i].rb, r.t1), dot(p[i].ti, r.t1), moveA, moveB);
+        push(a, b, makeAxis(ma, mb, p[i].ra, p[i].rb, r.t2), dot(p[i].ti, r.t2), moveA, moveB);
+    }
+}
+
+// The 3D contact solve: solveLanes for one lane (src/lanes.cpp:220-315), which is also the 3D part
+// of solveContacts (src/solver.cpp:370-417).
+TF_HD void solveRow(const Row& r, RowPoint* p, const Mass& ma, const Mass& mb, Motion& a, Motion& b, const Ctx& c,
+                    bool useBias) {
+    const bool moveA = r.flags & kMoveA, moveB = r.flags & kMoveB;
+    const int count = (int)(r.flags & kCountMask);
+    const Soft soft = (moveA && moveB) ? c.contactSoft : c.staticSoft;
+    const V3 dp = b.dp - a.dp;
+    for (int i = 0; i < count; ++i) {
+        const V3 d = dp + turned(b, p[i].rb) - turned(a, p[i].ra);
+        const float sep = dot(d, r.n) + p[i].adj;
+        float bias = 0.0f, massScale = 1.0f, impulseScale = 0.0f;
+        if (sep > 0.0f) {
+            bias = sep * c.invH;
+        } else if (useBias) {
+            bias = maxf(soft.biasRate * sep, -c.maxPushout);
+            massScale = soft.massScale;
+            impulseScale = soft.impulseScale;
+        }
+        const Axis x = makeAxis(ma, mb, p[i].ra, p[i].rb, r.n);
+        const float vn = speedAlong(a, b, r.n, x);
+        float impulse = -x.mass * massScale * (vn + bias) - impulseScale * p[i].ni;
+        const float total = maxf(p[i].ni + impulse, 0.0f);
+        impulse = total - p[i].ni;
+        p[i].ni = total;
+        p[i].nimax = maxf(p[i].nimax, impulse);
+        push(a, b, x, impulse, moveA, moveB);
+    }
+    for (int j = 0; j < count; ++j) {
+        const int i = useBias ? j : count - 1 - j;
+        const Axis x1 = makeAxis(ma, mb, p[i].ra, p[i].rb, r.t1);
+        const Axis x2 = makeAxis(ma, mb, p[i].ra, p[i].rb, r.t2);
+        const float old1 = dot(p[i].ti, r.t1), old2 = dot(p[i].ti, r.t2);
+        float f1 = old1 - x1.mass * speedAlong(a, b, r.t1, x1);
+        float f2 = old2 - x2.mass * speedAlong(a, b, r.t2, x2);
+        const float limit = r.friction * p[i].ni;
+        const float mag = fsqrt(f1 * f1 + f2 * f2);
+        if (mag > limit) {
+            const float scale = mag > 0.0f ? limit / mag : 0.0f;
+            f1 *= scale;
+            f2 *= scale;
+        }
+        p[i].ti = r.t1 * f1 + r.t2 * f2;
+        push(a, b, x1, f1 - old1, moveA, moveB);
+        push(a, b, x2, f2 - old2, moveA, moveB);
+    }
+}
+
+// restituteLanes, one lane: src/lanes.cpp:318-352.
+TF_HD void restituteRow(const Row& r, RowPoint* p, const Mass& ma, const Mass& mb, Motion& a, Motion& b, const Ctx& c) {
+    const bool moveA = r.flags & kMoveA, moveB = r.flags & kMoveB;
+    const int count = (int)(r.flags & kCountMask);
+    const float e = r.restitution;
+    for (int i = 0; i < count; ++i) {
+        const bool on = !(e == 0.0f) && !(p[i].rv > -c.restitutionThreshold) && !(p[i].nimax == 0.0f);
+        const Axis x = makeAxis(ma, mb, p[i].ra, p[i].rb, r.n);
+        const float vn = speedAlong(a, b, r.n, x);
+        float impulse = -x.mass * (vn + e * p[i].rv);
+        const float total = maxf(p[i].ni + impulse, 0.0f);
+        impulse = total - p[i].ni;
+        if (on) {
+            p[i].ni = total;
+            p[i].nimax = maxf(p[i].nimax, impulse);
+        }
+        push(a, b, x, impulse, on && moveA, on && moveB);
+    }
+}
+
+// ---- contacts outside the colours (planar, rolling resistance, no colour left): one at a time -----
+
+// A body as prepareContact and the serial stages read it.
+struct BodyView {
+    uint8_t type, awake, planar;
+    V3 planeNormal;
+};
+
+TF_HD BodyView viewOf(const tfs_body& b) {
+    return BodyView{ b.type, b.awake, (uint8_t)(isPlanarLocks(b.locks) ? 1 : 0),
+                     rotate(ld4(b.lock_frame), vec(0, 0, 1)) };
+}
+TF_HD bool anchoredView(const BodyView& b) { return !(b.type == TFS_DYNAMIC && b.awake); }
+
+// planarContact: src/solver.cpp:66-79; n is written as the reference writes it.
+TF_HD bool planarContact(const BodyView& a, const BodyView& b, V3 normal, V3& n) {
+    const BodyView* first = nullptr;
+    const BodyView* xs[2] = { &a, &b };
+    for (int k = 0; k < 2; ++k) {
+        const BodyView* x = xs[k];
+        if (x->type == TFS_STATIC) { continue; }
+        if (x->type == TFS_KINEMATIC || !x->planar) { return false; }
+        if (!first) {
+            first = x;
+            n = x->planeNormal;
+        } else if (fabs_(dot(n, x->planeNormal)) < 0.9999f) {
+            return false;
+        }
+    }
+    return first && fabs_(dot(normal, n)) < 1e-3f;
+}
+
+// planarMass: src/solver.cpp:81-84.
+TF_HD float planarMass(const Mass& a, const Mass& b, float armA, float armB) {
+    const float k = a.pim + b.pim + a.pii * armA * armA + b.pii * armB * armB;
+    return k > 1e-6f * (a.invMass + b.invMass) ? 1.0f / k : 0.0f;
+}
+
+struct PlanarAxes {
+    float normalA, normalB, normalMass, tangentA, tangentB, tangentMass;
+};
+
+// makePlanarAxes, point i: src/solver.cpp:86-98.
+TF_HD PlanarAxes planarAxes(const Mass& a, const Mass& b, const tfs_contact& c, int i) {
+    const V3 n = ld3(c.plane);
+    const tfs_point& p = c.points[i];
+    PlanarAxes x;
+    x.normalA = dot(n, cross(ld3(p.anchor_a), ld3(c.normal)));
+    x.normalB = dot(n, cross(ld3(p.anchor_b), ld3(c.normal)));
+    x.normalMass = planarMass(a, b, x.normalA, x.normalB);
+    x.tangentA = dot(n, cross(ld3(p.anchor_a), ld3(c.tangent1)));
+    x.tangentB = dot(n, cross(ld3(p.anchor_b), ld3(c.tangent1)));
+    x.tangentMass = planarMass(a, b, x.tangentA, x.tangentB);
+    return x;
+}
+
+// PlanarState: src/solver.cpp:101-129.
+struct PlanarState {
+    V3 va, vb;
+    float wa, wb, ma, mb, ia, ib;
+
+    TF_HD PlanarState(const Motion& a, const Motion& b, const Mass& x, const Mass& y, V3 n)
+        : va(a.v), vb(b.v), wa(dot(a.w, n)), wb(dot(b.w, n)), ma(x.pim), mb(y.pim), ia(x.pii), ib(y.pii) {}
+    TF_HD float speed(V3 d, float armA, float armB) const { return dot(d, vb - va) + armB * wb - armA * wa; }
+    TF_HD void push(V3 d, float armA, float armB, float impulse) {
+        va = va - d * (ma * impulse);
+        wa -= ia * armA * impulse;
+        vb = vb + d * (mb * impulse);
+        wb += ib * armB * impulse;
+    }
+    TF_HD void store(Motion& a, Motion& b, V3 n, bool moveA, bool moveB) const {
+        if (moveA) {
+            a.v = va;
+            a.w = n * wa;
+        }
+        if (moveB) {
+            b.v = vb;
+            b.w = n * wb;
+        }
+    }
+};
+
+// prepareContact: src/solver.cpp:264-291.
+TF_HD void prepareContact(const BodyView& va, const BodyView& vb, const tfs_body& ta, const tfs_body& tb,
+                          const Motion& a, const Motion& b, const Mass& ma, const Mass& mb, tfs_contact& c) {
+    V3 plane = ld3(c.plane);
+    c.planar = planarContact(va, vb, ld3(c.normal), plane) ? 1 : 0;
+    st3(c.plane, plane);
+    if (c.planar) {
+        const V3 n = normalize(ld3(c.normal) - plane * dot(ld3(c.normal), plane));
+        st3(c.normal, n);
+        st3(c.tangent1, cross(plane, n));
+        st3(c.tangent2, plane);
+    }
+    const PlanarState st(a, b, ma, mb, plane);
+    const V3 normal = ld3(c.normal);
+    for (int i = 0; i < c.count; ++i) {
+        tfs_point& p = c.points[i];
+        const V3 ra = ld3(p.anchor_a), rb = ld3(p.anchor_b);
+        p.adjusted_separation = p.separation - dot(rb - ra, normal);
+        if (c.planar) {
+            const PlanarAxes x = planarAxes(ma, mb, c, i);
+            p.relative_velocity = st.speed(normal, x.normalA, x.normalB);
+        } else {
+            p.relative_velocity = dot(normal, b.v - a.v) + dot(cross(rb, normal), b.w) - dot(cross(ra, normal), a.w);
+        }
+        p.max_normal_impulse = 0.0f;
+    }
+    if (c.rolling_resistance > 0.0f && c.count > 0) {
+        float ha = 0.0f, hb = 0.0f;
+        for (int i = 0; i < c.count; ++i) {
+            ha += fabs_(dot(ld3(c.points[i].anchor_a), normal));
+            hb += fabs_(dot(ld3(c.points[i].anchor_b), normal));
+        }
+        if (ta.type != TFS_DYNAMIC) { ha = 0.0f; }
+        if (tb.type != TFS_DYNAMIC) { hb = 0.0f; }
+        c.rolling_radius = maxf(ha, hb) / (float)c.count;
+    }
+}
+
+// rebaseContact: src/solver.cpp:293-306.
+TF_HD void rebaseContact(const Motion& a, const Motion& b, tfs_contact& c) {
+    const float shift = dot(b.dp - a.dp, ld3(c.normal));
+    for (int i = 0; i < c.count; ++i) {
+        tfs_point& p = c.points[i];
+        st3(p.anchor_a, rotate(a.dq, ld3(p.anchor_a)));
+        st3(p.anchor_b, rotate(b.dq, ld3(p.anchor_b)));
+        p.adjusted_separation += shift;
+    }
+}
+
+// planarTurn: src/solver.cpp:132-135.
+TF_HD void planarTurn(const Motion& m, V3 n, float& c, float& s) {
+    const Q4 q = m.dq;
+    c = q.w * q.w - (q.x * q.x + q.y * q.y + q.z * q.z);
+    s = 2.0f * q.w * dot(n, vec(q.x, q.y, q.z));
+}
+
+// warmStart, one contact that is not laned: src/solver.cpp:319-358.
+TF_HD void warmRest(tfs_contact& c, const BodyView& va, const BodyView& vb, const Mass& ma, const Mass& mb, Motion& a,
+                    Motion& b) {
+    const V3 n = ld3(c.normal), t1 = ld3(c.tangent1), t2 = ld3(c.tangent2);
+    if (c.planar) {
+        bool held = c.rolling_resistance > 0.0f;
+        for (int i = 0; i < c.count; ++i) {
+            const V3 ti = ld3(c.points[i].tangent_impulse);
+            held = held || c.points[i].normal_impulse != 0.0f || dot(ti, ti) != 0.0f;
+        }
+        if (!held) { return; }
+        const V3 plane = ld3(c.plane);
+        PlanarState st(a, b, ma, mb, plane);
+        for (int i = 0; i < c.count; ++i) {
+            const tfs_point& p = c.points[i];
+            const PlanarAxes x = planarAxes(ma, mb, c, i);
+            st.push(n, x.normalA, x.normalB, p.normal_impulse);
+            st.push(t1, x.tangentA, x.tangentB, dot(ld3(p.tangent_impulse), t1));
+        }
+        if (c.rolling_resistance > 0.0f) {
+            const float r = dot(ld3(c.rolling_impulse), plane);
+            st.wa -= st.ia * r;
+            st.wb += st.ib * r;
+        }
+        st.store(a, b, plane, !anchoredView(va), !anchoredView(vb));
+        return;
+    }
+    for (int i = 0; i < c.count; ++i) {
+        const tfs_point& p = c.points[i];
+        const V3 ra = ld3(p.anchor_a), rb = ld3(p.anchor_b);
+        push(a, b, makeAxis(ma, mb, ra, rb, n), p.normal_impulse, true, true);
+        push(a, b, makeAxis(ma, mb, ra, rb, t1), dot(ld3(p.tangent_impulse), t1), true, true);
+        push(a, b, makeAxis(ma, mb, ra, rb, t2), dot(ld3(p.tangent_impulse), t2), true, true);
+    }
+    if (c.rolling_resistance > 0.0f) {
+        a.w = a.w - ma.iiw * ld3(c.rolling_impulse);
+        b.w = b.w + mb.iiw * ld3(c.rolling_impulse);
+    }
+}
+
+// solvePlanar: src/solver.cpp:137-209.
+TF_HD void solvePlanar(tfs_contact& c, const BodyView& va, const BodyView& vb, const Mass& ma, const Mass& mb,
+                       Motion& a, Motion& b, const Ctx& ctx, bool useBias) {
+    const bool moveA = !anchoredView(va), moveB = !anchoredView(vb);
+    const Soft soft = (!moveA || !moveB) ? ctx.staticSoft : ctx.contactSoft;
+    const V3 normal = ld3(c.normal), tangent1 = ld3(c.tangent1), plane = ld3(c.plane);
+    const float shift = dot(b.dp - a.dp, normal);
+    float tax, tay, tbx, tby;
+    planarTurn(a, plane, tax, tay);
+    planarTurn(b, plane, tbx, tby);
+    PlanarState st(a, b, ma, mb, plane);
+    bool changed = false;
+    for (int i = 0; i < c.count; ++i) {
+        tfs_point& p = c.points[i];
+        const PlanarAxes x = planarAxes(ma, mb, c, i);
+        const float reachA = tax * x.tangentA + tay * x.normalA;
+        const float reachB = tbx * x.tangentB + tby * x.normalB;
+        const float sep = shift + reachB - reachA + p.adjusted_separation;
+        float bias = 0.0f, massScale = 1.0f, impulseScale = 0.0f;
+        if (sep > 0.0f) {
+            bias = sep * ctx.invH;
+        } else if (useBias) {
+            bias = maxf(soft.biasRate * sep, -ctx.maxPushout);
+            massScale = soft.massScale;
+            impulseScale = soft.impulseScale;
+        }
+        const float vn = st.speed(normal, x.normalA, x.normalB);
+        float impulse = -x.normalMass * massScale * (vn + bias) - impulseScale * p.normal_impulse;
+        const float total = maxf(p.normal_impulse + impulse, 0.0f);
+        impulse = total - p.normal_impulse;
+        p.normal_impulse = total;
+        p.max_normal_impulse = maxf(p.max_normal_impulse, impulse);
+        if (impulse != 0.0f) {
+            st.push(normal, x.normalA, x.normalB, impulse);
+            changed = true;
+        }
+    }
+    for (int j = 0; j < c.count; ++j) {
+        const int i = useBias ? j : c.count - 1 - j;
+        tfs_point& p = c.points[i];
+        const PlanarAxes x = planarAxes(ma, mb, c, i);
+        const float old = dot(ld3(p.tangent_impulse), tangent1);
+        if (p.normal_impulse == 0.0f && old == 0.0f) { continue; }
+        changed = true;
+        const float limit = c.friction * p.normal_impulse;
+        const float t = minf(maxf(old - x.tangentMass * st.speed(tangent1, x.tangentA, x.tangentB), -limit), limit);
+        st3(p.tangent_impulse, tangent1 * t);
+        st.push(tangent1, x.tangentA, x.tangentB, t - old);
+    }
+    if (c.rolling_resistance > 0.0f) {
+        changed = true;
+        float total = dot(ld3(c.rolling_impulse), plane);
+        const float spin = st.wb - st.wa;
+        const float k = st.ia + st.ib;
+        if (spin != 0.0f && k > 0.0f) {
+            const float lambda = -spin / k;
+            total += lambda;
+            st.wa -= st.ia * lambda;
+            st.wb += st.ib * lambda;
+        }
+        float normalSum = 0.0f;
+        for (int i = 0; i < c.count; ++i) { normalSum += c.points[i].normal_impulse; }
+        const float limit = c.rolling_resistance * c.rolling_radius * normalSum;
+        const float held = minf(maxf(total, -limit), limit);
+        st.wa -= st.ia * (held - total);
+        st.wb += st.ib * (held - total);
+        st3(c.rolling_impulse, plane * held);
+    }
+    if (changed) { st.store(a, b, plane, moveA, moveB); }
+}
+
+// resistTurning: src/solver.cpp:225-251.
+TF_HD void resistTurning(tfs_contact& c, const Mass& ma, const Mass& mb, Motion& a, Motion& b) {
+    const V3 spin = b.w - a.w;
+    const float speed = length(spin);
+    V3 total = ld3(c.rolling_impulse);
+    if (speed > 0.0f) {
+        const V3 d = spin * (1.0f / speed);
+        const V3 ia = ma.iiw * d, ib = mb.iiw * d;
+        const float k = dot(d, ia) + dot(d, ib);
+        if (k > 0.0f) {
+            const float lambda = -speed / k;
+            total = total + d * lambda;
+            a.w = a.w - ia * lambda;
+            b.w = b.w + ib * lambda;
+        }
+    }
+    float normalSum = 0.0f;
+    for (int i = 0; i < c.count; ++i) { normalSum += c.points[i].normal_impulse; }
+    const float limit = c.rolling_resistance * c.rolling_radius * normalSum;
+    const float mag = length(total);
+    if (mag > limit) {
+        const V3 fix = total * (limit / mag) - total;
+        a.w = a.w - ma.iiw * fix;
+        b.w = b.w + mb.iiw * fix;
+        total = total + fix;
+    }
+    st3(c.rolling_impulse, total);
+}
+
+// Loads the points of a contact into row form and back.
+TF_HD void pointsOf(const tfs_contact& c, RowPoint* p) {
+    for (int i = 0; i < c.count; ++i) {
+        const tfs_point& q = c.points[i];
+        p[i].ra = ld3(q.anchor_a);
+        p[i].rb = ld3(q.anchor_b);
+        p[i].adj = q.adjusted_separation;
+        p[i].rv = q.relative_velocity;
+        p[i].ni = q.normal_impulse;
+        p[i].ti = ld3(q.tangent_impulse);
+        p[i].nimax = q.max_normal_impulse;
+    }
+}
+TF_HD void storeImpulses(tfs_contact& c, const RowPoint* p) {
+    for (int i = 0; i < c.count; ++i) {
+        c.points[i].normal_impulse = p[i].ni;
+        st3(c.points[i].tangent_impulse, p[i].ti);
+        c.points[i].max_normal_impulse = p[i].nimax;
+    }
+}
+TF_HD Row rowOf(const tfs_contact& c, bool moveA, bool moveB) {
+    return Row{ (uint32_t)c.count | (moveA ? kMoveA : 0u) | (moveB ? kMoveB : 0u), ld3(c.normal), ld3(c.tangent1),
+                ld3(c.tangent2), c.friction, c.restitution };
+}
+
+// solveContacts, one contact of the rest: src/solver.cpp:360-421.
+TF_HD void solveRest(tfs_contact& c, const BodyView& va, const BodyView& vb, const Mass& ma, const Mass& mb, Motion& a,
+                     Motion& b, const Ctx& ctx, bool useBias) {
+    if (c.planar) {
+        solvePlanar(c, va, vb, ma, mb, a, b, ctx, useBias);
+        return;
+    }
+    RowPoint p[4];
+    pointsOf(c, p);
+    solveRow(rowOf(c, !anchoredView(va), !anchoredView(vb)), p, ma, mb, a, b, ctx, useBias);
+    storeImpulses(c, p);
+    if (c.rolling_resistance > 0.0f) { resistTurning(c, ma, mb, a, b); }
+}
+
+// applyRestitution, one contact that is not laned: src/solver.cpp:423-460.
+TF_HD void restituteRest(tfs_contact& c, const BodyView& va, const BodyView& vb, const Mass& ma, const Mass& mb,
+                         Motion& a, Motion& b, const Ctx& ctx) {
+    if (c.restitution == 0.0f || c.laned) { return; }
+    const V3 n = ld3(c.normal);
+    if (c.planar) {
+        PlanarState st(a, b, ma, mb, ld3(c.plane));
+        for (int i = 0; i < c.count; ++i) {
+            tfs_point& p = c.points[i];
+            if (p.relative_velocity > -ctx.restitutionThreshold || p.max_normal_impulse == 0.0f) { continue; }
+            const PlanarAxes x = planarAxes(ma, mb, c, i);
+            const float vn = st.speed(n, x.normalA, x.normalB);
+            float impulse = -x.normalMass * (vn + c.restitution * p.relative_velocity);
+            const float total = maxf(p.normal_impulse + impulse, 0.0f);
+            impulse = total - p.normal_impulse;
+            p.normal_impulse = total;
+            p.max_normal_impulse = maxf(p.max_normal_impulse, impulse);
+            st.push(n, x.normalA, x.normalB, impulse);
+        }
+        st.store(a, b, ld3(c.plane), !anchoredView(va), !anchoredView(vb));
+        return;
+    }
+    for (int i = 0; i < c.count; ++i) {
+        tfs_point& p = c.points[i];
+        if (p.relative_velocity > -ctx.restitutionThreshold || p.max_normal_impulse == 0.0f) { continue; }
+        const Axis x = makeAxis(ma, mb, ld3(p.anchor_a), ld3(p.anchor_b), n);
+        const float vn = speedAlong(a, b, n, x);
+        float impulse = -x.mass * (vn + c.restitution * p.relative_velocity);
+        const float total = maxf(p.normal_impulse + impulse, 0.0f);
+        impulse = total - p.normal_impulse;
+        p.normal_impulse = total;
+        p.max_normal_impulse = maxf(p.max_normal_impulse, impulse);
+        push(a, b, x, impulse, true, true);
+    }
+}
+
+// nearestPoint: src/world.cpp:122-133.
+TF_HD int nearestPoint(const tfs_contact& old, V3 localA) {
+    int best = -1;
+    float bestSq = 3.40282347e+38f;
+    for (int k = 0; k < old.count; ++k) {
+        const V3 d = ld3(old.points[k].local_a) - localA;
+        const float dsq = dot(d, d);
+        if (dsq < bestSq) {
+            best = k;
+            bestSq = dsq;
+        }
+    }
+    return best;
+}
+
+// carryImpulses: src/world.cpp:102-120.
+TF_HD void carryImpulses(const tfs_contact& old, tfs_contact& c, float matchSq) {
+    int from[4] = { -1, -1, -1, -1 }, claims[4] = { 0, 0, 0, 0 };
+    for (int i = 0; i < c.count; ++i) {
+        from[i] = -1;
+        float bestSq = matchSq;
+        for (int k = 0; k < old.count; ++k) {
+            const V3 d = ld3(old.points[k].local_a) - ld3(c.points[i].local_a);
+            const float dsq = dot(d, d);
+            if (dsq <= bestSq) {
+                from[i] = k;
+                bestSq = dsq;
+            }
+        }
+        if (from[i] >= 0) { ++claims[from[i]]; }
+    }
+    const V3 n = ld3(c.normal);
+    for (int i = 0; i < c.count; ++i) {
+        if (from[i] < 0) { continue; }
+        const tfs_point& o = old.points[from[i]];
+        const float share = 1.0f / (float)claims[from[i]];
+        c.points[i].normal_impulse = o.normal_impulse * share;
+        const V3 t = ld3(o.tangent_impulse);
+        st3(c.points[i].tangent_impulse, (t - n * dot(n, t)) * share);
+    }
+    c.rolling_impulse[0] = old.rolling_impulse[0];
+    c.rolling_impulse[1] = old.rolling_impulse[1];
+    c.rolling_impulse[2] = old.rolling_impulse[2];
+}
+
+}  // namespace tf

@@ -1,0 +1,175 @@
+"""Host-side binding of the MI355X contact solver (libtwofold_hip.so, include/twofold_solver.h).
+
+`Solver` mirrors the part of twofold::World that the solve path owns (src/world.cpp:1564-1727): it
+takes the bodies and the contact manifolds the narrowphase produced, runs one step of the solve
+block and the sleep pass on the GPU, and hands back bodies, contacts and the integer results
+(lists, colours, row order). Errors raise SolverError with the library's message; there is no CPU
+fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _abi
+from ._abi import BODY_DTYPE, CONTACT_DTYPE, SETTINGS_DTYPE
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtwofold_hip.so")
+
+_vp = ctypes.c_void_p
+REFRESH_FN = ctypes.CFUNCTYPE(ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, ctypes.POINTER(ctypes.c_uint32),
+                              ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp)
+
+EXPORTS = ("tfs_abi_version", "tfs_sizeof_body", "tfs_sizeof_contact", "tfs_sizeof_settings", "tfs_default_settings",
+           "tfs_create", "tfs_destroy", "tfs_last_error", "tfs_set_settings", "tfs_upload_bodies",
+           "tfs_upload_contacts", "tfs_step", "tfs_download_bodies", "tfs_download_contacts", "tfs_download_list",
+           "tfs_last_step_ms", "tfs_last_step_counts")
+
+
+class SolverError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise SolverError(f"{path} is missing: build it with __graft_entry__.build() (hipcc, gfx950)")
+    lib = ctypes.CDLL(path)
+    lib.tfs_abi_version.restype = ctypes.c_int
+    for name in ("tfs_sizeof_body", "tfs_sizeof_contact", "tfs_sizeof_settings"):
+        getattr(lib, name).restype = ctypes.c_size_t
+    lib.tfs_default_settings.argtypes = [_vp]
+    lib.tfs_create.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_vp)]
+    lib.tfs_destroy.argtypes = [_vp]
+    lib.tfs_last_error.restype = ctypes.c_char_p
+    lib.tfs_last_error.argtypes = [_vp]
+    lib.tfs_set_settings.argtypes = [_vp, _vp]
+    lib.tfs_upload_bodies.argtypes = [_vp, _vp, ctypes.c_uint32]
+    lib.tfs_upload_contacts.argtypes = [_vp, _vp, ctypes.c_uint32]
+    lib.tfs_step.argtypes = [_vp, ctypes.c_float, REFRESH_FN, _vp]
+    lib.tfs_download_bodies.argtypes = [_vp, _vp, ctypes.c_uint32]
+    lib.tfs_download_contacts.argtypes = [_vp, _vp, ctypes.c_uint32]
+    lib.tfs_download_list.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+    lib.tfs_last_step_ms.argtypes = [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+    lib.tfs_last_step_counts.argtypes = [_vp, _vp]
+    if lib.tfs_abi_version() != 1:
+        raise SolverError("ABI version mismatch")
+    _abi.check_sizes(lib, "tfs")
+    if lib.tfs_sizeof_settings() != SETTINGS_DTYPE.itemsize:
+        raise SolverError("tfs_settings layout mismatch")
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_vp)
+
+
+class Solver:
+    """One world's solver state, resident on a GPU."""
+
+    def __init__(self, max_bodies=1, max_contacts=1, device=0, settings=None):
+        self.lib = load_library()
+        h = _vp()
+        rc = self.lib.tfs_create(device, max_bodies, max_contacts, ctypes.byref(h))
+        if rc != 0:
+            raise SolverError(f"tfs_create failed ({rc}); is a GPU visible?")
+        self.h = h
+        self.nb = 0
+        self.nc = 0
+        self._cb = None
+        if settings is not None:
+            self.set_settings(settings)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.tfs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.tfs_last_error(self.h).decode(errors="replace")
+            raise SolverError(f"{what} failed ({rc}): {msg}")
+
+    def set_settings(self, settings):
+        s = np.ascontiguousarray(np.asarray(settings, SETTINGS_DTYPE).reshape(()))
+        self._check(self.lib.tfs_set_settings(self.h, _ptr(s)), "tfs_set_settings")
+
+    def upload(self, bodies, contacts):
+        b = np.ascontiguousarray(bodies, BODY_DTYPE)
+        c = np.ascontiguousarray(contacts, CONTACT_DTYPE)
+        self._check(self.lib.tfs_upload_bodies(self.h, _ptr(b), len(b)), "tfs_upload_bodies")
+        self._check(self.lib.tfs_upload_contacts(self.h, _ptr(c), len(c)), "tfs_upload_contacts")
+        self.nb, self.nc = len(b), len(c)
+
+    def step(self, dt, refresh=None):
+        """One step. `refresh(substep, remaining, again, bodies, contacts) -> fresh contacts` is called
+        when fast contacts need new manifolds mid-step."""
+        if refresh is None:
+            cb = REFRESH_FN()
+        else:
+            def cb_impl(user, substep, remaining, again, n_again, bodies, n_bodies, contacts, n_contacts, fresh):
+                try:
+                    ag = np.ctypeslib.as_array(again, shape=(n_again,)).copy()
+                    bd = np.frombuffer((ctypes.c_char * (n_bodies * BODY_DTYPE.itemsize)).from_address(bodies),
+                                       BODY_DTYPE).copy()
+                    ct = np.frombuffer((ctypes.c_char * (n_contacts * CONTACT_DTYPE.itemsize)).from_address(contacts),
+                                       CONTACT_DTYPE).copy()
+                    out = np.ascontiguousarray(refresh(substep, remaining, ag, bd, ct), CONTACT_DTYPE)
+                    if len(out) != n_again:
+                        return -1
+                    ctypes.memmove(fresh, out.ctypes.data, out.nbytes)
+                    return 0
+                except Exception:  # an exception must not cross the C boundary
+                    return -1
+            cb = REFRESH_FN(cb_impl)
+        self._cb = cb
+        self._check(self.lib.tfs_step(self.h, ctypes.c_float(dt), cb, None), "tfs_step")
+
+    def bodies(self):
+        out = np.zeros(self.nb, BODY_DTYPE)
+        self._check(self.lib.tfs_download_bodies(self.h, _ptr(out), self.nb), "tfs_download_bodies")
+        return out
+
+    def contacts(self):
+        out = np.zeros(self.nc, CONTACT_DTYPE)
+        self._check(self.lib.tfs_download_contacts(self.h, _ptr(out), self.nc), "tfs_download_contacts")
+        return out
+
+    def list(self, kind):
+        n = ctypes.c_uint32()
+        self._check(self.lib.tfs_download_list(self.h, kind, None, 0, ctypes.byref(n)), "tfs_download_list")
+        out = np.zeros(n.value, np.uint8 if kind == _abi.LIST_COLORS else np.uint32)
+        self._check(self.lib.tfs_download_list(self.h, kind, _ptr(out), n.value, ctypes.byref(n)), "tfs_download_list")
+        return out
+
+    def lists(self):
+        """The integer results of the last step, named as the oracle names them."""
+        k = _abi
+        rows, imp_rows = self.list(k.LIST_ROWS), self.list(k.LIST_IMPACT_ROWS)
+        colors = self.list(k.LIST_COLORS)
+        rest = rows[colors[rows] >= k.COLORS] if len(rows) else rows
+        imp_rest = imp_rows[colors[imp_rows] >= k.COLORS] if len(imp_rows) else imp_rows
+        return dict(active=self.list(k.LIST_ACTIVE), fast=self.list(k.LIST_FAST), fast_solid=self.list(k.LIST_FAST_SOLID),
+                    impact=self.list(k.LIST_IMPACT), rows=rows, rest=rest, impact_rows=imp_rows, impact_rest=imp_rest,
+                    colors=colors)
+
+    def last_ms(self):
+        a, b = ctypes.c_float(), ctypes.c_float()
+        self._check(self.lib.tfs_last_step_ms(self.h, ctypes.byref(a), ctypes.byref(b)), "tfs_last_step_ms")
+        return a.value, b.value
+
+    def counts(self):
+        """active contacts, their points, impact contacts, their points, colours used, rest contacts."""
+        out = np.zeros(6, np.uint32)
+        self._check(self.lib.tfs_last_step_counts(self.h, _ptr(out)), "tfs_last_step_counts")
+        return dict(zip(("active", "points", "impact", "impact_points", "colors", "rest"), out.tolist()))
