@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Contact constraint-iterations/s of the soft-step solve path on one MI355X.
+
+Workload: the 32,768-body drop-and-settle pile (half boxes, half spheres, inside four static walls
+on a half-space; twofold_amd.scenes.pile). It is stepped `--settle` steps with the host narrowphase
+(twofold_amd/csrc/scene.cpp) and the GPU solver, then the contact set of the next step is kept
+resident and each timed step is one full solve block + sleep pass over it on the GPU
+(src/world.cpp:1591-1691, 1174-1247), fast pairs collided again mid-step by the native host callback
+as the reference's solve block does. A contact constraint-iteration is one contact point's normal +
+friction update in one velocity pass: per step, substeps x 2 passes x (active points + impact points).
+
+Also reported: the roofline of the solve-row kernels (algorithmic bytes per launch / event-timed
+launch duration, against 8 TB/s) and the CPU oracle timed single-threaded on the same input.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from twofold_amd import scenes  # noqa: E402
+from twofold_amd.world import Simulation  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def iterations(counts, substeps):
+    return substeps * 2 * (counts["points"] + counts["impact_points"])
+
+
+def cpu_baseline(sim, bodies, contacts, budget_s):
+    """The oracle (oracle/liboracle.so, a single-threaded CPU restatement of the reference solver, pinned
+    bit-exact to it) timed on the same input, with the same refresh callback run on one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import refworld
+
+    try:
+        ctypes.CDLL("libgomp.so.1").omp_set_num_threads(1)
+    except OSError:
+        pass
+    try:
+        os.sched_setaffinity(0, {min(os.sched_getaffinity(0))})
+    except (AttributeError, OSError):
+        pass
+    lib = refworld._load_oracle()
+    b = np.ascontiguousarray(bodies.copy())
+    c = np.ascontiguousarray(contacts.copy())
+    n = max(len(c), 1)
+    arrays = [np.zeros(n, np.uint32) for _ in range(6)]
+    colors = np.zeros(n, np.uint8)
+    L = refworld._Lists()
+    for name, a in zip(("active", "fast", "fast_solid", "impact", "by_color", "impact_by_color"), arrays):
+        setattr(L, name, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    L.colors = colors.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    fn = ctypes.cast(sim.world.refresh_fn, refworld.REFRESH_FN)
+    steps, its, spent = 0, 0, 0.0
+    substeps = int(sim.settings["substeps"])
+    while steps == 0 or spent < budget_s:
+        t0 = time.perf_counter()
+        rc = lib.tfo_step(b.ctypes.data, len(b), c.ctypes.data, len(c), sim.settings.ctypes.data, np.float32(sim.dt),
+                          fn, ctypes.c_void_p(sim.world.h), ctypes.byref(L))
+        spent += time.perf_counter() - t0
+        if rc != 0:
+            raise RuntimeError(f"oracle step failed ({rc})")
+        its += substeps * 2 * (L.points_active + L.points_impact)
+        steps += 1
+    return dict(value=its / spent, unit="iterations/s", cores=1, kind="port",
+                sample=f"{steps} solve steps of the same settled pile from the same input, oracle/liboracle.so "
+                       f"(-O2, one thread, refresh narrowphase on the same thread), {spent:.1f} s, CPU: {cpu_model()}",
+                ms_per_step=1e3 * spent / steps)
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--settle", type=int, default=240, help="steps of drop-and-settle before the timed window")
+    ap.add_argument("--side", type=int, default=32, help="bodies per side of the grid (32 -> 32,768)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world_size = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world_size != 1:
+        print("bench.py runs on one GPU only: this build measures the solve path on a single MI355X "
+              "(--gpus 1); multi-GPU runs are not part of it.", file=sys.stderr)
+        sys.exit(2)
+
+    scene = scenes.pile(side=args.side)
+    t0 = time.perf_counter()
+    sim = Simulation(scene)
+    log(f"pile of {len(sim.bodies)} bodies built in {time.perf_counter() - t0:.1f} s; settling {args.settle} steps")
+    for k in range(args.settle):
+        sim.step()
+        if (k + 1) % 20 == 0:
+            c = sim.solver.counts()
+            log(f"settle step {k + 1}: {len(sim.contacts)} contacts, {c['points']} points, {c['colors']} colours, "
+                f"{time.perf_counter() - t0:.1f} s")
+    sim.collide()
+    bodies0, contacts0 = sim.solver.bodies(), sim.solver.contacts()
+    substeps = int(sim.settings["substeps"])
+
+    for _ in range(args.warmup):
+        sim.solver.step_native(sim.dt, sim.world.refresh_fn, sim.world.h)
+    try:
+        import torch
+
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except ImportError:
+        sync = lambda: None  # noqa: E731
+    sim.solver.set_profiling(True)
+    sync()
+    its = 0
+    solve_ms = 0.0
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        sim.solver.step_native(sim.dt, sim.world.refresh_fn, sim.world.h)
+        counts = sim.solver.counts()
+        its += iterations(counts, substeps)
+        solve_ms += sim.solver.last_ms()[0]
+    sync()
+    wall = time.perf_counter() - t1
+    launches, ms, algo = sim.solver.profile(1)
+    achieved = algo / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    log(f"timed {args.steps} steps in {wall:.3f} s; solve rows: {launches} launches, {ms:.2f} ms, "
+        f"{algo / max(launches, 1) / 1e6:.2f} MB per launch")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(sim, bodies0, contacts0, args.cpu_seconds)
+        log(f"cpu baseline: {cpu['value']:.3e} iterations/s ({cpu['ms_per_step']:.0f} ms/step)")
+
+    out = {
+        "metric": "contact constraint-iterations/s",
+        "value": its / wall,
+        "unit": "iterations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded pile recipe, settled on the GPU with the host narrowphase",
+        "config": {
+            "workload": f"settled {len(bodies0)}-body box/sphere pile in walls (after {args.settle} drop steps), "
+                        f"dt 1/60, {substeps} substeps, one solve block + sleep pass per step",
+            "bodies": len(bodies0), "contacts": len(contacts0), "active_points": counts["points"],
+            "impact_points": counts["impact_points"], "colors": counts["colors"], "rest": counts["rest"],
+            "world_steps_per_s": args.steps / wall, "gpu_solve_ms_per_step": solve_ms / args.steps,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None, "kernel": "k_rows<bias|relax> (one launch per colour)", "launches": launches,
+            "bytes_per_launch": algo / max(launches, 1), "ms_per_launch": ms / max(launches, 1),
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -155,8 +155,9 @@ typedef struct tfs_settings {
  * then prepare, nearest-point carry and impulse carry of the new points) and asks the host for the
  * new manifolds through this callback.
  *
- * On entry, bodies[0..n_bodies) holds the current body state and contacts[again[k]] the contact
- * before the refresh. For each k the callback writes new[k]: normal, tangent1, tangent2, count and
+ * On entry, contacts[again[k]] holds the contact before the refresh and bodies[] the current state
+ * of the bodies those contacts join (the arrays are indexed as uploaded; other entries are not
+ * current and must not be read). For each k the callback writes fresh[k]: normal, tangent1, tangent2, count and
  * per point local_a, separation, anchor_a, anchor_b (other fields are ignored); count 0 drops the
  * points. `remaining` is the time left in the step, (substeps - i) * h. Returns 0 or an error.
  */
@@ -214,6 +215,17 @@ int tfs_download_list(tfs_ctx* ctx, int kind, void* out, uint32_t cap, uint32_t*
 /* Solve-stage timing of the last step in milliseconds, measured with HIP events on the stream,
  * as the reference's StepProfile::solver brackets it (src/world.cpp:1644-1692). */
 int tfs_last_step_ms(tfs_ctx* ctx, float* solve_ms, float* sleep_ms);
+
+/*
+ * Measurement (no counterpart in the reference): with profiling on, every launch of the coloured-row
+ * kernels is bracketed by HIP events on the context's stream, and per family (0 warm start, 1 solve:
+ * biased and relax passes including the impact repeats, 2 restitution) the library sums the launch
+ * count, the event time in milliseconds and the algorithmic bytes of the rows launched (per contact
+ * with n points: warm start 224 + 244n, solve 224 + 296n, restitution 224 + 96n). Turning it on or
+ * off clears the sums.
+ */
+int tfs_set_profiling(tfs_ctx* ctx, int on);
+int tfs_read_profile(tfs_ctx* ctx, int family, uint64_t* launches, double* ms, double* bytes);
 
 /* Counts of the last step: active contacts, their points, impact contacts and their points,
  * colours used, rest contacts. Constraint-iterations per step follow from these. */

@@ -1,0 +1,690 @@
+// scene.cpp - a host world that drives the solver end to end: dynamic and static boxes and spheres
+// and static half-spaces, a uniform-grid broadphase and a narrowphase that hands the solver contact
+// manifolds in its input form (include/twofold_solver.h).
+//
+// It stands in for the reference's broadphase and narrowphase (src/world.cpp:752-993,
+// src/collide3d.cpp), which sit outside the solve path, so that the solver can be run on the GPU
+// box on the piles it is measured on. Its manifolds follow the reference's conventions - normal
+// from shape A to shape B, shapes ordered by index, points at the midpoint between the two
+// surfaces, a speculative margin grown by the pair's travel (src/world.cpp:484-486, 509-511),
+// anchors lifted onto each surface for separated points (src/world.cpp:536-540), impulses carried
+// to the nearest new point (src/world.cpp:102-120), friction and restitution mixed as
+// src/world.cpp:39-40 - but not its bits: box-box uses separating axes and face clipping of its own.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/twofold_solver.h"
+
+namespace {
+
+struct V {
+    float x, y, z;
+};
+V operator+(V a, V b) { return { a.x + b.x, a.y + b.y, a.z + b.z }; }
+V operator-(V a, V b) { return { a.x - b.x, a.y - b.y, a.z - b.z }; }
+V operator*(V a, float s) { return { a.x * s, a.y * s, a.z * s }; }
+V operator-(V a) { return { -a.x, -a.y, -a.z }; }
+float dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+V cross(V a, V b) { return { a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x }; }
+float len(V a) { return std::sqrt(dot(a, a)); }
+V unit(V a) {
+    const float l = len(a);
+    return l > 0 ? a * (1.0f / l) : V{ 0, 1, 0 };
+}
+V ld(const float* p) { return { p[0], p[1], p[2] }; }
+void st(float* p, V v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+
+struct Q {
+    float x, y, z, w;
+};
+V rotate(Q q, V v) {
+    const V u{ q.x, q.y, q.z };
+    const V t = cross(u, v) * 2.0f;
+    return v + t * q.w + cross(u, t);
+}
+V unrotate(Q q, V v) { return rotate(Q{ -q.x, -q.y, -q.z, q.w }, v); }
+Q ldq(const float* p) { return { p[0], p[1], p[2], p[3] }; }
+
+V perpendicular(V n) {
+    if (std::fabs(n.x) > 0.57735f) { return unit(V{ n.y, -n.x, 0.0f }); }
+    return unit(V{ 0.0f, n.z, -n.y });
+}
+
+enum Kind { kSphere = 0, kBox = 1, kHalfSpace = 4 };
+
+struct Shape {
+    int kind = kBox;
+    V half{ 0.5f, 0.5f, 0.5f };
+    float radius = 0.5f;
+    float density = 1.0f, friction = 0.5f, restitution = 0.0f, rolling = 0.0f;
+};
+
+struct Point {
+    V pos;       // midpoint between the surfaces
+    float sep;   // along the normal, negative when overlapping
+};
+
+struct Manifold {
+    V n;
+    int count = 0;
+    Point p[4];
+};
+
+struct Pose {
+    V c;
+    Q q;
+    V ax[3];  // box axes in the world
+};
+
+Pose poseOf(const tfs_body& b) {
+    Pose p;
+    p.c = ld(b.com);
+    p.q = ldq(b.q);
+    p.ax[0] = rotate(p.q, V{ 1, 0, 0 });
+    p.ax[1] = rotate(p.q, V{ 0, 1, 0 });
+    p.ax[2] = rotate(p.q, V{ 0, 0, 1 });
+    return p;
+}
+
+float comp(V v, int i) { return i == 0 ? v.x : i == 1 ? v.y : v.z; }
+
+// Keeps at most four points: the deepest, the farthest from it, and the two farthest on either side
+// of the line between them.
+void reduce(std::vector<Point>& pts, V n, Manifold& m) {
+    if (pts.size() <= 4) {
+        m.count = (int)pts.size();
+        for (size_t i = 0; i < pts.size(); ++i) { m.p[i] = pts[i]; }
+        return;
+    }
+    size_t a = 0;
+    for (size_t i = 1; i < pts.size(); ++i) {
+        if (pts[i].sep < pts[a].sep) { a = i; }
+    }
+    size_t b = a == 0 ? 1 : 0;
+    float best = -1.0f;
+    for (size_t i = 0; i < pts.size(); ++i) {
+        const V d = pts[i].pos - pts[a].pos;
+        if (i != a && dot(d, d) > best) {
+            best = dot(d, d);
+            b = i;
+        }
+    }
+    const V ab = pts[b].pos - pts[a].pos;
+    size_t c = SIZE_MAX, d = SIZE_MAX;
+    float hi = 0.0f, lo = 0.0f;
+    for (size_t i = 0; i < pts.size(); ++i) {
+        if (i == a || i == b) { continue; }
+        const float s = dot(cross(ab, pts[i].pos - pts[a].pos), n);
+        if (s > hi) {
+            hi = s;
+            c = i;
+        }
+        if (s < lo) {
+            lo = s;
+            d = i;
+        }
+    }
+    m.count = 0;
+    for (size_t i : { a, b, c, d }) {
+        if (i != SIZE_MAX) { m.p[m.count++] = pts[i]; }
+    }
+}
+
+void sphereSphere(V ca, float ra, V cb, float rb, float margin, Manifold& m) {
+    const V d = cb - ca;
+    const float dist = len(d);
+    const V n = dist > 1e-9f ? d * (1.0f / dist) : V{ 0, 1, 0 };
+    const float sep = dist - ra - rb;
+    m.count = 0;
+    if (sep > margin) { return; }
+    m.n = n;
+    const V pa = ca + n * ra, pb = cb - n * rb;
+    m.p[0] = { (pa + pb) * 0.5f, sep };
+    m.count = 1;
+}
+
+// Half-space A (below its local xy plane) against a sphere or box B.
+void halfSpaceVs(const Pose& a, const Shape& sb, const Pose& b, float margin, Manifold& m) {
+    const V n = a.ax[2];
+    const float offset = dot(n, a.c);
+    m.count = 0;
+    m.n = n;
+    if (sb.kind == kSphere) {
+        const float sep = dot(n, b.c) - offset - sb.radius;
+        if (sep > margin) { return; }
+        const V pb = b.c - n * sb.radius;
+        m.p[0] = { pb - n * (0.5f * sep), sep };
+        m.count = 1;
+        return;
+    }
+    std::vector<Point> pts;
+    for (int k = 0; k < 8; ++k) {
+        const V v = b.c + b.ax[0] * ((k & 1) ? sb.half.x : -sb.half.x) + b.ax[1] * ((k & 2) ? sb.half.y : -sb.half.y) +
+                    b.ax[2] * ((k & 4) ? sb.half.z : -sb.half.z);
+        const float sep = dot(n, v) - offset;
+        if (sep <= margin) { pts.push_back({ v - n * (0.5f * sep), sep }); }
+    }
+    reduce(pts, n, m);
+}
+
+// Box A against sphere B; the normal points from the box to the sphere.
+void boxSphere(const Shape& sa, const Pose& a, V cs, float r, float margin, Manifold& m) {
+    const V d = cs - a.c;
+    const V c{ dot(d, a.ax[0]), dot(d, a.ax[1]), dot(d, a.ax[2]) };
+    const V h = sa.half;
+    const V q{ std::max(-h.x, std::min(h.x, c.x)), std::max(-h.y, std::min(h.y, c.y)), std::max(-h.z, std::min(h.z, c.z)) };
+    V nl, onBox;
+    float sep;
+    const V off = c - q;
+    const float dist = len(off);
+    if (dist > 1e-7f) {
+        nl = off * (1.0f / dist);
+        sep = dist - r;
+        onBox = q;
+    } else {
+        int axis = 0;
+        float pen = h.x - std::fabs(c.x);
+        for (int i = 1; i < 3; ++i) {
+            const float p = comp(h, i) - std::fabs(comp(c, i));
+            if (p < pen) {
+                pen = p;
+                axis = i;
+            }
+        }
+        const float s = comp(c, axis) >= 0 ? 1.0f : -1.0f;
+        nl = V{ axis == 0 ? s : 0.0f, axis == 1 ? s : 0.0f, axis == 2 ? s : 0.0f };
+        sep = -pen - r;
+        onBox = c;
+        if (axis == 0) { onBox.x = s * h.x; }
+        if (axis == 1) { onBox.y = s * h.y; }
+        if (axis == 2) { onBox.z = s * h.z; }
+    }
+    m.count = 0;
+    if (sep > margin) { return; }
+    const V n = a.ax[0] * nl.x + a.ax[1] * nl.y + a.ax[2] * nl.z;
+    const V pa = a.c + a.ax[0] * onBox.x + a.ax[1] * onBox.y + a.ax[2] * onBox.z;
+    const V pb = cs - n * r;
+    m.n = n;
+    m.p[0] = { (pa + pb) * 0.5f, sep };
+    m.count = 1;
+}
+
+// Clips a convex polygon against the plane dot(n, x) <= d.
+void clip(std::vector<V>& poly, V n, float d) {
+    std::vector<V> out;
+    const size_t k = poly.size();
+    for (size_t i = 0; i < k; ++i) {
+        const V p = poly[i], q = poly[(i + 1) % k];
+        const float dp = dot(n, p) - d, dq = dot(n, q) - d;
+        if (dp <= 0) { out.push_back(p); }
+        if ((dp < 0 && dq > 0) || (dp > 0 && dq < 0)) { out.push_back(p + (q - p) * (dp / (dp - dq))); }
+    }
+    poly.swap(out);
+}
+
+// Box A against box B by separating axes: a face of one box clipped against the other's
+// incident face, or the closest points of two edges.
+void boxBox(const Shape& sa, const Pose& a, const Shape& sb, const Pose& b, float margin, Manifold& m) {
+    m.count = 0;
+    const V t = b.c - a.c;
+    const float ha[3] = { sa.half.x, sa.half.y, sa.half.z }, hb[3] = { sb.half.x, sb.half.y, sb.half.z };
+    auto extentAlong = [](const Pose& p, const float* h, V l) {
+        return h[0] * std::fabs(dot(p.ax[0], l)) + h[1] * std::fabs(dot(p.ax[1], l)) + h[2] * std::fabs(dot(p.ax[2], l));
+    };
+    float bestFace = -1e30f, bestEdge = -1e30f;
+    int faceAxis = -1, edgeI = -1, edgeJ = -1;
+    V faceN{}, edgeN{};
+    for (int i = 0; i < 6; ++i) {
+        const V l = i < 3 ? a.ax[i] : b.ax[i - 3];
+        const float sep = std::fabs(dot(t, l)) - extentAlong(a, ha, l) - extentAlong(b, hb, l);
+        if (sep > margin) { return; }
+        if (sep > bestFace) {
+            bestFace = sep;
+            faceAxis = i;
+            faceN = dot(t, l) >= 0 ? l : -l;
+        }
+    }
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) {
+            V l = cross(a.ax[i], b.ax[j]);
+            const float l2 = dot(l, l);
+            if (l2 < 1e-6f) { continue; }
+            l = l * (1.0f / std::sqrt(l2));
+            const float sep = std::fabs(dot(t, l)) - extentAlong(a, ha, l) - extentAlong(b, hb, l);
+            if (sep > margin) { return; }
+            if (sep > bestEdge) {
+                bestEdge = sep;
+                edgeI = i;
+                edgeJ = j;
+                edgeN = dot(t, l) >= 0 ? l : -l;
+            }
+        }
+    }
+    if (edgeI >= 0 && bestEdge > 0.95f * bestFace + 0.01f) {
+        // Two edges: the edge of A farthest along n and the edge of B farthest along -n.
+        V pa = a.c, pb = b.c;
+        for (int k = 0; k < 3; ++k) {
+            if (k != edgeI) { pa = pa + a.ax[k] * (dot(a.ax[k], edgeN) >= 0 ? ha[k] : -ha[k]); }
+            if (k != edgeJ) { pb = pb + b.ax[k] * (dot(b.ax[k], edgeN) >= 0 ? -hb[k] : hb[k]); }
+        }
+        const V da = a.ax[edgeI], db = b.ax[edgeJ];
+        const V r = pa - pb;
+        const float e = dot(db, db), f = dot(db, r), c = dot(da, r), bb = dot(da, db);
+        const float den = 1.0f - bb * bb;
+        float s = den > 1e-9f ? (bb * f - c * e) / den : 0.0f;
+        s = std::max(-ha[edgeI], std::min(ha[edgeI], s));
+        float u = (bb * s + f) / e;
+        u = std::max(-hb[edgeJ], std::min(hb[edgeJ], u));
+        const V qa = pa + da * s, qb = pb + db * u;
+        const float sep = dot(qb - qa, edgeN);
+        if (sep > margin) { return; }
+        m.n = edgeN;
+        m.p[0] = { (qa + qb) * 0.5f, sep };
+        m.count = 1;
+        return;
+    }
+    const bool refA = faceAxis < 3;
+    const Pose& R = refA ? a : b;
+    const Pose& I = refA ? b : a;
+    const float* hr = refA ? ha : hb;
+    const float* hi = refA ? hb : ha;
+    const int ri = refA ? faceAxis : faceAxis - 3;
+    const V rn = refA ? faceN : -faceN;  // the reference face's outward normal, toward the other box
+    int ii = 0;
+    float most = -1.0f;
+    for (int k = 0; k < 3; ++k) {
+        const float d = std::fabs(dot(I.ax[k], rn));
+        if (d > most) {
+            most = d;
+            ii = k;
+        }
+    }
+    const V in = dot(I.ax[ii], rn) > 0 ? -I.ax[ii] : I.ax[ii];  // incident face normal, facing the reference face
+    const int i1 = (ii + 1) % 3, i2 = (ii + 2) % 3;
+    const V fc = I.c + in * hi[ii];
+    std::vector<V> poly = { fc + I.ax[i1] * hi[i1] + I.ax[i2] * hi[i2], fc - I.ax[i1] * hi[i1] + I.ax[i2] * hi[i2],
+                            fc - I.ax[i1] * hi[i1] - I.ax[i2] * hi[i2], fc + I.ax[i1] * hi[i1] - I.ax[i2] * hi[i2] };
+    for (int k = 0; k < 3 && !poly.empty(); ++k) {
+        if (k == ri) { continue; }
+        clip(poly, R.ax[k], dot(R.ax[k], R.c) + hr[k]);
+        clip(poly, -R.ax[k], -dot(R.ax[k], R.c) + hr[k]);
+    }
+    std::vector<Point> pts;
+    const float plane = dot(rn, R.c) + hr[ri];
+    for (const V& p : poly) {
+        const float sep = dot(rn, p) - plane;
+        if (sep > margin) { continue; }
+        pts.push_back({ p - rn * (0.5f * sep), sep });
+    }
+    m.n = faceN;
+    reduce(pts, faceN, m);
+}
+
+struct World {
+    tfs_settings settings{};
+    std::vector<tfs_body> bodies;
+    std::vector<Shape> shapes;  // one per body, at the body's origin
+    std::vector<uint32_t> planes, statics, movers;
+};
+
+float travelOf(const tfs_body& b, float dt) {
+    if (b.type == TFS_STATIC) { return 0.0f; }
+    return (len(ld(b.v)) + len(ld(b.w)) * b.extent) * dt;
+}
+
+// One manifold for the pair (a < b), or none.
+bool collide(const World& w, const tfs_body* bodies, uint32_t ia, uint32_t ib, float dt, Manifold& m) {
+    const tfs_body& a = bodies[ia];
+    const tfs_body& b = bodies[ib];
+    const float travel =
+        (len(ld(b.v) - ld(a.v)) + len(ld(a.w)) * a.extent + len(ld(b.w)) * b.extent) * dt;
+    const float margin = w.settings.speculative + travel;
+    const Shape& sa = w.shapes[ia];
+    const Shape& sb = w.shapes[ib];
+    const Pose pa = poseOf(a), pb = poseOf(b);
+    m.count = 0;
+    if (sa.kind == kHalfSpace && sb.kind != kHalfSpace) {
+        halfSpaceVs(pa, sb, pb, margin, m);
+    } else if (sb.kind == kHalfSpace) {
+        return false;
+    } else if (sa.kind == kSphere && sb.kind == kSphere) {
+        sphereSphere(pa.c, sa.radius, pb.c, sb.radius, margin, m);
+    } else if (sa.kind == kBox && sb.kind == kSphere) {
+        boxSphere(sa, pa, pb.c, sb.radius, margin, m);
+    } else if (sa.kind == kSphere && sb.kind == kBox) {
+        boxSphere(sb, pb, pa.c, sa.radius, margin, m);
+        m.n = -m.n;
+    } else {
+        boxBox(sa, pa, sb, pb, margin, m);
+    }
+    return m.count > 0;
+}
+
+// The contact of pair (ia, ib) from a manifold (src/world.cpp:920-950 conventions).
+void toContact(const World& w, const tfs_body* bodies, uint32_t ia, uint32_t ib, const Manifold& m, tfs_contact& c) {
+    std::memset(&c, 0, sizeof c);
+    const tfs_body& a = bodies[ia];
+    const tfs_body& b = bodies[ib];
+    c.key = ((uint64_t)ia << 32) | ib;
+    c.body_a = ia;
+    c.body_b = ib;
+    const V n = unit(m.n);
+    const V t1 = perpendicular(n);
+    st(c.normal, n);
+    st(c.tangent1, t1);
+    st(c.tangent2, cross(n, t1));
+    const Shape& sa = w.shapes[ia];
+    const Shape& sb = w.shapes[ib];
+    c.friction = std::sqrt(sa.friction * sb.friction);
+    c.restitution = std::max(sa.restitution, sb.restitution);
+    c.rolling_resistance = std::max(sa.rolling, sb.rolling);
+    c.count = m.count;
+    const Q qa = ldq(a.q);
+    for (int i = 0; i < m.count; ++i) {
+        tfs_point& p = c.points[i];
+        const V pos = m.p[i].pos;
+        st(p.local_a, unrotate(qa, pos - ld(a.p)));
+        p.separation = m.p[i].sep;
+        const V up = p.separation > w.settings.speculative ? n * (0.5f * p.separation) : V{ 0, 0, 0 };
+        st(p.anchor_a, pos - up - ld(a.com));
+        st(p.anchor_b, pos + up - ld(b.com));
+    }
+}
+
+// carryImpulses: src/world.cpp:102-120.
+void carry(const tfs_contact& old, tfs_contact& c, float matchSq) {
+    int from[4], claims[4] = {};
+    for (int i = 0; i < c.count; ++i) {
+        from[i] = -1;
+        float bestSq = matchSq;
+        for (int k = 0; k < old.count; ++k) {
+            const V d = ld(old.points[k].local_a) - ld(c.points[i].local_a);
+            if (dot(d, d) <= bestSq) {
+                from[i] = k;
+                bestSq = dot(d, d);
+            }
+        }
+        if (from[i] >= 0) { ++claims[from[i]]; }
+    }
+    const V n = ld(c.normal);
+    for (int i = 0; i < c.count; ++i) {
+        if (from[i] < 0) { continue; }
+        const tfs_point& o = old.points[from[i]];
+        const float share = 1.0f / (float)claims[from[i]];
+        c.points[i].normal_impulse = o.normal_impulse * share;
+        const V t = ld(o.tangent_impulse);
+        st(c.points[i].tangent_impulse, (t - n * dot(n, t)) * share);
+    }
+    std::memcpy(c.rolling_impulse, old.rolling_impulse, sizeof c.rolling_impulse);
+}
+
+void aabb(const World& w, const tfs_body& b, uint32_t i, float grow, V& lo, V& hi) {
+    const Shape& s = w.shapes[i];
+    const Pose p = poseOf(b);
+    V e;
+    if (s.kind == kSphere) {
+        e = V{ s.radius, s.radius, s.radius };
+    } else {
+        e = V{ std::fabs(p.ax[0].x) * s.half.x + std::fabs(p.ax[1].x) * s.half.y + std::fabs(p.ax[2].x) * s.half.z,
+               std::fabs(p.ax[0].y) * s.half.x + std::fabs(p.ax[1].y) * s.half.y + std::fabs(p.ax[2].y) * s.half.z,
+               std::fabs(p.ax[0].z) * s.half.x + std::fabs(p.ax[1].z) * s.half.y + std::fabs(p.ax[2].z) * s.half.z };
+    }
+    e = e + V{ grow, grow, grow };
+    lo = p.c - e;
+    hi = p.c + e;
+}
+
+bool overlap(V alo, V ahi, V blo, V bhi) {
+    return alo.x <= bhi.x && blo.x <= ahi.x && alo.y <= bhi.y && blo.y <= ahi.y && alo.z <= bhi.z && blo.z <= ahi.z;
+}
+
+bool moving(const tfs_body& b) { return b.type == TFS_DYNAMIC && b.awake; }
+
+// Candidate pairs (a < b) whose grown bounds overlap, sorted, with at least one awake dynamic body.
+std::vector<uint64_t> candidates(const World& w, const tfs_body* bodies, float dt) {
+    const uint32_t nb = (uint32_t)w.bodies.size();
+    std::vector<V> lo(nb), hi(nb);
+    float cell = 0.0f;
+    for (uint32_t i : w.movers) {
+        const float g = 0.5f * w.settings.speculative + travelOf(bodies[i], dt);
+        aabb(w, bodies[i], i, g, lo[i], hi[i]);
+        cell = std::max(cell, std::max(hi[i].x - lo[i].x, std::max(hi[i].y - lo[i].y, hi[i].z - lo[i].z)));
+    }
+    std::vector<uint64_t> out;
+    if (cell <= 0.0f) { return out; }
+    const float inv = 1.0f / cell;
+    std::unordered_map<uint64_t, std::vector<uint32_t>> grid;
+    grid.reserve(w.movers.size() * 2);
+    auto key = [](int x, int y, int z) {
+        return ((uint64_t)(uint32_t)(x + (1 << 20)) << 42) | ((uint64_t)(uint32_t)(y + (1 << 20)) << 21) |
+               (uint64_t)(uint32_t)(z + (1 << 20));
+    };
+    for (uint32_t i : w.movers) {
+        const int x = (int)std::floor(lo[i].x * inv), y = (int)std::floor(lo[i].y * inv), z = (int)std::floor(lo[i].z * inv);
+        grid[key(x, y, z)].push_back(i);
+    }
+    for (uint32_t i : w.movers) {
+        const int x0 = (int)std::floor(lo[i].x * inv), y0 = (int)std::floor(lo[i].y * inv), z0 = (int)std::floor(lo[i].z * inv);
+        for (int dx = -1; dx <= 1; ++dx) {
+            for (int dy = -1; dy <= 1; ++dy) {
+                for (int dz = -1; dz <= 1; ++dz) {
+                    auto it = grid.find(key(x0 + dx, y0 + dy, z0 + dz));
+                    if (it == grid.end()) { continue; }
+                    for (uint32_t j : it->second) {
+                        if (j <= i) { continue; }
+                        if (!moving(bodies[i]) && !moving(bodies[j])) { continue; }
+                        if (overlap(lo[i], hi[i], lo[j], hi[j])) { out.push_back(((uint64_t)i << 32) | j); }
+                    }
+                }
+            }
+        }
+    }
+    for (uint32_t s : w.statics) {
+        V slo, shi;
+        aabb(w, bodies[s], s, 0.5f * w.settings.speculative, slo, shi);
+        for (uint32_t i : w.movers) {
+            if (!moving(bodies[i])) { continue; }
+            if (overlap(slo, shi, lo[i], hi[i])) { out.push_back(((uint64_t)std::min(s, i) << 32) | std::max(s, i)); }
+        }
+    }
+    for (uint32_t p : w.planes) {
+        const Pose pp = poseOf(bodies[p]);
+        const V n = pp.ax[2];
+        const float off = dot(n, pp.c);
+        for (uint32_t i : w.movers) {
+            if (!moving(bodies[i])) { continue; }
+            const V c = (lo[i] + hi[i]) * 0.5f, e = (hi[i] - lo[i]) * 0.5f;
+            const float reach = e.x * std::fabs(n.x) + e.y * std::fabs(n.y) + e.z * std::fabs(n.z);
+            if (dot(c, n) - off <= reach + 0.5f * w.settings.speculative) {
+                out.push_back(((uint64_t)std::min(p, i) << 32) | std::max(p, i));
+            }
+        }
+    }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+    return out;
+}
+
+void massProperties(tfs_body& b, const Shape& s) {
+    float m = 0.0f;
+    float I[3] = { 0, 0, 0 };
+    if (s.kind == kBox) {
+        m = s.density * 8.0f * s.half.x * s.half.y * s.half.z;
+        const float x2 = s.half.x * s.half.x, y2 = s.half.y * s.half.y, z2 = s.half.z * s.half.z;
+        I[0] = m * (y2 + z2) / 3.0f;
+        I[1] = m * (x2 + z2) / 3.0f;
+        I[2] = m * (x2 + y2) / 3.0f;
+        b.extent = len(s.half);
+    } else if (s.kind == kSphere) {
+        m = s.density * 4.0f / 3.0f * 3.14159265f * s.radius * s.radius * s.radius;
+        I[0] = I[1] = I[2] = 0.4f * m * s.radius * s.radius;
+        b.extent = s.radius;
+    }
+    std::memset(b.inertia, 0, sizeof b.inertia);
+    if (b.type == TFS_DYNAMIC && m > 0.0f) {
+        b.inv_mass = 1.0f / m;
+        b.inertia[0] = I[0];
+        b.inertia[4] = I[1];
+        b.inertia[8] = I[2];
+    } else {
+        b.inv_mass = 0.0f;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+void* tfw_create(const tfs_settings* s) {
+    World* w = new World();
+    if (s) {
+        w->settings = *s;
+    } else {
+        tfs_settings d{};
+        d.gravity[1] = -9.81f;
+        d.length_unit = 1.0f;
+        d.substeps = 4;
+        d.speculative = 0.02f;
+        w->settings = d;
+    }
+    return w;
+}
+
+void tfw_destroy(void* h) { delete (World*)h; }
+
+int tfw_add_body(void* h, int type, const float* pos, const float* rot, const float* v, const float* wv, int locks,
+                 const float* lockFrame, int awake, float linearDamping, float angularDamping, float gravityScale) {
+    World& w = *(World*)h;
+    tfs_body b;
+    std::memset(&b, 0, sizeof b);
+    b.type = (uint8_t)type;
+    for (int k = 0; k < 3; ++k) {
+        b.p[k] = pos ? pos[k] : 0.0f;
+        b.com[k] = b.p[k];
+        b.v[k] = v ? v[k] : 0.0f;
+        b.w[k] = wv ? wv[k] : 0.0f;
+    }
+    for (int k = 0; k < 4; ++k) {
+        b.q[k] = rot ? rot[k] : (k == 3 ? 1.0f : 0.0f);
+        b.lock_frame[k] = lockFrame ? lockFrame[k] : (k == 3 ? 1.0f : 0.0f);
+    }
+    b.locks = (uint8_t)locks;
+    b.awake = awake ? 1 : 0;
+    b.can_sleep = 1;
+    b.alive = 1;
+    b.kept = 1.0f;
+    b.island = UINT32_MAX;
+    b.linear_damping = linearDamping;
+    b.angular_damping = angularDamping;
+    b.gravity_scale = gravityScale;
+    w.bodies.push_back(b);
+    w.shapes.push_back(Shape{ -1 });
+    return (int)w.bodies.size() - 1;
+}
+
+// One shape per body, at its origin: 0 sphere {r}, 1 box {hx, hy, hz}, 4 half-space {}.
+int tfw_add_shape(void* h, int body, int kind, const float* dims, int ndims, const float* pos, const float* rot,
+                  float density, float friction, float restitution, float rolling) {
+    World& w = *(World*)h;
+    if (body < 0 || (size_t)body >= w.bodies.size() || w.shapes[(size_t)body].kind != -1) { return -1; }
+    if ((pos && (pos[0] != 0 || pos[1] != 0 || pos[2] != 0)) || (rot && (rot[0] != 0 || rot[1] != 0 || rot[2] != 0))) {
+        return -1;
+    }
+    Shape s;
+    s.kind = kind;
+    if (kind == kSphere && ndims >= 1) {
+        s.radius = dims[0];
+    } else if (kind == kBox && ndims >= 3) {
+        s.half = V{ dims[0], dims[1], dims[2] };
+    } else if (kind != kHalfSpace) {
+        return -1;
+    }
+    s.density = density;
+    s.friction = friction;
+    s.restitution = restitution;
+    s.rolling = rolling;
+    tfs_body& b = w.bodies[(size_t)body];
+    if (kind == kHalfSpace && b.type != TFS_STATIC) { return -1; }
+    w.shapes[(size_t)body] = s;
+    massProperties(b, s);
+    if (kind == kHalfSpace) {
+        w.planes.push_back((uint32_t)body);
+    } else if (b.type == TFS_STATIC) {
+        w.statics.push_back((uint32_t)body);
+    } else {
+        w.movers.push_back((uint32_t)body);
+    }
+    return body;
+}
+
+uint32_t tfw_bodies(void* h, tfs_body* out, uint32_t cap) {
+    World& w = *(World*)h;
+    const uint32_t n = (uint32_t)w.bodies.size();
+    if (out) { std::memcpy(out, w.bodies.data(), sizeof(tfs_body) * std::min(n, cap)); }
+    return n;
+}
+
+// The contacts of the current poses, in key order, with impulses carried from `prev` (the last
+// step's contacts, in key order). Returns the count; writes at most cap. Sleeping dynamic bodies in
+// a touching pair with an awake one are woken in `bodies`; *woke counts them.
+uint32_t tfw_collide(void* h, tfs_body* bodies, uint32_t nb, float dt, const tfs_contact* prev, uint32_t nprev,
+                     tfs_contact* out, uint32_t cap, uint32_t* woke) {
+    World& w = *(World*)h;
+    if (nb != w.bodies.size()) { return 0; }
+    const std::vector<uint64_t> pairs = candidates(w, bodies, dt);
+    std::vector<tfs_contact> found(pairs.size());
+    std::vector<uint8_t> hit(pairs.size(), 0);
+    const float matchSq = w.settings.speculative * w.settings.speculative;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long k = 0; k < (long)pairs.size(); ++k) {
+        const uint32_t ia = (uint32_t)(pairs[(size_t)k] >> 32), ib = (uint32_t)pairs[(size_t)k];
+        Manifold m;
+        if (!collide(w, bodies, ia, ib, dt, m)) { continue; }
+        tfs_contact& c = found[(size_t)k];
+        toContact(w, bodies, ia, ib, m, c);
+        const tfs_contact* it = std::lower_bound(prev, prev + nprev, c.key,
+                                                 [](const tfs_contact& x, uint64_t key) { return x.key < key; });
+        if (it != prev + nprev && it->key == c.key) { carry(*it, c, matchSq); }
+        hit[(size_t)k] = 1;
+    }
+    uint32_t n = 0, wakes = 0;
+    for (size_t k = 0; k < pairs.size(); ++k) {
+        if (!hit[k]) { continue; }
+        if (out && n < cap) { out[n] = found[k]; }
+        ++n;
+        tfs_body& a = bodies[found[k].body_a];
+        tfs_body& b = bodies[found[k].body_b];
+        for (int s = 0; s < 2; ++s) {
+            tfs_body& x = s ? b : a;
+            const tfs_body& y = s ? a : b;
+            if (x.type == TFS_DYNAMIC && !x.awake && moving(y)) {
+                x.awake = 1;
+                x.sleep_time = 0.0f;
+                ++wakes;
+            }
+        }
+    }
+    if (woke) { *woke = wakes; }
+    return n;
+}
+
+// tfs_refresh_fn: collides the listed contacts' pairs again at the bodies' current poses.
+int tfw_refresh(void* h, int, float remaining, const uint32_t* again, uint32_t n, const tfs_body* bodies, uint32_t,
+                const tfs_contact* contacts, uint32_t, tfs_contact* fresh) {
+    World& w = *(World*)h;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long k = 0; k < (long)n; ++k) {
+        const tfs_contact& c = contacts[again[k]];
+        Manifold m;
+        tfs_contact& f = fresh[k];
+        std::memset(&f, 0, sizeof f);
+        if (collide(w, bodies, c.body_a, c.body_b, remaining, m)) { toContact(w, bodies, c.body_a, c.body_b, m, f); }
+    }
+    return 0;
+}
+
+}  // extern "C"

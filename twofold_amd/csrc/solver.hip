@@ -46,6 +46,8 @@ struct DevCounts {
     uint32_t nAgain;
     uint32_t colorStart[TFS_COLORS + 1];
     uint32_t impactColorStart[TFS_COLORS + 1];
+    uint32_t colorPoints[TFS_COLORS + 1];
+    uint32_t impactColorPoints[TFS_COLORS + 1];
 };
 
 // ---- body records ----------------------------------------------------------------------------
@@ -384,7 +386,8 @@ __global__ void k_row_hist(const uint32_t* list, const uint32_t* nPtr, const tfs
 }
 
 __global__ void k_row_scatter(const uint32_t* list, const uint32_t* nPtr, const tfs_contact* contacts,
-                              const uint8_t* colors, const uint32_t* offsets, uint32_t nBlocks, uint32_t* out) {
+                              const uint8_t* colors, const uint32_t* offsets, uint32_t nBlocks, uint32_t* out,
+                              uint32_t* points) {
     __shared__ uint32_t keys[kThreads];
     const uint32_t n = *nPtr;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -399,6 +402,7 @@ __global__ void k_row_scatter(const uint32_t* list, const uint32_t* nPtr, const 
         uint32_t before = 0;
         for (uint32_t t = 0; t < threadIdx.x; ++t) { before += keys[t] == key ? 1u : 0u; }
         out[offsets[key * nBlocks + blockIdx.x] + before] = ci;
+        atomicAdd(&points[colors[ci] < TFS_COLORS ? colors[ci] : TFS_COLORS], (uint32_t)contacts[ci].count);
     }
 }
 
@@ -467,8 +471,6 @@ __global__ void k_unpack(uint32_t n, Rows rows, tfs_contact* contacts) {
     }
 }
 
-enum RowMode { kWarm = 0, kBias = 1, kRelax = 2, kRestitute = 3 };
-
 // One colour of rows. Rows of one colour move no body in common (src/internal.h:607-609), so each
 // thread owns its row's two moving bodies for the whole launch. Impact rows keep their impulses in
 // the active row of the same contact.
@@ -478,38 +480,38 @@ __global__ void __launch_bounds__(kThreads) k_rows(Rows rows, uint32_t begin, ui
     const uint32_t r = begin + gid();
     if (r >= end) { return; }
     const uint4 h = rows.hdr[r];
+    const int count = (int)(h.z & kCountMask);
+    if (count == 0) { return; }
     const float4 n = rows.nrm[r], t1 = rows.tg1[r], t2 = rows.tg2[r];
     const Row row{ h.z, vec(n.x, n.y, n.z), vec(t1.x, t1.y, t1.z), vec(t2.x, t2.y, t2.z), n.w, t1.w };
     if (MODE == kRestitute && row.restitution == 0.0f) { return; }
-    const int count = (int)(h.z & kCountMask);
-    if (count == 0) { return; }
     const uint32_t ir = rows.at ? rows.at[r] : r;
     RowPoint p[4];
-    for (int k = 0; k < count; ++k) {
-        const float4 a = rows.pa[(uint32_t)k * rows.cap + r];
-        const float4 b = rows.pb[(uint32_t)k * rows.cap + r];
-        const float4 im = imp.imp[(uint32_t)k * imp.cap + ir];
-        p[k].ra = vec(a.x, a.y, a.z);
-        p[k].adj = a.w;
-        p[k].rb = vec(b.x, b.y, b.z);
-        p[k].rv = b.w;
-        p[k].ni = im.x;
-        p[k].ti = vec(im.y, im.z, im.w);
-        p[k].nimax = imp.peak[(uint32_t)k * imp.cap + ir];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k < count) {
+            const float4 a = rows.pa[(uint32_t)k * rows.cap + r];
+            const float4 b = rows.pb[(uint32_t)k * rows.cap + r];
+            const float4 im = imp.imp[(uint32_t)k * imp.cap + ir];
+            p[k].ra = vec(a.x, a.y, a.z);
+            p[k].adj = a.w;
+            p[k].rb = vec(b.x, b.y, b.z);
+            p[k].rv = b.w;
+            p[k].ni = im.x;
+            p[k].ti = vec(im.y, im.z, im.w);
+            p[k].nimax = imp.peak[(uint32_t)k * imp.cap + ir];
+        }
     }
     Motion A = loadMotion(motion, h.x), B = loadMotion(motion, h.y);
     const Mass MA = loadMass(mass, h.x), MB = loadMass(mass, h.y);
-    if (MODE == kWarm) {
-        warmRow(row, p, MA, MB, A, B);
-    } else if (MODE == kRestitute) {
-        restituteRow(row, p, MA, MB, A, B, ctx);
-    } else {
-        solveRow(row, p, MA, MB, A, B, ctx, MODE == kBias);
-    }
+    tf::rows<MODE>(row, p, MA, MB, A, B, ctx);
     if (MODE != kWarm) {
-        for (int k = 0; k < count; ++k) {
-            imp.imp[(uint32_t)k * imp.cap + ir] = make_float4(p[k].ni, p[k].ti.x, p[k].ti.y, p[k].ti.z);
-            imp.peak[(uint32_t)k * imp.cap + ir] = p[k].nimax;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k < count) {
+                imp.imp[(uint32_t)k * imp.cap + ir] = make_float4(p[k].ni, p[k].ti.x, p[k].ti.y, p[k].ti.z);
+                imp.peak[(uint32_t)k * imp.cap + ir] = p[k].nimax;
+            }
         }
     }
     if (h.z & kMoveA) { storeVelocity(motion, h.x, A); }
@@ -624,6 +626,17 @@ __global__ void k_reset_pose(tfs_body* bodies, uint32_t n, float4* motion) {
     }
 }
 
+// The contacts to collide again and their two bodies, packed for one copy to the host.
+__global__ void k_gather_again(const uint32_t* again, uint32_t n, const tfs_contact* contacts, const tfs_body* bodies,
+                               tfs_contact* outC, tfs_body* outB) {
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        const tfs_contact& c = contacts[again[k]];
+        outC[k] = c;
+        outB[2 * k] = bodies[c.body_a];
+        outB[2 * k + 1] = bodies[c.body_b];
+    }
+}
+
 // The solver side of each refreshed fast contact (src/world.cpp:1108-1138).
 __global__ void k_refresh_apply(const uint32_t* again, uint32_t n, const tfs_contact* fresh, tfs_contact* contacts,
                                 const tfs_body* bodies, const float4* motion, const float4* mass, float matchSq) {
@@ -702,11 +715,13 @@ __global__ void k_sleep_kinematic(const tfs_contact* contacts, uint32_t nc, tfs_
     }
 }
 
+// Parents only ever move to smaller indices of the same island, so a stale parent read within a
+// launch is still an ancestor: plain loads are enough, and the next round sees every update.
 __device__ __forceinline__ uint32_t rootOf(const uint32_t* parent, uint32_t x) {
-    uint32_t p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t p = parent[x];
     while (p != x) {
         x = p;
-        p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        p = parent[x];
     }
     return x;
 }
@@ -739,7 +754,9 @@ __global__ void k_island_time(tfs_body* bodies, uint32_t n, const uint32_t* pare
         b.island = r;
         uint32_t t;
         memcpy(&t, &b.sleep_time, 4);
-        atomicMin(&islandTime[r], t);
+        // Sleep times are never negative, so their bits order as the floats do. Most bodies of a large
+        // island carry the same minimum (often zero): read first and only contend when it helps.
+        if (t < __hip_atomic_load(&islandTime[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { atomicMin(&islandTime[r], t); }
     }
 }
 
@@ -795,6 +812,18 @@ struct tfs_ctx {
     bool haveBodies = false, haveContacts = false;
     uint32_t serial = 0;
     float solveMs = 0.0f, sleepMs = 0.0f;
+    // Measurement: HIP events around each row launch, summed per family (0 warm start, 1 solve,
+    // 2 restitution), with the algorithmic bytes of the rows each launch covers.
+    bool profiling = false;
+    std::vector<hipEvent_t> evPool;
+    struct Pending {
+        int family;
+        size_t ev;
+        double bytes;
+    };
+    std::vector<Pending> pending;
+    uint64_t profLaunches[3] = {};
+    double profMs[3] = {}, profBytes[3] = {};
     uint32_t lastCounts[6] = {};
 
     // bodies
@@ -840,6 +869,11 @@ struct tfs_ctx {
     std::vector<tfs_body> hBodies;
     std::vector<tfs_contact> hContacts, hFresh;
     std::vector<uint32_t> hAgain;
+    std::vector<tfs_contact> hGatherC;
+    std::vector<tfs_body> hGatherB;
+    tfs_contact* gatherC = nullptr;
+    tfs_body* gatherB = nullptr;
+    uint32_t gatherCap = 0;
 };
 
 namespace {
@@ -950,29 +984,66 @@ int scan(tfs_ctx* ctx, const uint32_t* in, const uint32_t* nPtr, uint32_t nMax, 
 }
 
 // Stable row order of `list` (length *nPtr <= nMax) into out; colour starts into starts[0..32].
-int rowOrder(tfs_ctx* ctx, const uint32_t* list, const uint32_t* nPtr, uint32_t nMax, uint32_t* out, uint32_t* starts) {
+int rowOrder(tfs_ctx* ctx, const uint32_t* list, const uint32_t* nPtr, uint32_t nMax, uint32_t* out, uint32_t* starts,
+             uint32_t* points) {
     const uint32_t nBlocks = blocksFor(nMax, kThreads, 1u << 30);
     int r;
     if ((r = ensureScratch(ctx, (size_t)kKeys * nBlocks, 0))) { return r; }
     LAUNCH(k_row_hist, nBlocks, kThreads, list, nPtr, ctx->contacts, ctx->colors, ctx->hist, nBlocks);
     if ((r = scan(ctx, ctx->hist, nullptr, kKeys * nBlocks, ctx->hist, nullptr, &ctx->cnt->scanTotal))) { return r; }
-    LAUNCH(k_row_scatter, nBlocks, kThreads, list, nPtr, ctx->contacts, ctx->colors, ctx->hist, nBlocks, out);
+    LAUNCH(k_row_scatter, nBlocks, kThreads, list, nPtr, ctx->contacts, ctx->colors, ctx->hist, nBlocks, out, points);
     LAUNCH(k_row_starts, 1, 64, ctx->hist, nBlocks, starts);
     return 0;
 }
 
-int runRows(tfs_ctx* ctx, int mode, const Rows& rows, const uint32_t* starts, const Ctx& c) {
+// Algorithmic bytes of one pass over a contact with n points, from the reference's row model
+// (src/internal.h:579-605): contact data, two bodies gathered and scattered, and per point the three
+// axes, anchors, separation and impulses.
+double rowBytes(int mode, uint32_t rows, uint32_t points) {
+    switch (mode) {
+        case kWarm: return 224.0 * rows + 244.0 * points;
+        case kRestitute: return 224.0 * rows + 96.0 * points;
+        default: return 224.0 * rows + 296.0 * points;
+    }
+}
+
+int runRows(tfs_ctx* ctx, int mode, const Rows& rows, const uint32_t* starts, const uint32_t* points, const Ctx& c) {
     for (int col = 0; col < TFS_COLORS; ++col) {
         const uint32_t b = starts[col], e = starts[col + 1];
         if (e <= b) { continue; }
         const uint32_t grid = (e - b + kThreads - 1) / kThreads;
+        size_t ev = 0;
+        if (ctx->profiling) {
+            ev = ctx->pending.size() * 2;
+            while (ctx->evPool.size() < ev + 2) {
+                hipEvent_t x;
+                HIP_TRY(hipEventCreate(&x));
+                ctx->evPool.push_back(x);
+            }
+            const int family = mode == kWarm ? 0 : mode == kRestitute ? 2 : 1;
+            ctx->pending.push_back({ family, ev, rowBytes(mode, e - b, points[col]) });
+            HIP_TRY(hipEventRecord(ctx->evPool[ev], ctx->stream));
+        }
         switch (mode) {
             case kWarm: LAUNCH(k_rows<kWarm>, grid, kThreads, rows, b, e, ctx->rows, ctx->motion, ctx->mass, c); break;
             case kBias: LAUNCH(k_rows<kBias>, grid, kThreads, rows, b, e, ctx->rows, ctx->motion, ctx->mass, c); break;
             case kRelax: LAUNCH(k_rows<kRelax>, grid, kThreads, rows, b, e, ctx->rows, ctx->motion, ctx->mass, c); break;
             default: LAUNCH(k_rows<kRestitute>, grid, kThreads, rows, b, e, ctx->rows, ctx->motion, ctx->mass, c); break;
         }
+        if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
     }
+    return 0;
+}
+
+int collectProfile(tfs_ctx* ctx) {
+    for (const auto& p : ctx->pending) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->evPool[p.ev], ctx->evPool[p.ev + 1]));
+        ctx->profLaunches[p.family] += 1;
+        ctx->profMs[p.family] += ms;
+        ctx->profBytes[p.family] += p.bytes;
+    }
+    ctx->pending.clear();
     return 0;
 }
 
@@ -1000,14 +1071,29 @@ int refresh(tfs_ctx* ctx, int substep, const uint32_t* again, uint32_t nAgain, f
     LAUNCH(k_rebase, blocksFor(hc.nActive), kThreads, ctx->active, ctx->cnt, ctx->stamp, serial, ctx->contacts, ctx->motion);
     LAUNCH(k_reset_pose, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion);
     LAUNCH(k_writeback, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion);
+    // Only the contacts collided again and their bodies go to the host; the callback reads no others.
+    if (nAgain > ctx->gatherCap) {
+        int e;
+        if ((e = grow(ctx, ctx->gatherC, nAgain)) || (e = grow(ctx, ctx->gatherB, 2 * (size_t)nAgain))) { return e; }
+        ctx->gatherCap = nAgain;
+    }
+    LAUNCH(k_gather_again, blocksFor(nAgain), kThreads, again, nAgain, ctx->contacts, ctx->bodies, ctx->gatherC, ctx->gatherB);
     ctx->hBodies.resize(nb);
     ctx->hContacts.resize(nc);
     ctx->hAgain.resize(nAgain);
     ctx->hFresh.assign(nAgain, tfs_contact{});
-    HIP_TRY(hipMemcpyAsync(ctx->hBodies.data(), ctx->bodies, sizeof(tfs_body) * nb, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->hContacts.data(), ctx->contacts, sizeof(tfs_contact) * nc, hipMemcpyDeviceToHost, ctx->stream));
+    ctx->hGatherC.resize(nAgain);
+    ctx->hGatherB.resize(2 * (size_t)nAgain);
+    HIP_TRY(hipMemcpyAsync(ctx->hGatherC.data(), ctx->gatherC, sizeof(tfs_contact) * nAgain, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->hGatherB.data(), ctx->gatherB, sizeof(tfs_body) * 2 * nAgain, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipMemcpyAsync(ctx->hAgain.data(), again, sizeof(uint32_t) * nAgain, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (uint32_t k = 0; k < nAgain; ++k) {
+        const tfs_contact& gc = ctx->hGatherC[k];
+        ctx->hContacts[ctx->hAgain[k]] = gc;
+        ctx->hBodies[gc.body_a] = ctx->hGatherB[2 * k];
+        ctx->hBodies[gc.body_b] = ctx->hGatherB[2 * k + 1];
+    }
     if (cb(user, substep, remaining, ctx->hAgain.data(), nAgain, ctx->hBodies.data(), nb, ctx->hContacts.data(), nc,
            ctx->hFresh.data()) != 0) {
         return fail(ctx, TFS_E_CALLBACK, "the refresh callback failed at substep %d", substep);
@@ -1068,8 +1154,10 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
            ctx->rank, ctx->word, ctx->colors, ctx->ticket, ctx->cnt);
 
     // row order and packing
-    if ((r = rowOrder(ctx, ctx->active, &ctx->cnt->nActive, nc, ctx->rowsList, ctx->cnt->colorStart)) ||
-        (r = rowOrder(ctx, ctx->impact, &ctx->cnt->nImpact, nc, ctx->impactRowsList, ctx->cnt->impactColorStart))) {
+    if ((r = rowOrder(ctx, ctx->active, &ctx->cnt->nActive, nc, ctx->rowsList, ctx->cnt->colorStart,
+                      ctx->cnt->colorPoints)) ||
+        (r = rowOrder(ctx, ctx->impact, &ctx->cnt->nImpact, nc, ctx->impactRowsList, ctx->cnt->impactColorStart,
+                      ctx->cnt->impactColorPoints))) {
         return r;
     }
     if ((r = syncCounts(ctx))) { return r; }
@@ -1095,31 +1183,31 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
             }
         }
         LAUNCH(k_integrate_velocities, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c);
-        if ((r = runRows(ctx, kWarm, ctx->rows, hc.colorStart, c)) ||
+        if ((r = runRows(ctx, kWarm, ctx->rows, hc.colorStart, hc.colorPoints, c)) ||
             (r = runRest(ctx, kWarm, ctx->rowsList, laned, hc.nActive, c)) ||
-            (r = runRows(ctx, kBias, ctx->rows, hc.colorStart, c)) ||
+            (r = runRows(ctx, kBias, ctx->rows, hc.colorStart, hc.colorPoints, c)) ||
             (r = runRest(ctx, kBias, ctx->rowsList, laned, hc.nActive, c))) {
             return r;
         }
         if (hc.nImpact) {
-            if ((r = runRows(ctx, kBias, ctx->impactRows, hc.impactColorStart, c)) ||
+            if ((r = runRows(ctx, kBias, ctx->impactRows, hc.impactColorStart, hc.impactColorPoints, c)) ||
                 (r = runRest(ctx, kBias, ctx->impactRowsList, impactLaned, hc.nImpact, c))) {
                 return r;
             }
         }
         LAUNCH(k_integrate_positions, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c);
-        if ((r = runRows(ctx, kRelax, ctx->rows, hc.colorStart, c)) ||
+        if ((r = runRows(ctx, kRelax, ctx->rows, hc.colorStart, hc.colorPoints, c)) ||
             (r = runRest(ctx, kRelax, ctx->rowsList, laned, hc.nActive, c))) {
             return r;
         }
         if (hc.nImpact) {
-            if ((r = runRows(ctx, kRelax, ctx->impactRows, hc.impactColorStart, c)) ||
+            if ((r = runRows(ctx, kRelax, ctx->impactRows, hc.impactColorStart, hc.impactColorPoints, c)) ||
                 (r = runRest(ctx, kRelax, ctx->impactRowsList, impactLaned, hc.nImpact, c))) {
                 return r;
             }
         }
     }
-    if ((r = runRows(ctx, kRestitute, ctx->rows, hc.colorStart, c)) ||
+    if ((r = runRows(ctx, kRestitute, ctx->rows, hc.colorStart, hc.colorPoints, c)) ||
         (r = runRest(ctx, kRestitute, ctx->rowsList, laned, hc.nActive, c))) {
         return r;
     }
@@ -1149,6 +1237,7 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     HIP_TRY(hipEventSynchronize(ctx->ev[2]));
     HIP_TRY(hipEventElapsedTime(&ctx->solveMs, ctx->ev[0], ctx->ev[1]));
     HIP_TRY(hipEventElapsedTime(&ctx->sleepMs, ctx->ev[1], ctx->ev[2]));
+    if ((r = collectProfile(ctx))) { return r; }
     uint32_t nColors = 0;
     for (int col = 0; col < TFS_COLORS; ++col) {
         if (hc.colorStart[col + 1] > hc.colorStart[col]) { nColors = (uint32_t)col + 1; }
@@ -1222,7 +1311,7 @@ void tfs_destroy(tfs_ctx* ctx) {
                      ctx->stamp, ctx->hist, ctx->tileSums, ctx->ticket, ctx->cnt, ctx->rows.hdr, ctx->rows.nrm,
                      ctx->rows.tg1, ctx->rows.tg2, ctx->rows.pa, ctx->rows.pb, ctx->rows.imp, ctx->rows.peak,
                      ctx->impactRows.hdr, ctx->impactRows.nrm, ctx->impactRows.tg1, ctx->impactRows.tg2,
-                     ctx->impactRows.pa, ctx->impactRows.pb, ctx->impactRows.at };
+                     ctx->impactRows.pa, ctx->impactRows.pb, ctx->impactRows.at, ctx->gatherC, ctx->gatherB };
     for (void* p : ptrs) {
         if (p) { (void)hipFree(p); }
     }
@@ -1230,6 +1319,7 @@ void tfs_destroy(tfs_ctx* ctx) {
     for (hipEvent_t e : ctx->ev) {
         if (e) { (void)hipEventDestroy(e); }
     }
+    for (hipEvent_t e : ctx->evPool) { (void)hipEventDestroy(e); }
     if (ctx->stream) { (void)hipStreamDestroy(ctx->stream); }
     delete ctx;
 }
@@ -1333,6 +1423,26 @@ int tfs_last_step_ms(tfs_ctx* ctx, float* solveMs, float* sleepMs) {
     if (!ctx) { return TFS_E_ARG; }
     if (solveMs) { *solveMs = ctx->solveMs; }
     if (sleepMs) { *sleepMs = ctx->sleepMs; }
+    return 0;
+}
+
+int tfs_set_profiling(tfs_ctx* ctx, int on) {
+    if (!ctx) { return TFS_E_ARG; }
+    ctx->profiling = on != 0;
+    ctx->pending.clear();
+    for (int f = 0; f < 3; ++f) {
+        ctx->profLaunches[f] = 0;
+        ctx->profMs[f] = 0.0;
+        ctx->profBytes[f] = 0.0;
+    }
+    return 0;
+}
+
+int tfs_read_profile(tfs_ctx* ctx, int family, uint64_t* launches, double* ms, double* bytes) {
+    if (!ctx || family < 0 || family > 2) { return fail(ctx, TFS_E_ARG, "unknown profile family"); }
+    if (launches) { *launches = ctx->profLaunches[family]; }
+    if (ms) { *ms = ctx->profMs[family]; }
+    if (bytes) { *bytes = ctx->profBytes[family]; }
     return 0;
 }
 

@@ -234,11 +234,15 @@ struct Row {
     float friction, restitution;
 };
 
+// The row functions take the point count as a template argument, so every point index is known at
+// compile time and the points stay in registers; rows() below picks the instance.
+
 // warmStartLanes, one lane: src/lanes.cpp:195-217.
-TF_HD void warmRow(const Row& r, const RowPoint* p, const Mass& ma, const Mass& mb, Motion& a, Motion& b) {
+template <int N>
+TF_HD void warmRowN(const Row& r, const RowPoint* p, const Mass& ma, const Mass& mb, Motion& a, Motion& b) {
     const bool moveA = r.flags & kMoveA, moveB = r.flags & kMoveB;
-    const int count = (int)(r.flags & kCountMask);
-    for (int i = 0; i < count; ++i) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
         push(a, b, makeAxis(ma, mb, p[i].ra, p[i].rb, r.n), p[i].ni, moveA, moveB);
         push(a, b, makeAxis(ma, mb, p[i].ra, p[i].rb, r.t1), dot(p[i].ti, r.t1), moveA, moveB);
         push(a, b, makeAxis(ma, mb, p[i].ra, p[i].rb, r.t2), dot(p[i].ti, r.t2), moveA, moveB);
@@ -247,19 +251,19 @@ TF_HD void warmRow(const Row& r, const RowPoint* p, const Mass& ma, const Mass& 
 
 // The 3D contact solve: solveLanes for one lane (src/lanes.cpp:220-315), which is also the 3D part
 // of solveContacts (src/solver.cpp:370-417).
-TF_HD void solveRow(const Row& r, RowPoint* p, const Mass& ma, const Mass& mb, Motion& a, Motion& b, const Ctx& c,
-                    bool useBias) {
+template <int N, bool BIAS>
+TF_HD void solveRowN(const Row& r, RowPoint* p, const Mass& ma, const Mass& mb, Motion& a, Motion& b, const Ctx& c) {
     const bool moveA = r.flags & kMoveA, moveB = r.flags & kMoveB;
-    const int count = (int)(r.flags & kCountMask);
     const Soft soft = (moveA && moveB) ? c.contactSoft : c.staticSoft;
     const V3 dp = b.dp - a.dp;
-    for (int i = 0; i < count; ++i) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
         const V3 d = dp + turned(b, p[i].rb) - turned(a, p[i].ra);
         const float sep = dot(d, r.n) + p[i].adj;
         float bias = 0.0f, massScale = 1.0f, impulseScale = 0.0f;
         if (sep > 0.0f) {
             bias = sep * c.invH;
-        } else if (useBias) {
+        } else if (BIAS) {
             bias = maxf(soft.biasRate * sep, -c.maxPushout);
             massScale = soft.massScale;
             impulseScale = soft.impulseScale;
@@ -273,8 +277,9 @@ TF_HD void solveRow(const Row& r, RowPoint* p, const Mass& ma, const Mass& mb, M
         p[i].nimax = maxf(p[i].nimax, impulse);
         push(a, b, x, impulse, moveA, moveB);
     }
-    for (int j = 0; j < count; ++j) {
-        const int i = useBias ? j : count - 1 - j;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        const int i = BIAS ? j : N - 1 - j;
         const Axis x1 = makeAxis(ma, mb, p[i].ra, p[i].rb, r.t1);
         const Axis x2 = makeAxis(ma, mb, p[i].ra, p[i].rb, r.t2);
         const float old1 = dot(p[i].ti, r.t1), old2 = dot(p[i].ti, r.t2);
@@ -294,11 +299,13 @@ TF_HD void solveRow(const Row& r, RowPoint* p, const Mass& ma, const Mass& mb, M
 }
 
 // restituteLanes, one lane: src/lanes.cpp:318-352.
-TF_HD void restituteRow(const Row& r, RowPoint* p, const Mass& ma, const Mass& mb, Motion& a, Motion& b, const Ctx& c) {
+template <int N>
+TF_HD void restituteRowN(const Row& r, RowPoint* p, const Mass& ma, const Mass& mb, Motion& a, Motion& b,
+                         const Ctx& c) {
     const bool moveA = r.flags & kMoveA, moveB = r.flags & kMoveB;
-    const int count = (int)(r.flags & kCountMask);
     const float e = r.restitution;
-    for (int i = 0; i < count; ++i) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
         const bool on = !(e == 0.0f) && !(p[i].rv > -c.restitutionThreshold) && !(p[i].nimax == 0.0f);
         const Axis x = makeAxis(ma, mb, p[i].ra, p[i].rb, r.n);
         const float vn = speedAlong(a, b, r.n, x);
@@ -310,6 +317,31 @@ TF_HD void restituteRow(const Row& r, RowPoint* p, const Mass& ma, const Mass& m
             p[i].nimax = maxf(p[i].nimax, impulse);
         }
         push(a, b, x, impulse, on && moveA, on && moveB);
+    }
+}
+
+enum RowMode { kWarm = 0, kBias = 1, kRelax = 2, kRestitute = 3 };
+
+template <int MODE, int N>
+TF_HD void rowN(const Row& r, RowPoint* p, const Mass& ma, const Mass& mb, Motion& a, Motion& b, const Ctx& c) {
+    if (MODE == kWarm) {
+        warmRowN<N>(r, p, ma, mb, a, b);
+    } else if (MODE == kRestitute) {
+        restituteRowN<N>(r, p, ma, mb, a, b, c);
+    } else {
+        solveRowN<N, MODE == kBias>(r, p, ma, mb, a, b, c);
+    }
+}
+
+// One row in the given mode, for the row's point count.
+template <int MODE>
+TF_HD void rows(const Row& r, RowPoint* p, const Mass& ma, const Mass& mb, Motion& a, Motion& b, const Ctx& c) {
+    switch (r.flags & kCountMask) {
+        case 1: rowN<MODE, 1>(r, p, ma, mb, a, b, c); break;
+        case 2: rowN<MODE, 2>(r, p, ma, mb, a, b, c); break;
+        case 3: rowN<MODE, 3>(r, p, ma, mb, a, b, c); break;
+        case 4: rowN<MODE, 4>(r, p, ma, mb, a, b, c); break;
+        default: break;
     }
 }
 
@@ -625,7 +657,12 @@ TF_HD void solveRest(tfs_contact& c, const BodyView& va, const BodyView& vb, con
     }
     RowPoint p[4];
     pointsOf(c, p);
-    solveRow(rowOf(c, !anchoredView(va), !anchoredView(vb)), p, ma, mb, a, b, ctx, useBias);
+    const Row row = rowOf(c, !anchoredView(va), !anchoredView(vb));
+    if (useBias) {
+        rows<kBias>(row, p, ma, mb, a, b, ctx);
+    } else {
+        rows<kRelax>(row, p, ma, mb, a, b, ctx);
+    }
     storeImpulses(c, p);
     if (c.rolling_resistance > 0.0f) { resistTurning(c, ma, mb, a, b); }
 }

@@ -23,7 +23,7 @@ REFRESH_FN = ctypes.CFUNCTYPE(ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, c
 EXPORTS = ("tfs_abi_version", "tfs_sizeof_body", "tfs_sizeof_contact", "tfs_sizeof_settings", "tfs_default_settings",
            "tfs_create", "tfs_destroy", "tfs_last_error", "tfs_set_settings", "tfs_upload_bodies",
            "tfs_upload_contacts", "tfs_step", "tfs_download_bodies", "tfs_download_contacts", "tfs_download_list",
-           "tfs_last_step_ms", "tfs_last_step_counts")
+           "tfs_last_step_ms", "tfs_last_step_counts", "tfs_set_profiling", "tfs_read_profile")
 
 
 class SolverError(RuntimeError):
@@ -58,6 +58,9 @@ def load_library(path=LIB_PATH):
     lib.tfs_download_list.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
     lib.tfs_last_step_ms.argtypes = [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
     lib.tfs_last_step_counts.argtypes = [_vp, _vp]
+    lib.tfs_set_profiling.argtypes = [_vp, ctypes.c_int]
+    lib.tfs_read_profile.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double)]
     if lib.tfs_abi_version() != 1:
         raise SolverError("ABI version mismatch")
     _abi.check_sizes(lib, "tfs")
@@ -110,6 +113,15 @@ class Solver:
         self._check(self.lib.tfs_upload_bodies(self.h, _ptr(b), len(b)), "tfs_upload_bodies")
         self._check(self.lib.tfs_upload_contacts(self.h, _ptr(c), len(c)), "tfs_upload_contacts")
         self.nb, self.nc = len(b), len(c)
+
+    def upload_contacts(self, contacts):
+        c = np.ascontiguousarray(contacts, CONTACT_DTYPE)
+        self._check(self.lib.tfs_upload_contacts(self.h, _ptr(c), len(c)), "tfs_upload_contacts")
+        self.nc = len(c)
+
+    def step_native(self, dt, fn, user):
+        """One step with a native refresh callback (a REFRESH_FN pointer and its user pointer)."""
+        self._check(self.lib.tfs_step(self.h, ctypes.c_float(dt), fn, user), "tfs_step")
 
     def step(self, dt, refresh=None):
         """One step. `refresh(substep, remaining, again, bodies, contacts) -> fresh contacts` is called
@@ -167,6 +179,17 @@ class Solver:
         a, b = ctypes.c_float(), ctypes.c_float()
         self._check(self.lib.tfs_last_step_ms(self.h, ctypes.byref(a), ctypes.byref(b)), "tfs_last_step_ms")
         return a.value, b.value
+
+    def set_profiling(self, on):
+        self._check(self.lib.tfs_set_profiling(self.h, 1 if on else 0), "tfs_set_profiling")
+
+    def profile(self, family):
+        """(launches, milliseconds, algorithmic bytes) summed since profiling was turned on; family 0 warm
+        start, 1 solve, 2 restitution."""
+        n, ms, by = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.tfs_read_profile(self.h, family, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by)),
+                    "tfs_read_profile")
+        return n.value, ms.value, by.value
 
     def counts(self):
         """active contacts, their points, impact contacts, their points, colours used, rest contacts."""

@@ -1,0 +1,135 @@
+"""A host world that drives the GPU solver end to end (libtwofold_scene.so + libtwofold_hip.so).
+
+HostWorld holds boxes, spheres and half-spaces built from a scene recipe (twofold_amd.scenes) and runs
+the broadphase and narrowphase on the host; Simulation steps it the way World::step does
+(src/world.cpp:1564-1727): contacts from the narrowphase, then the solve block and the sleep pass on
+the GPU, with fast pairs collided again mid-step through the native refresh callback. Only the solver
+is the product's hot path; this module is how a user (and the benchmark) feeds it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _abi
+from ._abi import BODY_DTYPE, CONTACT_DTYPE, SETTINGS_DTYPE
+from .solver import REFRESH_FN, Solver, SolverError
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtwofold_scene.so")
+
+_vp = ctypes.c_void_p
+_fp = ctypes.POINTER(ctypes.c_float)
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SolverError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        lib = ctypes.CDLL(LIB_PATH)
+        lib.tfw_create.restype = _vp
+        lib.tfw_create.argtypes = [_vp]
+        lib.tfw_destroy.argtypes = [_vp]
+        lib.tfw_add_body.restype = ctypes.c_int
+        lib.tfw_add_body.argtypes = [_vp, ctypes.c_int, _fp, _fp, _fp, _fp, ctypes.c_int, _fp, ctypes.c_int,
+                                     ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        lib.tfw_add_shape.restype = ctypes.c_int
+        lib.tfw_add_shape.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _fp, ctypes.c_int, _fp, _fp, ctypes.c_float,
+                                      ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        lib.tfw_bodies.restype = ctypes.c_uint32
+        lib.tfw_bodies.argtypes = [_vp, _vp, ctypes.c_uint32]
+        lib.tfw_collide.restype = ctypes.c_uint32
+        lib.tfw_collide.argtypes = [_vp, _vp, ctypes.c_uint32, ctypes.c_float, _vp, ctypes.c_uint32, _vp,
+                                    ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        _lib = lib
+    return _lib
+
+
+def _f(v):
+    return np.ascontiguousarray(v, np.float32).ctypes.data_as(_fp)
+
+
+class HostWorld:
+    """Bodies and shapes of a scene recipe, with a narrowphase for them."""
+
+    def __init__(self, scene, settings=None):
+        self.lib = _load()
+        self.settings = settings if settings is not None else _abi.default_settings(substeps=scene.substeps)
+        self.h = self.lib.tfw_create(self.settings.ctypes.data_as(_vp))
+        self.dt = scene.dt
+        for b in scene.bodies:
+            self.lib.tfw_add_body(self.h, b["type"], _f(b["pos"]), _f(b["rot"]), _f(b["v"]), _f(b["w"]), b["locks"],
+                                  _f(b["lock_frame"]), 1 if b["awake"] else 0, b["linear_damping"],
+                                  b["angular_damping"], b["gravity_scale"])
+        for sh in scene.shapes:
+            dims = sh["dims"] if len(sh["dims"]) else np.zeros(1, np.float32)
+            if self.lib.tfw_add_shape(self.h, sh["body"], sh["kind"], _f(dims), len(sh["dims"]), _f(sh["pos"]),
+                                      _f(sh["rot"]), sh["density"], sh["friction"], sh["restitution"],
+                                      sh["rolling"]) < 0:
+                raise ValueError("the host world takes one box, sphere or half-space per body, at its origin")
+        # the native refresh callback, handed straight to the solver
+        self.refresh_fn = REFRESH_FN(("tfw_refresh", self.lib))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.tfw_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def bodies(self):
+        n = self.lib.tfw_bodies(self.h, None, 0)
+        out = np.zeros(n, BODY_DTYPE)
+        self.lib.tfw_bodies(self.h, out.ctypes.data_as(_vp), n)
+        return out
+
+    def collide(self, bodies, dt, prev=None):
+        """Contacts at the bodies' poses, in key order, impulses carried from prev. May wake bodies in
+        place; returns (contacts, woken)."""
+        prev = np.zeros(0, CONTACT_DTYPE) if prev is None else np.ascontiguousarray(prev, CONTACT_DTYPE)
+        woke = ctypes.c_uint32()
+        cap = max(4 * len(bodies), 1024)
+        while True:
+            out = np.zeros(cap, CONTACT_DTYPE)
+            n = self.lib.tfw_collide(self.h, bodies.ctypes.data_as(_vp), len(bodies), np.float32(dt),
+                                     prev.ctypes.data_as(_vp), len(prev), out.ctypes.data_as(_vp), cap,
+                                     ctypes.byref(woke))
+            if n <= cap:
+                return out[:n], woke.value
+            cap = n
+
+
+class Simulation:
+    """A scene stepped with the host narrowphase and the GPU solver."""
+
+    def __init__(self, scene, settings=None, device=0):
+        self.world = HostWorld(scene, settings)
+        self.settings = self.world.settings
+        self.dt = scene.dt
+        self.bodies = self.world.bodies()
+        self.contacts = np.zeros(0, CONTACT_DTYPE)
+        self.solver = Solver(len(self.bodies), max(4 * len(self.bodies), 64), device=device, settings=self.settings)
+        self.solver.upload(self.bodies, self.contacts)
+        self.steps = 0
+
+    def collide(self):
+        """The narrowphase for the next step; uploads its contacts (and bodies it woke)."""
+        self.bodies = self.solver.bodies() if self.steps else self.bodies
+        contacts, woke = self.world.collide(self.bodies, self.dt, self.contacts)
+        if woke or not self.steps:
+            self.solver.upload(self.bodies, contacts)
+        else:
+            self.solver.upload_contacts(contacts)
+        self.contacts = contacts
+
+    def solve(self):
+        """The solve block and sleep pass on the GPU, over the uploaded contacts."""
+        self.solver.step_native(self.dt, self.world.refresh_fn, self.world.h)
+        self.contacts = self.solver.contacts()
+        self.steps += 1
+
+    def step(self):
+        self.collide()
+        self.solve()
