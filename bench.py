@@ -10,7 +10,8 @@ as the reference's solve block does. A contact constraint-iteration is one conta
 friction update in one velocity pass: per step, substeps x 2 passes x (active points + impact points).
 
 Also reported: the roofline of the solve-row kernels (algorithmic bytes per launch / event-timed
-launch duration, against 8 TB/s) and the CPU oracle timed single-threaded on the same input.
+launch duration, against 8 TB/s) and a single-threaded CPU baseline on the same pile state: the
+reference engine built from its sources where that build is present, else the CPU oracle.
 """
 import argparse
 import ctypes
@@ -48,12 +49,7 @@ def iterations(counts, substeps):
     return substeps * 2 * (counts["points"] + counts["impact_points"])
 
 
-def cpu_baseline(sim, bodies, contacts, budget_s):
-    """The oracle (oracle/liboracle.so, a single-threaded CPU restatement of the reference solver, pinned
-    bit-exact to it) timed on the same input, with the same refresh callback run on one thread."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import refworld
-
+def _one_core():
     try:
         ctypes.CDLL("libgomp.so.1").omp_set_num_threads(1)
     except OSError:
@@ -62,6 +58,42 @@ def cpu_baseline(sim, bodies, contacts, budget_s):
         os.sched_setaffinity(0, {min(os.sched_getaffinity(0))})
     except (AttributeError, OSError):
         pass
+
+
+def cpu_reference(scene, bodies, substeps, budget_s):
+    """The reference engine itself (oracle/_ref/libtwofold_ref.so, built from its sources), one thread:
+    the same pile put in the same body state, stepped with its own narrowphase, its solve block timed by
+    its own StepProfile::solver (src/world.cpp:1644-1692)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import refworld
+
+    if not refworld.ref_available():
+        return None
+    _one_core()
+    w = refworld.RefWorld(scene)
+    w.set_bodies(bodies)
+    for _ in range(2):  # contacts and their warm-start impulses build up
+        w.step(record=False)
+    steps, its, spent = 0, 0, 0.0
+    while steps < 2 or spent < budget_s:
+        w.step(record=False)
+        spent += w.profile()["solver"] * 1e-3
+        c = w.counts()
+        its += substeps * 2 * int(c[1] + c[3])
+        steps += 1
+    return dict(value=its / spent, unit="iterations/s", cores=1, kind="reference",
+                sample=f"{steps} steps of the reference on the same settled pile from the same body state "
+                       f"(its own narrowphase), solve block timed by StepProfile::solver: {spent:.1f} s of solve, "
+                       f"one thread, CPU: {cpu_model()}", ms_per_step=1e3 * spent / steps)
+
+
+def cpu_port(sim, bodies, contacts, budget_s):
+    """The oracle (oracle/liboracle.so, a single-threaded CPU restatement of the reference solver, pinned
+    bit-exact to it) timed on the same input, with the same refresh callback run on one thread."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import refworld
+
+    _one_core()
     lib = refworld._load_oracle()
     b = np.ascontiguousarray(bodies.copy())
     c = np.ascontiguousarray(contacts.copy())
@@ -147,8 +179,9 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(sim, bodies0, contacts0, args.cpu_seconds)
-        log(f"cpu baseline: {cpu['value']:.3e} iterations/s ({cpu['ms_per_step']:.0f} ms/step)")
+        cpu = cpu_reference(scene, bodies0, substeps, args.cpu_seconds) or cpu_port(sim, bodies0, contacts0,
+                                                                                    args.cpu_seconds)
+        log(f"cpu baseline ({cpu['kind']}): {cpu['value']:.3e} iterations/s ({cpu['ms_per_step']:.0f} ms/step)")
 
     out = {
         "metric": "contact constraint-iterations/s",

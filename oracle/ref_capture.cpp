@@ -172,10 +172,14 @@ struct Capture {
 struct Handle {
     World world;
     Capture cap;
+    uint32_t counts[4] = {};
     explicit Handle(const WorldSettings& s) : world(s) {}
 };
 
 Capture* g_cap = nullptr;
+// Point counts of the last step (active contacts, their points, impact contacts, their points),
+// gathered in the batchContacts hook for the CPU timing leg.
+uint32_t* g_counts = nullptr;
 World::Impl* g_impl = nullptr;
 const Contact* g_contactBase = nullptr;
 int g_substep = 0;
@@ -214,7 +218,16 @@ void tfcap_colorContacts(SolverData& s, const std::vector<uint32_t>& list, Conta
 void tfcap_batchContacts(SolverData& s, const std::vector<uint32_t>& list, const ContactColors& colors,
                          const SolverContext& ctx, ContactBatches& out) {
     batchContacts(s, list, colors, ctx, out);
-    if (!g_cap) { return; }
+    if (g_counts) {
+        uint32_t points = 0;
+        for (uint32_t ci : list) { points += (uint32_t)s.contacts[ci].count; }
+        g_counts[g_batchCalls == 0 ? 0 : 2] = (uint32_t)list.size();
+        g_counts[g_batchCalls == 0 ? 1 : 3] = points;
+    }
+    if (!g_cap) {
+        ++g_batchCalls;
+        return;
+    }
     Capture& c = *g_cap;
     if (g_batchCalls == 0) {
         c.byColor = out.byColor;
@@ -402,19 +415,40 @@ int tfr_add_shape(void* h, int body, int kind, const float* dims, int ndims, con
 int tfr_step(void* h, float dt, int record) {
     Handle& hd = *(Handle*)h;
     hd.cap = Capture{};
-    g_cap = record ? &hd.cap : nullptr;
+    g_cap = record == 1 ? &hd.cap : nullptr;
+    std::memset(hd.counts, 0, sizeof hd.counts);
+    g_counts = hd.counts;
     g_impl = hd.world.impl_.get();
     g_substep = 0;
     g_batchCalls = 0;
     g_lanesCalls = 0;
     g_final = false;
     hd.world.step(dt);
-    if (record) {
+    if (record == 1) {
         for (const Body& b : g_impl->bodies) { hd.cap.stepBodies.push_back(toBody(b)); }
     }
     g_cap = nullptr;
+    g_counts = nullptr;
     g_impl = nullptr;
     return hd.cap.solved ? 1 : 0;
+}
+
+// Active contacts, their points, impact contacts, their points of the last step.
+void tfr_counts(void* h, uint32_t* out) { std::memcpy(out, ((Handle*)h)->counts, sizeof ((Handle*)h)->counts); }
+
+// Moves every body that is not static to the pose and velocities of `b` (indexed as the world's
+// bodies), through the public API; bodies asleep in `b` are put to sleep.
+void tfr_set_bodies(void* h, const tfs_body* b, uint32_t n) {
+    World& w = ((Handle*)h)->world;
+    for (uint32_t i = 0; i < n && i < w.impl_->bodies.size(); ++i) {
+        const Body& x = w.impl_->bodies[i];
+        if (!x.alive || x.type == MotionType::Static) { continue; }
+        const BodyId id{ i, x.generation };
+        w.setTransform(id, vec3(b[i].p), quat4(b[i].q));
+        w.setLinearVelocity(id, vec3(b[i].v));
+        w.setAngularVelocity(id, vec3(b[i].w));
+        if (!b[i].awake) { w.sleep(id); }
+    }
 }
 
 // StepProfile of the last step: total, broadphase, narrowphase, solver, continuous, sleep (ms).

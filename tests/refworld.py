@@ -53,6 +53,8 @@ def _load_ref():
         lib.tfr_step.restype = ctypes.c_int
         lib.tfr_step.argtypes = [_vp, ctypes.c_float, ctypes.c_int]
         lib.tfr_profile.argtypes = [_vp, _fp]
+        lib.tfr_counts.argtypes = [_vp, _vp]
+        lib.tfr_set_bodies.argtypes = [_vp, _vp, ctypes.c_uint32]
         for name in ("tfr_num_bodies", "tfr_num_contacts", "tfr_cap_num_refreshes"):
             getattr(lib, name).restype = ctypes.c_uint32
             getattr(lib, name).argtypes = [_vp]
@@ -108,9 +110,19 @@ class RefWorld:
             self.lib.tfr_destroy(self.h)
             self.h = None
 
+    def set_bodies(self, bodies):
+        b = np.ascontiguousarray(bodies, BODY_DTYPE)
+        self.lib.tfr_set_bodies(self.h, _ptr(b), len(b))
+
+    def counts(self):
+        """Active contacts, their points, impact contacts, their points, of the last step."""
+        out = np.zeros(4, np.uint32)
+        self.lib.tfr_counts(self.h, _ptr(out))
+        return out
+
     def step(self, dt=None, record=True):
         dt = self.dt if dt is None else np.float32(dt)
-        solved = self.lib.tfr_step(self.h, dt, 1 if record else 0)
+        solved = self.lib.tfr_step(self.h, dt, 1 if record else 2)
         if not record:
             return None
         if not solved:

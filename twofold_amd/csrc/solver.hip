@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -42,8 +43,11 @@ struct DevCounts {
     uint32_t pointsActive, pointsImpact;
     uint32_t scanTotal;
     uint32_t error;
-    uint32_t changed;
-    uint32_t nAgain;
+    uint32_t anyRestitution;
+    uint32_t ticket;
+    uint32_t pad_;
+    unsigned long long barrier;
+    uint32_t changed[64];
     uint32_t colorStart[TFS_COLORS + 1];
     uint32_t impactColorStart[TFS_COLORS + 1];
     uint32_t colorPoints[TFS_COLORS + 1];
@@ -163,8 +167,20 @@ __global__ void k_scan_apply(const uint32_t* in, const uint32_t* nPtr, uint32_t 
 // ---- step state and lists ------------------------------------------------------------------------
 
 // src/world.cpp:1591-1601: pose at step start, travel bound, dp and dq reset, inverse mass.
-__global__ void k_step_init(tfs_body* bodies, uint32_t n, float dt, float4* motion, float4* mass, uint8_t* moving) {
+// Also clears the per-body scratch of the step (impact marks, colouring words and degrees, warm-start
+// degrees and cursors), which saves a memset each.
+struct BodyScratch {
+    uint8_t* struck;
+    uint32_t *deg, *cursor, *bdeg, *bcursor;
+    unsigned long long* word;
+};
+
+__global__ void k_step_init(tfs_body* bodies, uint32_t n, float dt, float4* motion, float4* mass, uint8_t* moving,
+                            BodyScratch z) {
     for (uint32_t i = gid(); i < n; i += gstride()) {
+        z.struck[i] = 0;
+        z.deg[i] = z.cursor[i] = z.bdeg[i] = z.bcursor[i] = 0;
+        z.word[i] = 0ull;
         tfs_body b = bodies[i];
         if (b.alive) {
             for (int k = 0; k < 4; ++k) {
@@ -187,8 +203,9 @@ __global__ void k_step_init(tfs_body* bodies, uint32_t n, float dt, float4* moti
 
 // src/world.cpp:1604-1621: flags for the active, fast and fast-solid lists.
 __global__ void k_classify(const tfs_contact* contacts, uint32_t nc, const tfs_body* bodies, float dt, float speculative,
-                           uint32_t* active, uint32_t* fast, uint32_t* fastSolid) {
+                           uint32_t* active, uint32_t* fast, uint32_t* fastSolid, uint8_t* colors) {
     for (uint32_t i = gid(); i < nc; i += gstride()) {
+        colors[i] = 0xff;
         const tfs_contact& c = contacts[i];
         const tfs_body& a = bodies[c.body_a];
         const tfs_body& b = bodies[c.body_b];
@@ -210,7 +227,7 @@ __global__ void k_classify(const tfs_contact* contacts, uint32_t nc, const tfs_b
 // (src/world.cpp:1029-1041).
 __global__ void k_prepare(const uint32_t* active, const DevCounts* cnt, tfs_contact* contacts, const tfs_body* bodies,
                           const float4* motion, const float4* mass, float threshold, uint8_t* struck,
-                          uint32_t* points) {
+                          uint32_t* points, uint32_t* anyRestitution) {
     const uint32_t n = cnt->nActive;
     for (uint32_t k = gid(); k < n; k += gstride()) {
         const uint32_t ci = active[k];
@@ -221,6 +238,7 @@ __global__ void k_prepare(const uint32_t* active, const DevCounts* cnt, tfs_cont
                        loadMass(mass, c.body_a), loadMass(mass, c.body_b), c);
         contacts[ci] = c;
         atomicAdd(points, (uint32_t)c.count);
+        if (c.restitution != 0.0f && c.count > 0) { anyRestitution[0] = 1u; }
         for (int i = 0; i < c.count; ++i) {
             if (c.points[i].relative_velocity >= -threshold) { continue; }
             if (ta.type == TFS_DYNAMIC) { struck[c.body_a] = 1; }
@@ -385,22 +403,36 @@ __global__ void k_row_hist(const uint32_t* list, const uint32_t* nPtr, const tfs
     for (uint32_t key = threadIdx.x; key < kKeys; key += blockDim.x) { hist[key * nBlocks + blockIdx.x] = h[key]; }
 }
 
-__global__ void k_row_scatter(const uint32_t* list, const uint32_t* nPtr, const tfs_contact* contacts,
-                              const uint8_t* colors, const uint32_t* offsets, uint32_t nBlocks, uint32_t* out,
-                              uint32_t* points) {
-    __shared__ uint32_t keys[kThreads];
+__global__ void __launch_bounds__(kThreads) k_row_scatter(const uint32_t* list, const uint32_t* nPtr,
+                                                          const tfs_contact* contacts, const uint8_t* colors,
+                                                          const uint32_t* offsets, uint32_t nBlocks, uint32_t* out,
+                                                          uint32_t* points) {
+    // Rank of each item among the earlier items of the block with the same key: within the wave from
+    // ballots on the key's bits, across the block's waves from per-wave key counts.
+    __shared__ uint32_t waveCount[kThreads / 64][kKeys];
     const uint32_t n = *nPtr;
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t key = 0xffffffffu, ci = 0;
-    if (k < n) {
+    const uint32_t wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    for (uint32_t x = threadIdx.x; x < (kThreads / 64) * kKeys; x += blockDim.x) { (&waveCount[0][0])[x] = 0; }
+    __syncthreads();
+    const bool on = k < n;
+    uint32_t key = kKeys, ci = 0;
+    if (on) {
         ci = list[k];
         key = rowKey(contacts[ci], colors[ci]);
     }
-    keys[threadIdx.x] = key;
+    unsigned long long same = __ballot(on);
+    for (int bit = 0; bit < 8; ++bit) {
+        const unsigned long long b = __ballot((key >> bit) & 1u);
+        same &= ((key >> bit) & 1u) ? b : ~b;
+    }
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    const uint32_t inWave = (uint32_t)__popcll(same & below);
+    if (on && inWave == 0) { waveCount[wave][key] = (uint32_t)__popcll(same); }
     __syncthreads();
-    if (k < n) {
-        uint32_t before = 0;
-        for (uint32_t t = 0; t < threadIdx.x; ++t) { before += keys[t] == key ? 1u : 0u; }
+    if (on) {
+        uint32_t before = inWave;
+        for (uint32_t w = 0; w < wave; ++w) { before += waveCount[w][key]; }
         out[offsets[key * nBlocks + blockIdx.x] + before] = ci;
         atomicAdd(&points[colors[ci] < TFS_COLORS ? colors[ci] : TFS_COLORS], (uint32_t)contacts[ci].count);
     }
@@ -414,7 +446,8 @@ __global__ void k_row_starts(const uint32_t* offsets, uint32_t nBlocks, uint32_t
 
 // ---- rows --------------------------------------------------------------------------------------
 
-// Structure-of-arrays rows: entry [r] or [k * cap + r] for point k.
+// Structure-of-arrays rows: entry [r] or [k * cap + r] for point k. Impact rows are a list of
+// active row indices (`at`): the same contacts, solved again in their own order.
 struct Rows {
     uint4* hdr;     // body a, body b, flags, contact index
     float4* nrm;    // normal, friction
@@ -424,12 +457,49 @@ struct Rows {
     float4* pb;     // anchor b, relative velocity
     float4* imp;    // normal impulse, tangent impulse
     float* peak;    // largest normal impulse
-    uint32_t* at;   // impact rows: the active row holding the impulses
+    float4* ax;     // point axes: [((k * 3 + d) * 5 + q) * cap + r], d = normal, tangent1, tangent2
     uint32_t cap;
 };
 
-__global__ void k_pack(const uint32_t* list, uint32_t n, const tfs_contact* contacts, const uint8_t* moving, Rows rows,
-                       uint32_t* rowOf, const uint32_t* activeRowOf) {
+// An axis as 5 float4: q0 = linearA, angularA.x; q1 = angularA.yz, linearB.xy; q2 = linearB.z,
+// angularB; q3 = armA, armB.x; q4 = armB.yz, mass. A body's side of the axis (what a push applies to
+// it) is q0-q1 for body A and q1-q2 for body B, so the warm start reads two float4 per axis.
+__device__ __forceinline__ Axis axisOf(float4 a, float4 b, float4 e, float4 f, float4 g) {
+    Axis x;
+    x.linearA = vec(a.x, a.y, a.z);
+    x.angularA = vec(a.w, b.x, b.y);
+    x.linearB = vec(b.z, b.w, e.x);
+    x.angularB = vec(e.y, e.z, e.w);
+    x.armA = vec(f.x, f.y, f.z);
+    x.armB = vec(f.w, g.x, g.y);
+    x.mass = g.z;
+    return x;
+}
+__device__ __forceinline__ void storeAxis(const Rows& rows, uint32_t r, int k, int d, const Axis& x) {
+    float4* q = rows.ax + (size_t)((k * 3 + d) * 5) * rows.cap + r;
+    const size_t c = rows.cap;
+    q[0] = make_float4(x.linearA.x, x.linearA.y, x.linearA.z, x.angularA.x);
+    q[c] = make_float4(x.angularA.y, x.angularA.z, x.linearB.x, x.linearB.y);
+    q[2 * c] = make_float4(x.linearB.z, x.angularB.x, x.angularB.y, x.angularB.z);
+    q[3 * c] = make_float4(x.armA.x, x.armA.y, x.armA.z, x.armB.x);
+    q[4 * c] = make_float4(x.armB.y, x.armB.z, x.mass, 0.0f);
+}
+__device__ __forceinline__ Axis loadAxis(const Rows& rows, uint32_t r, int k, int d) {
+    const float4* q = rows.ax + (size_t)((k * 3 + d) * 5) * rows.cap + r;
+    const size_t c = rows.cap;
+    return axisOf(q[0], q[c], q[2 * c], q[3 * c], q[4 * c]);
+}
+
+struct LoadedAxes {
+    const Rows& rows;
+    uint32_t r;
+    __device__ __forceinline__ Axis operator()(int i, int d) const { return loadAxis(rows, r, i, d); }
+};
+
+// Packs the coloured contacts into rows in row order and builds their point axes from the bodies'
+// current masses, as prepareLanes does (src/lanes.cpp:66-110, 409-411).
+__global__ void k_pack(const uint32_t* list, uint32_t n, const tfs_contact* contacts, const uint8_t* moving,
+                       const float4* mass, Rows rows, uint32_t* rowOf) {
     for (uint32_t r = gid(); r < n; r += gstride()) {
         const uint32_t ci = list[r];
         const tfs_contact& c = contacts[ci];
@@ -438,19 +508,24 @@ __global__ void k_pack(const uint32_t* list, uint32_t n, const tfs_contact* cont
         rows.nrm[r] = make_float4(c.normal[0], c.normal[1], c.normal[2], c.friction);
         rows.tg1[r] = make_float4(c.tangent1[0], c.tangent1[1], c.tangent1[2], c.restitution);
         rows.tg2[r] = make_float4(c.tangent2[0], c.tangent2[1], c.tangent2[2], 0.0f);
+        const Mass ma = loadMass(mass, c.body_a), mb = loadMass(mass, c.body_b);
+        const V3 dir[3] = { ld3(c.normal), ld3(c.tangent1), ld3(c.tangent2) };
         for (int k = 0; k < c.count; ++k) {
             const tfs_point& p = c.points[k];
             const uint32_t j = (uint32_t)k * rows.cap + r;
             rows.pa[j] = make_float4(p.anchor_a[0], p.anchor_a[1], p.anchor_a[2], p.adjusted_separation);
             rows.pb[j] = make_float4(p.anchor_b[0], p.anchor_b[1], p.anchor_b[2], p.relative_velocity);
-            if (!activeRowOf) {
-                rows.imp[j] = make_float4(p.normal_impulse, p.tangent_impulse[0], p.tangent_impulse[1], p.tangent_impulse[2]);
-                rows.peak[j] = p.max_normal_impulse;
-            }
+            rows.imp[j] = make_float4(p.normal_impulse, p.tangent_impulse[0], p.tangent_impulse[1], p.tangent_impulse[2]);
+            rows.peak[j] = p.max_normal_impulse;
+            for (int d = 0; d < 3; ++d) { storeAxis(rows, r, k, d, makeAxis(ma, mb, ld3(p.anchor_a), ld3(p.anchor_b), dir[d])); }
         }
-        if (rowOf) { rowOf[ci] = r; }
-        if (activeRowOf) { rows.at[r] = activeRowOf[ci]; }
+        rowOf[ci] = r;
     }
+}
+
+// The impact rows' places among the active rows.
+__global__ void k_impact_rows(const uint32_t* list, uint32_t n, const uint32_t* rowOf, uint32_t* at) {
+    for (uint32_t r = gid(); r < n; r += gstride()) { at[r] = rowOf[list[r]]; }
 }
 
 // Impulses of the coloured rows back into the contacts.
@@ -471,51 +546,272 @@ __global__ void k_unpack(uint32_t n, Rows rows, tfs_contact* contacts) {
     }
 }
 
-// One colour of rows. Rows of one colour move no body in common (src/internal.h:607-609), so each
-// thread owns its row's two moving bodies for the whole launch. Impact rows keep their impulses in
-// the active row of the same contact.
-template <int MODE>
-__global__ void __launch_bounds__(kThreads) k_rows(Rows rows, uint32_t begin, uint32_t end, Rows imp, float4* motion,
-                                                   const float4* mass, Ctx ctx) {
-    const uint32_t r = begin + gid();
-    if (r >= end) { return; }
+// Velocity and impulse accesses of the rows. Within one launch per colour, kernel boundaries make
+// every write visible to the next colour, and plain loads and stores serve. In a cooperative pass
+// the colours follow each other inside one launch, across blocks on different XCDs; there the rows
+// read and write velocities and impulses with device-coherent (agent-scope) atomic loads and stores,
+// so that a barrier only has to order them, not flush caches.
+template <bool COHERENT>
+struct Access {
+    __device__ __forceinline__ static float4 ld(const float4* p) {
+        if (!COHERENT) { return *p; }
+        const unsigned long long* q = (const unsigned long long*)p;
+        const unsigned long long a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        float4 r;
+        memcpy(&r.x, &a, 8);
+        memcpy(&r.z, &b, 8);
+        return r;
+    }
+    __device__ __forceinline__ static void st(float4* p, float4 v) {
+        if (!COHERENT) {
+            *p = v;
+            return;
+        }
+        unsigned long long a, b;
+        memcpy(&a, &v.x, 8);
+        memcpy(&b, &v.z, 8);
+        unsigned long long* q = (unsigned long long*)p;
+        __hip_atomic_store(q, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(q + 1, b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ __forceinline__ static float ld1(const float* p) {
+        if (!COHERENT) { return *p; }
+        const uint32_t a = __hip_atomic_load((const uint32_t*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        float r;
+        memcpy(&r, &a, 4);
+        return r;
+    }
+    __device__ __forceinline__ static void st1(float* p, float v) {
+        if (!COHERENT) {
+            *p = v;
+            return;
+        }
+        uint32_t a;
+        memcpy(&a, &v, 4);
+        __hip_atomic_store((uint32_t*)p, a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+};
+
+// The packed axes of a row, already in registers: axv[point][axis * 5 + q].
+template <int Q>
+struct RegAxes {
+    const float4 (&axv)[4][Q];
+    __device__ __forceinline__ Axis operator()(int i, int d) const {
+        return axisOf(axv[i][d * 5], axv[i][d * 5 + 1], axv[i][d * 5 + 2], axv[i][d * 5 + 3], axv[i][d * 5 + 4]);
+    }
+};
+
+// The axes of a row, built on the spot from the bodies' masses (the same bits as the packed ones).
+struct RowMadeAxes {
+    Mass ma, mb;
+    const Row& r;
+    const RowPoint* p;
+    __device__ __forceinline__ Axis operator()(int i, int d) const {
+        return makeAxis(ma, mb, p[i].ra, p[i].rb, d == 0 ? r.n : d == 1 ? r.t1 : r.t2);
+    }
+};
+
+// One row. Rows of one colour move no body in common (src/internal.h:607-609), so within a colour
+// each row's thread owns the row's two moving bodies. `at` maps impact rows to active rows. MADE
+// builds the point axes on the spot from the masses instead of reading the packed ones.
+template <int MODE, bool COHERENT, bool MADE>
+__device__ __forceinline__ void rowStep(const Rows& rows, const uint32_t* at, uint32_t i, float4* motion,
+                                        const float4* mass, const Ctx& ctx) {
+    using X = Access<COHERENT>;
+    const uint32_t r = at ? at[i] : i;
     const uint4 h = rows.hdr[r];
     const int count = (int)(h.z & kCountMask);
     if (count == 0) { return; }
     const float4 n = rows.nrm[r], t1 = rows.tg1[r], t2 = rows.tg2[r];
     const Row row{ h.z, vec(n.x, n.y, n.z), vec(t1.x, t1.y, t1.z), vec(t2.x, t2.y, t2.z), n.w, t1.w };
     if (MODE == kRestitute && row.restitution == 0.0f) { return; }
-    const uint32_t ir = rows.at ? rows.at[r] : r;
+    // Every load the row needs is issued here, before any arithmetic, so the row waits for memory
+    // twice (its header, then everything else) rather than once per axis.
     RowPoint p[4];
+    float4 axv[4][MADE ? 1 : 15];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (k < count) {
-            const float4 a = rows.pa[(uint32_t)k * rows.cap + r];
-            const float4 b = rows.pb[(uint32_t)k * rows.cap + r];
-            const float4 im = imp.imp[(uint32_t)k * imp.cap + ir];
+            const uint32_t j = (uint32_t)k * rows.cap + r;
+            const float4 a = rows.pa[j], b = rows.pb[j], im = X::ld(&rows.imp[j]);
             p[k].ra = vec(a.x, a.y, a.z);
             p[k].adj = a.w;
             p[k].rb = vec(b.x, b.y, b.z);
             p[k].rv = b.w;
             p[k].ni = im.x;
             p[k].ti = vec(im.y, im.z, im.w);
-            p[k].nimax = imp.peak[(uint32_t)k * imp.cap + ir];
+            p[k].nimax = X::ld1(&rows.peak[j]);
+            if (!MADE) {
+#pragma unroll
+                for (int q = 0; q < 15; ++q) { axv[k][q] = rows.ax[(size_t)(k * 15 + q) * rows.cap + r]; }
+            }
         }
     }
-    Motion A = loadMotion(motion, h.x), B = loadMotion(motion, h.y);
-    const Mass MA = loadMass(mass, h.x), MB = loadMass(mass, h.y);
-    tf::rows<MODE>(row, p, MA, MB, A, B, ctx);
+    const float4 va = X::ld(&motion[4 * h.x]), wa = X::ld(&motion[4 * h.x + 1]);
+    const float4 vb = X::ld(&motion[4 * h.y]), wb = X::ld(&motion[4 * h.y + 1]);
+    const float4 pa = motion[4 * h.x + 2], qa = motion[4 * h.x + 3], pb = motion[4 * h.y + 2], qb = motion[4 * h.y + 3];
+    Motion A{ vec(va.x, va.y, va.z), vec(wa.x, wa.y, wa.z), vec(pa.x, pa.y, pa.z), quat(qa.x, qa.y, qa.z, qa.w) };
+    Motion B{ vec(vb.x, vb.y, vb.z), vec(wb.x, wb.y, wb.z), vec(pb.x, pb.y, pb.z), quat(qb.x, qb.y, qb.z, qb.w) };
+    if (MADE) {
+        const RowMadeAxes ax{ loadMass(mass, h.x), loadMass(mass, h.y), row, p };
+        tf::rows<MODE>(row, p, ax, A, B, ctx);
+    } else {
+        const RegAxes<MADE ? 1 : 15> ax{ axv };
+        tf::rows<MODE>(row, p, ax, A, B, ctx);
+    }
     if (MODE != kWarm) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (k < count) {
-                imp.imp[(uint32_t)k * imp.cap + ir] = make_float4(p[k].ni, p[k].ti.x, p[k].ti.y, p[k].ti.z);
-                imp.peak[(uint32_t)k * imp.cap + ir] = p[k].nimax;
+                const uint32_t j = (uint32_t)k * rows.cap + r;
+                X::st(&rows.imp[j], make_float4(p[k].ni, p[k].ti.x, p[k].ti.y, p[k].ti.z));
+                X::st1(&rows.peak[j], p[k].nimax);
             }
         }
     }
-    if (h.z & kMoveA) { storeVelocity(motion, h.x, A); }
-    if (h.z & kMoveB) { storeVelocity(motion, h.y, B); }
+    if (h.z & kMoveA) {
+        X::st(&motion[4 * h.x], make_float4(A.v.x, A.v.y, A.v.z, 0.0f));
+        X::st(&motion[4 * h.x + 1], make_float4(A.w.x, A.w.y, A.w.z, 0.0f));
+    }
+    if (h.z & kMoveB) {
+        X::st(&motion[4 * h.y], make_float4(B.v.x, B.v.y, B.v.z, 0.0f));
+        X::st(&motion[4 * h.y + 1], make_float4(B.w.x, B.w.y, B.w.z, 0.0f));
+    }
+}
+
+// One colour per launch.
+template <int MODE, bool MADE>
+__global__ void __launch_bounds__(kThreads) k_rows(Rows rows, const uint32_t* at, uint32_t begin, uint32_t end,
+                                                   float4* motion, const float4* mass, Ctx ctx) {
+    const uint32_t i = begin + gid();
+    if (i < end) { rowStep<MODE, false, MADE>(rows, at, i, motion, mass, ctx); }
+}
+
+// A whole pass in one cooperative launch: the colours of the active rows, then those of the impact
+// rows, one after the other with a grid-wide barrier between them.
+constexpr int kPassThreads = 256;
+constexpr int kMaxSegments = 2 * TFS_COLORS;
+
+struct PassPlan {
+    uint32_t n;                 // segments
+    uint32_t impactFrom;        // segments from here are impact rows
+    uint32_t begin[kMaxSegments], end[kMaxSegments];
+};
+
+// Every block waits for its own coherent stores to complete, then its first thread arrives on a
+// 64-bit counter and waits until every block has. Rows exchange data only through coherent
+// accesses, so no cache needs flushing.
+__device__ __forceinline__ void gridSync(unsigned long long* counter, unsigned long long& phase, DevCounts* cnt) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        phase += gridDim.x;
+        atomicAdd(counter, 1ull);
+        uint32_t spins = 0;
+        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < phase) {
+            if (++spins > kSpinLimit) {
+                atomicOr(&cnt->error, 2u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+}
+
+template <int MODE, bool MADE>
+__global__ void __launch_bounds__(kPassThreads) k_pass(Rows rows, const uint32_t* at, PassPlan plan, float4* motion,
+                                                       const float4* mass, Ctx ctx, unsigned long long* counter,
+                                                       unsigned long long base, DevCounts* cnt) {
+    unsigned long long phase = base;
+    for (uint32_t s = 0; s < plan.n; ++s) {
+        const uint32_t* map = s >= plan.impactFrom ? at : nullptr;
+        for (uint32_t i = plan.begin[s] + gid(); i < plan.end[s]; i += gstride()) {
+            rowStep<MODE, true, MADE>(rows, map, i, motion, mass, ctx);
+        }
+        if (s + 1 < plan.n) { gridSync(counter, phase, cnt); }
+    }
+}
+
+// ---- warm start, one body at a time ----------------------------------------------------------------
+//
+// warmStartBatches applies each coloured row's stored impulses to its moving bodies, row by row
+// (src/lanes.cpp:195-217, 413-417). A warm-start push does not depend on the velocities it changes,
+// so each body can apply its own rows' pushes, in row order, by itself: one launch instead of one
+// per colour, with the same operations on each body in the same order.
+
+__global__ void k_bep_count(uint32_t n, const uint4* hdr, uint32_t* deg) {
+    for (uint32_t r = gid(); r < n; r += gstride()) {
+        const uint4 h = hdr[r];
+        if ((h.z & kCountMask) == 0) { continue; }
+        if (h.z & kMoveA) { atomicAdd(&deg[h.x], 1u); }
+        if (h.z & kMoveB) { atomicAdd(&deg[h.y], 1u); }
+    }
+}
+
+__global__ void k_bep_fill(uint32_t n, const uint4* hdr, const uint32_t* start, uint32_t* cursor, uint32_t* ep) {
+    for (uint32_t r = gid(); r < n; r += gstride()) {
+        const uint4 h = hdr[r];
+        if ((h.z & kCountMask) == 0) { continue; }
+        if (h.z & kMoveA) { ep[start[h.x] + atomicAdd(&cursor[h.x], 1u)] = r << 1; }
+        if (h.z & kMoveB) { ep[start[h.y] + atomicAdd(&cursor[h.y], 1u)] = (r << 1) | 1u; }
+    }
+}
+
+__global__ void k_bep_sort(uint32_t nb, const uint32_t* deg, const uint32_t* start, uint32_t* ep) {
+    for (uint32_t x = gid(); x < nb; x += gstride()) {
+        uint32_t* e = ep + start[x];
+        const uint32_t d = deg[x];
+        for (uint32_t i = 1; i < d; ++i) {
+            const uint32_t v = e[i];
+            uint32_t j = i;
+            while (j > 0 && e[j - 1] > v) {
+                e[j] = e[j - 1];
+                --j;
+            }
+            e[j] = v;
+        }
+    }
+}
+
+__global__ void k_warm_bodies(uint32_t nb, const uint32_t* deg, const uint32_t* start, const uint32_t* ep, Rows rows,
+                              float4* motion) {
+    for (uint32_t x = gid(); x < nb; x += gstride()) {
+        const uint32_t d = deg[x];
+        if (d == 0) { continue; }
+        Motion m = loadMotion(motion, x);
+        for (uint32_t k = 0; k < d; ++k) {
+            const uint32_t e = ep[start[x] + k];
+            const uint32_t r = e >> 1;
+            const bool sideB = e & 1u;
+            const int count = (int)(rows.hdr[r].z & kCountMask);
+            const float4 t1 = rows.tg1[r], t2 = rows.tg2[r];
+            const V3 dir1 = vec(t1.x, t1.y, t1.z), dir2 = vec(t2.x, t2.y, t2.z);
+            for (int i = 0; i < count; ++i) {
+                const float4 im = rows.imp[(uint32_t)i * rows.cap + r];
+                // this body's side of the point's three axes
+                float4 q[3][2];
+                for (int a = 0; a < 3; ++a) {
+                    const float4* base = rows.ax + (size_t)((i * 3 + a) * 5 + (sideB ? 1 : 0)) * rows.cap + r;
+                    q[a][0] = base[0];
+                    q[a][1] = base[rows.cap];
+                }
+                const V3 ti = vec(im.y, im.z, im.w);
+                const float lambda[3] = { im.x, dot(ti, dir1), dot(ti, dir2) };
+                for (int a = 0; a < 3; ++a) {
+                    if (sideB) {
+                        m.v = m.v + vec(q[a][0].z, q[a][0].w, q[a][1].x) * lambda[a];
+                        m.w = m.w + vec(q[a][1].y, q[a][1].z, q[a][1].w) * lambda[a];
+                    } else {
+                        m.v = m.v - vec(q[a][0].x, q[a][0].y, q[a][0].z) * lambda[a];
+                        m.w = m.w - vec(q[a][0].w, q[a][1].x, q[a][1].y) * lambda[a];
+                    }
+                }
+            }
+        }
+        storeVelocity(motion, x, m);
+    }
 }
 
 // Contacts outside the colours, one at a time in list order, as solveContacts, warmStart and
@@ -593,47 +889,65 @@ __global__ void k_writeback(tfs_body* bodies, uint32_t n, const float4* motion) 
 
 // ---- refresh: src/world.cpp:1660-1668, 1057-1139 --------------------------------------------------
 
-__global__ void k_update_mass_awake(tfs_body* bodies, uint32_t n, float4* mass) {
-    for (uint32_t i = gid(); i < n; i += gstride()) {
-        tfs_body& b = bodies[i];
-        if (!(b.alive && b.type == TFS_DYNAMIC && b.awake)) { continue; }
-        updateInverseMass(b);
-        storeMass(mass, i, b);
-    }
-}
-
-__global__ void k_mark(const uint32_t* list, uint32_t n, uint32_t* stamp, uint32_t serial) {
-    for (uint32_t k = gid(); k < n; k += gstride()) { stamp[list[k]] = serial; }
-}
-
-__global__ void k_rebase(const uint32_t* active, const DevCounts* cnt, const uint32_t* stamp, uint32_t serial,
-                         tfs_contact* contacts, const float4* motion) {
+// Before the narrowphase of a refresh, over the active contacts: the coloured rows' impulses go back
+// into their contacts (the refresh carries them), and every contact that is not collided again is
+// rebased onto the bodies' current poses (src/world.cpp:1061-1067). `again` marks the contacts
+// collided again: the fast flags of the step (src/world.cpp:1659).
+__global__ void k_refresh_pre(const uint32_t* active, const DevCounts* cnt, const uint32_t* againFlag,
+                              tfs_contact* contacts, Rows rows, const uint32_t* rowOf, const uint8_t* colors,
+                              const float4* motion) {
     const uint32_t n = cnt->nActive;
     for (uint32_t k = gid(); k < n; k += gstride()) {
         const uint32_t ci = active[k];
-        if (stamp[ci] == serial) { continue; }
         tfs_contact& c = contacts[ci];
-        rebaseContact(loadMotion(motion, c.body_a), loadMotion(motion, c.body_b), c);
+        if (colors[ci] < TFS_COLORS) {
+            const uint32_t r = rowOf[ci];
+            for (int i = 0; i < c.count; ++i) {
+                const uint32_t j = (uint32_t)i * rows.cap + r;
+                const float4 im = rows.imp[j];
+                c.points[i].normal_impulse = im.x;
+                c.points[i].tangent_impulse[0] = im.y;
+                c.points[i].tangent_impulse[1] = im.z;
+                c.points[i].tangent_impulse[2] = im.w;
+                c.points[i].max_normal_impulse = rows.peak[j];
+            }
+        }
+        if (!againFlag[ci]) { rebaseContact(loadMotion(motion, c.body_a), loadMotion(motion, c.body_b), c); }
     }
 }
 
-__global__ void k_reset_pose(tfs_body* bodies, uint32_t n, float4* motion) {
+// Then over the bodies: the inverse mass of awake dynamic bodies (src/world.cpp:1662-1664), and every
+// body's dp, dq and base orientation reset (src/world.cpp:1068-1072).
+__global__ void k_refresh_bodies(tfs_body* bodies, uint32_t n, float4* motion, float4* mass) {
     for (uint32_t i = gid(); i < n; i += gstride()) {
         tfs_body& b = bodies[i];
+        if (b.alive && b.type == TFS_DYNAMIC && b.awake) {
+            updateInverseMass(b);
+            storeMass(mass, i, b);
+        }
         for (int k = 0; k < 4; ++k) { b.q_base[k] = b.q[k]; }
         motion[4 * i + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         motion[4 * i + 3] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
     }
 }
 
-// The contacts to collide again and their two bodies, packed for one copy to the host.
+// The contacts to collide again and their two bodies (velocities from the motion records), packed for
+// one copy to the host.
 __global__ void k_gather_again(const uint32_t* again, uint32_t n, const tfs_contact* contacts, const tfs_body* bodies,
-                               tfs_contact* outC, tfs_body* outB) {
+                               const float4* motion, tfs_contact* outC, tfs_body* outB) {
     for (uint32_t k = gid(); k < n; k += gstride()) {
         const tfs_contact& c = contacts[again[k]];
         outC[k] = c;
-        outB[2 * k] = bodies[c.body_a];
-        outB[2 * k + 1] = bodies[c.body_b];
+        for (int e = 0; e < 2; ++e) {
+            const uint32_t i = e ? c.body_b : c.body_a;
+            tfs_body b = bodies[i];
+            const Motion m = loadMotion(motion, i);
+            st3(b.v, m.v);
+            st3(b.w, m.w);
+            st3(b.dp, m.dp);
+            st4(b.dq, m.dq);
+            outB[2 * k + e] = b;
+        }
     }
 }
 
@@ -747,16 +1061,29 @@ __global__ void k_sleep_jump(uint32_t* parent, uint32_t n) {
 }
 
 __global__ void k_island_time(tfs_body* bodies, uint32_t n, const uint32_t* parent, uint32_t* islandTime) {
-    for (uint32_t i = gid(); i < n; i += gstride()) {
-        tfs_body& b = bodies[i];
-        if (!b.alive || !awakeDynamic(b)) { continue; }
-        const uint32_t r = parent[i];
-        b.island = r;
-        uint32_t t;
-        memcpy(&t, &b.sleep_time, 4);
-        // Sleep times are never negative, so their bits order as the floats do. Most bodies of a large
-        // island carry the same minimum (often zero): read first and only contend when it helps.
-        if (t < __hip_atomic_load(&islandTime[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) { atomicMin(&islandTime[r], t); }
+    // Sleep times are never negative, so their bits order as the floats do. A large island fills whole
+    // waves: its lanes take their minimum across the wave first and make one atomic between them.
+    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gstride()) {
+        const uint32_t i = base + threadIdx.x;
+        bool valid = false;
+        uint32_t r = 0, t = 0xffffffffu;
+        if (i < n) {
+            tfs_body& b = bodies[i];
+            if (b.alive && awakeDynamic(b)) {
+                valid = true;
+                r = parent[i];
+                b.island = r;
+                memcpy(&t, &b.sleep_time, 4);
+            }
+        }
+        const unsigned long long act = __ballot(valid);
+        if (act == 0ull) { continue; }
+        const int first = __ffsll((long long)act) - 1;
+        const uint32_t r0 = (uint32_t)__shfl((int)r, first);
+        uint32_t m = (valid && r == r0) ? t : 0xffffffffu;
+        for (int off = 32; off > 0; off >>= 1) { m = min(m, (uint32_t)__shfl_xor((int)m, off)); }
+        if (valid && r != r0) { atomicMin(&islandTime[r], t); }
+        if ((int)(threadIdx.x % 64) == first) { atomicMin(&islandTime[r0], m); }
     }
 }
 
@@ -810,7 +1137,6 @@ struct tfs_ctx {
     tfs_settings settings{};
     uint32_t nb = 0, nc = 0, capB = 0, capC = 0;
     bool haveBodies = false, haveContacts = false;
-    uint32_t serial = 0;
     float solveMs = 0.0f, sleepMs = 0.0f;
     // Measurement: HIP events around each row launch, summed per family (0 warm start, 1 solve,
     // 2 restitution), with the algorithmic bytes of the rows each launch covers.
@@ -856,13 +1182,21 @@ struct tfs_ctx {
     uint32_t* ep = nullptr;
     uint32_t* rank = nullptr;
     uint32_t* rowOf = nullptr;
-    uint32_t* stamp = nullptr;
     uint32_t* hist = nullptr;
     size_t histCap = 0;
     uint32_t* tileSums = nullptr;
     size_t tileCap = 0;
-    uint32_t* ticket = nullptr;
-    Rows rows{}, impactRows{};
+    Rows rows{};
+    uint32_t* impactAt = nullptr;   // impact rows as active row indices
+    unsigned long long barrierBase = 0;
+    uint32_t maxPassBlocks = 0;
+    bool usePass = true;
+    bool madeAxes = false;
+    // per-body lists of coloured rows for the warm start
+    uint32_t* bdeg = nullptr;
+    uint32_t* bstart = nullptr;
+    uint32_t* bcursor = nullptr;
+    uint32_t* bep = nullptr;
     DevCounts* cnt = nullptr;
     DevCounts* hostCnt = nullptr;
     // host staging for the refresh callback
@@ -874,6 +1208,11 @@ struct tfs_ctx {
     tfs_contact* gatherC = nullptr;
     tfs_body* gatherB = nullptr;
     uint32_t gatherCap = 0;
+    // pinned host staging of the refresh
+    tfs_contact* pinC = nullptr;
+    tfs_body* pinB = nullptr;
+    uint32_t* pinAgain = nullptr;
+    tfs_contact* pinFresh = nullptr;
 };
 
 namespace {
@@ -915,22 +1254,20 @@ int allocBodies(tfs_ctx* ctx, uint32_t cap) {
         (r = grow(ctx, ctx->mass, 6 * (size_t)cap)) || (r = grow(ctx, ctx->moving, cap)) ||
         (r = grow(ctx, ctx->struck, cap)) || (r = grow(ctx, ctx->deg, cap)) || (r = grow(ctx, ctx->epStart, cap)) ||
         (r = grow(ctx, ctx->cursor, cap)) || (r = grow(ctx, ctx->word, cap)) || (r = grow(ctx, ctx->parent, cap)) ||
-        (r = grow(ctx, ctx->islandTime, cap))) {
+        (r = grow(ctx, ctx->islandTime, cap)) || (r = grow(ctx, ctx->bdeg, cap)) || (r = grow(ctx, ctx->bstart, cap)) ||
+        (r = grow(ctx, ctx->bcursor, cap))) {
         return r;
     }
     ctx->capB = cap;
     return 0;
 }
 
-int allocRows(tfs_ctx* ctx, Rows& r, uint32_t cap, bool impact) {
+int allocRows(tfs_ctx* ctx, Rows& r, uint32_t cap) {
     int e;
     if ((e = grow(ctx, r.hdr, cap)) || (e = grow(ctx, r.nrm, cap)) || (e = grow(ctx, r.tg1, cap)) ||
-        (e = grow(ctx, r.tg2, cap)) || (e = grow(ctx, r.pa, 4 * (size_t)cap)) || (e = grow(ctx, r.pb, 4 * (size_t)cap))) {
-        return e;
-    }
-    if (impact) {
-        if ((e = grow(ctx, r.at, cap))) { return e; }
-    } else if ((e = grow(ctx, r.imp, 4 * (size_t)cap)) || (e = grow(ctx, r.peak, 4 * (size_t)cap))) {
+        (e = grow(ctx, r.tg2, cap)) || (e = grow(ctx, r.pa, 4 * (size_t)cap)) || (e = grow(ctx, r.pb, 4 * (size_t)cap)) ||
+        (e = grow(ctx, r.imp, 4 * (size_t)cap)) || (e = grow(ctx, r.peak, 4 * (size_t)cap)) ||
+        (e = grow(ctx, r.ax, 60 * (size_t)cap))) {
         return e;
     }
     r.cap = cap;
@@ -946,13 +1283,12 @@ int allocContacts(tfs_ctx* ctx, uint32_t cap) {
         (r = grow(ctx, ctx->impact, cap)) || (r = grow(ctx, ctx->rowsList, cap)) ||
         (r = grow(ctx, ctx->impactRowsList, cap)) || (r = grow(ctx, ctx->colors, cap)) ||
         (r = grow(ctx, ctx->ep, 2 * (size_t)cap)) || (r = grow(ctx, ctx->rank, 2 * (size_t)cap)) ||
-        (r = grow(ctx, ctx->rowOf, cap)) || (r = grow(ctx, ctx->stamp, cap))) {
+        (r = grow(ctx, ctx->rowOf, cap))) {
         return r;
     }
-    if ((r = allocRows(ctx, ctx->rows, cap, false)) || (r = allocRows(ctx, ctx->impactRows, cap, true))) { return r; }
-    ctx->rows.at = nullptr;
-    ctx->impactRows.imp = ctx->rows.imp;
-    ctx->impactRows.peak = ctx->rows.peak;
+    if ((r = allocRows(ctx, ctx->rows, cap)) || (r = grow(ctx, ctx->impactAt, cap)) || (r = grow(ctx, ctx->bep, 2 * (size_t)cap))) {
+        return r;
+    }
     ctx->capC = cap;
     return 0;
 }
@@ -1007,7 +1343,7 @@ double rowBytes(int mode, uint32_t rows, uint32_t points) {
     }
 }
 
-int runRows(tfs_ctx* ctx, int mode, const Rows& rows, const uint32_t* starts, const uint32_t* points, const Ctx& c) {
+int runRows(tfs_ctx* ctx, int mode, const uint32_t* at, const uint32_t* starts, const uint32_t* points, const Ctx& c) {
     for (int col = 0; col < TFS_COLORS; ++col) {
         const uint32_t b = starts[col], e = starts[col + 1];
         if (e <= b) { continue; }
@@ -1024,14 +1360,131 @@ int runRows(tfs_ctx* ctx, int mode, const Rows& rows, const uint32_t* starts, co
             ctx->pending.push_back({ family, ev, rowBytes(mode, e - b, points[col]) });
             HIP_TRY(hipEventRecord(ctx->evPool[ev], ctx->stream));
         }
-        switch (mode) {
-            case kWarm: LAUNCH(k_rows<kWarm>, grid, kThreads, rows, b, e, ctx->rows, ctx->motion, ctx->mass, c); break;
-            case kBias: LAUNCH(k_rows<kBias>, grid, kThreads, rows, b, e, ctx->rows, ctx->motion, ctx->mass, c); break;
-            case kRelax: LAUNCH(k_rows<kRelax>, grid, kThreads, rows, b, e, ctx->rows, ctx->motion, ctx->mass, c); break;
-            default: LAUNCH(k_rows<kRestitute>, grid, kThreads, rows, b, e, ctx->rows, ctx->motion, ctx->mass, c); break;
+        const float4* mass = ctx->mass;
+        if (ctx->madeAxes) {
+            switch (mode) {
+                case kBias: LAUNCH((k_rows<kBias, true>), grid, kThreads, ctx->rows, at, b, e, ctx->motion, mass, c); break;
+                case kRelax: LAUNCH((k_rows<kRelax, true>), grid, kThreads, ctx->rows, at, b, e, ctx->motion, mass, c); break;
+                default: LAUNCH((k_rows<kRestitute, true>), grid, kThreads, ctx->rows, at, b, e, ctx->motion, mass, c); break;
+            }
+        } else {
+            switch (mode) {
+                case kBias: LAUNCH((k_rows<kBias, false>), grid, kThreads, ctx->rows, at, b, e, ctx->motion, mass, c); break;
+                case kRelax: LAUNCH((k_rows<kRelax, false>), grid, kThreads, ctx->rows, at, b, e, ctx->motion, mass, c); break;
+                default: LAUNCH((k_rows<kRestitute, false>), grid, kThreads, ctx->rows, at, b, e, ctx->motion, mass, c); break;
+            }
         }
         if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
     }
+    return 0;
+}
+
+// A pass over the listed row sets (active colours, then impact colours when `withImpact`) in one
+// cooperative launch.
+int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool withImpact, const Ctx& c) {
+    PassPlan plan{};
+    uint32_t widest = 0;
+    double bytes = 0.0;
+    auto add = [&](const uint32_t* starts, const uint32_t* points) {
+        for (int col = 0; col < TFS_COLORS; ++col) {
+            if (starts[col + 1] <= starts[col]) { continue; }
+            plan.begin[plan.n] = starts[col];
+            plan.end[plan.n] = starts[col + 1];
+            widest = std::max(widest, starts[col + 1] - starts[col]);
+            bytes += rowBytes(mode, starts[col + 1] - starts[col], points[col]);
+            ++plan.n;
+        }
+    };
+    if (withActive) { add(hc.colorStart, hc.colorPoints); }
+    plan.impactFrom = plan.n;
+    if (withImpact) { add(hc.impactColorStart, hc.impactColorPoints); }
+    if (plan.n == 0) { return 0; }
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((widest + kPassThreads - 1) / kPassThreads,
+                                                                    ctx->maxPassBlocks));
+    size_t ev = 0;
+    if (ctx->profiling) {
+        ev = ctx->pending.size() * 2;
+        while (ctx->evPool.size() < ev + 2) {
+            hipEvent_t x;
+            HIP_TRY(hipEventCreate(&x));
+            ctx->evPool.push_back(x);
+        }
+        ctx->pending.push_back({ 1, ev, bytes });
+        HIP_TRY(hipEventRecord(ctx->evPool[ev], ctx->stream));
+    }
+    const uint32_t* at = ctx->impactAt;
+    unsigned long long* counter = &ctx->cnt->barrier;
+    const unsigned long long base = ctx->barrierBase;
+    const float4* mass = ctx->mass;
+    void* args[] = { &ctx->rows, (void*)&at, &plan, &ctx->motion, (void*)&mass, (void*)&c, &counter, (void*)&base, &ctx->cnt };
+    const void* fn = ctx->madeAxes ? (mode == kBias ? (const void*)k_pass<kBias, true> : (const void*)k_pass<kRelax, true>)
+                                   : (mode == kBias ? (const void*)k_pass<kBias, false> : (const void*)k_pass<kRelax, false>);
+    HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kPassThreads), args, 0, ctx->stream));
+    ctx->barrierBase += (unsigned long long)(plan.n - 1) * grid;
+    if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
+    return 0;
+}
+
+int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32_t end, const Ctx& c);
+
+// One solve pass, biased or relax: the coloured active rows, the active rest in order, then the same
+// for the impact rows (src/world.cpp:1675-1680; src/lanes.cpp:419-424).
+int solvePass(tfs_ctx* ctx, int mode, const DevCounts& hc, uint32_t laned, uint32_t impactLaned, const Ctx& c) {
+    int r;
+    const bool restA = hc.nActive > laned, impact = hc.nImpact > 0;
+    if (!ctx->usePass) {
+        if ((r = runRows(ctx, mode, nullptr, hc.colorStart, hc.colorPoints, c)) ||
+            (r = runRest(ctx, mode, ctx->rowsList, laned, hc.nActive, c))) {
+            return r;
+        }
+        if (impact && ((r = runRows(ctx, mode, ctx->impactAt, hc.impactColorStart, hc.impactColorPoints, c)) ||
+                       (r = runRest(ctx, mode, ctx->impactRowsList, impactLaned, hc.nImpact, c)))) {
+            return r;
+        }
+        return 0;
+    }
+    if (restA) {
+        if ((r = runPass(ctx, mode, hc, true, false, c)) || (r = runRest(ctx, mode, ctx->rowsList, laned, hc.nActive, c))) {
+            return r;
+        }
+        if (impact && (r = runPass(ctx, mode, hc, false, true, c))) { return r; }
+    } else if ((r = runPass(ctx, mode, hc, true, impact, c))) {
+        return r;
+    }
+    if (impact && (r = runRest(ctx, mode, ctx->impactRowsList, impactLaned, hc.nImpact, c))) { return r; }
+    return 0;
+}
+
+// The warm start of every coloured row, one launch over the bodies (see k_warm_bodies).
+int warmRows(tfs_ctx* ctx, uint32_t laned, uint32_t points) {
+    size_t ev = 0;
+    if (ctx->profiling) {
+        ev = ctx->pending.size() * 2;
+        while (ctx->evPool.size() < ev + 2) {
+            hipEvent_t x;
+            HIP_TRY(hipEventCreate(&x));
+            ctx->evPool.push_back(x);
+        }
+        ctx->pending.push_back({ 0, ev, rowBytes(kWarm, laned, points) });
+        HIP_TRY(hipEventRecord(ctx->evPool[ev], ctx->stream));
+    }
+    LAUNCH(k_warm_bodies, blocksFor(ctx->nb), kThreads, ctx->nb, ctx->bdeg, ctx->bstart, ctx->bep, ctx->rows, ctx->motion);
+    if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
+    return 0;
+}
+
+// Packs the rows and builds the impact rows and each body's list of rows.
+int packRows(tfs_ctx* ctx, uint32_t laned, uint32_t impactLaned, bool lists) {
+    LAUNCH(k_pack, blocksFor(laned), kThreads, ctx->rowsList, laned, ctx->contacts, ctx->moving, ctx->mass, ctx->rows,
+           ctx->rowOf);
+    if (!lists) { return 0; }
+    LAUNCH(k_impact_rows, blocksFor(impactLaned), kThreads, ctx->impactRowsList, impactLaned, ctx->rowOf, ctx->impactAt);
+    const uint32_t nb = ctx->nb;
+    LAUNCH(k_bep_count, blocksFor(laned), kThreads, laned, ctx->rows.hdr, ctx->bdeg);
+    int r;
+    if ((r = scan(ctx, ctx->bdeg, nullptr, nb, ctx->bstart, nullptr, &ctx->cnt->scanTotal))) { return r; }
+    LAUNCH(k_bep_fill, blocksFor(laned), kThreads, laned, ctx->rows.hdr, ctx->bstart, ctx->bcursor, ctx->bep);
+    LAUNCH(k_bep_sort, blocksFor(nb), kThreads, nb, ctx->bdeg, ctx->bstart, ctx->bep);
     return 0;
 }
 
@@ -1060,54 +1513,58 @@ int syncCounts(tfs_ctx* ctx) {
     return 0;
 }
 
-int refresh(tfs_ctx* ctx, int substep, const uint32_t* again, uint32_t nAgain, float remaining, tfs_refresh_fn cb,
-            void* user, const DevCounts& hc) {
+template <class T>
+int growPinned(tfs_ctx* ctx, T*& p, size_t n) {
+    if (p) { HIP_TRY(hipHostFree(p)); }
+    p = nullptr;
+    HIP_TRY(hipHostMalloc((void**)&p, sizeof(T) * (n ? n : 1), hipHostMallocDefault));
+    return 0;
+}
+
+int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaining, tfs_refresh_fn cb, void* user,
+            const DevCounts& hc) {
     const uint32_t nb = ctx->nb, nc = ctx->nc;
     const uint32_t laned = hc.colorStart[TFS_COLORS], impactLaned = hc.impactColorStart[TFS_COLORS];
-    LAUNCH(k_update_mass_awake, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->mass);
-    LAUNCH(k_unpack, blocksFor(laned), kThreads, laned, ctx->rows, ctx->contacts);
-    const uint32_t serial = ++ctx->serial;
-    LAUNCH(k_mark, blocksFor(nAgain), kThreads, again, nAgain, ctx->stamp, serial);
-    LAUNCH(k_rebase, blocksFor(hc.nActive), kThreads, ctx->active, ctx->cnt, ctx->stamp, serial, ctx->contacts, ctx->motion);
-    LAUNCH(k_reset_pose, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion);
-    LAUNCH(k_writeback, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion);
+    const uint32_t* again = half ? ctx->fast : ctx->fastSolid;
+    const uint32_t* againFlag = half ? ctx->fFast : ctx->fFastSolid;
+    LAUNCH(k_refresh_pre, blocksFor(hc.nActive), kThreads, ctx->active, ctx->cnt, againFlag, ctx->contacts, ctx->rows,
+           ctx->rowOf, ctx->colors, ctx->motion);
+    LAUNCH(k_refresh_bodies, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, ctx->mass);
     // Only the contacts collided again and their bodies go to the host; the callback reads no others.
     if (nAgain > ctx->gatherCap) {
         int e;
-        if ((e = grow(ctx, ctx->gatherC, nAgain)) || (e = grow(ctx, ctx->gatherB, 2 * (size_t)nAgain))) { return e; }
+        if ((e = grow(ctx, ctx->gatherC, nAgain)) || (e = grow(ctx, ctx->gatherB, 2 * (size_t)nAgain)) ||
+            (e = growPinned(ctx, ctx->pinC, nAgain)) || (e = growPinned(ctx, ctx->pinB, 2 * (size_t)nAgain)) ||
+            (e = growPinned(ctx, ctx->pinAgain, nAgain)) || (e = growPinned(ctx, ctx->pinFresh, nAgain))) {
+            return e;
+        }
         ctx->gatherCap = nAgain;
     }
-    LAUNCH(k_gather_again, blocksFor(nAgain), kThreads, again, nAgain, ctx->contacts, ctx->bodies, ctx->gatherC, ctx->gatherB);
+    LAUNCH(k_gather_again, blocksFor(nAgain), kThreads, again, nAgain, ctx->contacts, ctx->bodies, ctx->motion,
+           ctx->gatherC, ctx->gatherB);
+    HIP_TRY(hipMemcpyAsync(ctx->pinC, ctx->gatherC, sizeof(tfs_contact) * nAgain, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->pinB, ctx->gatherB, sizeof(tfs_body) * 2 * nAgain, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->pinAgain, again, sizeof(uint32_t) * nAgain, hipMemcpyDeviceToHost, ctx->stream));
     ctx->hBodies.resize(nb);
     ctx->hContacts.resize(nc);
-    ctx->hAgain.resize(nAgain);
-    ctx->hFresh.assign(nAgain, tfs_contact{});
-    ctx->hGatherC.resize(nAgain);
-    ctx->hGatherB.resize(2 * (size_t)nAgain);
-    HIP_TRY(hipMemcpyAsync(ctx->hGatherC.data(), ctx->gatherC, sizeof(tfs_contact) * nAgain, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->hGatherB.data(), ctx->gatherB, sizeof(tfs_body) * 2 * nAgain, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->hAgain.data(), again, sizeof(uint32_t) * nAgain, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     for (uint32_t k = 0; k < nAgain; ++k) {
-        const tfs_contact& gc = ctx->hGatherC[k];
-        ctx->hContacts[ctx->hAgain[k]] = gc;
-        ctx->hBodies[gc.body_a] = ctx->hGatherB[2 * k];
-        ctx->hBodies[gc.body_b] = ctx->hGatherB[2 * k + 1];
+        const tfs_contact& gc = ctx->pinC[k];
+        ctx->hContacts[ctx->pinAgain[k]] = gc;
+        ctx->hBodies[gc.body_a] = ctx->pinB[2 * k];
+        ctx->hBodies[gc.body_b] = ctx->pinB[2 * k + 1];
     }
-    if (cb(user, substep, remaining, ctx->hAgain.data(), nAgain, ctx->hBodies.data(), nb, ctx->hContacts.data(), nc,
-           ctx->hFresh.data()) != 0) {
+    memset(ctx->pinFresh, 0, sizeof(tfs_contact) * nAgain);
+    if (cb(user, substep, remaining, ctx->pinAgain, nAgain, ctx->hBodies.data(), nb, ctx->hContacts.data(), nc,
+           ctx->pinFresh) != 0) {
         return fail(ctx, TFS_E_CALLBACK, "the refresh callback failed at substep %d", substep);
     }
-    HIP_TRY(hipMemcpyAsync(ctx->fresh, ctx->hFresh.data(), sizeof(tfs_contact) * nAgain, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ctx->fresh, ctx->pinFresh, sizeof(tfs_contact) * nAgain, hipMemcpyHostToDevice, ctx->stream));
     const float matchSq = ctx->settings.speculative * ctx->settings.speculative;
     LAUNCH(k_refresh_apply, blocksFor(nAgain), kThreads, again, nAgain, ctx->fresh, ctx->contacts, ctx->bodies,
            ctx->motion, ctx->mass, matchSq);
-    LAUNCH(k_pack, blocksFor(laned), kThreads, ctx->rowsList, laned, ctx->contacts, ctx->moving, ctx->rows, ctx->rowOf,
-           (const uint32_t*)nullptr);
-    LAUNCH(k_pack, blocksFor(impactLaned), kThreads, ctx->impactRowsList, impactLaned, ctx->contacts, ctx->moving,
-           ctx->impactRows, (uint32_t*)nullptr, ctx->rowOf);
-    // The host waits for the copy of `fresh` before reusing its staging buffer next time.
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    // A refreshed contact keeps its row; its points and axes are packed again, as prepareLanes does.
+    if (int e = packRows(ctx, laned, impactLaned, false)) { return e; }
     return 0;
 }
 
@@ -1118,11 +1575,13 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     const int n = s.substeps > 1 ? s.substeps : 1;
     HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
     HIP_TRY(hipMemsetAsync(ctx->cnt, 0, sizeof(DevCounts), ctx->stream));
+    ctx->barrierBase = 0;
 
     // step state and lists
-    LAUNCH(k_step_init, blocksFor(nb), kThreads, ctx->bodies, nb, dt, ctx->motion, ctx->mass, ctx->moving);
+    const BodyScratch zero{ ctx->struck, ctx->deg, ctx->cursor, ctx->bdeg, ctx->bcursor, ctx->word };
+    LAUNCH(k_step_init, blocksFor(nb), kThreads, ctx->bodies, nb, dt, ctx->motion, ctx->mass, ctx->moving, zero);
     LAUNCH(k_classify, blocksFor(nc), kThreads, ctx->contacts, nc, ctx->bodies, dt, s.speculative, ctx->fActive,
-           ctx->fFast, ctx->fFastSolid);
+           ctx->fFast, ctx->fFastSolid, ctx->colors);
     int r;
     if ((r = scan(ctx, ctx->fActive, nullptr, nc, nullptr, ctx->active, &ctx->cnt->nActive)) ||
         (r = scan(ctx, ctx->fFast, nullptr, nc, nullptr, ctx->fast, &ctx->cnt->nFast)) ||
@@ -1131,27 +1590,21 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     }
 
     // prepare, impact
-    HIP_TRY(hipMemsetAsync(ctx->struck, 0, nb ? nb : 1, ctx->stream));
     LAUNCH(k_prepare, blocksFor(nc), kThreads, ctx->active, ctx->cnt, ctx->contacts, ctx->bodies, ctx->motion, ctx->mass,
-           c.restitutionThreshold, ctx->struck, &ctx->cnt->pointsActive);
+           c.restitutionThreshold, ctx->struck, &ctx->cnt->pointsActive, &ctx->cnt->anyRestitution);
     LAUNCH(k_impact_flag, blocksFor(nc), kThreads, ctx->active, ctx->cnt, ctx->contacts, ctx->struck, ctx->impactFlag);
     if ((r = scan(ctx, ctx->impactFlag, &ctx->cnt->nActive, nc, nullptr, ctx->impactPos, &ctx->cnt->nImpact))) { return r; }
     LAUNCH(k_gather_list, blocksFor(nc), kThreads, ctx->impactPos, &ctx->cnt->nImpact, ctx->active, ctx->impact);
     LAUNCH(k_points_of, blocksFor(nc), kThreads, ctx->impact, &ctx->cnt->nImpact, ctx->contacts, &ctx->cnt->pointsImpact);
 
     // colouring
-    HIP_TRY(hipMemsetAsync(ctx->deg, 0, sizeof(uint32_t) * (nb ? nb : 1), ctx->stream));
-    HIP_TRY(hipMemsetAsync(ctx->cursor, 0, sizeof(uint32_t) * (nb ? nb : 1), ctx->stream));
-    HIP_TRY(hipMemsetAsync(ctx->word, 0, sizeof(unsigned long long) * (nb ? nb : 1), ctx->stream));
-    HIP_TRY(hipMemsetAsync(ctx->ticket, 0, sizeof(uint32_t), ctx->stream));
-    HIP_TRY(hipMemsetAsync(ctx->colors, 0xff, nc ? nc : 1, ctx->stream));
     LAUNCH(k_ep_count, blocksFor(nc), kThreads, ctx->active, ctx->cnt, ctx->contacts, ctx->moving, ctx->deg);
     if ((r = scan(ctx, ctx->deg, nullptr, nb, ctx->epStart, nullptr, &ctx->cnt->scanTotal))) { return r; }
     LAUNCH(k_ep_fill, blocksFor(nc), kThreads, ctx->active, ctx->cnt, ctx->contacts, ctx->moving, ctx->epStart,
            ctx->cursor, ctx->ep);
     LAUNCH(k_ep_rank, blocksFor(nb), kThreads, nb, ctx->deg, ctx->epStart, ctx->ep, ctx->rank);
     LAUNCH(k_color, blocksFor(nc, kThreads, 2048), kThreads, ctx->active, ctx->cnt, ctx->contacts, ctx->moving,
-           ctx->rank, ctx->word, ctx->colors, ctx->ticket, ctx->cnt);
+           ctx->rank, ctx->word, ctx->colors, &ctx->cnt->ticket, ctx->cnt);
 
     // row order and packing
     if ((r = rowOrder(ctx, ctx->active, &ctx->cnt->nActive, nc, ctx->rowsList, ctx->cnt->colorStart,
@@ -1168,47 +1621,30 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
         for (int i = 1; i < n; ++i) { needs = needs || (i == n / 2 ? hc.nFast : hc.nFastSolid) > 0; }
         if (needs) { return fail(ctx, TFS_E_CALLBACK, "fast contacts need a refresh callback"); }
     }
-    LAUNCH(k_pack, blocksFor(laned), kThreads, ctx->rowsList, laned, ctx->contacts, ctx->moving, ctx->rows, ctx->rowOf,
-           (const uint32_t*)nullptr);
-    LAUNCH(k_pack, blocksFor(impactLaned), kThreads, ctx->impactRowsList, impactLaned, ctx->contacts, ctx->moving,
-           ctx->impactRows, (uint32_t*)nullptr, ctx->rowOf);
+    if ((r = packRows(ctx, laned, impactLaned, true))) { return r; }
 
     // substeps: src/world.cpp:1658-1681
     for (int i = 0; i < n; ++i) {
         const bool half = i == n / 2;
         const uint32_t nAgain = half ? hc.nFast : hc.nFastSolid;
         if (i > 0 && nAgain > 0) {
-            if ((r = refresh(ctx, i, half ? ctx->fast : ctx->fastSolid, nAgain, (float)(n - i) * c.h, cb, user, hc))) {
+            if ((r = refresh(ctx, i, half, nAgain, (float)(n - i) * c.h, cb, user, hc))) {
                 return r;
             }
         }
         LAUNCH(k_integrate_velocities, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c);
-        if ((r = runRows(ctx, kWarm, ctx->rows, hc.colorStart, hc.colorPoints, c)) ||
+        if ((r = warmRows(ctx, laned, hc.pointsActive)) ||
             (r = runRest(ctx, kWarm, ctx->rowsList, laned, hc.nActive, c)) ||
-            (r = runRows(ctx, kBias, ctx->rows, hc.colorStart, hc.colorPoints, c)) ||
-            (r = runRest(ctx, kBias, ctx->rowsList, laned, hc.nActive, c))) {
+            (r = solvePass(ctx, kBias, hc, laned, impactLaned, c))) {
             return r;
-        }
-        if (hc.nImpact) {
-            if ((r = runRows(ctx, kBias, ctx->impactRows, hc.impactColorStart, hc.impactColorPoints, c)) ||
-                (r = runRest(ctx, kBias, ctx->impactRowsList, impactLaned, hc.nImpact, c))) {
-                return r;
-            }
         }
         LAUNCH(k_integrate_positions, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c);
-        if ((r = runRows(ctx, kRelax, ctx->rows, hc.colorStart, hc.colorPoints, c)) ||
-            (r = runRest(ctx, kRelax, ctx->rowsList, laned, hc.nActive, c))) {
-            return r;
-        }
-        if (hc.nImpact) {
-            if ((r = runRows(ctx, kRelax, ctx->impactRows, hc.impactColorStart, hc.impactColorPoints, c)) ||
-                (r = runRest(ctx, kRelax, ctx->impactRowsList, impactLaned, hc.nImpact, c))) {
-                return r;
-            }
-        }
+        if ((r = solvePass(ctx, kRelax, hc, laned, impactLaned, c))) { return r; }
     }
-    if ((r = runRows(ctx, kRestitute, ctx->rows, hc.colorStart, hc.colorPoints, c)) ||
-        (r = runRest(ctx, kRestitute, ctx->rowsList, laned, hc.nActive, c))) {
+    // restituteBatches + applyRestitution (src/world.cpp:1682-1683): nothing to do unless some active
+    // contact has restitution.
+    if (hc.anyRestitution && ((r = runRows(ctx, kRestitute, nullptr, hc.colorStart, hc.colorPoints, c)) ||
+                              (r = runRest(ctx, kRestitute, ctx->rowsList, laned, hc.nActive, c)))) {
         return r;
     }
     LAUNCH(k_limit_speeds, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c);
@@ -1221,20 +1657,20 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
            ctx->islandTime);
     LAUNCH(k_sleep_kinematic, blocksFor(nc), kThreads, ctx->contacts, nc, ctx->bodies);
     for (int it = 0; it < 64; ++it) {
-        HIP_TRY(hipMemsetAsync(&ctx->cnt->changed, 0, sizeof(uint32_t), ctx->stream));
-        LAUNCH(k_sleep_hook, blocksFor(nc), kThreads, ctx->contacts, nc, ctx->bodies, ctx->parent, &ctx->cnt->changed);
+        LAUNCH(k_sleep_hook, blocksFor(nc), kThreads, ctx->contacts, nc, ctx->bodies, ctx->parent, &ctx->cnt->changed[it]);
         LAUNCH(k_sleep_jump, blocksFor(nb), kThreads, ctx->parent, nb);
-        uint32_t changed = 0;
-        HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->changed, &ctx->cnt->changed, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->changed[it], &ctx->cnt->changed[it], sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        changed = ctx->hostCnt->changed;
-        if (!changed) { break; }
+        if (!ctx->hostCnt->changed[it]) { break; }
     }
     LAUNCH(k_island_time, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime);
     LAUNCH(k_sleep_apply, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime, s.time_to_sleep,
            ctx->motion, ctx->mass);
+    HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
     HIP_TRY(hipEventSynchronize(ctx->ev[2]));
+    if (ctx->hostCnt->error) { return fail(ctx, TFS_E_KERNEL, "a kernel's bounded wait ran out (error %u)", ctx->hostCnt->error); }
     HIP_TRY(hipEventElapsedTime(&ctx->solveMs, ctx->ev[0], ctx->ev[1]));
     HIP_TRY(hipEventElapsedTime(&ctx->sleepMs, ctx->ev[1], ctx->ev[2]));
     if ((r = collectProfile(ctx))) { return r; }
@@ -1289,7 +1725,25 @@ int tfs_create(int device, uint32_t maxBodies, uint32_t maxContacts, tfs_ctx** o
     }
     if (!r && hipMalloc((void**)&ctx->cnt, sizeof(DevCounts)) != hipSuccess) { r = TFS_E_DEVICE; }
     if (!r && hipHostMalloc((void**)&ctx->hostCnt, sizeof(DevCounts), hipHostMallocDefault) != hipSuccess) { r = TFS_E_DEVICE; }
-    if (!r && hipMalloc((void**)&ctx->ticket, sizeof(uint32_t)) != hipSuccess) { r = TFS_E_DEVICE; }
+    if (!r) {
+        int perCu = 0, cus = 0, coop = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, (const void*)k_pass<kRelax, false>, kPassThreads, 0) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+            hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device) != hipSuccess) {
+            r = TFS_E_DEVICE;
+        }
+        // Every block of a cooperative pass must be resident at once (half the occupancy limit keeps well
+        // inside that). The barrier's cost grows with the block count (a few microseconds at 32 blocks,
+        // about 14 at 256), so the pass uses at most 64 blocks.
+        ctx->maxPassBlocks = (uint32_t)std::max(1, std::min(perCu * cus / 2, 64));
+        ctx->usePass = coop != 0 && perCu > 0;
+        // Tuning knobs, read once: TWOFOLD_PASS=launch|coop, TWOFOLD_PASS_BLOCKS=n, TWOFOLD_AXES=packed|made.
+        if (const char* e = getenv("TWOFOLD_PASS")) { ctx->usePass = ctx->usePass && strcmp(e, "launch") != 0; }
+        if (const char* e = getenv("TWOFOLD_PASS_BLOCKS")) {
+            ctx->maxPassBlocks = (uint32_t)std::max(1, std::min(atoi(e), perCu * cus / 2));
+        }
+        if (const char* e = getenv("TWOFOLD_AXES")) { ctx->madeAxes = strcmp(e, "made") == 0; }
+    }
     if (!r) { r = allocBodies(ctx, maxBodies ? maxBodies : 1); }
     if (!r) { r = allocContacts(ctx, maxContacts ? maxContacts : 1); }
     if (r) {
@@ -1308,14 +1762,17 @@ void tfs_destroy(tfs_ctx* ctx) {
                      ctx->word, ctx->parent, ctx->islandTime, ctx->contacts, ctx->fresh, ctx->fActive, ctx->fFast,
                      ctx->fFastSolid, ctx->active, ctx->fast, ctx->fastSolid, ctx->impactFlag, ctx->impactPos,
                      ctx->impact, ctx->rowsList, ctx->impactRowsList, ctx->colors, ctx->ep, ctx->rank, ctx->rowOf,
-                     ctx->stamp, ctx->hist, ctx->tileSums, ctx->ticket, ctx->cnt, ctx->rows.hdr, ctx->rows.nrm,
+                     ctx->hist, ctx->tileSums, ctx->cnt, ctx->rows.hdr, ctx->rows.nrm,
                      ctx->rows.tg1, ctx->rows.tg2, ctx->rows.pa, ctx->rows.pb, ctx->rows.imp, ctx->rows.peak,
-                     ctx->impactRows.hdr, ctx->impactRows.nrm, ctx->impactRows.tg1, ctx->impactRows.tg2,
-                     ctx->impactRows.pa, ctx->impactRows.pb, ctx->impactRows.at, ctx->gatherC, ctx->gatherB };
+                     ctx->rows.ax, ctx->impactAt, ctx->bdeg, ctx->bstart, ctx->bcursor, ctx->bep, ctx->gatherC,
+                     ctx->gatherB };
     for (void* p : ptrs) {
         if (p) { (void)hipFree(p); }
     }
     if (ctx->hostCnt) { (void)hipHostFree(ctx->hostCnt); }
+    for (void* h : { (void*)ctx->pinC, (void*)ctx->pinB, (void*)ctx->pinAgain, (void*)ctx->pinFresh }) {
+        if (h) { (void)hipHostFree(h); }
+    }
     for (hipEvent_t e : ctx->ev) {
         if (e) { (void)hipEventDestroy(e); }
     }

@@ -235,24 +235,38 @@ struct Row {
 };
 
 // The row functions take the point count as a template argument, so every point index is known at
-// compile time and the points stay in registers; rows() below picks the instance.
+// compile time and the points stay in registers, and an axis source: ax(i, d) gives point i's axis
+// along the normal (d = 0), tangent1 (1) or tangent2 (2). The kernels read axes that the packing
+// kernel built once per step, as prepareBatch builds them (src/lanes.cpp:98-109); the serial path
+// builds them on the spot from the same masses and anchors (makeAxis), which gives the same bits.
+
+// The axes of a row built from the bodies' masses.
+struct MadeAxes {
+    const Mass& ma;
+    const Mass& mb;
+    const Row& r;
+    const RowPoint* p;
+    TF_HD Axis operator()(int i, int d) const {
+        return makeAxis(ma, mb, p[i].ra, p[i].rb, d == 0 ? r.n : d == 1 ? r.t1 : r.t2);
+    }
+};
 
 // warmStartLanes, one lane: src/lanes.cpp:195-217.
-template <int N>
-TF_HD void warmRowN(const Row& r, const RowPoint* p, const Mass& ma, const Mass& mb, Motion& a, Motion& b) {
+template <int N, class AX>
+TF_HD void warmRowN(const Row& r, const RowPoint* p, const AX& ax, Motion& a, Motion& b) {
     const bool moveA = r.flags & kMoveA, moveB = r.flags & kMoveB;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
-        push(a, b, makeAxis(ma, mb, p[i].ra, p[i].rb, r.n), p[i].ni, moveA, moveB);
-        push(a, b, makeAxis(ma, mb, p[i].ra, p[i].rb, r.t1), dot(p[i].ti, r.t1), moveA, moveB);
-        push(a, b, makeAxis(ma, mb, p[i].ra, p[i].rb, r.t2), dot(p[i].ti, r.t2), moveA, moveB);
+        push(a, b, ax(i, 0), p[i].ni, moveA, moveB);
+        push(a, b, ax(i, 1), dot(p[i].ti, r.t1), moveA, moveB);
+        push(a, b, ax(i, 2), dot(p[i].ti, r.t2), moveA, moveB);
     }
 }
 
 // The 3D contact solve: solveLanes for one lane (src/lanes.cpp:220-315), which is also the 3D part
 // of solveContacts (src/solver.cpp:370-417).
-template <int N, bool BIAS>
-TF_HD void solveRowN(const Row& r, RowPoint* p, const Mass& ma, const Mass& mb, Motion& a, Motion& b, const Ctx& c) {
+template <int N, bool BIAS, class AX>
+TF_HD void solveRowN(const Row& r, RowPoint* p, const AX& ax, Motion& a, Motion& b, const Ctx& c) {
     const bool moveA = r.flags & kMoveA, moveB = r.flags & kMoveB;
     const Soft soft = (moveA && moveB) ? c.contactSoft : c.staticSoft;
     const V3 dp = b.dp - a.dp;
@@ -268,7 +282,7 @@ TF_HD void solveRowN(const Row& r, RowPoint* p, const Mass& ma, const Mass& mb, 
             massScale = soft.massScale;
             impulseScale = soft.impulseScale;
         }
-        const Axis x = makeAxis(ma, mb, p[i].ra, p[i].rb, r.n);
+        const Axis x = ax(i, 0);
         const float vn = speedAlong(a, b, r.n, x);
         float impulse = -x.mass * massScale * (vn + bias) - impulseScale * p[i].ni;
         const float total = maxf(p[i].ni + impulse, 0.0f);
@@ -280,8 +294,8 @@ TF_HD void solveRowN(const Row& r, RowPoint* p, const Mass& ma, const Mass& mb, 
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         const int i = BIAS ? j : N - 1 - j;
-        const Axis x1 = makeAxis(ma, mb, p[i].ra, p[i].rb, r.t1);
-        const Axis x2 = makeAxis(ma, mb, p[i].ra, p[i].rb, r.t2);
+        const Axis x1 = ax(i, 1);
+        const Axis x2 = ax(i, 2);
         const float old1 = dot(p[i].ti, r.t1), old2 = dot(p[i].ti, r.t2);
         float f1 = old1 - x1.mass * speedAlong(a, b, r.t1, x1);
         float f2 = old2 - x2.mass * speedAlong(a, b, r.t2, x2);
@@ -299,15 +313,14 @@ TF_HD void solveRowN(const Row& r, RowPoint* p, const Mass& ma, const Mass& mb, 
 }
 
 // restituteLanes, one lane: src/lanes.cpp:318-352.
-template <int N>
-TF_HD void restituteRowN(const Row& r, RowPoint* p, const Mass& ma, const Mass& mb, Motion& a, Motion& b,
-                         const Ctx& c) {
+template <int N, class AX>
+TF_HD void restituteRowN(const Row& r, RowPoint* p, const AX& ax, Motion& a, Motion& b, const Ctx& c) {
     const bool moveA = r.flags & kMoveA, moveB = r.flags & kMoveB;
     const float e = r.restitution;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         const bool on = !(e == 0.0f) && !(p[i].rv > -c.restitutionThreshold) && !(p[i].nimax == 0.0f);
-        const Axis x = makeAxis(ma, mb, p[i].ra, p[i].rb, r.n);
+        const Axis x = ax(i, 0);
         const float vn = speedAlong(a, b, r.n, x);
         float impulse = -x.mass * (vn + e * p[i].rv);
         const float total = maxf(p[i].ni + impulse, 0.0f);
@@ -322,25 +335,25 @@ TF_HD void restituteRowN(const Row& r, RowPoint* p, const Mass& ma, const Mass& 
 
 enum RowMode { kWarm = 0, kBias = 1, kRelax = 2, kRestitute = 3 };
 
-template <int MODE, int N>
-TF_HD void rowN(const Row& r, RowPoint* p, const Mass& ma, const Mass& mb, Motion& a, Motion& b, const Ctx& c) {
+template <int MODE, int N, class AX>
+TF_HD void rowN(const Row& r, RowPoint* p, const AX& ax, Motion& a, Motion& b, const Ctx& c) {
     if (MODE == kWarm) {
-        warmRowN<N>(r, p, ma, mb, a, b);
+        warmRowN<N>(r, p, ax, a, b);
     } else if (MODE == kRestitute) {
-        restituteRowN<N>(r, p, ma, mb, a, b, c);
+        restituteRowN<N>(r, p, ax, a, b, c);
     } else {
-        solveRowN<N, MODE == kBias>(r, p, ma, mb, a, b, c);
+        solveRowN<N, MODE == kBias>(r, p, ax, a, b, c);
     }
 }
 
 // One row in the given mode, for the row's point count.
-template <int MODE>
-TF_HD void rows(const Row& r, RowPoint* p, const Mass& ma, const Mass& mb, Motion& a, Motion& b, const Ctx& c) {
+template <int MODE, class AX>
+TF_HD void rows(const Row& r, RowPoint* p, const AX& ax, Motion& a, Motion& b, const Ctx& c) {
     switch (r.flags & kCountMask) {
-        case 1: rowN<MODE, 1>(r, p, ma, mb, a, b, c); break;
-        case 2: rowN<MODE, 2>(r, p, ma, mb, a, b, c); break;
-        case 3: rowN<MODE, 3>(r, p, ma, mb, a, b, c); break;
-        case 4: rowN<MODE, 4>(r, p, ma, mb, a, b, c); break;
+        case 1: rowN<MODE, 1>(r, p, ax, a, b, c); break;
+        case 2: rowN<MODE, 2>(r, p, ax, a, b, c); break;
+        case 3: rowN<MODE, 3>(r, p, ax, a, b, c); break;
+        case 4: rowN<MODE, 4>(r, p, ax, a, b, c); break;
         default: break;
     }
 }
@@ -658,10 +671,11 @@ TF_HD void solveRest(tfs_contact& c, const BodyView& va, const BodyView& vb, con
     RowPoint p[4];
     pointsOf(c, p);
     const Row row = rowOf(c, !anchoredView(va), !anchoredView(vb));
+    const MadeAxes ax{ ma, mb, row, p };
     if (useBias) {
-        rows<kBias>(row, p, ma, mb, a, b, ctx);
+        rows<kBias>(row, p, ax, a, b, ctx);
     } else {
-        rows<kRelax>(row, p, ma, mb, a, b, ctx);
+        rows<kRelax>(row, p, ax, a, b, ctx);
     }
     storeImpulses(c, p);
     if (c.rolling_resistance > 0.0f) { resistTurning(c, ma, mb, a, b); }
