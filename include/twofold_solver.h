@@ -221,8 +221,10 @@ int tfs_last_step_ms(tfs_ctx* ctx, float* solve_ms, float* sleep_ms);
  * kernels is bracketed by HIP events on the context's stream, and per family (0 warm start, 1 solve:
  * biased and relax passes including the impact repeats, 2 restitution) the library sums the launch
  * count, the event time in milliseconds and the algorithmic bytes of the rows launched (per contact
- * with n points: warm start 224 + 244n, solve 224 + 296n, restitution 224 + 96n). Turning it on or
- * off clears the sums.
+ * with n points: warm start 224 + 244n, solve 224 + 296n, restitution 224 + 96n). Two more families
+ * time the host side of each refresh with the host clock: 3 the wait for the contacts collided again
+ * to reach host memory, 4 the refresh callback itself (its "bytes" sum counts the pairs collided
+ * again). Turning it on clears the sums; turning it off keeps them for reading.
  */
 int tfs_set_profiling(tfs_ctx* ctx, int on);
 int tfs_read_profile(tfs_ctx* ctx, int family, uint64_t* launches, double* ms, double* bytes);

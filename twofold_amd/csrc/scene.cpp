@@ -10,9 +10,12 @@
 // anchors lifted onto each surface for separated points (src/world.cpp:536-540), impulses carried
 // to the nearest new point (src/world.cpp:102-120), friction and restitution mixed as
 // src/world.cpp:39-40 - but not its bits: box-box uses separating axes and face clipping of its own.
+#include <omp.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <unordered_map>
 #include <vector>
@@ -326,6 +329,7 @@ void boxBox(const Shape& sa, const Pose& a, const Shape& sb, const Pose& b, floa
 
 struct World {
     tfs_settings settings{};
+    int threads = 1;  // host threads of the narrowphase loops
     std::vector<tfs_body> bodies;
     std::vector<Shape> shapes;  // one per body, at the body's origin
     std::vector<uint32_t> planes, statics, movers;
@@ -539,8 +543,13 @@ void massProperties(tfs_body& b, const Shape& s) {
 
 extern "C" {
 
+// Half the OpenMP threads by default (TWOFOLD_HOST_THREADS overrides): a process held to a CPU quota
+// that its spinning workers use up is stopped for the rest of the quota period, which stalls a solve
+// step waiting on its refresh callback for milliseconds.
 void* tfw_create(const tfs_settings* s) {
     World* w = new World();
+    w->threads = std::max(1, omp_get_max_threads() / 2);
+    if (const char* e = std::getenv("TWOFOLD_HOST_THREADS")) { w->threads = std::max(1, std::atoi(e)); }
     if (s) {
         w->settings = *s;
     } else {
@@ -639,7 +648,7 @@ uint32_t tfw_collide(void* h, tfs_body* bodies, uint32_t nb, float dt, const tfs
     std::vector<tfs_contact> found(pairs.size());
     std::vector<uint8_t> hit(pairs.size(), 0);
     const float matchSq = w.settings.speculative * w.settings.speculative;
-#pragma omp parallel for schedule(dynamic, 256)
+#pragma omp parallel for schedule(dynamic, 256) num_threads(w.threads)
     for (long k = 0; k < (long)pairs.size(); ++k) {
         const uint32_t ia = (uint32_t)(pairs[(size_t)k] >> 32), ib = (uint32_t)pairs[(size_t)k];
         Manifold m;
@@ -676,7 +685,7 @@ uint32_t tfw_collide(void* h, tfs_body* bodies, uint32_t nb, float dt, const tfs
 int tfw_refresh(void* h, int, float remaining, const uint32_t* again, uint32_t n, const tfs_body* bodies, uint32_t,
                 const tfs_contact* contacts, uint32_t, tfs_contact* fresh) {
     World& w = *(World*)h;
-#pragma omp parallel for schedule(dynamic, 64)
+#pragma omp parallel for schedule(dynamic, 64) num_threads(w.threads)
     for (long k = 0; k < (long)n; ++k) {
         const tfs_contact& c = contacts[again[k]];
         Manifold m;

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -34,9 +35,11 @@ using namespace tf;
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kTile = 1024;            // scan tile: 256 threads x 4 items
+constexpr int kScanSlots = 16;         // scans in one step
+constexpr int kProfileFamilies = 5;
 constexpr int kKeys = 5 * TFS_COLORS + 1; // row-order buckets: colour x point count, then the rest
 constexpr uint32_t kSpinLimit = 1u << 26;
+constexpr int kIslandRounds = 64;      // hook-and-jump rounds of the island pass before it gives up
 
 struct DevCounts {
     uint32_t nActive, nFast, nFastSolid, nImpact;
@@ -45,9 +48,10 @@ struct DevCounts {
     uint32_t error;
     uint32_t anyRestitution;
     uint32_t ticket;
+    uint32_t scanTicket[kScanSlots];
     uint32_t pad_;
     unsigned long long barrier;
-    uint32_t changed[64];
+    uint32_t changed[kIslandRounds];
     uint32_t colorStart[TFS_COLORS + 1];
     uint32_t impactColorStart[TFS_COLORS + 1];
     uint32_t colorPoints[TFS_COLORS + 1];
@@ -95,73 +99,153 @@ __device__ __forceinline__ void storeMass(float4* m, uint32_t i, const tfs_body&
 __device__ __forceinline__ uint32_t gid() { return blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ uint32_t gstride() { return gridDim.x * blockDim.x; }
 
-// ---- scans and compaction ---------------------------------------------------------------------
-
-__device__ uint32_t blockExclusive(uint32_t v, uint32_t* lds, uint32_t& total) {
-    const uint32_t t = threadIdx.x;
-    lds[t] = v;
-    __syncthreads();
-    for (uint32_t off = 1; off < kThreads; off <<= 1) {
-        const uint32_t x = t >= off ? lds[t - off] : 0u;
-        __syncthreads();
-        lds[t] += x;
-        __syncthreads();
-    }
-    total = lds[kThreads - 1];
-    const uint32_t incl = lds[t];
-    __syncthreads();
-    return incl - v;
+// Adds every lane's v to *p with one atomic per wave (a counter that every thread of a launch adds to
+// would otherwise serialize them). Called by all lanes of the wave.
+__device__ __forceinline__ void waveAdd(uint32_t* p, uint32_t v) {
+    for (int off = 32; off > 0; off >>= 1) { v += (uint32_t)__shfl_xor((int)v, off); }
+    if ((threadIdx.x % 64) == 0 && v != 0) { atomicAdd(p, v); }
 }
+
+// ---- scans and compaction ---------------------------------------------------------------------
+//
+// One launch per scan, decoupled look-back: each block takes a tile of 4096 items in ticket order,
+// scans it in LDS, publishes its tile total, then adds up its predecessors' published totals (or
+// the first inclusive prefix it meets) and publishes its own inclusive prefix. A tile only waits on
+// tiles that took earlier tickets, which publish their totals without waiting on anything, so every
+// wait ends. Status words carry the launch's epoch, so old words never need clearing.
 
 __device__ __forceinline__ uint32_t countOf(const uint32_t* nPtr, uint32_t n) { return nPtr ? *nPtr : n; }
 
-// Sum of each 1024-item tile.
-__global__ void k_scan_tiles(const uint32_t* in, const uint32_t* nPtr, uint32_t nConst, uint32_t* tileSums) {
-    __shared__ uint32_t lds[kThreads];
-    const uint32_t n = countOf(nPtr, nConst);
-    const uint32_t base = blockIdx.x * kTile + threadIdx.x * 4;
-    uint32_t s = 0;
-    for (int k = 0; k < 4; ++k) { s += base + k < n ? in[base + k] : 0u; }
-    uint32_t total;
-    blockExclusive(s, lds, total);
-    if (threadIdx.x == 0) { tileSums[blockIdx.x] = total; }
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr uint32_t kScanTile = kScanThreads * kScanItems;
+constexpr uint32_t kFlagTotal = 1, kFlagPrefix = 2;
+constexpr uint32_t kEpochLimit = 1u << 30;
+
+struct ScanJob {
+    const uint32_t* in;
+    const uint32_t* nPtr;  // item count on the device, or null for nConst
+    uint32_t nConst;
+    uint32_t* out;   // exclusive prefix (may be `in`), or null
+    uint32_t* list;  // i written to list[prefix] where in[i] != 0, or null
+    uint32_t* total;
+};
+struct ScanJobs {
+    ScanJob job[3];
+    uint32_t tiles;  // tiles per job
+};
+
+__device__ __forceinline__ unsigned long long statusWord(uint32_t epoch, uint32_t flag, uint32_t value) {
+    return ((unsigned long long)((epoch << 2) | flag) << 32) | value;
 }
 
-// Exclusive scan of the tile sums in place, one block; the grand total goes to *total.
-__global__ void k_scan_sums(uint32_t* tileSums, uint32_t nTiles, uint32_t* total) {
-    __shared__ uint32_t lds[kThreads];
-    uint32_t carry = 0;
-    for (uint32_t base = 0; base < nTiles; base += kThreads) {
-        const uint32_t i = base + threadIdx.x;
-        const uint32_t v = i < nTiles ? tileSums[i] : 0u;
-        uint32_t sum;
-        const uint32_t ex = blockExclusive(v, lds, sum);
-        if (i < nTiles) { tileSums[i] = carry + ex; }
-        carry += sum;
-    }
-    if (threadIdx.x == 0 && total) { *total = carry; }
-}
+// LDS index of item i of a tile, padded so that a thread's 16 consecutive items fall in different banks.
+__device__ __forceinline__ uint32_t padded(uint32_t i) { return i + i / kScanItems; }
 
-// out[i] = exclusive prefix of in; with `list` set, writes i to list[prefix] where in[i] != 0.
-__global__ void k_scan_apply(const uint32_t* in, const uint32_t* nPtr, uint32_t nConst, const uint32_t* tileOffsets,
-                             uint32_t* out, uint32_t* list) {
-    __shared__ uint32_t lds[kThreads];
-    const uint32_t n = countOf(nPtr, nConst);
-    const uint32_t base = blockIdx.x * kTile + threadIdx.x * 4;
-    uint32_t v[4], s = 0;
-    for (int k = 0; k < 4; ++k) {
-        v[k] = base + k < n ? in[base + k] : 0u;
-        s += v[k];
+__global__ void __launch_bounds__(kScanThreads) k_scan(ScanJobs jobs, unsigned long long* status, uint32_t epoch,
+                                                       uint32_t* ticket, DevCounts* cnt) {
+    __shared__ uint32_t lds[kScanTile + kScanTile / kScanItems];
+    __shared__ uint32_t waveSum[kScanThreads / 64];
+    __shared__ uint32_t order, tileOffset;
+    const uint32_t t = threadIdx.x, lane = t % 64, wave = t / 64;
+    if (t == 0) { order = atomicAdd(ticket, 1u); }
+    __syncthreads();
+    const uint32_t jobNo = order / jobs.tiles, tile = order % jobs.tiles;
+    const ScanJob J = jobs.job[jobNo];
+    const uint32_t n = countOf(J.nPtr, J.nConst);
+    const uint32_t base = tile * kScanTile;
+    unsigned long long* st = status + (size_t)jobNo * jobs.tiles;
+
+    uint32_t v[kScanItems];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint32_t i = base + (uint32_t)k * kScanThreads + t;
+        v[k] = i < n ? J.in[i] : 0u;
+        lds[padded((uint32_t)k * kScanThreads + t)] = v[k];
     }
-    uint32_t total;
-    uint32_t run = blockExclusive(s, lds, total) + tileOffsets[blockIdx.x];
-    for (int k = 0; k < 4; ++k) {
-        if (base + k < n) {
-            if (out) { out[base + k] = run; }
-            if (list && v[k]) { list[run] = base + k; }
+    __syncthreads();
+    // each thread's 16 consecutive items: exclusive prefixes in place, and their sum
+    uint32_t run = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) {
+        const uint32_t at = padded(t * kScanItems + (uint32_t)j);
+        const uint32_t x = lds[at];
+        lds[at] = run;
+        run += x;
+    }
+    uint32_t incl = run;
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)incl, off);
+        if (lane >= off) { incl += y; }
+    }
+    if (lane == 63) { waveSum[wave] = incl; }
+    __syncthreads();
+    uint32_t before = 0, tileTotal = 0;
+    for (uint32_t w = 0; w < kScanThreads / 64; ++w) {
+        if (w < wave) { before += waveSum[w]; }
+        tileTotal += waveSum[w];
+    }
+    const uint32_t threadBefore = before + incl - run;
+
+    if (wave == 0) {
+        uint32_t offset = 0;
+        if (tile == 0) {
+            if (lane == 0) { __hip_atomic_store(&st[0], statusWord(epoch, kFlagPrefix, tileTotal), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+        } else {
+            if (lane == 0) { __hip_atomic_store(&st[tile], statusWord(epoch, kFlagTotal, tileTotal), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+            int look = (int)tile - 1;  // lane l reads tile look - l
+            uint32_t spins = 0;
+            bool done = false;
+            while (!done) {
+                const int j = look - (int)lane;
+                uint32_t flag = kFlagPrefix, value = 0;  // before tile 0: an empty prefix
+                if (j >= 0) {
+                    const unsigned long long w = __hip_atomic_load(&st[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    const uint32_t hi = (uint32_t)(w >> 32);
+                    flag = (hi >> 2) == epoch ? (hi & 3u) : 0u;
+                    value = (uint32_t)w;
+                }
+                if (__ballot(flag == 0u) != 0ull) {
+                    if (++spins > kSpinLimit) {
+                        if (lane == 0) { atomicOr(&cnt->error, 4u); }
+                        done = true;
+                    } else {
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    continue;
+                }
+                const unsigned long long prefixes = __ballot(flag == kFlagPrefix);
+                const int stop = prefixes ? __ffsll((long long)prefixes) - 1 : 63;
+                uint32_t add = (int)lane <= stop ? value : 0u;
+                for (int off = 32; off > 0; off >>= 1) { add += (uint32_t)__shfl_xor((int)add, off); }
+                offset += add;
+                if (prefixes) {
+                    done = true;
+                } else {
+                    look -= 64;
+                }
+            }
+            if (lane == 0) {
+                __hip_atomic_store(&st[tile], statusWord(epoch, kFlagPrefix, offset + tileTotal), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
-        run += v[k];
+        if (lane == 0) { tileOffset = offset; }
     }
+    __syncthreads();
+    const uint32_t add = tileOffset + threadBefore;
+#pragma unroll
+    for (int j = 0; j < kScanItems; ++j) { lds[padded(t * kScanItems + (uint32_t)j)] += add; }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const uint32_t i = base + (uint32_t)k * kScanThreads + t;
+        if (i < n) {
+            const uint32_t pre = lds[padded((uint32_t)k * kScanThreads + t)];
+            if (J.out) { J.out[i] = pre; }
+            if (J.list && v[k]) { J.list[pre] = i; }
+        }
+    }
+    if (t == 0 && tile == jobs.tiles - 1 && J.total) { *J.total = tileOffset + tileTotal; }
 }
 
 // ---- step state and lists ------------------------------------------------------------------------
@@ -229,6 +313,7 @@ __global__ void k_prepare(const uint32_t* active, const DevCounts* cnt, tfs_cont
                           const float4* motion, const float4* mass, float threshold, uint8_t* struck,
                           uint32_t* points, uint32_t* anyRestitution) {
     const uint32_t n = cnt->nActive;
+    uint32_t sum = 0, bounce = 0;
     for (uint32_t k = gid(); k < n; k += gstride()) {
         const uint32_t ci = active[k];
         tfs_contact c = contacts[ci];
@@ -237,8 +322,8 @@ __global__ void k_prepare(const uint32_t* active, const DevCounts* cnt, tfs_cont
         prepareContact(viewOf(ta), viewOf(tb), ta, tb, loadMotion(motion, c.body_a), loadMotion(motion, c.body_b),
                        loadMass(mass, c.body_a), loadMass(mass, c.body_b), c);
         contacts[ci] = c;
-        atomicAdd(points, (uint32_t)c.count);
-        if (c.restitution != 0.0f && c.count > 0) { anyRestitution[0] = 1u; }
+        sum += (uint32_t)c.count;
+        if (c.restitution != 0.0f && c.count > 0) { bounce = 1u; }
         for (int i = 0; i < c.count; ++i) {
             if (c.points[i].relative_velocity >= -threshold) { continue; }
             if (ta.type == TFS_DYNAMIC) { struck[c.body_a] = 1; }
@@ -246,6 +331,8 @@ __global__ void k_prepare(const uint32_t* active, const DevCounts* cnt, tfs_cont
             break;
         }
     }
+    waveAdd(points, sum);
+    if (__ballot(bounce != 0u) != 0ull && (threadIdx.x % 64) == 0) { anyRestitution[0] = 1u; }
 }
 
 // impactContacts' list (src/world.cpp:1042-1048): active contacts that touch a struck body.
@@ -266,7 +353,9 @@ __global__ void k_gather_list(const uint32_t* positions, const uint32_t* nPtr, c
 
 __global__ void k_points_of(const uint32_t* list, const uint32_t* nPtr, const tfs_contact* contacts, uint32_t* points) {
     const uint32_t n = *nPtr;
-    for (uint32_t k = gid(); k < n; k += gstride()) { atomicAdd(points, (uint32_t)contacts[list[k]].count); }
+    uint32_t sum = 0;
+    for (uint32_t k = gid(); k < n; k += gstride()) { sum += (uint32_t)contacts[list[k]].count; }
+    waveAdd(points, sum);
 }
 
 // ---- colouring ----------------------------------------------------------------------------------
@@ -405,8 +494,7 @@ __global__ void k_row_hist(const uint32_t* list, const uint32_t* nPtr, const tfs
 
 __global__ void __launch_bounds__(kThreads) k_row_scatter(const uint32_t* list, const uint32_t* nPtr,
                                                           const tfs_contact* contacts, const uint8_t* colors,
-                                                          const uint32_t* offsets, uint32_t nBlocks, uint32_t* out,
-                                                          uint32_t* points) {
+                                                          const uint32_t* offsets, uint32_t nBlocks, uint32_t* out) {
     // Rank of each item among the earlier items of the block with the same key: within the wave from
     // ballots on the key's bits, across the block's waves from per-wave key counts.
     __shared__ uint32_t waveCount[kThreads / 64][kKeys];
@@ -434,14 +522,22 @@ __global__ void __launch_bounds__(kThreads) k_row_scatter(const uint32_t* list, 
         uint32_t before = inWave;
         for (uint32_t w = 0; w < wave; ++w) { before += waveCount[w][key]; }
         out[offsets[key * nBlocks + blockIdx.x] + before] = ci;
-        atomicAdd(&points[colors[ci] < TFS_COLORS ? colors[ci] : TFS_COLORS], (uint32_t)contacts[ci].count);
     }
 }
 
-// Start of each colour's rows; entry 32 is the start of the rest.
-__global__ void k_row_starts(const uint32_t* offsets, uint32_t nBlocks, uint32_t* starts) {
+// Start of each colour's rows (entry 32 is the start of the rest) and each colour's point count: a
+// colour's five keys hold its rows with 4, 3, 2, 1 and 0 points.
+__global__ void k_row_starts(const uint32_t* offsets, uint32_t nBlocks, uint32_t* starts, uint32_t* points) {
     const uint32_t c = threadIdx.x;
     if (c <= TFS_COLORS) { starts[c] = offsets[(5u * c) * nBlocks]; }
+    if (c < TFS_COLORS) {
+        uint32_t sum = 0;
+        for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t rowsWith = offsets[(5u * c + j + 1) * nBlocks] - offsets[(5u * c + j) * nBlocks];
+            sum += (4u - j) * rowsWith;
+        }
+        points[c] = sum;
+    }
 }
 
 // ---- rows --------------------------------------------------------------------------------------
@@ -498,20 +594,30 @@ struct LoadedAxes {
 
 // Packs the coloured contacts into rows in row order and builds their point axes from the bodies'
 // current masses, as prepareLanes does (src/lanes.cpp:66-110, 409-411).
-__global__ void k_pack(const uint32_t* list, uint32_t n, const tfs_contact* contacts, const uint8_t* moving,
-                       const float4* mass, Rows rows, uint32_t* rowOf) {
+// The contact's points are read before anything is written, so one row waits for memory once.
+__global__ void k_pack(const uint32_t* __restrict__ list, uint32_t n, const tfs_contact* __restrict__ contacts,
+                       const uint8_t* __restrict__ moving, const float4* __restrict__ mass, Rows rows,
+                       uint32_t* __restrict__ rowOf) {
     for (uint32_t r = gid(); r < n; r += gstride()) {
         const uint32_t ci = list[r];
         const tfs_contact& c = contacts[ci];
-        const uint32_t flags = (uint32_t)c.count | (moving[c.body_a] ? kMoveA : 0u) | (moving[c.body_b] ? kMoveB : 0u);
+        const int count = c.count;
+        tfs_point pt[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k < count) { pt[k] = c.points[k]; }
+        }
+        const uint32_t flags = (uint32_t)count | (moving[c.body_a] ? kMoveA : 0u) | (moving[c.body_b] ? kMoveB : 0u);
+        const Mass ma = loadMass(mass, c.body_a), mb = loadMass(mass, c.body_b);
+        const V3 dir[3] = { ld3(c.normal), ld3(c.tangent1), ld3(c.tangent2) };
         rows.hdr[r] = make_uint4(c.body_a, c.body_b, flags, ci);
         rows.nrm[r] = make_float4(c.normal[0], c.normal[1], c.normal[2], c.friction);
         rows.tg1[r] = make_float4(c.tangent1[0], c.tangent1[1], c.tangent1[2], c.restitution);
         rows.tg2[r] = make_float4(c.tangent2[0], c.tangent2[1], c.tangent2[2], 0.0f);
-        const Mass ma = loadMass(mass, c.body_a), mb = loadMass(mass, c.body_b);
-        const V3 dir[3] = { ld3(c.normal), ld3(c.tangent1), ld3(c.tangent2) };
-        for (int k = 0; k < c.count; ++k) {
-            const tfs_point& p = c.points[k];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k >= count) { break; }
+            const tfs_point& p = pt[k];
             const uint32_t j = (uint32_t)k * rows.cap + r;
             rows.pa[j] = make_float4(p.anchor_a[0], p.anchor_a[1], p.anchor_a[2], p.adjusted_separation);
             rows.pb[j] = make_float4(p.anchor_b[0], p.anchor_b[1], p.anchor_b[2], p.relative_velocity);
@@ -523,9 +629,14 @@ __global__ void k_pack(const uint32_t* list, uint32_t n, const tfs_contact* cont
     }
 }
 
-// The impact rows' places among the active rows.
-__global__ void k_impact_rows(const uint32_t* list, uint32_t n, const uint32_t* rowOf, uint32_t* at) {
-    for (uint32_t r = gid(); r < n; r += gstride()) { at[r] = rowOf[list[r]]; }
+// The impact rows' headers: their active row's header with the active row index in place of the
+// contact index.
+__global__ void k_impact_rows(const uint32_t* list, uint32_t n, const uint32_t* rowOf, const uint4* hdr, uint4* impactHdr) {
+    for (uint32_t i = gid(); i < n; i += gstride()) {
+        const uint32_t r = rowOf[list[i]];
+        const uint4 h = hdr[r];
+        impactHdr[i] = make_uint4(h.x, h.y, h.z, r);
+    }
 }
 
 // Impulses of the coloured rows back into the contacts.
@@ -612,22 +723,20 @@ struct RowMadeAxes {
     }
 };
 
-// One row. Rows of one colour move no body in common (src/internal.h:607-609), so within a colour
-// each row's thread owns the row's two moving bodies. `at` maps impact rows to active rows. MADE
-// builds the point axes on the spot from the masses instead of reading the packed ones.
+// One row, r, with header h (bodies, flags). Rows of one colour move no body in common
+// (src/internal.h:607-609), so within a colour each row's thread owns the row's two moving bodies.
+// MADE builds the point axes on the spot from the masses instead of reading the packed ones.
 template <int MODE, bool COHERENT, bool MADE>
-__device__ __forceinline__ void rowStep(const Rows& rows, const uint32_t* at, uint32_t i, float4* motion,
-                                        const float4* mass, const Ctx& ctx) {
+__device__ __forceinline__ void rowStep(const Rows& rows, uint32_t r, uint4 h, float4* motion, const float4* mass,
+                                        const Ctx& ctx) {
     using X = Access<COHERENT>;
-    const uint32_t r = at ? at[i] : i;
-    const uint4 h = rows.hdr[r];
     const int count = (int)(h.z & kCountMask);
     if (count == 0) { return; }
     const float4 n = rows.nrm[r], t1 = rows.tg1[r], t2 = rows.tg2[r];
     const Row row{ h.z, vec(n.x, n.y, n.z), vec(t1.x, t1.y, t1.z), vec(t2.x, t2.y, t2.z), n.w, t1.w };
     if (MODE == kRestitute && row.restitution == 0.0f) { return; }
     // Every load the row needs is issued here, before any arithmetic, so the row waits for memory
-    // twice (its header, then everything else) rather than once per axis.
+    // once rather than once per axis.
     RowPoint p[4];
     float4 axv[4][MADE ? 1 : 15];
 #pragma unroll
@@ -680,12 +789,14 @@ __device__ __forceinline__ void rowStep(const Rows& rows, const uint32_t* at, ui
     }
 }
 
-// One colour per launch.
+// One colour per launch; impactHdr is set for impact rows.
 template <int MODE, bool MADE>
-__global__ void __launch_bounds__(kThreads) k_rows(Rows rows, const uint32_t* at, uint32_t begin, uint32_t end,
+__global__ void __launch_bounds__(kThreads) k_rows(Rows rows, const uint4* impactHdr, uint32_t begin, uint32_t end,
                                                    float4* motion, const float4* mass, Ctx ctx) {
     const uint32_t i = begin + gid();
-    if (i < end) { rowStep<MODE, false, MADE>(rows, at, i, motion, mass, ctx); }
+    if (i >= end) { return; }
+    const uint4 h = impactHdr ? impactHdr[i] : rows.hdr[i];
+    rowStep<MODE, false, MADE>(rows, impactHdr ? h.w : i, h, motion, mass, ctx);
 }
 
 // A whole pass in one cooperative launch: the colours of the active rows, then those of the impact
@@ -720,15 +831,26 @@ __device__ __forceinline__ void gridSync(unsigned long long* counter, unsigned l
     __syncthreads();
 }
 
+// Headers do not change during a pass, so each thread loads the header of its first row in the
+// next segment before the barrier; after the barrier every other load of that row goes out at once.
 template <int MODE, bool MADE>
-__global__ void __launch_bounds__(kPassThreads) k_pass(Rows rows, const uint32_t* at, PassPlan plan, float4* motion,
-                                                       const float4* mass, Ctx ctx, unsigned long long* counter,
-                                                       unsigned long long base, DevCounts* cnt) {
+__global__ void __launch_bounds__(kPassThreads) k_pass(Rows rows, const uint4* impactHdr, PassPlan plan,
+                                                       float4* motion, const float4* mass, Ctx ctx,
+                                                       unsigned long long* counter, unsigned long long base,
+                                                       DevCounts* cnt) {
     unsigned long long phase = base;
+    const uint32_t g = gid(), stride = gstride();
+    auto header = [&](uint32_t s, uint32_t i) { return s >= plan.impactFrom ? impactHdr[i] : rows.hdr[i]; };
+    uint4 next = make_uint4(0u, 0u, 0u, 0u);
+    if (plan.n > 0 && plan.begin[0] + g < plan.end[0]) { next = header(0, plan.begin[0] + g); }
     for (uint32_t s = 0; s < plan.n; ++s) {
-        const uint32_t* map = s >= plan.impactFrom ? at : nullptr;
-        for (uint32_t i = plan.begin[s] + gid(); i < plan.end[s]; i += gstride()) {
-            rowStep<MODE, true, MADE>(rows, map, i, motion, mass, ctx);
+        const bool impact = s >= plan.impactFrom;
+        const uint32_t first = plan.begin[s] + g;
+        const uint4 firstHdr = next;
+        if (s + 1 < plan.n && plan.begin[s + 1] + g < plan.end[s + 1]) { next = header(s + 1, plan.begin[s + 1] + g); }
+        for (uint32_t i = first; i < plan.end[s]; i += stride) {
+            const uint4 h = i == first ? firstHdr : header(s, i);
+            rowStep<MODE, true, MADE>(rows, impact ? h.w : i, h, motion, mass, ctx);
         }
         if (s + 1 < plan.n) { gridSync(counter, phase, cnt); }
     }
@@ -775,37 +897,48 @@ __global__ void k_bep_sort(uint32_t nb, const uint32_t* deg, const uint32_t* sta
     }
 }
 
-__global__ void k_warm_bodies(uint32_t nb, const uint32_t* deg, const uint32_t* start, const uint32_t* ep, Rows rows,
-                              float4* motion) {
+// Each row's loads go out together (all its points, this body's side only), and the next row's
+// index is read while the current row is applied.
+__global__ void k_warm_bodies(uint32_t nb, const uint32_t* __restrict__ deg, const uint32_t* __restrict__ start,
+                              const uint32_t* __restrict__ ep, Rows rows, float4* __restrict__ motion) {
     for (uint32_t x = gid(); x < nb; x += gstride()) {
         const uint32_t d = deg[x];
         if (d == 0) { continue; }
+        const uint32_t* list = ep + start[x];
         Motion m = loadMotion(motion, x);
+        uint32_t e = list[0];
         for (uint32_t k = 0; k < d; ++k) {
-            const uint32_t e = ep[start[x] + k];
             const uint32_t r = e >> 1;
             const bool sideB = e & 1u;
+            if (k + 1 < d) { e = list[k + 1]; }
             const int count = (int)(rows.hdr[r].z & kCountMask);
             const float4 t1 = rows.tg1[r], t2 = rows.tg2[r];
-            const V3 dir1 = vec(t1.x, t1.y, t1.z), dir2 = vec(t2.x, t2.y, t2.z);
-            for (int i = 0; i < count; ++i) {
-                const float4 im = rows.imp[(uint32_t)i * rows.cap + r];
-                // this body's side of the point's three axes
-                float4 q[3][2];
-                for (int a = 0; a < 3; ++a) {
-                    const float4* base = rows.ax + (size_t)((i * 3 + a) * 5 + (sideB ? 1 : 0)) * rows.cap + r;
-                    q[a][0] = base[0];
-                    q[a][1] = base[rows.cap];
+            float4 im[4], q[4][3][2];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (i < count) {
+                    im[i] = rows.imp[(uint32_t)i * rows.cap + r];
+                    for (int a = 0; a < 3; ++a) {
+                        const float4* base = rows.ax + (size_t)((i * 3 + a) * 5 + (sideB ? 1 : 0)) * rows.cap + r;
+                        q[i][a][0] = base[0];
+                        q[i][a][1] = base[rows.cap];
+                    }
                 }
-                const V3 ti = vec(im.y, im.z, im.w);
-                const float lambda[3] = { im.x, dot(ti, dir1), dot(ti, dir2) };
+            }
+            const V3 dir1 = vec(t1.x, t1.y, t1.z), dir2 = vec(t2.x, t2.y, t2.z);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (i >= count) { break; }
+                const V3 ti = vec(im[i].y, im[i].z, im[i].w);
+                const float lambda[3] = { im[i].x, dot(ti, dir1), dot(ti, dir2) };
                 for (int a = 0; a < 3; ++a) {
+                    const float4 q0 = q[i][a][0], q1 = q[i][a][1];
                     if (sideB) {
-                        m.v = m.v + vec(q[a][0].z, q[a][0].w, q[a][1].x) * lambda[a];
-                        m.w = m.w + vec(q[a][1].y, q[a][1].z, q[a][1].w) * lambda[a];
+                        m.v = m.v + vec(q0.z, q0.w, q1.x) * lambda[a];
+                        m.w = m.w + vec(q1.y, q1.z, q1.w) * lambda[a];
                     } else {
-                        m.v = m.v - vec(q[a][0].x, q[a][0].y, q[a][0].z) * lambda[a];
-                        m.w = m.w - vec(q[a][0].w, q[a][1].x, q[a][1].y) * lambda[a];
+                        m.v = m.v - vec(q0.x, q0.y, q0.z) * lambda[a];
+                        m.w = m.w - vec(q0.w, q1.x, q1.y) * lambda[a];
                     }
                 }
             }
@@ -931,22 +1064,40 @@ __global__ void k_refresh_bodies(tfs_body* bodies, uint32_t n, float4* motion, f
     }
 }
 
-// The contacts to collide again and their two bodies (velocities from the motion records), packed for
-// one copy to the host.
+// The contacts to collide again and their two bodies (velocities and travel from the motion records),
+// one wave per contact copying word by word, straight into the host's arrays at their own indices.
 __global__ void k_gather_again(const uint32_t* again, uint32_t n, const tfs_contact* contacts, const tfs_body* bodies,
-                               const float4* motion, tfs_contact* outC, tfs_body* outB) {
-    for (uint32_t k = gid(); k < n; k += gstride()) {
-        const tfs_contact& c = contacts[again[k]];
-        outC[k] = c;
+                               const float4* motion, uint32_t* hostAgain, tfs_contact* hostC, tfs_body* hostB) {
+    static_assert(sizeof(tfs_contact) % 4 == 0 && sizeof(tfs_body) % 4 == 0, "word copies");
+    constexpr uint32_t cw = sizeof(tfs_contact) / 4, bw = sizeof(tfs_body) / 4;
+    constexpr uint32_t ov = offsetof(tfs_body, v) / 4, ow = offsetof(tfs_body, w) / 4;
+    constexpr uint32_t odp = offsetof(tfs_body, dp) / 4, odq = offsetof(tfs_body, dq) / 4;
+    const uint32_t lane = threadIdx.x % 64, waves = gstride() / 64;
+    for (uint32_t k = gid() / 64; k < n; k += waves) {
+        const uint32_t ci = again[k];
+        if (lane == 0) { hostAgain[k] = ci; }
+        const uint32_t* src = (const uint32_t*)(contacts + ci);
+        uint32_t* dst = (uint32_t*)(hostC + ci);
+        for (uint32_t d = lane; d < cw; d += 64) { dst[d] = src[d]; }
+        const uint32_t ids[2] = { contacts[ci].body_a, contacts[ci].body_b };
         for (int e = 0; e < 2; ++e) {
-            const uint32_t i = e ? c.body_b : c.body_a;
-            tfs_body b = bodies[i];
-            const Motion m = loadMotion(motion, i);
-            st3(b.v, m.v);
-            st3(b.w, m.w);
-            st3(b.dp, m.dp);
-            st4(b.dq, m.dq);
-            outB[2 * k + e] = b;
+            const uint32_t i = ids[e];
+            const uint32_t* bs = (const uint32_t*)(bodies + i);
+            const uint32_t* ms = (const uint32_t*)(motion + 4 * (size_t)i);
+            uint32_t* bd = (uint32_t*)(hostB + i);
+            for (uint32_t d = lane; d < bw; d += 64) {
+                uint32_t x = bs[d];
+                if (d >= ov && d < ov + 3) {
+                    x = ms[d - ov];
+                } else if (d >= ow && d < ow + 3) {
+                    x = ms[4 + d - ow];
+                } else if (d >= odp && d < odp + 3) {
+                    x = ms[8 + d - odp];
+                } else if (d >= odq && d < odq + 4) {
+                    x = ms[12 + d - odq];
+                }
+                bd[d] = x;
+            }
         }
     }
 }
@@ -1013,14 +1164,20 @@ __global__ void k_sleep_time(tfs_body* bodies, uint32_t n, float dt, float thres
 
 __device__ __forceinline__ bool awakeDynamic(const tfs_body& b) { return b.type == TFS_DYNAMIC && b.awake; }
 
-__global__ void k_sleep_kinematic(const tfs_contact* contacts, uint32_t nc, tfs_body* bodies) {
+// Kinematic bodies that move keep their dynamic partners awake. Also writes the island graph's
+// edges: edges[k] is the contact's two bodies when both are awake and dynamic, else (0, 0).
+__global__ void k_sleep_kinematic(const tfs_contact* contacts, uint32_t nc, tfs_body* bodies, uint2* edges) {
     for (uint32_t k = gid(); k < nc; k += gstride()) {
         const tfs_contact& c = contacts[k];
+        edges[k] = make_uint2(0u, 0u);
         if (c.count == 0) { continue; }
         tfs_body& a = bodies[c.body_a];
         tfs_body& b = bodies[c.body_b];
         const bool da = awakeDynamic(a), db = awakeDynamic(b);
-        if (da && db) { continue; }
+        if (da && db) {
+            edges[k] = make_uint2(c.body_a, c.body_b);
+            continue;
+        }
         if (da && b.type == TFS_KINEMATIC && movingAwake(b, ld3(b.v), ld3(b.w))) {
             a.sleep_time = 0.0f;
         } else if (db && a.type == TFS_KINEMATIC && movingAwake(a, ld3(a.v), ld3(a.w))) {
@@ -1043,17 +1200,17 @@ __device__ __forceinline__ uint32_t rootOf(const uint32_t* parent, uint32_t x) {
 // Links the islands of two awake dynamic bodies in contact. Every link points from a larger index
 // to a smaller one, so each island ends with its lowest body index as root, as the reference's
 // union by the smaller root gives (src/world.cpp:1193-1194, include/twofold/world.h:464-465).
-__global__ void k_sleep_hook(const tfs_contact* contacts, uint32_t nc, const tfs_body* bodies, uint32_t* parent,
-                             uint32_t* changed) {
+__global__ void k_sleep_hook(const uint2* edges, uint32_t nc, uint32_t* parent, uint32_t* changed) {
+    bool linked = false;
     for (uint32_t k = gid(); k < nc; k += gstride()) {
-        const tfs_contact& c = contacts[k];
-        if (c.count == 0) { continue; }
-        if (!awakeDynamic(bodies[c.body_a]) || !awakeDynamic(bodies[c.body_b])) { continue; }
-        const uint32_t ra = rootOf(parent, c.body_a), rb = rootOf(parent, c.body_b);
+        const uint2 e = edges[k];
+        if (e.x == e.y) { continue; }
+        const uint32_t ra = rootOf(parent, e.x), rb = rootOf(parent, e.y);
         if (ra == rb) { continue; }
         atomicMin(&parent[ra > rb ? ra : rb], ra < rb ? ra : rb);
-        *changed = 1;
+        linked = true;
     }
+    if (__ballot(linked) != 0ull && (threadIdx.x % 64) == 0) { *changed = 1; }
 }
 
 __global__ void k_sleep_jump(uint32_t* parent, uint32_t n) {
@@ -1148,8 +1305,10 @@ struct tfs_ctx {
         double bytes;
     };
     std::vector<Pending> pending;
-    uint64_t profLaunches[3] = {};
-    double profMs[3] = {}, profBytes[3] = {};
+    // families: 0 warm start, 1 solve, 2 restitution (event-timed launches); 3 waiting for the
+    // refresh's contacts to reach the host, 4 the refresh callback (host clock)
+    uint64_t profLaunches[kProfileFamilies] = {};
+    double profMs[kProfileFamilies] = {}, profBytes[kProfileFamilies] = {};
     uint32_t lastCounts[6] = {};
 
     // bodies
@@ -1166,7 +1325,7 @@ struct tfs_ctx {
     uint32_t* islandTime = nullptr;
     // contacts
     tfs_contact* contacts = nullptr;
-    tfs_contact* fresh = nullptr;
+    tfs_contact* fresh = nullptr;   // the refresh callback's manifolds, copied in
     uint32_t* fActive = nullptr;
     uint32_t* fFast = nullptr;
     uint32_t* fFastSolid = nullptr;
@@ -1178,16 +1337,19 @@ struct tfs_ctx {
     uint32_t* impact = nullptr;
     uint32_t* rowsList = nullptr;
     uint32_t* impactRowsList = nullptr;
+    uint2* edges = nullptr;
     uint8_t* colors = nullptr;
     uint32_t* ep = nullptr;
     uint32_t* rank = nullptr;
     uint32_t* rowOf = nullptr;
     uint32_t* hist = nullptr;
     size_t histCap = 0;
-    uint32_t* tileSums = nullptr;
-    size_t tileCap = 0;
+    unsigned long long* status = nullptr;  // look-back scan status words
+    size_t statusCap = 0;
+    uint32_t scanEpoch = 0;
+    int scanSlot = 0;
     Rows rows{};
-    uint32_t* impactAt = nullptr;   // impact rows as active row indices
+    uint4* impactHdr = nullptr;     // impact rows' headers, .w = their active row
     unsigned long long barrierBase = 0;
     uint32_t maxPassBlocks = 0;
     bool usePass = true;
@@ -1199,20 +1361,13 @@ struct tfs_ctx {
     uint32_t* bep = nullptr;
     DevCounts* cnt = nullptr;
     DevCounts* hostCnt = nullptr;
-    // host staging for the refresh callback
-    std::vector<tfs_body> hBodies;
-    std::vector<tfs_contact> hContacts, hFresh;
-    std::vector<uint32_t> hAgain;
-    std::vector<tfs_contact> hGatherC;
-    std::vector<tfs_body> hGatherB;
-    tfs_contact* gatherC = nullptr;
-    tfs_body* gatherB = nullptr;
-    uint32_t gatherCap = 0;
-    // pinned host staging of the refresh
-    tfs_contact* pinC = nullptr;
-    tfs_body* pinB = nullptr;
+    // pinned host arrays of the refresh callback: bodies and contacts at their own indices (only the
+    // entries of the contacts collided again are written), the list, and the fresh manifolds
+    tfs_body* pinBodies = nullptr;
+    tfs_contact* pinContacts = nullptr;
     uint32_t* pinAgain = nullptr;
     tfs_contact* pinFresh = nullptr;
+    uint32_t pinBodiesCap = 0, pinContactsCap = 0;
 };
 
 namespace {
@@ -1248,6 +1403,8 @@ int grow(tfs_ctx* ctx, T*& p, size_t n) {
     return 0;
 }
 
+int ensureScratch(tfs_ctx* ctx);
+
 int allocBodies(tfs_ctx* ctx, uint32_t cap) {
     int r;
     if ((r = grow(ctx, ctx->bodies, cap)) || (r = grow(ctx, ctx->motion, 4 * (size_t)cap)) ||
@@ -1259,7 +1416,7 @@ int allocBodies(tfs_ctx* ctx, uint32_t cap) {
         return r;
     }
     ctx->capB = cap;
-    return 0;
+    return ensureScratch(ctx);
 }
 
 int allocRows(tfs_ctx* ctx, Rows& r, uint32_t cap) {
@@ -1283,26 +1440,51 @@ int allocContacts(tfs_ctx* ctx, uint32_t cap) {
         (r = grow(ctx, ctx->impact, cap)) || (r = grow(ctx, ctx->rowsList, cap)) ||
         (r = grow(ctx, ctx->impactRowsList, cap)) || (r = grow(ctx, ctx->colors, cap)) ||
         (r = grow(ctx, ctx->ep, 2 * (size_t)cap)) || (r = grow(ctx, ctx->rank, 2 * (size_t)cap)) ||
-        (r = grow(ctx, ctx->rowOf, cap))) {
+        (r = grow(ctx, ctx->rowOf, cap)) || (r = grow(ctx, ctx->edges, cap))) {
         return r;
     }
-    if ((r = allocRows(ctx, ctx->rows, cap)) || (r = grow(ctx, ctx->impactAt, cap)) || (r = grow(ctx, ctx->bep, 2 * (size_t)cap))) {
+    if ((r = allocRows(ctx, ctx->rows, cap)) || (r = grow(ctx, ctx->impactHdr, cap)) || (r = grow(ctx, ctx->bep, 2 * (size_t)cap))) {
         return r;
     }
     ctx->capC = cap;
+    return ensureScratch(ctx);
+}
+
+// The row-order histogram and the scans' status words, sized for the body and contact capacities so
+// that a step never allocates.
+int ensureScratch(tfs_ctx* ctx) {
+    int r;
+    const size_t hist = (size_t)kKeys * blocksFor(ctx->capC, kThreads, 1u << 30);
+    if (hist > ctx->histCap) {
+        if ((r = grow(ctx, ctx->hist, hist))) { return r; }
+        ctx->histCap = hist;
+    }
+    const size_t most = std::max<size_t>(std::max<size_t>(ctx->capB, ctx->capC), hist);
+    const size_t words = 3 * std::max<size_t>(1, (most + kScanTile - 1) / kScanTile);
+    if (words > ctx->statusCap) {
+        if ((r = grow(ctx, ctx->status, words))) { return r; }
+        LAUNCH(k_fill_u64, blocksFor((uint32_t)words), kThreads, ctx->status, (uint32_t)words);
+        ctx->statusCap = words;
+        ctx->scanEpoch = 0;
+    }
     return 0;
 }
 
-int ensureScratch(tfs_ctx* ctx, size_t histEntries, size_t tiles) {
-    int r;
-    if (histEntries > ctx->histCap) {
-        if ((r = grow(ctx, ctx->hist, histEntries))) { return r; }
-        ctx->histCap = histEntries;
+// Scans of up to three arrays of at most nMax items each, in one launch. Each launch of a step takes
+// its own ticket counter (cleared with the step's counters) and a fresh epoch for its status words.
+int scanJobs(tfs_ctx* ctx, const ScanJob* job, int count, uint32_t nMax) {
+    const uint32_t tiles = std::max<uint32_t>(1, (nMax + kScanTile - 1) / kScanTile);
+    if ((size_t)tiles * 3 > ctx->statusCap) { return fail(ctx, TFS_E_STATE, "scan of %u items beyond capacity", nMax); }
+    if (ctx->scanSlot >= kScanSlots) { return fail(ctx, TFS_E_DEVICE, "more than %d scans in one step", kScanSlots); }
+    if (++ctx->scanEpoch >= kEpochLimit) {
+        LAUNCH(k_fill_u64, blocksFor((uint32_t)ctx->statusCap), kThreads, ctx->status, (uint32_t)ctx->statusCap);
+        ctx->scanEpoch = 1;
     }
-    if (tiles > ctx->tileCap) {
-        if ((r = grow(ctx, ctx->tileSums, tiles))) { return r; }
-        ctx->tileCap = tiles;
-    }
+    ScanJobs jobs{};
+    for (int k = 0; k < count; ++k) { jobs.job[k] = job[k]; }
+    jobs.tiles = tiles;
+    LAUNCH(k_scan, tiles * (uint32_t)count, kScanThreads, jobs, ctx->status, ctx->scanEpoch,
+           &ctx->cnt->scanTicket[ctx->scanSlot++], ctx->cnt);
     return 0;
 }
 
@@ -1310,25 +1492,20 @@ int ensureScratch(tfs_ctx* ctx, size_t histEntries, size_t tiles) {
 // the prefix array and/or the compacted index list; the total goes to *total.
 int scan(tfs_ctx* ctx, const uint32_t* in, const uint32_t* nPtr, uint32_t nMax, uint32_t* out, uint32_t* list,
          uint32_t* total) {
-    const uint32_t tiles = (nMax + kTile - 1) / kTile ? (nMax + kTile - 1) / kTile : 1;
-    int r;
-    if ((r = ensureScratch(ctx, 0, tiles))) { return r; }
-    LAUNCH(k_scan_tiles, tiles, kThreads, in, nPtr, nMax, ctx->tileSums);
-    LAUNCH(k_scan_sums, 1, kThreads, ctx->tileSums, tiles, total);
-    LAUNCH(k_scan_apply, tiles, kThreads, in, nPtr, nMax, ctx->tileSums, out, list);
-    return 0;
+    const ScanJob job{ in, nPtr, nMax, out, list, total };
+    return scanJobs(ctx, &job, 1, nMax);
 }
 
 // Stable row order of `list` (length *nPtr <= nMax) into out; colour starts into starts[0..32].
 int rowOrder(tfs_ctx* ctx, const uint32_t* list, const uint32_t* nPtr, uint32_t nMax, uint32_t* out, uint32_t* starts,
              uint32_t* points) {
     const uint32_t nBlocks = blocksFor(nMax, kThreads, 1u << 30);
+    if ((size_t)kKeys * nBlocks > ctx->histCap) { return fail(ctx, TFS_E_STATE, "row order beyond capacity"); }
     int r;
-    if ((r = ensureScratch(ctx, (size_t)kKeys * nBlocks, 0))) { return r; }
     LAUNCH(k_row_hist, nBlocks, kThreads, list, nPtr, ctx->contacts, ctx->colors, ctx->hist, nBlocks);
     if ((r = scan(ctx, ctx->hist, nullptr, kKeys * nBlocks, ctx->hist, nullptr, &ctx->cnt->scanTotal))) { return r; }
-    LAUNCH(k_row_scatter, nBlocks, kThreads, list, nPtr, ctx->contacts, ctx->colors, ctx->hist, nBlocks, out, points);
-    LAUNCH(k_row_starts, 1, 64, ctx->hist, nBlocks, starts);
+    LAUNCH(k_row_scatter, nBlocks, kThreads, list, nPtr, ctx->contacts, ctx->colors, ctx->hist, nBlocks, out);
+    LAUNCH(k_row_starts, 1, 64, ctx->hist, nBlocks, starts, points);
     return 0;
 }
 
@@ -1343,7 +1520,7 @@ double rowBytes(int mode, uint32_t rows, uint32_t points) {
     }
 }
 
-int runRows(tfs_ctx* ctx, int mode, const uint32_t* at, const uint32_t* starts, const uint32_t* points, const Ctx& c) {
+int runRows(tfs_ctx* ctx, int mode, const uint4* at, const uint32_t* starts, const uint32_t* points, const Ctx& c) {
     for (int col = 0; col < TFS_COLORS; ++col) {
         const uint32_t b = starts[col], e = starts[col + 1];
         if (e <= b) { continue; }
@@ -1412,7 +1589,7 @@ int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool w
         ctx->pending.push_back({ 1, ev, bytes });
         HIP_TRY(hipEventRecord(ctx->evPool[ev], ctx->stream));
     }
-    const uint32_t* at = ctx->impactAt;
+    const uint4* at = ctx->impactHdr;
     unsigned long long* counter = &ctx->cnt->barrier;
     const unsigned long long base = ctx->barrierBase;
     const float4* mass = ctx->mass;
@@ -1437,7 +1614,7 @@ int solvePass(tfs_ctx* ctx, int mode, const DevCounts& hc, uint32_t laned, uint3
             (r = runRest(ctx, mode, ctx->rowsList, laned, hc.nActive, c))) {
             return r;
         }
-        if (impact && ((r = runRows(ctx, mode, ctx->impactAt, hc.impactColorStart, hc.impactColorPoints, c)) ||
+        if (impact && ((r = runRows(ctx, mode, ctx->impactHdr, hc.impactColorStart, hc.impactColorPoints, c)) ||
                        (r = runRest(ctx, mode, ctx->impactRowsList, impactLaned, hc.nImpact, c)))) {
             return r;
         }
@@ -1477,8 +1654,11 @@ int warmRows(tfs_ctx* ctx, uint32_t laned, uint32_t points) {
 int packRows(tfs_ctx* ctx, uint32_t laned, uint32_t impactLaned, bool lists) {
     LAUNCH(k_pack, blocksFor(laned), kThreads, ctx->rowsList, laned, ctx->contacts, ctx->moving, ctx->mass, ctx->rows,
            ctx->rowOf);
+    if (impactLaned > 0) {
+        LAUNCH(k_impact_rows, blocksFor(impactLaned), kThreads, ctx->impactRowsList, impactLaned, ctx->rowOf,
+               ctx->rows.hdr, ctx->impactHdr);
+    }
     if (!lists) { return 0; }
-    LAUNCH(k_impact_rows, blocksFor(impactLaned), kThreads, ctx->impactRowsList, impactLaned, ctx->rowOf, ctx->impactAt);
     const uint32_t nb = ctx->nb;
     LAUNCH(k_bep_count, blocksFor(laned), kThreads, laned, ctx->rows.hdr, ctx->bdeg);
     int r;
@@ -1531,38 +1711,41 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
            ctx->rowOf, ctx->colors, ctx->motion);
     LAUNCH(k_refresh_bodies, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, ctx->mass);
     // Only the contacts collided again and their bodies go to the host; the callback reads no others.
-    if (nAgain > ctx->gatherCap) {
-        int e;
-        if ((e = grow(ctx, ctx->gatherC, nAgain)) || (e = grow(ctx, ctx->gatherB, 2 * (size_t)nAgain)) ||
-            (e = growPinned(ctx, ctx->pinC, nAgain)) || (e = growPinned(ctx, ctx->pinB, 2 * (size_t)nAgain)) ||
-            (e = growPinned(ctx, ctx->pinAgain, nAgain)) || (e = growPinned(ctx, ctx->pinFresh, nAgain))) {
+    // Sized for the capacities at the first refresh, so later steps do not allocate.
+    int e;
+    if (ctx->capB > ctx->pinBodiesCap) {
+        if ((e = growPinned(ctx, ctx->pinBodies, ctx->capB))) { return e; }
+        ctx->pinBodiesCap = ctx->capB;
+    }
+    if (ctx->capC > ctx->pinContactsCap) {
+        if ((e = growPinned(ctx, ctx->pinContacts, ctx->capC)) || (e = growPinned(ctx, ctx->pinAgain, ctx->capC)) ||
+            (e = growPinned(ctx, ctx->pinFresh, ctx->capC))) {
             return e;
         }
-        ctx->gatherCap = nAgain;
+        ctx->pinContactsCap = ctx->capC;
     }
-    LAUNCH(k_gather_again, blocksFor(nAgain), kThreads, again, nAgain, ctx->contacts, ctx->bodies, ctx->motion,
-           ctx->gatherC, ctx->gatherB);
-    HIP_TRY(hipMemcpyAsync(ctx->pinC, ctx->gatherC, sizeof(tfs_contact) * nAgain, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->pinB, ctx->gatherB, sizeof(tfs_body) * 2 * nAgain, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(ctx->pinAgain, again, sizeof(uint32_t) * nAgain, hipMemcpyDeviceToHost, ctx->stream));
-    ctx->hBodies.resize(nb);
-    ctx->hContacts.resize(nc);
+    const uint32_t waves = std::min<uint32_t>(nAgain, 4u * 8192u);
+    LAUNCH(k_gather_again, (waves + 3) / 4, kThreads, again, nAgain, ctx->contacts, ctx->bodies, ctx->motion,
+           ctx->pinAgain, ctx->pinContacts, ctx->pinBodies);
+    using Clock = std::chrono::steady_clock;
+    const Clock::time_point t0 = Clock::now();
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    for (uint32_t k = 0; k < nAgain; ++k) {
-        const tfs_contact& gc = ctx->pinC[k];
-        ctx->hContacts[ctx->pinAgain[k]] = gc;
-        ctx->hBodies[gc.body_a] = ctx->pinB[2 * k];
-        ctx->hBodies[gc.body_b] = ctx->pinB[2 * k + 1];
-    }
-    memset(ctx->pinFresh, 0, sizeof(tfs_contact) * nAgain);
-    if (cb(user, substep, remaining, ctx->pinAgain, nAgain, ctx->hBodies.data(), nb, ctx->hContacts.data(), nc,
-           ctx->pinFresh) != 0) {
+    const Clock::time_point t1 = Clock::now();
+    if (cb(user, substep, remaining, ctx->pinAgain, nAgain, ctx->pinBodies, nb, ctx->pinContacts, nc, ctx->pinFresh) != 0) {
         return fail(ctx, TFS_E_CALLBACK, "the refresh callback failed at substep %d", substep);
+    }
+    if (ctx->profiling) {
+        const Clock::time_point t2 = Clock::now();
+        ctx->profLaunches[3] += 1;
+        ctx->profMs[3] += std::chrono::duration<double, std::milli>(t1 - t0).count();
+        ctx->profLaunches[4] += 1;
+        ctx->profMs[4] += std::chrono::duration<double, std::milli>(t2 - t1).count();
+        ctx->profBytes[4] += nAgain;
     }
     HIP_TRY(hipMemcpyAsync(ctx->fresh, ctx->pinFresh, sizeof(tfs_contact) * nAgain, hipMemcpyHostToDevice, ctx->stream));
     const float matchSq = ctx->settings.speculative * ctx->settings.speculative;
-    LAUNCH(k_refresh_apply, blocksFor(nAgain), kThreads, again, nAgain, ctx->fresh, ctx->contacts, ctx->bodies,
-           ctx->motion, ctx->mass, matchSq);
+    LAUNCH(k_refresh_apply, blocksFor(nAgain), kThreads, again, nAgain, ctx->fresh,
+           ctx->contacts, ctx->bodies, ctx->motion, ctx->mass, matchSq);
     // A refreshed contact keeps its row; its points and axes are packed again, as prepareLanes does.
     if (int e = packRows(ctx, laned, impactLaned, false)) { return e; }
     return 0;
@@ -1576,6 +1759,7 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
     HIP_TRY(hipMemsetAsync(ctx->cnt, 0, sizeof(DevCounts), ctx->stream));
     ctx->barrierBase = 0;
+    ctx->scanSlot = 0;
 
     // step state and lists
     const BodyScratch zero{ ctx->struck, ctx->deg, ctx->cursor, ctx->bdeg, ctx->bcursor, ctx->word };
@@ -1583,11 +1767,10 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     LAUNCH(k_classify, blocksFor(nc), kThreads, ctx->contacts, nc, ctx->bodies, dt, s.speculative, ctx->fActive,
            ctx->fFast, ctx->fFastSolid, ctx->colors);
     int r;
-    if ((r = scan(ctx, ctx->fActive, nullptr, nc, nullptr, ctx->active, &ctx->cnt->nActive)) ||
-        (r = scan(ctx, ctx->fFast, nullptr, nc, nullptr, ctx->fast, &ctx->cnt->nFast)) ||
-        (r = scan(ctx, ctx->fFastSolid, nullptr, nc, nullptr, ctx->fastSolid, &ctx->cnt->nFastSolid))) {
-        return r;
-    }
+    const ScanJob lists[3] = { { ctx->fActive, nullptr, nc, nullptr, ctx->active, &ctx->cnt->nActive },
+                               { ctx->fFast, nullptr, nc, nullptr, ctx->fast, &ctx->cnt->nFast },
+                               { ctx->fFastSolid, nullptr, nc, nullptr, ctx->fastSolid, &ctx->cnt->nFastSolid } };
+    if ((r = scanJobs(ctx, lists, 3, nc))) { return r; }
 
     // prepare, impact
     LAUNCH(k_prepare, blocksFor(nc), kThreads, ctx->active, ctx->cnt, ctx->contacts, ctx->bodies, ctx->motion, ctx->mass,
@@ -1655,14 +1838,22 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     // sleep and islands
     LAUNCH(k_sleep_time, blocksFor(nb), kThreads, ctx->bodies, nb, dt, s.sleep_threshold * s.length_unit, ctx->parent,
            ctx->islandTime);
-    LAUNCH(k_sleep_kinematic, blocksFor(nc), kThreads, ctx->contacts, nc, ctx->bodies);
-    for (int it = 0; it < 64; ++it) {
-        LAUNCH(k_sleep_hook, blocksFor(nc), kThreads, ctx->contacts, nc, ctx->bodies, ctx->parent, &ctx->cnt->changed[it]);
-        LAUNCH(k_sleep_jump, blocksFor(nb), kThreads, ctx->parent, nb);
-        HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->changed[it], &ctx->cnt->changed[it], sizeof(uint32_t), hipMemcpyDeviceToHost,
-                               ctx->stream));
+    LAUNCH(k_sleep_kinematic, blocksFor(nc), kThreads, ctx->contacts, nc, ctx->bodies, ctx->edges);
+    // Hook-and-jump rounds until one links nothing; rounds after that change nothing, so they are
+    // launched a few at a time and only the last round of each batch is checked on the host.
+    for (int it = 0;;) {
+        const int batch = it == 0 ? 4 : 2;
+        if (it + batch > kIslandRounds) {
+            return fail(ctx, TFS_E_KERNEL, "the island pass did not settle in %d rounds", kIslandRounds);
+        }
+        for (int j = 0; j < batch; ++j, ++it) {
+            LAUNCH(k_sleep_hook, blocksFor(nc), kThreads, ctx->edges, nc, ctx->parent, &ctx->cnt->changed[it]);
+            LAUNCH(k_sleep_jump, blocksFor(nb), kThreads, ctx->parent, nb);
+        }
+        HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->changed[it - 1], &ctx->cnt->changed[it - 1], sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        if (!ctx->hostCnt->changed[it]) { break; }
+        if (!ctx->hostCnt->changed[it - 1]) { break; }
     }
     LAUNCH(k_island_time, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime);
     LAUNCH(k_sleep_apply, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime, s.time_to_sleep,
@@ -1733,9 +1924,10 @@ int tfs_create(int device, uint32_t maxBodies, uint32_t maxContacts, tfs_ctx** o
             r = TFS_E_DEVICE;
         }
         // Every block of a cooperative pass must be resident at once (half the occupancy limit keeps well
-        // inside that). The barrier's cost grows with the block count (a few microseconds at 32 blocks,
-        // about 14 at 256), so the pass uses at most 64 blocks.
-        ctx->maxPassBlocks = (uint32_t)std::max(1, std::min(perCu * cus / 2, 64));
+        // inside that). The barrier's cost grows with the block count (about a microsecond at 64
+        // blocks, 4 at 256), so the pass uses at most 128 blocks, one row per thread up to 32k rows
+        // a colour.
+        ctx->maxPassBlocks = (uint32_t)std::max(1, std::min(perCu * cus / 2, 128));
         ctx->usePass = coop != 0 && perCu > 0;
         // Tuning knobs, read once: TWOFOLD_PASS=launch|coop, TWOFOLD_PASS_BLOCKS=n, TWOFOLD_AXES=packed|made.
         if (const char* e = getenv("TWOFOLD_PASS")) { ctx->usePass = ctx->usePass && strcmp(e, "launch") != 0; }
@@ -1762,15 +1954,15 @@ void tfs_destroy(tfs_ctx* ctx) {
                      ctx->word, ctx->parent, ctx->islandTime, ctx->contacts, ctx->fresh, ctx->fActive, ctx->fFast,
                      ctx->fFastSolid, ctx->active, ctx->fast, ctx->fastSolid, ctx->impactFlag, ctx->impactPos,
                      ctx->impact, ctx->rowsList, ctx->impactRowsList, ctx->colors, ctx->ep, ctx->rank, ctx->rowOf,
-                     ctx->hist, ctx->tileSums, ctx->cnt, ctx->rows.hdr, ctx->rows.nrm,
+                     ctx->hist, ctx->status, ctx->cnt, ctx->rows.hdr, ctx->rows.nrm,
                      ctx->rows.tg1, ctx->rows.tg2, ctx->rows.pa, ctx->rows.pb, ctx->rows.imp, ctx->rows.peak,
-                     ctx->rows.ax, ctx->impactAt, ctx->bdeg, ctx->bstart, ctx->bcursor, ctx->bep, ctx->gatherC,
-                     ctx->gatherB };
+                     ctx->rows.ax, ctx->impactHdr, ctx->bdeg, ctx->bstart, ctx->bcursor, ctx->bep,
+                     ctx->edges };
     for (void* p : ptrs) {
         if (p) { (void)hipFree(p); }
     }
     if (ctx->hostCnt) { (void)hipHostFree(ctx->hostCnt); }
-    for (void* h : { (void*)ctx->pinC, (void*)ctx->pinB, (void*)ctx->pinAgain, (void*)ctx->pinFresh }) {
+    for (void* h : { (void*)ctx->pinBodies, (void*)ctx->pinContacts, (void*)ctx->pinAgain, (void*)ctx->pinFresh }) {
         if (h) { (void)hipHostFree(h); }
     }
     for (hipEvent_t e : ctx->ev) {
@@ -1886,8 +2078,9 @@ int tfs_last_step_ms(tfs_ctx* ctx, float* solveMs, float* sleepMs) {
 int tfs_set_profiling(tfs_ctx* ctx, int on) {
     if (!ctx) { return TFS_E_ARG; }
     ctx->profiling = on != 0;
+    if (!on) { return 0; }
     ctx->pending.clear();
-    for (int f = 0; f < 3; ++f) {
+    for (int f = 0; f < kProfileFamilies; ++f) {
         ctx->profLaunches[f] = 0;
         ctx->profMs[f] = 0.0;
         ctx->profBytes[f] = 0.0;
@@ -1896,7 +2089,7 @@ int tfs_set_profiling(tfs_ctx* ctx, int on) {
 }
 
 int tfs_read_profile(tfs_ctx* ctx, int family, uint64_t* launches, double* ms, double* bytes) {
-    if (!ctx || family < 0 || family > 2) { return fail(ctx, TFS_E_ARG, "unknown profile family"); }
+    if (!ctx || family < 0 || family >= kProfileFamilies) { return fail(ctx, TFS_E_ARG, "unknown profile family"); }
     if (launches) { *launches = ctx->profLaunches[family]; }
     if (ms) { *ms = ctx->profMs[family]; }
     if (bytes) { *bytes = ctx->profBytes[family]; }
