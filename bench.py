@@ -33,6 +33,12 @@ from twofold_amd import scenes  # noqa: E402
 from twofold_amd.world import Simulation  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# Memory-side bytes per k_pass launch on this workload, from two rocprofv3 --pmc runs of this script
+# (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per dispatch, the factors from a 256 MiB copy measured in the same
+# runs, --copy-mib 256); counters cannot be read from inside the process, so the figure is carried here.
+K_PASS_TRAFFIC_BYTES = 97.2e6
+K_PASS_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, mean of the last 32 k_pass dispatches, scaled by a "
+                         "256 MiB copy (profiles/k_pass_pile32k_traffic.md)")
 
 
 def log(*a):
@@ -136,12 +142,25 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=10, help="event-timed steps for the roofline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline leg")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB: the reference reading for scaling "
+                         "the FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run")
     args = ap.parse_args()
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world_size != 1:
         print("bench.py runs on one GPU only: this build measures the solve path on a single MI355X "
               "(--gpus 1); multi-GPU runs are not part of it.", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_mib > 0:
+        import torch
+
+        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copied {args.copy_mib} MiB device to device")
+        del src, dst
 
     scene = scenes.pile(side=args.side)
     t0 = time.perf_counter()
@@ -162,20 +181,35 @@ def main():
     except ImportError:
         sync = lambda: None  # noqa: E731
 
+    transfer = [0.0]
+
     def solve_step():
-        """Host narrowphase and upload (untimed), then the solve step on resident inputs (timed)."""
-        sim.collide()
+        """Host narrowphase and upload (untimed), then the solve step on resident inputs (timed). The
+        upload of the contacts (and of the bodies, when the narrowphase woke some) and the download of
+        both after the step are timed apart, for the rate with PCIe transfers included."""
+        sim.bodies = sim.solver.bodies()
+        contacts, woke = sim.world.collide(sim.bodies, sim.dt, sim.contacts)
+        t = time.perf_counter()
+        if woke:
+            sim.solver.upload(sim.bodies, contacts)
+        else:
+            sim.solver.upload_contacts(contacts)
+        transfer[0] += time.perf_counter() - t
         sync()
         t = time.perf_counter()
         sim.solver.step_native(sim.dt, sim.world.refresh_fn, sim.world.h)
         sync()
         spent = time.perf_counter() - t
+        t = time.perf_counter()
         sim.contacts = sim.solver.contacts()
+        sim.bodies = sim.solver.bodies()
+        transfer[0] += time.perf_counter() - t
         sim.steps += 1
         return spent
 
     for _ in range(args.warmup):
         solve_step()
+    transfer[0] = 0.0
     its = 0
     solve_ms = 0.0
     wall = 0.0
@@ -186,6 +220,7 @@ def main():
         its += iterations(counts, substeps)
         solve_ms += sim.solver.last_ms()[0]
         contacts += len(sim.contacts)
+    transfers = transfer[0]
     sim.solver.set_profiling(True)
     for _ in range(args.profile_steps):
         solve_step()
@@ -222,10 +257,13 @@ def main():
             "bodies": len(bodies0), "contacts_per_step": contacts / args.steps, "active_points": counts["points"],
             "impact_points": counts["impact_points"], "colors": counts["colors"], "rest": counts["rest"],
             "world_steps_per_s": args.steps / wall, "gpu_solve_ms_per_step": solve_ms / args.steps,
+            "transfer_ms_per_step": 1e3 * transfers / args.steps,
+            "value_with_transfers": its / (wall + transfers),
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None, "kernel": "k_pass<bias|relax> (one cooperative launch per solve pass)",
+            "traffic": K_PASS_TRAFFIC_BYTES if args.side == 32 else None, "traffic_source": K_PASS_TRAFFIC_SOURCE,
+            "kernel": "k_pass<bias|relax> (one cooperative launch per solve pass)",
             "launches": launches, "profiled_steps": args.profile_steps,
             "bytes_per_launch": algo / max(launches, 1), "ms_per_launch": ms / max(launches, 1),
         },
