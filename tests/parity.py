@@ -68,3 +68,61 @@ def record_dict(rec):
     return dict(lists=rec.lists, colors=rec.colors, post_bodies=rec.post_bodies, step_bodies=rec.step_bodies,
                 post_contacts=rec.post_contacts, pre_bodies=rec.pre_bodies, pre_contacts=rec.pre_contacts,
                 refreshes=rec.refreshes, dt=rec.dt)
+
+
+def invariants(pre_bodies, pre_contacts, contacts, bodies, lists):
+    """Size-independent properties of a step's results, for any input size: the colouring is the
+    sequential greedy one over moving bodies (src/lanes.cpp:356-374), the row order is colour-major
+    with more points first (point counts as the step began; a refresh may change them later) and ties
+    in list order (src/lanes.cpp:376-407), and every island is one connected component of touching
+    awake dynamic bodies labelled by its lowest body index (src/world.cpp:1186-1237). `contacts` are
+    the contacts after the step. Returns a list of violations."""
+    bad = []
+    moving = (pre_bodies["type"] == 2) & (pre_bodies["awake"] != 0) & (pre_bodies["alive"] != 0)
+    colors = np.asarray(lists["colors"])
+    act = np.asarray(lists["active"])
+    taken = {}
+    for ci in act:
+        c = contacts[ci]
+        if c["planar"] or c["rolling_resistance"] > 0:
+            if colors[ci] < 32:
+                bad.append(f"contact {ci}: planar or rolling but coloured")
+            continue
+        ends = [int(x) for x in (c["body_a"], c["body_b"]) if moving[x]]
+        used = 0
+        for e in ends:
+            used |= taken.get(e, 0)
+        want = next((k for k in range(32) if not used >> k & 1), 32)
+        if colors[ci] != want:
+            bad.append(f"contact {ci}: colour {colors[ci]}, greedy gives {want}")
+            break
+        if want < 32:
+            for e in ends:
+                taken[e] = taken.get(e, 0) | (1 << want)
+    rows = np.asarray(lists["rows"])
+    pos = {int(ci): k for k, ci in enumerate(act)}
+    key = [(int(colors[ci]) if colors[ci] < 32 else 32, -int(pre_contacts[ci]["count"]) if colors[ci] < 32 else 0,
+            pos[int(ci)]) for ci in rows]
+    if key != sorted(key):
+        bad.append("row order is not colour-major, points-descending, stable")
+    n = len(bodies)
+    awake = (bodies["type"] == 2) & (pre_bodies["awake"] != 0) & (bodies["alive"] != 0)
+    parent = np.arange(n)
+
+    def root(x):
+        while parent[x] != x:
+            parent[x] = parent[parent[x]]
+            x = parent[x]
+        return x
+
+    for c in contacts:
+        if c["count"] == 0 or not (awake[c["body_a"]] and awake[c["body_b"]]):
+            continue
+        ra, rb = root(int(c["body_a"])), root(int(c["body_b"]))
+        if ra != rb:
+            parent[max(ra, rb)] = min(ra, rb)
+    labels = np.array([root(i) for i in range(n)])
+    wrong = np.nonzero(awake & (bodies["island"] != labels))[0]
+    if len(wrong):
+        bad.append(f"island ids at {wrong[:5].tolist()}")
+    return bad

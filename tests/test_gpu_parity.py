@@ -7,8 +7,9 @@ sleep flags exactly, and every velocity, pose and impulse bit for bit."""
 import numpy as np
 import pytest
 
-from twofold_amd import scenes
+from twofold_amd import _abi, scenes
 from twofold_amd.solver import Solver
+from twofold_amd.world import HostWorld, Simulation
 import golden_io
 import parity
 import refworld
@@ -68,7 +69,7 @@ def run_against_reference(solver, scene, steps=None):
 
 @needs_ref
 @pytest.mark.parametrize("make", [scenes.pyramid, scenes.stack, scenes.bounce, scenes.rolling, scenes.planar_mix,
-                                  scenes.tumble], ids=lambda f: f.__name__)
+                                  scenes.tumble, scenes.hub], ids=lambda f: f.__name__)
 def test_gpu_matches_reference_every_step(solver, make):
     scene = make()
     print(f"reference run {scene.name}", flush=True)
@@ -84,5 +85,82 @@ def test_gpu_matches_reference_pile():
     s = Solver(1024, 4096)
     try:
         assert run_against_reference(s, scene) > 0
+    finally:
+        s.close()
+
+
+@needs_ref
+def test_gpu_matches_reference_full_pile():
+    """The 32,768-body pile at full size: 240 drop-and-settle steps with the host world and the GPU
+    solver, then two steps of the reference from that body state (its own narrowphase and refresh
+    manifolds) replayed on the GPU, bit for bit, and the results' size-independent properties."""
+    scene = scenes.pile(side=32)
+    print("full pile: settling 240 steps", flush=True)
+    sim = Simulation(scene)
+    for _ in range(240):
+        sim.step()
+    bodies = sim.solver.bodies()
+    sim.solver.close()
+    w = refworld.RefWorld(scene)
+    w.set_bodies(bodies)
+    s = Solver(len(bodies), 4 * len(bodies))
+    try:
+        for k in range(2):
+            rec = w.step()
+            assert rec is not None
+            case = parity.record_dict(rec)
+            print(f"full pile: step {k}, {len(case['pre_contacts'])} contacts", flush=True)
+            b, c, lists = gpu_step(s, w.settings, case)
+            assert parity.compare(case, b, c, lists) == []
+            assert parity.invariants(case["pre_bodies"], case["pre_contacts"], c, b, lists) == []
+            assert len(lists["active"]) > 50000 and len(set(lists["colors"][lists["active"]])) > 8
+    finally:
+        s.close()
+
+
+def same_bits(a, b):
+    """Fields of two structured arrays that differ in any bit."""
+    out = []
+    for f in a.dtype.names:
+        x, y = np.ascontiguousarray(a[f]), np.ascontiguousarray(b[f])
+        if x.view(np.uint8).tobytes() != y.view(np.uint8).tobytes():
+            out.append(f)
+    return out
+
+
+def test_gpu_empty_world():
+    print("empty world", flush=True)
+    s = Solver(1, 1)
+    try:
+        s.upload(np.zeros(0, _abi.BODY_DTYPE), np.zeros(0, _abi.CONTACT_DTYPE))
+        s.step(np.float32(1 / 60))
+        assert len(s.bodies()) == 0 and s.counts()["active"] == 0
+    finally:
+        s.close()
+
+
+def test_gpu_bodies_without_contacts_match_oracle():
+    """Free flight: bodies of every motion type and no contact at all."""
+    print("bodies without contacts", flush=True)
+    sc = scenes.Scene("free")
+    rng = scenes.Rng(11)
+    for i in range(40):
+        kind = (_abi.DYNAMIC, _abi.DYNAMIC, _abi.KINEMATIC, _abi.STATIC)[i % 4]
+        b = sc.body(kind, pos=(rng.range(-5, 5), rng.range(0, 9), rng.range(-5, 5)),
+                    v=(rng.range(-3, 3), rng.range(-3, 3), rng.range(-3, 3)),
+                    w=(rng.range(-4, 4), rng.range(-4, 4), rng.range(-4, 4)))
+        sc.shape(b, scenes.BOX if i % 2 else scenes.SPHERE, (0.3, 0.2, 0.4) if i % 2 else (0.3,), density=800.0)
+    bodies = HostWorld(sc).bodies()
+    contacts = np.zeros(0, _abi.CONTACT_DTYPE)
+    settings = _abi.default_settings(substeps=4)
+    s = Solver(len(bodies), 1, settings=settings)
+    try:
+        for step in range(3):
+            want, _, _ = refworld.oracle_step(bodies, contacts, settings, np.float32(1 / 60))
+            s.upload(bodies, contacts)
+            s.step(np.float32(1 / 60))
+            got = s.bodies()
+            assert same_bits(want, got) == [], f"step {step}"
+            bodies = got
     finally:
         s.close()

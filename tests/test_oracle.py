@@ -23,7 +23,7 @@ needs_ref = pytest.mark.skipif(not refworld.ref_available(), reason="reference c
 
 @needs_ref
 @pytest.mark.parametrize("make", [scenes.pyramid, scenes.stack, scenes.bounce, scenes.rolling, scenes.planar_mix,
-                                  scenes.tumble], ids=lambda f: f.__name__)
+                                  scenes.tumble, scenes.hub], ids=lambda f: f.__name__)
 def test_oracle_matches_reference_every_step(make):
     scene = make()
     w = refworld.RefWorld(scene)
@@ -176,3 +176,29 @@ def test_coloring_brute_force_and_invariants():
                     if moving[x]:
                         assert x not in seen
                         seen.add(x)
+
+
+@needs_ref
+def test_invariants_hold_and_catch_planted_mistakes():
+    """The size-independent checks used at full size: clean on a real step of a pile, and each one
+    fails when a mistake is planted in the oracle's results."""
+    scene = scenes.pile(side=6, half_room=4.0, steps=60)
+    w = refworld.RefWorld(scene)
+    rec = None
+    for _ in range(60):
+        rec = w.step() or rec
+    b, c, lists = refworld.oracle_step(rec.pre_bodies, rec.pre_contacts, w.settings, scene.dt,
+                                       refworld.recorded_refresh(rec.refreshes))
+    assert parity.invariants(rec.pre_bodies, rec.pre_contacts, c, b, lists) == []
+    coloured = [ci for ci in lists["rows"] if lists["colors"][ci] < 32]
+    assert len(coloured) > 10
+    bad_colors = dict(lists, colors=lists["colors"].copy())
+    bad_colors["colors"][coloured[-1]] = (bad_colors["colors"][coloured[-1]] + 1) % 32
+    assert any("colour" in m for m in parity.invariants(rec.pre_bodies, rec.pre_contacts, c, b, bad_colors))
+    rows = lists["rows"].copy()
+    rows[[0, -1]] = rows[[-1, 0]]
+    assert any("row order" in m for m in parity.invariants(rec.pre_bodies, rec.pre_contacts, c, b, dict(lists, rows=rows)))
+    awake = np.nonzero((b["type"] == 2) & (rec.pre_bodies["awake"] != 0))[0]
+    bad_b = b.copy()
+    bad_b["island"][awake[-1]] += 1
+    assert any("island" in m for m in parity.invariants(rec.pre_bodies, rec.pre_contacts, c, bad_b, lists))

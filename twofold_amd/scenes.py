@@ -243,3 +243,21 @@ def tumble(n=24, seed=7, steps=180):
             s.shape(b, SPHERE, (rng.range(0.2, 0.5),), friction=rng.range(0.2, 1.0),
                     restitution=rng.range(0, 0.4))
     return s
+
+
+def hub(n=7, steps=90):
+    """A dynamic slab carrying n x n spheres on a half-space. The slab has n * n + 1 contacts, so once
+    its 32 colours are taken the rest of its contacts go to the serial rest list
+    (src/lanes.cpp:356-374) and are solved one at a time after the coloured rows."""
+    s = Scene(f"hub-{n}", steps=steps)
+    s.ground(0.6)
+    half = f32(0.25) * f32(n) + f32(0.05)
+    slab = s.body(DYNAMIC, pos=(0, 0.25, 0))
+    s.shape(slab, BOX, (half, 0.25, half), density=500.0, friction=0.5)
+    for i in range(n):
+        for k in range(n):
+            x = (f32(i) - f32(n - 1) / f32(2)) * f32(0.5)
+            z = (f32(k) - f32(n - 1) / f32(2)) * f32(0.5)
+            b = s.body(DYNAMIC, pos=(x, 0.7, z))
+            s.shape(b, SPHERE, (0.2,), density=1000.0, friction=0.5)
+    return s
