@@ -30,7 +30,8 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 from twofold_amd import scenes  # noqa: E402
-from twofold_amd.world import Simulation  # noqa: E402
+from twofold_amd.solver import Solver  # noqa: E402
+from twofold_amd.world import HostWorld, Simulation  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # Memory-side bytes per k_pass launch on this workload, from two rocprofv3 --pmc runs of this script
@@ -132,12 +133,134 @@ def cpu_port(sim, bodies, contacts, budget_s):
                 ms_per_step=1e3 * spent / steps)
 
 
+def device_sync():
+    try:
+        import torch
+
+        return torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except ImportError:
+        return lambda: None
+
+
+class SolveSteps:
+    """Steps a Simulation the way the benchmark times it: host narrowphase and upload (untimed), the
+    solve step on resident inputs (timed, bracketed by device syncs), then the download (timed apart
+    with the upload, for the rate with PCIe transfers included)."""
+
+    def __init__(self, sim):
+        self.sim = sim
+        self.sync = device_sync()
+        self.transfer = 0.0
+
+    def __call__(self):
+        sim = self.sim
+        sim.bodies = sim.solver.bodies()
+        contacts, woke = sim.world.collide(sim.bodies, sim.dt, sim.contacts)
+        t = time.perf_counter()
+        if woke:
+            sim.upload(sim.bodies, contacts)
+        else:
+            sim.solver.upload_contacts(contacts)
+        self.transfer += time.perf_counter() - t
+        self.sync()
+        t = time.perf_counter()
+        sim.solver.step_native(sim.dt, sim.world.refresh_fn, sim.world.h)
+        self.sync()
+        spent = time.perf_counter() - t
+        t = time.perf_counter()
+        sim.contacts = sim.solver.contacts()
+        sim.bodies = sim.solver.bodies()
+        self.transfer += time.perf_counter() - t
+        sim.steps += 1
+        return spent
+
+
+def run_batched(args):
+    """Batched worlds: `--worlds` independent piles of log-uniform [8, 1024] bodies (the pile recipe
+    per world, its own friction and restitution), solved together in one resident set
+    (tfs_set_worlds), timed per solve step as the pile is; against looping the plain path over a
+    sample of the same worlds, each alone in its own solve step from the same state."""
+    sizes = scenes.batch_sizes(args.worlds)
+    t0 = time.perf_counter()
+    worlds = [scenes.batch_world(n, 0x5EED0000 + k) for k, n in enumerate(sizes)]
+    sim = Simulation(worlds)
+    log(f"{len(worlds)} worlds, {len(sim.bodies)} bodies, built in {time.perf_counter() - t0:.1f} s; settling "
+        f"{args.settle} steps")
+    step = SolveSteps(sim)
+    for k in range(args.settle):
+        step()
+        if (k + 1) % 10 == 0:
+            log(f"settle step {k + 1}: {len(sim.contacts)} contacts, {time.perf_counter() - t0:.1f} s")
+    for _ in range(args.warmup):
+        step()
+    wall = 0.0
+    contacts = 0
+    for _ in range(args.steps):
+        wall += step()
+        contacts += len(sim.contacts)
+    batched = len(worlds) * args.steps / wall
+    log(f"batched: {args.steps} steps in {wall:.3f} s of solve, {batched:.4g} world-steps/s")
+
+    # looping: the plain path on each sampled world alone, from the batch's current state
+    rng = scenes.Rng(0x1005)
+    sample = sorted({int(rng.next() * len(worlds)) % len(worlds) for _ in range(args.loop_sample)})
+    sync = device_sync()
+    solver = Solver(64, 256, settings=sim.settings)
+    per_world = []
+    try:
+        for k in sample:
+            lo, hi = sim.world.body_start[k], sim.world.body_start[k + 1]
+            host = HostWorld(worlds[k], sim.settings)
+            bodies = np.ascontiguousarray(sim.bodies[lo:hi])
+            contacts_k, _ = host.collide(bodies.copy(), sim.dt)
+            for timed in (False, True):
+                solver.upload(bodies, contacts_k)
+                sync()
+                t = time.perf_counter()
+                solver.step_native(sim.dt, host.refresh_fn, host.h)
+                sync()
+                if timed:
+                    per_world.append(time.perf_counter() - t)
+            host.close()
+    finally:
+        solver.close()
+    looping = 1.0 / (sum(per_world) / len(per_world))
+    log(f"looping the plain path over {len(sample)} worlds: {1e3 * sum(per_world) / len(per_world):.3f} ms per world-step")
+    return {
+        "metric": "batched world-steps/s",
+        "value": batched,
+        "unit": "world-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded per-world pile recipes, stepped on the GPU with the host narrowphase",
+        "config": {
+            "workload": f"{len(worlds)} independent worlds of log-uniform [8, 1024] bodies ({len(sim.bodies)} "
+                        f"bodies), after {args.settle} drop steps, dt 1/60, 4 substeps; per step: host narrowphase "
+                        f"+ upload (untimed), then one solve block + sleep pass over all worlds (timed)",
+            "worlds": len(worlds), "bodies": len(sim.bodies), "contacts_per_step": contacts / args.steps,
+            "looping_world_steps_per_s": looping, "looping_sample_worlds": len(sample),
+            "batched_over_looping": batched / looping,
+        },
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--workload", choices=("pile", "batched"), default="pile",
+                    help="pile: the 32k-body pile (the headline metric); batched: independent worlds")
+    ap.add_argument("--worlds", type=int, default=4096, help="batched: number of worlds")
+    ap.add_argument("--loop-sample", type=int, default=64, help="batched: worlds timed alone for the looping rate")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=60)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--settle", type=int, default=240, help="steps of drop-and-settle before the timed window")
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (pile 60, batched 10)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps before them (pile 5, batched 2)")
+    ap.add_argument("--settle", type=int, default=None,
+                    help="steps of drop-and-settle before the warm-up (pile 240, batched 60)")
     ap.add_argument("--side", type=int, default=32, help="bodies per side of the grid (32 -> 32,768)")
     ap.add_argument("--profile-steps", type=int, default=10, help="event-timed steps for the roofline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline leg")
@@ -146,6 +269,10 @@ def main():
                     help="first run one device-to-device copy of this many MiB: the reference reading for scaling "
                          "the FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run")
     args = ap.parse_args()
+    pile = args.workload == "pile"
+    args.steps = args.steps if args.steps is not None else (60 if pile else 10)
+    args.warmup = args.warmup if args.warmup is not None else (5 if pile else 2)
+    args.settle = args.settle if args.settle is not None else (240 if pile else 60)
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world_size != 1:
         print("bench.py runs on one GPU only: this build measures the solve path on a single MI355X "
@@ -162,6 +289,10 @@ def main():
         log(f"copied {args.copy_mib} MiB device to device")
         del src, dst
 
+    if args.workload == "batched":
+        print(json.dumps(run_batched(args)), flush=True)
+        return
+
     scene = scenes.pile(side=args.side)
     t0 = time.perf_counter()
     sim = Simulation(scene)
@@ -174,42 +305,10 @@ def main():
                 f"{time.perf_counter() - t0:.1f} s")
     bodies0 = sim.solver.bodies()
     substeps = int(sim.settings["substeps"])
-    try:
-        import torch
-
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except ImportError:
-        sync = lambda: None  # noqa: E731
-
-    transfer = [0.0]
-
-    def solve_step():
-        """Host narrowphase and upload (untimed), then the solve step on resident inputs (timed). The
-        upload of the contacts (and of the bodies, when the narrowphase woke some) and the download of
-        both after the step are timed apart, for the rate with PCIe transfers included."""
-        sim.bodies = sim.solver.bodies()
-        contacts, woke = sim.world.collide(sim.bodies, sim.dt, sim.contacts)
-        t = time.perf_counter()
-        if woke:
-            sim.solver.upload(sim.bodies, contacts)
-        else:
-            sim.solver.upload_contacts(contacts)
-        transfer[0] += time.perf_counter() - t
-        sync()
-        t = time.perf_counter()
-        sim.solver.step_native(sim.dt, sim.world.refresh_fn, sim.world.h)
-        sync()
-        spent = time.perf_counter() - t
-        t = time.perf_counter()
-        sim.contacts = sim.solver.contacts()
-        sim.bodies = sim.solver.bodies()
-        transfer[0] += time.perf_counter() - t
-        sim.steps += 1
-        return spent
-
+    solve_step = SolveSteps(sim)
     for _ in range(args.warmup):
         solve_step()
-    transfer[0] = 0.0
+    solve_step.transfer = 0.0
     its = 0
     solve_ms = 0.0
     wall = 0.0
@@ -220,7 +319,7 @@ def main():
         its += iterations(counts, substeps)
         solve_ms += sim.solver.last_ms()[0]
         contacts += len(sim.contacts)
-    transfers = transfer[0]
+    transfers = solve_step.transfer
     sim.solver.set_profiling(True)
     for _ in range(args.profile_steps):
         solve_step()

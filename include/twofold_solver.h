@@ -192,6 +192,20 @@ int tfs_upload_bodies(tfs_ctx* ctx, const tfs_body* bodies, uint32_t n);
 int tfs_upload_contacts(tfs_ctx* ctx, const tfs_contact* contacts, uint32_t n);
 
 /*
+ * Batched worlds (no counterpart in the reference, which steps one World at a time): declares that
+ * the resident bodies form n_worlds independent worlds, world w owning bodies
+ * [body_start[w], body_start[w + 1]) (body_start has n_worlds + 1 entries, from 0 to n_bodies).
+ * Contacts are then uploaded world after world, each world's in its own key order, with the global
+ * body indices of the concatenation; a contact may not join two worlds. tfs_step solves every world
+ * as it would solve that world alone, to the bit: worlds share no body, so colours, row order,
+ * islands and the serial rest order stay per world, and a mid-step refresh touches only the worlds
+ * that have contacts to collide again (the refresh callback still sees global indices). Island ids
+ * come back local to their world, as that world alone would have them. Call after
+ * tfs_upload_bodies, which resets to one world; it drops the resident contacts, so upload them after.
+ */
+int tfs_set_worlds(tfs_ctx* ctx, uint32_t n_worlds, const uint32_t* body_start);
+
+/*
  * One step of the path: src/world.cpp:1591-1691 followed by updateSleep (src/world.cpp:1174-1247)
  * over the resident bodies and contacts. `refresh` may be NULL when no refresh is needed (then a
  * step that would need one fails with TFS_E_CALLBACK before changing any state).

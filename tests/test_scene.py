@@ -96,3 +96,23 @@ def test_manifolds_match_reference_shape_on_a_pile():
     assert np.median(cos) > 0.999
     same = np.mean([rk[k]["count"] == mk[k]["count"] for k in found])
     assert same > 0.7
+
+
+def test_groups_keep_worlds_apart():
+    """Two worlds in one host world (same coordinates, separate collision groups) give each world's
+    own contacts, with body indices and keys shifted by the first world's bodies."""
+    worlds = [scenes.stack(6), scenes.hub(3)]
+    alone = []
+    for sc in worlds:
+        w = HostWorld(sc)
+        alone.append(w.collide(w.bodies(), sc.dt)[0])
+    both = HostWorld(worlds)
+    got, _ = both.collide(both.bodies(), worlds[0].dt)
+    off = both.body_start[1]
+    assert len(got) == len(alone[0]) + len(alone[1]) and len(alone[1]) > 0
+    second = alone[1].copy()
+    second["body_a"] += off
+    second["body_b"] += off
+    second["key"] += (np.uint64(off) << np.uint64(32)) + np.uint64(off)
+    want = np.concatenate([alone[0], second])
+    assert want.tobytes() == got.tobytes()

@@ -330,6 +330,9 @@ void boxBox(const Shape& sa, const Pose& a, const Shape& sb, const Pose& b, floa
 struct World {
     tfs_settings settings{};
     int threads = 1;  // host threads of the narrowphase loops
+    // Collision groups: bodies of different groups never collide (independent worlds in one set).
+    std::vector<uint32_t> group;
+    uint32_t groups = 1;
     std::vector<tfs_body> bodies;
     std::vector<Shape> shapes;  // one per body, at the body's origin
     std::vector<uint32_t> planes, statics, movers;
@@ -463,23 +466,27 @@ std::vector<uint64_t> candidates(const World& w, const tfs_body* bodies, float d
     const float inv = 1.0f / cell;
     std::unordered_map<uint64_t, std::vector<uint32_t>> grid;
     grid.reserve(w.movers.size() * 2);
-    auto key = [](int x, int y, int z) {
-        return ((uint64_t)(uint32_t)(x + (1 << 20)) << 42) | ((uint64_t)(uint32_t)(y + (1 << 20)) << 21) |
-               (uint64_t)(uint32_t)(z + (1 << 20));
+    const bool grouped = w.groups > 1;
+    // A cell of one group; with several groups the cell key is mixed with the group (a rare clash only
+    // costs extra candidates, which the group test below drops).
+    auto key = [&](uint32_t body, int x, int y, int z) {
+        const uint64_t k = ((uint64_t)(uint32_t)(x + (1 << 20)) << 42) | ((uint64_t)(uint32_t)(y + (1 << 20)) << 21) |
+                           (uint64_t)(uint32_t)(z + (1 << 20));
+        return grouped ? (k * 0x9E3779B97F4A7C15ull) ^ w.group[body] : k;
     };
     for (uint32_t i : w.movers) {
         const int x = (int)std::floor(lo[i].x * inv), y = (int)std::floor(lo[i].y * inv), z = (int)std::floor(lo[i].z * inv);
-        grid[key(x, y, z)].push_back(i);
+        grid[key(i, x, y, z)].push_back(i);
     }
     for (uint32_t i : w.movers) {
         const int x0 = (int)std::floor(lo[i].x * inv), y0 = (int)std::floor(lo[i].y * inv), z0 = (int)std::floor(lo[i].z * inv);
         for (int dx = -1; dx <= 1; ++dx) {
             for (int dy = -1; dy <= 1; ++dy) {
                 for (int dz = -1; dz <= 1; ++dz) {
-                    auto it = grid.find(key(x0 + dx, y0 + dy, z0 + dz));
+                    auto it = grid.find(key(i, x0 + dx, y0 + dy, z0 + dz));
                     if (it == grid.end()) { continue; }
                     for (uint32_t j : it->second) {
-                        if (j <= i) { continue; }
+                        if (j <= i || (grouped && w.group[j] != w.group[i])) { continue; }
                         if (!moving(bodies[i]) && !moving(bodies[j])) { continue; }
                         if (overlap(lo[i], hi[i], lo[j], hi[j])) { out.push_back(((uint64_t)i << 32) | j); }
                     }
@@ -487,10 +494,13 @@ std::vector<uint64_t> candidates(const World& w, const tfs_body* bodies, float d
             }
         }
     }
+    // movers by group, for the static bodies and planes of each group
+    std::vector<std::vector<uint32_t>> moversOf(w.groups);
+    for (uint32_t i : w.movers) { moversOf[grouped ? w.group[i] : 0].push_back(i); }
     for (uint32_t s : w.statics) {
         V slo, shi;
         aabb(w, bodies[s], s, 0.5f * w.settings.speculative, slo, shi);
-        for (uint32_t i : w.movers) {
+        for (uint32_t i : moversOf[grouped ? w.group[s] : 0]) {
             if (!moving(bodies[i])) { continue; }
             if (overlap(slo, shi, lo[i], hi[i])) { out.push_back(((uint64_t)std::min(s, i) << 32) | std::max(s, i)); }
         }
@@ -499,7 +509,7 @@ std::vector<uint64_t> candidates(const World& w, const tfs_body* bodies, float d
         const Pose pp = poseOf(bodies[p]);
         const V n = pp.ax[2];
         const float off = dot(n, pp.c);
-        for (uint32_t i : w.movers) {
+        for (uint32_t i : moversOf[grouped ? w.group[p] : 0]) {
             if (!moving(bodies[i])) { continue; }
             const V c = (lo[i] + hi[i]) * 0.5f, e = (hi[i] - lo[i]) * 0.5f;
             const float reach = e.x * std::fabs(n.x) + e.y * std::fabs(n.y) + e.z * std::fabs(n.z);
@@ -592,7 +602,17 @@ int tfw_add_body(void* h, int type, const float* pos, const float* rot, const fl
     b.gravity_scale = gravityScale;
     w.bodies.push_back(b);
     w.shapes.push_back(Shape{ -1 });
+    w.group.push_back(0);
     return (int)w.bodies.size() - 1;
+}
+
+// Puts a body in a collision group (default 0); bodies of different groups never collide.
+int tfw_set_group(void* h, uint32_t body, uint32_t group) {
+    World& w = *(World*)h;
+    if (body >= w.bodies.size()) { return -1; }
+    w.group[body] = group;
+    w.groups = std::max(w.groups, group + 1);
+    return 0;
 }
 
 // One shape per body, at its origin: 0 sphere {r}, 1 box {hx, hy, hz}, 4 half-space {}.

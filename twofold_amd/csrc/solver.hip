@@ -1026,13 +1026,27 @@ __global__ void k_writeback(tfs_body* bodies, uint32_t n, const float4* motion) 
 // into their contacts (the refresh carries them), and every contact that is not collided again is
 // rebased onto the bodies' current poses (src/world.cpp:1061-1067). `again` marks the contacts
 // collided again: the fast flags of the step (src/world.cpp:1659).
+// With several worlds resident, only the worlds that have contacts to collide again take part in a
+// refresh (the reference refreshes a world only then): `refreshing` flags them by world, and
+// `bodyWorld` maps a body to its world; both are null for a single world.
+__device__ __forceinline__ bool takesPart(const uint8_t* refreshing, const uint32_t* bodyWorld, uint32_t body) {
+    return !refreshing || refreshing[bodyWorld[body]];
+}
+
+__global__ void k_refresh_worlds(const uint32_t* again, uint32_t n, const tfs_contact* contacts,
+                                 const uint32_t* bodyWorld, uint8_t* refreshing) {
+    for (uint32_t k = gid(); k < n; k += gstride()) { refreshing[bodyWorld[contacts[again[k]].body_a]] = 1; }
+}
+
 __global__ void k_refresh_pre(const uint32_t* active, const DevCounts* cnt, const uint32_t* againFlag,
                               tfs_contact* contacts, Rows rows, const uint32_t* rowOf, const uint8_t* colors,
-                              const float4* motion) {
+                              const float4* motion, const uint8_t* refreshing, const uint32_t* bodyWorld) {
     const uint32_t n = cnt->nActive;
     for (uint32_t k = gid(); k < n; k += gstride()) {
         const uint32_t ci = active[k];
         tfs_contact& c = contacts[ci];
+        // Every coloured row's impulses go back to its contact, in every world: all rows are packed
+        // again after the refresh.
         if (colors[ci] < TFS_COLORS) {
             const uint32_t r = rowOf[ci];
             for (int i = 0; i < c.count; ++i) {
@@ -1045,14 +1059,18 @@ __global__ void k_refresh_pre(const uint32_t* active, const DevCounts* cnt, cons
                 c.points[i].max_normal_impulse = rows.peak[j];
             }
         }
-        if (!againFlag[ci]) { rebaseContact(loadMotion(motion, c.body_a), loadMotion(motion, c.body_b), c); }
+        if (!againFlag[ci] && takesPart(refreshing, bodyWorld, c.body_a)) {
+            rebaseContact(loadMotion(motion, c.body_a), loadMotion(motion, c.body_b), c);
+        }
     }
 }
 
 // Then over the bodies: the inverse mass of awake dynamic bodies (src/world.cpp:1662-1664), and every
 // body's dp, dq and base orientation reset (src/world.cpp:1068-1072).
-__global__ void k_refresh_bodies(tfs_body* bodies, uint32_t n, float4* motion, float4* mass) {
+__global__ void k_refresh_bodies(tfs_body* bodies, uint32_t n, float4* motion, float4* mass,
+                                 const uint8_t* refreshing, const uint32_t* bodyWorld) {
     for (uint32_t i = gid(); i < n; i += gstride()) {
+        if (!takesPart(refreshing, bodyWorld, i)) { continue; }
         tfs_body& b = bodies[i];
         if (b.alive && b.type == TFS_DYNAMIC && b.awake) {
             updateInverseMass(b);
@@ -1217,7 +1235,9 @@ __global__ void k_sleep_jump(uint32_t* parent, uint32_t n) {
     for (uint32_t i = gid(); i < n; i += gstride()) { parent[i] = rootOf(parent, i); }
 }
 
-__global__ void k_island_time(tfs_body* bodies, uint32_t n, const uint32_t* parent, uint32_t* islandTime) {
+// Island ids are the lowest body index of the island; with several worlds, local to the body's world.
+__global__ void k_island_time(tfs_body* bodies, uint32_t n, const uint32_t* parent, uint32_t* islandTime,
+                              const uint32_t* bodyWorld, const uint32_t* worldStart) {
     // Sleep times are never negative, so their bits order as the floats do. A large island fills whole
     // waves: its lanes take their minimum across the wave first and make one atomic between them.
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gstride()) {
@@ -1229,7 +1249,7 @@ __global__ void k_island_time(tfs_body* bodies, uint32_t n, const uint32_t* pare
             if (b.alive && awakeDynamic(b)) {
                 valid = true;
                 r = parent[i];
-                b.island = r;
+                b.island = bodyWorld ? r - worldStart[bodyWorld[i]] : r;
                 memcpy(&t, &b.sleep_time, 4);
             }
         }
@@ -1293,6 +1313,13 @@ struct tfs_ctx {
     std::string error;
     tfs_settings settings{};
     uint32_t nb = 0, nc = 0, capB = 0, capC = 0;
+    // independent worlds (tfs_set_worlds): body -> world, per-world refresh flags, host copy of the map
+    uint32_t nWorlds = 1;
+    uint32_t* bodyWorld = nullptr;
+    uint8_t* refreshing = nullptr;
+    uint32_t* worldStart = nullptr;
+    uint32_t worldCap = 0;
+    std::vector<uint32_t> hostBodyWorld;
     bool haveBodies = false, haveContacts = false;
     float solveMs = 0.0f, sleepMs = 0.0f;
     // Measurement: HIP events around each row launch, summed per family (0 warm start, 1 solve,
@@ -1412,7 +1439,7 @@ int allocBodies(tfs_ctx* ctx, uint32_t cap) {
         (r = grow(ctx, ctx->struck, cap)) || (r = grow(ctx, ctx->deg, cap)) || (r = grow(ctx, ctx->epStart, cap)) ||
         (r = grow(ctx, ctx->cursor, cap)) || (r = grow(ctx, ctx->word, cap)) || (r = grow(ctx, ctx->parent, cap)) ||
         (r = grow(ctx, ctx->islandTime, cap)) || (r = grow(ctx, ctx->bdeg, cap)) || (r = grow(ctx, ctx->bstart, cap)) ||
-        (r = grow(ctx, ctx->bcursor, cap))) {
+        (r = grow(ctx, ctx->bcursor, cap)) || (r = grow(ctx, ctx->bodyWorld, cap))) {
         return r;
     }
     ctx->capB = cap;
@@ -1707,9 +1734,18 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
     const uint32_t laned = hc.colorStart[TFS_COLORS], impactLaned = hc.impactColorStart[TFS_COLORS];
     const uint32_t* again = half ? ctx->fast : ctx->fastSolid;
     const uint32_t* againFlag = half ? ctx->fFast : ctx->fFastSolid;
+    const uint8_t* refreshing = nullptr;
+    const uint32_t* bodyWorld = nullptr;
+    if (ctx->nWorlds > 1) {
+        LAUNCH(k_fill_u8, blocksFor(ctx->nWorlds), kThreads, ctx->refreshing, ctx->nWorlds, (uint8_t)0);
+        LAUNCH(k_refresh_worlds, blocksFor(nAgain), kThreads, again, nAgain, ctx->contacts, ctx->bodyWorld,
+               ctx->refreshing);
+        refreshing = ctx->refreshing;
+        bodyWorld = ctx->bodyWorld;
+    }
     LAUNCH(k_refresh_pre, blocksFor(hc.nActive), kThreads, ctx->active, ctx->cnt, againFlag, ctx->contacts, ctx->rows,
-           ctx->rowOf, ctx->colors, ctx->motion);
-    LAUNCH(k_refresh_bodies, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, ctx->mass);
+           ctx->rowOf, ctx->colors, ctx->motion, refreshing, bodyWorld);
+    LAUNCH(k_refresh_bodies, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, ctx->mass, refreshing, bodyWorld);
     // Only the contacts collided again and their bodies go to the host; the callback reads no others.
     // Sized for the capacities at the first refresh, so later steps do not allocate.
     int e;
@@ -1855,7 +1891,9 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         if (!ctx->hostCnt->changed[it - 1]) { break; }
     }
-    LAUNCH(k_island_time, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime);
+    const bool worlds = ctx->nWorlds > 1;
+    LAUNCH(k_island_time, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime,
+           worlds ? ctx->bodyWorld : nullptr, worlds ? ctx->worldStart : nullptr);
     LAUNCH(k_sleep_apply, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime, s.time_to_sleep,
            ctx->motion, ctx->mass);
     HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
@@ -1957,7 +1995,7 @@ void tfs_destroy(tfs_ctx* ctx) {
                      ctx->hist, ctx->status, ctx->cnt, ctx->rows.hdr, ctx->rows.nrm,
                      ctx->rows.tg1, ctx->rows.tg2, ctx->rows.pa, ctx->rows.pb, ctx->rows.imp, ctx->rows.peak,
                      ctx->rows.ax, ctx->impactHdr, ctx->bdeg, ctx->bstart, ctx->bcursor, ctx->bep,
-                     ctx->edges };
+                     ctx->edges, ctx->bodyWorld, ctx->refreshing, ctx->worldStart };
     for (void* p : ptrs) {
         if (p) { (void)hipFree(p); }
     }
@@ -1993,6 +2031,36 @@ int tfs_upload_bodies(tfs_ctx* ctx, const tfs_body* bodies, uint32_t n) {
     if (n) { HIP_TRY(hipMemcpy(ctx->bodies, bodies, sizeof(tfs_body) * n, hipMemcpyHostToDevice)); }
     ctx->nb = n;
     ctx->haveBodies = true;
+    ctx->nWorlds = 1;
+    ctx->hostBodyWorld.clear();
+    return 0;
+}
+
+int tfs_set_worlds(tfs_ctx* ctx, uint32_t n_worlds, const uint32_t* body_start) {
+    if (!ctx || !body_start || n_worlds == 0) { return fail(ctx, TFS_E_ARG, "null argument or no world"); }
+    if (!ctx->haveBodies) { return fail(ctx, TFS_E_STATE, "upload bodies before setting worlds"); }
+    if (body_start[0] != 0 || body_start[n_worlds] != ctx->nb) {
+        return fail(ctx, TFS_E_ARG, "world ranges must cover the %u bodies", ctx->nb);
+    }
+    std::vector<uint32_t> map(ctx->nb);
+    for (uint32_t w = 0; w < n_worlds; ++w) {
+        if (body_start[w + 1] < body_start[w]) { return fail(ctx, TFS_E_ARG, "world %u has a negative range", w); }
+        for (uint32_t i = body_start[w]; i < body_start[w + 1]; ++i) { map[i] = w; }
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (n_worlds > ctx->worldCap) {
+        int r;
+        if ((r = grow(ctx, ctx->refreshing, n_worlds)) || (r = grow(ctx, ctx->worldStart, (size_t)n_worlds + 1))) { return r; }
+        ctx->worldCap = n_worlds;
+    }
+    if (ctx->nb) { HIP_TRY(hipMemcpy(ctx->bodyWorld, map.data(), sizeof(uint32_t) * ctx->nb, hipMemcpyHostToDevice)); }
+    HIP_TRY(hipMemcpy(ctx->worldStart, body_start, sizeof(uint32_t) * ((size_t)n_worlds + 1), hipMemcpyHostToDevice));
+    ctx->nWorlds = n_worlds;
+    ctx->hostBodyWorld.swap(map);
+    // Contacts are uploaded again after this, so that each is checked to stay inside one world.
+    ctx->nc = 0;
+    ctx->haveContacts = false;
     return 0;
 }
 
@@ -2005,6 +2073,9 @@ int tfs_upload_contacts(tfs_ctx* ctx, const tfs_contact* contacts, uint32_t n) {
             return fail(ctx, TFS_E_ARG, "contact %u names body %u/%u of %u", i, c.body_a, c.body_b, ctx->nb);
         }
         if (c.count < 0 || c.count > 4) { return fail(ctx, TFS_E_ARG, "contact %u has %d points", i, c.count); }
+        if (ctx->nWorlds > 1 && ctx->hostBodyWorld[c.body_a] != ctx->hostBodyWorld[c.body_b]) {
+            return fail(ctx, TFS_E_ARG, "contact %u joins bodies of two worlds", i);
+        }
     }
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));

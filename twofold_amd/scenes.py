@@ -146,19 +146,46 @@ def pile(side=32, spacing=1.2, jitter=0.1, walls=True, seed=0x9E3779B9, density=
     rng = Rng(seed)
     s.ground(0.6)
     if walls:
-        h = f32(half_room)
-        t = f32(0.5)
-        for x in (-1, 1):
-            wb = s.body(STATIC, pos=(f32(x) * (h + t), h, 0))
-            s.shape(wb, BOX, (t, h, h + t), friction=0.6)
-        for z in (-1, 1):
-            wb = s.body(STATIC, pos=(0, h, f32(z) * (h + t)))
-            s.shape(wb, BOX, (h + t, h, t), friction=0.6)
+        _walls(s, half_room)
+    _pile_bodies(s, rng, side, side ** 3, spacing, jitter, density, friction_range, restitution_range)
+    return s
+
+
+def batch_world(n, seed, steps=240):
+    """One world of the batched workload: the pile recipe on a ceil(cbrt(n))^3 grid truncated to its
+    first n bodies, inside walls sized to the grid, with one friction U[0.2, 1.0] and one
+    restitution U[0, 0.3] (the world's first two draws) on all its shapes."""
+    side = 1
+    while side ** 3 < n:
+        side += 1
+    s = Scene(f"world-{n}-{seed:08x}", substeps=4, steps=steps)
+    rng = Rng(seed)
+    fr, e = rng.range(0.2, 1.0), rng.range(0.0, 0.3)
+    s.ground(fr, restitution=e)
+    _walls(s, f32(0.6) * f32(side) + f32(1.0), friction=fr, restitution=e)
+    _pile_bodies(s, rng, side, n, 1.2, 0.1, 1000.0, (fr, fr), (e, e))
+    return s
+
+
+def _walls(s, half_room, friction=0.6, restitution=0.0):
+    h = f32(half_room)
+    t = f32(0.5)
+    for x in (-1, 1):
+        wb = s.body(STATIC, pos=(f32(x) * (h + t), h, 0))
+        s.shape(wb, BOX, (t, h, h + t), friction=friction, restitution=restitution)
+    for z in (-1, 1):
+        wb = s.body(STATIC, pos=(0, h, f32(z) * (h + t)))
+        s.shape(wb, BOX, (h + t, h, t), friction=friction, restitution=restitution)
+
+
+def _pile_bodies(s, rng, side, count, spacing, jitter, density, friction_range, restitution_range):
     k = 0
     half = f32(0.5) * f32(side - 1)
     for y in range(side):
         for z in range(side):
             for x in range(side):
+                if k == count:
+                    return
                 jx, jy, jz = rng.range(-jitter, jitter), rng.range(-jitter, jitter), rng.range(-jitter, jitter)
                 pos = ((f32(x) - half) * f32(spacing) + jx, f32(1.0) + f32(spacing) * f32(y) + jy,
                        (f32(z) - half) * f32(spacing) + jz)
@@ -173,7 +200,16 @@ def pile(side=32, spacing=1.2, jitter=0.1, walls=True, seed=0x9E3779B9, density=
                 b = s.body(DYNAMIC, pos=pos, rot=axis_angle(axis, angle))
                 s.shape(b, kind, dims, density=density, friction=fr, restitution=e)
                 k += 1
-    return s
+
+
+def batch_sizes(worlds, seed=0x9E3779B9, lo=8, hi=1024):
+    """Body counts of the batched workload: round(exp(U[ln lo, ln hi])) per world."""
+    rng = Rng(seed)
+    out = []
+    for _ in range(worlds):
+        u = rng.range(math.log(lo), math.log(hi))
+        out.append(int(round(math.exp(float(u)))))
+    return out
 
 
 def stack(n=10, steps=240):

@@ -22,7 +22,7 @@ REFRESH_FN = ctypes.CFUNCTYPE(ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, c
 
 EXPORTS = ("tfs_abi_version", "tfs_sizeof_body", "tfs_sizeof_contact", "tfs_sizeof_settings", "tfs_default_settings",
            "tfs_create", "tfs_destroy", "tfs_last_error", "tfs_set_settings", "tfs_upload_bodies",
-           "tfs_upload_contacts", "tfs_step", "tfs_download_bodies", "tfs_download_contacts", "tfs_download_list",
+           "tfs_upload_contacts", "tfs_set_worlds", "tfs_step", "tfs_download_bodies", "tfs_download_contacts", "tfs_download_list",
            "tfs_last_step_ms", "tfs_last_step_counts", "tfs_set_profiling", "tfs_read_profile")
 
 
@@ -52,6 +52,7 @@ def load_library(path=LIB_PATH):
     lib.tfs_set_settings.argtypes = [_vp, _vp]
     lib.tfs_upload_bodies.argtypes = [_vp, _vp, ctypes.c_uint32]
     lib.tfs_upload_contacts.argtypes = [_vp, _vp, ctypes.c_uint32]
+    lib.tfs_set_worlds.argtypes = [_vp, ctypes.c_uint32, _vp]
     lib.tfs_step.argtypes = [_vp, ctypes.c_float, REFRESH_FN, _vp]
     lib.tfs_download_bodies.argtypes = [_vp, _vp, ctypes.c_uint32]
     lib.tfs_download_contacts.argtypes = [_vp, _vp, ctypes.c_uint32]
@@ -113,6 +114,21 @@ class Solver:
         self._check(self.lib.tfs_upload_bodies(self.h, _ptr(b), len(b)), "tfs_upload_bodies")
         self._check(self.lib.tfs_upload_contacts(self.h, _ptr(c), len(c)), "tfs_upload_contacts")
         self.nb, self.nc = len(b), len(c)
+
+    def set_worlds(self, body_start):
+        """Independent worlds over the resident bodies (tfs_set_worlds); upload contacts after."""
+        bs = np.ascontiguousarray(body_start, np.uint32)
+        self._check(self.lib.tfs_set_worlds(self.h, len(bs) - 1, _ptr(bs)), "tfs_set_worlds")
+        self.nc = 0
+
+    def upload_worlds(self, bodies, body_start, contacts):
+        """Independent worlds: their bodies and contacts concatenated world after world, the contacts'
+        body indices global (tfs_upload_bodies, tfs_set_worlds, tfs_upload_contacts)."""
+        b = np.ascontiguousarray(bodies, BODY_DTYPE)
+        self._check(self.lib.tfs_upload_bodies(self.h, _ptr(b), len(b)), "tfs_upload_bodies")
+        self.nb = len(b)
+        self.set_worlds(body_start)
+        self.upload_contacts(contacts)
 
     def upload_contacts(self, contacts):
         c = np.ascontiguousarray(contacts, CONTACT_DTYPE)

@@ -37,6 +37,8 @@ def _load():
         lib.tfw_add_shape.restype = ctypes.c_int
         lib.tfw_add_shape.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _fp, ctypes.c_int, _fp, _fp, ctypes.c_float,
                                       ctypes.c_float, ctypes.c_float, ctypes.c_float]
+        lib.tfw_set_group.restype = ctypes.c_int
+        lib.tfw_set_group.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32]
         lib.tfw_bodies.restype = ctypes.c_uint32
         lib.tfw_bodies.argtypes = [_vp, _vp, ctypes.c_uint32]
         lib.tfw_collide.restype = ctypes.c_uint32
@@ -51,23 +53,34 @@ def _f(v):
 
 
 class HostWorld:
-    """Bodies and shapes of a scene recipe, with a narrowphase for them."""
+    """Bodies and shapes of a scene recipe, with a narrowphase for them. Given a list of recipes, it
+    holds them as independent worlds, one after the other (`body_start`), each its own collision
+    group."""
 
     def __init__(self, scene, settings=None):
         self.lib = _load()
-        self.settings = settings if settings is not None else _abi.default_settings(substeps=scene.substeps)
+        worlds = list(scene) if isinstance(scene, (list, tuple)) else [scene]
+        first = worlds[0]
+        self.settings = settings if settings is not None else _abi.default_settings(substeps=first.substeps)
         self.h = self.lib.tfw_create(self.settings.ctypes.data_as(_vp))
-        self.dt = scene.dt
-        for b in scene.bodies:
-            self.lib.tfw_add_body(self.h, b["type"], _f(b["pos"]), _f(b["rot"]), _f(b["v"]), _f(b["w"]), b["locks"],
-                                  _f(b["lock_frame"]), 1 if b["awake"] else 0, b["linear_damping"],
-                                  b["angular_damping"], b["gravity_scale"])
-        for sh in scene.shapes:
-            dims = sh["dims"] if len(sh["dims"]) else np.zeros(1, np.float32)
-            if self.lib.tfw_add_shape(self.h, sh["body"], sh["kind"], _f(dims), len(sh["dims"]), _f(sh["pos"]),
-                                      _f(sh["rot"]), sh["density"], sh["friction"], sh["restitution"],
-                                      sh["rolling"]) < 0:
-                raise ValueError("the host world takes one box, sphere or half-space per body, at its origin")
+        self.dt = first.dt
+        self.body_start = [0]
+        for g, sc in enumerate(worlds):
+            base = self.body_start[-1]
+            for b in sc.bodies:
+                i = self.lib.tfw_add_body(self.h, b["type"], _f(b["pos"]), _f(b["rot"]), _f(b["v"]), _f(b["w"]),
+                                          b["locks"], _f(b["lock_frame"]), 1 if b["awake"] else 0,
+                                          b["linear_damping"], b["angular_damping"], b["gravity_scale"])
+                if len(worlds) > 1:
+                    self.lib.tfw_set_group(self.h, i, g)
+            for sh in sc.shapes:
+                dims = sh["dims"] if len(sh["dims"]) else np.zeros(1, np.float32)
+                if self.lib.tfw_add_shape(self.h, base + sh["body"], sh["kind"], _f(dims), len(sh["dims"]),
+                                          _f(sh["pos"]), _f(sh["rot"]), sh["density"], sh["friction"],
+                                          sh["restitution"], sh["rolling"]) < 0:
+                    raise ValueError("the host world takes one box, sphere or half-space per body, at its origin")
+            self.body_start.append(base + len(sc.bodies))
+        self.worlds = len(worlds)
         # the native refresh callback, handed straight to the solver
         self.refresh_fn = REFRESH_FN(("tfw_refresh", self.lib))
 
@@ -102,24 +115,31 @@ class HostWorld:
 
 
 class Simulation:
-    """A scene stepped with the host narrowphase and the GPU solver."""
+    """A scene (or a list of scenes: independent worlds solved together) stepped with the host
+    narrowphase and the GPU solver."""
 
     def __init__(self, scene, settings=None, device=0):
         self.world = HostWorld(scene, settings)
         self.settings = self.world.settings
-        self.dt = scene.dt
+        self.dt = self.world.dt
         self.bodies = self.world.bodies()
         self.contacts = np.zeros(0, CONTACT_DTYPE)
         self.solver = Solver(len(self.bodies), max(4 * len(self.bodies), 64), device=device, settings=self.settings)
-        self.solver.upload(self.bodies, self.contacts)
+        self.upload(self.bodies, self.contacts)
         self.steps = 0
+
+    def upload(self, bodies, contacts):
+        if self.world.worlds > 1:
+            self.solver.upload_worlds(bodies, self.world.body_start, contacts)
+        else:
+            self.solver.upload(bodies, contacts)
 
     def collide(self):
         """The narrowphase for the next step; uploads its contacts (and bodies it woke)."""
         self.bodies = self.solver.bodies() if self.steps else self.bodies
         contacts, woke = self.world.collide(self.bodies, self.dt, self.contacts)
         if woke or not self.steps:
-            self.solver.upload(self.bodies, contacts)
+            self.upload(self.bodies, contacts)
         else:
             self.solver.upload_contacts(contacts)
         self.contacts = contacts
