@@ -155,16 +155,15 @@ typedef struct tfs_settings {
  * then prepare, nearest-point carry and impulse carry of the new points) and asks the host for the
  * new manifolds through this callback.
  *
- * On entry, contacts[again[k]] holds the contact before the refresh and bodies[] the current state
- * of the bodies those contacts join (the arrays are indexed as uploaded; other entries are not
- * current and must not be read). For each k the callback writes fresh[k]: normal, tangent1, tangent2, count and
+ * On entry, for k < n_again, again[k] is the index (as uploaded) of the k-th contact to collide
+ * again, again_contacts[k] that contact as it stands before the refresh, and again_bodies[2k] and
+ * again_bodies[2k + 1] the current state of its bodies a and b (all three arrays are compact, in the
+ * order of `again`). For each k the callback writes fresh[k]: normal, tangent1, tangent2, count and
  * per point local_a, separation, anchor_a, anchor_b (other fields are ignored); count 0 drops the
  * points. `remaining` is the time left in the step, (substeps - i) * h. Returns 0 or an error.
  */
-typedef int (*tfs_refresh_fn)(void* user, int substep, float remaining, const uint32_t* again,
-                              uint32_t n_again, const tfs_body* bodies, uint32_t n_bodies,
-                              const tfs_contact* contacts, uint32_t n_contacts,
-                              tfs_contact* fresh);
+typedef int (*tfs_refresh_fn)(void* user, int substep, float remaining, uint32_t n_again, const uint32_t* again,
+                              const tfs_contact* again_contacts, const tfs_body* again_bodies, tfs_contact* fresh);
 
 typedef struct tfs_ctx tfs_ctx;
 

@@ -128,10 +128,6 @@ static Q4 integrate(Q4 q, V3 w, float h) {                                /* mat
     const Q4 d = qmul(quat(w.x, w.y, w.z, 0.0f), q);
     return qnormalize(quat(q.x + 0.5f * h * d.x, q.y + 0.5f * h * d.y, q.z + 0.5f * h * d.z, q.w + 0.5f * h * d.w));
 }
-static V3 perpendicular(V3 n) {                                           /* math.h:70-73 */
-    if (fabsf(n.x) > 0.57735f) { return normalize(vec(n.y, -n.x, 0.0f)); }
-    return normalize(vec(0.0f, n.z, -n.y));
-}
 
 /* ---- bodies ---------------------------------------------------------------------------------- */
 
@@ -884,9 +880,25 @@ static int refreshContacts(World* w, const uint32_t* active, uint32_t na, const 
         stq(b->dq, quat(0, 0, 0, 1));
         memcpy(b->q_base, b->q, sizeof b->q_base);
     }
-    tfs_contact* fresh = (tfs_contact*)calloc(nagain, sizeof(tfs_contact));
-    if (!fresh) { return TFS_E_ARG; }
-    if (cb(user, substep, remaining, again, nagain, w->bodies, w->nb, w->contacts, w->nc, fresh) != 0) {
+    /* the callback's compact view: each contact to collide again and its two bodies */
+    tfs_contact* fresh = (tfs_contact*)calloc(nagain ? nagain : 1, sizeof(tfs_contact));
+    tfs_contact* pairs = (tfs_contact*)malloc((nagain ? nagain : 1) * sizeof(tfs_contact));
+    tfs_body* ends = (tfs_body*)malloc((nagain ? 2 * nagain : 1) * sizeof(tfs_body));
+    if (!fresh || !pairs || !ends) {
+        free(fresh);
+        free(pairs);
+        free(ends);
+        return TFS_E_ARG;
+    }
+    for (uint32_t k = 0; k < nagain; ++k) {
+        pairs[k] = w->contacts[again[k]];
+        ends[2 * k] = w->bodies[pairs[k].body_a];
+        ends[2 * k + 1] = w->bodies[pairs[k].body_b];
+    }
+    const int rc = cb(user, substep, remaining, nagain, again, pairs, ends, fresh);
+    free(pairs);
+    free(ends);
+    if (rc != 0) {
         free(fresh);
         return TFS_E_CALLBACK;
     }

@@ -187,8 +187,8 @@ class RefWorld:
 
 # ---- the oracle -------------------------------------------------------------------------------
 
-REFRESH_FN = ctypes.CFUNCTYPE(ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, ctypes.POINTER(ctypes.c_uint32),
-                              ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp)
+REFRESH_FN = ctypes.CFUNCTYPE(ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, ctypes.c_uint32,
+                              ctypes.POINTER(ctypes.c_uint32), _vp, _vp, _vp)
 
 
 class _Lists(ctypes.Structure):
@@ -224,7 +224,7 @@ def recorded_refresh(refreshes):
     """A refresh callback that hands back the manifolds the reference's narrowphase found."""
     by_sub = {r["substep"]: r for r in refreshes}
 
-    def cb(user, substep, remaining, again, n_again, bodies, n_bodies, contacts, n_contacts, fresh):
+    def cb(user, substep, remaining, n_again, again, contacts, bodies, fresh):
         r = by_sub.get(substep)
         if r is None or len(r["again"]) != n_again:
             return -1

@@ -31,7 +31,7 @@ def concatenate(recs):
     contacts = np.concatenate(parts)
     cs = np.cumsum([0] + [len(r.pre_contacts) for r in recs]).astype(np.int64)
 
-    def refresh(substep, remaining, again, b, c):
+    def refresh(substep, remaining, again, c, b):
         want, fresh = [], []
         for r, off in zip(recs, cs[:-1]):
             for x in r.refreshes:

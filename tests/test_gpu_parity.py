@@ -20,7 +20,7 @@ pytestmark = pytest.mark.gpu
 def recorded(refreshes):
     by = {r["substep"]: r for r in refreshes}
 
-    def fresh(substep, remaining, again, bodies, contacts):
+    def fresh(substep, remaining, again, contacts, bodies):
         r = by[substep]
         assert np.array_equal(again, r["again"]), "the GPU asked to refresh other contacts"
         return r["fresh"]

@@ -344,9 +344,7 @@ float travelOf(const tfs_body& b, float dt) {
 }
 
 // One manifold for the pair (a < b), or none.
-bool collide(const World& w, const tfs_body* bodies, uint32_t ia, uint32_t ib, float dt, Manifold& m) {
-    const tfs_body& a = bodies[ia];
-    const tfs_body& b = bodies[ib];
+bool collide(const World& w, uint32_t ia, const tfs_body& a, uint32_t ib, const tfs_body& b, float dt, Manifold& m) {
     const float travel =
         (len(ld(b.v) - ld(a.v)) + len(ld(a.w)) * a.extent + len(ld(b.w)) * b.extent) * dt;
     const float margin = w.settings.speculative + travel;
@@ -372,10 +370,9 @@ bool collide(const World& w, const tfs_body* bodies, uint32_t ia, uint32_t ib, f
 }
 
 // The contact of pair (ia, ib) from a manifold (src/world.cpp:920-950 conventions).
-void toContact(const World& w, const tfs_body* bodies, uint32_t ia, uint32_t ib, const Manifold& m, tfs_contact& c) {
+void toContact(const World& w, uint32_t ia, const tfs_body& a, uint32_t ib, const tfs_body& b, const Manifold& m,
+               tfs_contact& c) {
     std::memset(&c, 0, sizeof c);
-    const tfs_body& a = bodies[ia];
-    const tfs_body& b = bodies[ib];
     c.key = ((uint64_t)ia << 32) | ib;
     c.body_a = ia;
     c.body_b = ib;
@@ -672,9 +669,9 @@ uint32_t tfw_collide(void* h, tfs_body* bodies, uint32_t nb, float dt, const tfs
     for (long k = 0; k < (long)pairs.size(); ++k) {
         const uint32_t ia = (uint32_t)(pairs[(size_t)k] >> 32), ib = (uint32_t)pairs[(size_t)k];
         Manifold m;
-        if (!collide(w, bodies, ia, ib, dt, m)) { continue; }
+        if (!collide(w, ia, bodies[ia], ib, bodies[ib], dt, m)) { continue; }
         tfs_contact& c = found[(size_t)k];
-        toContact(w, bodies, ia, ib, m, c);
+        toContact(w, ia, bodies[ia], ib, bodies[ib], m, c);
         const tfs_contact* it = std::lower_bound(prev, prev + nprev, c.key,
                                                  [](const tfs_contact& x, uint64_t key) { return x.key < key; });
         if (it != prev + nprev && it->key == c.key) { carry(*it, c, matchSq); }
@@ -702,16 +699,18 @@ uint32_t tfw_collide(void* h, tfs_body* bodies, uint32_t nb, float dt, const tfs
 }
 
 // tfs_refresh_fn: collides the listed contacts' pairs again at the bodies' current poses.
-int tfw_refresh(void* h, int, float remaining, const uint32_t* again, uint32_t n, const tfs_body* bodies, uint32_t,
-                const tfs_contact* contacts, uint32_t, tfs_contact* fresh) {
+int tfw_refresh(void* h, int, float remaining, uint32_t n, const uint32_t*, const tfs_contact* contacts,
+                const tfs_body* bodies, tfs_contact* fresh) {
     World& w = *(World*)h;
 #pragma omp parallel for schedule(dynamic, 64) num_threads(w.threads)
     for (long k = 0; k < (long)n; ++k) {
-        const tfs_contact& c = contacts[again[k]];
+        const tfs_contact& c = contacts[k];
+        const tfs_body& a = bodies[2 * k];
+        const tfs_body& b = bodies[2 * k + 1];
         Manifold m;
         tfs_contact& f = fresh[k];
         std::memset(&f, 0, sizeof f);
-        if (collide(w, bodies, c.body_a, c.body_b, remaining, m)) { toContact(w, bodies, c.body_a, c.body_b, m, f); }
+        if (collide(w, c.body_a, a, c.body_b, b, remaining, m)) { toContact(w, c.body_a, a, c.body_b, b, m, f); }
     }
     return 0;
 }

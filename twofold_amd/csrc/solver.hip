@@ -1083,7 +1083,8 @@ __global__ void k_refresh_bodies(tfs_body* bodies, uint32_t n, float4* motion, f
 }
 
 // The contacts to collide again and their two bodies (velocities and travel from the motion records),
-// one wave per contact copying word by word, straight into the host's arrays at their own indices.
+// one wave per contact copying word by word, straight into compact host arrays in the list's order:
+// hostC[k] and hostB[2k], hostB[2k + 1].
 __global__ void k_gather_again(const uint32_t* again, uint32_t n, const tfs_contact* contacts, const tfs_body* bodies,
                                const float4* motion, uint32_t* hostAgain, tfs_contact* hostC, tfs_body* hostB) {
     static_assert(sizeof(tfs_contact) % 4 == 0 && sizeof(tfs_body) % 4 == 0, "word copies");
@@ -1095,14 +1096,14 @@ __global__ void k_gather_again(const uint32_t* again, uint32_t n, const tfs_cont
         const uint32_t ci = again[k];
         if (lane == 0) { hostAgain[k] = ci; }
         const uint32_t* src = (const uint32_t*)(contacts + ci);
-        uint32_t* dst = (uint32_t*)(hostC + ci);
+        uint32_t* dst = (uint32_t*)(hostC + k);
         for (uint32_t d = lane; d < cw; d += 64) { dst[d] = src[d]; }
         const uint32_t ids[2] = { contacts[ci].body_a, contacts[ci].body_b };
         for (int e = 0; e < 2; ++e) {
             const uint32_t i = ids[e];
             const uint32_t* bs = (const uint32_t*)(bodies + i);
             const uint32_t* ms = (const uint32_t*)(motion + 4 * (size_t)i);
-            uint32_t* bd = (uint32_t*)(hostB + i);
+            uint32_t* bd = (uint32_t*)(hostB + 2 * (size_t)k + e);
             for (uint32_t d = lane; d < bw; d += 64) {
                 uint32_t x = bs[d];
                 if (d >= ov && d < ov + 3) {
@@ -1388,13 +1389,13 @@ struct tfs_ctx {
     uint32_t* bep = nullptr;
     DevCounts* cnt = nullptr;
     DevCounts* hostCnt = nullptr;
-    // pinned host arrays of the refresh callback: bodies and contacts at their own indices (only the
-    // entries of the contacts collided again are written), the list, and the fresh manifolds
+    // pinned host arrays of the refresh callback, compact in the order of the list: the list, its
+    // contacts, their bodies (two per contact), and the fresh manifolds
     tfs_body* pinBodies = nullptr;
     tfs_contact* pinContacts = nullptr;
     uint32_t* pinAgain = nullptr;
     tfs_contact* pinFresh = nullptr;
-    uint32_t pinBodiesCap = 0, pinContactsCap = 0;
+    uint32_t pinCap = 0;
 };
 
 namespace {
@@ -1730,7 +1731,7 @@ int growPinned(tfs_ctx* ctx, T*& p, size_t n) {
 
 int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaining, tfs_refresh_fn cb, void* user,
             const DevCounts& hc) {
-    const uint32_t nb = ctx->nb, nc = ctx->nc;
+    const uint32_t nb = ctx->nb;
     const uint32_t laned = hc.colorStart[TFS_COLORS], impactLaned = hc.impactColorStart[TFS_COLORS];
     const uint32_t* again = half ? ctx->fast : ctx->fastSolid;
     const uint32_t* againFlag = half ? ctx->fFast : ctx->fFastSolid;
@@ -1747,18 +1748,15 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
            ctx->rowOf, ctx->colors, ctx->motion, refreshing, bodyWorld);
     LAUNCH(k_refresh_bodies, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, ctx->mass, refreshing, bodyWorld);
     // Only the contacts collided again and their bodies go to the host; the callback reads no others.
-    // Sized for the capacities at the first refresh, so later steps do not allocate.
+    // Grown by half again whenever a longer list comes, so that steady stepping does not allocate.
     int e;
-    if (ctx->capB > ctx->pinBodiesCap) {
-        if ((e = growPinned(ctx, ctx->pinBodies, ctx->capB))) { return e; }
-        ctx->pinBodiesCap = ctx->capB;
-    }
-    if (ctx->capC > ctx->pinContactsCap) {
-        if ((e = growPinned(ctx, ctx->pinContacts, ctx->capC)) || (e = growPinned(ctx, ctx->pinAgain, ctx->capC)) ||
-            (e = growPinned(ctx, ctx->pinFresh, ctx->capC))) {
+    if (nAgain > ctx->pinCap) {
+        const uint32_t cap = std::max<uint32_t>(std::max<uint32_t>(nAgain, ctx->pinCap + ctx->pinCap / 2), 1024u);
+        if ((e = growPinned(ctx, ctx->pinAgain, cap)) || (e = growPinned(ctx, ctx->pinContacts, cap)) ||
+            (e = growPinned(ctx, ctx->pinBodies, 2 * (size_t)cap)) || (e = growPinned(ctx, ctx->pinFresh, cap))) {
             return e;
         }
-        ctx->pinContactsCap = ctx->capC;
+        ctx->pinCap = cap;
     }
     const uint32_t waves = std::min<uint32_t>(nAgain, 4u * 8192u);
     LAUNCH(k_gather_again, (waves + 3) / 4, kThreads, again, nAgain, ctx->contacts, ctx->bodies, ctx->motion,
@@ -1767,7 +1765,7 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
     const Clock::time_point t0 = Clock::now();
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     const Clock::time_point t1 = Clock::now();
-    if (cb(user, substep, remaining, ctx->pinAgain, nAgain, ctx->pinBodies, nb, ctx->pinContacts, nc, ctx->pinFresh) != 0) {
+    if (cb(user, substep, remaining, nAgain, ctx->pinAgain, ctx->pinContacts, ctx->pinBodies, ctx->pinFresh) != 0) {
         return fail(ctx, TFS_E_CALLBACK, "the refresh callback failed at substep %d", substep);
     }
     if (ctx->profiling) {

@@ -17,8 +17,8 @@ from ._abi import BODY_DTYPE, CONTACT_DTYPE, SETTINGS_DTYPE
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtwofold_hip.so")
 
 _vp = ctypes.c_void_p
-REFRESH_FN = ctypes.CFUNCTYPE(ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, ctypes.POINTER(ctypes.c_uint32),
-                              ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp)
+REFRESH_FN = ctypes.CFUNCTYPE(ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, ctypes.c_uint32,
+                              ctypes.POINTER(ctypes.c_uint32), _vp, _vp, _vp)
 
 EXPORTS = ("tfs_abi_version", "tfs_sizeof_body", "tfs_sizeof_contact", "tfs_sizeof_settings", "tfs_default_settings",
            "tfs_create", "tfs_destroy", "tfs_last_error", "tfs_set_settings", "tfs_upload_bodies",
@@ -140,19 +140,20 @@ class Solver:
         self._check(self.lib.tfs_step(self.h, ctypes.c_float(dt), fn, user), "tfs_step")
 
     def step(self, dt, refresh=None):
-        """One step. `refresh(substep, remaining, again, bodies, contacts) -> fresh contacts` is called
-        when fast contacts need new manifolds mid-step."""
+        """One step. `refresh(substep, remaining, again, contacts, bodies) -> fresh contacts` is called
+        when fast contacts need new manifolds mid-step: `again` the contacts' indices, `contacts` those
+        contacts and `bodies` their two bodies each (2k, 2k + 1), as tfs_refresh_fn hands them."""
         if refresh is None:
             cb = REFRESH_FN()
         else:
-            def cb_impl(user, substep, remaining, again, n_again, bodies, n_bodies, contacts, n_contacts, fresh):
+            def cb_impl(user, substep, remaining, n_again, again, contacts, bodies, fresh):
                 try:
                     ag = np.ctypeslib.as_array(again, shape=(n_again,)).copy()
-                    bd = np.frombuffer((ctypes.c_char * (n_bodies * BODY_DTYPE.itemsize)).from_address(bodies),
-                                       BODY_DTYPE).copy()
-                    ct = np.frombuffer((ctypes.c_char * (n_contacts * CONTACT_DTYPE.itemsize)).from_address(contacts),
+                    ct = np.frombuffer((ctypes.c_char * (n_again * CONTACT_DTYPE.itemsize)).from_address(contacts),
                                        CONTACT_DTYPE).copy()
-                    out = np.ascontiguousarray(refresh(substep, remaining, ag, bd, ct), CONTACT_DTYPE)
+                    bd = np.frombuffer((ctypes.c_char * (2 * n_again * BODY_DTYPE.itemsize)).from_address(bodies),
+                                       BODY_DTYPE).copy()
+                    out = np.ascontiguousarray(refresh(substep, remaining, ag, ct, bd), CONTACT_DTYPE)
                     if len(out) != n_again:
                         return -1
                     ctypes.memmove(fresh, out.ctypes.data, out.nbytes)
