@@ -252,8 +252,9 @@ def run_batched(args):
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--workload", choices=("pile", "batched"), default="pile",
-                    help="pile: the 32k-body pile (the headline metric); batched: independent worlds")
+    ap.add_argument("--workload", choices=("pile", "batched", "planar"), default="pile",
+                    help="pile: the 32k-body pile (the headline metric); batched: independent worlds; planar: 65,536 "
+                         "bodies locked to a plane with free spheres")
     ap.add_argument("--worlds", type=int, default=4096, help="batched: number of worlds")
     ap.add_argument("--loop-sample", type=int, default=64, help="batched: worlds timed alone for the looping rate")
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,9 +271,10 @@ def main():
                          "the FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run")
     args = ap.parse_args()
     pile = args.workload == "pile"
-    args.steps = args.steps if args.steps is not None else (60 if pile else 10)
+    planar = args.workload == "planar"
+    args.steps = args.steps if args.steps is not None else (60 if pile else 10 if not planar else 20)
     args.warmup = args.warmup if args.warmup is not None else (5 if pile else 2)
-    args.settle = args.settle if args.settle is not None else (240 if pile else 60)
+    args.settle = args.settle if args.settle is not None else (240 if pile else 60 if not planar else 120)
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world_size != 1:
         print("bench.py runs on one GPU only: this build measures the solve path on a single MI355X "
@@ -293,10 +295,10 @@ def main():
         print(json.dumps(run_batched(args)), flush=True)
         return
 
-    scene = scenes.pile(side=args.side)
+    scene = scenes.planar_pile() if planar else scenes.pile(side=args.side)
     t0 = time.perf_counter()
     sim = Simulation(scene)
-    log(f"pile of {len(sim.bodies)} bodies built in {time.perf_counter() - t0:.1f} s; settling {args.settle} steps")
+    log(f"{scene.name}: {len(sim.bodies)} bodies built in {time.perf_counter() - t0:.1f} s; settling {args.settle} steps")
     for k in range(args.settle):
         sim.step()
         if (k + 1) % 20 == 0:
@@ -348,10 +350,12 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: seeded pile recipe, settled on the GPU with the host narrowphase",
+        "data": f"synthetic: seeded {scene.name} recipe, stepped on the GPU with the host narrowphase",
         "config": {
-            "workload": f"settled {len(bodies0)}-body box/sphere pile in walls (after {args.settle} drop steps), "
-                        f"dt 1/60, {substeps} substeps; per step: host narrowphase + upload (untimed), then one "
+            "workload": (f"{len(bodies0)} bodies: 65,536 planar boxes locked to the xy plane and 4,096 free spheres in "
+                         f"a U container (after {args.settle} drop steps)" if planar else
+                         f"settled {len(bodies0)}-body box/sphere pile in walls (after {args.settle} drop steps)")
+                        + f", dt 1/60, {substeps} substeps; per step: host narrowphase + upload (untimed), then one "
                         f"solve block + sleep pass on the GPU (timed)",
             "bodies": len(bodies0), "contacts_per_step": contacts / args.steps, "active_points": counts["points"],
             "impact_points": counts["impact_points"], "colors": counts["colors"], "rest": counts["rest"],
@@ -361,7 +365,8 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": K_PASS_TRAFFIC_BYTES if args.side == 32 else None, "traffic_source": K_PASS_TRAFFIC_SOURCE,
+            "traffic": K_PASS_TRAFFIC_BYTES if pile and args.side == 32 else None,
+            "traffic_source": K_PASS_TRAFFIC_SOURCE if pile and args.side == 32 else None,
             "kernel": "k_pass<bias|relax> (one cooperative launch per solve pass)",
             "launches": launches, "profiled_steps": args.profile_steps,
             "bytes_per_launch": algo / max(launches, 1), "ms_per_launch": ms / max(launches, 1),

@@ -69,7 +69,8 @@ def run_against_reference(solver, scene, steps=None):
 
 @needs_ref
 @pytest.mark.parametrize("make", [scenes.pyramid, scenes.stack, scenes.bounce, scenes.rolling, scenes.planar_mix,
-                                  scenes.tumble, scenes.hub], ids=lambda f: f.__name__)
+                                  scenes.tumble, scenes.hub, scenes.planar_pile_small],
+                         ids=lambda f: f.__name__)
 def test_gpu_matches_reference_every_step(solver, make):
     scene = make()
     print(f"reference run {scene.name}", flush=True)

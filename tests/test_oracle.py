@@ -23,7 +23,8 @@ needs_ref = pytest.mark.skipif(not refworld.ref_available(), reason="reference c
 
 @needs_ref
 @pytest.mark.parametrize("make", [scenes.pyramid, scenes.stack, scenes.bounce, scenes.rolling, scenes.planar_mix,
-                                  scenes.tumble, scenes.hub], ids=lambda f: f.__name__)
+                                  scenes.tumble, scenes.hub, scenes.planar_pile_small],
+                         ids=lambda f: f.__name__)
 def test_oracle_matches_reference_every_step(make):
     scene = make()
     w = refworld.RefWorld(scene)

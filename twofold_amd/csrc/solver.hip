@@ -48,6 +48,8 @@ struct DevCounts {
     uint32_t error;
     uint32_t anyRestitution;
     uint32_t ticket;
+    uint32_t restTicket[2];   // level dataflow of the active and impact rest lists
+    uint32_t restLevels[2];
     uint32_t scanTicket[kScanSlots];
     uint32_t pad_;
     unsigned long long barrier;
@@ -947,29 +949,150 @@ __global__ void k_warm_bodies(uint32_t nb, const uint32_t* __restrict__ deg, con
     }
 }
 
-// Contacts outside the colours, one at a time in list order, as solveContacts, warmStart and
-// applyRestitution take them (src/solver.cpp:319-460).
-__global__ void k_rest(const uint32_t* list, uint32_t n, int mode, tfs_contact* contacts, const tfs_body* bodies,
-                       float4* motion, const float4* mass, Ctx ctx) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) { return; }
-    for (uint32_t k = 0; k < n; ++k) {
-        tfs_contact& c = contacts[list[k]];
-        const BodyView va = viewOf(bodies[c.body_a]), vb = viewOf(bodies[c.body_b]);
-        Motion a = loadMotion(motion, c.body_a), b = loadMotion(motion, c.body_b);
-        const Mass ma = loadMass(mass, c.body_a), mb = loadMass(mass, c.body_b);
-        if (mode == kWarm) {
-            warmRest(c, va, vb, ma, mb, a, b);
-        } else if (mode == kRestitute) {
-            restituteRest(c, va, vb, ma, mb, a, b, ctx);
-        } else {
-            solveRest(c, va, vb, ma, mb, a, b, ctx, mode == kBias);
-        }
-        storeVelocity(motion, c.body_a, a);
-        storeVelocity(motion, c.body_b, b);
+// ---- the rest list in levels ------------------------------------------------------------------------
+//
+// Contacts outside the colours are solved one at a time in list order (solveContacts, warmStart,
+// applyRestitution: src/solver.cpp:319-460). A contact changes the velocities of its moving bodies
+// only, except that rolling resistance (resistTurning) writes both bodies whatever they are. So a
+// body is tracked when it moves or a rolling contact of the list touches it, and two contacts that
+// share no tracked body commute. Each contact's level is 1 + the highest level among the earlier
+// contacts of the list that share a tracked body with it; solving level after level, the contacts
+// of one level in parallel, gives every tracked body the same updates in the same order as the list
+// does, to the bit.
+
+struct RestScratch {
+    uint8_t* tracked;
+    uint32_t *deg, *start, *cursor, *ep, *rank, *level, *count;
+    unsigned long long* word;
+};
+
+__device__ __forceinline__ bool rollingContact(const tfs_contact& c) { return c.rolling_resistance > 0.0f; }
+
+__global__ void k_rest_track(uint32_t nb, const uint8_t* moving, RestScratch z) {
+    for (uint32_t i = gid(); i < nb; i += gstride()) {
+        z.tracked[i] = moving[i];
+        z.deg[i] = z.cursor[i] = 0;
+        z.word[i] = 0ull;
     }
 }
 
-// ---- bodies per substep ------------------------------------------------------------------------
+__global__ void k_rest_roll(const uint32_t* list, uint32_t n, const tfs_contact* contacts, RestScratch z) {
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        const tfs_contact& c = contacts[list[k]];
+        if (rollingContact(c)) { z.tracked[c.body_a] = z.tracked[c.body_b] = 1; }
+        z.count[k] = 0;
+    }
+}
+
+__global__ void k_rest_ep_count(const uint32_t* list, uint32_t n, const tfs_contact* contacts, RestScratch z) {
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        const tfs_contact& c = contacts[list[k]];
+        if (z.tracked[c.body_a]) { atomicAdd(&z.deg[c.body_a], 1u); }
+        if (z.tracked[c.body_b]) { atomicAdd(&z.deg[c.body_b], 1u); }
+    }
+}
+
+__global__ void k_rest_ep_fill(const uint32_t* list, uint32_t n, const tfs_contact* contacts, RestScratch z) {
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        const tfs_contact& c = contacts[list[k]];
+        if (z.tracked[c.body_a]) { z.ep[z.start[c.body_a] + atomicAdd(&z.cursor[c.body_a], 1u)] = k << 1; }
+        if (z.tracked[c.body_b]) { z.ep[z.start[c.body_b] + atomicAdd(&z.cursor[c.body_b], 1u)] = (k << 1) | 1u; }
+    }
+}
+
+// The levels, as the colouring's dataflow: a contact proceeds once its tracked bodies have seen all
+// their earlier contacts; each body's word is (contacts seen << 32) | its highest level so far.
+__global__ void k_rest_level(const uint32_t* list, uint32_t n, const tfs_contact* contacts, RestScratch z,
+                             uint32_t* ticket, uint32_t* levels, DevCounts* cnt) {
+    __shared__ uint32_t chunk;
+    for (;;) {
+        if (threadIdx.x == 0) { chunk = atomicAdd(ticket, 1u); }
+        __syncthreads();
+        const uint32_t k = chunk * blockDim.x + threadIdx.x;
+        if (chunk * blockDim.x >= n) { break; }
+        if (k < n) {
+            const tfs_contact& c = contacts[list[k]];
+            const uint32_t a = c.body_a, b = c.body_b;
+            const bool ta = z.tracked[a], tb = z.tracked[b];
+            const uint32_t ra = ta ? z.rank[k << 1] : 0u, rb = tb ? z.rank[(k << 1) | 1u] : 0u;
+            bool done = false;
+            uint32_t spins = 0;
+            while (!done) {
+                const unsigned long long wa = ta ? loadWord(&z.word[a]) : 0ull;
+                const unsigned long long wb = tb ? loadWord(&z.word[b]) : 0ull;
+                if ((!ta || (uint32_t)(wa >> 32) == ra) && (!tb || (uint32_t)(wb >> 32) == rb)) {
+                    const uint32_t lv = 1u + max((uint32_t)wa, (uint32_t)wb);
+                    if (ta) { storeWord(&z.word[a], ((unsigned long long)(ra + 1) << 32) | lv); }
+                    if (tb) { storeWord(&z.word[b], ((unsigned long long)(rb + 1) << 32) | lv); }
+                    z.level[k] = lv - 1u;
+                    atomicAdd(&z.count[lv - 1u], 1u);
+                    atomicMax(levels, lv);
+                    done = true;
+                } else if (++spins > kSpinLimit) {
+                    atomicOr(&cnt->error, 8u);
+                    z.level[k] = 0u;
+                    done = true;
+                } else {
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Contacts of the list grouped by level (any order within a level: its contacts commute).
+__global__ void k_rest_place(uint32_t n, RestScratch z, const uint32_t* levelStart, uint32_t* order) {
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        const uint32_t lv = z.level[k];
+        order[levelStart[lv] + atomicSub(&z.count[lv], 1u) - 1u] = k;
+    }
+}
+
+// The rest list level after level in one cooperative launch; velocities cross blocks through
+// coherent accesses, as in k_pass.
+template <int MODE>
+__global__ void __launch_bounds__(kPassThreads) k_rest_levels(const uint32_t* list, const uint32_t* order,
+                                                              const uint32_t* levelStart, uint32_t levels,
+                                                              tfs_contact* contacts, const tfs_body* bodies,
+                                                              float4* motion, const float4* mass, Ctx ctx,
+                                                              unsigned long long* counter, unsigned long long base,
+                                                              DevCounts* cnt) {
+    using X = Access<true>;
+    unsigned long long phase = base;
+    for (uint32_t lv = 0; lv < levels; ++lv) {
+        for (uint32_t i = levelStart[lv] + gid(); i < levelStart[lv + 1]; i += gstride()) {
+            tfs_contact& c = contacts[list[order[i]]];
+            const uint32_t ia = c.body_a, ib = c.body_b;
+            const tfs_body& ba = bodies[ia];
+            const tfs_body& bb = bodies[ib];
+            const BodyView va = viewOf(ba), vb = viewOf(bb);
+            const float4 v0 = X::ld(&motion[4 * ia]), w0 = X::ld(&motion[4 * ia + 1]);
+            const float4 v1 = X::ld(&motion[4 * ib]), w1 = X::ld(&motion[4 * ib + 1]);
+            const float4 p0 = motion[4 * ia + 2], q0 = motion[4 * ia + 3], p1 = motion[4 * ib + 2], q1 = motion[4 * ib + 3];
+            Motion a{ vec(v0.x, v0.y, v0.z), vec(w0.x, w0.y, w0.z), vec(p0.x, p0.y, p0.z), quat(q0.x, q0.y, q0.z, q0.w) };
+            Motion b{ vec(v1.x, v1.y, v1.z), vec(w1.x, w1.y, w1.z), vec(p1.x, p1.y, p1.z), quat(q1.x, q1.y, q1.z, q1.w) };
+            const Mass ma = loadMass(mass, ia), mb = loadMass(mass, ib);
+            if (MODE == kWarm) {
+                warmRest(c, va, vb, ma, mb, a, b);
+            } else if (MODE == kRestitute) {
+                restituteRest(c, va, vb, ma, mb, a, b, ctx);
+            } else {
+                solveRest(c, va, vb, ma, mb, a, b, ctx, MODE == kBias);
+            }
+            const bool rolling = rollingContact(c);
+            if (rolling || movingBody(ba)) {
+                X::st(&motion[4 * ia], make_float4(a.v.x, a.v.y, a.v.z, 0.0f));
+                X::st(&motion[4 * ia + 1], make_float4(a.w.x, a.w.y, a.w.z, 0.0f));
+            }
+            if (rolling || movingBody(bb)) {
+                X::st(&motion[4 * ib], make_float4(b.v.x, b.v.y, b.v.z, 0.0f));
+                X::st(&motion[4 * ib + 1], make_float4(b.w.x, b.w.y, b.w.z, 0.0f));
+            }
+        }
+        if (lv + 1 < levels) { gridSync(counter, phase, cnt); }
+    }
+}
 
 __global__ void k_integrate_velocities(tfs_body* bodies, uint32_t n, float4* motion, Ctx ctx) {
     for (uint32_t i = gid(); i < n; i += gstride()) {
@@ -1366,6 +1489,14 @@ struct tfs_ctx {
     uint32_t* rowsList = nullptr;
     uint32_t* impactRowsList = nullptr;
     uint2* edges = nullptr;
+    // the rest lists in levels (active rest, impact rest): scratch, and per list its order and level starts
+    uint8_t* restTracked = nullptr;
+    uint32_t *restDeg = nullptr, *restStart = nullptr, *restCursor = nullptr, *restEp = nullptr, *restRank = nullptr,
+             *restLevel = nullptr, *restCount = nullptr;
+    unsigned long long* restWord = nullptr;
+    uint32_t* restOrder[2] = {};
+    uint32_t* restLevelStart[2] = {};
+    uint32_t restLevels[2] = {};
     uint8_t* colors = nullptr;
     uint32_t* ep = nullptr;
     uint32_t* rank = nullptr;
@@ -1440,7 +1571,10 @@ int allocBodies(tfs_ctx* ctx, uint32_t cap) {
         (r = grow(ctx, ctx->struck, cap)) || (r = grow(ctx, ctx->deg, cap)) || (r = grow(ctx, ctx->epStart, cap)) ||
         (r = grow(ctx, ctx->cursor, cap)) || (r = grow(ctx, ctx->word, cap)) || (r = grow(ctx, ctx->parent, cap)) ||
         (r = grow(ctx, ctx->islandTime, cap)) || (r = grow(ctx, ctx->bdeg, cap)) || (r = grow(ctx, ctx->bstart, cap)) ||
-        (r = grow(ctx, ctx->bcursor, cap)) || (r = grow(ctx, ctx->bodyWorld, cap))) {
+        (r = grow(ctx, ctx->bcursor, cap)) || (r = grow(ctx, ctx->bodyWorld, cap)) ||
+        (r = grow(ctx, ctx->restTracked, cap)) || (r = grow(ctx, ctx->restDeg, cap)) ||
+        (r = grow(ctx, ctx->restStart, cap)) || (r = grow(ctx, ctx->restCursor, cap)) ||
+        (r = grow(ctx, ctx->restWord, cap))) {
         return r;
     }
     ctx->capB = cap;
@@ -1468,7 +1602,12 @@ int allocContacts(tfs_ctx* ctx, uint32_t cap) {
         (r = grow(ctx, ctx->impact, cap)) || (r = grow(ctx, ctx->rowsList, cap)) ||
         (r = grow(ctx, ctx->impactRowsList, cap)) || (r = grow(ctx, ctx->colors, cap)) ||
         (r = grow(ctx, ctx->ep, 2 * (size_t)cap)) || (r = grow(ctx, ctx->rank, 2 * (size_t)cap)) ||
-        (r = grow(ctx, ctx->rowOf, cap)) || (r = grow(ctx, ctx->edges, cap))) {
+        (r = grow(ctx, ctx->rowOf, cap)) || (r = grow(ctx, ctx->edges, cap)) ||
+        (r = grow(ctx, ctx->restEp, 2 * (size_t)cap)) || (r = grow(ctx, ctx->restRank, 2 * (size_t)cap)) ||
+        (r = grow(ctx, ctx->restLevel, cap)) || (r = grow(ctx, ctx->restCount, cap)) ||
+        (r = grow(ctx, ctx->restOrder[0], cap)) || (r = grow(ctx, ctx->restOrder[1], cap)) ||
+        (r = grow(ctx, ctx->restLevelStart[0], (size_t)cap + 1)) ||
+        (r = grow(ctx, ctx->restLevelStart[1], (size_t)cap + 1))) {
         return r;
     }
     if ((r = allocRows(ctx, ctx->rows, cap)) || (r = grow(ctx, ctx->impactHdr, cap)) || (r = grow(ctx, ctx->bep, 2 * (size_t)cap))) {
@@ -1708,9 +1847,51 @@ int collectProfile(tfs_ctx* ctx) {
     return 0;
 }
 
+// The levels of a rest list (list[0..n), the active or the impact rest), built once per step.
+int buildRestLevels(tfs_ctx* ctx, int which, const uint32_t* list, uint32_t n) {
+    if (n == 0) { return 0; }
+    const uint32_t nb = ctx->nb;
+    const RestScratch z{ ctx->restTracked, ctx->restDeg, ctx->restStart, ctx->restCursor, ctx->restEp, ctx->restRank,
+                         ctx->restLevel, ctx->restCount, ctx->restWord };
+    int r;
+    LAUNCH(k_rest_track, blocksFor(nb), kThreads, nb, ctx->moving, z);
+    LAUNCH(k_rest_roll, blocksFor(n), kThreads, list, n, ctx->contacts, z);
+    LAUNCH(k_rest_ep_count, blocksFor(n), kThreads, list, n, ctx->contacts, z);
+    if ((r = scan(ctx, z.deg, nullptr, nb, z.start, nullptr, &ctx->cnt->scanTotal))) { return r; }
+    LAUNCH(k_rest_ep_fill, blocksFor(n), kThreads, list, n, ctx->contacts, z);
+    LAUNCH(k_ep_rank, blocksFor(nb), kThreads, nb, z.deg, z.start, z.ep, z.rank);
+    LAUNCH(k_rest_level, blocksFor(n, kThreads, 2048), kThreads, list, n, ctx->contacts, z, &ctx->cnt->restTicket[which],
+           &ctx->cnt->restLevels[which], ctx->cnt);
+    // level starts: the exclusive prefix of the level sizes, and n after the last level
+    if ((r = scan(ctx, z.count, nullptr, n, ctx->restLevelStart[which], nullptr, ctx->restLevelStart[which] + n))) {
+        return r;
+    }
+    LAUNCH(k_rest_place, blocksFor(n), kThreads, n, z, ctx->restLevelStart[which], ctx->restOrder[which]);
+    return 0;
+}
+
+// A rest pass: the list's levels in one cooperative launch.
 int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32_t end, const Ctx& c) {
     if (end <= begin) { return 0; }
-    LAUNCH(k_rest, 1, 64, list + begin, end - begin, mode, ctx->contacts, ctx->bodies, ctx->motion, ctx->mass, c);
+    const int which = list == ctx->impactRowsList ? 1 : 0;
+    const uint32_t levels = ctx->restLevels[which];
+    if (levels == 0) { return fail(ctx, TFS_E_STATE, "rest list without levels"); }
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((end - begin + kPassThreads - 1) / kPassThreads,
+                                                                   ctx->maxPassBlocks));
+    const uint32_t* l = list + begin;
+    const uint32_t* order = ctx->restOrder[which];
+    const uint32_t* starts = ctx->restLevelStart[which];
+    unsigned long long* counter = &ctx->cnt->barrier;
+    const unsigned long long base = ctx->barrierBase;
+    const float4* mass = ctx->mass;
+    void* args[] = { (void*)&l, (void*)&order, (void*)&starts, (void*)&levels, &ctx->contacts, &ctx->bodies, &ctx->motion,
+                     (void*)&mass, (void*)&c, &counter, (void*)&base, &ctx->cnt };
+    const void* fn = mode == kWarm ? (const void*)k_rest_levels<kWarm>
+                     : mode == kBias ? (const void*)k_rest_levels<kBias>
+                     : mode == kRelax ? (const void*)k_rest_levels<kRelax>
+                                      : (const void*)k_rest_levels<kRestitute>;
+    HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kPassThreads), args, 0, ctx->stream));
+    ctx->barrierBase += (unsigned long long)(levels - 1) * grid;
     return 0;
 }
 
@@ -1839,6 +2020,20 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
         if (needs) { return fail(ctx, TFS_E_CALLBACK, "fast contacts need a refresh callback"); }
     }
     if ((r = packRows(ctx, laned, impactLaned, true))) { return r; }
+    ctx->restLevels[0] = ctx->restLevels[1] = 0;
+    if (hc.nActive > laned || hc.nImpact > impactLaned) {
+        if ((r = buildRestLevels(ctx, 0, ctx->rowsList + laned, hc.nActive - laned)) ||
+            (r = buildRestLevels(ctx, 1, ctx->impactRowsList + impactLaned, hc.nImpact - impactLaned))) {
+            return r;
+        }
+        HIP_TRY(hipMemcpyAsync(ctx->hostCnt->restLevels, ctx->cnt->restLevels, sizeof(ctx->cnt->restLevels),
+                               hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (ctx->hostCnt->error) { return fail(ctx, TFS_E_KERNEL, "a kernel's bounded wait ran out (error %u)", ctx->hostCnt->error); }
+        ctx->restLevels[0] = ctx->hostCnt->restLevels[0];
+        ctx->restLevels[1] = ctx->hostCnt->restLevels[1];
+    }
 
     // substeps: src/world.cpp:1658-1681
     for (int i = 0; i < n; ++i) {
@@ -1993,7 +2188,10 @@ void tfs_destroy(tfs_ctx* ctx) {
                      ctx->hist, ctx->status, ctx->cnt, ctx->rows.hdr, ctx->rows.nrm,
                      ctx->rows.tg1, ctx->rows.tg2, ctx->rows.pa, ctx->rows.pb, ctx->rows.imp, ctx->rows.peak,
                      ctx->rows.ax, ctx->impactHdr, ctx->bdeg, ctx->bstart, ctx->bcursor, ctx->bep,
-                     ctx->edges, ctx->bodyWorld, ctx->refreshing, ctx->worldStart };
+                     ctx->edges, ctx->bodyWorld, ctx->refreshing, ctx->worldStart, ctx->restTracked,
+                     ctx->restDeg, ctx->restStart, ctx->restCursor, ctx->restEp, ctx->restRank, ctx->restLevel,
+                     ctx->restCount, ctx->restWord, ctx->restOrder[0], ctx->restOrder[1], ctx->restLevelStart[0],
+                     ctx->restLevelStart[1] };
     for (void* p : ptrs) {
         if (p) { (void)hipFree(p); }
     }

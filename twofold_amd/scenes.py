@@ -297,3 +297,38 @@ def hub(n=7, steps=90):
             b = s.body(DYNAMIC, pos=(x, 0.7, z))
             s.shape(b, SPHERE, (0.2,), density=1000.0, friction=0.5)
     return s
+
+
+def planar_pile(side=256, spheres=4096, spacing=1.3, seed=0x9E3779B9, steps=240):
+    """The planar path at scale: side x side bodies locked to the xy plane (LockPlanar, z = 0) on a
+    grid in x-y from y = 1, inside a static U container (half-space floor, two walls of z half-depth
+    5 m), with free spheres (radius 0.3) dropped above them. Every planar body is a box with
+    half-extents (U[0.2, 0.6], U[0.2, 0.6], 0.5) turned about z by U[0, pi]; its friction is 0.6.
+    Draws per planar body, rows bottom-up: half-extent x, y, angle; per sphere: x, y offset, z."""
+    s = Scene(f"planar-{side * side}", substeps=8, steps=steps)
+    rng = Rng(seed)
+    s.ground(0.6)
+    half = f32(0.5) * f32(side - 1)
+    wall_x = (half + f32(1.0)) * f32(spacing)
+    top = f32(1.0) + f32(spacing) * f32(side)
+    for x in (-1, 1):
+        wb = s.body(STATIC, pos=(f32(x) * (wall_x + f32(0.5)), top * f32(0.5), 0))
+        s.shape(wb, BOX, (0.5, top * f32(0.5) + f32(4.0), 5.0), friction=0.6)
+    for j in range(side):
+        for i in range(side):
+            hx, hy, angle = rng.range(0.2, 0.6), rng.range(0.2, 0.6), rng.range(0, PI)
+            b = s.body(DYNAMIC, pos=((f32(i) - half) * f32(spacing), f32(1.0) + f32(spacing) * f32(j), 0),
+                       rot=axis_angle((0, 0, 1), angle), locks=LOCK_PLANAR)
+            s.shape(b, BOX, (hx, hy, 0.5), density=1000.0, friction=0.6)
+    for k in range(spheres):
+        x = rng.range(-wall_x + f32(1.0), wall_x - f32(1.0))
+        y = top + f32(2.0) + rng.range(0.0, 64.0)
+        z = rng.range(-0.3, 0.3)
+        b = s.body(DYNAMIC, pos=(x, y, z))
+        s.shape(b, SPHERE, (0.3,), density=1000.0, friction=0.6)
+    return s
+
+
+def planar_pile_small(steps=150):
+    """A 10 x 10 planar pile with 12 spheres: a rest list of up to ~160 planar contacts."""
+    return planar_pile(side=10, spheres=12, steps=steps)
