@@ -359,6 +359,7 @@ def main():
                         f"solve block + sleep pass on the GPU (timed)",
             "bodies": len(bodies0), "contacts_per_step": contacts / args.steps, "active_points": counts["points"],
             "impact_points": counts["impact_points"], "colors": counts["colors"], "rest": counts["rest"],
+            "rest_levels": counts["rest_levels"], "impact_rest_levels": counts["impact_rest_levels"],
             "world_steps_per_s": args.steps / wall, "gpu_solve_ms_per_step": solve_ms / args.steps,
             "transfer_ms_per_step": 1e3 * transfers / args.steps,
             "value_with_transfers": its / (wall + transfers),

@@ -243,8 +243,9 @@ int tfs_set_profiling(tfs_ctx* ctx, int on);
 int tfs_read_profile(tfs_ctx* ctx, int family, uint64_t* launches, double* ms, double* bytes);
 
 /* Counts of the last step: active contacts, their points, impact contacts and their points,
- * colours used, rest contacts. Constraint-iterations per step follow from these. */
-int tfs_last_step_counts(tfs_ctx* ctx, uint32_t counts[6]);
+ * colours used, rest contacts, and the dependency levels of the active and of the impact rest list
+ * (the rest is solved level after level). Constraint-iterations per step follow from these. */
+int tfs_last_step_counts(tfs_ctx* ctx, uint32_t counts[8]);
 
 #ifdef __cplusplus
 }

@@ -1041,29 +1041,50 @@ __global__ void k_rest_level(const uint32_t* list, uint32_t n, const tfs_contact
     }
 }
 
-// Contacts of the list grouped by level (any order within a level: its contacts commute).
-__global__ void k_rest_place(uint32_t n, RestScratch z, const uint32_t* levelStart, uint32_t* order) {
+// Contacts of the list grouped by level (any order within a level: its contacts commute), each entry
+// (contact, body a, body b, which bodies it writes: 1 a, 2 b).
+__global__ void k_rest_place(const uint32_t* list, uint32_t n, const tfs_contact* contacts, const uint8_t* moving,
+                             RestScratch z, const uint32_t* levelStart, uint4* entries) {
     for (uint32_t k = gid(); k < n; k += gstride()) {
         const uint32_t lv = z.level[k];
-        order[levelStart[lv] + atomicSub(&z.count[lv], 1u) - 1u] = k;
+        const uint32_t ci = list[k];
+        const tfs_contact& c = contacts[ci];
+        const bool rolling = rollingContact(c);
+        const uint32_t writes = (rolling || moving[c.body_a] ? 1u : 0u) | (rolling || moving[c.body_b] ? 2u : 0u);
+        entries[levelStart[lv] + atomicSub(&z.count[lv], 1u) - 1u] = make_uint4(ci, c.body_a, c.body_b, writes);
     }
 }
 
-// The rest list level after level in one cooperative launch; velocities cross blocks through
-// coherent accesses, as in k_pass.
-template <int MODE>
-__global__ void __launch_bounds__(kPassThreads) k_rest_levels(const uint32_t* list, const uint32_t* order,
-                                                              const uint32_t* levelStart, uint32_t levels,
-                                                              tfs_contact* contacts, const tfs_body* bodies,
-                                                              float4* motion, const float4* mass, Ctx ctx,
-                                                              unsigned long long* counter, unsigned long long base,
-                                                              DevCounts* cnt) {
-    using X = Access<true>;
+// The rest list level after level. ONE_BLOCK: one work-group walks the levels with __syncthreads
+// between them (narrow levels: the block barrier is cheap and one CU's accesses need no coherence
+// protocol); otherwise one cooperative launch over many blocks, velocities crossing blocks through
+// coherent accesses as in k_pass. Each thread loads its next entry at the start of a level. A contact
+// is touched by one thread only in a launch, so plain accesses serve for it.
+constexpr int kRestBlockThreads = 512;
+
+template <int MODE, bool ONE_BLOCK>
+__global__ void __launch_bounds__(ONE_BLOCK ? kRestBlockThreads : kPassThreads)
+    k_rest_levels(const uint4* entries, const uint32_t* levelStart, uint32_t levels, tfs_contact* contacts,
+                  const tfs_body* bodies, float4* motion, const float4* mass, Ctx ctx, unsigned long long* counter,
+                  unsigned long long base, DevCounts* cnt) {
+    using X = Access<!ONE_BLOCK>;
     unsigned long long phase = base;
+    const uint32_t g = ONE_BLOCK ? threadIdx.x : gid();
+    const uint32_t stride = ONE_BLOCK ? blockDim.x : gstride();
+    uint32_t lo = levelStart[0], hi = levels > 0 ? levelStart[1] : 0u;
+    uint4 next = lo + g < hi ? entries[lo + g] : make_uint4(0u, 0u, 0u, 0u);
     for (uint32_t lv = 0; lv < levels; ++lv) {
-        for (uint32_t i = levelStart[lv] + gid(); i < levelStart[lv + 1]; i += gstride()) {
-            tfs_contact& c = contacts[list[order[i]]];
-            const uint32_t ia = c.body_a, ib = c.body_b;
+        const uint32_t first = lo + g, end = hi;
+        const uint4 firstEntry = next;
+        if (lv + 1 < levels) {
+            lo = end;
+            hi = levelStart[lv + 2];
+            if (lo + g < hi) { next = entries[lo + g]; }
+        }
+        for (uint32_t i = first; i < end; i += stride) {
+            const uint4 e = i == first ? firstEntry : entries[i];
+            tfs_contact& c = contacts[e.x];
+            const uint32_t ia = e.y, ib = e.z;
             const tfs_body& ba = bodies[ia];
             const tfs_body& bb = bodies[ib];
             const BodyView va = viewOf(ba), vb = viewOf(bb);
@@ -1080,17 +1101,22 @@ __global__ void __launch_bounds__(kPassThreads) k_rest_levels(const uint32_t* li
             } else {
                 solveRest(c, va, vb, ma, mb, a, b, ctx, MODE == kBias);
             }
-            const bool rolling = rollingContact(c);
-            if (rolling || movingBody(ba)) {
+            if (e.w & 1u) {
                 X::st(&motion[4 * ia], make_float4(a.v.x, a.v.y, a.v.z, 0.0f));
                 X::st(&motion[4 * ia + 1], make_float4(a.w.x, a.w.y, a.w.z, 0.0f));
             }
-            if (rolling || movingBody(bb)) {
+            if (e.w & 2u) {
                 X::st(&motion[4 * ib], make_float4(b.v.x, b.v.y, b.v.z, 0.0f));
                 X::st(&motion[4 * ib + 1], make_float4(b.w.x, b.w.y, b.w.z, 0.0f));
             }
         }
-        if (lv + 1 < levels) { gridSync(counter, phase, cnt); }
+        if (lv + 1 < levels) {
+            if (ONE_BLOCK) {
+                __syncthreads();
+            } else {
+                gridSync(counter, phase, cnt);
+            }
+        }
     }
 }
 
@@ -1460,7 +1486,7 @@ struct tfs_ctx {
     // refresh's contacts to reach the host, 4 the refresh callback (host clock)
     uint64_t profLaunches[kProfileFamilies] = {};
     double profMs[kProfileFamilies] = {}, profBytes[kProfileFamilies] = {};
-    uint32_t lastCounts[6] = {};
+    uint32_t lastCounts[8] = {};
 
     // bodies
     tfs_body* bodies = nullptr;
@@ -1494,7 +1520,7 @@ struct tfs_ctx {
     uint32_t *restDeg = nullptr, *restStart = nullptr, *restCursor = nullptr, *restEp = nullptr, *restRank = nullptr,
              *restLevel = nullptr, *restCount = nullptr;
     unsigned long long* restWord = nullptr;
-    uint32_t* restOrder[2] = {};
+    uint4* restEntries[2] = {};
     uint32_t* restLevelStart[2] = {};
     uint32_t restLevels[2] = {};
     uint8_t* colors = nullptr;
@@ -1605,7 +1631,7 @@ int allocContacts(tfs_ctx* ctx, uint32_t cap) {
         (r = grow(ctx, ctx->rowOf, cap)) || (r = grow(ctx, ctx->edges, cap)) ||
         (r = grow(ctx, ctx->restEp, 2 * (size_t)cap)) || (r = grow(ctx, ctx->restRank, 2 * (size_t)cap)) ||
         (r = grow(ctx, ctx->restLevel, cap)) || (r = grow(ctx, ctx->restCount, cap)) ||
-        (r = grow(ctx, ctx->restOrder[0], cap)) || (r = grow(ctx, ctx->restOrder[1], cap)) ||
+        (r = grow(ctx, ctx->restEntries[0], cap)) || (r = grow(ctx, ctx->restEntries[1], cap)) ||
         (r = grow(ctx, ctx->restLevelStart[0], (size_t)cap + 1)) ||
         (r = grow(ctx, ctx->restLevelStart[1], (size_t)cap + 1))) {
         return r;
@@ -1866,30 +1892,42 @@ int buildRestLevels(tfs_ctx* ctx, int which, const uint32_t* list, uint32_t n) {
     if ((r = scan(ctx, z.count, nullptr, n, ctx->restLevelStart[which], nullptr, ctx->restLevelStart[which] + n))) {
         return r;
     }
-    LAUNCH(k_rest_place, blocksFor(n), kThreads, n, z, ctx->restLevelStart[which], ctx->restOrder[which]);
+    LAUNCH(k_rest_place, blocksFor(n), kThreads, list, n, ctx->contacts, ctx->moving, z, ctx->restLevelStart[which],
+           ctx->restEntries[which]);
     return 0;
 }
 
-// A rest pass: the list's levels in one cooperative launch.
+// A rest pass: the list's levels, in one work-group when they are narrow (on average under a block of
+// threads per level), else in one cooperative launch.
 int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32_t end, const Ctx& c) {
     if (end <= begin) { return 0; }
     const int which = list == ctx->impactRowsList ? 1 : 0;
-    const uint32_t levels = ctx->restLevels[which];
+    const uint32_t levels = ctx->restLevels[which], n = end - begin;
     if (levels == 0) { return fail(ctx, TFS_E_STATE, "rest list without levels"); }
-    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((end - begin + kPassThreads - 1) / kPassThreads,
-                                                                   ctx->maxPassBlocks));
-    const uint32_t* l = list + begin;
-    const uint32_t* order = ctx->restOrder[which];
+    const bool oneBlock = n / levels < (uint32_t)kRestBlockThreads;
+    const uint32_t grid = oneBlock ? 1u
+                                   : std::max<uint32_t>(1, std::min<uint32_t>((n + kPassThreads - 1) / kPassThreads,
+                                                                              ctx->maxPassBlocks));
+    const uint4* entries = ctx->restEntries[which];
     const uint32_t* starts = ctx->restLevelStart[which];
     unsigned long long* counter = &ctx->cnt->barrier;
     const unsigned long long base = ctx->barrierBase;
     const float4* mass = ctx->mass;
-    void* args[] = { (void*)&l, (void*)&order, (void*)&starts, (void*)&levels, &ctx->contacts, &ctx->bodies, &ctx->motion,
+    if (oneBlock) {
+        switch (mode) {
+            case kWarm: LAUNCH((k_rest_levels<kWarm, true>), 1, kRestBlockThreads, entries, starts, levels, ctx->contacts, ctx->bodies, ctx->motion, mass, c, counter, base, ctx->cnt); break;
+            case kBias: LAUNCH((k_rest_levels<kBias, true>), 1, kRestBlockThreads, entries, starts, levels, ctx->contacts, ctx->bodies, ctx->motion, mass, c, counter, base, ctx->cnt); break;
+            case kRelax: LAUNCH((k_rest_levels<kRelax, true>), 1, kRestBlockThreads, entries, starts, levels, ctx->contacts, ctx->bodies, ctx->motion, mass, c, counter, base, ctx->cnt); break;
+            default: LAUNCH((k_rest_levels<kRestitute, true>), 1, kRestBlockThreads, entries, starts, levels, ctx->contacts, ctx->bodies, ctx->motion, mass, c, counter, base, ctx->cnt); break;
+        }
+        return 0;
+    }
+    void* args[] = { (void*)&entries, (void*)&starts, (void*)&levels, &ctx->contacts, &ctx->bodies, &ctx->motion,
                      (void*)&mass, (void*)&c, &counter, (void*)&base, &ctx->cnt };
-    const void* fn = mode == kWarm ? (const void*)k_rest_levels<kWarm>
-                     : mode == kBias ? (const void*)k_rest_levels<kBias>
-                     : mode == kRelax ? (const void*)k_rest_levels<kRelax>
-                                      : (const void*)k_rest_levels<kRestitute>;
+    const void* fn = mode == kWarm ? (const void*)k_rest_levels<kWarm, false>
+                     : mode == kBias ? (const void*)k_rest_levels<kBias, false>
+                     : mode == kRelax ? (const void*)k_rest_levels<kRelax, false>
+                                      : (const void*)k_rest_levels<kRestitute, false>;
     HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kPassThreads), args, 0, ctx->stream));
     ctx->barrierBase += (unsigned long long)(levels - 1) * grid;
     return 0;
@@ -2100,7 +2138,8 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     for (int col = 0; col < TFS_COLORS; ++col) {
         if (hc.colorStart[col + 1] > hc.colorStart[col]) { nColors = (uint32_t)col + 1; }
     }
-    const uint32_t counts[6] = { hc.nActive, hc.pointsActive, hc.nImpact, hc.pointsImpact, nColors, hc.nActive - laned };
+    const uint32_t counts[8] = { hc.nActive, hc.pointsActive, hc.nImpact, hc.pointsImpact, nColors, hc.nActive - laned,
+                                  ctx->restLevels[0], ctx->restLevels[1] };
     memcpy(ctx->lastCounts, counts, sizeof counts);
     return 0;
 }
@@ -2190,7 +2229,7 @@ void tfs_destroy(tfs_ctx* ctx) {
                      ctx->rows.ax, ctx->impactHdr, ctx->bdeg, ctx->bstart, ctx->bcursor, ctx->bep,
                      ctx->edges, ctx->bodyWorld, ctx->refreshing, ctx->worldStart, ctx->restTracked,
                      ctx->restDeg, ctx->restStart, ctx->restCursor, ctx->restEp, ctx->restRank, ctx->restLevel,
-                     ctx->restCount, ctx->restWord, ctx->restOrder[0], ctx->restOrder[1], ctx->restLevelStart[0],
+                     ctx->restCount, ctx->restWord, ctx->restEntries[0], ctx->restEntries[1], ctx->restLevelStart[0],
                      ctx->restLevelStart[1] };
     for (void* p : ptrs) {
         if (p) { (void)hipFree(p); }
@@ -2363,7 +2402,7 @@ int tfs_read_profile(tfs_ctx* ctx, int family, uint64_t* launches, double* ms, d
     return 0;
 }
 
-int tfs_last_step_counts(tfs_ctx* ctx, uint32_t counts[6]) {
+int tfs_last_step_counts(tfs_ctx* ctx, uint32_t counts[8]) {
     if (!ctx || !counts) { return TFS_E_ARG; }
     memcpy(counts, ctx->lastCounts, sizeof ctx->lastCounts);
     return 0;

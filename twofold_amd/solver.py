@@ -210,7 +210,9 @@ class Solver:
         return n.value, ms.value, by.value
 
     def counts(self):
-        """active contacts, their points, impact contacts, their points, colours used, rest contacts."""
-        out = np.zeros(6, np.uint32)
+        """active contacts, their points, impact contacts, their points, colours used, rest contacts, and
+        the dependency levels of the active and the impact rest list."""
+        out = np.zeros(8, np.uint32)
         self._check(self.lib.tfs_last_step_counts(self.h, _ptr(out)), "tfs_last_step_counts")
-        return dict(zip(("active", "points", "impact", "impact_points", "colors", "rest"), out.tolist()))
+        return dict(zip(("active", "points", "impact", "impact_points", "colors", "rest", "rest_levels",
+                         "impact_rest_levels"), out.tolist()))
