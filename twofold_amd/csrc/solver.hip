@@ -311,7 +311,7 @@ __global__ void k_classify(const tfs_contact* contacts, uint32_t nc, const tfs_b
 
 // prepareContacts (src/solver.cpp:308-313) and the marking half of impactContacts
 // (src/world.cpp:1029-1041).
-__global__ void k_prepare(const uint32_t* active, const DevCounts* cnt, tfs_contact* contacts, const tfs_body* bodies,
+__global__ void __launch_bounds__(kThreads) k_prepare(const uint32_t* active, const DevCounts* cnt, tfs_contact* contacts, const tfs_body* bodies,
                           const float4* motion, const float4* mass, float threshold, uint8_t* struck,
                           uint32_t* points, uint32_t* anyRestitution) {
     const uint32_t n = cnt->nActive;
@@ -326,11 +326,11 @@ __global__ void k_prepare(const uint32_t* active, const DevCounts* cnt, tfs_cont
         contacts[ci] = c;
         sum += (uint32_t)c.count;
         if (c.restitution != 0.0f && c.count > 0) { bounce = 1u; }
-        for (int i = 0; i < c.count; ++i) {
-            if (c.points[i].relative_velocity >= -threshold) { continue; }
+        bool hit = false;
+        TF_POINTS(i, c.count) { hit = hit || !(c.points[i].relative_velocity >= -threshold); }
+        if (hit) {
             if (ta.type == TFS_DYNAMIC) { struck[c.body_a] = 1; }
             if (tb.type == TFS_DYNAMIC) { struck[c.body_b] = 1; }
-            break;
         }
     }
     waveAdd(points, sum);
@@ -597,7 +597,7 @@ struct LoadedAxes {
 // Packs the coloured contacts into rows in row order and builds their point axes from the bodies'
 // current masses, as prepareLanes does (src/lanes.cpp:66-110, 409-411).
 // The contact's points are read before anything is written, so one row waits for memory once.
-__global__ void k_pack(const uint32_t* __restrict__ list, uint32_t n, const tfs_contact* __restrict__ contacts,
+__global__ void __launch_bounds__(kThreads) k_pack(const uint32_t* __restrict__ list, uint32_t n, const tfs_contact* __restrict__ contacts,
                        const uint8_t* __restrict__ moving, const float4* __restrict__ mass, Rows rows,
                        uint32_t* __restrict__ rowOf) {
     for (uint32_t r = gid(); r < n; r += gstride()) {
@@ -625,6 +625,7 @@ __global__ void k_pack(const uint32_t* __restrict__ list, uint32_t n, const tfs_
             rows.pb[j] = make_float4(p.anchor_b[0], p.anchor_b[1], p.anchor_b[2], p.relative_velocity);
             rows.imp[j] = make_float4(p.normal_impulse, p.tangent_impulse[0], p.tangent_impulse[1], p.tangent_impulse[2]);
             rows.peak[j] = p.max_normal_impulse;
+#pragma unroll
             for (int d = 0; d < 3; ++d) { storeAxis(rows, r, k, d, makeAxis(ma, mb, ld3(p.anchor_a), ld3(p.anchor_b), dir[d])); }
         }
         rowOf[ci] = r;
@@ -901,7 +902,7 @@ __global__ void k_bep_sort(uint32_t nb, const uint32_t* deg, const uint32_t* sta
 
 // Each row's loads go out together (all its points, this body's side only), and the next row's
 // index is read while the current row is applied.
-__global__ void k_warm_bodies(uint32_t nb, const uint32_t* __restrict__ deg, const uint32_t* __restrict__ start,
+__global__ void __launch_bounds__(kThreads) k_warm_bodies(uint32_t nb, const uint32_t* __restrict__ deg, const uint32_t* __restrict__ start,
                               const uint32_t* __restrict__ ep, Rows rows, float4* __restrict__ motion) {
     for (uint32_t x = gid(); x < nb; x += gstride()) {
         const uint32_t d = deg[x];
@@ -920,6 +921,7 @@ __global__ void k_warm_bodies(uint32_t nb, const uint32_t* __restrict__ deg, con
             for (int i = 0; i < 4; ++i) {
                 if (i < count) {
                     im[i] = rows.imp[(uint32_t)i * rows.cap + r];
+#pragma unroll
                     for (int a = 0; a < 3; ++a) {
                         const float4* base = rows.ax + (size_t)((i * 3 + a) * 5 + (sideB ? 1 : 0)) * rows.cap + r;
                         q[i][a][0] = base[0];
@@ -930,9 +932,10 @@ __global__ void k_warm_bodies(uint32_t nb, const uint32_t* __restrict__ deg, con
             const V3 dir1 = vec(t1.x, t1.y, t1.z), dir2 = vec(t2.x, t2.y, t2.z);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                if (i >= count) { break; }
+                if (i >= count) { continue; }
                 const V3 ti = vec(im[i].y, im[i].z, im[i].w);
                 const float lambda[3] = { im[i].x, dot(ti, dir1), dot(ti, dir2) };
+#pragma unroll
                 for (int a = 0; a < 3; ++a) {
                     const float4 q0 = q[i][a][0], q1 = q[i][a][1];
                     if (sideB) {
@@ -1271,7 +1274,7 @@ __global__ void k_gather_again(const uint32_t* again, uint32_t n, const tfs_cont
 }
 
 // The solver side of each refreshed fast contact (src/world.cpp:1108-1138).
-__global__ void k_refresh_apply(const uint32_t* again, uint32_t n, const tfs_contact* fresh, tfs_contact* contacts,
+__global__ void __launch_bounds__(kThreads) k_refresh_apply(const uint32_t* again, uint32_t n, const tfs_contact* fresh, tfs_contact* contacts,
                                 const tfs_body* bodies, const float4* motion, const float4* mass, float matchSq) {
     for (uint32_t k = gid(); k < n; k += gstride()) {
         tfs_contact& c = contacts[again[k]];
@@ -1280,7 +1283,8 @@ __global__ void k_refresh_apply(const uint32_t* again, uint32_t n, const tfs_con
             c.count = 0;
             continue;
         }
-        const tfs_contact old = c;
+        // The contact as it was (read in place: c is written once, at the end) and as it is refreshed.
+        const tfs_contact& old = c;
         tfs_contact x = c;
         for (int j = 0; j < 3; ++j) {
             x.normal[j] = f.normal[j];
@@ -1288,7 +1292,7 @@ __global__ void k_refresh_apply(const uint32_t* again, uint32_t n, const tfs_con
             x.tangent2[j] = f.tangent2[j];
         }
         x.count = f.count > 4 ? 4 : f.count;
-        for (int i = 0; i < x.count; ++i) {
+        TF_POINTS(i, x.count) {
             tfs_point& p = x.points[i];
             p = tfs_point{};
             for (int j = 0; j < 3; ++j) {
@@ -1302,7 +1306,7 @@ __global__ void k_refresh_apply(const uint32_t* again, uint32_t n, const tfs_con
         const tfs_body& tb = bodies[x.body_b];
         prepareContact(viewOf(ta), viewOf(tb), ta, tb, loadMotion(motion, x.body_a), loadMotion(motion, x.body_b),
                        loadMass(mass, x.body_a), loadMass(mass, x.body_b), x);
-        for (int i = 0; i < x.count; ++i) {
+        TF_POINTS(i, x.count) {
             const int near = nearestPoint(old, ld3(x.points[i].local_a));
             if (near < 0) { continue; }
             x.points[i].relative_velocity = old.points[near].relative_velocity;

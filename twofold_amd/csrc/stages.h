@@ -12,6 +12,11 @@
 
 namespace tf {
 
+// The points of a contact, i = 0 .. n-1 of at most 4, as a loop unrolled over the 4 slots: every
+// index is known at compile time, so a contact held in registers stays there. A `continue` in the
+// body skips to the next point.
+#define TF_POINTS(i, n) _Pragma("unroll") for (int i = 0; i < 4; ++i) if (i < (n))
+
 // Row flags: point count, which bodies move, whether the contact is solved with the static
 // softness (src/lanes.cpp:77-84).
 enum : uint32_t { kCountMask = 7u, kMoveA = 8u, kMoveB = 16u };
@@ -234,8 +239,10 @@ struct Row {
     float friction, restitution;
 };
 
-// The row functions take the point count as a template argument, so every point index is known at
-// compile time and the points stay in registers, and an axis source: ax(i, d) gives point i's axis
+// The row functions unroll over N = 4 point slots, so every point index is known at compile time and
+// the points stay in registers, and solve the first `count` of them: a wave whose rows have different
+// point counts runs one body of code, masked per lane, rather than one per count (up to 1 + 2 + 3 + 4
+// point solves). They also take an axis source: ax(i, d) gives point i's axis
 // along the normal (d = 0), tangent1 (1) or tangent2 (2). The kernels read axes that the packing
 // kernel built once per step, as prepareBatch builds them (src/lanes.cpp:98-109); the serial path
 // builds them on the spot from the same masses and anchors (makeAxis), which gives the same bits.
@@ -253,10 +260,11 @@ struct MadeAxes {
 
 // warmStartLanes, one lane: src/lanes.cpp:195-217.
 template <int N, class AX>
-TF_HD void warmRowN(const Row& r, const RowPoint* p, const AX& ax, Motion& a, Motion& b) {
+TF_HD void warmRowN(const Row& r, const RowPoint* p, const AX& ax, Motion& a, Motion& b, int count) {
     const bool moveA = r.flags & kMoveA, moveB = r.flags & kMoveB;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
+        if (i >= count) { continue; }
         push(a, b, ax(i, 0), p[i].ni, moveA, moveB);
         push(a, b, ax(i, 1), dot(p[i].ti, r.t1), moveA, moveB);
         push(a, b, ax(i, 2), dot(p[i].ti, r.t2), moveA, moveB);
@@ -266,12 +274,13 @@ TF_HD void warmRowN(const Row& r, const RowPoint* p, const AX& ax, Motion& a, Mo
 // The 3D contact solve: solveLanes for one lane (src/lanes.cpp:220-315), which is also the 3D part
 // of solveContacts (src/solver.cpp:370-417).
 template <int N, bool BIAS, class AX>
-TF_HD void solveRowN(const Row& r, RowPoint* p, const AX& ax, Motion& a, Motion& b, const Ctx& c) {
+TF_HD void solveRowN(const Row& r, RowPoint* p, const AX& ax, Motion& a, Motion& b, const Ctx& c, int count) {
     const bool moveA = r.flags & kMoveA, moveB = r.flags & kMoveB;
     const Soft soft = (moveA && moveB) ? c.contactSoft : c.staticSoft;
     const V3 dp = b.dp - a.dp;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
+        if (i >= count) { continue; }
         const V3 d = dp + turned(b, p[i].rb) - turned(a, p[i].ra);
         const float sep = dot(d, r.n) + p[i].adj;
         float bias = 0.0f, massScale = 1.0f, impulseScale = 0.0f;
@@ -294,6 +303,7 @@ TF_HD void solveRowN(const Row& r, RowPoint* p, const AX& ax, Motion& a, Motion&
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         const int i = BIAS ? j : N - 1 - j;
+        if (i >= count) { continue; }
         const Axis x1 = ax(i, 1);
         const Axis x2 = ax(i, 2);
         const float old1 = dot(p[i].ti, r.t1), old2 = dot(p[i].ti, r.t2);
@@ -314,11 +324,12 @@ TF_HD void solveRowN(const Row& r, RowPoint* p, const AX& ax, Motion& a, Motion&
 
 // restituteLanes, one lane: src/lanes.cpp:318-352.
 template <int N, class AX>
-TF_HD void restituteRowN(const Row& r, RowPoint* p, const AX& ax, Motion& a, Motion& b, const Ctx& c) {
+TF_HD void restituteRowN(const Row& r, RowPoint* p, const AX& ax, Motion& a, Motion& b, const Ctx& c, int count) {
     const bool moveA = r.flags & kMoveA, moveB = r.flags & kMoveB;
     const float e = r.restitution;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
+        if (i >= count) { continue; }
         const bool on = !(e == 0.0f) && !(p[i].rv > -c.restitutionThreshold) && !(p[i].nimax == 0.0f);
         const Axis x = ax(i, 0);
         const float vn = speedAlong(a, b, r.n, x);
@@ -335,26 +346,18 @@ TF_HD void restituteRowN(const Row& r, RowPoint* p, const AX& ax, Motion& a, Mot
 
 enum RowMode { kWarm = 0, kBias = 1, kRelax = 2, kRestitute = 3 };
 
-template <int MODE, int N, class AX>
-TF_HD void rowN(const Row& r, RowPoint* p, const AX& ax, Motion& a, Motion& b, const Ctx& c) {
-    if (MODE == kWarm) {
-        warmRowN<N>(r, p, ax, a, b);
-    } else if (MODE == kRestitute) {
-        restituteRowN<N>(r, p, ax, a, b, c);
-    } else {
-        solveRowN<N, MODE == kBias>(r, p, ax, a, b, c);
-    }
-}
-
-// One row in the given mode, for the row's point count.
+// One row in the given mode: its first (flags & kCountMask) points, at most 4. In relax mode the
+// friction half runs from the last point down, so skipping the empty slots 3..count gives the order
+// count-1..0 that a solve over exactly `count` points has.
 template <int MODE, class AX>
 TF_HD void rows(const Row& r, RowPoint* p, const AX& ax, Motion& a, Motion& b, const Ctx& c) {
-    switch (r.flags & kCountMask) {
-        case 1: rowN<MODE, 1>(r, p, ax, a, b, c); break;
-        case 2: rowN<MODE, 2>(r, p, ax, a, b, c); break;
-        case 3: rowN<MODE, 3>(r, p, ax, a, b, c); break;
-        case 4: rowN<MODE, 4>(r, p, ax, a, b, c); break;
-        default: break;
+    const int count = (int)(r.flags & kCountMask);
+    if (MODE == kWarm) {
+        warmRowN<4>(r, p, ax, a, b, count);
+    } else if (MODE == kRestitute) {
+        restituteRowN<4>(r, p, ax, a, b, c, count);
+    } else {
+        solveRowN<4, MODE == kBias>(r, p, ax, a, b, c, count);
     }
 }
 
@@ -454,7 +457,7 @@ TF_HD void prepareContact(const BodyView& va, const BodyView& vb, const tfs_body
     }
     const PlanarState st(a, b, ma, mb, plane);
     const V3 normal = ld3(c.normal);
-    for (int i = 0; i < c.count; ++i) {
+    TF_POINTS(i, c.count) {
         tfs_point& p = c.points[i];
         const V3 ra = ld3(p.anchor_a), rb = ld3(p.anchor_b);
         p.adjusted_separation = p.separation - dot(rb - ra, normal);
@@ -468,7 +471,7 @@ TF_HD void prepareContact(const BodyView& va, const BodyView& vb, const tfs_body
     }
     if (c.rolling_resistance > 0.0f && c.count > 0) {
         float ha = 0.0f, hb = 0.0f;
-        for (int i = 0; i < c.count; ++i) {
+        TF_POINTS(i, c.count) {
             ha += fabs_(dot(ld3(c.points[i].anchor_a), normal));
             hb += fabs_(dot(ld3(c.points[i].anchor_b), normal));
         }
@@ -481,7 +484,7 @@ TF_HD void prepareContact(const BodyView& va, const BodyView& vb, const tfs_body
 // rebaseContact: src/solver.cpp:293-306.
 TF_HD void rebaseContact(const Motion& a, const Motion& b, tfs_contact& c) {
     const float shift = dot(b.dp - a.dp, ld3(c.normal));
-    for (int i = 0; i < c.count; ++i) {
+    TF_POINTS(i, c.count) {
         tfs_point& p = c.points[i];
         st3(p.anchor_a, rotate(a.dq, ld3(p.anchor_a)));
         st3(p.anchor_b, rotate(b.dq, ld3(p.anchor_b)));
@@ -502,14 +505,14 @@ TF_HD void warmRest(tfs_contact& c, const BodyView& va, const BodyView& vb, cons
     const V3 n = ld3(c.normal), t1 = ld3(c.tangent1), t2 = ld3(c.tangent2);
     if (c.planar) {
         bool held = c.rolling_resistance > 0.0f;
-        for (int i = 0; i < c.count; ++i) {
+        TF_POINTS(i, c.count) {
             const V3 ti = ld3(c.points[i].tangent_impulse);
             held = held || c.points[i].normal_impulse != 0.0f || dot(ti, ti) != 0.0f;
         }
         if (!held) { return; }
         const V3 plane = ld3(c.plane);
         PlanarState st(a, b, ma, mb, plane);
-        for (int i = 0; i < c.count; ++i) {
+        TF_POINTS(i, c.count) {
             const tfs_point& p = c.points[i];
             const PlanarAxes x = planarAxes(ma, mb, c, i);
             st.push(n, x.normalA, x.normalB, p.normal_impulse);
@@ -523,7 +526,7 @@ TF_HD void warmRest(tfs_contact& c, const BodyView& va, const BodyView& vb, cons
         st.store(a, b, plane, !anchoredView(va), !anchoredView(vb));
         return;
     }
-    for (int i = 0; i < c.count; ++i) {
+    TF_POINTS(i, c.count) {
         const tfs_point& p = c.points[i];
         const V3 ra = ld3(p.anchor_a), rb = ld3(p.anchor_b);
         push(a, b, makeAxis(ma, mb, ra, rb, n), p.normal_impulse, true, true);
@@ -548,7 +551,7 @@ TF_HD void solvePlanar(tfs_contact& c, const BodyView& va, const BodyView& vb, c
     planarTurn(b, plane, tbx, tby);
     PlanarState st(a, b, ma, mb, plane);
     bool changed = false;
-    for (int i = 0; i < c.count; ++i) {
+    TF_POINTS(i, c.count) {
         tfs_point& p = c.points[i];
         const PlanarAxes x = planarAxes(ma, mb, c, i);
         const float reachA = tax * x.tangentA + tay * x.normalA;
@@ -597,7 +600,7 @@ TF_HD void solvePlanar(tfs_contact& c, const BodyView& va, const BodyView& vb, c
             st.wb += st.ib * lambda;
         }
         float normalSum = 0.0f;
-        for (int i = 0; i < c.count; ++i) { normalSum += c.points[i].normal_impulse; }
+        TF_POINTS(i, c.count) { normalSum += c.points[i].normal_impulse; }
         const float limit = c.rolling_resistance * c.rolling_radius * normalSum;
         const float held = minf(maxf(total, -limit), limit);
         st.wa -= st.ia * (held - total);
@@ -624,7 +627,7 @@ TF_HD void resistTurning(tfs_contact& c, const Mass& ma, const Mass& mb, Motion&
         }
     }
     float normalSum = 0.0f;
-    for (int i = 0; i < c.count; ++i) { normalSum += c.points[i].normal_impulse; }
+    TF_POINTS(i, c.count) { normalSum += c.points[i].normal_impulse; }
     const float limit = c.rolling_resistance * c.rolling_radius * normalSum;
     const float mag = length(total);
     if (mag > limit) {
@@ -638,7 +641,9 @@ TF_HD void resistTurning(tfs_contact& c, const Mass& ma, const Mass& mb, Motion&
 
 // Loads the points of a contact into row form and back.
 TF_HD void pointsOf(const tfs_contact& c, RowPoint* p) {
-    for (int i = 0; i < c.count; ++i) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (i >= c.count) { break; }
         const tfs_point& q = c.points[i];
         p[i].ra = ld3(q.anchor_a);
         p[i].rb = ld3(q.anchor_b);
@@ -650,7 +655,9 @@ TF_HD void pointsOf(const tfs_contact& c, RowPoint* p) {
     }
 }
 TF_HD void storeImpulses(tfs_contact& c, const RowPoint* p) {
-    for (int i = 0; i < c.count; ++i) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (i >= c.count) { break; }
         c.points[i].normal_impulse = p[i].ni;
         st3(c.points[i].tangent_impulse, p[i].ti);
         c.points[i].max_normal_impulse = p[i].nimax;
@@ -688,7 +695,7 @@ TF_HD void restituteRest(tfs_contact& c, const BodyView& va, const BodyView& vb,
     const V3 n = ld3(c.normal);
     if (c.planar) {
         PlanarState st(a, b, ma, mb, ld3(c.plane));
-        for (int i = 0; i < c.count; ++i) {
+        TF_POINTS(i, c.count) {
             tfs_point& p = c.points[i];
             if (p.relative_velocity > -ctx.restitutionThreshold || p.max_normal_impulse == 0.0f) { continue; }
             const PlanarAxes x = planarAxes(ma, mb, c, i);
@@ -703,7 +710,7 @@ TF_HD void restituteRest(tfs_contact& c, const BodyView& va, const BodyView& vb,
         st.store(a, b, ld3(c.plane), !anchoredView(va), !anchoredView(vb));
         return;
     }
-    for (int i = 0; i < c.count; ++i) {
+    TF_POINTS(i, c.count) {
         tfs_point& p = c.points[i];
         if (p.relative_velocity > -ctx.restitutionThreshold || p.max_normal_impulse == 0.0f) { continue; }
         const Axis x = makeAxis(ma, mb, ld3(p.anchor_a), ld3(p.anchor_b), n);
@@ -721,7 +728,7 @@ TF_HD void restituteRest(tfs_contact& c, const BodyView& va, const BodyView& vb,
 TF_HD int nearestPoint(const tfs_contact& old, V3 localA) {
     int best = -1;
     float bestSq = 3.40282347e+38f;
-    for (int k = 0; k < old.count; ++k) {
+    TF_POINTS(k, old.count) {
         const V3 d = ld3(old.points[k].local_a) - localA;
         const float dsq = dot(d, d);
         if (dsq < bestSq) {
@@ -734,11 +741,10 @@ TF_HD int nearestPoint(const tfs_contact& old, V3 localA) {
 
 // carryImpulses: src/world.cpp:102-120.
 TF_HD void carryImpulses(const tfs_contact& old, tfs_contact& c, float matchSq) {
-    int from[4] = { -1, -1, -1, -1 }, claims[4] = { 0, 0, 0, 0 };
-    for (int i = 0; i < c.count; ++i) {
-        from[i] = -1;
+    int from[4] = { -1, -1, -1, -1 };
+    TF_POINTS(i, c.count) {
         float bestSq = matchSq;
-        for (int k = 0; k < old.count; ++k) {
+        TF_POINTS(k, old.count) {
             const V3 d = ld3(old.points[k].local_a) - ld3(c.points[i].local_a);
             const float dsq = dot(d, d);
             if (dsq <= bestSq) {
@@ -746,13 +752,15 @@ TF_HD void carryImpulses(const tfs_contact& old, tfs_contact& c, float matchSq) 
                 bestSq = dsq;
             }
         }
-        if (from[i] >= 0) { ++claims[from[i]]; }
     }
     const V3 n = ld3(c.normal);
-    for (int i = 0; i < c.count; ++i) {
+    TF_POINTS(i, c.count) {
         if (from[i] < 0) { continue; }
+        // The number of new points that take their impulse from the same old point.
+        int claims = 0;
+        TF_POINTS(j, c.count) { claims += from[j] == from[i] ? 1 : 0; }
         const tfs_point& o = old.points[from[i]];
-        const float share = 1.0f / (float)claims[from[i]];
+        const float share = 1.0f / (float)claims;
         c.points[i].normal_impulse = o.normal_impulse * share;
         const V3 t = ld3(o.tangent_impulse);
         st3(c.points[i].tangent_impulse, (t - n * dot(n, t)) * share);
