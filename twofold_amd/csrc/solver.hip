@@ -51,8 +51,6 @@ struct DevCounts {
     uint32_t restTicket[2];   // level dataflow of the active and impact rest lists
     uint32_t restLevels[2];
     uint32_t scanTicket[kScanSlots];
-    uint32_t pad_;
-    unsigned long long barrier;
     uint32_t changed[kIslandRounds];
     uint32_t colorStart[TFS_COLORS + 1];
     uint32_t impactColorStart[TFS_COLORS + 1];
@@ -813,17 +811,54 @@ struct PassPlan {
     uint32_t begin[kMaxSegments], end[kMaxSegments];
 };
 
-// Every block waits for its own coherent stores to complete, then its first thread arrives on a
-// 64-bit counter and waits until every block has. Rows exchange data only through coherent
-// accesses, so no cache needs flushing.
-__device__ __forceinline__ void gridSync(unsigned long long* counter, unsigned long long& phase, DevCounts* cnt) {
+// The grid-wide barrier of a cooperative launch. Blocks arrive in kBarrierGroups groups (block b in
+// group b % kBarrierGroups), each on a counter of its own cache line, so arrivals on different
+// counters do not queue behind each other; the last block of a group to arrive adds the group's
+// arrival to the top counter, which every block waits on. Counters only grow: a launch is given
+// each counter's value before it (BarrierBase) and knows how much each barrier adds.
+constexpr uint32_t kBarrierGroups = 8;
+struct alignas(128) BarrierLine {
+    unsigned long long n;
+    unsigned long long pad_[15];
+};
+struct BarrierLines {
+    BarrierLine top;
+    BarrierLine group[kBarrierGroups];
+};
+struct BarrierBase {
+    unsigned long long top;
+    unsigned long long group[kBarrierGroups];
+};
+
+// The targets one block waits for, advanced by one barrier at a time.
+struct BarrierPhase {
+    unsigned long long top, group;
+    __device__ __forceinline__ explicit BarrierPhase(const BarrierBase& b)
+        : top(b.top), group(b.group[blockIdx.x % kBarrierGroups]) {}
+};
+
+// The base of the launch after one with `barriers` barriers over `grid` blocks.
+inline void advanceBarrier(BarrierBase& b, uint32_t grid, uint32_t barriers) {
+    const uint32_t groups = std::min(grid, kBarrierGroups);
+    for (uint32_t gi = 0; gi < groups; ++gi) {
+        b.group[gi] += (unsigned long long)barriers * ((grid - gi + kBarrierGroups - 1) / kBarrierGroups);
+    }
+    b.top += (unsigned long long)barriers * groups;
+}
+
+// Every block waits for its own coherent stores to complete, then its first thread arrives and
+// waits until every block has. Rows exchange data only through coherent accesses, so no cache needs
+// flushing.
+__device__ __forceinline__ void gridSync(BarrierLines* bar, BarrierPhase& phase, DevCounts* cnt) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
-        phase += gridDim.x;
-        atomicAdd(counter, 1ull);
+        const uint32_t gi = blockIdx.x % kBarrierGroups;
+        phase.group += (gridDim.x - gi + kBarrierGroups - 1) / kBarrierGroups;
+        phase.top += gridDim.x < kBarrierGroups ? gridDim.x : kBarrierGroups;
+        if (atomicAdd(&bar->group[gi].n, 1ull) + 1ull == phase.group) { atomicAdd(&bar->top.n, 1ull); }
         uint32_t spins = 0;
-        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < phase) {
+        while (__hip_atomic_load(&bar->top.n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < phase.top) {
             if (++spins > kSpinLimit) {
                 atomicOr(&cnt->error, 2u);
                 break;
@@ -834,15 +869,21 @@ __device__ __forceinline__ void gridSync(unsigned long long* counter, unsigned l
     __syncthreads();
 }
 
+// A thread's slot in a pass over a list: slot t serves list entry begin + t. Consecutive slots form
+// waves (64 entries that load together), and consecutive waves go to different blocks, so a short
+// list spreads over as many compute units as the grid has blocks.
+__device__ __forceinline__ uint32_t passSlot() {
+    return ((threadIdx.x >> 6) * gridDim.x + blockIdx.x) * 64u + (threadIdx.x & 63u);
+}
+
 // Headers do not change during a pass, so each thread loads the header of its first row in the
 // next segment before the barrier; after the barrier every other load of that row goes out at once.
 template <int MODE, bool MADE>
 __global__ void __launch_bounds__(kPassThreads) k_pass(Rows rows, const uint4* impactHdr, PassPlan plan,
                                                        float4* motion, const float4* mass, Ctx ctx,
-                                                       unsigned long long* counter, unsigned long long base,
-                                                       DevCounts* cnt) {
-    unsigned long long phase = base;
-    const uint32_t g = gid(), stride = gstride();
+                                                       BarrierLines* bar, BarrierBase base, DevCounts* cnt) {
+    BarrierPhase phase(base);
+    const uint32_t g = passSlot(), stride = gstride();
     auto header = [&](uint32_t s, uint32_t i) { return s >= plan.impactFrom ? impactHdr[i] : rows.hdr[i]; };
     uint4 next = make_uint4(0u, 0u, 0u, 0u);
     if (plan.n > 0 && plan.begin[0] + g < plan.end[0]) { next = header(0, plan.begin[0] + g); }
@@ -855,7 +896,7 @@ __global__ void __launch_bounds__(kPassThreads) k_pass(Rows rows, const uint4* i
             const uint4 h = i == first ? firstHdr : header(s, i);
             rowStep<MODE, true, MADE>(rows, impact ? h.w : i, h, motion, mass, ctx);
         }
-        if (s + 1 < plan.n) { gridSync(counter, phase, cnt); }
+        if (s + 1 < plan.n) { gridSync(bar, phase, cnt); }
     }
 }
 
@@ -1068,11 +1109,11 @@ constexpr int kRestBlockThreads = 512;
 template <int MODE, bool ONE_BLOCK>
 __global__ void __launch_bounds__(ONE_BLOCK ? kRestBlockThreads : kPassThreads)
     k_rest_levels(const uint4* entries, const uint32_t* levelStart, uint32_t levels, tfs_contact* contacts,
-                  const tfs_body* bodies, float4* motion, const float4* mass, Ctx ctx, unsigned long long* counter,
-                  unsigned long long base, DevCounts* cnt) {
+                  const tfs_body* bodies, float4* motion, const float4* mass, Ctx ctx, BarrierLines* bar,
+                  BarrierBase base, DevCounts* cnt) {
     using X = Access<!ONE_BLOCK>;
-    unsigned long long phase = base;
-    const uint32_t g = ONE_BLOCK ? threadIdx.x : gid();
+    BarrierPhase phase(base);
+    const uint32_t g = ONE_BLOCK ? threadIdx.x : passSlot();
     const uint32_t stride = ONE_BLOCK ? blockDim.x : gstride();
     uint32_t lo = levelStart[0], hi = levels > 0 ? levelStart[1] : 0u;
     uint4 next = lo + g < hi ? entries[lo + g] : make_uint4(0u, 0u, 0u, 0u);
@@ -1117,7 +1158,7 @@ __global__ void __launch_bounds__(ONE_BLOCK ? kRestBlockThreads : kPassThreads)
             if (ONE_BLOCK) {
                 __syncthreads();
             } else {
-                gridSync(counter, phase, cnt);
+                gridSync(bar, phase, cnt);
             }
         }
     }
@@ -1539,7 +1580,8 @@ struct tfs_ctx {
     int scanSlot = 0;
     Rows rows{};
     uint4* impactHdr = nullptr;     // impact rows' headers, .w = their active row
-    unsigned long long barrierBase = 0;
+    BarrierLines* bar = nullptr;   // grid barrier counters
+    BarrierBase barrierBase{};
     uint32_t maxPassBlocks = 0;
     bool usePass = true;
     bool madeAxes = false;
@@ -1773,8 +1815,7 @@ int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool w
     plan.impactFrom = plan.n;
     if (withImpact) { add(hc.impactColorStart, hc.impactColorPoints); }
     if (plan.n == 0) { return 0; }
-    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((widest + kPassThreads - 1) / kPassThreads,
-                                                                    ctx->maxPassBlocks));
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((widest + 63) / 64, ctx->maxPassBlocks));
     size_t ev = 0;
     if (ctx->profiling) {
         ev = ctx->pending.size() * 2;
@@ -1787,14 +1828,13 @@ int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool w
         HIP_TRY(hipEventRecord(ctx->evPool[ev], ctx->stream));
     }
     const uint4* at = ctx->impactHdr;
-    unsigned long long* counter = &ctx->cnt->barrier;
-    const unsigned long long base = ctx->barrierBase;
+    const BarrierBase base = ctx->barrierBase;
     const float4* mass = ctx->mass;
-    void* args[] = { &ctx->rows, (void*)&at, &plan, &ctx->motion, (void*)&mass, (void*)&c, &counter, (void*)&base, &ctx->cnt };
+    void* args[] = { &ctx->rows, (void*)&at, &plan, &ctx->motion, (void*)&mass, (void*)&c, &ctx->bar, (void*)&base, &ctx->cnt };
     const void* fn = ctx->madeAxes ? (mode == kBias ? (const void*)k_pass<kBias, true> : (const void*)k_pass<kRelax, true>)
                                    : (mode == kBias ? (const void*)k_pass<kBias, false> : (const void*)k_pass<kRelax, false>);
     HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kPassThreads), args, 0, ctx->stream));
-    ctx->barrierBase += (unsigned long long)(plan.n - 1) * grid;
+    advanceBarrier(ctx->barrierBase, grid, plan.n - 1);
     if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
     return 0;
 }
@@ -1910,30 +1950,29 @@ int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32
     if (levels == 0) { return fail(ctx, TFS_E_STATE, "rest list without levels"); }
     const bool oneBlock = n / levels < (uint32_t)kRestBlockThreads;
     const uint32_t grid = oneBlock ? 1u
-                                   : std::max<uint32_t>(1, std::min<uint32_t>((n + kPassThreads - 1) / kPassThreads,
-                                                                              ctx->maxPassBlocks));
+                                   : std::max<uint32_t>(1, std::min<uint32_t>((n + 63) / 64, ctx->maxPassBlocks));
     const uint4* entries = ctx->restEntries[which];
     const uint32_t* starts = ctx->restLevelStart[which];
-    unsigned long long* counter = &ctx->cnt->barrier;
-    const unsigned long long base = ctx->barrierBase;
+    BarrierLines* bar = ctx->bar;
+    const BarrierBase base = ctx->barrierBase;
     const float4* mass = ctx->mass;
     if (oneBlock) {
         switch (mode) {
-            case kWarm: LAUNCH((k_rest_levels<kWarm, true>), 1, kRestBlockThreads, entries, starts, levels, ctx->contacts, ctx->bodies, ctx->motion, mass, c, counter, base, ctx->cnt); break;
-            case kBias: LAUNCH((k_rest_levels<kBias, true>), 1, kRestBlockThreads, entries, starts, levels, ctx->contacts, ctx->bodies, ctx->motion, mass, c, counter, base, ctx->cnt); break;
-            case kRelax: LAUNCH((k_rest_levels<kRelax, true>), 1, kRestBlockThreads, entries, starts, levels, ctx->contacts, ctx->bodies, ctx->motion, mass, c, counter, base, ctx->cnt); break;
-            default: LAUNCH((k_rest_levels<kRestitute, true>), 1, kRestBlockThreads, entries, starts, levels, ctx->contacts, ctx->bodies, ctx->motion, mass, c, counter, base, ctx->cnt); break;
+            case kWarm: LAUNCH((k_rest_levels<kWarm, true>), 1, kRestBlockThreads, entries, starts, levels, ctx->contacts, ctx->bodies, ctx->motion, mass, c, bar, base, ctx->cnt); break;
+            case kBias: LAUNCH((k_rest_levels<kBias, true>), 1, kRestBlockThreads, entries, starts, levels, ctx->contacts, ctx->bodies, ctx->motion, mass, c, bar, base, ctx->cnt); break;
+            case kRelax: LAUNCH((k_rest_levels<kRelax, true>), 1, kRestBlockThreads, entries, starts, levels, ctx->contacts, ctx->bodies, ctx->motion, mass, c, bar, base, ctx->cnt); break;
+            default: LAUNCH((k_rest_levels<kRestitute, true>), 1, kRestBlockThreads, entries, starts, levels, ctx->contacts, ctx->bodies, ctx->motion, mass, c, bar, base, ctx->cnt); break;
         }
         return 0;
     }
     void* args[] = { (void*)&entries, (void*)&starts, (void*)&levels, &ctx->contacts, &ctx->bodies, &ctx->motion,
-                     (void*)&mass, (void*)&c, &counter, (void*)&base, &ctx->cnt };
+                     (void*)&mass, (void*)&c, (void*)&bar, (void*)&base, &ctx->cnt };
     const void* fn = mode == kWarm ? (const void*)k_rest_levels<kWarm, false>
                      : mode == kBias ? (const void*)k_rest_levels<kBias, false>
                      : mode == kRelax ? (const void*)k_rest_levels<kRelax, false>
                                       : (const void*)k_rest_levels<kRestitute, false>;
     HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kPassThreads), args, 0, ctx->stream));
-    ctx->barrierBase += (unsigned long long)(levels - 1) * grid;
+    advanceBarrier(ctx->barrierBase, grid, levels - 1);
     return 0;
 }
 
@@ -2015,7 +2054,8 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     const int n = s.substeps > 1 ? s.substeps : 1;
     HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
     HIP_TRY(hipMemsetAsync(ctx->cnt, 0, sizeof(DevCounts), ctx->stream));
-    ctx->barrierBase = 0;
+    HIP_TRY(hipMemsetAsync(ctx->bar, 0, sizeof(BarrierLines), ctx->stream));
+    ctx->barrierBase = BarrierBase{};
     ctx->scanSlot = 0;
 
     // step state and lists
@@ -2189,6 +2229,7 @@ int tfs_create(int device, uint32_t maxBodies, uint32_t maxContacts, tfs_ctx** o
         if (hipEventCreate(&e) != hipSuccess) { r = TFS_E_DEVICE; }
     }
     if (!r && hipMalloc((void**)&ctx->cnt, sizeof(DevCounts)) != hipSuccess) { r = TFS_E_DEVICE; }
+    if (!r && hipMalloc((void**)&ctx->bar, sizeof(BarrierLines)) != hipSuccess) { r = TFS_E_DEVICE; }
     if (!r && hipHostMalloc((void**)&ctx->hostCnt, sizeof(DevCounts), hipHostMallocDefault) != hipSuccess) { r = TFS_E_DEVICE; }
     if (!r) {
         int perCu = 0, cus = 0, coop = 0;
@@ -2206,7 +2247,7 @@ int tfs_create(int device, uint32_t maxBodies, uint32_t maxContacts, tfs_ctx** o
         // Tuning knobs, read once: TWOFOLD_PASS=launch|coop, TWOFOLD_PASS_BLOCKS=n, TWOFOLD_AXES=packed|made.
         if (const char* e = getenv("TWOFOLD_PASS")) { ctx->usePass = ctx->usePass && strcmp(e, "launch") != 0; }
         if (const char* e = getenv("TWOFOLD_PASS_BLOCKS")) {
-            ctx->maxPassBlocks = (uint32_t)std::max(1, std::min(atoi(e), perCu * cus / 2));
+            ctx->maxPassBlocks = (uint32_t)std::max(1, std::min(atoi(e), perCu * cus));
         }
         if (const char* e = getenv("TWOFOLD_AXES")) { ctx->madeAxes = strcmp(e, "made") == 0; }
     }
@@ -2228,7 +2269,7 @@ void tfs_destroy(tfs_ctx* ctx) {
                      ctx->word, ctx->parent, ctx->islandTime, ctx->contacts, ctx->fresh, ctx->fActive, ctx->fFast,
                      ctx->fFastSolid, ctx->active, ctx->fast, ctx->fastSolid, ctx->impactFlag, ctx->impactPos,
                      ctx->impact, ctx->rowsList, ctx->impactRowsList, ctx->colors, ctx->ep, ctx->rank, ctx->rowOf,
-                     ctx->hist, ctx->status, ctx->cnt, ctx->rows.hdr, ctx->rows.nrm,
+                     ctx->hist, ctx->status, ctx->cnt, ctx->bar, ctx->rows.hdr, ctx->rows.nrm,
                      ctx->rows.tg1, ctx->rows.tg2, ctx->rows.pa, ctx->rows.pb, ctx->rows.imp, ctx->rows.peak,
                      ctx->rows.ax, ctx->impactHdr, ctx->bdeg, ctx->bstart, ctx->bcursor, ctx->bep,
                      ctx->edges, ctx->bodyWorld, ctx->refreshing, ctx->worldStart, ctx->restTracked,
