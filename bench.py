@@ -250,12 +250,122 @@ def run_batched(args):
     }
 
 
+def run_adjoint(args):
+    """The fixed-contact model and its adjoint on `--worlds` throw worlds (64 boxes thrown at a 10-box
+    wall each; twofold_amd.scenes.throw_world): `--steps` forward steps (each checkpointed), then the
+    reverse sweep of every step from its checkpoint (tfs_adjoint_step: the step again with its tape
+    recorded, then backwards through it), seeded with d loss / d com of the squared distance of the
+    final centres of mass to seeded targets. Both legs are timed per step, device-synchronised; the
+    headline is adjoint time / forward time."""
+    t0 = time.perf_counter()
+    worlds = [scenes.throw_world(0xAD000000 + k, steps=args.steps) for k in range(args.worlds)]
+    host = HostWorld(worlds)
+    bodies = host.bodies()
+    contacts, _ = host.collide(bodies, host.dt)
+    body_start = np.asarray(host.body_start, np.uint32)
+    moving = bodies["type"] == 2
+    target = bodies["com"].astype(np.float64) + np.random.default_rng(0xAD).uniform(-0.5, 0.5, (len(bodies), 3))
+    log(f"{len(worlds)} worlds, {len(bodies)} bodies, {len(contacts)} contacts, built in {time.perf_counter() - t0:.1f} s")
+    sync = device_sync()
+    solver = Solver(len(bodies), len(contacts), settings=host.settings)
+    try:
+        for rep in range(args.warmup + 1):
+            solver.upload_worlds(bodies, body_start, contacts)
+            solver.fixed_begin(host.dt, args.steps)
+            sync()
+            t = time.perf_counter()
+            for _ in range(args.steps):
+                solver.fixed_step()
+            sync()
+            forward = time.perf_counter() - t
+            com = solver.bodies()["com"].astype(np.float64)
+            dl = np.zeros_like(com)
+            dl[moving] = 2.0 * (com[moving] - target[moving])
+            loss = float(np.sum((com[moving] - target[moving]) ** 2))
+            solver.adjoint_seed(dl)
+            sync()
+            t = time.perf_counter()
+            for _ in range(args.steps):
+                solver.adjoint_step()
+            sync()
+            backward = time.perf_counter() - t
+            gv, gw, gmu = solver.adjoint_read()
+            log(f"round {rep}: forward {1e3 * forward / args.steps:.3f} ms/step, adjoint {1e3 * backward / args.steps:.3f} "
+                f"ms/step, loss {loss:.6g}, |dL/dv0| {np.linalg.norm(gv):.6g}")
+        counts = solver.counts()
+    finally:
+        solver.close()
+    ratio = backward / forward
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_adjoint(worlds[:args.cpu_worlds], args)
+    return {
+        "metric": "adjoint/forward time",
+        "value": ratio,
+        "unit": "x",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * backward / args.steps,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded throw worlds (64 boxes with N(0, 2 m/s), N(0, 1 rad/s) at a 10-box wall)",
+        "config": {
+            "workload": f"{len(worlds)} worlds x 74 boxes ({len(bodies)} bodies, {len(contacts)} contacts fixed at the "
+                        f"first step), {args.steps} steps of dt 1/60, 4 substeps; forward = checkpoint + step, adjoint = "
+                        f"step again recording + reverse sweep, per step",
+            "worlds": len(worlds), "bodies": len(bodies), "contacts": len(contacts),
+            "forward_ms_per_step": 1e3 * forward / args.steps, "adjoint_ms_per_step": 1e3 * backward / args.steps,
+            "forward_world_steps_per_s": len(worlds) * args.steps / forward,
+            "adjoint_world_steps_per_s": len(worlds) * args.steps / backward,
+            "colors": counts["colors"], "points": counts["points"],
+        },
+        "cpu_baseline": cpu,
+    }
+
+
+def cpu_adjoint(worlds, args):
+    """The float64 model of oracle/fixed_contacts.py (numpy, one thread) on the first worlds of the
+    workload: forward and reverse over the same steps, world-steps/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    try:
+        import refworld
+        from fixed_contacts import FixedContacts
+    except ImportError as e:
+        log(f"cpu baseline skipped: {e}")
+        return None
+    host = HostWorld(worlds)
+    bodies = host.bodies()
+    contacts, _ = host.collide(bodies, host.dt)
+    lib = refworld._load_oracle()
+    b = bodies.copy()
+    for i in range(len(b)):
+        lib.tfo_update_inverse_mass(b[i:i + 1].ctypes.data)
+    colors = np.full(len(contacts), 255, np.uint8)
+    lst = np.arange(len(contacts), dtype=np.uint32)
+    lib.tfo_color(b.ctypes.data, len(b), contacts.ctypes.data, lst.ctypes.data, len(lst), colors.ctypes.data)
+    model = FixedContacts(bodies, contacts, colors, host.settings, float(host.dt))
+    steps = args.cpu_steps
+    t = time.perf_counter()
+    model.adjoint(model.com0, steps=steps)
+    spent = time.perf_counter() - t
+    return {"value": len(worlds) * steps / spent, "unit": "world-steps/s (forward + reverse)", "cores": 1,
+            "kind": "port", "cpu": cpu_model(),
+            "sample": f"{len(worlds)} worlds x {steps} steps of the float64 numpy model (oracle/fixed_contacts.py)"}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--workload", choices=("pile", "batched", "planar"), default="pile",
+    ap.add_argument("--workload", choices=("pile", "batched", "planar", "adjoint"), default="pile",
                     help="pile: the 32k-body pile (the headline metric); batched: independent worlds; planar: 65,536 "
-                         "bodies locked to a plane with free spheres")
-    ap.add_argument("--worlds", type=int, default=4096, help="batched: number of worlds")
+                         "bodies locked to a plane with free spheres; adjoint: the fixed-contact model's reverse sweep "
+                         "on 1,024 throw worlds")
+    ap.add_argument("--worlds", type=int, default=None, help="batched: number of worlds (4096); adjoint (1024)")
+    ap.add_argument("--cpu-worlds", type=int, default=4, help="adjoint: worlds of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=4, help="adjoint: steps of the CPU baseline sample")
     ap.add_argument("--loop-sample", type=int, default=64, help="batched: worlds timed alone for the looping rate")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (pile 60, batched 10)")
@@ -272,7 +382,9 @@ def main():
     args = ap.parse_args()
     pile = args.workload == "pile"
     planar = args.workload == "planar"
-    args.steps = args.steps if args.steps is not None else (60 if pile else 10 if not planar else 20)
+    adjoint = args.workload == "adjoint"
+    args.worlds = args.worlds if args.worlds is not None else (1024 if adjoint else 4096)
+    args.steps = args.steps if args.steps is not None else (60 if pile else 32 if adjoint else 10 if not planar else 20)
     args.warmup = args.warmup if args.warmup is not None else (5 if pile else 2)
     args.settle = args.settle if args.settle is not None else (240 if pile else 60 if not planar else 120)
     world_size = int(os.environ.get("WORLD_SIZE", "1"))
@@ -293,6 +405,9 @@ def main():
 
     if args.workload == "batched":
         print(json.dumps(run_batched(args)), flush=True)
+        return
+    if adjoint:
+        print(json.dumps(run_adjoint(args)), flush=True)
         return
 
     scene = scenes.planar_pile() if planar else scenes.pile(side=args.side)

@@ -247,6 +247,36 @@ int tfs_read_profile(tfs_ctx* ctx, int family, uint64_t* launches, double* ms, d
  * (the rest is solved level after level). Constraint-iterations per step follow from these. */
 int tfs_last_step_counts(tfs_ctx* ctx, uint32_t counts[8]);
 
+/*
+ * The fixed-contact model and its adjoint. No counterpart in the reference (which has no
+ * differentiation); it is the solve block of src/world.cpp:1644-1681 run for several steps over
+ * the resident contacts, fixed at the first step:
+ *  - tfs_fixed_begin runs the step's set-up (src/world.cpp:1591-1657: step state, prepare,
+ *    colouring, row order, packing) once and keeps its rows: contact frames, arms and effective
+ *    masses stay those of that step. Every active contact must get a colour (TFS_E_STATE if not).
+ *    Room is made for `max_steps` checkpoints.
+ *  - tfs_fixed_step checkpoints the state (velocities, centres of mass, stored impulses, adjusted
+ *    separations), then takes one step: per substep integrateVelocities, the warm start, the biased
+ *    pass, the centres of mass and travel (com, dp += v h), the relax pass; no impact pass, no
+ *    restitution. Between steps each point's adjusted separation is rebased by the travel along the
+ *    normal (rebaseContact, src/solver.cpp:293-306, with rotations left out of the point positions).
+ *    Bodies can be downloaded after any step.
+ *  - tfs_adjoint_seed takes d loss / d com of every body after the last step (3 floats per body)
+ *    and clears the other adjoints.
+ *  - tfs_adjoint_step reverses the last step not yet reversed: it restores that step's checkpoint,
+ *    takes the step again recording what each row saw, then runs the reverse sweep. It leaves the
+ *    bodies at that step's end state.
+ *  - tfs_adjoint_read gives d loss / d v and d loss / d w at the start of the earliest step reversed
+ *    (3 floats per body each) and d loss / d friction per contact index (0 for contacts not in a
+ *    row). Any pointer may be NULL.
+ * Any upload or tfs_step ends the model (TFS_E_STATE until tfs_fixed_begin again).
+ */
+int tfs_fixed_begin(tfs_ctx* ctx, float dt, uint32_t max_steps);
+int tfs_fixed_step(tfs_ctx* ctx);
+int tfs_adjoint_seed(tfs_ctx* ctx, const float* dl_dcom);
+int tfs_adjoint_step(tfs_ctx* ctx);
+int tfs_adjoint_read(tfs_ctx* ctx, float* dl_dv, float* dl_dw, float* dl_dfriction);
+
 #ifdef __cplusplus
 }
 #endif

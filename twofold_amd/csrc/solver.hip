@@ -724,12 +724,22 @@ struct RowMadeAxes {
     }
 };
 
+// What a recording pass keeps of each row it solves, for the reverse sweep of the fixed-contact
+// model: the velocities the row saw (v[q * cap + r], q = vA, wA, vB, wB), its points' impulses
+// (imp[k * cap + r]) and separations (sep[r], one lane per point). Rows are indexed as in Rows.
+struct Tape {
+    float4* v;
+    float4* imp;
+    float4* sep;
+    uint32_t cap;
+};
+
 // One row, r, with header h (bodies, flags). Rows of one colour move no body in common
 // (src/internal.h:607-609), so within a colour each row's thread owns the row's two moving bodies.
 // MADE builds the point axes on the spot from the masses instead of reading the packed ones.
-template <int MODE, bool COHERENT, bool MADE>
+template <int MODE, bool COHERENT, bool MADE, bool TAPE>
 __device__ __forceinline__ void rowStep(const Rows& rows, uint32_t r, uint4 h, float4* motion, const float4* mass,
-                                        const Ctx& ctx) {
+                                        const Ctx& ctx, const Tape& tape) {
     using X = Access<COHERENT>;
     const int count = (int)(h.z & kCountMask);
     if (count == 0) { return; }
@@ -763,6 +773,23 @@ __device__ __forceinline__ void rowStep(const Rows& rows, uint32_t r, uint4 h, f
     const float4 pa = motion[4 * h.x + 2], qa = motion[4 * h.x + 3], pb = motion[4 * h.y + 2], qb = motion[4 * h.y + 3];
     Motion A{ vec(va.x, va.y, va.z), vec(wa.x, wa.y, wa.z), vec(pa.x, pa.y, pa.z), quat(qa.x, qa.y, qa.z, qa.w) };
     Motion B{ vec(vb.x, vb.y, vb.z), vec(wb.x, wb.y, wb.z), vec(pb.x, pb.y, pb.z), quat(qb.x, qb.y, qb.z, qb.w) };
+    if (TAPE && tape.v) {
+        const uint32_t c = tape.cap;
+        tape.v[r] = va;
+        tape.v[c + r] = wa;
+        tape.v[2 * c + r] = vb;
+        tape.v[3 * c + r] = wb;
+        const V3 dp = B.dp - A.dp;
+        float sp[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k < count) {
+                tape.imp[(uint32_t)k * c + r] = make_float4(p[k].ni, p[k].ti.x, p[k].ti.y, p[k].ti.z);
+                sp[k] = pointSeparation(A, B, dp, p[k], row.n);
+            }
+        }
+        tape.sep[r] = make_float4(sp[0], sp[1], sp[2], sp[3]);
+    }
     if (MADE) {
         const RowMadeAxes ax{ loadMass(mass, h.x), loadMass(mass, h.y), row, p };
         tf::rows<MODE>(row, p, ax, A, B, ctx);
@@ -793,11 +820,11 @@ __device__ __forceinline__ void rowStep(const Rows& rows, uint32_t r, uint4 h, f
 // One colour per launch; impactHdr is set for impact rows.
 template <int MODE, bool MADE>
 __global__ void __launch_bounds__(kThreads) k_rows(Rows rows, const uint4* impactHdr, uint32_t begin, uint32_t end,
-                                                   float4* motion, const float4* mass, Ctx ctx) {
+                                                   float4* motion, const float4* mass, Ctx ctx, Tape tape) {
     const uint32_t i = begin + gid();
     if (i >= end) { return; }
     const uint4 h = impactHdr ? impactHdr[i] : rows.hdr[i];
-    rowStep<MODE, false, MADE>(rows, impactHdr ? h.w : i, h, motion, mass, ctx);
+    rowStep<MODE, false, MADE, true>(rows, impactHdr ? h.w : i, h, motion, mass, ctx, impactHdr ? Tape{} : tape);
 }
 
 // A whole pass in one cooperative launch: the colours of the active rows, then those of the impact
@@ -878,10 +905,11 @@ __device__ __forceinline__ uint32_t passSlot() {
 
 // Headers do not change during a pass, so each thread loads the header of its first row in the
 // next segment before the barrier; after the barrier every other load of that row goes out at once.
-template <int MODE, bool MADE>
+template <int MODE, bool MADE, bool TAPE>
 __global__ void __launch_bounds__(kPassThreads) k_pass(Rows rows, const uint4* impactHdr, PassPlan plan,
                                                        float4* motion, const float4* mass, Ctx ctx,
-                                                       BarrierLines* bar, BarrierBase base, DevCounts* cnt) {
+                                                       BarrierLines* bar, BarrierBase base, DevCounts* cnt,
+                                                       Tape tape) {
     BarrierPhase phase(base);
     const uint32_t g = passSlot(), stride = gstride();
     auto header = [&](uint32_t s, uint32_t i) { return s >= plan.impactFrom ? impactHdr[i] : rows.hdr[i]; };
@@ -894,7 +922,7 @@ __global__ void __launch_bounds__(kPassThreads) k_pass(Rows rows, const uint4* i
         if (s + 1 < plan.n && plan.begin[s + 1] + g < plan.end[s + 1]) { next = header(s + 1, plan.begin[s + 1] + g); }
         for (uint32_t i = first; i < plan.end[s]; i += stride) {
             const uint4 h = i == first ? firstHdr : header(s, i);
-            rowStep<MODE, true, MADE>(rows, impact ? h.w : i, h, motion, mass, ctx);
+            rowStep<MODE, true, MADE, TAPE>(rows, impact ? h.w : i, h, motion, mass, ctx, impact ? Tape{} : tape);
         }
         if (s + 1 < plan.n) { gridSync(bar, phase, cnt); }
     }
@@ -991,6 +1019,285 @@ __global__ void __launch_bounds__(kThreads) k_warm_bodies(uint32_t nb, const uin
         }
         storeVelocity(motion, x, m);
     }
+}
+
+// ---- the fixed-contact model and its adjoint ---------------------------------------------------------
+//
+// The solve block run for several steps over one contact set, fixed at the first step: no
+// narrowphase between steps, the contact frames, arms and effective masses (the packed rows) kept,
+// and each point's adjusted separation rebased by the bodies' travel along the normal between
+// steps (rebaseContact, src/solver.cpp:293-306, with dq = identity: rotations are left out of the
+// point positions). Per substep: integrateVelocities, the warm start, the biased pass, the
+// positions (com and dp only), the relax pass (src/world.cpp:1671-1681); no impact pass and no
+// restitution. The adjoint runs the reverse sweep of each step from a checkpoint of the step's start,
+// solving the step again to record what each row saw (Tape).
+
+__device__ __forceinline__ bool movesFreely(const tfs_body& b) {
+    return b.alive && b.type != TFS_STATIC && !(b.type == TFS_DYNAMIC && !b.awake);
+}
+
+// integratePosition's centre of mass and travel (src/world.cpp:1161-1172), without the rotation.
+__global__ void k_fx_positions(tfs_body* bodies, uint32_t n, float4* motion, float h) {
+    for (uint32_t i = gid(); i < n; i += gstride()) {
+        tfs_body& b = bodies[i];
+        if (!movesFreely(b)) { continue; }
+        const float4 v = motion[4 * i], dp = motion[4 * i + 2];
+        const V3 vh = vec(v.x, v.y, v.z) * h;
+        const V3 com = ld3(b.com) + vh;
+        st3(b.com, com);
+        const V3 d = vec(dp.x, dp.y, dp.z) + vh;
+        motion[4 * i + 2] = make_float4(d.x, d.y, d.z, 0.0f);
+    }
+}
+
+// rebaseContact's separation shift for every coloured row (src/solver.cpp:293-306).
+__global__ void k_fx_rebase(Rows rows, uint32_t n, const float4* motion) {
+    for (uint32_t r = gid(); r < n; r += gstride()) {
+        const uint4 h = rows.hdr[r];
+        const int count = (int)(h.z & kCountMask);
+        const float4 pa = motion[4 * h.x + 2], pb = motion[4 * h.y + 2], nr = rows.nrm[r];
+        const float shift = dot(vec(pb.x, pb.y, pb.z) - vec(pa.x, pa.y, pa.z), vec(nr.x, nr.y, nr.z));
+        TF_POINTS(k, count) { rows.pa[(uint32_t)k * rows.cap + r].w += shift; }
+    }
+}
+
+__global__ void k_fx_reset_dp(uint32_t n, float4* motion) {
+    for (uint32_t i = gid(); i < n; i += gstride()) { motion[4 * i + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f); }
+}
+
+// A step's checkpoint: per body its velocities and centre of mass (3 float4), per row its points'
+// impulses and adjusted separations (4 + 1 float4).
+struct Checkpoint {
+    float4* body;   // [3 * i + q]
+    float4* row;    // [q * cap + r], q = 0..3 impulses, 4 separations
+};
+
+__global__ void k_fx_save(const tfs_body* bodies, uint32_t nb, const float4* motion, Rows rows, uint32_t nr,
+                          Checkpoint cp) {
+    for (uint32_t i = gid(); i < nb; i += gstride()) {
+        cp.body[3 * i] = motion[4 * i];
+        cp.body[3 * i + 1] = motion[4 * i + 1];
+        const V3 c = ld3(bodies[i].com);
+        cp.body[3 * i + 2] = make_float4(c.x, c.y, c.z, 0.0f);
+    }
+    const uint32_t cap = rows.cap;
+    for (uint32_t r = gid(); r < nr; r += gstride()) {
+        float4 sep = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float* sp = &sep.x;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            cp.row[(uint32_t)k * cap + r] = rows.imp[(uint32_t)k * cap + r];
+            sp[k] = rows.pa[(uint32_t)k * cap + r].w;
+        }
+        cp.row[4u * cap + r] = sep;
+    }
+}
+
+__global__ void k_fx_restore(tfs_body* bodies, uint32_t nb, float4* motion, Rows rows, uint32_t nr, Checkpoint cp) {
+    for (uint32_t i = gid(); i < nb; i += gstride()) {
+        motion[4 * i] = cp.body[3 * i];
+        motion[4 * i + 1] = cp.body[3 * i + 1];
+        motion[4 * i + 2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const float4 c = cp.body[3 * i + 2];
+        st3(bodies[i].com, vec(c.x, c.y, c.z));
+    }
+    const uint32_t cap = rows.cap;
+    for (uint32_t r = gid(); r < nr; r += gstride()) {
+        const float4 sep = cp.row[4u * cap + r];
+        const float* sp = &sep.x;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            rows.imp[(uint32_t)k * cap + r] = cp.row[(uint32_t)k * cap + r];
+            rows.pa[(uint32_t)k * cap + r].w = sp[k];
+        }
+    }
+}
+
+// The adjoint state: per body d loss / d (v, w) (bar[2i], bar[2i + 1]), d loss / d dp and the
+// fixed d loss / d com of the last step; per row point d loss / d (normal impulse, tangent impulse)
+// and d loss / d adjusted separation; per row d loss / d friction.
+struct Adjoint {
+    float4* bar;
+    float4* dpBar;
+    float4* comBar;
+    float4* impBar;   // [k * cap + r]
+    float* sepBar;    // [k * cap + r]
+    float* muBar;     // [r]
+};
+
+__global__ void k_adj_seed(uint32_t nb, const float* dlcom, uint32_t nr, uint32_t cap, Adjoint g) {
+    for (uint32_t i = gid(); i < nb; i += gstride()) {
+        g.comBar[i] = make_float4(dlcom[3 * i], dlcom[3 * i + 1], dlcom[3 * i + 2], 0.0f);
+        g.bar[2 * i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        g.bar[2 * i + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        g.dpBar[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    for (uint32_t r = gid(); r < nr; r += gstride()) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            g.impBar[(uint32_t)k * cap + r] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            g.sepBar[(uint32_t)k * cap + r] = 0.0f;
+        }
+        g.muBar[r] = 0.0f;
+    }
+}
+
+// The reverse of the step's separation rebase: each moving body's d loss / d dp at the end of the
+// step gathers +-n times its rows' separation adjoints (+ on side B, - on side A), in row order.
+__global__ void k_adj_rebase(uint32_t nb, const uint32_t* deg, const uint32_t* start, const uint32_t* ep, Rows rows,
+                             Adjoint g) {
+    for (uint32_t x = gid(); x < nb; x += gstride()) {
+        const uint32_t d = deg[x];
+        const uint32_t* list = ep + start[x];
+        V3 sum = vec(0.0f, 0.0f, 0.0f);
+        for (uint32_t k = 0; k < d; ++k) {
+            const uint32_t r = list[k] >> 1;
+            const int count = (int)(rows.hdr[r].z & kCountMask);
+            float s = 0.0f;
+            TF_POINTS(q, count) { s += g.sepBar[(uint32_t)q * rows.cap + r]; }
+            const float4 nr = rows.nrm[r];
+            const V3 t = vec(nr.x, nr.y, nr.z) * s;
+            sum = (list[k] & 1u) ? sum + t : sum - t;
+        }
+        g.dpBar[x] = make_float4(sum.x, sum.y, sum.z, 0.0f);
+    }
+}
+
+// The reverse of k_fx_positions: com += v h and dp += v h give v their adjoints times h.
+__global__ void k_adj_positions(const tfs_body* bodies, uint32_t n, float h, Adjoint g) {
+    for (uint32_t i = gid(); i < n; i += gstride()) {
+        if (!movesFreely(bodies[i])) { continue; }
+        const float4 v = g.bar[2 * i], c = g.comBar[i], d = g.dpBar[i];
+        const V3 x = vec(v.x, v.y, v.z) + (vec(c.x, c.y, c.z) + vec(d.x, d.y, d.z)) * h;
+        g.bar[2 * i] = make_float4(x.x, x.y, x.z, 0.0f);
+    }
+}
+
+// The reverse of the warm start: the pushes do not depend on the velocities, so the velocity
+// adjoints pass through and each stored impulse takes its push's adjoint (pushBar).
+__global__ void __launch_bounds__(kThreads) k_adj_warm(Rows rows, uint32_t n, Adjoint g) {
+    for (uint32_t r = gid(); r < n; r += gstride()) {
+        const uint4 h = rows.hdr[r];
+        const int count = (int)(h.z & kCountMask);
+        if (count == 0) { continue; }
+        const bool moveA = h.z & kMoveA, moveB = h.z & kMoveB;
+        const float4 a0 = g.bar[2 * h.x], a1 = g.bar[2 * h.x + 1], b0 = g.bar[2 * h.y], b1 = g.bar[2 * h.y + 1];
+        const MotionBar mb{ vec(a0.x, a0.y, a0.z), vec(a1.x, a1.y, a1.z), vec(b0.x, b0.y, b0.z), vec(b1.x, b1.y, b1.z) };
+        const float4 t1 = rows.tg1[r], t2 = rows.tg2[r];
+        const V3 d1 = vec(t1.x, t1.y, t1.z), d2 = vec(t2.x, t2.y, t2.z);
+        const LoadedAxes ax{ rows, r };
+        TF_POINTS(k, count) {
+            const uint32_t j = (uint32_t)k * rows.cap + r;
+            const float4 ib = g.impBar[j];
+            const float n0 = pushBar(ax(k, 0), mb, moveA, moveB);
+            const float n1 = pushBar(ax(k, 1), mb, moveA, moveB);
+            const float n2 = pushBar(ax(k, 2), mb, moveA, moveB);
+            const V3 t = vec(ib.y, ib.z, ib.w) + (d1 * n1 + d2 * n2);
+            g.impBar[j] = make_float4(ib.x + n0, t.x, t.y, t.z);
+        }
+    }
+}
+
+// The reverse of one row of a pass, from what the recording pass saw (Tape).
+template <bool BIAS, bool COHERENT, bool MADE>
+__device__ __forceinline__ void rowAdjoint(const Rows& rows, uint32_t r, const Tape& tape, const float4* mass,
+                                           const Ctx& ctx, const Adjoint& g) {
+    using X = Access<COHERENT>;
+    const uint4 h = rows.hdr[r];
+    const int count = (int)(h.z & kCountMask);
+    if (count == 0) { return; }
+    const float4 n = rows.nrm[r], t1 = rows.tg1[r], t2 = rows.tg2[r];
+    const Row row{ h.z, vec(n.x, n.y, n.z), vec(t1.x, t1.y, t1.z), vec(t2.x, t2.y, t2.z), n.w, t1.w };
+    const uint32_t c = tape.cap;
+    RowPoint p[4];
+    float niB[4] = { 0.0f, 0.0f, 0.0f, 0.0f }, sepB[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+    V3 tiB[4];
+    float4 axv[4][MADE ? 1 : 15];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k < count) {
+            const uint32_t j = (uint32_t)k * rows.cap + r;
+            const float4 im = tape.imp[(uint32_t)k * c + r], ib = g.impBar[j];
+            if (MADE) {
+                const float4 a = rows.pa[j], b = rows.pb[j];
+                p[k].ra = vec(a.x, a.y, a.z);
+                p[k].rb = vec(b.x, b.y, b.z);
+            }
+            p[k].ni = im.x;
+            p[k].ti = vec(im.y, im.z, im.w);
+            p[k].nimax = 0.0f;
+            niB[k] = ib.x;
+            tiB[k] = vec(ib.y, ib.z, ib.w);
+            if (!MADE) {
+#pragma unroll
+                for (int q = 0; q < 15; ++q) { axv[k][q] = rows.ax[(size_t)(k * 15 + q) * rows.cap + r]; }
+            }
+        }
+    }
+    const float4 sp4 = tape.sep[r];
+    const float sep[4] = { sp4.x, sp4.y, sp4.z, sp4.w };
+    const float4 va = tape.v[r], wa = tape.v[c + r], vb = tape.v[2 * c + r], wb = tape.v[3 * c + r];
+    const Q4 one = quat(0.0f, 0.0f, 0.0f, 1.0f);
+    const V3 zero = vec(0.0f, 0.0f, 0.0f);
+    const Motion A{ vec(va.x, va.y, va.z), vec(wa.x, wa.y, wa.z), zero, one };
+    const Motion B{ vec(vb.x, vb.y, vb.z), vec(wb.x, wb.y, wb.z), zero, one };
+    const float4 ga = X::ld(&g.bar[2 * h.x]), gwa = X::ld(&g.bar[2 * h.x + 1]);
+    const float4 gb = X::ld(&g.bar[2 * h.y]), gwb = X::ld(&g.bar[2 * h.y + 1]);
+    MotionBar mb{ vec(ga.x, ga.y, ga.z), vec(gwa.x, gwa.y, gwa.z), vec(gb.x, gb.y, gb.z), vec(gwb.x, gwb.y, gwb.z) };
+    float muB = 0.0f;
+    if (MADE) {
+        const RowMadeAxes ax{ loadMass(mass, h.x), loadMass(mass, h.y), row, p };
+        solveRowAdjoint<BIAS>(row, p, sep, ax, A, B, ctx, count, mb, niB, tiB, sepB, muB);
+    } else {
+        const RegAxes<MADE ? 1 : 15> ax{ axv };
+        solveRowAdjoint<BIAS>(row, p, sep, ax, A, B, ctx, count, mb, niB, tiB, sepB, muB);
+    }
+    float sepSum = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k < count) {
+            const uint32_t j = (uint32_t)k * rows.cap + r;
+            g.impBar[j] = make_float4(niB[k], tiB[k].x, tiB[k].y, tiB[k].z);
+            g.sepBar[j] += sepB[k];
+            sepSum += sepB[k];
+        }
+    }
+    g.muBar[r] += muB;
+    const V3 dn = row.n * sepSum;
+    if (h.z & kMoveA) {
+        X::st(&g.bar[2 * h.x], make_float4(mb.va.x, mb.va.y, mb.va.z, 0.0f));
+        X::st(&g.bar[2 * h.x + 1], make_float4(mb.wa.x, mb.wa.y, mb.wa.z, 0.0f));
+        const float4 d = X::ld(&g.dpBar[h.x]);
+        const V3 x = vec(d.x, d.y, d.z) - dn;
+        X::st(&g.dpBar[h.x], make_float4(x.x, x.y, x.z, 0.0f));
+    }
+    if (h.z & kMoveB) {
+        X::st(&g.bar[2 * h.y], make_float4(mb.vb.x, mb.vb.y, mb.vb.z, 0.0f));
+        X::st(&g.bar[2 * h.y + 1], make_float4(mb.wb.x, mb.wb.y, mb.wb.z, 0.0f));
+        const float4 d = X::ld(&g.dpBar[h.y]);
+        const V3 x = vec(d.x, d.y, d.z) + dn;
+        X::st(&g.dpBar[h.y], make_float4(x.x, x.y, x.z, 0.0f));
+    }
+}
+
+// The reverse of a pass in one cooperative launch: the plan lists the colours last to first.
+template <bool BIAS, bool MADE>
+__global__ void __launch_bounds__(kPassThreads) k_pass_adj(Rows rows, PassPlan plan, Tape tape, const float4* mass,
+                                                           Ctx ctx, Adjoint g, BarrierLines* bar, BarrierBase base,
+                                                           DevCounts* cnt) {
+    BarrierPhase phase(base);
+    const uint32_t slot = passSlot(), stride = gstride();
+    for (uint32_t s = 0; s < plan.n; ++s) {
+        for (uint32_t r = plan.begin[s] + slot; r < plan.end[s]; r += stride) {
+            rowAdjoint<BIAS, true, MADE>(rows, r, tape, mass, ctx, g);
+        }
+        if (s + 1 < plan.n) { gridSync(bar, phase, cnt); }
+    }
+}
+
+// Per row, d loss / d friction goes to its contact.
+__global__ void k_adj_friction(Rows rows, uint32_t n, const float* muBar, float* perContact) {
+    for (uint32_t r = gid(); r < n; r += gstride()) { perContact[rows.hdr[r].w] = muBar[r]; }
 }
 
 // ---- the rest list in levels ------------------------------------------------------------------------
@@ -1599,6 +1906,18 @@ struct tfs_ctx {
     uint32_t* pinAgain = nullptr;
     tfs_contact* pinFresh = nullptr;
     uint32_t pinCap = 0;
+    // the fixed-contact model (tfs_fixed_*) and its adjoint (tfs_adjoint_*): the counts of the step
+    // that fixed the contacts, one checkpoint per step taken, one step's tape, the adjoint state
+    bool fixedOn = false;
+    float fixedDt = 0.0f;
+    uint32_t fixedSteps = 0, fixedMax = 0, fixedLeft = 0;
+    DevCounts fixedCounts{};
+    float4* ckBody = nullptr;
+    float4* ckRow = nullptr;
+    float4* tapeBuf = nullptr;
+    Adjoint adj{};
+    float* dlcom = nullptr;
+    float* muContact = nullptr;
 };
 
 namespace {
@@ -1759,7 +2078,8 @@ double rowBytes(int mode, uint32_t rows, uint32_t points) {
     }
 }
 
-int runRows(tfs_ctx* ctx, int mode, const uint4* at, const uint32_t* starts, const uint32_t* points, const Ctx& c) {
+int runRows(tfs_ctx* ctx, int mode, const uint4* at, const uint32_t* starts, const uint32_t* points, const Ctx& c,
+            const Tape& tape = Tape{}) {
     for (int col = 0; col < TFS_COLORS; ++col) {
         const uint32_t b = starts[col], e = starts[col + 1];
         if (e <= b) { continue; }
@@ -1779,15 +2099,15 @@ int runRows(tfs_ctx* ctx, int mode, const uint4* at, const uint32_t* starts, con
         const float4* mass = ctx->mass;
         if (ctx->madeAxes) {
             switch (mode) {
-                case kBias: LAUNCH((k_rows<kBias, true>), grid, kThreads, ctx->rows, at, b, e, ctx->motion, mass, c); break;
-                case kRelax: LAUNCH((k_rows<kRelax, true>), grid, kThreads, ctx->rows, at, b, e, ctx->motion, mass, c); break;
-                default: LAUNCH((k_rows<kRestitute, true>), grid, kThreads, ctx->rows, at, b, e, ctx->motion, mass, c); break;
+                case kBias: LAUNCH((k_rows<kBias, true>), grid, kThreads, ctx->rows, at, b, e, ctx->motion, mass, c, tape); break;
+                case kRelax: LAUNCH((k_rows<kRelax, true>), grid, kThreads, ctx->rows, at, b, e, ctx->motion, mass, c, tape); break;
+                default: LAUNCH((k_rows<kRestitute, true>), grid, kThreads, ctx->rows, at, b, e, ctx->motion, mass, c, tape); break;
             }
         } else {
             switch (mode) {
-                case kBias: LAUNCH((k_rows<kBias, false>), grid, kThreads, ctx->rows, at, b, e, ctx->motion, mass, c); break;
-                case kRelax: LAUNCH((k_rows<kRelax, false>), grid, kThreads, ctx->rows, at, b, e, ctx->motion, mass, c); break;
-                default: LAUNCH((k_rows<kRestitute, false>), grid, kThreads, ctx->rows, at, b, e, ctx->motion, mass, c); break;
+                case kBias: LAUNCH((k_rows<kBias, false>), grid, kThreads, ctx->rows, at, b, e, ctx->motion, mass, c, tape); break;
+                case kRelax: LAUNCH((k_rows<kRelax, false>), grid, kThreads, ctx->rows, at, b, e, ctx->motion, mass, c, tape); break;
+                default: LAUNCH((k_rows<kRestitute, false>), grid, kThreads, ctx->rows, at, b, e, ctx->motion, mass, c, tape); break;
             }
         }
         if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
@@ -1797,7 +2117,8 @@ int runRows(tfs_ctx* ctx, int mode, const uint4* at, const uint32_t* starts, con
 
 // A pass over the listed row sets (active colours, then impact colours when `withImpact`) in one
 // cooperative launch.
-int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool withImpact, const Ctx& c) {
+int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool withImpact, const Ctx& c,
+            const Tape& tape = Tape{}) {
     PassPlan plan{};
     uint32_t widest = 0;
     double bytes = 0.0;
@@ -1830,9 +2151,14 @@ int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool w
     const uint4* at = ctx->impactHdr;
     const BarrierBase base = ctx->barrierBase;
     const float4* mass = ctx->mass;
-    void* args[] = { &ctx->rows, (void*)&at, &plan, &ctx->motion, (void*)&mass, (void*)&c, &ctx->bar, (void*)&base, &ctx->cnt };
-    const void* fn = ctx->madeAxes ? (mode == kBias ? (const void*)k_pass<kBias, true> : (const void*)k_pass<kRelax, true>)
-                                   : (mode == kBias ? (const void*)k_pass<kBias, false> : (const void*)k_pass<kRelax, false>);
+    void* args[] = { &ctx->rows, (void*)&at, &plan, &ctx->motion, (void*)&mass, (void*)&c, &ctx->bar, (void*)&base, &ctx->cnt,
+                     (void*)&tape };
+    const bool rec = tape.v != nullptr;
+    const void* fn = ctx->madeAxes
+                         ? (mode == kBias ? (rec ? (const void*)k_pass<kBias, true, true> : (const void*)k_pass<kBias, true, false>)
+                                          : (rec ? (const void*)k_pass<kRelax, true, true> : (const void*)k_pass<kRelax, true, false>))
+                         : (mode == kBias ? (rec ? (const void*)k_pass<kBias, false, true> : (const void*)k_pass<kBias, false, false>)
+                                          : (rec ? (const void*)k_pass<kRelax, false, true> : (const void*)k_pass<kRelax, false, false>));
     HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kPassThreads), args, 0, ctx->stream));
     advanceBarrier(ctx->barrierBase, grid, plan.n - 1);
     if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
@@ -2047,12 +2373,12 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
     return 0;
 }
 
-int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
+// The solve block up to its substeps (src/world.cpp:1591-1657): step state and lists, prepare, impact
+// list, colouring, row order and packing. Leaves the counts in `hc`.
+int beginStep(tfs_ctx* ctx, float dt, DevCounts& hc) {
     const uint32_t nb = ctx->nb, nc = ctx->nc;
     const tfs_settings& s = ctx->settings;
     const Ctx c = makeCtx(s, dt);
-    const int n = s.substeps > 1 ? s.substeps : 1;
-    HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
     HIP_TRY(hipMemsetAsync(ctx->cnt, 0, sizeof(DevCounts), ctx->stream));
     HIP_TRY(hipMemsetAsync(ctx->bar, 0, sizeof(BarrierLines), ctx->stream));
     ctx->barrierBase = BarrierBase{};
@@ -2094,14 +2420,25 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
         return r;
     }
     if ((r = syncCounts(ctx))) { return r; }
-    const DevCounts hc = *ctx->hostCnt;
+    hc = *ctx->hostCnt;
+    return packRows(ctx, hc.colorStart[TFS_COLORS], hc.impactColorStart[TFS_COLORS], true);
+}
+
+int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
+    const uint32_t nb = ctx->nb, nc = ctx->nc;
+    const tfs_settings& s = ctx->settings;
+    const Ctx c = makeCtx(s, dt);
+    const int n = s.substeps > 1 ? s.substeps : 1;
+    HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
+    DevCounts hc{};
+    int r;
+    if ((r = beginStep(ctx, dt, hc))) { return r; }
     const uint32_t laned = hc.colorStart[TFS_COLORS], impactLaned = hc.impactColorStart[TFS_COLORS];
     if (n > 1 && !cb) {
         bool needs = false;
         for (int i = 1; i < n; ++i) { needs = needs || (i == n / 2 ? hc.nFast : hc.nFastSolid) > 0; }
         if (needs) { return fail(ctx, TFS_E_CALLBACK, "fast contacts need a refresh callback"); }
     }
-    if ((r = packRows(ctx, laned, impactLaned, true))) { return r; }
     ctx->restLevels[0] = ctx->restLevels[1] = 0;
     if (hc.nActive > laned || hc.nImpact > impactLaned) {
         if ((r = buildRestLevels(ctx, 0, ctx->rowsList + laned, hc.nActive - laned)) ||
@@ -2188,6 +2525,143 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     return 0;
 }
 
+// ---- the fixed-contact model and its adjoint ----------------------------------------------------
+
+void freeFixed(tfs_ctx* ctx) {
+    for (void* p : { (void*)ctx->ckBody, (void*)ctx->ckRow, (void*)ctx->tapeBuf, (void*)ctx->adj.bar,
+                     (void*)ctx->adj.dpBar, (void*)ctx->adj.comBar, (void*)ctx->adj.impBar, (void*)ctx->adj.sepBar,
+                     (void*)ctx->adj.muBar, (void*)ctx->dlcom, (void*)ctx->muContact }) {
+        if (p) { (void)hipFree(p); }
+    }
+    ctx->ckBody = ctx->ckRow = ctx->tapeBuf = nullptr;
+    ctx->adj = Adjoint{};
+    ctx->dlcom = ctx->muContact = nullptr;
+    ctx->fixedOn = false;
+    ctx->fixedSteps = ctx->fixedMax = ctx->fixedLeft = 0;
+}
+
+// The tape of pass `i` of a step (pass 2s is substep s's biased pass, 2s + 1 its relax pass).
+Tape tapeOf(const tfs_ctx* ctx, int i) {
+    const size_t cap = ctx->rows.cap;
+    float4* base = ctx->tapeBuf + (size_t)i * 9 * cap;
+    return Tape{ base, base + 4 * cap, base + 8 * cap, (uint32_t)cap };
+}
+
+Checkpoint checkpointOf(const tfs_ctx* ctx, uint32_t k) {
+    return Checkpoint{ ctx->ckBody + (size_t)k * 3 * ctx->nb, ctx->ckRow + (size_t)k * 5 * ctx->rows.cap };
+}
+
+// One step of the fixed-contact model, recording each pass into the tape when `record`.
+int fixedStep(tfs_ctx* ctx, bool record) {
+    const uint32_t nb = ctx->nb;
+    const DevCounts& hc = ctx->fixedCounts;
+    const uint32_t laned = hc.colorStart[TFS_COLORS];
+    const Ctx c = makeCtx(ctx->settings, ctx->fixedDt);
+    const int n = ctx->settings.substeps > 1 ? ctx->settings.substeps : 1;
+    int r;
+    auto pass = [&](int mode, int slot) {
+        const Tape t = record ? tapeOf(ctx, slot) : Tape{};
+        return ctx->usePass ? runPass(ctx, mode, hc, true, false, c, t)
+                            : runRows(ctx, mode, nullptr, hc.colorStart, hc.colorPoints, c, t);
+    };
+    for (int i = 0; i < n; ++i) {
+        LAUNCH(k_integrate_velocities, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c);
+        if ((r = warmRows(ctx, laned, hc.pointsActive)) || (r = pass(kBias, 2 * i))) { return r; }
+        LAUNCH(k_fx_positions, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c.h);
+        if ((r = pass(kRelax, 2 * i + 1))) { return r; }
+    }
+    LAUNCH(k_fx_rebase, blocksFor(laned), kThreads, ctx->rows, laned, ctx->motion);
+    LAUNCH(k_fx_reset_dp, blocksFor(nb), kThreads, nb, ctx->motion);
+    LAUNCH(k_writeback, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion);
+    return 0;
+}
+
+// The reverse of one pass: the colours last to first, in one cooperative launch (or one launch per
+// colour where cooperative launches are off).
+int passAdjoint(tfs_ctx* ctx, int mode, int slot, const Ctx& c) {
+    const DevCounts& hc = ctx->fixedCounts;
+    const Tape tape = tapeOf(ctx, slot);
+    const float4* mass = ctx->mass;
+    const bool biased = mode == kBias;
+    PassPlan plan{};
+    uint32_t widest = 0;
+    for (int col = TFS_COLORS - 1; col >= 0; --col) {
+        if (hc.colorStart[col + 1] <= hc.colorStart[col]) { continue; }
+        plan.begin[plan.n] = hc.colorStart[col];
+        plan.end[plan.n] = hc.colorStart[col + 1];
+        widest = std::max(widest, plan.end[plan.n] - plan.begin[plan.n]);
+        ++plan.n;
+    }
+    if (plan.n == 0) { return 0; }
+    const void* fn = ctx->madeAxes ? (biased ? (const void*)k_pass_adj<true, true> : (const void*)k_pass_adj<false, true>)
+                                   : (biased ? (const void*)k_pass_adj<true, false> : (const void*)k_pass_adj<false, false>);
+    if (ctx->usePass) {
+        const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((widest + 63) / 64, ctx->maxPassBlocks));
+        const BarrierBase base = ctx->barrierBase;
+        void* args[] = { &ctx->rows, &plan, (void*)&tape, (void*)&mass, (void*)&c, &ctx->adj, &ctx->bar, (void*)&base,
+                         &ctx->cnt };
+        HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kPassThreads), args, 0, ctx->stream));
+        advanceBarrier(ctx->barrierBase, grid, plan.n - 1);
+        return 0;
+    }
+    for (uint32_t k = 0; k < plan.n; ++k) {
+        PassPlan one{};
+        one.n = 1;
+        one.begin[0] = plan.begin[k];
+        one.end[0] = plan.end[k];
+        const uint32_t grid = (one.end[0] - one.begin[0] + kPassThreads - 1) / kPassThreads;
+        const BarrierBase base = ctx->barrierBase;
+        void* args[] = { &ctx->rows, &one, (void*)&tape, (void*)&mass, (void*)&c, &ctx->adj, &ctx->bar, (void*)&base,
+                         &ctx->cnt };
+        HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(kPassThreads), args, 0, ctx->stream));
+    }
+    return 0;
+}
+
+int checkErrors(tfs_ctx* ctx) {
+    HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->hostCnt->error) { return fail(ctx, TFS_E_KERNEL, "a kernel's bounded wait ran out (error %u)", ctx->hostCnt->error); }
+    return 0;
+}
+
+int fixedBegin(tfs_ctx* ctx, float dt, uint32_t maxSteps) {
+    freeFixed(ctx);
+    DevCounts hc{};
+    int r;
+    if ((r = beginStep(ctx, dt, hc))) { return r; }
+    const uint32_t laned = hc.colorStart[TFS_COLORS];
+    if (hc.nActive > laned) {
+        return fail(ctx, TFS_E_STATE, "the fixed-contact model needs every active contact in a colour (%u of %u are not)",
+                    hc.nActive - laned, hc.nActive);
+    }
+    const size_t cap = ctx->rows.cap, nb = ctx->nb, passes = 2 * (size_t)std::max(ctx->settings.substeps, 1);
+    auto grab = [&](void** p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 16)) == hipSuccess; };
+    if (!grab((void**)&ctx->ckBody, maxSteps * 3 * nb * sizeof(float4)) ||
+        !grab((void**)&ctx->ckRow, maxSteps * 5 * cap * sizeof(float4)) ||
+        !grab((void**)&ctx->tapeBuf, passes * 9 * cap * sizeof(float4)) ||
+        !grab((void**)&ctx->adj.bar, 2 * nb * sizeof(float4)) || !grab((void**)&ctx->adj.dpBar, nb * sizeof(float4)) ||
+        !grab((void**)&ctx->adj.comBar, nb * sizeof(float4)) ||
+        !grab((void**)&ctx->adj.impBar, 4 * cap * sizeof(float4)) ||
+        !grab((void**)&ctx->adj.sepBar, 4 * cap * sizeof(float)) || !grab((void**)&ctx->adj.muBar, cap * sizeof(float)) ||
+        !grab((void**)&ctx->dlcom, 3 * nb * sizeof(float)) ||
+        !grab((void**)&ctx->muContact, std::max<size_t>(ctx->nc, 1) * sizeof(float))) {
+        freeFixed(ctx);
+        return fail(ctx, TFS_E_DEVICE, "out of device memory for %u checkpoints", maxSteps);
+    }
+    uint32_t nColors = 0;
+    for (int col = 0; col < TFS_COLORS; ++col) {
+        if (hc.colorStart[col + 1] > hc.colorStart[col]) { nColors = (uint32_t)col + 1; }
+    }
+    const uint32_t counts[8] = { hc.nActive, hc.pointsActive, 0, 0, nColors, 0, 0, 0 };
+    memcpy(ctx->lastCounts, counts, sizeof counts);
+    ctx->fixedCounts = hc;
+    ctx->fixedDt = dt;
+    ctx->fixedMax = maxSteps;
+    ctx->fixedOn = true;
+    return checkErrors(ctx);
+}
+
 }  // namespace
 
 extern "C" {
@@ -2233,7 +2707,7 @@ int tfs_create(int device, uint32_t maxBodies, uint32_t maxContacts, tfs_ctx** o
     if (!r && hipHostMalloc((void**)&ctx->hostCnt, sizeof(DevCounts), hipHostMallocDefault) != hipSuccess) { r = TFS_E_DEVICE; }
     if (!r) {
         int perCu = 0, cus = 0, coop = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, (const void*)k_pass<kRelax, false>, kPassThreads, 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, (const void*)k_pass<kRelax, false, true>, kPassThreads, 0) != hipSuccess ||
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
             hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device) != hipSuccess) {
             r = TFS_E_DEVICE;
@@ -2275,7 +2749,8 @@ void tfs_destroy(tfs_ctx* ctx) {
                      ctx->edges, ctx->bodyWorld, ctx->refreshing, ctx->worldStart, ctx->restTracked,
                      ctx->restDeg, ctx->restStart, ctx->restCursor, ctx->restEp, ctx->restRank, ctx->restLevel,
                      ctx->restCount, ctx->restWord, ctx->restEntries[0], ctx->restEntries[1], ctx->restLevelStart[0],
-                     ctx->restLevelStart[1] };
+                     ctx->restLevelStart[1], ctx->ckBody, ctx->ckRow, ctx->tapeBuf, ctx->adj.bar, ctx->adj.dpBar,
+                     ctx->adj.comBar, ctx->adj.impBar, ctx->adj.sepBar, ctx->adj.muBar, ctx->dlcom, ctx->muContact };
     for (void* p : ptrs) {
         if (p) { (void)hipFree(p); }
     }
@@ -2302,6 +2777,7 @@ int tfs_set_settings(tfs_ctx* ctx, const tfs_settings* s) {
 
 int tfs_upload_bodies(tfs_ctx* ctx, const tfs_body* bodies, uint32_t n) {
     if (!ctx || (!bodies && n)) { return fail(ctx, TFS_E_ARG, "null argument"); }
+    ctx->fixedOn = false;
     HIP_TRY(hipSetDevice(ctx->device));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (n > ctx->capB) {
@@ -2318,6 +2794,7 @@ int tfs_upload_bodies(tfs_ctx* ctx, const tfs_body* bodies, uint32_t n) {
 
 int tfs_set_worlds(tfs_ctx* ctx, uint32_t n_worlds, const uint32_t* body_start) {
     if (!ctx || !body_start || n_worlds == 0) { return fail(ctx, TFS_E_ARG, "null argument or no world"); }
+    ctx->fixedOn = false;
     if (!ctx->haveBodies) { return fail(ctx, TFS_E_STATE, "upload bodies before setting worlds"); }
     if (body_start[0] != 0 || body_start[n_worlds] != ctx->nb) {
         return fail(ctx, TFS_E_ARG, "world ranges must cover the %u bodies", ctx->nb);
@@ -2346,6 +2823,7 @@ int tfs_set_worlds(tfs_ctx* ctx, uint32_t n_worlds, const uint32_t* body_start) 
 
 int tfs_upload_contacts(tfs_ctx* ctx, const tfs_contact* contacts, uint32_t n) {
     if (!ctx || (!contacts && n)) { return fail(ctx, TFS_E_ARG, "null argument"); }
+    ctx->fixedOn = false;
     if (!ctx->haveBodies) { return fail(ctx, TFS_E_STATE, "upload bodies before contacts"); }
     for (uint32_t i = 0; i < n; ++i) {
         const tfs_contact& c = contacts[i];
@@ -2374,7 +2852,100 @@ int tfs_step(tfs_ctx* ctx, float dt, tfs_refresh_fn refresh, void* user) {
     if (!ctx->haveBodies || !ctx->haveContacts) { return fail(ctx, TFS_E_STATE, "upload bodies and contacts first"); }
     if (!(dt > 0.0f) || !(dt <= 3.4e38f)) { return 0; }
     if (hipSetDevice(ctx->device) != hipSuccess) { return fail(ctx, TFS_E_DEVICE, "hipSetDevice failed"); }
+    ctx->fixedOn = false;
     return step(ctx, dt, refresh, user);
+}
+
+int tfs_fixed_begin(tfs_ctx* ctx, float dt, uint32_t max_steps) {
+    if (!ctx) { return TFS_E_ARG; }
+    if (!ctx->haveBodies || !ctx->haveContacts) { return fail(ctx, TFS_E_STATE, "upload bodies and contacts first"); }
+    if (!(dt > 0.0f) || !(dt <= 3.4e38f) || max_steps == 0) { return fail(ctx, TFS_E_ARG, "dt and max_steps must be positive"); }
+    if (hipSetDevice(ctx->device) != hipSuccess) { return fail(ctx, TFS_E_DEVICE, "hipSetDevice failed"); }
+    return fixedBegin(ctx, dt, max_steps);
+}
+
+int tfs_fixed_step(tfs_ctx* ctx) {
+    if (!ctx) { return TFS_E_ARG; }
+    if (!ctx->fixedOn) { return fail(ctx, TFS_E_STATE, "call tfs_fixed_begin first"); }
+    if (ctx->fixedSteps >= ctx->fixedMax) { return fail(ctx, TFS_E_STATE, "all %u checkpoints are taken", ctx->fixedMax); }
+    HIP_TRY(hipSetDevice(ctx->device));
+    const uint32_t nr = ctx->fixedCounts.colorStart[TFS_COLORS];
+    LAUNCH(k_fx_save, blocksFor(std::max(ctx->nb, nr)), kThreads, ctx->bodies, ctx->nb, ctx->motion, ctx->rows, nr,
+           checkpointOf(ctx, ctx->fixedSteps));
+    int r;
+    if ((r = fixedStep(ctx, false))) { return r; }
+    ++ctx->fixedSteps;
+    ctx->fixedLeft = 0;
+    return checkErrors(ctx);
+}
+
+int tfs_adjoint_seed(tfs_ctx* ctx, const float* dl_dcom) {
+    if (!ctx || !dl_dcom) { return TFS_E_ARG; }
+    if (!ctx->fixedOn || ctx->fixedSteps == 0) { return fail(ctx, TFS_E_STATE, "take fixed-contact steps first"); }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipMemcpyAsync(ctx->dlcom, dl_dcom, 3 * sizeof(float) * ctx->nb, hipMemcpyHostToDevice, ctx->stream));
+    const uint32_t nr = ctx->fixedCounts.colorStart[TFS_COLORS];
+    LAUNCH(k_adj_seed, blocksFor(std::max(ctx->nb, nr)), kThreads, ctx->nb, ctx->dlcom, nr, ctx->rows.cap, ctx->adj);
+    ctx->fixedLeft = ctx->fixedSteps;
+    return checkErrors(ctx);
+}
+
+int tfs_adjoint_step(tfs_ctx* ctx) {
+    if (!ctx) { return TFS_E_ARG; }
+    if (!ctx->fixedOn || ctx->fixedLeft == 0) { return fail(ctx, TFS_E_STATE, "no recorded step left to reverse"); }
+    HIP_TRY(hipSetDevice(ctx->device));
+    const uint32_t nb = ctx->nb, nr = ctx->fixedCounts.colorStart[TFS_COLORS];
+    const uint32_t k = ctx->fixedLeft - 1;
+    const Ctx c = makeCtx(ctx->settings, ctx->fixedDt);
+    const int n = ctx->settings.substeps > 1 ? ctx->settings.substeps : 1;
+    // the step again from its checkpoint, recording what each row saw
+    LAUNCH(k_fx_restore, blocksFor(std::max(nb, nr)), kThreads, ctx->bodies, nb, ctx->motion, ctx->rows, nr,
+           checkpointOf(ctx, k));
+    int r;
+    if ((r = fixedStep(ctx, true))) { return r; }
+    // then backwards through it
+    LAUNCH(k_adj_rebase, blocksFor(nb), kThreads, nb, ctx->bdeg, ctx->bstart, ctx->bep, ctx->rows, ctx->adj);
+    for (int i = n - 1; i >= 0; --i) {
+        if ((r = passAdjoint(ctx, kRelax, 2 * i + 1, c))) { return r; }
+        LAUNCH(k_adj_positions, blocksFor(nb), kThreads, ctx->bodies, nb, c.h, ctx->adj);
+        if ((r = passAdjoint(ctx, kBias, 2 * i, c))) { return r; }
+        LAUNCH(k_adj_warm, blocksFor(nr), kThreads, ctx->rows, nr, ctx->adj);
+    }
+    ctx->fixedLeft = k;
+    return checkErrors(ctx);
+}
+
+int tfs_adjoint_read(tfs_ctx* ctx, float* dl_dv, float* dl_dw, float* dl_dfriction) {
+    if (!ctx) { return TFS_E_ARG; }
+    if (!ctx->fixedOn) { return fail(ctx, TFS_E_STATE, "call tfs_fixed_begin first"); }
+    HIP_TRY(hipSetDevice(ctx->device));
+    const uint32_t nb = ctx->nb, nr = ctx->fixedCounts.colorStart[TFS_COLORS];
+    if (dl_dfriction && ctx->nc) {
+        HIP_TRY(hipMemsetAsync(ctx->muContact, 0, sizeof(float) * ctx->nc, ctx->stream));
+        LAUNCH(k_adj_friction, blocksFor(nr), kThreads, ctx->rows, nr, ctx->adj.muBar, ctx->muContact);
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (dl_dv || dl_dw) {
+        std::vector<float4> bar(2 * (size_t)nb);
+        if (nb) { HIP_TRY(hipMemcpy(bar.data(), ctx->adj.bar, sizeof(float4) * 2 * nb, hipMemcpyDeviceToHost)); }
+        for (uint32_t i = 0; i < nb; ++i) {
+            const float4 v = bar[2 * i], w = bar[2 * i + 1];
+            if (dl_dv) {
+                dl_dv[3 * i] = v.x;
+                dl_dv[3 * i + 1] = v.y;
+                dl_dv[3 * i + 2] = v.z;
+            }
+            if (dl_dw) {
+                dl_dw[3 * i] = w.x;
+                dl_dw[3 * i + 1] = w.y;
+                dl_dw[3 * i + 2] = w.z;
+            }
+        }
+    }
+    if (dl_dfriction && ctx->nc) {
+        HIP_TRY(hipMemcpy(dl_dfriction, ctx->muContact, sizeof(float) * ctx->nc, hipMemcpyDeviceToHost));
+    }
+    return 0;
 }
 
 int tfs_download_bodies(tfs_ctx* ctx, tfs_body* out, uint32_t n) {

@@ -271,6 +271,72 @@ TF_HD void warmRowN(const Row& r, const RowPoint* p, const AX& ax, Motion& a, Mo
     }
 }
 
+// The current separation of a point (solveLanes, src/lanes.cpp:236-240): its anchors turned by the
+// bodies' rotation this step, plus their travel dp = b.dp - a.dp, along the normal.
+TF_HD float pointSeparation(const Motion& a, const Motion& b, V3 dp, const RowPoint& p, V3 n) {
+    const V3 d = dp + turned(b, p.rb) - turned(a, p.ra);
+    return dot(d, n) + p.adj;
+}
+
+// What the reverse sweep needs of one point's normal solve: the scales its bias branch picked, d bias
+// / d sep, and whether the clamp at zero let the impulse through.
+struct NormalTrace {
+    float massScale, impulseScale, dbias;
+    bool active;
+};
+
+// One point's normal row (src/lanes.cpp:241-262).
+template <bool BIAS>
+TF_HD NormalTrace normalPoint(RowPoint& p, float sep, const Axis& x, V3 n, Motion& a, Motion& b, const Ctx& c,
+                              const Soft& soft, bool moveA, bool moveB) {
+    float bias = 0.0f, massScale = 1.0f, impulseScale = 0.0f, dbias = 0.0f;
+    if (sep > 0.0f) {
+        bias = sep * c.invH;
+        dbias = c.invH;
+    } else if (BIAS) {
+        bias = maxf(soft.biasRate * sep, -c.maxPushout);
+        massScale = soft.massScale;
+        impulseScale = soft.impulseScale;
+        dbias = soft.biasRate * sep > -c.maxPushout ? soft.biasRate : 0.0f;
+    }
+    const float vn = speedAlong(a, b, n, x);
+    float impulse = -x.mass * massScale * (vn + bias) - impulseScale * p.ni;
+    const float sum = p.ni + impulse;
+    const float total = maxf(sum, 0.0f);
+    impulse = total - p.ni;
+    p.ni = total;
+    p.nimax = maxf(p.nimax, impulse);
+    push(a, b, x, impulse, moveA, moveB);
+    return NormalTrace{ massScale, impulseScale, dbias, sum > 0.0f };
+}
+
+// What the reverse sweep needs of one point's friction solve: the unclamped forces, their length,
+// the cone's radius and whether the forces were scaled onto it.
+struct FrictionTrace {
+    float f1, f2, mag, limit;
+    bool scaled;
+};
+
+// One point's friction cone (src/lanes.cpp:263-304).
+TF_HD FrictionTrace frictionPoint(RowPoint& p, const Axis& x1, const Axis& x2, const Row& r, Motion& a, Motion& b,
+                                  bool moveA, bool moveB) {
+    const float old1 = dot(p.ti, r.t1), old2 = dot(p.ti, r.t2);
+    float f1 = old1 - x1.mass * speedAlong(a, b, r.t1, x1);
+    float f2 = old2 - x2.mass * speedAlong(a, b, r.t2, x2);
+    const float limit = r.friction * p.ni;
+    const float mag = fsqrt(f1 * f1 + f2 * f2);
+    const FrictionTrace t{ f1, f2, mag, limit, mag > limit };
+    if (mag > limit) {
+        const float scale = mag > 0.0f ? limit / mag : 0.0f;
+        f1 *= scale;
+        f2 *= scale;
+    }
+    p.ti = r.t1 * f1 + r.t2 * f2;
+    push(a, b, x1, f1 - old1, moveA, moveB);
+    push(a, b, x2, f2 - old2, moveA, moveB);
+    return t;
+}
+
 // The 3D contact solve: solveLanes for one lane (src/lanes.cpp:220-315), which is also the 3D part
 // of solveContacts (src/solver.cpp:370-417).
 template <int N, bool BIAS, class AX>
@@ -281,44 +347,111 @@ TF_HD void solveRowN(const Row& r, RowPoint* p, const AX& ax, Motion& a, Motion&
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         if (i >= count) { continue; }
-        const V3 d = dp + turned(b, p[i].rb) - turned(a, p[i].ra);
-        const float sep = dot(d, r.n) + p[i].adj;
-        float bias = 0.0f, massScale = 1.0f, impulseScale = 0.0f;
-        if (sep > 0.0f) {
-            bias = sep * c.invH;
-        } else if (BIAS) {
-            bias = maxf(soft.biasRate * sep, -c.maxPushout);
-            massScale = soft.massScale;
-            impulseScale = soft.impulseScale;
-        }
-        const Axis x = ax(i, 0);
-        const float vn = speedAlong(a, b, r.n, x);
-        float impulse = -x.mass * massScale * (vn + bias) - impulseScale * p[i].ni;
-        const float total = maxf(p[i].ni + impulse, 0.0f);
-        impulse = total - p[i].ni;
-        p[i].ni = total;
-        p[i].nimax = maxf(p[i].nimax, impulse);
-        push(a, b, x, impulse, moveA, moveB);
+        normalPoint<BIAS>(p[i], pointSeparation(a, b, dp, p[i], r.n), ax(i, 0), r.n, a, b, c, soft, moveA, moveB);
     }
 #pragma unroll
     for (int j = 0; j < N; ++j) {
         const int i = BIAS ? j : N - 1 - j;
         if (i >= count) { continue; }
+        frictionPoint(p[i], ax(i, 1), ax(i, 2), r, a, b, moveA, moveB);
+    }
+}
+
+// ---- the adjoint of a row ----------------------------------------------------------------------
+
+// Adjoints of the two bodies' velocities.
+struct MotionBar {
+    V3 va, wa, vb, wb;
+};
+
+// d push / d impulse, dotted with the velocity adjoints: how the loss moves with the impulse a push
+// applies (push, src/solver.cpp:23-28, masked as the lanes mask it).
+TF_HD float pushBar(const Axis& x, const MotionBar& g, bool moveA, bool moveB) {
+    float s = 0.0f;
+    if (moveA) { s = s - dot(x.linearA, g.va) - dot(x.angularA, g.wa); }
+    if (moveB) { s = s + dot(x.linearB, g.vb) + dot(x.angularB, g.wb); }
+    return s;
+}
+
+// speedAlong (src/solver.cpp:19-21) is linear in the velocities: its adjoint s adds J^T s.
+TF_HD void speedBar(const Axis& x, V3 d, float s, MotionBar& g) {
+    g.va = g.va - d * s;
+    g.wa = g.wa - x.armA * s;
+    g.vb = g.vb + d * s;
+    g.wb = g.wb + x.armB * s;
+}
+
+// The reverse sweep of solveRowN for one row. Inputs are what the forward solve saw: the bodies'
+// velocities a, b, the points' impulses p (ni, ti) and separations sep. The row is solved again from
+// them (the same operations, so the same clamps), then the adjoints of its outputs, g (velocities),
+// niB and tiB (impulses), are turned into those of its inputs, in place; sepB receives d loss /
+// d separation per point, and muB adds d loss / d friction. The friction cone's projection
+// f * limit / |f| has the derivative (limit / |f|) (I - f^ f^T) in f and f^ in the limit; with a zero
+// limit the cone is a point and the forces do not depend on f at all.
+template <bool BIAS, class AX>
+TF_HD void solveRowAdjoint(const Row& r, RowPoint* p, const float* sep, const AX& ax, Motion a, Motion b, const Ctx& c,
+                           int count, MotionBar& g, float* niB, V3* tiB, float* sepB, float& muB) {
+    const bool moveA = r.flags & kMoveA, moveB = r.flags & kMoveB;
+    const Soft soft = (moveA && moveB) ? c.contactSoft : c.staticSoft;
+    NormalTrace nt[4];
+    FrictionTrace ft[4];
+    float lamF[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (i >= count) { continue; }
+        nt[i] = normalPoint<BIAS>(p[i], sep[i], ax(i, 0), r.n, a, b, c, soft, moveA, moveB);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int i = BIAS ? j : 3 - j;
+        if (i >= count) { continue; }
+        lamF[i] = p[i].ni;
+        ft[i] = frictionPoint(p[i], ax(i, 1), ax(i, 2), r, a, b, moveA, moveB);
+    }
+#pragma unroll
+    for (int j = 3; j >= 0; --j) {
+        const int i = BIAS ? j : 3 - j;
+        if (i >= count) { continue; }
         const Axis x1 = ax(i, 1);
         const Axis x2 = ax(i, 2);
-        const float old1 = dot(p[i].ti, r.t1), old2 = dot(p[i].ti, r.t2);
-        float f1 = old1 - x1.mass * speedAlong(a, b, r.t1, x1);
-        float f2 = old2 - x2.mass * speedAlong(a, b, r.t2, x2);
-        const float limit = r.friction * p[i].ni;
-        const float mag = fsqrt(f1 * f1 + f2 * f2);
-        if (mag > limit) {
-            const float scale = mag > 0.0f ? limit / mag : 0.0f;
-            f1 *= scale;
-            f2 *= scale;
+        const float d1 = pushBar(x1, g, moveA, moveB), d2 = pushBar(x2, g, moveA, moveB);
+        const float g1 = d1 + dot(r.t1, tiB[i]), g2 = d2 + dot(r.t2, tiB[i]);
+        const FrictionTrace& t = ft[i];
+        float f1b = g1, f2b = g2, limB = 0.0f;
+        if (t.scaled) {
+            const float h1 = t.mag > 0.0f ? t.f1 / t.mag : 0.0f, h2 = t.mag > 0.0f ? t.f2 / t.mag : 0.0f;
+            const float along = h1 * g1 + h2 * g2;
+            limB = along;
+            const float s = t.mag > 0.0f ? t.limit / t.mag : 0.0f;
+            f1b = s * (g1 - h1 * along);
+            f2b = s * (g2 - h2 * along);
         }
-        p[i].ti = r.t1 * f1 + r.t2 * f2;
-        push(a, b, x1, f1 - old1, moveA, moveB);
-        push(a, b, x2, f2 - old2, moveA, moveB);
+        if (!(t.limit > 0.0f)) {
+            f1b = 0.0f;
+            f2b = 0.0f;
+        }
+        muB += lamF[i] * limB;
+        niB[i] += r.friction * limB;
+        speedBar(x1, r.t1, -x1.mass * f1b, g);
+        speedBar(x2, r.t2, -x2.mass * f2b, g);
+        tiB[i] = r.t1 * (f1b - d1) + r.t2 * (f2b - d2);
+    }
+#pragma unroll
+    for (int i = 3; i >= 0; --i) {
+        if (i >= count) { continue; }
+        const Axis x = ax(i, 0);
+        const float d = pushBar(x, g, moveA, moveB);
+        const float totalB = niB[i] + d;
+        float inB = -d, impulseB = 0.0f;
+        if (nt[i].active) {
+            inB += totalB;
+            impulseB = totalB;
+        }
+        const float vnB = -x.mass * nt[i].massScale * impulseB;
+        inB -= nt[i].impulseScale * impulseB;
+        speedBar(x, r.n, vnB, g);
+        sepB[i] = nt[i].dbias * vnB;
+        niB[i] = inB;
     }
 }
 

@@ -332,3 +332,38 @@ def planar_pile(side=256, spheres=4096, spacing=1.3, seed=0x9E3779B9, steps=240)
 def planar_pile_small(steps=150):
     """A 10 x 10 planar pile with 12 spheres: a rest list of up to ~160 planar contacts."""
     return planar_pile(side=10, spheres=12, steps=steps)
+
+
+def throw_world(seed, steps=32, jitter=0.005):
+    """One world of the adjoint workload: 64 boxes of half-extent 0.25 in a 4 x 4 x 4 block of touching
+    boxes (spacing 0.5, created y-major: for y, z, x) and a wall of 10 such boxes (2 high, 5 wide)
+    against the block's +x face, offset half a box in z. No static body: everything is free and falls
+    together. Draws, in order: the world's friction U[0.2, 1.0] (on every shape); per block box a
+    jitter x, y, z in U[-jitter, jitter], then v0 ~ N(0, 2 m/s) per axis and w0 ~ N(0, 1 rad/s) per
+    axis (Box-Muller, two uniform draws each); per wall box a jitter x, y, z, then v0 and w0 ~ N(0,
+    0.05) per axis. Restitution is 0. The jitter keeps the points off zero separation, where the bias
+    switches form, and the wall's small velocities keep its contacts off exactly zero relative
+    velocity, where a clamp switches: both are kinks of the loss."""
+    s = Scene(f"throw-{seed:08x}", substeps=4, steps=steps)
+    rng = Rng(seed)
+    fr = rng.range(0.2, 1.0)
+    half = (0.25, 0.25, 0.25)
+    for y in range(4):
+        for z in range(4):
+            for x in range(4):
+                j = [rng.range(-jitter, jitter) for _ in range(3)]
+                p = ((f32(x) - f32(1.5)) * f32(0.5) + j[0], (f32(y) - f32(1.5)) * f32(0.5) + j[1],
+                     (f32(z) - f32(1.5)) * f32(0.5) + j[2])
+                v = [rng.normal(2.0) for _ in range(3)]
+                w = [rng.normal(1.0) for _ in range(3)]
+                b = s.body(DYNAMIC, pos=p, v=v, w=w)
+                s.shape(b, BOX, half, density=1000.0, friction=fr)
+    for y in range(2):
+        for z in range(5):
+            j = [rng.range(-jitter, jitter) for _ in range(3)]
+            p = (f32(1.25) + j[0], (f32(y) - f32(0.5)) * f32(0.5) + j[1], (f32(z) - f32(2.0)) * f32(0.5) + j[2])
+            v = [rng.normal(0.05) for _ in range(3)]
+            w = [rng.normal(0.05) for _ in range(3)]
+            b = s.body(DYNAMIC, pos=p, v=v, w=w)
+            s.shape(b, BOX, half, density=1000.0, friction=fr)
+    return s

@@ -23,7 +23,8 @@ REFRESH_FN = ctypes.CFUNCTYPE(ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, c
 EXPORTS = ("tfs_abi_version", "tfs_sizeof_body", "tfs_sizeof_contact", "tfs_sizeof_settings", "tfs_default_settings",
            "tfs_create", "tfs_destroy", "tfs_last_error", "tfs_set_settings", "tfs_upload_bodies",
            "tfs_upload_contacts", "tfs_set_worlds", "tfs_step", "tfs_download_bodies", "tfs_download_contacts", "tfs_download_list",
-           "tfs_last_step_ms", "tfs_last_step_counts", "tfs_set_profiling", "tfs_read_profile")
+           "tfs_last_step_ms", "tfs_last_step_counts", "tfs_set_profiling", "tfs_read_profile", "tfs_fixed_begin",
+           "tfs_fixed_step", "tfs_adjoint_seed", "tfs_adjoint_step", "tfs_adjoint_read")
 
 
 class SolverError(RuntimeError):
@@ -60,6 +61,11 @@ def load_library(path=LIB_PATH):
     lib.tfs_last_step_ms.argtypes = [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
     lib.tfs_last_step_counts.argtypes = [_vp, _vp]
     lib.tfs_set_profiling.argtypes = [_vp, ctypes.c_int]
+    lib.tfs_fixed_begin.argtypes = [_vp, ctypes.c_float, ctypes.c_uint32]
+    lib.tfs_fixed_step.argtypes = [_vp]
+    lib.tfs_adjoint_seed.argtypes = [_vp, _vp]
+    lib.tfs_adjoint_step.argtypes = [_vp]
+    lib.tfs_adjoint_read.argtypes = [_vp, _vp, _vp, _vp]
     lib.tfs_read_profile.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_double)]
     if lib.tfs_abi_version() != 1:
@@ -208,6 +214,34 @@ class Solver:
         self._check(self.lib.tfs_read_profile(self.h, family, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by)),
                     "tfs_read_profile")
         return n.value, ms.value, by.value
+
+    # ---- the fixed-contact model and its adjoint ----
+
+    def fixed_begin(self, dt, max_steps):
+        """Fixes the resident contacts for the fixed-contact model, with room for max_steps steps."""
+        self._check(self.lib.tfs_fixed_begin(self.h, ctypes.c_float(dt), max_steps), "tfs_fixed_begin")
+
+    def fixed_step(self):
+        """Checkpoints the state, then takes one step of the fixed-contact model."""
+        self._check(self.lib.tfs_fixed_step(self.h), "tfs_fixed_step")
+
+    def adjoint_seed(self, dl_dcom):
+        """d loss / d com after the last step (n x 3); clears the other adjoints."""
+        g = np.ascontiguousarray(dl_dcom, np.float32).reshape(self.nb, 3)
+        self._check(self.lib.tfs_adjoint_seed(self.h, _ptr(g)), "tfs_adjoint_seed")
+
+    def adjoint_step(self):
+        """Reverses the last step not yet reversed."""
+        self._check(self.lib.tfs_adjoint_step(self.h), "tfs_adjoint_step")
+
+    def adjoint_read(self):
+        """(d loss / d v, d loss / d w) per body (n x 3 each) at the start of the earliest step reversed,
+        and d loss / d friction per contact."""
+        gv = np.zeros((self.nb, 3), np.float32)
+        gw = np.zeros((self.nb, 3), np.float32)
+        gm = np.zeros(max(self.nc, 1), np.float32)
+        self._check(self.lib.tfs_adjoint_read(self.h, _ptr(gv), _ptr(gw), _ptr(gm)), "tfs_adjoint_read")
+        return gv, gw, gm[:self.nc]
 
     def counts(self):
         """active contacts, their points, impact contacts, their points, colours used, rest contacts, and
