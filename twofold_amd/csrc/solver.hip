@@ -835,65 +835,79 @@ constexpr int kMaxSegments = 2 * TFS_COLORS;
 struct PassPlan {
     uint32_t n;                 // segments
     uint32_t impactFrom;        // segments from here are impact rows
+    uint32_t tailFrom;          // segments from here are solved by block 0 alone (passTail)
     uint32_t begin[kMaxSegments], end[kMaxSegments];
 };
 
 // The grid-wide barrier of a cooperative launch. Blocks arrive in kBarrierGroups groups (block b in
-// group b % kBarrierGroups), each on a counter of its own cache line, so arrivals on different
-// counters do not queue behind each other; the last block of a group to arrive adds the group's
-// arrival to the top counter, which every block waits on. Counters only grow: a launch is given
-// each counter's value before it (BarrierBase) and knows how much each barrier adds.
+// group b % kBarrierGroups), each group on a counter of its own cache line, with an atomic add whose
+// result nobody waits for; then the first wave of every block reads all the group counters at once,
+// one lane per group, until every group is complete. Counters only grow: a launch is given each
+// counter's value before it (BarrierBase) and knows how much each barrier adds.
 constexpr uint32_t kBarrierGroups = 8;
 struct alignas(128) BarrierLine {
     unsigned long long n;
     unsigned long long pad_[15];
 };
 struct BarrierLines {
-    BarrierLine top;
     BarrierLine group[kBarrierGroups];
 };
 struct BarrierBase {
-    unsigned long long top;
     unsigned long long group[kBarrierGroups];
 };
 
-// The targets one block waits for, advanced by one barrier at a time.
+__host__ __device__ __forceinline__ uint32_t groupBlocks(uint32_t grid, uint32_t gi) {
+    return gi < grid ? (grid - gi + kBarrierGroups - 1) / kBarrierGroups : 0u;
+}
+
+// What lane l of a block's first wave waits for: group l's count after the next barrier (lanes past
+// the last group watch nothing).
 struct BarrierPhase {
-    unsigned long long top, group;
-    __device__ __forceinline__ explicit BarrierPhase(const BarrierBase& b)
-        : top(b.top), group(b.group[blockIdx.x % kBarrierGroups]) {}
+    unsigned long long target;
+    uint32_t step;
+    __device__ __forceinline__ explicit BarrierPhase(const BarrierBase& b) {
+        const uint32_t lane = threadIdx.x & 63u;
+        step = lane < kBarrierGroups ? groupBlocks(gridDim.x, lane) : 0u;
+        target = step ? b.group[lane] : 0ull;
+    }
 };
 
 // The base of the launch after one with `barriers` barriers over `grid` blocks.
 inline void advanceBarrier(BarrierBase& b, uint32_t grid, uint32_t barriers) {
-    const uint32_t groups = std::min(grid, kBarrierGroups);
-    for (uint32_t gi = 0; gi < groups; ++gi) {
-        b.group[gi] += (unsigned long long)barriers * ((grid - gi + kBarrierGroups - 1) / kBarrierGroups);
-    }
-    b.top += (unsigned long long)barriers * groups;
+    for (uint32_t gi = 0; gi < kBarrierGroups; ++gi) { b.group[gi] += (unsigned long long)barriers * groupBlocks(grid, gi); }
 }
 
-// Every block waits for its own coherent stores to complete, then its first thread arrives and
-// waits until every block has. Rows exchange data only through coherent accesses, so no cache needs
-// flushing.
-__device__ __forceinline__ void gridSync(BarrierLines* bar, BarrierPhase& phase, DevCounts* cnt) {
+// Every block waits for its own coherent stores to complete, then arrives (gridArrive) and waits
+// until every block has (gridWait). Rows exchange data only through coherent accesses, so no cache
+// needs flushing.
+__device__ __forceinline__ void gridArrive(BarrierLines* bar, BarrierPhase& phase) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t gi = blockIdx.x % kBarrierGroups;
-        phase.group += (gridDim.x - gi + kBarrierGroups - 1) / kBarrierGroups;
-        phase.top += gridDim.x < kBarrierGroups ? gridDim.x : kBarrierGroups;
-        if (atomicAdd(&bar->group[gi].n, 1ull) + 1ull == phase.group) { atomicAdd(&bar->top.n, 1ull); }
+    if (threadIdx.x == 0) { atomicAdd(&bar->group[blockIdx.x % kBarrierGroups].n, 1ull); }
+    phase.target += phase.step;
+}
+
+__device__ __forceinline__ void gridWait(BarrierLines* bar, const BarrierPhase& phase, DevCounts* cnt) {
+    if (threadIdx.x < 64) {
+        const uint32_t lane = threadIdx.x;
         uint32_t spins = 0;
-        while (__hip_atomic_load(&bar->top.n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < phase.top) {
+        for (;;) {
+            const bool done = phase.step == 0 ||
+                              __hip_atomic_load(&bar->group[lane].n, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= phase.target;
+            if (__all(done)) { break; }
             if (++spins > kSpinLimit) {
-                atomicOr(&cnt->error, 2u);
+                if (lane == 0) { atomicOr(&cnt->error, 2u); }
                 break;
             }
             __builtin_amdgcn_s_sleep(1);
         }
     }
     __syncthreads();
+}
+
+__device__ __forceinline__ void gridSync(BarrierLines* bar, BarrierPhase& phase, DevCounts* cnt) {
+    gridArrive(bar, phase);
+    gridWait(bar, phase, cnt);
 }
 
 // A thread's slot in a pass over a list: slot t serves list entry begin + t. Consecutive slots form
@@ -903,8 +917,231 @@ __device__ __forceinline__ uint32_t passSlot() {
     return ((threadIdx.x >> 6) * gridDim.x + blockIdx.x) * 64u + (threadIdx.x & 63u);
 }
 
+// ---- the tail of a pass: its last small colours in one block ----------------------------------------
+//
+// The last colours of a pass are often a few rows each (the last active colours, and the impact rows'
+// colours), and each would cost a grid barrier plus a full round trip to memory for its rows. The
+// tail hands them to block 0 alone: every row of the tail gets one thread, which loads the row's data
+// into registers once and keeps its impulses there across the tail (an impact row whose active row
+// is also in the tail is the same thread's second visit); the bodies' velocities live in LDS slots
+// (a hash of body index to slot) shared by the tail's threads, and colours follow each other with
+// __syncthreads. The other blocks leave after the last grid barrier without waiting at it. Every
+// row's operations and their order are those of the grid path, so the results are the same bits.
+
+constexpr uint32_t kTailSlots = 512;      // LDS body slots (at most 2 per tail row)
+constexpr uint32_t kTailSegRows = 64;     // a tail colour has at most one wave of rows
+constexpr uint32_t kNoSlot = 0xffffffffu;
+
+// What a row reads that the tail does not change: directions, anchors, axes, the bodies' position
+// and rotation changes.
+template <bool MADE>
+struct RowData {
+    float4 n, t1, t2;
+    float4 pa[4], pb[4];
+    float4 axv[4][MADE ? 1 : 15];
+    float4 dpA, dqA, dpB, dqB;
+};
+
+template <bool MADE>
+__device__ __forceinline__ void loadRowData(const Rows& rows, uint32_t r, uint4 h, const float4* motion, RowData<MADE>& d) {
+    const int count = (int)(h.z & kCountMask);
+    d.n = rows.nrm[r];
+    d.t1 = rows.tg1[r];
+    d.t2 = rows.tg2[r];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k < count) {
+            const uint32_t j = (uint32_t)k * rows.cap + r;
+            d.pa[k] = rows.pa[j];
+            d.pb[k] = rows.pb[j];
+            if (!MADE) {
+#pragma unroll
+                for (int q = 0; q < 15; ++q) { d.axv[k][q] = rows.ax[(size_t)(k * 15 + q) * rows.cap + r]; }
+            }
+        }
+    }
+    d.dpA = motion[4 * h.x + 2];
+    d.dqA = motion[4 * h.x + 3];
+    d.dpB = motion[4 * h.y + 2];
+    d.dqB = motion[4 * h.y + 3];
+}
+
+__device__ __forceinline__ uint32_t tailSlot(uint32_t* key, uint32_t body) {
+    uint32_t k = (body * 2654435761u) >> 23;   // 9 bits: kTailSlots = 512
+    for (uint32_t probe = 0; probe < kTailSlots; ++probe, k = (k + 1) & (kTailSlots - 1)) {
+        const uint32_t old = atomicCAS(&key[k], kNoSlot, body);
+        if (old == kNoSlot || old == body) { return k; }
+    }
+    return kNoSlot;
+}
+
+// Block 0's part of a pass from segment plan.tailFrom on. With `wait`, the grid phase before it
+// ended with gridArrive: data that no row of the pass writes is loaded before gridWait, velocities and
+// impulses after it.
+template <int MODE, bool MADE, bool TAPE>
+__device__ __forceinline__ void passTail(const Rows& rows, const uint4* impactHdr, const PassPlan& plan, float4* motion,
+                                         const float4* mass, const Ctx& ctx, const Tape& tape, BarrierLines* bar,
+                                         const BarrierPhase& phase, DevCounts* cnt, bool wait) {
+    using X = Access<true>;
+    __shared__ uint32_t slotKey[kTailSlots];
+    __shared__ float4 slotV[kTailSlots], slotW[kTailSlots];
+    __shared__ uint32_t forward[kPassThreads];
+    const uint32_t t = threadIdx.x, tf = plan.tailFrom;
+    const bool activeTail = tf < plan.impactFrom;
+    const uint32_t atBegin = activeTail ? plan.begin[tf] : 0u, atEnd = activeTail ? plan.end[plan.impactFrom - 1] : 0u;
+    const uint32_t firstImpact = tf > plan.impactFrom ? tf : plan.impactFrom;
+    const uint32_t itBegin = firstImpact < plan.n ? plan.begin[firstImpact] : 0u;
+    const uint32_t itEnd = firstImpact < plan.n ? plan.end[plan.n - 1] : 0u;
+    const uint32_t atCount = atEnd - atBegin, itCount = itEnd - itBegin;
+    for (uint32_t k = t; k < kTailSlots; k += blockDim.x) { slotKey[k] = kNoSlot; }
+    forward[t] = kNoSlot;
+    __syncthreads();
+    // Ownership: thread t < atCount owns active row atBegin + t; thread atCount + k the k-th impact
+    // visit, unless its active row is in the tail, whose owner then also takes the visit.
+    uint32_t row = kNoSlot, activeSeg = kNoSlot, impactSeg = kNoSlot;
+    uint4 h = make_uint4(0u, 0u, 0u, 0u);
+    if (t < atCount) {
+        row = atBegin + t;
+        h = rows.hdr[row];
+        for (uint32_t s = tf; s < plan.impactFrom; ++s) {
+            if (row >= plan.begin[s] && row < plan.end[s]) { activeSeg = s; }
+        }
+    } else if (t < atCount + itCount) {
+        const uint32_t i = itBegin + (t - atCount);
+        const uint4 hi = impactHdr[i];
+        uint32_t seg = kNoSlot;
+        for (uint32_t s = firstImpact; s < plan.n; ++s) {
+            if (i >= plan.begin[s] && i < plan.end[s]) { seg = s; }
+        }
+        if (hi.w >= atBegin && hi.w < atEnd) {
+            forward[hi.w - atBegin] = seg;
+        } else {
+            row = hi.w;
+            h = hi;
+            impactSeg = seg;
+        }
+    }
+    __syncthreads();
+    if (t < atCount) { impactSeg = forward[t]; }
+    const int count = (int)(h.z & kCountMask);
+    RowData<MADE> d;
+    uint32_t sa = kNoSlot, sb = kNoSlot;
+    if (row != kNoSlot) {
+        loadRowData<MADE>(rows, row, h, motion, d);
+        sa = tailSlot(slotKey, h.x);
+        sb = tailSlot(slotKey, h.y);
+        if (sa == kNoSlot || sb == kNoSlot) { atomicOr(&cnt->error, 4u); }
+    }
+    // (the slot table holds every body of the tail, at most two per thread, so it cannot fill up)
+    const bool owner = row != kNoSlot && sa != kNoSlot && sb != kNoSlot;
+    if (wait) {
+        gridWait(bar, phase, cnt);
+    } else {
+        __syncthreads();
+    }
+    // Velocities and impulses, written by any block before the barrier: coherent loads.
+    for (uint32_t k = t; k < kTailSlots; k += blockDim.x) {
+        const uint32_t b = slotKey[k];
+        if (b != kNoSlot) {
+            slotV[k] = X::ld(&motion[4 * b]);
+            slotW[k] = X::ld(&motion[4 * b + 1]);
+        }
+    }
+    float4 im[4];
+    float peak[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (owner && k < count) {
+            const uint32_t j = (uint32_t)k * rows.cap + row;
+            im[k] = X::ld(&rows.imp[j]);
+            peak[k] = X::ld1(&rows.peak[j]);
+        }
+    }
+    __syncthreads();
+    const Row rowv{ h.z, vec(d.n.x, d.n.y, d.n.z), vec(d.t1.x, d.t1.y, d.t1.z), vec(d.t2.x, d.t2.y, d.t2.z), d.n.w, d.t1.w };
+    for (uint32_t s = tf; s < plan.n; ++s) {
+        if (owner && count > 0 && (s == activeSeg || s == impactSeg)) {
+            RowPoint p[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (k < count) {
+                    p[k].ra = vec(d.pa[k].x, d.pa[k].y, d.pa[k].z);
+                    p[k].adj = d.pa[k].w;
+                    p[k].rb = vec(d.pb[k].x, d.pb[k].y, d.pb[k].z);
+                    p[k].rv = d.pb[k].w;
+                    p[k].ni = im[k].x;
+                    p[k].ti = vec(im[k].y, im[k].z, im[k].w);
+                    p[k].nimax = peak[k];
+                }
+            }
+            const float4 va = slotV[sa], wa = slotW[sa], vb = slotV[sb], wb = slotW[sb];
+            Motion A{ vec(va.x, va.y, va.z), vec(wa.x, wa.y, wa.z), vec(d.dpA.x, d.dpA.y, d.dpA.z),
+                      quat(d.dqA.x, d.dqA.y, d.dqA.z, d.dqA.w) };
+            Motion B{ vec(vb.x, vb.y, vb.z), vec(wb.x, wb.y, wb.z), vec(d.dpB.x, d.dpB.y, d.dpB.z),
+                      quat(d.dqB.x, d.dqB.y, d.dqB.z, d.dqB.w) };
+            if (TAPE && tape.v && s == activeSeg) {
+                const uint32_t c = tape.cap;
+                tape.v[row] = va;
+                tape.v[c + row] = wa;
+                tape.v[2 * c + row] = vb;
+                tape.v[3 * c + row] = wb;
+                const V3 dp = B.dp - A.dp;
+                float sp[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    if (k < count) {
+                        tape.imp[(uint32_t)k * c + row] = im[k];
+                        sp[k] = pointSeparation(A, B, dp, p[k], rowv.n);
+                    }
+                }
+                tape.sep[row] = make_float4(sp[0], sp[1], sp[2], sp[3]);
+            }
+            if (MADE) {
+                const RowMadeAxes ax{ loadMass(mass, h.x), loadMass(mass, h.y), rowv, p };
+                tf::rows<MODE>(rowv, p, ax, A, B, ctx);
+            } else {
+                const RegAxes<MADE ? 1 : 15> ax{ d.axv };
+                tf::rows<MODE>(rowv, p, ax, A, B, ctx);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (k < count) {
+                    im[k] = make_float4(p[k].ni, p[k].ti.x, p[k].ti.y, p[k].ti.z);
+                    peak[k] = p[k].nimax;
+                }
+            }
+            if (h.z & kMoveA) {
+                slotV[sa] = make_float4(A.v.x, A.v.y, A.v.z, 0.0f);
+                slotW[sa] = make_float4(A.w.x, A.w.y, A.w.z, 0.0f);
+            }
+            if (h.z & kMoveB) {
+                slotV[sb] = make_float4(B.v.x, B.v.y, B.v.z, 0.0f);
+                slotW[sb] = make_float4(B.w.x, B.w.y, B.w.z, 0.0f);
+            }
+        }
+        __syncthreads();
+    }
+    // Back to memory: the impulses of every tail row, the velocities of every slot.
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (owner && k < count) {
+            const uint32_t j = (uint32_t)k * rows.cap + row;
+            rows.imp[j] = im[k];
+            rows.peak[j] = peak[k];
+        }
+    }
+    for (uint32_t k = t; k < kTailSlots; k += blockDim.x) {
+        const uint32_t b = slotKey[k];
+        if (b != kNoSlot) {
+            motion[4 * b] = slotV[k];
+            motion[4 * b + 1] = slotW[k];
+        }
+    }
+}
+
 // Headers do not change during a pass, so each thread loads the header of its first row in the
 // next segment before the barrier; after the barrier every other load of that row goes out at once.
+// Segments from plan.tailFrom on are the tail (passTail).
 template <int MODE, bool MADE, bool TAPE>
 __global__ void __launch_bounds__(kPassThreads) k_pass(Rows rows, const uint4* impactHdr, PassPlan plan,
                                                        float4* motion, const float4* mass, Ctx ctx,
@@ -913,18 +1150,26 @@ __global__ void __launch_bounds__(kPassThreads) k_pass(Rows rows, const uint4* i
     BarrierPhase phase(base);
     const uint32_t g = passSlot(), stride = gstride();
     auto header = [&](uint32_t s, uint32_t i) { return s >= plan.impactFrom ? impactHdr[i] : rows.hdr[i]; };
+    const uint32_t grid = plan.tailFrom;
     uint4 next = make_uint4(0u, 0u, 0u, 0u);
-    if (plan.n > 0 && plan.begin[0] + g < plan.end[0]) { next = header(0, plan.begin[0] + g); }
-    for (uint32_t s = 0; s < plan.n; ++s) {
+    if (grid > 0 && plan.begin[0] + g < plan.end[0]) { next = header(0, plan.begin[0] + g); }
+    for (uint32_t s = 0; s < grid; ++s) {
         const bool impact = s >= plan.impactFrom;
         const uint32_t first = plan.begin[s] + g;
         const uint4 firstHdr = next;
-        if (s + 1 < plan.n && plan.begin[s + 1] + g < plan.end[s + 1]) { next = header(s + 1, plan.begin[s + 1] + g); }
+        if (s + 1 < grid && plan.begin[s + 1] + g < plan.end[s + 1]) { next = header(s + 1, plan.begin[s + 1] + g); }
         for (uint32_t i = first; i < plan.end[s]; i += stride) {
             const uint4 h = i == first ? firstHdr : header(s, i);
             rowStep<MODE, true, MADE, TAPE>(rows, impact ? h.w : i, h, motion, mass, ctx, impact ? Tape{} : tape);
         }
-        if (s + 1 < plan.n) { gridSync(bar, phase, cnt); }
+        if (s + 1 < grid) {
+            gridSync(bar, phase, cnt);
+        } else if (grid < plan.n) {
+            gridArrive(bar, phase);
+        }
+    }
+    if (grid < plan.n && blockIdx.x == 0) {
+        passTail<MODE, MADE, TAPE>(rows, impactHdr, plan, motion, mass, ctx, tape, bar, phase, cnt, grid > 0);
     }
 }
 
@@ -1891,6 +2136,7 @@ struct tfs_ctx {
     BarrierBase barrierBase{};
     uint32_t maxPassBlocks = 0;
     bool usePass = true;
+    bool passTail = true;
     bool madeAxes = false;
     // per-body lists of coloured rows for the warm start
     uint32_t* bdeg = nullptr;
@@ -2136,7 +2382,21 @@ int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool w
     plan.impactFrom = plan.n;
     if (withImpact) { add(hc.impactColorStart, hc.impactColorPoints); }
     if (plan.n == 0) { return 0; }
+    // The tail: the trailing colours of at most one wave of rows each, at most one block of rows in
+    // all (the active rows and the impact visits), when there are at least two of them.
+    uint32_t tailFrom = plan.n, tailRows = 0;
+    while (ctx->passTail && tailFrom > 0) {
+        const uint32_t w = plan.end[tailFrom - 1] - plan.begin[tailFrom - 1];
+        if (w > kTailSegRows || tailRows + w > (uint32_t)kPassThreads) { break; }
+        tailRows += w;
+        --tailFrom;
+    }
+    if (plan.n - tailFrom < 2) { tailFrom = plan.n; }
+    plan.tailFrom = tailFrom;
+    widest = 0;
+    for (uint32_t s = 0; s < tailFrom; ++s) { widest = std::max(widest, plan.end[s] - plan.begin[s]); }
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((widest + 63) / 64, ctx->maxPassBlocks));
+    const uint32_t barriers = tailFrom == plan.n ? plan.n - 1 : tailFrom;
     size_t ev = 0;
     if (ctx->profiling) {
         ev = ctx->pending.size() * 2;
@@ -2160,7 +2420,7 @@ int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool w
                          : (mode == kBias ? (rec ? (const void*)k_pass<kBias, false, true> : (const void*)k_pass<kBias, false, false>)
                                           : (rec ? (const void*)k_pass<kRelax, false, true> : (const void*)k_pass<kRelax, false, false>));
     HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kPassThreads), args, 0, ctx->stream));
-    advanceBarrier(ctx->barrierBase, grid, plan.n - 1);
+    advanceBarrier(ctx->barrierBase, grid, barriers);
     if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
     return 0;
 }
@@ -2718,12 +2978,14 @@ int tfs_create(int device, uint32_t maxBodies, uint32_t maxContacts, tfs_ctx** o
         // a colour.
         ctx->maxPassBlocks = (uint32_t)std::max(1, std::min(perCu * cus / 2, 128));
         ctx->usePass = coop != 0 && perCu > 0;
-        // Tuning knobs, read once: TWOFOLD_PASS=launch|coop, TWOFOLD_PASS_BLOCKS=n, TWOFOLD_AXES=packed|made.
+        // Tuning knobs, read once: TWOFOLD_PASS=launch|coop, TWOFOLD_PASS_BLOCKS=n, TWOFOLD_AXES=packed|made,
+        // TWOFOLD_TAIL=on|off.
         if (const char* e = getenv("TWOFOLD_PASS")) { ctx->usePass = ctx->usePass && strcmp(e, "launch") != 0; }
         if (const char* e = getenv("TWOFOLD_PASS_BLOCKS")) {
             ctx->maxPassBlocks = (uint32_t)std::max(1, std::min(atoi(e), perCu * cus));
         }
         if (const char* e = getenv("TWOFOLD_AXES")) { ctx->madeAxes = strcmp(e, "made") == 0; }
+        if (const char* e = getenv("TWOFOLD_TAIL")) { ctx->passTail = strcmp(e, "off") != 0; }
     }
     if (!r) { r = allocBodies(ctx, maxBodies ? maxBodies : 1); }
     if (!r) { r = allocContacts(ctx, maxContacts ? maxContacts : 1); }
