@@ -1180,6 +1180,11 @@ __global__ void __launch_bounds__(kPassThreads) k_pass(Rows rows, const uint4* i
 // so each body can apply its own rows' pushes, in row order, by itself: one launch instead of one
 // per colour, with the same operations on each body in the same order.
 
+// A body's entry for one of its rows: the row and the body's side (entries sort by row). The row's
+// point count is read from its header: a refresh can change it while the lists stay.
+__device__ __forceinline__ uint32_t warmEntry(uint32_t r, bool sideB) { return (r << 1) | (sideB ? 1u : 0u); }
+__device__ __forceinline__ uint32_t entryRow(uint32_t e) { return e >> 1; }
+
 __global__ void k_bep_count(uint32_t n, const uint4* hdr, uint32_t* deg) {
     for (uint32_t r = gid(); r < n; r += gstride()) {
         const uint4 h = hdr[r];
@@ -1193,8 +1198,9 @@ __global__ void k_bep_fill(uint32_t n, const uint4* hdr, const uint32_t* start, 
     for (uint32_t r = gid(); r < n; r += gstride()) {
         const uint4 h = hdr[r];
         if ((h.z & kCountMask) == 0) { continue; }
-        if (h.z & kMoveA) { ep[start[h.x] + atomicAdd(&cursor[h.x], 1u)] = r << 1; }
-        if (h.z & kMoveB) { ep[start[h.y] + atomicAdd(&cursor[h.y], 1u)] = (r << 1) | 1u; }
+        const uint32_t tag = warmEntry(r, false);
+        if (h.z & kMoveA) { ep[start[h.x] + atomicAdd(&cursor[h.x], 1u)] = tag; }
+        if (h.z & kMoveB) { ep[start[h.y] + atomicAdd(&cursor[h.y], 1u)] = tag | 1u; }
     }
 }
 
@@ -1214,8 +1220,62 @@ __global__ void k_bep_sort(uint32_t nb, const uint32_t* deg, const uint32_t* sta
     }
 }
 
-// Each row's loads go out together (all its points, this body's side only), and the next row's
-// index is read while the current row is applied.
+// One row's warm-start data, this body's side only: impulses, tangents and two float4 per point and
+// axis.
+struct WarmSide {
+    int count;
+    float4 t1, t2;
+    float4 im[4];
+    float4 q[4][3][2];
+};
+
+__device__ __forceinline__ void warmLoad(const Rows& rows, uint32_t e, WarmSide& w) {
+    const uint32_t r = entryRow(e);
+    const int count = (int)(rows.hdr[r].z & kCountMask);
+    const uint32_t side = e & 1u;
+    w.count = count;
+    w.t1 = rows.tg1[r];
+    w.t2 = rows.tg2[r];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (i < count) {
+            w.im[i] = rows.imp[(uint32_t)i * rows.cap + r];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const float4* base = rows.ax + (size_t)((i * 3 + a) * 5 + side) * rows.cap + r;
+                w.q[i][a][0] = base[0];
+                w.q[i][a][1] = base[rows.cap];
+            }
+        }
+    }
+}
+
+// warmStartLanes' pushes of one row on this body (src/lanes.cpp:195-217), in point and axis order.
+__device__ __forceinline__ void warmApply(uint32_t e, const WarmSide& w, Motion& m) {
+    const int count = w.count;
+    const bool sideB = e & 1u;
+    const V3 dir1 = vec(w.t1.x, w.t1.y, w.t1.z), dir2 = vec(w.t2.x, w.t2.y, w.t2.z);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (i >= count) { continue; }
+        const V3 ti = vec(w.im[i].y, w.im[i].z, w.im[i].w);
+        const float lambda[3] = { w.im[i].x, dot(ti, dir1), dot(ti, dir2) };
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float4 q0 = w.q[i][a][0], q1 = w.q[i][a][1];
+            if (sideB) {
+                m.v = m.v + vec(q0.z, q0.w, q1.x) * lambda[a];
+                m.w = m.w + vec(q1.y, q1.z, q1.w) * lambda[a];
+            } else {
+                m.v = m.v - vec(q0.x, q0.y, q0.z) * lambda[a];
+                m.w = m.w - vec(q0.w, q1.x, q1.y) * lambda[a];
+            }
+        }
+    }
+}
+
+// Each body applies its rows in row order; a row's loads go out while the row before it is applied
+// (two rows in flight).
 __global__ void __launch_bounds__(kThreads) k_warm_bodies(uint32_t nb, const uint32_t* __restrict__ deg, const uint32_t* __restrict__ start,
                               const uint32_t* __restrict__ ep, Rows rows, float4* __restrict__ motion) {
     for (uint32_t x = gid(); x < nb; x += gstride()) {
@@ -1223,44 +1283,18 @@ __global__ void __launch_bounds__(kThreads) k_warm_bodies(uint32_t nb, const uin
         if (d == 0) { continue; }
         const uint32_t* list = ep + start[x];
         Motion m = loadMotion(motion, x);
-        uint32_t e = list[0];
-        for (uint32_t k = 0; k < d; ++k) {
-            const uint32_t r = e >> 1;
-            const bool sideB = e & 1u;
-            if (k + 1 < d) { e = list[k + 1]; }
-            const int count = (int)(rows.hdr[r].z & kCountMask);
-            const float4 t1 = rows.tg1[r], t2 = rows.tg2[r];
-            float4 im[4], q[4][3][2];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                if (i < count) {
-                    im[i] = rows.imp[(uint32_t)i * rows.cap + r];
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        const float4* base = rows.ax + (size_t)((i * 3 + a) * 5 + (sideB ? 1 : 0)) * rows.cap + r;
-                        q[i][a][0] = base[0];
-                        q[i][a][1] = base[rows.cap];
-                    }
-                }
-            }
-            const V3 dir1 = vec(t1.x, t1.y, t1.z), dir2 = vec(t2.x, t2.y, t2.z);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                if (i >= count) { continue; }
-                const V3 ti = vec(im[i].y, im[i].z, im[i].w);
-                const float lambda[3] = { im[i].x, dot(ti, dir1), dot(ti, dir2) };
-#pragma unroll
-                for (int a = 0; a < 3; ++a) {
-                    const float4 q0 = q[i][a][0], q1 = q[i][a][1];
-                    if (sideB) {
-                        m.v = m.v + vec(q0.z, q0.w, q1.x) * lambda[a];
-                        m.w = m.w + vec(q1.y, q1.z, q1.w) * lambda[a];
-                    } else {
-                        m.v = m.v - vec(q0.x, q0.y, q0.z) * lambda[a];
-                        m.w = m.w - vec(q0.w, q1.x, q1.y) * lambda[a];
-                    }
-                }
-            }
+        WarmSide a, b;
+        uint32_t ea = list[0], eb = d > 1 ? list[1] : 0u;
+        warmLoad(rows, ea, a);
+        for (uint32_t k = 0; k < d; k += 2) {
+            if (k + 1 < d) { warmLoad(rows, eb, b); }
+            const uint32_t ec = k + 2 < d ? list[k + 2] : 0u;
+            warmApply(ea, a, m);
+            if (k + 2 < d) { warmLoad(rows, ec, a); }
+            const uint32_t ed = k + 3 < d ? list[k + 3] : 0u;
+            if (k + 1 < d) { warmApply(eb, b, m); }
+            ea = ec;
+            eb = ed;
         }
         storeVelocity(motion, x, m);
     }
@@ -1396,7 +1430,7 @@ __global__ void k_adj_rebase(uint32_t nb, const uint32_t* deg, const uint32_t* s
         const uint32_t* list = ep + start[x];
         V3 sum = vec(0.0f, 0.0f, 0.0f);
         for (uint32_t k = 0; k < d; ++k) {
-            const uint32_t r = list[k] >> 1;
+            const uint32_t r = entryRow(list[k]);
             const int count = (int)(rows.hdr[r].z & kCountMask);
             float s = 0.0f;
             TF_POINTS(q, count) { s += g.sepBar[(uint32_t)q * rows.cap + r]; }
