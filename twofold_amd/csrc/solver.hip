@@ -1680,8 +1680,109 @@ __global__ void k_rest_place(const uint32_t* list, uint32_t n, const tfs_contact
         const uint32_t ci = list[k];
         const tfs_contact& c = contacts[ci];
         const bool rolling = rollingContact(c);
-        const uint32_t writes = (rolling || moving[c.body_a] ? 1u : 0u) | (rolling || moving[c.body_b] ? 2u : 0u);
+        const uint32_t writes = (rolling || moving[c.body_a] ? 1u : 0u) | (rolling || moving[c.body_b] ? 2u : 0u) |
+                                (moving[c.body_a] ? 0u : 4u) | (moving[c.body_b] ? 0u : 8u);
         entries[levelStart[lv] + atomicSub(&z.count[lv], 1u) - 1u] = make_uint4(ci, c.body_a, c.body_b, writes);
+    }
+}
+
+// The active rest contacts, packed in the level order of the active rest list (entry k at position
+// k) so that a level's loads are coalesced, and read for the impact rest list through `pos`
+// (contact -> position). h: [q * cap + k], q = normal + friction, tangent1 + restitution, tangent2 +
+// rolling resistance, plane + rolling radius, rolling impulse + (count | planar << 3); p: per point
+// m [(m * 3 + q) * cap + k], q = anchor a + adjusted separation, anchor b + relative velocity,
+// normal impulse + tangent impulse; peak: [m * cap + k]. The contacts themselves get the impulses
+// back before a refresh reads them and at the end of the step.
+struct RestPack {
+    float4* h;
+    float4* p;
+    float* peak;
+    uint32_t* pos;
+    uint32_t cap;
+};
+
+__global__ void k_rest_pack(const uint4* entries, uint32_t n, const tfs_contact* contacts, RestPack rp) {
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        const uint32_t ci = entries[k].x;
+        const tfs_contact& c = contacts[ci];
+        const size_t cap = rp.cap;
+        const uint32_t meta = (uint32_t)c.count | (c.planar ? 8u : 0u);
+        rp.h[k] = make_float4(c.normal[0], c.normal[1], c.normal[2], c.friction);
+        rp.h[cap + k] = make_float4(c.tangent1[0], c.tangent1[1], c.tangent1[2], c.restitution);
+        rp.h[2 * cap + k] = make_float4(c.tangent2[0], c.tangent2[1], c.tangent2[2], c.rolling_resistance);
+        rp.h[3 * cap + k] = make_float4(c.plane[0], c.plane[1], c.plane[2], c.rolling_radius);
+        rp.h[4 * cap + k] = make_float4(c.rolling_impulse[0], c.rolling_impulse[1], c.rolling_impulse[2], __uint_as_float(meta));
+        TF_POINTS(m, c.count) {
+            const tfs_point& q = c.points[m];
+            rp.p[(size_t)(m * 3) * cap + k] = make_float4(q.anchor_a[0], q.anchor_a[1], q.anchor_a[2], q.adjusted_separation);
+            rp.p[(size_t)(m * 3 + 1) * cap + k] = make_float4(q.anchor_b[0], q.anchor_b[1], q.anchor_b[2], q.relative_velocity);
+            rp.p[(size_t)(m * 3 + 2) * cap + k] =
+                make_float4(q.normal_impulse, q.tangent_impulse[0], q.tangent_impulse[1], q.tangent_impulse[2]);
+            rp.peak[(size_t)m * cap + k] = q.max_normal_impulse;
+        }
+        rp.pos[ci] = k;
+    }
+}
+
+// What the rest solves change goes back to the contacts: point impulses and the rolling impulse.
+__global__ void k_rest_unpack(const uint4* entries, uint32_t n, RestPack rp, tfs_contact* contacts) {
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        tfs_contact& c = contacts[entries[k].x];
+        const size_t cap = rp.cap;
+        const float4 ri = rp.h[4 * cap + k];
+        c.rolling_impulse[0] = ri.x;
+        c.rolling_impulse[1] = ri.y;
+        c.rolling_impulse[2] = ri.z;
+        TF_POINTS(m, c.count) {
+            const float4 im = rp.p[(size_t)(m * 3 + 2) * cap + k];
+            c.points[m].normal_impulse = im.x;
+            c.points[m].tangent_impulse[0] = im.y;
+            c.points[m].tangent_impulse[1] = im.z;
+            c.points[m].tangent_impulse[2] = im.w;
+            c.points[m].max_normal_impulse = rp.peak[(size_t)m * cap + k];
+        }
+    }
+}
+
+// A rest contact from the pack, in the fields the rest solves read (the others are left unset).
+__device__ __forceinline__ void loadRest(const RestPack& rp, uint32_t k, tfs_contact& c) {
+    const size_t cap = rp.cap;
+    const float4 a = rp.h[k], b = rp.h[cap + k], d = rp.h[2 * cap + k], e = rp.h[3 * cap + k], f = rp.h[4 * cap + k];
+    const uint32_t meta = __float_as_uint(f.w);
+    c.count = (int32_t)(meta & 7u);
+    c.planar = (meta & 8u) ? 1 : 0;
+    c.laned = 0;
+    c.normal[0] = a.x; c.normal[1] = a.y; c.normal[2] = a.z; c.friction = a.w;
+    c.tangent1[0] = b.x; c.tangent1[1] = b.y; c.tangent1[2] = b.z; c.restitution = b.w;
+    c.tangent2[0] = d.x; c.tangent2[1] = d.y; c.tangent2[2] = d.z; c.rolling_resistance = d.w;
+    c.plane[0] = e.x; c.plane[1] = e.y; c.plane[2] = e.z; c.rolling_radius = e.w;
+    c.rolling_impulse[0] = f.x; c.rolling_impulse[1] = f.y; c.rolling_impulse[2] = f.z;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        if (m < c.count) {
+            const float4 pa = rp.p[(size_t)(m * 3) * cap + k], pb = rp.p[(size_t)(m * 3 + 1) * cap + k];
+            const float4 im = rp.p[(size_t)(m * 3 + 2) * cap + k];
+            tfs_point& q = c.points[m];
+            q.anchor_a[0] = pa.x; q.anchor_a[1] = pa.y; q.anchor_a[2] = pa.z; q.adjusted_separation = pa.w;
+            q.anchor_b[0] = pb.x; q.anchor_b[1] = pb.y; q.anchor_b[2] = pb.z; q.relative_velocity = pb.w;
+            q.normal_impulse = im.x; q.tangent_impulse[0] = im.y; q.tangent_impulse[1] = im.z; q.tangent_impulse[2] = im.w;
+            q.max_normal_impulse = rp.peak[(size_t)m * cap + k];
+        }
+    }
+}
+
+__device__ __forceinline__ void storeRest(const tfs_contact& c, const RestPack& rp, uint32_t k) {
+    const size_t cap = rp.cap;
+    const float4 f = rp.h[4 * cap + k];
+    rp.h[4 * cap + k] = make_float4(c.rolling_impulse[0], c.rolling_impulse[1], c.rolling_impulse[2], f.w);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        if (m < c.count) {
+            const tfs_point& q = c.points[m];
+            rp.p[(size_t)(m * 3 + 2) * cap + k] =
+                make_float4(q.normal_impulse, q.tangent_impulse[0], q.tangent_impulse[1], q.tangent_impulse[2]);
+            rp.peak[(size_t)m * cap + k] = q.max_normal_impulse;
+        }
     }
 }
 
@@ -1694,9 +1795,8 @@ constexpr int kRestBlockThreads = 512;
 
 template <int MODE, bool ONE_BLOCK>
 __global__ void __launch_bounds__(ONE_BLOCK ? kRestBlockThreads : kPassThreads)
-    k_rest_levels(const uint4* entries, const uint32_t* levelStart, uint32_t levels, tfs_contact* contacts,
-                  const tfs_body* bodies, float4* motion, const float4* mass, Ctx ctx, BarrierLines* bar,
-                  BarrierBase base, DevCounts* cnt) {
+    k_rest_levels(const uint4* entries, const uint32_t* levelStart, uint32_t levels, RestPack rp, bool viaPos,
+                  float4* motion, const float4* mass, Ctx ctx, BarrierLines* bar, BarrierBase base, DevCounts* cnt) {
     using X = Access<!ONE_BLOCK>;
     BarrierPhase phase(base);
     const uint32_t g = ONE_BLOCK ? threadIdx.x : passSlot();
@@ -1713,11 +1813,13 @@ __global__ void __launch_bounds__(ONE_BLOCK ? kRestBlockThreads : kPassThreads)
         }
         for (uint32_t i = first; i < end; i += stride) {
             const uint4 e = i == first ? firstEntry : entries[i];
-            tfs_contact& c = contacts[e.x];
+            const uint32_t k = viaPos ? rp.pos[e.x] : i;
+            tfs_contact c;
+            loadRest(rp, k, c);
             const uint32_t ia = e.y, ib = e.z;
-            const tfs_body& ba = bodies[ia];
-            const tfs_body& bb = bodies[ib];
-            const BodyView va = viewOf(ba), vb = viewOf(bb);
+            const V3 none = vec(0.0f, 0.0f, 0.0f);
+            const BodyView va{ (uint8_t)((e.w & 4u) ? TFS_STATIC : TFS_DYNAMIC), 1, 0, none };
+            const BodyView vb{ (uint8_t)((e.w & 8u) ? TFS_STATIC : TFS_DYNAMIC), 1, 0, none };
             const float4 v0 = X::ld(&motion[4 * ia]), w0 = X::ld(&motion[4 * ia + 1]);
             const float4 v1 = X::ld(&motion[4 * ib]), w1 = X::ld(&motion[4 * ib + 1]);
             const float4 p0 = motion[4 * ia + 2], q0 = motion[4 * ia + 3], p1 = motion[4 * ib + 2], q1 = motion[4 * ib + 3];
@@ -1728,8 +1830,10 @@ __global__ void __launch_bounds__(ONE_BLOCK ? kRestBlockThreads : kPassThreads)
                 warmRest(c, va, vb, ma, mb, a, b);
             } else if (MODE == kRestitute) {
                 restituteRest(c, va, vb, ma, mb, a, b, ctx);
+                storeRest(c, rp, k);
             } else {
-                solveRest(c, va, vb, ma, mb, a, b, ctx, MODE == kBias);
+                solveRest<MODE == kBias>(c, va, vb, ma, mb, a, b, ctx);
+                storeRest(c, rp, k);
             }
             if (e.w & 1u) {
                 X::st(&motion[4 * ia], make_float4(a.v.x, a.v.y, a.v.z, 0.0f));
@@ -2154,6 +2258,10 @@ struct tfs_ctx {
     uint4* restEntries[2] = {};
     uint32_t* restLevelStart[2] = {};
     uint32_t restLevels[2] = {};
+    float4* restH = nullptr;        // the active rest contacts packed in level order (RestPack)
+    float4* restP = nullptr;
+    float* restPeak = nullptr;
+    uint32_t* restPos = nullptr;
     uint8_t* colors = nullptr;
     uint32_t* ep = nullptr;
     uint32_t* rank = nullptr;
@@ -2278,7 +2386,9 @@ int allocContacts(tfs_ctx* ctx, uint32_t cap) {
         (r = grow(ctx, ctx->restLevel, cap)) || (r = grow(ctx, ctx->restCount, cap)) ||
         (r = grow(ctx, ctx->restEntries[0], cap)) || (r = grow(ctx, ctx->restEntries[1], cap)) ||
         (r = grow(ctx, ctx->restLevelStart[0], (size_t)cap + 1)) ||
-        (r = grow(ctx, ctx->restLevelStart[1], (size_t)cap + 1))) {
+        (r = grow(ctx, ctx->restLevelStart[1], (size_t)cap + 1)) || (r = grow(ctx, ctx->restH, 5 * (size_t)cap)) ||
+        (r = grow(ctx, ctx->restP, 12 * (size_t)cap)) || (r = grow(ctx, ctx->restPeak, 4 * (size_t)cap)) ||
+        (r = grow(ctx, ctx->restPos, cap))) {
         return r;
     }
     if ((r = allocRows(ctx, ctx->rows, cap)) || (r = grow(ctx, ctx->impactHdr, cap)) || (r = grow(ctx, ctx->bep, 2 * (size_t)cap))) {
@@ -2563,6 +2673,8 @@ int buildRestLevels(tfs_ctx* ctx, int which, const uint32_t* list, uint32_t n) {
 
 // A rest pass: the list's levels, in one work-group when they are narrow (on average under a block of
 // threads per level), else in one cooperative launch.
+RestPack restPack(const tfs_ctx* ctx) { return RestPack{ ctx->restH, ctx->restP, ctx->restPeak, ctx->restPos, ctx->capC }; }
+
 int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32_t end, const Ctx& c) {
     if (end <= begin) { return 0; }
     const int which = list == ctx->impactRowsList ? 1 : 0;
@@ -2576,16 +2688,18 @@ int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32
     BarrierLines* bar = ctx->bar;
     const BarrierBase base = ctx->barrierBase;
     const float4* mass = ctx->mass;
+    const RestPack rp = restPack(ctx);
+    const bool viaPos = which == 1;
     if (oneBlock) {
         switch (mode) {
-            case kWarm: LAUNCH((k_rest_levels<kWarm, true>), 1, kRestBlockThreads, entries, starts, levels, ctx->contacts, ctx->bodies, ctx->motion, mass, c, bar, base, ctx->cnt); break;
-            case kBias: LAUNCH((k_rest_levels<kBias, true>), 1, kRestBlockThreads, entries, starts, levels, ctx->contacts, ctx->bodies, ctx->motion, mass, c, bar, base, ctx->cnt); break;
-            case kRelax: LAUNCH((k_rest_levels<kRelax, true>), 1, kRestBlockThreads, entries, starts, levels, ctx->contacts, ctx->bodies, ctx->motion, mass, c, bar, base, ctx->cnt); break;
-            default: LAUNCH((k_rest_levels<kRestitute, true>), 1, kRestBlockThreads, entries, starts, levels, ctx->contacts, ctx->bodies, ctx->motion, mass, c, bar, base, ctx->cnt); break;
+            case kWarm: LAUNCH((k_rest_levels<kWarm, true>), 1, kRestBlockThreads, entries, starts, levels, rp, viaPos, ctx->motion, mass, c, bar, base, ctx->cnt); break;
+            case kBias: LAUNCH((k_rest_levels<kBias, true>), 1, kRestBlockThreads, entries, starts, levels, rp, viaPos, ctx->motion, mass, c, bar, base, ctx->cnt); break;
+            case kRelax: LAUNCH((k_rest_levels<kRelax, true>), 1, kRestBlockThreads, entries, starts, levels, rp, viaPos, ctx->motion, mass, c, bar, base, ctx->cnt); break;
+            default: LAUNCH((k_rest_levels<kRestitute, true>), 1, kRestBlockThreads, entries, starts, levels, rp, viaPos, ctx->motion, mass, c, bar, base, ctx->cnt); break;
         }
         return 0;
     }
-    void* args[] = { (void*)&entries, (void*)&starts, (void*)&levels, &ctx->contacts, &ctx->bodies, &ctx->motion,
+    void* args[] = { (void*)&entries, (void*)&starts, (void*)&levels, (void*)&rp, (void*)&viaPos, &ctx->motion,
                      (void*)&mass, (void*)&c, (void*)&bar, (void*)&base, &ctx->cnt };
     const void* fn = mode == kWarm ? (const void*)k_rest_levels<kWarm, false>
                      : mode == kBias ? (const void*)k_rest_levels<kBias, false>
@@ -2626,6 +2740,10 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
         refreshing = ctx->refreshing;
         bodyWorld = ctx->bodyWorld;
     }
+    const uint32_t nRest = hc.nActive - laned;
+    if (nRest > 0) {
+        LAUNCH(k_rest_unpack, blocksFor(nRest), kThreads, ctx->restEntries[0], nRest, restPack(ctx), ctx->contacts);
+    }
     LAUNCH(k_refresh_pre, blocksFor(hc.nActive), kThreads, ctx->active, ctx->cnt, againFlag, ctx->contacts, ctx->rows,
            ctx->rowOf, ctx->colors, ctx->motion, refreshing, bodyWorld);
     LAUNCH(k_refresh_bodies, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, ctx->mass, refreshing, bodyWorld);
@@ -2664,6 +2782,7 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
            ctx->contacts, ctx->bodies, ctx->motion, ctx->mass, matchSq);
     // A refreshed contact keeps its row; its points and axes are packed again, as prepareLanes does.
     if (int e = packRows(ctx, laned, impactLaned, false)) { return e; }
+    if (nRest > 0) { LAUNCH(k_rest_pack, blocksFor(nRest), kThreads, ctx->restEntries[0], nRest, ctx->contacts, restPack(ctx)); }
     return 0;
 }
 
@@ -2746,6 +2865,10 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
         if (ctx->hostCnt->error) { return fail(ctx, TFS_E_KERNEL, "a kernel's bounded wait ran out (error %u)", ctx->hostCnt->error); }
         ctx->restLevels[0] = ctx->hostCnt->restLevels[0];
         ctx->restLevels[1] = ctx->hostCnt->restLevels[1];
+        if (hc.nActive > laned) {
+            LAUNCH(k_rest_pack, blocksFor(hc.nActive - laned), kThreads, ctx->restEntries[0], hc.nActive - laned,
+                   ctx->contacts, restPack(ctx));
+        }
     }
 
     // substeps: src/world.cpp:1658-1681
@@ -2774,6 +2897,10 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     }
     LAUNCH(k_limit_speeds, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c);
     LAUNCH(k_unpack, blocksFor(laned), kThreads, laned, ctx->rows, ctx->contacts);
+    if (hc.nActive > laned) {
+        LAUNCH(k_rest_unpack, blocksFor(hc.nActive - laned), kThreads, ctx->restEntries[0], hc.nActive - laned,
+               restPack(ctx), ctx->contacts);
+    }
     LAUNCH(k_writeback, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion);
     HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
 
@@ -3045,7 +3172,7 @@ void tfs_destroy(tfs_ctx* ctx) {
                      ctx->edges, ctx->bodyWorld, ctx->refreshing, ctx->worldStart, ctx->restTracked,
                      ctx->restDeg, ctx->restStart, ctx->restCursor, ctx->restEp, ctx->restRank, ctx->restLevel,
                      ctx->restCount, ctx->restWord, ctx->restEntries[0], ctx->restEntries[1], ctx->restLevelStart[0],
-                     ctx->restLevelStart[1], ctx->ckBody, ctx->ckRow, ctx->tapeBuf, ctx->adj.bar, ctx->adj.dpBar,
+                     ctx->restLevelStart[1], ctx->restH, ctx->restP, ctx->restPeak, ctx->restPos, ctx->ckBody, ctx->ckRow, ctx->tapeBuf, ctx->adj.bar, ctx->adj.dpBar,
                      ctx->adj.comBar, ctx->adj.impBar, ctx->adj.sepBar, ctx->adj.muBar, ctx->dlcom, ctx->muContact };
     for (void* p : ptrs) {
         if (p) { (void)hipFree(p); }

@@ -673,8 +673,9 @@ TF_HD void warmRest(tfs_contact& c, const BodyView& va, const BodyView& vb, cons
 }
 
 // solvePlanar: src/solver.cpp:137-209.
+template <bool useBias>
 TF_HD void solvePlanar(tfs_contact& c, const BodyView& va, const BodyView& vb, const Mass& ma, const Mass& mb,
-                       Motion& a, Motion& b, const Ctx& ctx, bool useBias) {
+                       Motion& a, Motion& b, const Ctx& ctx) {
     const bool moveA = !anchoredView(va), moveB = !anchoredView(vb);
     const Soft soft = (!moveA || !moveB) ? ctx.staticSoft : ctx.contactSoft;
     const V3 normal = ld3(c.normal), tangent1 = ld3(c.tangent1), plane = ld3(c.plane);
@@ -709,8 +710,11 @@ TF_HD void solvePlanar(tfs_contact& c, const BodyView& va, const BodyView& vb, c
             changed = true;
         }
     }
-    for (int j = 0; j < c.count; ++j) {
-        const int i = useBias ? j : c.count - 1 - j;
+    // relax: from the last point down; the unrolled slots keep every index known at compile time
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int i = useBias ? j : 3 - j;
+        if (i >= c.count) { continue; }
         tfs_point& p = c.points[i];
         const PlanarAxes x = planarAxes(ma, mb, c, i);
         const float old = dot(ld3(p.tangent_impulse), tangent1);
@@ -802,10 +806,11 @@ TF_HD Row rowOf(const tfs_contact& c, bool moveA, bool moveB) {
 }
 
 // solveContacts, one contact of the rest: src/solver.cpp:360-421.
+template <bool useBias>
 TF_HD void solveRest(tfs_contact& c, const BodyView& va, const BodyView& vb, const Mass& ma, const Mass& mb, Motion& a,
-                     Motion& b, const Ctx& ctx, bool useBias) {
+                     Motion& b, const Ctx& ctx) {
     if (c.planar) {
-        solvePlanar(c, va, vb, ma, mb, a, b, ctx, useBias);
+        solvePlanar<useBias>(c, va, vb, ma, mb, a, b, ctx);
         return;
     }
     RowPoint p[4];
