@@ -75,6 +75,14 @@ __device__ __forceinline__ void storeMotion(float4* m, uint32_t i, const Motion&
 }
 // 24 floats: world inverse mass (9), world inverse inertia (9), inverse mass, plane inverse mass,
 // plane inverse inertia, 3 spare.
+__device__ __forceinline__ Mass loadPlanarMass(const float4* m, uint32_t i) {
+    const float4 e = m[6 * i + 4], f = m[6 * i + 5];
+    Mass x{};
+    x.invMass = e.z;
+    x.pim = e.w;
+    x.pii = f.x;
+    return x;
+}
 __device__ __forceinline__ Mass loadMass(const float4* m, uint32_t i) {
     const float4 a = m[6 * i], b = m[6 * i + 1], c = m[6 * i + 2], d = m[6 * i + 3], e = m[6 * i + 4], f = m[6 * i + 5];
     Mass x;
@@ -1786,6 +1794,49 @@ __device__ __forceinline__ void storeRest(const tfs_contact& c, const RestPack& 
     }
 }
 
+// The warm start of the rest contacts, one body at a time: a warm-start push does not depend on the
+// velocities, so each body applies its rest contacts' effects (warmRestSide) in list order by
+// itself, after its coloured rows (warmStart after warmStartBatches, src/world.cpp:1672-1673). A
+// body's rest contacts in list order are its entries in level order (contacts sharing a body that
+// they write follow each other's levels). Entries: pack position << 1 | side, for the sides each
+// contact writes.
+__global__ void k_wrest_count(const uint4* entries, uint32_t n, uint32_t* deg) {
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        const uint4 e = entries[k];
+        if (e.w & 1u) { atomicAdd(&deg[e.y], 1u); }
+        if (e.w & 2u) { atomicAdd(&deg[e.z], 1u); }
+    }
+}
+
+__global__ void k_wrest_fill(const uint4* entries, uint32_t n, const uint32_t* start, uint32_t* cursor, uint32_t* ep) {
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        const uint4 e = entries[k];
+        if (e.w & 1u) { ep[start[e.y] + atomicAdd(&cursor[e.y], 1u)] = k << 1; }
+        if (e.w & 2u) { ep[start[e.z] + atomicAdd(&cursor[e.z], 1u)] = (k << 1) | 1u; }
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_warm_rest_bodies(uint32_t nb, const uint32_t* deg, const uint32_t* start,
+                                                               const uint32_t* ep, const uint4* entries, RestPack rp,
+                                                               float4* motion, const float4* mass) {
+    for (uint32_t x = gid(); x < nb; x += gstride()) {
+        const uint32_t d = deg[x];
+        if (d == 0) { continue; }
+        const uint32_t* list = ep + start[x];
+        Motion m = loadMotion(motion, x);
+        const Mass mx = loadMass(mass, x);
+        for (uint32_t j = 0; j < d; ++j) {
+            const uint32_t en = list[j], k = en >> 1;
+            const bool sideB = en & 1u;
+            const uint32_t flags = entries[k].w;
+            tfs_contact c;
+            loadRest(rp, k, c);
+            warmRestSide(c, sideB, !(flags & (sideB ? 8u : 4u)), mx, m);
+        }
+        storeVelocity(motion, x, m);
+    }
+}
+
 // The rest list level after level. ONE_BLOCK: one work-group walks the levels with __syncthreads
 // between them (narrow levels: the block barrier is cheap and one CU's accesses need no coherence
 // protocol); otherwise one cooperative launch over many blocks, velocities crossing blocks through
@@ -1825,7 +1876,9 @@ __global__ void __launch_bounds__(ONE_BLOCK ? kRestBlockThreads : kPassThreads)
             const float4 p0 = motion[4 * ia + 2], q0 = motion[4 * ia + 3], p1 = motion[4 * ib + 2], q1 = motion[4 * ib + 3];
             Motion a{ vec(v0.x, v0.y, v0.z), vec(w0.x, w0.y, w0.z), vec(p0.x, p0.y, p0.z), quat(q0.x, q0.y, q0.z, q0.w) };
             Motion b{ vec(v1.x, v1.y, v1.z), vec(w1.x, w1.y, w1.z), vec(p1.x, p1.y, p1.z), quat(q1.x, q1.y, q1.z, q1.w) };
-            const Mass ma = loadMass(mass, ia), mb = loadMass(mass, ib);
+            // a planar contact reads only the inverse mass and the planar inverse mass and inertia
+            const Mass ma = c.planar ? loadPlanarMass(mass, ia) : loadMass(mass, ia);
+            const Mass mb = c.planar ? loadPlanarMass(mass, ib) : loadMass(mass, ib);
             if (MODE == kWarm) {
                 warmRest(c, va, vb, ma, mb, a, b);
             } else if (MODE == kRestitute) {
@@ -2258,6 +2311,7 @@ struct tfs_ctx {
     uint4* restEntries[2] = {};
     uint32_t* restLevelStart[2] = {};
     uint32_t restLevels[2] = {};
+    uint32_t *wrestDeg = nullptr, *wrestStart = nullptr, *wrestCursor = nullptr, *wrestEp = nullptr;  // rest warm start lists
     float4* restH = nullptr;        // the active rest contacts packed in level order (RestPack)
     float4* restP = nullptr;
     float* restPeak = nullptr;
@@ -2353,7 +2407,8 @@ int allocBodies(tfs_ctx* ctx, uint32_t cap) {
         (r = grow(ctx, ctx->bcursor, cap)) || (r = grow(ctx, ctx->bodyWorld, cap)) ||
         (r = grow(ctx, ctx->restTracked, cap)) || (r = grow(ctx, ctx->restDeg, cap)) ||
         (r = grow(ctx, ctx->restStart, cap)) || (r = grow(ctx, ctx->restCursor, cap)) ||
-        (r = grow(ctx, ctx->restWord, cap))) {
+        (r = grow(ctx, ctx->restWord, cap)) || (r = grow(ctx, ctx->wrestDeg, cap)) ||
+        (r = grow(ctx, ctx->wrestStart, cap)) || (r = grow(ctx, ctx->wrestCursor, cap))) {
         return r;
     }
     ctx->capB = cap;
@@ -2388,7 +2443,7 @@ int allocContacts(tfs_ctx* ctx, uint32_t cap) {
         (r = grow(ctx, ctx->restLevelStart[0], (size_t)cap + 1)) ||
         (r = grow(ctx, ctx->restLevelStart[1], (size_t)cap + 1)) || (r = grow(ctx, ctx->restH, 5 * (size_t)cap)) ||
         (r = grow(ctx, ctx->restP, 12 * (size_t)cap)) || (r = grow(ctx, ctx->restPeak, 4 * (size_t)cap)) ||
-        (r = grow(ctx, ctx->restPos, cap))) {
+        (r = grow(ctx, ctx->restPos, cap)) || (r = grow(ctx, ctx->wrestEp, 2 * (size_t)cap))) {
         return r;
     }
     if ((r = allocRows(ctx, ctx->rows, cap)) || (r = grow(ctx, ctx->impactHdr, cap)) || (r = grow(ctx, ctx->bep, 2 * (size_t)cap))) {
@@ -2866,8 +2921,15 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
         ctx->restLevels[0] = ctx->hostCnt->restLevels[0];
         ctx->restLevels[1] = ctx->hostCnt->restLevels[1];
         if (hc.nActive > laned) {
-            LAUNCH(k_rest_pack, blocksFor(hc.nActive - laned), kThreads, ctx->restEntries[0], hc.nActive - laned,
-                   ctx->contacts, restPack(ctx));
+            const uint32_t nRest = hc.nActive - laned;
+            LAUNCH(k_rest_pack, blocksFor(nRest), kThreads, ctx->restEntries[0], nRest, ctx->contacts, restPack(ctx));
+            LAUNCH(k_fill_u32, blocksFor(nb), kThreads, ctx->wrestDeg, nb, 0u);
+            LAUNCH(k_fill_u32, blocksFor(nb), kThreads, ctx->wrestCursor, nb, 0u);
+            LAUNCH(k_wrest_count, blocksFor(nRest), kThreads, ctx->restEntries[0], nRest, ctx->wrestDeg);
+            if ((r = scan(ctx, ctx->wrestDeg, nullptr, nb, ctx->wrestStart, nullptr, &ctx->cnt->scanTotal))) { return r; }
+            LAUNCH(k_wrest_fill, blocksFor(nRest), kThreads, ctx->restEntries[0], nRest, ctx->wrestStart, ctx->wrestCursor,
+                   ctx->wrestEp);
+            LAUNCH(k_bep_sort, blocksFor(nb), kThreads, nb, ctx->wrestDeg, ctx->wrestStart, ctx->wrestEp);
         }
     }
 
@@ -2881,11 +2943,12 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
             }
         }
         LAUNCH(k_integrate_velocities, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c);
-        if ((r = warmRows(ctx, laned, hc.pointsActive)) ||
-            (r = runRest(ctx, kWarm, ctx->rowsList, laned, hc.nActive, c)) ||
-            (r = solvePass(ctx, kBias, hc, laned, impactLaned, c))) {
-            return r;
+        if ((r = warmRows(ctx, laned, hc.pointsActive))) { return r; }
+        if (hc.nActive > laned) {
+            LAUNCH(k_warm_rest_bodies, blocksFor(nb), kThreads, nb, ctx->wrestDeg, ctx->wrestStart, ctx->wrestEp,
+                   ctx->restEntries[0], restPack(ctx), ctx->motion, ctx->mass);
         }
+        if ((r = solvePass(ctx, kBias, hc, laned, impactLaned, c))) { return r; }
         LAUNCH(k_integrate_positions, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c);
         if ((r = solvePass(ctx, kRelax, hc, laned, impactLaned, c))) { return r; }
     }
@@ -3172,7 +3235,8 @@ void tfs_destroy(tfs_ctx* ctx) {
                      ctx->edges, ctx->bodyWorld, ctx->refreshing, ctx->worldStart, ctx->restTracked,
                      ctx->restDeg, ctx->restStart, ctx->restCursor, ctx->restEp, ctx->restRank, ctx->restLevel,
                      ctx->restCount, ctx->restWord, ctx->restEntries[0], ctx->restEntries[1], ctx->restLevelStart[0],
-                     ctx->restLevelStart[1], ctx->restH, ctx->restP, ctx->restPeak, ctx->restPos, ctx->ckBody, ctx->ckRow, ctx->tapeBuf, ctx->adj.bar, ctx->adj.dpBar,
+                     ctx->restLevelStart[1], ctx->restH, ctx->restP, ctx->restPeak, ctx->restPos, ctx->wrestDeg,
+                     ctx->wrestStart, ctx->wrestCursor, ctx->wrestEp, ctx->ckBody, ctx->ckRow, ctx->tapeBuf, ctx->adj.bar, ctx->adj.dpBar,
                      ctx->adj.comBar, ctx->adj.impBar, ctx->adj.sepBar, ctx->adj.muBar, ctx->dlcom, ctx->muContact };
     for (void* p : ptrs) {
         if (p) { (void)hipFree(p); }

@@ -672,6 +672,79 @@ TF_HD void warmRest(tfs_contact& c, const BodyView& va, const BodyView& vb, cons
     }
 }
 
+// warmRest's effect on one of the contact's bodies (sideB: body B), which the caller applies to the
+// bodies the level-by-level pass would write back. Each body's operations are warmRest's on that
+// body, in the same order: a planar contact changes a body only when something is held and the body
+// moves (PlanarState::push, ::store), a 3D contact pushes both bodies (push with both flags set) and
+// turns them by the rolling impulse.
+TF_HD void warmRestSide(const tfs_contact& c, bool sideB, bool moves, const Mass& m, Motion& x) {
+    const V3 n = ld3(c.normal), t1 = ld3(c.tangent1), t2 = ld3(c.tangent2);
+    if (c.planar) {
+        bool held = c.rolling_resistance > 0.0f;
+        TF_POINTS(i, c.count) {
+            const V3 ti = ld3(c.points[i].tangent_impulse);
+            held = held || c.points[i].normal_impulse != 0.0f || dot(ti, ti) != 0.0f;
+        }
+        if (!held || !moves) { return; }
+        const V3 plane = ld3(c.plane);
+        V3 v = x.v;
+        float w = dot(x.w, plane);
+        TF_POINTS(i, c.count) {
+            const tfs_point& p = c.points[i];
+            const V3 r = sideB ? ld3(p.anchor_b) : ld3(p.anchor_a);
+            const float armN = dot(plane, cross(r, n)), armT = dot(plane, cross(r, t1));
+            const float in = p.normal_impulse, it = dot(ld3(p.tangent_impulse), t1);
+            if (sideB) {
+                v = v + n * (m.pim * in);
+                w += m.pii * armN * in;
+                v = v + t1 * (m.pim * it);
+                w += m.pii * armT * it;
+            } else {
+                v = v - n * (m.pim * in);
+                w -= m.pii * armN * in;
+                v = v - t1 * (m.pim * it);
+                w -= m.pii * armT * it;
+            }
+        }
+        if (c.rolling_resistance > 0.0f) {
+            const float r = dot(ld3(c.rolling_impulse), plane);
+            if (sideB) {
+                w += m.pii * r;
+            } else {
+                w -= m.pii * r;
+            }
+        }
+        x.v = v;
+        x.w = plane * w;
+        return;
+    }
+    TF_POINTS(i, c.count) {
+        const tfs_point& p = c.points[i];
+        const V3 r = sideB ? ld3(p.anchor_b) : ld3(p.anchor_a);
+        const V3 dirs[3] = { n, t1, t2 };
+        const float imp[3] = { p.normal_impulse, dot(ld3(p.tangent_impulse), t1), dot(ld3(p.tangent_impulse), t2) };
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const V3 lin = m.imw * dirs[d];
+            const V3 ang = m.iiw * cross(r, dirs[d]);
+            if (sideB) {
+                x.v = x.v + lin * imp[d];
+                x.w = x.w + ang * imp[d];
+            } else {
+                x.v = x.v - lin * imp[d];
+                x.w = x.w - ang * imp[d];
+            }
+        }
+    }
+    if (c.rolling_resistance > 0.0f) {
+        if (sideB) {
+            x.w = x.w + m.iiw * ld3(c.rolling_impulse);
+        } else {
+            x.w = x.w - m.iiw * ld3(c.rolling_impulse);
+        }
+    }
+}
+
 // solvePlanar: src/solver.cpp:137-209.
 template <bool useBias>
 TF_HD void solvePlanar(tfs_contact& c, const BodyView& va, const BodyView& vb, const Mass& ma, const Mass& mb,
