@@ -277,6 +277,12 @@ int tfs_adjoint_seed(tfs_ctx* ctx, const float* dl_dcom);
 int tfs_adjoint_step(tfs_ctx* ctx);
 int tfs_adjoint_read(tfs_ctx* ctx, float* dl_dv, float* dl_dw, float* dl_dfriction);
 
+/* No counterpart in the reference: a check of the device's one-instruction max and min, which the
+ * solves use where the reference calls maxf / minf (src/internal.h:28-45). For n pairs (a[i], b[i])
+ * in host memory it writes out[2i] = max, out[2i + 1] = min as computed on `device`. They must equal
+ * maxf(a, b) and minf(a, b) bit for bit unless a or b is a signalling NaN. */
+int tfs_check_minmax(int device, const float* a, const float* b, uint32_t n, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,7 +4,8 @@
 // (include/twofold/math.h, src/internal.h:28-45), in the same order, so that with
 // -ffp-contract=off (no fused multiply-add) and IEEE division and square root the kernels give
 // the reference's bits. Nothing here may be rewritten into a cheaper but different sequence
-// (reciprocal-multiply for division, rsqrt, min/max instructions for maxf/minf, fma).
+// (reciprocal-multiply for division, rsqrt, fma); maxq/minq below are the min/max instructions
+// where they give maxf/minf's bits.
 #pragma once
 
 #include <stdint.h>
@@ -48,6 +49,30 @@ TF_HD float minf(float a, float b) {
         return a == 0.0f ? -(-a + -b) : a;
     }
     return a < b ? a : b;
+}
+
+// maxf / minf as one instruction, for operands that are never signalling NaNs (results of float
+// arithmetic, and values only ever written from such results). The gfx950 v_max_f32 / v_min_f32
+// order -0 below +0 and return the other operand when one is NaN, as maxf / minf do; when both are
+// NaN they return their first operand where maxf / minf return their second, hence the swapped
+// operands. A signalling NaN operand would come back quieted, which maxf / minf do not do.
+TF_HD float maxq(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    float r;
+    asm("v_max_f32 %0, %1, %2" : "=v"(r) : "v"(b), "v"(a));
+    return r;
+#else
+    return maxf(a, b);
+#endif
+}
+TF_HD float minq(float a, float b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    float r;
+    asm("v_min_f32 %0, %1, %2" : "=v"(r) : "v"(b), "v"(a));
+    return r;
+#else
+    return minf(a, b);
+#endif
 }
 
 TF_HD float fsqrt(float x) { return sqrtf(x); }

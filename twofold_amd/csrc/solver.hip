@@ -50,6 +50,7 @@ struct DevCounts {
     uint32_t ticket;
     uint32_t restTicket[2];   // level dataflow of the active and impact rest lists
     uint32_t restLevels[2];
+    uint32_t restNarrow[2];   // ~(first level no wider than a rest work-group), 0 when every level is wider
     uint32_t scanTicket[kScanSlots];
     uint32_t changed[kIslandRounds];
     uint32_t colorStart[TFS_COLORS + 1];
@@ -1638,6 +1639,15 @@ __global__ void k_rest_ep_fill(const uint32_t* list, uint32_t n, const tfs_conta
     }
 }
 
+// The first level of a rest list that fits one rest work-group (levels before it are wide: they run
+// over many blocks). Written as ~level with atomicMax, so the counters' zero start means "none".
+__global__ void k_rest_narrow(const uint32_t* levelStart, const uint32_t* levels, uint32_t width, uint32_t* first) {
+    const uint32_t n = *levels;
+    for (uint32_t lv = gid(); lv < n; lv += gstride()) {
+        if (levelStart[lv + 1] - levelStart[lv] <= width) { atomicMax(first, ~lv); }
+    }
+}
+
 // The levels, as the colouring's dataflow: a contact proceeds once its tracked bodies have seen all
 // their earlier contacts; each body's word is (contacts seen << 32) | its highest level so far.
 __global__ void k_rest_level(const uint32_t* list, uint32_t n, const tfs_contact* contacts, RestScratch z,
@@ -1699,19 +1709,29 @@ __global__ void k_rest_place(const uint32_t* list, uint32_t n, const tfs_contact
 // (contact -> position). h: [q * cap + k], q = normal + friction, tangent1 + restitution, tangent2 +
 // rolling resistance, plane + rolling radius, rolling impulse + (count | planar << 3); p: per point
 // m [(m * 3 + q) * cap + k], q = anchor a + adjusted separation, anchor b + relative velocity,
-// normal impulse + tangent impulse; peak: [m * cap + k]. The contacts themselves get the impulses
-// back before a refresh reads them and at the end of the step.
+// normal impulse + tangent impulse; peak: [m * cap + k]; ax: a planar contact's axes per point m
+// (makePlanarAxes, computed here once as the reference computes them at prepare and rebase),
+// [2m * cap + k] = normal arms A, B, normal mass, tangent arm A; [(2m + 1) * cap + k] = tangent arm B,
+// tangent mass. Every slot of the four points exists for every contact, so the solves load them
+// all at once, without waiting for the count. The contacts themselves get the impulses back before a
+// refresh reads them and at the end of the step; the pack is built again after a refresh.
 struct RestPack {
     float4* h;
     float4* p;
     float* peak;
+    float4* ax;
     uint32_t* pos;
     uint32_t cap;
 };
 
-__global__ void k_rest_pack(const uint4* entries, uint32_t n, const tfs_contact* contacts, RestPack rp) {
+// The impact rest entries carry their contact's pack position in entry.w >> 4 (k_rest_impact_pos).
+constexpr uint32_t kRestPosShift = 4;
+
+__global__ void k_rest_pack(const uint4* entries, uint32_t n, const tfs_contact* contacts, const float4* mass,
+                            RestPack rp) {
     for (uint32_t k = gid(); k < n; k += gstride()) {
-        const uint32_t ci = entries[k].x;
+        const uint4 e = entries[k];
+        const uint32_t ci = e.x;
         const tfs_contact& c = contacts[ci];
         const size_t cap = rp.cap;
         const uint32_t meta = (uint32_t)c.count | (c.planar ? 8u : 0u);
@@ -1728,7 +1748,23 @@ __global__ void k_rest_pack(const uint4* entries, uint32_t n, const tfs_contact*
                 make_float4(q.normal_impulse, q.tangent_impulse[0], q.tangent_impulse[1], q.tangent_impulse[2]);
             rp.peak[(size_t)m * cap + k] = q.max_normal_impulse;
         }
+        if (c.planar) {
+            const Mass ma = loadPlanarMass(mass, e.y), mb = loadPlanarMass(mass, e.z);
+            TF_POINTS(m, c.count) {
+                const PlanarAxes x = planarAxes(ma, mb, c, m);
+                rp.ax[(size_t)(2 * m) * cap + k] = make_float4(x.normalA, x.normalB, x.normalMass, x.tangentA);
+                rp.ax[(size_t)(2 * m + 1) * cap + k] = make_float4(x.tangentB, x.tangentMass, 0.0f, 0.0f);
+            }
+        }
         rp.pos[ci] = k;
+    }
+}
+
+__global__ void k_rest_impact_pos(uint4* entries, uint32_t n, const uint32_t* pos) {
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        uint4 e = entries[k];
+        e.w = (e.w & ((1u << kRestPosShift) - 1u)) | (pos[e.x] << kRestPosShift);
+        entries[k] = e;
     }
 }
 
@@ -1752,10 +1788,23 @@ __global__ void k_rest_unpack(const uint4* entries, uint32_t n, RestPack rp, tfs
     }
 }
 
-// A rest contact from the pack, in the fields the rest solves read (the others are left unset).
-__device__ __forceinline__ void loadRest(const RestPack& rp, uint32_t k, tfs_contact& c) {
+// A rest contact from the pack, in the fields the rest solves read (the others are left unset), and
+// its planar axes. The slots of all four points are loaded whatever the count, so that no load waits
+// for another; the solves read only the first count of them.
+__device__ __forceinline__ void loadRest(const RestPack& rp, uint32_t k, tfs_contact& c, PlanarAxes* ax) {
     const size_t cap = rp.cap;
     const float4 a = rp.h[k], b = rp.h[cap + k], d = rp.h[2 * cap + k], e = rp.h[3 * cap + k], f = rp.h[4 * cap + k];
+    float4 pa[4], pb[4], im[4], x0[4], x1[4];
+    float peak[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        pa[m] = rp.p[(size_t)(m * 3) * cap + k];
+        pb[m] = rp.p[(size_t)(m * 3 + 1) * cap + k];
+        im[m] = rp.p[(size_t)(m * 3 + 2) * cap + k];
+        peak[m] = rp.peak[(size_t)m * cap + k];
+        x0[m] = rp.ax[(size_t)(2 * m) * cap + k];
+        x1[m] = rp.ax[(size_t)(2 * m + 1) * cap + k];
+    }
     const uint32_t meta = __float_as_uint(f.w);
     c.count = (int32_t)(meta & 7u);
     c.planar = (meta & 8u) ? 1 : 0;
@@ -1767,15 +1816,12 @@ __device__ __forceinline__ void loadRest(const RestPack& rp, uint32_t k, tfs_con
     c.rolling_impulse[0] = f.x; c.rolling_impulse[1] = f.y; c.rolling_impulse[2] = f.z;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-        if (m < c.count) {
-            const float4 pa = rp.p[(size_t)(m * 3) * cap + k], pb = rp.p[(size_t)(m * 3 + 1) * cap + k];
-            const float4 im = rp.p[(size_t)(m * 3 + 2) * cap + k];
-            tfs_point& q = c.points[m];
-            q.anchor_a[0] = pa.x; q.anchor_a[1] = pa.y; q.anchor_a[2] = pa.z; q.adjusted_separation = pa.w;
-            q.anchor_b[0] = pb.x; q.anchor_b[1] = pb.y; q.anchor_b[2] = pb.z; q.relative_velocity = pb.w;
-            q.normal_impulse = im.x; q.tangent_impulse[0] = im.y; q.tangent_impulse[1] = im.z; q.tangent_impulse[2] = im.w;
-            q.max_normal_impulse = rp.peak[(size_t)m * cap + k];
-        }
+        tfs_point& q = c.points[m];
+        q.anchor_a[0] = pa[m].x; q.anchor_a[1] = pa[m].y; q.anchor_a[2] = pa[m].z; q.adjusted_separation = pa[m].w;
+        q.anchor_b[0] = pb[m].x; q.anchor_b[1] = pb[m].y; q.anchor_b[2] = pb[m].z; q.relative_velocity = pb[m].w;
+        q.normal_impulse = im[m].x; q.tangent_impulse[0] = im[m].y; q.tangent_impulse[1] = im[m].z; q.tangent_impulse[2] = im[m].w;
+        q.max_normal_impulse = peak[m];
+        ax[m] = PlanarAxes{ x0[m].x, x0[m].y, x0[m].z, x0[m].w, x1[m].x, x1[m].y };
     }
 }
 
@@ -1830,8 +1876,9 @@ __global__ void __launch_bounds__(kThreads) k_warm_rest_bodies(uint32_t nb, cons
             const bool sideB = en & 1u;
             const uint32_t flags = entries[k].w;
             tfs_contact c;
-            loadRest(rp, k, c);
-            warmRestSide(c, sideB, !(flags & (sideB ? 8u : 4u)), mx, m);
+            PlanarAxes ax[4];
+            loadRest(rp, k, c, ax);
+            warmRestSide(c, sideB, !(flags & (sideB ? 8u : 4u)), mx, m, ax);
         }
         storeVelocity(motion, x, m);
     }
@@ -1864,9 +1911,10 @@ __global__ void __launch_bounds__(ONE_BLOCK ? kRestBlockThreads : kPassThreads)
         }
         for (uint32_t i = first; i < end; i += stride) {
             const uint4 e = i == first ? firstEntry : entries[i];
-            const uint32_t k = viaPos ? rp.pos[e.x] : i;
+            const uint32_t k = viaPos ? e.w >> kRestPosShift : i;
             tfs_contact c;
-            loadRest(rp, k, c);
+            PlanarAxes ax[4];
+            loadRest(rp, k, c, ax);
             const uint32_t ia = e.y, ib = e.z;
             const V3 none = vec(0.0f, 0.0f, 0.0f);
             const BodyView va{ (uint8_t)((e.w & 4u) ? TFS_STATIC : TFS_DYNAMIC), 1, 0, none };
@@ -1879,15 +1927,12 @@ __global__ void __launch_bounds__(ONE_BLOCK ? kRestBlockThreads : kPassThreads)
             // a planar contact reads only the inverse mass and the planar inverse mass and inertia
             const Mass ma = c.planar ? loadPlanarMass(mass, ia) : loadMass(mass, ia);
             const Mass mb = c.planar ? loadPlanarMass(mass, ib) : loadMass(mass, ib);
-            if (MODE == kWarm) {
-                warmRest(c, va, vb, ma, mb, a, b);
-            } else if (MODE == kRestitute) {
-                restituteRest(c, va, vb, ma, mb, a, b, ctx);
-                storeRest(c, rp, k);
+            if (MODE == kRestitute) {
+                restituteRest(c, va, vb, ma, mb, a, b, ctx, ax);
             } else {
-                solveRest<MODE == kBias>(c, va, vb, ma, mb, a, b, ctx);
-                storeRest(c, rp, k);
+                solveRest<MODE == kBias>(c, va, vb, ma, mb, a, b, ctx, ax);
             }
+            storeRest(c, rp, k);
             if (e.w & 1u) {
                 X::st(&motion[4 * ia], make_float4(a.v.x, a.v.y, a.v.z, 0.0f));
                 X::st(&motion[4 * ia + 1], make_float4(a.w.x, a.w.y, a.w.z, 0.0f));
@@ -2311,10 +2356,12 @@ struct tfs_ctx {
     uint4* restEntries[2] = {};
     uint32_t* restLevelStart[2] = {};
     uint32_t restLevels[2] = {};
+    uint32_t restWide[2] = {};     // leading levels wider than a rest work-group
     uint32_t *wrestDeg = nullptr, *wrestStart = nullptr, *wrestCursor = nullptr, *wrestEp = nullptr;  // rest warm start lists
     float4* restH = nullptr;        // the active rest contacts packed in level order (RestPack)
     float4* restP = nullptr;
     float* restPeak = nullptr;
+    float4* restAx = nullptr;
     uint32_t* restPos = nullptr;
     uint8_t* colors = nullptr;
     uint32_t* ep = nullptr;
@@ -2429,6 +2476,9 @@ int allocRows(tfs_ctx* ctx, Rows& r, uint32_t cap) {
 
 int allocContacts(tfs_ctx* ctx, uint32_t cap) {
     int r;
+    if (cap >= (1u << (32 - kRestPosShift))) {
+        return fail(ctx, TFS_E_ARG, "contact capacity %u is beyond the limit of %u", cap, (1u << (32 - kRestPosShift)) - 1u);
+    }
     if ((r = grow(ctx, ctx->contacts, cap)) || (r = grow(ctx, ctx->fresh, cap)) || (r = grow(ctx, ctx->fActive, cap)) ||
         (r = grow(ctx, ctx->fFast, cap)) || (r = grow(ctx, ctx->fFastSolid, cap)) || (r = grow(ctx, ctx->active, cap)) ||
         (r = grow(ctx, ctx->fast, cap)) || (r = grow(ctx, ctx->fastSolid, cap)) ||
@@ -2442,7 +2492,7 @@ int allocContacts(tfs_ctx* ctx, uint32_t cap) {
         (r = grow(ctx, ctx->restEntries[0], cap)) || (r = grow(ctx, ctx->restEntries[1], cap)) ||
         (r = grow(ctx, ctx->restLevelStart[0], (size_t)cap + 1)) ||
         (r = grow(ctx, ctx->restLevelStart[1], (size_t)cap + 1)) || (r = grow(ctx, ctx->restH, 5 * (size_t)cap)) ||
-        (r = grow(ctx, ctx->restP, 12 * (size_t)cap)) || (r = grow(ctx, ctx->restPeak, 4 * (size_t)cap)) ||
+        (r = grow(ctx, ctx->restP, 12 * (size_t)cap)) || (r = grow(ctx, ctx->restPeak, 4 * (size_t)cap)) || (r = grow(ctx, ctx->restAx, 8 * (size_t)cap)) ||
         (r = grow(ctx, ctx->restPos, cap)) || (r = grow(ctx, ctx->wrestEp, 2 * (size_t)cap))) {
         return r;
     }
@@ -2723,21 +2773,23 @@ int buildRestLevels(tfs_ctx* ctx, int which, const uint32_t* list, uint32_t n) {
     }
     LAUNCH(k_rest_place, blocksFor(n), kThreads, list, n, ctx->contacts, ctx->moving, z, ctx->restLevelStart[which],
            ctx->restEntries[which]);
+    LAUNCH(k_rest_narrow, blocksFor(n), kThreads, ctx->restLevelStart[which], &ctx->cnt->restLevels[which],
+           (uint32_t)kRestBlockThreads, &ctx->cnt->restNarrow[which]);
     return 0;
 }
 
-// A rest pass: the list's levels, in one work-group when they are narrow (on average under a block of
-// threads per level), else in one cooperative launch.
-RestPack restPack(const tfs_ctx* ctx) { return RestPack{ ctx->restH, ctx->restP, ctx->restPeak, ctx->restPos, ctx->capC }; }
+// A rest pass: the list's leading wide levels (more contacts than a rest work-group has threads) in
+// one cooperative launch over many blocks, the levels from the first narrow one on in one work-group.
+RestPack restPack(const tfs_ctx* ctx) {
+    return RestPack{ ctx->restH, ctx->restP, ctx->restPeak, ctx->restAx, ctx->restPos, ctx->capC };
+}
 
 int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32_t end, const Ctx& c) {
     if (end <= begin) { return 0; }
     const int which = list == ctx->impactRowsList ? 1 : 0;
     const uint32_t levels = ctx->restLevels[which], n = end - begin;
     if (levels == 0) { return fail(ctx, TFS_E_STATE, "rest list without levels"); }
-    const bool oneBlock = n / levels < (uint32_t)kRestBlockThreads;
-    const uint32_t grid = oneBlock ? 1u
-                                   : std::max<uint32_t>(1, std::min<uint32_t>((n + 63) / 64, ctx->maxPassBlocks));
+    const uint32_t wide = std::min(ctx->restWide[which], levels);
     const uint4* entries = ctx->restEntries[which];
     const uint32_t* starts = ctx->restLevelStart[which];
     BarrierLines* bar = ctx->bar;
@@ -2745,23 +2797,25 @@ int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32
     const float4* mass = ctx->mass;
     const RestPack rp = restPack(ctx);
     const bool viaPos = which == 1;
-    if (oneBlock) {
-        switch (mode) {
-            case kWarm: LAUNCH((k_rest_levels<kWarm, true>), 1, kRestBlockThreads, entries, starts, levels, rp, viaPos, ctx->motion, mass, c, bar, base, ctx->cnt); break;
-            case kBias: LAUNCH((k_rest_levels<kBias, true>), 1, kRestBlockThreads, entries, starts, levels, rp, viaPos, ctx->motion, mass, c, bar, base, ctx->cnt); break;
-            case kRelax: LAUNCH((k_rest_levels<kRelax, true>), 1, kRestBlockThreads, entries, starts, levels, rp, viaPos, ctx->motion, mass, c, bar, base, ctx->cnt); break;
-            default: LAUNCH((k_rest_levels<kRestitute, true>), 1, kRestBlockThreads, entries, starts, levels, rp, viaPos, ctx->motion, mass, c, bar, base, ctx->cnt); break;
-        }
-        return 0;
+    if (wide > 0) {
+        const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((n + 63) / 64, ctx->maxPassBlocks));
+        void* args[] = { (void*)&entries, (void*)&starts, (void*)&wide, (void*)&rp, (void*)&viaPos, &ctx->motion,
+                         (void*)&mass, (void*)&c, (void*)&bar, (void*)&base, &ctx->cnt };
+        const void* fn = mode == kBias ? (const void*)k_rest_levels<kBias, false>
+                         : mode == kRelax ? (const void*)k_rest_levels<kRelax, false>
+                                          : (const void*)k_rest_levels<kRestitute, false>;
+        HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kPassThreads), args, 0, ctx->stream));
+        advanceBarrier(ctx->barrierBase, grid, wide - 1);
     }
-    void* args[] = { (void*)&entries, (void*)&starts, (void*)&levels, (void*)&rp, (void*)&viaPos, &ctx->motion,
-                     (void*)&mass, (void*)&c, (void*)&bar, (void*)&base, &ctx->cnt };
-    const void* fn = mode == kWarm ? (const void*)k_rest_levels<kWarm, false>
-                     : mode == kBias ? (const void*)k_rest_levels<kBias, false>
-                     : mode == kRelax ? (const void*)k_rest_levels<kRelax, false>
-                                      : (const void*)k_rest_levels<kRestitute, false>;
-    HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kPassThreads), args, 0, ctx->stream));
-    advanceBarrier(ctx->barrierBase, grid, levels - 1);
+    if (wide < levels) {
+        const uint32_t* from = starts + wide;
+        const uint32_t left = levels - wide;
+        switch (mode) {
+            case kBias: LAUNCH((k_rest_levels<kBias, true>), 1, kRestBlockThreads, entries, from, left, rp, viaPos, ctx->motion, mass, c, bar, base, ctx->cnt); break;
+            case kRelax: LAUNCH((k_rest_levels<kRelax, true>), 1, kRestBlockThreads, entries, from, left, rp, viaPos, ctx->motion, mass, c, bar, base, ctx->cnt); break;
+            default: LAUNCH((k_rest_levels<kRestitute, true>), 1, kRestBlockThreads, entries, from, left, rp, viaPos, ctx->motion, mass, c, bar, base, ctx->cnt); break;
+        }
+    }
     return 0;
 }
 
@@ -2837,7 +2891,7 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
            ctx->contacts, ctx->bodies, ctx->motion, ctx->mass, matchSq);
     // A refreshed contact keeps its row; its points and axes are packed again, as prepareLanes does.
     if (int e = packRows(ctx, laned, impactLaned, false)) { return e; }
-    if (nRest > 0) { LAUNCH(k_rest_pack, blocksFor(nRest), kThreads, ctx->restEntries[0], nRest, ctx->contacts, restPack(ctx)); }
+    if (nRest > 0) { LAUNCH(k_rest_pack, blocksFor(nRest), kThreads, ctx->restEntries[0], nRest, ctx->contacts, ctx->mass, restPack(ctx)); }
     return 0;
 }
 
@@ -2907,22 +2961,33 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
         for (int i = 1; i < n; ++i) { needs = needs || (i == n / 2 ? hc.nFast : hc.nFastSolid) > 0; }
         if (needs) { return fail(ctx, TFS_E_CALLBACK, "fast contacts need a refresh callback"); }
     }
-    ctx->restLevels[0] = ctx->restLevels[1] = 0;
+    ctx->restLevels[0] = ctx->restLevels[1] = ctx->restWide[0] = ctx->restWide[1] = 0;
     if (hc.nActive > laned || hc.nImpact > impactLaned) {
         if ((r = buildRestLevels(ctx, 0, ctx->rowsList + laned, hc.nActive - laned)) ||
             (r = buildRestLevels(ctx, 1, ctx->impactRowsList + impactLaned, hc.nImpact - impactLaned))) {
             return r;
         }
-        HIP_TRY(hipMemcpyAsync(ctx->hostCnt->restLevels, ctx->cnt->restLevels, sizeof(ctx->cnt->restLevels),
+        static_assert(offsetof(DevCounts, restNarrow) == offsetof(DevCounts, restLevels) + 2 * sizeof(uint32_t),
+                      "restLevels and restNarrow are read back together");
+        HIP_TRY(hipMemcpyAsync(ctx->hostCnt->restLevels, ctx->cnt->restLevels, 4 * sizeof(uint32_t),
                                hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         if (ctx->hostCnt->error) { return fail(ctx, TFS_E_KERNEL, "a kernel's bounded wait ran out (error %u)", ctx->hostCnt->error); }
-        ctx->restLevels[0] = ctx->hostCnt->restLevels[0];
-        ctx->restLevels[1] = ctx->hostCnt->restLevels[1];
+        for (int w = 0; w < 2; ++w) {
+            ctx->restLevels[w] = ctx->hostCnt->restLevels[w];
+            const uint32_t narrow = ~ctx->hostCnt->restNarrow[w];  // ~0 -> no narrow level
+            ctx->restWide[w] = std::min(narrow, ctx->restLevels[w]);
+        }
         if (hc.nActive > laned) {
             const uint32_t nRest = hc.nActive - laned;
-            LAUNCH(k_rest_pack, blocksFor(nRest), kThreads, ctx->restEntries[0], nRest, ctx->contacts, restPack(ctx));
+            LAUNCH(k_rest_pack, blocksFor(nRest), kThreads, ctx->restEntries[0], nRest, ctx->contacts, ctx->mass, restPack(ctx));
+            // the impact rest is a part of the active rest: both lists take the active colouring
+            // (src/world.cpp:1651-1657), so every impact rest contact has a place in the pack
+            if (hc.nImpact > impactLaned) {
+                LAUNCH(k_rest_impact_pos, blocksFor(hc.nImpact - impactLaned), kThreads, ctx->restEntries[1],
+                       hc.nImpact - impactLaned, ctx->restPos);
+            }
             LAUNCH(k_fill_u32, blocksFor(nb), kThreads, ctx->wrestDeg, nb, 0u);
             LAUNCH(k_fill_u32, blocksFor(nb), kThreads, ctx->wrestCursor, nb, 0u);
             LAUNCH(k_wrest_count, blocksFor(nRest), kThreads, ctx->restEntries[0], nRest, ctx->wrestDeg);
@@ -3148,9 +3213,43 @@ int fixedBegin(tfs_ctx* ctx, float dt, uint32_t maxSteps) {
 
 }  // namespace
 
+// The solves' max and min (maxq, minq) over given pairs, for tfs_check_minmax.
+__global__ void k_minmax(const float* a, const float* b, uint32_t n, float* out) {
+    for (uint32_t i = gid(); i < n; i += gstride()) {
+        out[2 * i] = maxq(a[i], b[i]);
+        out[2 * i + 1] = minq(a[i], b[i]);
+    }
+}
+
 extern "C" {
 
 int tfs_abi_version(void) { return TFS_ABI_VERSION; }
+
+int tfs_check_minmax(int device, const float* a, const float* b, uint32_t n, float* out) {
+    if (!a || !b || !out) { return TFS_E_ARG; }
+    if (n == 0) { return 0; }
+    if (hipSetDevice(device) != hipSuccess) { return TFS_E_DEVICE; }
+    float *da = nullptr, *db = nullptr, *dout = nullptr;
+    int r = 0;
+    if (hipMalloc((void**)&da, sizeof(float) * n) != hipSuccess || hipMalloc((void**)&db, sizeof(float) * n) != hipSuccess ||
+        hipMalloc((void**)&dout, 2 * sizeof(float) * n) != hipSuccess ||
+        hipMemcpy(da, a, sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(db, b, sizeof(float) * n, hipMemcpyHostToDevice) != hipSuccess) {
+        r = TFS_E_DEVICE;
+    }
+    if (!r) {
+        hipLaunchKernelGGL(k_minmax, dim3(std::min<uint32_t>((n + kThreads - 1) / kThreads, 4096u)), dim3(kThreads), 0, 0,
+                           da, db, n, dout);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpy(out, dout, 2 * sizeof(float) * n, hipMemcpyDeviceToHost) != hipSuccess) {
+            r = TFS_E_DEVICE;
+        }
+    }
+    for (float* p : { da, db, dout }) {
+        if (p) { (void)hipFree(p); }
+    }
+    return r;
+}
 size_t tfs_sizeof_body(void) { return sizeof(tfs_body); }
 size_t tfs_sizeof_contact(void) { return sizeof(tfs_contact); }
 size_t tfs_sizeof_settings(void) { return sizeof(tfs_settings); }
@@ -3235,7 +3334,7 @@ void tfs_destroy(tfs_ctx* ctx) {
                      ctx->edges, ctx->bodyWorld, ctx->refreshing, ctx->worldStart, ctx->restTracked,
                      ctx->restDeg, ctx->restStart, ctx->restCursor, ctx->restEp, ctx->restRank, ctx->restLevel,
                      ctx->restCount, ctx->restWord, ctx->restEntries[0], ctx->restEntries[1], ctx->restLevelStart[0],
-                     ctx->restLevelStart[1], ctx->restH, ctx->restP, ctx->restPeak, ctx->restPos, ctx->wrestDeg,
+                     ctx->restLevelStart[1], ctx->restH, ctx->restP, ctx->restPeak, ctx->restAx, ctx->restPos, ctx->wrestDeg,
                      ctx->wrestStart, ctx->wrestCursor, ctx->wrestEp, ctx->ckBody, ctx->ckRow, ctx->tapeBuf, ctx->adj.bar, ctx->adj.dpBar,
                      ctx->adj.comBar, ctx->adj.impBar, ctx->adj.sepBar, ctx->adj.muBar, ctx->dlcom, ctx->muContact };
     for (void* p : ptrs) {

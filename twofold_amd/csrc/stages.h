@@ -302,10 +302,10 @@ TF_HD NormalTrace normalPoint(RowPoint& p, float sep, const Axis& x, V3 n, Motio
     const float vn = speedAlong(a, b, n, x);
     float impulse = -x.mass * massScale * (vn + bias) - impulseScale * p.ni;
     const float sum = p.ni + impulse;
-    const float total = maxf(sum, 0.0f);
+    const float total = maxq(sum, 0.0f);
     impulse = total - p.ni;
     p.ni = total;
-    p.nimax = maxf(p.nimax, impulse);
+    p.nimax = maxq(p.nimax, impulse);
     push(a, b, x, impulse, moveA, moveB);
     return NormalTrace{ massScale, impulseScale, dbias, sum > 0.0f };
 }
@@ -467,11 +467,11 @@ TF_HD void restituteRowN(const Row& r, RowPoint* p, const AX& ax, Motion& a, Mot
         const Axis x = ax(i, 0);
         const float vn = speedAlong(a, b, r.n, x);
         float impulse = -x.mass * (vn + e * p[i].rv);
-        const float total = maxf(p[i].ni + impulse, 0.0f);
+        const float total = maxq(p[i].ni + impulse, 0.0f);
         impulse = total - p[i].ni;
         if (on) {
             p[i].ni = total;
-            p[i].nimax = maxf(p[i].nimax, impulse);
+            p[i].nimax = maxq(p[i].nimax, impulse);
         }
         push(a, b, x, impulse, on && moveA, on && moveB);
     }
@@ -632,52 +632,12 @@ TF_HD void planarTurn(const Motion& m, V3 n, float& c, float& s) {
     s = 2.0f * q.w * dot(n, vec(q.x, q.y, q.z));
 }
 
-// warmStart, one contact that is not laned: src/solver.cpp:319-358.
-TF_HD void warmRest(tfs_contact& c, const BodyView& va, const BodyView& vb, const Mass& ma, const Mass& mb, Motion& a,
-                    Motion& b) {
-    const V3 n = ld3(c.normal), t1 = ld3(c.tangent1), t2 = ld3(c.tangent2);
-    if (c.planar) {
-        bool held = c.rolling_resistance > 0.0f;
-        TF_POINTS(i, c.count) {
-            const V3 ti = ld3(c.points[i].tangent_impulse);
-            held = held || c.points[i].normal_impulse != 0.0f || dot(ti, ti) != 0.0f;
-        }
-        if (!held) { return; }
-        const V3 plane = ld3(c.plane);
-        PlanarState st(a, b, ma, mb, plane);
-        TF_POINTS(i, c.count) {
-            const tfs_point& p = c.points[i];
-            const PlanarAxes x = planarAxes(ma, mb, c, i);
-            st.push(n, x.normalA, x.normalB, p.normal_impulse);
-            st.push(t1, x.tangentA, x.tangentB, dot(ld3(p.tangent_impulse), t1));
-        }
-        if (c.rolling_resistance > 0.0f) {
-            const float r = dot(ld3(c.rolling_impulse), plane);
-            st.wa -= st.ia * r;
-            st.wb += st.ib * r;
-        }
-        st.store(a, b, plane, !anchoredView(va), !anchoredView(vb));
-        return;
-    }
-    TF_POINTS(i, c.count) {
-        const tfs_point& p = c.points[i];
-        const V3 ra = ld3(p.anchor_a), rb = ld3(p.anchor_b);
-        push(a, b, makeAxis(ma, mb, ra, rb, n), p.normal_impulse, true, true);
-        push(a, b, makeAxis(ma, mb, ra, rb, t1), dot(ld3(p.tangent_impulse), t1), true, true);
-        push(a, b, makeAxis(ma, mb, ra, rb, t2), dot(ld3(p.tangent_impulse), t2), true, true);
-    }
-    if (c.rolling_resistance > 0.0f) {
-        a.w = a.w - ma.iiw * ld3(c.rolling_impulse);
-        b.w = b.w + mb.iiw * ld3(c.rolling_impulse);
-    }
-}
-
-// warmRest's effect on one of the contact's bodies (sideB: body B), which the caller applies to the
-// bodies the level-by-level pass would write back. Each body's operations are warmRest's on that
-// body, in the same order: a planar contact changes a body only when something is held and the body
-// moves (PlanarState::push, ::store), a 3D contact pushes both bodies (push with both flags set) and
-// turns them by the rolling impulse.
-TF_HD void warmRestSide(const tfs_contact& c, bool sideB, bool moves, const Mass& m, Motion& x) {
+// warmStart's effect (src/solver.cpp:319-358) of one contact that is not laned on one of its bodies
+// (sideB: body B), in the order warmStart applies it to that body: a planar contact changes a body
+// only when something is held and the body moves (PlanarState::push, ::store; the arms are the
+// contact's planar axes), a 3D contact pushes both bodies (push with both flags set) and turns them by
+// the rolling impulse.
+TF_HD void warmRestSide(const tfs_contact& c, bool sideB, bool moves, const Mass& m, Motion& x, const PlanarAxes* ax) {
     const V3 n = ld3(c.normal), t1 = ld3(c.tangent1), t2 = ld3(c.tangent2);
     if (c.planar) {
         bool held = c.rolling_resistance > 0.0f;
@@ -691,8 +651,7 @@ TF_HD void warmRestSide(const tfs_contact& c, bool sideB, bool moves, const Mass
         float w = dot(x.w, plane);
         TF_POINTS(i, c.count) {
             const tfs_point& p = c.points[i];
-            const V3 r = sideB ? ld3(p.anchor_b) : ld3(p.anchor_a);
-            const float armN = dot(plane, cross(r, n)), armT = dot(plane, cross(r, t1));
+            const float armN = sideB ? ax[i].normalB : ax[i].normalA, armT = sideB ? ax[i].tangentB : ax[i].tangentA;
             const float in = p.normal_impulse, it = dot(ld3(p.tangent_impulse), t1);
             if (sideB) {
                 v = v + n * (m.pim * in);
@@ -748,7 +707,7 @@ TF_HD void warmRestSide(const tfs_contact& c, bool sideB, bool moves, const Mass
 // solvePlanar: src/solver.cpp:137-209.
 template <bool useBias>
 TF_HD void solvePlanar(tfs_contact& c, const BodyView& va, const BodyView& vb, const Mass& ma, const Mass& mb,
-                       Motion& a, Motion& b, const Ctx& ctx) {
+                       Motion& a, Motion& b, const Ctx& ctx, const PlanarAxes* axes) {
     const bool moveA = !anchoredView(va), moveB = !anchoredView(vb);
     const Soft soft = (!moveA || !moveB) ? ctx.staticSoft : ctx.contactSoft;
     const V3 normal = ld3(c.normal), tangent1 = ld3(c.tangent1), plane = ld3(c.plane);
@@ -760,7 +719,7 @@ TF_HD void solvePlanar(tfs_contact& c, const BodyView& va, const BodyView& vb, c
     bool changed = false;
     TF_POINTS(i, c.count) {
         tfs_point& p = c.points[i];
-        const PlanarAxes x = planarAxes(ma, mb, c, i);
+        const PlanarAxes& x = axes[i];
         const float reachA = tax * x.tangentA + tay * x.normalA;
         const float reachB = tbx * x.tangentB + tby * x.normalB;
         const float sep = shift + reachB - reachA + p.adjusted_separation;
@@ -774,10 +733,10 @@ TF_HD void solvePlanar(tfs_contact& c, const BodyView& va, const BodyView& vb, c
         }
         const float vn = st.speed(normal, x.normalA, x.normalB);
         float impulse = -x.normalMass * massScale * (vn + bias) - impulseScale * p.normal_impulse;
-        const float total = maxf(p.normal_impulse + impulse, 0.0f);
+        const float total = maxq(p.normal_impulse + impulse, 0.0f);
         impulse = total - p.normal_impulse;
         p.normal_impulse = total;
-        p.max_normal_impulse = maxf(p.max_normal_impulse, impulse);
+        p.max_normal_impulse = maxq(p.max_normal_impulse, impulse);
         if (impulse != 0.0f) {
             st.push(normal, x.normalA, x.normalB, impulse);
             changed = true;
@@ -789,12 +748,12 @@ TF_HD void solvePlanar(tfs_contact& c, const BodyView& va, const BodyView& vb, c
         const int i = useBias ? j : 3 - j;
         if (i >= c.count) { continue; }
         tfs_point& p = c.points[i];
-        const PlanarAxes x = planarAxes(ma, mb, c, i);
+        const PlanarAxes& x = axes[i];
         const float old = dot(ld3(p.tangent_impulse), tangent1);
         if (p.normal_impulse == 0.0f && old == 0.0f) { continue; }
         changed = true;
         const float limit = c.friction * p.normal_impulse;
-        const float t = minf(maxf(old - x.tangentMass * st.speed(tangent1, x.tangentA, x.tangentB), -limit), limit);
+        const float t = minq(maxq(old - x.tangentMass * st.speed(tangent1, x.tangentA, x.tangentB), -limit), limit);
         st3(p.tangent_impulse, tangent1 * t);
         st.push(tangent1, x.tangentA, x.tangentB, t - old);
     }
@@ -812,7 +771,7 @@ TF_HD void solvePlanar(tfs_contact& c, const BodyView& va, const BodyView& vb, c
         float normalSum = 0.0f;
         TF_POINTS(i, c.count) { normalSum += c.points[i].normal_impulse; }
         const float limit = c.rolling_resistance * c.rolling_radius * normalSum;
-        const float held = minf(maxf(total, -limit), limit);
+        const float held = minq(maxq(total, -limit), limit);
         st.wa -= st.ia * (held - total);
         st.wb += st.ib * (held - total);
         st3(c.rolling_impulse, plane * held);
@@ -881,9 +840,9 @@ TF_HD Row rowOf(const tfs_contact& c, bool moveA, bool moveB) {
 // solveContacts, one contact of the rest: src/solver.cpp:360-421.
 template <bool useBias>
 TF_HD void solveRest(tfs_contact& c, const BodyView& va, const BodyView& vb, const Mass& ma, const Mass& mb, Motion& a,
-                     Motion& b, const Ctx& ctx) {
+                     Motion& b, const Ctx& ctx, const PlanarAxes* axes) {
     if (c.planar) {
-        solvePlanar<useBias>(c, va, vb, ma, mb, a, b, ctx);
+        solvePlanar<useBias>(c, va, vb, ma, mb, a, b, ctx, axes);
         return;
     }
     RowPoint p[4];
@@ -901,7 +860,7 @@ TF_HD void solveRest(tfs_contact& c, const BodyView& va, const BodyView& vb, con
 
 // applyRestitution, one contact that is not laned: src/solver.cpp:423-460.
 TF_HD void restituteRest(tfs_contact& c, const BodyView& va, const BodyView& vb, const Mass& ma, const Mass& mb,
-                         Motion& a, Motion& b, const Ctx& ctx) {
+                         Motion& a, Motion& b, const Ctx& ctx, const PlanarAxes* axes) {
     if (c.restitution == 0.0f || c.laned) { return; }
     const V3 n = ld3(c.normal);
     if (c.planar) {
@@ -909,13 +868,13 @@ TF_HD void restituteRest(tfs_contact& c, const BodyView& va, const BodyView& vb,
         TF_POINTS(i, c.count) {
             tfs_point& p = c.points[i];
             if (p.relative_velocity > -ctx.restitutionThreshold || p.max_normal_impulse == 0.0f) { continue; }
-            const PlanarAxes x = planarAxes(ma, mb, c, i);
+            const PlanarAxes& x = axes[i];
             const float vn = st.speed(n, x.normalA, x.normalB);
             float impulse = -x.normalMass * (vn + c.restitution * p.relative_velocity);
-            const float total = maxf(p.normal_impulse + impulse, 0.0f);
+            const float total = maxq(p.normal_impulse + impulse, 0.0f);
             impulse = total - p.normal_impulse;
             p.normal_impulse = total;
-            p.max_normal_impulse = maxf(p.max_normal_impulse, impulse);
+            p.max_normal_impulse = maxq(p.max_normal_impulse, impulse);
             st.push(n, x.normalA, x.normalB, impulse);
         }
         st.store(a, b, ld3(c.plane), !anchoredView(va), !anchoredView(vb));
@@ -927,10 +886,10 @@ TF_HD void restituteRest(tfs_contact& c, const BodyView& va, const BodyView& vb,
         const Axis x = makeAxis(ma, mb, ld3(p.anchor_a), ld3(p.anchor_b), n);
         const float vn = speedAlong(a, b, n, x);
         float impulse = -x.mass * (vn + c.restitution * p.relative_velocity);
-        const float total = maxf(p.normal_impulse + impulse, 0.0f);
+        const float total = maxq(p.normal_impulse + impulse, 0.0f);
         impulse = total - p.normal_impulse;
         p.normal_impulse = total;
-        p.max_normal_impulse = maxf(p.max_normal_impulse, impulse);
+        p.max_normal_impulse = maxq(p.max_normal_impulse, impulse);
         push(a, b, x, impulse, true, true);
     }
 }

@@ -24,7 +24,7 @@ EXPORTS = ("tfs_abi_version", "tfs_sizeof_body", "tfs_sizeof_contact", "tfs_size
            "tfs_create", "tfs_destroy", "tfs_last_error", "tfs_set_settings", "tfs_upload_bodies",
            "tfs_upload_contacts", "tfs_set_worlds", "tfs_step", "tfs_download_bodies", "tfs_download_contacts", "tfs_download_list",
            "tfs_last_step_ms", "tfs_last_step_counts", "tfs_set_profiling", "tfs_read_profile", "tfs_fixed_begin",
-           "tfs_fixed_step", "tfs_adjoint_seed", "tfs_adjoint_step", "tfs_adjoint_read")
+           "tfs_fixed_step", "tfs_adjoint_seed", "tfs_adjoint_step", "tfs_adjoint_read", "tfs_check_minmax")
 
 
 class SolverError(RuntimeError):
@@ -66,6 +66,7 @@ def load_library(path=LIB_PATH):
     lib.tfs_adjoint_seed.argtypes = [_vp, _vp]
     lib.tfs_adjoint_step.argtypes = [_vp]
     lib.tfs_adjoint_read.argtypes = [_vp, _vp, _vp, _vp]
+    lib.tfs_check_minmax.argtypes = [ctypes.c_int, _vp, _vp, ctypes.c_uint32, _vp]
     lib.tfs_read_profile.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_double)]
     if lib.tfs_abi_version() != 1:
@@ -75,6 +76,20 @@ def load_library(path=LIB_PATH):
         raise SolverError("tfs_settings layout mismatch")
     _lib = lib
     return lib
+
+
+def device_minmax(a, b, device=0):
+    """The solves' device max and min of float32 pairs: (max, min) arrays (tfs_check_minmax)."""
+    lib = load_library()
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    if a.shape != b.shape or a.ndim != 1:
+        raise SolverError("a and b must be 1-D arrays of one length")
+    out = np.zeros(2 * len(a), np.float32)
+    r = lib.tfs_check_minmax(device, _ptr(a), _ptr(b), len(a), _ptr(out))
+    if r != 0:
+        raise SolverError(f"tfs_check_minmax failed ({r})")
+    return out[0::2].copy(), out[1::2].copy()
 
 
 def _ptr(a):
