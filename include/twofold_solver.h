@@ -237,7 +237,9 @@ int tfs_last_step_ms(tfs_ctx* ctx, float* solve_ms, float* sleep_ms);
  * with n points: warm start 224 + 244n, solve 224 + 296n, restitution 224 + 96n). Two more families
  * time the host side of each refresh with the host clock: 3 the wait for the contacts collided again
  * to reach host memory, 4 the refresh callback itself (its "bytes" sum counts the pairs collided
- * again). Turning it on clears the sums; turning it off keeps them for reading.
+ * again). Family 5 is the biased and relax passes over the contacts outside the colours (the rest
+ * lists, active and impact), event-timed per pass over one list, with bytes per contact of 248 + 68n
+ * (planar) or 224 + 296n (3D). Turning it on clears the sums; turning it off keeps them for reading.
  */
 int tfs_set_profiling(tfs_ctx* ctx, int on);
 int tfs_read_profile(tfs_ctx* ctx, int family, uint64_t* launches, double* ms, double* bytes);

@@ -36,7 +36,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kScanSlots = 16;         // scans in one step
-constexpr int kProfileFamilies = 5;
+constexpr int kProfileFamilies = 6;
 constexpr int kKeys = 5 * TFS_COLORS + 1; // row-order buckets: colour x point count, then the rest
 constexpr uint32_t kSpinLimit = 1u << 26;
 constexpr int kIslandRounds = 64;      // hook-and-jump rounds of the island pass before it gives up
@@ -51,6 +51,7 @@ struct DevCounts {
     uint32_t restTicket[2];   // level dataflow of the active and impact rest lists
     uint32_t restLevels[2];
     uint32_t restNarrow[2];   // ~(first level no wider than a rest work-group), 0 when every level is wider
+    uint32_t restShape[2][4]; // per rest list: planar contacts, their points, 3D contacts, their points
     uint32_t scanTicket[kScanSlots];
     uint32_t changed[kIslandRounds];
     uint32_t colorStart[TFS_COLORS + 1];
@@ -1692,7 +1693,8 @@ __global__ void k_rest_level(const uint32_t* list, uint32_t n, const tfs_contact
 // Contacts of the list grouped by level (any order within a level: its contacts commute), each entry
 // (contact, body a, body b, which bodies it writes: 1 a, 2 b).
 __global__ void k_rest_place(const uint32_t* list, uint32_t n, const tfs_contact* contacts, const uint8_t* moving,
-                             RestScratch z, const uint32_t* levelStart, uint4* entries) {
+                             RestScratch z, const uint32_t* levelStart, uint4* entries, uint32_t* shape) {
+    uint32_t sum[4] = { 0u, 0u, 0u, 0u };
     for (uint32_t k = gid(); k < n; k += gstride()) {
         const uint32_t lv = z.level[k];
         const uint32_t ci = list[k];
@@ -1701,7 +1703,11 @@ __global__ void k_rest_place(const uint32_t* list, uint32_t n, const tfs_contact
         const uint32_t writes = (rolling || moving[c.body_a] ? 1u : 0u) | (rolling || moving[c.body_b] ? 2u : 0u) |
                                 (moving[c.body_a] ? 0u : 4u) | (moving[c.body_b] ? 0u : 8u);
         entries[levelStart[lv] + atomicSub(&z.count[lv], 1u) - 1u] = make_uint4(ci, c.body_a, c.body_b, writes);
+        sum[c.planar ? 0 : 2] += 1u;
+        sum[c.planar ? 1 : 3] += (uint32_t)c.count;
     }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { waveAdd(&shape[q], sum[q]); }
 }
 
 // The active rest contacts, packed in the level order of the active rest list (entry k at position
@@ -2316,7 +2322,8 @@ struct tfs_ctx {
     };
     std::vector<Pending> pending;
     // families: 0 warm start, 1 solve, 2 restitution (event-timed launches); 3 waiting for the
-    // refresh's contacts to reach the host, 4 the refresh callback (host clock)
+    // refresh's contacts to reach the host, 4 the refresh callback (host clock); 5 the solve passes
+    // over the rest lists (event-timed, all launches of one pass over one list together)
     uint64_t profLaunches[kProfileFamilies] = {};
     double profMs[kProfileFamilies] = {}, profBytes[kProfileFamilies] = {};
     uint32_t lastCounts[8] = {};
@@ -2357,6 +2364,7 @@ struct tfs_ctx {
     uint32_t* restLevelStart[2] = {};
     uint32_t restLevels[2] = {};
     uint32_t restWide[2] = {};     // leading levels wider than a rest work-group
+    double restBytes[2] = {};      // algorithmic bytes of one solve pass over each rest list
     uint32_t *wrestDeg = nullptr, *wrestStart = nullptr, *wrestCursor = nullptr, *wrestEp = nullptr;  // rest warm start lists
     float4* restH = nullptr;        // the active rest contacts packed in level order (RestPack)
     float4* restP = nullptr;
@@ -2573,6 +2581,12 @@ double rowBytes(int mode, uint32_t rows, uint32_t points) {
     }
 }
 
+// Algorithmic bytes of one solve pass over rest contacts, from the same row model: a planar contact
+// with n points 248 + 68n (2D axes, planar inverse mass and inertia), a 3D one 224 + 296n.
+double restPassBytes(uint32_t planar, uint32_t planarPoints, uint32_t solid, uint32_t solidPoints) {
+    return 248.0 * planar + 68.0 * planarPoints + 224.0 * solid + 296.0 * solidPoints;
+}
+
 int runRows(tfs_ctx* ctx, int mode, const uint4* at, const uint32_t* starts, const uint32_t* points, const Ctx& c,
             const Tape& tape = Tape{}) {
     for (int col = 0; col < TFS_COLORS; ++col) {
@@ -2772,7 +2786,7 @@ int buildRestLevels(tfs_ctx* ctx, int which, const uint32_t* list, uint32_t n) {
         return r;
     }
     LAUNCH(k_rest_place, blocksFor(n), kThreads, list, n, ctx->contacts, ctx->moving, z, ctx->restLevelStart[which],
-           ctx->restEntries[which]);
+           ctx->restEntries[which], ctx->cnt->restShape[which]);
     LAUNCH(k_rest_narrow, blocksFor(n), kThreads, ctx->restLevelStart[which], &ctx->cnt->restLevels[which],
            (uint32_t)kRestBlockThreads, &ctx->cnt->restNarrow[which]);
     return 0;
@@ -2797,6 +2811,18 @@ int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32
     const float4* mass = ctx->mass;
     const RestPack rp = restPack(ctx);
     const bool viaPos = which == 1;
+    size_t ev = 0;
+    const bool timed = ctx->profiling && (mode == kBias || mode == kRelax);
+    if (timed) {
+        ev = ctx->pending.size() * 2;
+        while (ctx->evPool.size() < ev + 2) {
+            hipEvent_t x;
+            HIP_TRY(hipEventCreate(&x));
+            ctx->evPool.push_back(x);
+        }
+        ctx->pending.push_back({ 5, ev, ctx->restBytes[which] });
+        HIP_TRY(hipEventRecord(ctx->evPool[ev], ctx->stream));
+    }
     if (wide > 0) {
         const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((n + 63) / 64, ctx->maxPassBlocks));
         void* args[] = { (void*)&entries, (void*)&starts, (void*)&wide, (void*)&rp, (void*)&viaPos, &ctx->motion,
@@ -2816,6 +2842,7 @@ int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32
             default: LAUNCH((k_rest_levels<kRestitute, true>), 1, kRestBlockThreads, entries, from, left, rp, viaPos, ctx->motion, mass, c, bar, base, ctx->cnt); break;
         }
     }
+    if (timed) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
     return 0;
 }
 
@@ -2962,14 +2989,16 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
         if (needs) { return fail(ctx, TFS_E_CALLBACK, "fast contacts need a refresh callback"); }
     }
     ctx->restLevels[0] = ctx->restLevels[1] = ctx->restWide[0] = ctx->restWide[1] = 0;
+    ctx->restBytes[0] = ctx->restBytes[1] = 0.0;
     if (hc.nActive > laned || hc.nImpact > impactLaned) {
         if ((r = buildRestLevels(ctx, 0, ctx->rowsList + laned, hc.nActive - laned)) ||
             (r = buildRestLevels(ctx, 1, ctx->impactRowsList + impactLaned, hc.nImpact - impactLaned))) {
             return r;
         }
-        static_assert(offsetof(DevCounts, restNarrow) == offsetof(DevCounts, restLevels) + 2 * sizeof(uint32_t),
-                      "restLevels and restNarrow are read back together");
-        HIP_TRY(hipMemcpyAsync(ctx->hostCnt->restLevels, ctx->cnt->restLevels, 4 * sizeof(uint32_t),
+        static_assert(offsetof(DevCounts, restNarrow) == offsetof(DevCounts, restLevels) + 2 * sizeof(uint32_t) &&
+                          offsetof(DevCounts, restShape) == offsetof(DevCounts, restLevels) + 4 * sizeof(uint32_t),
+                      "restLevels, restNarrow and restShape are read back together");
+        HIP_TRY(hipMemcpyAsync(ctx->hostCnt->restLevels, ctx->cnt->restLevels, 12 * sizeof(uint32_t),
                                hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -2978,6 +3007,8 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
             ctx->restLevels[w] = ctx->hostCnt->restLevels[w];
             const uint32_t narrow = ~ctx->hostCnt->restNarrow[w];  // ~0 -> no narrow level
             ctx->restWide[w] = std::min(narrow, ctx->restLevels[w]);
+            const uint32_t* shape = ctx->hostCnt->restShape[w];
+            ctx->restBytes[w] = restPassBytes(shape[0], shape[1], shape[2], shape[3]);
         }
         if (hc.nActive > laned) {
             const uint32_t nRest = hc.nActive - laned;
