@@ -175,6 +175,25 @@ class SolveSteps:
         return spent
 
 
+def tune(solver, args, k):
+    """Starts the pass-shape autotuning at the settle step that lets it end with the settle (tuning
+    candidates x rounds steps; about 36 at most)."""
+    if args.tune_rounds > 0 and k == max(0, args.settle - 18 * args.tune_rounds):
+        solver.autotune(args.tune_rounds)
+
+
+def finish_tuning(solver, step):
+    """Steps on until tuning is done (when the settle was shorter than it; at most 200 more steps),
+    then the shape kept."""
+    for _ in range(200):
+        if solver.pass_shape()["tuning_steps_left"] == 0:
+            break
+        step()
+    solver.autotune(0)
+    x = solver.pass_shape()
+    return {"blocks": x["blocks"], "tail_rows": x["tail_rows"], "made_axes": x["made_axes"]}
+
+
 def run_batched(args):
     """Batched worlds: `--worlds` independent piles of log-uniform [8, 1024] bodies (the pile recipe
     per world, its own friction and restitution), solved together in one resident set
@@ -188,9 +207,11 @@ def run_batched(args):
         f"{args.settle} steps")
     step = SolveSteps(sim)
     for k in range(args.settle):
+        tune(sim.solver, args, k)
         step()
         if (k + 1) % 10 == 0:
             log(f"settle step {k + 1}: {len(sim.contacts)} contacts, {time.perf_counter() - t0:.1f} s")
+    shape = finish_tuning(sim.solver, step)
     for _ in range(args.warmup):
         step()
     wall = 0.0
@@ -245,7 +266,7 @@ def run_batched(args):
                         f"+ upload (untimed), then one solve block + sleep pass over all worlds (timed)",
             "worlds": len(worlds), "bodies": len(sim.bodies), "contacts_per_step": contacts / args.steps,
             "looping_world_steps_per_s": looping, "looping_sample_worlds": len(sample),
-            "batched_over_looping": batched / looping,
+            "batched_over_looping": batched / looping, "pass_shape": shape,
         },
     }
 
@@ -376,6 +397,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=10, help="event-timed steps for the roofline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline leg")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune-rounds", type=int, default=2,
+                    help="rounds of pass-shape autotuning over the last settle steps (0: the default shape)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB: the reference reading for scaling "
                          "the FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run")
@@ -415,11 +438,13 @@ def main():
     sim = Simulation(scene)
     log(f"{scene.name}: {len(sim.bodies)} bodies built in {time.perf_counter() - t0:.1f} s; settling {args.settle} steps")
     for k in range(args.settle):
+        tune(sim.solver, args, k)
         sim.step()
         if (k + 1) % 20 == 0:
             c = sim.solver.counts()
             log(f"settle step {k + 1}: {len(sim.contacts)} contacts, {c['points']} points, {c['colors']} colours, "
                 f"{time.perf_counter() - t0:.1f} s")
+    shape = finish_tuning(sim.solver, sim.step)
     bodies0 = sim.solver.bodies()
     substeps = int(sim.settings["substeps"])
     solve_step = SolveSteps(sim)
@@ -478,7 +503,7 @@ def main():
             "rest_levels": counts["rest_levels"], "impact_rest_levels": counts["impact_rest_levels"],
             "world_steps_per_s": args.steps / wall, "gpu_solve_ms_per_step": solve_ms / args.steps,
             "transfer_ms_per_step": 1e3 * transfers / args.steps,
-            "value_with_transfers": its / (wall + transfers),
+            "value_with_transfers": its / (wall + transfers), "pass_shape": shape,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

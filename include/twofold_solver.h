@@ -244,6 +244,25 @@ int tfs_last_step_ms(tfs_ctx* ctx, float* solve_ms, float* sleep_ms);
 int tfs_set_profiling(tfs_ctx* ctx, int on);
 int tfs_read_profile(tfs_ctx* ctx, int family, uint64_t* launches, double* ms, double* bytes);
 
+/*
+ * Pass-shape autotuning (no counterpart in the reference). The coloured passes run as one
+ * cooperative launch each; its shape is the number of blocks (one per compute unit at most; more
+ * blocks spread a colour's rows wider but make each grid barrier dearer), the widest colour that the
+ * one-block tail takes at the end of a pass (0: no tail), and whether each row loads its packed
+ * axes or makes them from the bodies' inverse masses. Results are the same, bit for bit, for every
+ * shape. tfs_autotune(ctx, rounds) makes the next rounds * C calls of tfs_step (C candidate shapes:
+ * 64, 128 and 256 blocks as far as they stay resident, tail off / 64 / 128 rows, axes loaded / made)
+ * each run one candidate, in rotating order, and then keeps the shape whose steps spent the least
+ * time in their solve passes (coloured and rest, event-timed; median over its rounds); a step with
+ * no solve pass does not count. rounds = 0 stops tuning and keeps the current shape. The fixed-contact
+ * model's steps run the current shape and do not count. tfs_pass_shape reads shape[5]: blocks, tail rows (0: off), made axes (0/1), steps
+ * of tuning left, candidates (0 when not tuning). tfs_set_pass_shape sets a shape and stops tuning
+ * (TFS_E_ARG for blocks outside 1..the resident limit or tail_rows above 256).
+ */
+int tfs_autotune(tfs_ctx* ctx, uint32_t rounds);
+int tfs_pass_shape(tfs_ctx* ctx, uint32_t shape[5]);
+int tfs_set_pass_shape(tfs_ctx* ctx, uint32_t blocks, uint32_t tail_rows, int made_axes);
+
 /* Counts of the last step: active contacts, their points, impact contacts and their points,
  * colours used, rest contacts, and the dependency levels of the active and of the impact rest list
  * (the rest is solved level after level). Constraint-iterations per step follow from these. */

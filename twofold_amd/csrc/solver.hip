@@ -2386,9 +2386,24 @@ struct tfs_ctx {
     BarrierLines* bar = nullptr;   // grid barrier counters
     BarrierBase barrierBase{};
     uint32_t maxPassBlocks = 0;
+    uint32_t passBlockLimit = 0;    // blocks of a cooperative pass that are resident at once (half the occupancy)
     bool usePass = true;
     bool passTail = true;
+    uint32_t tailSegRows = kTailSegRows;   // widest colour the tail takes
     bool madeAxes = false;
+    // Autotuning of the pass shape (grid blocks, tail colour width, axes loaded or made): while on,
+    // tuning step s runs candidate (s + s / C) % C of the C shapes, and after `tuneRounds` rounds the
+    // shape whose steps spent the least time in their solve passes (median) is kept. A step without
+    // a pass does not count. Every shape gives the same results.
+    struct PassShape {
+        uint32_t blocks, tailRows;
+        bool made;
+    };
+    std::vector<PassShape> tuneShapes;
+    std::vector<std::vector<float>> tuneMs;
+    uint32_t tuneStep = 0, tuneRounds = 0;
+    float tunePassMs = 0.0f;        // this step's solve passes (coloured and rest), event-timed
+    bool tuning = false;
     // per-body lists of coloured rows for the warm start
     uint32_t* bdeg = nullptr;
     uint32_t* bstart = nullptr;
@@ -2650,7 +2665,7 @@ int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool w
     uint32_t tailFrom = plan.n, tailRows = 0;
     while (ctx->passTail && tailFrom > 0) {
         const uint32_t w = plan.end[tailFrom - 1] - plan.begin[tailFrom - 1];
-        if (w > kTailSegRows || tailRows + w > (uint32_t)kPassThreads) { break; }
+        if (w > ctx->tailSegRows || tailRows + w > (uint32_t)kPassThreads) { break; }
         tailRows += w;
         --tailFrom;
     }
@@ -2661,7 +2676,8 @@ int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool w
     const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((widest + 63) / 64, ctx->maxPassBlocks));
     const uint32_t barriers = tailFrom == plan.n ? plan.n - 1 : tailFrom;
     size_t ev = 0;
-    if (ctx->profiling) {
+    const bool timed = ctx->profiling || (ctx->tuning && !ctx->fixedOn);
+    if (timed) {
         ev = ctx->pending.size() * 2;
         while (ctx->evPool.size() < ev + 2) {
             hipEvent_t x;
@@ -2684,7 +2700,7 @@ int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool w
                                           : (rec ? (const void*)k_pass<kRelax, false, true> : (const void*)k_pass<kRelax, false, false>));
     HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kPassThreads), args, 0, ctx->stream));
     advanceBarrier(ctx->barrierBase, grid, barriers);
-    if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
+    if (timed) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
     return 0;
 }
 
@@ -2758,6 +2774,8 @@ int collectProfile(tfs_ctx* ctx) {
     for (const auto& p : ctx->pending) {
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, ctx->evPool[p.ev], ctx->evPool[p.ev + 1]));
+        if (ctx->tuning && (p.family == 1 || p.family == 5)) { ctx->tunePassMs += ms; }
+        if (!ctx->profiling) { continue; }
         ctx->profLaunches[p.family] += 1;
         ctx->profMs[p.family] += ms;
         ctx->profBytes[p.family] += p.bytes;
@@ -2812,7 +2830,7 @@ int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32
     const RestPack rp = restPack(ctx);
     const bool viaPos = which == 1;
     size_t ev = 0;
-    const bool timed = ctx->profiling && (mode == kBias || mode == kRelax);
+    const bool timed = (ctx->profiling || (ctx->tuning && !ctx->fixedOn)) && (mode == kBias || mode == kRelax);
     if (timed) {
         ev = ctx->pending.size() * 2;
         while (ctx->evPool.size() < ev + 2) {
@@ -2973,7 +2991,40 @@ int beginStep(tfs_ctx* ctx, float dt, DevCounts& hc) {
     return packRows(ctx, hc.colorStart[TFS_COLORS], hc.impactColorStart[TFS_COLORS], true);
 }
 
+void applyShape(tfs_ctx* ctx, const tfs_ctx::PassShape& x) {
+    ctx->maxPassBlocks = std::max<uint32_t>(1, std::min(x.blocks, ctx->passBlockLimit));
+    ctx->passTail = x.tailRows > 0;
+    ctx->tailSegRows = x.tailRows > 0 ? x.tailRows : kTailSegRows;
+    ctx->madeAxes = x.made;
+}
+
+// The candidate a tuning step runs, and the bookkeeping after it.
+uint32_t tuneCandidate(const tfs_ctx* ctx) {
+    const uint32_t c = (uint32_t)ctx->tuneShapes.size();
+    return (ctx->tuneStep + ctx->tuneStep / c) % c;
+}
+
+void tuneRecord(tfs_ctx* ctx, float ms) {
+    const uint32_t c = (uint32_t)ctx->tuneShapes.size();
+    ctx->tuneMs[tuneCandidate(ctx)].push_back(ms);
+    if (++ctx->tuneStep < ctx->tuneRounds * c) { return; }
+    uint32_t best = 0;
+    float bestMs = 0.0f;
+    for (uint32_t k = 0; k < c; ++k) {
+        std::vector<float> t = ctx->tuneMs[k];
+        std::sort(t.begin(), t.end());
+        const float median = t[t.size() / 2];
+        if (k == 0 || median < bestMs) {
+            best = k;
+            bestMs = median;
+        }
+    }
+    applyShape(ctx, ctx->tuneShapes[best]);
+    ctx->tuning = false;
+}
+
 int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
+    if (ctx->tuning) { applyShape(ctx, ctx->tuneShapes[tuneCandidate(ctx)]); }
     const uint32_t nb = ctx->nb, nc = ctx->nc;
     const tfs_settings& s = ctx->settings;
     const Ctx c = makeCtx(s, dt);
@@ -3095,6 +3146,10 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     HIP_TRY(hipEventElapsedTime(&ctx->solveMs, ctx->ev[0], ctx->ev[1]));
     HIP_TRY(hipEventElapsedTime(&ctx->sleepMs, ctx->ev[1], ctx->ev[2]));
     if ((r = collectProfile(ctx))) { return r; }
+    if (ctx->tuning) {
+        if (ctx->tunePassMs > 0.0f) { tuneRecord(ctx, ctx->tunePassMs); }
+        ctx->tunePassMs = 0.0f;
+    }
     uint32_t nColors = 0;
     for (int col = 0; col < TFS_COLORS; ++col) {
         if (hc.colorStart[col + 1] > hc.colorStart[col]) { nColors = (uint32_t)col + 1; }
@@ -3330,7 +3385,8 @@ int tfs_create(int device, uint32_t maxBodies, uint32_t maxContacts, tfs_ctx** o
         // inside that). The barrier's cost grows with the block count (about a microsecond at 64
         // blocks, 4 at 256), so the pass uses at most 128 blocks, one row per thread up to 32k rows
         // a colour.
-        ctx->maxPassBlocks = (uint32_t)std::max(1, std::min(perCu * cus / 2, 128));
+        ctx->passBlockLimit = (uint32_t)std::max(1, perCu * cus / 2);
+        ctx->maxPassBlocks = std::min<uint32_t>(ctx->passBlockLimit, 128u);
         ctx->usePass = coop != 0 && perCu > 0;
         // Tuning knobs, read once: TWOFOLD_PASS=launch|coop, TWOFOLD_PASS_BLOCKS=n, TWOFOLD_AXES=packed|made,
         // TWOFOLD_TAIL=on|off.
@@ -3471,6 +3527,46 @@ int tfs_step(tfs_ctx* ctx, float dt, tfs_refresh_fn refresh, void* user) {
     if (hipSetDevice(ctx->device) != hipSuccess) { return fail(ctx, TFS_E_DEVICE, "hipSetDevice failed"); }
     ctx->fixedOn = false;
     return step(ctx, dt, refresh, user);
+}
+
+int tfs_autotune(tfs_ctx* ctx, uint32_t rounds) {
+    if (!ctx) { return TFS_E_ARG; }
+    ctx->tuning = false;
+    if (rounds == 0) { return 0; }
+    ctx->tuneShapes.clear();
+    for (uint32_t blocks : { 64u, 128u, 256u }) {
+        if (blocks > ctx->passBlockLimit) { continue; }
+        for (uint32_t tail : { 0u, 64u, 128u }) {
+            for (bool made : { false, true }) { ctx->tuneShapes.push_back(tfs_ctx::PassShape{ blocks, tail, made }); }
+        }
+    }
+    if (ctx->tuneShapes.empty()) { ctx->tuneShapes.push_back(tfs_ctx::PassShape{ ctx->passBlockLimit, 64u, false }); }
+    ctx->tuneMs.assign(ctx->tuneShapes.size(), std::vector<float>());
+    ctx->tuneStep = 0;
+    ctx->tuneRounds = rounds;
+    ctx->tuning = true;
+    return 0;
+}
+
+int tfs_pass_shape(tfs_ctx* ctx, uint32_t shape[5]) {
+    if (!ctx || !shape) { return TFS_E_ARG; }
+    shape[0] = ctx->maxPassBlocks;
+    shape[1] = ctx->passTail ? ctx->tailSegRows : 0u;
+    shape[2] = ctx->madeAxes ? 1u : 0u;
+    shape[3] = ctx->tuning ? (uint32_t)(ctx->tuneRounds * ctx->tuneShapes.size() - ctx->tuneStep) : 0u;
+    shape[4] = ctx->tuning ? (uint32_t)ctx->tuneShapes.size() : 0u;
+    return 0;
+}
+
+int tfs_set_pass_shape(tfs_ctx* ctx, uint32_t blocks, uint32_t tail_rows, int made_axes) {
+    if (!ctx) { return TFS_E_ARG; }
+    if (blocks == 0 || blocks > ctx->passBlockLimit) {
+        return fail(ctx, TFS_E_ARG, "blocks must be in 1..%u (the blocks resident at once)", ctx->passBlockLimit);
+    }
+    if (tail_rows > (uint32_t)kPassThreads) { return fail(ctx, TFS_E_ARG, "tail_rows must be at most %d", kPassThreads); }
+    ctx->tuning = false;
+    applyShape(ctx, tfs_ctx::PassShape{ blocks, tail_rows, made_axes != 0 });
+    return 0;
 }
 
 int tfs_fixed_begin(tfs_ctx* ctx, float dt, uint32_t max_steps) {

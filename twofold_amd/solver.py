@@ -24,7 +24,8 @@ EXPORTS = ("tfs_abi_version", "tfs_sizeof_body", "tfs_sizeof_contact", "tfs_size
            "tfs_create", "tfs_destroy", "tfs_last_error", "tfs_set_settings", "tfs_upload_bodies",
            "tfs_upload_contacts", "tfs_set_worlds", "tfs_step", "tfs_download_bodies", "tfs_download_contacts", "tfs_download_list",
            "tfs_last_step_ms", "tfs_last_step_counts", "tfs_set_profiling", "tfs_read_profile", "tfs_fixed_begin",
-           "tfs_fixed_step", "tfs_adjoint_seed", "tfs_adjoint_step", "tfs_adjoint_read", "tfs_check_minmax")
+           "tfs_fixed_step", "tfs_adjoint_seed", "tfs_adjoint_step", "tfs_adjoint_read", "tfs_check_minmax",
+           "tfs_autotune", "tfs_pass_shape", "tfs_set_pass_shape")
 
 
 class SolverError(RuntimeError):
@@ -67,6 +68,9 @@ def load_library(path=LIB_PATH):
     lib.tfs_adjoint_step.argtypes = [_vp]
     lib.tfs_adjoint_read.argtypes = [_vp, _vp, _vp, _vp]
     lib.tfs_check_minmax.argtypes = [ctypes.c_int, _vp, _vp, ctypes.c_uint32, _vp]
+    lib.tfs_autotune.argtypes = [_vp, ctypes.c_uint32]
+    lib.tfs_pass_shape.argtypes = [_vp, _vp]
+    lib.tfs_set_pass_shape.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
     lib.tfs_read_profile.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_double)]
     if lib.tfs_abi_version() != 1:
@@ -230,6 +234,23 @@ class Solver:
         self._check(self.lib.tfs_read_profile(self.h, family, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by)),
                     "tfs_read_profile")
         return n.value, ms.value, by.value
+
+    # ---- pass shape and its autotuning ----
+
+    def autotune(self, rounds):
+        """Tunes the pass shape over the next rounds x candidates steps (0 stops tuning)."""
+        self._check(self.lib.tfs_autotune(self.h, rounds), "tfs_autotune")
+
+    def pass_shape(self):
+        """The pass shape in use: dict of blocks, tail_rows (0 = no tail), made_axes, tuning_steps_left,
+        candidates (0 when not tuning)."""
+        out = np.zeros(5, np.uint32)
+        self._check(self.lib.tfs_pass_shape(self.h, _ptr(out)), "tfs_pass_shape")
+        return dict(blocks=int(out[0]), tail_rows=int(out[1]), made_axes=bool(out[2]), tuning_steps_left=int(out[3]),
+                    candidates=int(out[4]))
+
+    def set_pass_shape(self, blocks, tail_rows, made_axes):
+        self._check(self.lib.tfs_set_pass_shape(self.h, blocks, tail_rows, int(bool(made_axes))), "tfs_set_pass_shape")
 
     # ---- the fixed-contact model and its adjoint ----
 
