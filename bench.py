@@ -98,6 +98,35 @@ def cpu_reference(scene, bodies, substeps, budget_s):
                        f"one thread, CPU: {cpu_model()}", ms_per_step=1e3 * spent / steps)
 
 
+def cpu_reference_worlds(worlds, sample, states, timed_steps=4):
+    """The reference engine (oracle/_ref) stepping each sampled world alone on one thread, from the
+    batch's state of that world (2 untimed steps, then `timed_steps` timed by its own
+    StepProfile::solver): world-steps/s with every world weighted alike, as the batch steps them."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import refworld
+
+    if not refworld.ref_available():
+        return None
+    _one_core()
+    per_world = []
+    for k in sample:
+        w = refworld.RefWorld(worlds[k])
+        w.set_bodies(states[k])
+        for _ in range(2):
+            w.step(record=False)
+        t = 0.0
+        for _ in range(timed_steps):
+            w.step(record=False)
+            t += w.profile()["solver"] * 1e-3
+        per_world.append(t / timed_steps)
+    mean = sum(per_world) / len(per_world)
+    return dict(value=1.0 / mean, unit="world-steps/s", cores=1, kind="reference",
+                sample=f"{len(sample)} of the worlds, each alone from the batch's state of it, {timed_steps} steps each "
+                       f"of the reference (its own narrowphase), solve block timed by StepProfile::solver: "
+                       f"{1e3 * sum(per_world) * timed_steps:.0f} ms of solve, one thread, CPU: {cpu_model()}",
+                ms_per_world_step=1e3 * mean)
+
+
 def cpu_port(sim, bodies, contacts, budget_s):
     """The oracle (oracle/liboracle.so, a single-threaded CPU restatement of the reference solver, pinned
     bit-exact to it) timed on the same input, with the same refresh callback run on one thread."""
@@ -221,6 +250,13 @@ def run_batched(args):
         contacts += len(sim.contacts)
     batched = len(worlds) * args.steps / wall
     log(f"batched: {args.steps} steps in {wall:.3f} s of solve, {batched:.4g} world-steps/s")
+    sim.solver.set_profiling(True)
+    for _ in range(args.batch_profile_steps):
+        step()
+    sim.solver.set_profiling(False)
+    launches, ms, algo = sim.solver.profile(1)
+    achieved = algo / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    log(f"solve passes: {launches} launches, {ms:.2f} ms, {algo / max(launches, 1) / 1e6:.2f} MB per launch")
 
     # looping: the plain path on each sampled world alone, from the batch's current state
     rng = scenes.Rng(0x1005)
@@ -247,6 +283,12 @@ def run_batched(args):
         solver.close()
     looping = 1.0 / (sum(per_world) / len(per_world))
     log(f"looping the plain path over {len(sample)} worlds: {1e3 * sum(per_world) / len(per_world):.3f} ms per world-step")
+    cpu = None
+    if not args.no_cpu:
+        states = {k: np.ascontiguousarray(sim.bodies[sim.world.body_start[k]:sim.world.body_start[k + 1]]) for k in sample}
+        cpu = cpu_reference_worlds(worlds, sample, states)
+        if cpu is not None:
+            log(f"cpu baseline (reference): {cpu['value']:.4g} world-steps/s")
     return {
         "metric": "batched world-steps/s",
         "value": batched,
@@ -268,6 +310,14 @@ def run_batched(args):
             "looping_world_steps_per_s": looping, "looping_sample_worlds": len(sample),
             "batched_over_looping": batched / looping, "pass_shape": shape,
         },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None, "traffic_source": None,
+            "kernel": "k_pass<bias|relax> (one cooperative launch per solve pass over all worlds' rows)",
+            "launches": launches, "profiled_steps": args.batch_profile_steps,
+            "bytes_per_launch": algo / max(launches, 1), "ms_per_launch": ms / max(launches, 1),
+        },
+        "cpu_baseline": cpu,
     }
 
 
@@ -395,6 +445,7 @@ def main():
                     help="steps of drop-and-settle before the warm-up (pile 240, batched 60)")
     ap.add_argument("--side", type=int, default=32, help="bodies per side of the grid (32 -> 32,768)")
     ap.add_argument("--profile-steps", type=int, default=10, help="event-timed steps for the roofline")
+    ap.add_argument("--batch-profile-steps", type=int, default=3, help="batched: event-timed steps for the roofline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune-rounds", type=int, default=2,
@@ -493,8 +544,9 @@ def main():
         "dtype": "f32",
         "data": f"synthetic: seeded {scene.name} recipe, stepped on the GPU with the host narrowphase",
         "config": {
-            "workload": (f"{len(bodies0)} bodies: 65,536 planar boxes locked to the xy plane and 4,096 free spheres in "
-                         f"a U container (after {args.settle} drop steps)" if planar else
+            "workload": (f"{len(bodies0)} bodies: 65,536 bodies locked to the xy plane (40% boxes, 40% prisms of 3-8 "
+                         f"sides, 20% cylinders) and 4,096 free spheres in a U container (after {args.settle} drop "
+                         f"steps)" if planar else
                          f"settled {len(bodies0)}-body box/sphere pile in walls (after {args.settle} drop steps)")
                         + f", dt 1/60, {substeps} substeps; per step: host narrowphase + upload (untimed), then one "
                         f"solve block + sleep pass on the GPU (timed)",

@@ -1,6 +1,9 @@
-// scene.cpp - a host world that drives the solver end to end: dynamic and static boxes and spheres
-// and static half-spaces, a uniform-grid broadphase and a narrowphase that hands the solver contact
-// manifolds in its input form (include/twofold_solver.h).
+// scene.cpp - a host world that drives the solver end to end: dynamic and static boxes, spheres,
+// cylinders and prisms (convex polygons extruded along local z) and static half-spaces, a
+// uniform-grid broadphase and a narrowphase that hands the solver contact manifolds in its input form
+// (include/twofold_solver.h). Cylinders and prisms collide with half-spaces, spheres, and with other
+// shapes extruded along the world z when both bodies stay so (locked to the xy plane, or static): the
+// last as a 2D problem in the xy plane over the shared z range.
 //
 // It stands in for the reference's broadphase and narrowphase (src/world.cpp:752-993,
 // src/collide3d.cpp), which sit outside the solve path, so that the solver can be run on the GPU
@@ -9,7 +12,8 @@
 // surfaces, a speculative margin grown by the pair's travel (src/world.cpp:484-486, 509-511),
 // anchors lifted onto each surface for separated points (src/world.cpp:536-540), impulses carried
 // to the nearest new point (src/world.cpp:102-120), friction and restitution mixed as
-// src/world.cpp:39-40 - but not its bits: box-box uses separating axes and face clipping of its own.
+// src/world.cpp:39-40 - but not its bits: box-box uses separating axes and face clipping of its own,
+// and pairs of z-extruded shapes a 2D version of the same.
 #include <omp.h>
 
 #include <algorithm>
@@ -57,12 +61,15 @@ V perpendicular(V n) {
     return unit(V{ 0.0f, n.z, -n.y });
 }
 
-enum Kind { kSphere = 0, kBox = 1, kHalfSpace = 4 };
+enum Kind { kSphere = 0, kBox = 1, kCylinder = 2, kPrism = 3, kHalfSpace = 4 };
+constexpr int kMaxPrismVertices = 16;
 
 struct Shape {
     int kind = kBox;
-    V half{ 0.5f, 0.5f, 0.5f };
-    float radius = 0.5f;
+    V half{ 0.5f, 0.5f, 0.5f };   // a box's half-extents; the local bounds of a cylinder or prism
+    float radius = 0.5f;           // sphere, cylinder (whose axis is local z, from -half.z to half.z)
+    float zMin = -0.5f, zMax = 0.5f;
+    std::vector<float> poly;       // prism: x0, y0, x1, y1, ... counter-clockwise in the local xy plane
     float density = 1.0f, friction = 0.5f, restitution = 0.0f, rolling = 0.0f;
 };
 
@@ -150,7 +157,7 @@ void sphereSphere(V ca, float ra, V cb, float rb, float margin, Manifold& m) {
     m.count = 1;
 }
 
-// Half-space A (below its local xy plane) against a sphere or box B.
+// Half-space A (below its local xy plane) against a sphere, box, prism or cylinder B.
 void halfSpaceVs(const Pose& a, const Shape& sb, const Pose& b, float margin, Manifold& m) {
     const V n = a.ax[2];
     const float offset = dot(n, a.c);
@@ -164,10 +171,35 @@ void halfSpaceVs(const Pose& a, const Shape& sb, const Pose& b, float margin, Ma
         m.count = 1;
         return;
     }
+    std::vector<V> verts;
+    if (sb.kind == kBox) {
+        for (int k = 0; k < 8; ++k) {
+            verts.push_back(b.c + b.ax[0] * ((k & 1) ? sb.half.x : -sb.half.x) +
+                            b.ax[1] * ((k & 2) ? sb.half.y : -sb.half.y) + b.ax[2] * ((k & 4) ? sb.half.z : -sb.half.z));
+        }
+    } else if (sb.kind == kPrism) {
+        for (size_t i = 0; i + 1 < sb.poly.size(); i += 2) {
+            const V xy = b.c + b.ax[0] * sb.poly[i] + b.ax[1] * sb.poly[i + 1];
+            verts.push_back(xy + b.ax[2] * sb.zMin);
+            verts.push_back(xy + b.ax[2] * sb.zMax);
+        }
+    } else if (sb.kind == kCylinder) {
+        // each cap rim's lowest point along n, or four rim points of a cap lying flat on the plane
+        for (float side : { -1.0f, 1.0f }) {
+            const V cap = b.c + b.ax[2] * (side * sb.half.z);
+            const V down = -n - b.ax[2] * dot(-n, b.ax[2]);
+            const float dl = len(down);
+            if (dl > 1e-4f) {
+                verts.push_back(cap + down * (sb.radius / dl));
+            } else {
+                for (int k = 0; k < 4; ++k) {
+                    verts.push_back(cap + b.ax[k & 1] * ((k & 2) ? -sb.radius : sb.radius));
+                }
+            }
+        }
+    }
     std::vector<Point> pts;
-    for (int k = 0; k < 8; ++k) {
-        const V v = b.c + b.ax[0] * ((k & 1) ? sb.half.x : -sb.half.x) + b.ax[1] * ((k & 2) ? sb.half.y : -sb.half.y) +
-                    b.ax[2] * ((k & 4) ? sb.half.z : -sb.half.z);
+    for (const V& v : verts) {
         const float sep = dot(n, v) - offset;
         if (sep <= margin) { pts.push_back({ v - n * (0.5f * sep), sep }); }
     }
@@ -327,6 +359,340 @@ void boxBox(const Shape& sa, const Pose& a, const Shape& sb, const Pose& b, floa
     reduce(pts, faceN, m);
 }
 
+// ---- z-extruded shapes as a 2D problem ----------------------------------------------------------
+
+// A shape's cross-section in the world xy plane and its z range, for a shape extruded along the
+// world z: a circle or a counter-clockwise convex polygon.
+struct Profile {
+    bool circle = false;
+    float cx = 0, cy = 0, r = 0;
+    int n = 0;
+    float x[kMaxPrismVertices], y[kMaxPrismVertices];
+    float zlo = 0, zhi = 0;
+};
+
+bool profileOf(const Shape& s, const Pose& p, Profile& f) {
+    const float up = p.ax[2].z;
+    if (std::fabs(up) < 0.9999f) { return false; }
+    if (s.kind == kCylinder) {
+        f.circle = true;
+        f.cx = p.c.x;
+        f.cy = p.c.y;
+        f.r = s.radius;
+        f.zlo = p.c.z - s.half.z;
+        f.zhi = p.c.z + s.half.z;
+        return true;
+    }
+    float lx[kMaxPrismVertices], ly[kMaxPrismVertices], z0, z1;
+    int n;
+    if (s.kind == kBox) {
+        n = 4;
+        const float hx = s.half.x, hy = s.half.y;
+        const float cx[4] = { hx, -hx, -hx, hx }, cy[4] = { hy, hy, -hy, -hy };
+        for (int i = 0; i < 4; ++i) {
+            lx[i] = cx[i];
+            ly[i] = cy[i];
+        }
+        z0 = -s.half.z;
+        z1 = s.half.z;
+    } else if (s.kind == kPrism) {
+        n = (int)s.poly.size() / 2;
+        for (int i = 0; i < n; ++i) {
+            lx[i] = s.poly[2 * i];
+            ly[i] = s.poly[2 * i + 1];
+        }
+        z0 = s.zMin;
+        z1 = s.zMax;
+    } else {
+        return false;
+    }
+    f.circle = false;
+    f.n = n;
+    for (int i = 0; i < n; ++i) {
+        const V w = p.c + p.ax[0] * lx[i] + p.ax[1] * ly[i];
+        const int k = up > 0 ? i : n - 1 - i;  // a shape turned over mirrors its winding
+        f.x[k] = w.x;
+        f.y[k] = w.y;
+    }
+    f.zlo = up > 0 ? p.c.z + z0 : p.c.z - z1;
+    f.zhi = up > 0 ? p.c.z + z1 : p.c.z - z0;
+    return true;
+}
+
+// A body that stays extruded along the world z: static, or locked to a plane whose normal is z
+// (a free body that happens to stand upright is not: it may meet others along z).
+bool inPlane(const tfs_body& b) {
+    if (b.type == TFS_STATIC) { return true; }
+    if ((b.locks & TFS_LOCK_PLANAR) != TFS_LOCK_PLANAR) { return false; }
+    return std::fabs(rotate(ldq(b.lock_frame), V{ 0, 0, 1 }).z) > 0.9999f;
+}
+
+// A 2D manifold: normal from A to B, up to two points (midpoints) with their separations.
+struct Manifold2 {
+    float nx = 0, ny = 0;
+    int count = 0;
+    float px[2], py[2], sep[2];
+};
+
+void edgeNormal(const Profile& f, int i, float& nx, float& ny) {
+    const int j = (i + 1) % f.n;
+    const float ex = f.x[j] - f.x[i], ey = f.y[j] - f.y[i];
+    const float l = std::sqrt(ex * ex + ey * ey);
+    nx = l > 0 ? ey / l : 0.0f;   // outward for counter-clockwise order
+    ny = l > 0 ? -ex / l : 0.0f;
+}
+
+// The largest separation of Q from P's edges, and that edge.
+float edgeSeparation(const Profile& P, const Profile& Q, int& best) {
+    float most = -1e30f;
+    best = 0;
+    for (int i = 0; i < P.n; ++i) {
+        float nx, ny;
+        edgeNormal(P, i, nx, ny);
+        float least = 1e30f;
+        for (int j = 0; j < Q.n; ++j) { least = std::min(least, nx * (Q.x[j] - P.x[i]) + ny * (Q.y[j] - P.y[i])); }
+        if (least > most) {
+            most = least;
+            best = i;
+        }
+    }
+    return most;
+}
+
+void polygons2(const Profile& A, const Profile& B, float margin, Manifold2& m) {
+    m.count = 0;
+    int ea, eb;
+    const float sa = edgeSeparation(A, B, ea);
+    if (sa > margin) { return; }
+    const float sb = edgeSeparation(B, A, eb);
+    if (sb > margin) { return; }
+    const bool refB = sb > 0.98f * sa + 0.0005f;
+    const Profile& R = refB ? B : A;
+    const Profile& I = refB ? A : B;
+    const int k = refB ? eb : ea;
+    float nx, ny;
+    edgeNormal(R, k, nx, ny);
+    int j = 0;
+    float most = 1e30f;
+    for (int i = 0; i < I.n; ++i) {
+        float ix, iy;
+        edgeNormal(I, i, ix, iy);
+        const float d = ix * nx + iy * ny;
+        if (d < most) {
+            most = d;
+            j = i;
+        }
+    }
+    float sx[2] = { I.x[j], I.x[(j + 1) % I.n] }, sy[2] = { I.y[j], I.y[(j + 1) % I.n] };
+    const float r0x = R.x[k], r0y = R.y[k], r1x = R.x[(k + 1) % R.n], r1y = R.y[(k + 1) % R.n];
+    const float tx = -ny, ty = nx;  // along the reference edge, r0 -> r1
+    // clip the incident edge to the reference edge's extent
+    const float lo = tx * r0x + ty * r0y, hi = tx * r1x + ty * r1y;
+    for (int side = 0; side < 2; ++side) {
+        const float limit = side ? hi : lo;
+        const float dir = side ? 1.0f : -1.0f;  // keep dir * (t . p - limit) <= 0
+        const float d0 = dir * (tx * sx[0] + ty * sy[0] - limit), d1 = dir * (tx * sx[1] + ty * sy[1] - limit);
+        if (d0 > 0 && d1 > 0) { return; }
+        if (d0 > 0 || d1 > 0) {
+            const float u = d0 / (d0 - d1);
+            const float qx = sx[0] + (sx[1] - sx[0]) * u, qy = sy[0] + (sy[1] - sy[0]) * u;
+            const int out = d0 > 0 ? 0 : 1;
+            sx[out] = qx;
+            sy[out] = qy;
+        }
+    }
+    for (int i = 0; i < 2; ++i) {
+        const float sep = nx * (sx[i] - r0x) + ny * (sy[i] - r0y);
+        if (sep > margin) { continue; }
+        m.px[m.count] = sx[i] - nx * (0.5f * sep);
+        m.py[m.count] = sy[i] - ny * (0.5f * sep);
+        m.sep[m.count] = sep;
+        ++m.count;
+    }
+    m.nx = refB ? -nx : nx;
+    m.ny = refB ? -ny : ny;
+}
+
+// Circle C against polygon P; the normal points from P to C.
+void circlePolygon2(const Profile& C, const Profile& P, float margin, Manifold2& m) {
+    m.count = 0;
+    float best = -1e30f, bnx = 0, bny = 1;
+    bool outside = false;
+    for (int i = 0; i < P.n; ++i) {
+        float nx, ny;
+        edgeNormal(P, i, nx, ny);
+        const float s = nx * (C.cx - P.x[i]) + ny * (C.cy - P.y[i]);
+        if (s > 0) { outside = true; }
+        if (s > best) {
+            best = s;
+            bnx = nx;
+            bny = ny;
+        }
+    }
+    float qx, qy, nx, ny, sep;
+    if (!outside) {
+        nx = bnx;
+        ny = bny;
+        sep = best - C.r;
+        qx = C.cx - nx * best;
+        qy = C.cy - ny * best;
+    } else {
+        float dsq = 1e30f;
+        qx = P.x[0];
+        qy = P.y[0];
+        for (int i = 0; i < P.n; ++i) {
+            const int j = (i + 1) % P.n;
+            const float ex = P.x[j] - P.x[i], ey = P.y[j] - P.y[i];
+            const float ee = ex * ex + ey * ey;
+            float t = ee > 0 ? ((C.cx - P.x[i]) * ex + (C.cy - P.y[i]) * ey) / ee : 0.0f;
+            t = std::max(0.0f, std::min(1.0f, t));
+            const float x = P.x[i] + ex * t, y = P.y[i] + ey * t;
+            const float d = (C.cx - x) * (C.cx - x) + (C.cy - y) * (C.cy - y);
+            if (d < dsq) {
+                dsq = d;
+                qx = x;
+                qy = y;
+            }
+        }
+        const float dist = std::sqrt(dsq);
+        if (dist <= 0) { return; }
+        nx = (C.cx - qx) / dist;
+        ny = (C.cy - qy) / dist;
+        sep = dist - C.r;
+    }
+    if (sep > margin) { return; }
+    m.nx = nx;
+    m.ny = ny;
+    m.px[0] = 0.5f * (qx + C.cx - nx * C.r);
+    m.py[0] = 0.5f * (qy + C.cy - ny * C.r);
+    m.sep[0] = sep;
+    m.count = 1;
+}
+
+void circles2(const Profile& A, const Profile& B, float margin, Manifold2& m) {
+    m.count = 0;
+    const float dx = B.cx - A.cx, dy = B.cy - A.cy;
+    const float dist = std::sqrt(dx * dx + dy * dy);
+    const float sep = dist - A.r - B.r;
+    if (sep > margin) { return; }
+    m.nx = dist > 1e-9f ? dx / dist : 0.0f;
+    m.ny = dist > 1e-9f ? dy / dist : 1.0f;
+    m.px[0] = A.cx + m.nx * (A.r + 0.5f * sep);
+    m.py[0] = A.cy + m.ny * (A.r + 0.5f * sep);
+    m.sep[0] = sep;
+    m.count = 1;
+}
+
+// Two z-extruded shapes: the 2D contact of their cross-sections at both ends of their shared z range.
+void extruded(const Profile& A, const Profile& B, float margin, Manifold& m) {
+    m.count = 0;
+    const float zlo = std::max(A.zlo, B.zlo), zhi = std::min(A.zhi, B.zhi);
+    if (zlo > zhi) { return; }
+    Manifold2 q;
+    if (A.circle && B.circle) {
+        circles2(A, B, margin, q);
+    } else if (A.circle) {
+        circlePolygon2(A, B, margin, q);
+        q.nx = -q.nx;
+        q.ny = -q.ny;
+    } else if (B.circle) {
+        circlePolygon2(B, A, margin, q);
+    } else {
+        polygons2(A, B, margin, q);
+    }
+    m.n = V{ q.nx, q.ny, 0.0f };
+    const bool flat = zhi - zlo < 1e-6f;
+    for (int i = 0; i < q.count; ++i) {
+        m.p[m.count++] = { V{ q.px[i], q.py[i], zlo }, q.sep[i] };
+        if (!flat) { m.p[m.count++] = { V{ q.px[i], q.py[i], zhi }, q.sep[i] }; }
+    }
+}
+
+// A prism or cylinder's point nearest a sphere's centre (normal from the shape to the sphere).
+void roundOrPrismSphere(const Shape& s, const Pose& a, V cs, float r, float margin, Manifold& m) {
+    m.count = 0;
+    const V d = cs - a.c;
+    const V l{ dot(d, a.ax[0]), dot(d, a.ax[1]), dot(d, a.ax[2]) };
+    const float z0 = s.kind == kCylinder ? -s.half.z : s.zMin, z1 = s.kind == kCylinder ? s.half.z : s.zMax;
+    V q = l, nl{ 0, 0, 1 };
+    float sep;
+    bool inside2 = true;
+    float deepest = -1e30f, dnx = 0, dny = 0;  // the least deep side, when inside
+    if (s.kind == kCylinder) {
+        const float rho = std::sqrt(l.x * l.x + l.y * l.y);
+        if (rho > s.radius) {
+            inside2 = false;
+            q.x = l.x * (s.radius / rho);
+            q.y = l.y * (s.radius / rho);
+        }
+        deepest = rho - s.radius;
+        dnx = rho > 1e-9f ? l.x / rho : 1.0f;
+        dny = rho > 1e-9f ? l.y / rho : 0.0f;
+    } else {
+        const int n = (int)s.poly.size() / 2;
+        for (int i = 0; i < n; ++i) {
+            const int j = (i + 1) % n;
+            const float ex = s.poly[2 * j] - s.poly[2 * i], ey = s.poly[2 * j + 1] - s.poly[2 * i + 1];
+            const float el = std::sqrt(ex * ex + ey * ey);
+            const float nx = el > 0 ? ey / el : 0.0f, ny = el > 0 ? -ex / el : 0.0f;
+            const float sd = nx * (l.x - s.poly[2 * i]) + ny * (l.y - s.poly[2 * i + 1]);
+            if (sd > 0) { inside2 = false; }
+            if (sd > deepest) {
+                deepest = sd;
+                dnx = nx;
+                dny = ny;
+            }
+        }
+        if (!inside2) {
+            float dsq = 1e30f;
+            for (int i = 0; i < n; ++i) {
+                const int j = (i + 1) % n;
+                const float ax = s.poly[2 * i], ay = s.poly[2 * i + 1];
+                const float ex = s.poly[2 * j] - ax, ey = s.poly[2 * j + 1] - ay;
+                const float ee = ex * ex + ey * ey;
+                float t = ee > 0 ? ((l.x - ax) * ex + (l.y - ay) * ey) / ee : 0.0f;
+                t = std::max(0.0f, std::min(1.0f, t));
+                const float x = ax + ex * t, y = ay + ey * t;
+                const float dd = (l.x - x) * (l.x - x) + (l.y - y) * (l.y - y);
+                if (dd < dsq) {
+                    dsq = dd;
+                    q.x = x;
+                    q.y = y;
+                }
+            }
+        }
+    }
+    q.z = std::max(z0, std::min(z1, l.z));
+    const bool insideZ = l.z >= z0 && l.z <= z1;
+    if (inside2 && insideZ) {
+        // centre inside: out through the nearest side or cap
+        float best = deepest;
+        nl = V{ dnx, dny, 0.0f };
+        if (l.z - z1 > best) {
+            best = l.z - z1;
+            nl = V{ 0, 0, 1 };
+        }
+        if (z0 - l.z > best) {
+            best = z0 - l.z;
+            nl = V{ 0, 0, -1 };
+        }
+        sep = best - r;
+        q = l - nl * best;
+    } else {
+        const V off = l - q;
+        const float dist = len(off);
+        if (dist <= 1e-9f) { return; }
+        nl = off * (1.0f / dist);
+        sep = dist - r;
+    }
+    if (sep > margin) { return; }
+    const V n = a.ax[0] * nl.x + a.ax[1] * nl.y + a.ax[2] * nl.z;
+    const V pa = a.c + a.ax[0] * q.x + a.ax[1] * q.y + a.ax[2] * q.z;
+    m.n = n;
+    m.p[0] = { (pa + (cs - n * r)) * 0.5f, sep };
+    m.count = 1;
+}
+
 struct World {
     tfs_settings settings{};
     int threads = 1;  // host threads of the narrowphase loops
@@ -358,13 +724,26 @@ bool collide(const World& w, uint32_t ia, const tfs_body& a, uint32_t ib, const 
         return false;
     } else if (sa.kind == kSphere && sb.kind == kSphere) {
         sphereSphere(pa.c, sa.radius, pb.c, sb.radius, margin, m);
-    } else if (sa.kind == kBox && sb.kind == kSphere) {
-        boxSphere(sa, pa, pb.c, sb.radius, margin, m);
-    } else if (sa.kind == kSphere && sb.kind == kBox) {
-        boxSphere(sb, pb, pa.c, sa.radius, margin, m);
+    } else if (sb.kind == kSphere) {
+        if (sa.kind == kBox) {
+            boxSphere(sa, pa, pb.c, sb.radius, margin, m);
+        } else {
+            roundOrPrismSphere(sa, pa, pb.c, sb.radius, margin, m);
+        }
+    } else if (sa.kind == kSphere) {
+        if (sb.kind == kBox) {
+            boxSphere(sb, pb, pa.c, sa.radius, margin, m);
+        } else {
+            roundOrPrismSphere(sb, pb, pa.c, sa.radius, margin, m);
+        }
         m.n = -m.n;
     } else {
-        boxBox(sa, pa, sb, pb, margin, m);
+        Profile fa, fb;
+        if (inPlane(a) && inPlane(b) && profileOf(sa, pa, fa) && profileOf(sb, pb, fb)) {
+            extruded(fa, fb, margin, m);
+        } else if (sa.kind == kBox && sb.kind == kBox) {
+            boxBox(sa, pa, sb, pb, margin, m);
+        }  // a cylinder or prism against a shape not extruded along the world z: not supported
     }
     return m.count > 0;
 }
@@ -534,6 +913,33 @@ void massProperties(tfs_body& b, const Shape& s) {
         m = s.density * 4.0f / 3.0f * 3.14159265f * s.radius * s.radius * s.radius;
         I[0] = I[1] = I[2] = 0.4f * m * s.radius * s.radius;
         b.extent = s.radius;
+    } else if (s.kind == kCylinder) {
+        const float r2 = s.radius * s.radius, h = 2.0f * s.half.z;
+        m = s.density * 3.14159265f * r2 * h;
+        I[0] = I[1] = m * (3.0f * r2 + h * h) / 12.0f;
+        I[2] = 0.5f * m * r2;
+        b.extent = std::sqrt(r2 + s.half.z * s.half.z);
+    } else if (s.kind == kPrism) {
+        // polygon area and second moments (shoelace sums), about the local origin (the centroid)
+        const int n = (int)s.poly.size() / 2;
+        double area = 0, ixx = 0, iyy = 0, far = 0;
+        for (int i = 0; i < n; ++i) {
+            const int j = (i + 1) % n;
+            const double x0 = s.poly[2 * i], y0 = s.poly[2 * i + 1], x1 = s.poly[2 * j], y1 = s.poly[2 * j + 1];
+            const double c = x0 * y1 - x1 * y0;
+            area += 0.5 * c;
+            iyy += c * (x0 * x0 + x0 * x1 + x1 * x1) / 12.0;   // integral of x^2
+            ixx += c * (y0 * y0 + y0 * y1 + y1 * y1) / 12.0;   // integral of y^2
+            far = std::max(far, x0 * x0 + y0 * y0);
+        }
+        const double t = s.zMax - s.zMin, zc = 0.5 * (s.zMax + s.zMin);
+        const double mass = s.density * area * t;
+        const double zz = mass * (t * t / 12.0 + zc * zc);
+        m = (float)mass;
+        I[0] = (float)(s.density * t * ixx + zz);
+        I[1] = (float)(s.density * t * iyy + zz);
+        I[2] = (float)(s.density * t * (ixx + iyy));
+        b.extent = (float)std::sqrt(far + std::max(s.zMin * s.zMin, s.zMax * s.zMax));
     }
     std::memset(b.inertia, 0, sizeof b.inertia);
     if (b.type == TFS_DYNAMIC && m > 0.0f) {
@@ -612,7 +1018,9 @@ int tfw_set_group(void* h, uint32_t body, uint32_t group) {
     return 0;
 }
 
-// One shape per body, at its origin: 0 sphere {r}, 1 box {hx, hy, hz}, 4 half-space {}.
+// One shape per body, at its origin: 0 sphere {r}, 1 box {hx, hy, hz}, 2 cylinder {r, half height}
+// (axis local z), 3 prism {zMin, zMax, x0, y0, x1, y1, ...} (3 to 16 vertices of a convex polygon in
+// either winding, about the local origin, which should be its centroid), 4 half-space {}.
 int tfw_add_shape(void* h, int body, int kind, const float* dims, int ndims, const float* pos, const float* rot,
                   float density, float friction, float restitution, float rolling) {
     World& w = *(World*)h;
@@ -626,6 +1034,31 @@ int tfw_add_shape(void* h, int body, int kind, const float* dims, int ndims, con
         s.radius = dims[0];
     } else if (kind == kBox && ndims >= 3) {
         s.half = V{ dims[0], dims[1], dims[2] };
+    } else if (kind == kCylinder && ndims >= 2) {
+        s.radius = dims[0];
+        s.half = V{ dims[0], dims[0], dims[1] };
+    } else if (kind == kPrism && ndims >= 8 && ndims % 2 == 0 && ndims <= 2 + 2 * kMaxPrismVertices) {
+        s.zMin = dims[0];
+        s.zMax = dims[1];
+        s.poly.assign(dims + 2, dims + ndims);
+        const int n = (ndims - 2) / 2;
+        double twice = 0;
+        for (int i = 0; i < n; ++i) {
+            const int j = (i + 1) % n;
+            twice += (double)s.poly[2 * i] * s.poly[2 * j + 1] - (double)s.poly[2 * j] * s.poly[2 * i + 1];
+        }
+        if (twice < 0) {  // clockwise: turn it round
+            for (int i = 0, j = n - 1; i < j; ++i, --j) {
+                std::swap(s.poly[2 * i], s.poly[2 * j]);
+                std::swap(s.poly[2 * i + 1], s.poly[2 * j + 1]);
+            }
+        }
+        float hx = 0, hy = 0;
+        for (int i = 0; i < n; ++i) {
+            hx = std::max(hx, std::fabs(s.poly[2 * i]));
+            hy = std::max(hy, std::fabs(s.poly[2 * i + 1]));
+        }
+        s.half = V{ hx, hy, std::max(std::fabs(s.zMin), std::fabs(s.zMax)) };
     } else if (kind != kHalfSpace) {
         return -1;
     }

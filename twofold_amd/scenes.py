@@ -299,12 +299,40 @@ def hub(n=7, steps=90):
     return s
 
 
+def prism_profile(rng, sides, radius):
+    """A convex polygon of `sides` vertices on a circle of `radius`, vertex k at angle
+    (k + U[0.15, 0.85]) * 2 pi / sides (sorted random angles, no sliver), moved so that its centroid
+    is the origin. Returns x0, y0, x1, y1, ... (counter-clockwise, float32)."""
+    xs, ys = [], []
+    for k in range(sides):
+        a = (f32(k) + rng.range(0.15, 0.85)) * f32(2.0 * PI) / f32(sides)
+        xs.append(f32(radius * np.cos(a)))
+        ys.append(f32(radius * np.sin(a)))
+    # centroid of the polygon (shoelace), in float64, then rounded
+    area = cx = cy = 0.0
+    for i in range(sides):
+        j = (i + 1) % sides
+        c = float(xs[i]) * float(ys[j]) - float(xs[j]) * float(ys[i])
+        area += c
+        cx += (float(xs[i]) + float(xs[j])) * c
+        cy += (float(ys[i]) + float(ys[j])) * c
+    cx /= 3.0 * area
+    cy /= 3.0 * area
+    out = []
+    for x, y in zip(xs, ys):
+        out += [f32(float(x) - cx), f32(float(y) - cy)]
+    return out
+
+
 def planar_pile(side=256, spheres=4096, spacing=1.3, seed=0x9E3779B9, steps=240):
-    """The planar path at scale: side x side bodies locked to the xy plane (LockPlanar, z = 0) on a
-    grid in x-y from y = 1, inside a static U container (half-space floor, two walls of z half-depth
-    5 m), with free spheres (radius 0.3) dropped above them. Every planar body is a box with
-    half-extents (U[0.2, 0.6], U[0.2, 0.6], 0.5) turned about z by U[0, pi]; its friction is 0.6.
-    Draws per planar body, rows bottom-up: half-extent x, y, angle; per sphere: x, y offset, z."""
+    """The planar path at scale (the planar + 3D workload): side x side bodies locked to the xy plane
+    (LockPlanar, z = 0) on a grid in x-y from y = 1, inside a static U container (half-space floor,
+    two walls of z half-depth 5 m), with free spheres (radius 0.3) dropped above them. Planar bodies:
+    40% boxes of half-extents (U[0.2, 0.6], U[0.2, 0.6], 0.5) turned about z by U[0, pi]; 40% prisms
+    of 3-8 sides (prism_profile, radius U[0.2, 0.6], z from -0.5 to 0.5); 20% cylinders of radius
+    U[0.2, 0.6] and half height 0.5 whose axis is the plane normal. Friction 0.6. Draws per planar
+    body, rows bottom-up: the kind (U[0, 1)), then box: half-extent x, y, angle; prism: sides
+    (3 + floor(6 U)), radius, one draw per vertex; cylinder: radius. Per sphere: x, y offset, z."""
     s = Scene(f"planar-{side * side}", substeps=8, steps=steps)
     rng = Rng(seed)
     s.ground(0.6)
@@ -316,10 +344,20 @@ def planar_pile(side=256, spheres=4096, spacing=1.3, seed=0x9E3779B9, steps=240)
         s.shape(wb, BOX, (0.5, top * f32(0.5) + f32(4.0), 5.0), friction=0.6)
     for j in range(side):
         for i in range(side):
-            hx, hy, angle = rng.range(0.2, 0.6), rng.range(0.2, 0.6), rng.range(0, PI)
-            b = s.body(DYNAMIC, pos=((f32(i) - half) * f32(spacing), f32(1.0) + f32(spacing) * f32(j), 0),
-                       rot=axis_angle((0, 0, 1), angle), locks=LOCK_PLANAR)
-            s.shape(b, BOX, (hx, hy, 0.5), density=1000.0, friction=0.6)
+            pos = ((f32(i) - half) * f32(spacing), f32(1.0) + f32(spacing) * f32(j), 0)
+            u = rng.next()
+            if u < 0.4:
+                hx, hy, angle = rng.range(0.2, 0.6), rng.range(0.2, 0.6), rng.range(0, PI)
+                b = s.body(DYNAMIC, pos=pos, rot=axis_angle((0, 0, 1), angle), locks=LOCK_PLANAR)
+                s.shape(b, BOX, (hx, hy, 0.5), density=1000.0, friction=0.6)
+            elif u < 0.8:
+                sides = 3 + min(int(rng.next() * 6), 5)
+                radius = rng.range(0.2, 0.6)
+                b = s.body(DYNAMIC, pos=pos, locks=LOCK_PLANAR)
+                s.shape(b, PRISM, [-0.5, 0.5] + prism_profile(rng, sides, radius), density=1000.0, friction=0.6)
+            else:
+                b = s.body(DYNAMIC, pos=pos, locks=LOCK_PLANAR)
+                s.shape(b, CYLINDER, (rng.range(0.2, 0.6), 0.5), density=1000.0, friction=0.6)
     for k in range(spheres):
         x = rng.range(-wall_x + f32(1.0), wall_x - f32(1.0))
         y = top + f32(2.0) + rng.range(0.0, 64.0)
