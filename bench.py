@@ -519,6 +519,8 @@ def main():
     sim.solver.set_profiling(False)
     # the dominant kernel: the coloured passes on the pile, the rest passes (planar contacts) on the planar pile
     launches, ms, algo = sim.solver.profile(5 if planar else 1)
+    refreshes, wait_ms, _ = sim.solver.profile(3)
+    _, callback_ms, pairs = sim.solver.profile(4)
     achieved = algo / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     log(f"timed {args.steps} solve steps: {wall:.3f} s; {'rest' if planar else 'solve'} passes: {launches} launches, {ms:.2f} ms, "
         f"{algo / max(launches, 1) / 1e6:.2f} MB per launch")
@@ -556,6 +558,10 @@ def main():
             "world_steps_per_s": args.steps / wall, "gpu_solve_ms_per_step": solve_ms / args.steps,
             "transfer_ms_per_step": 1e3 * transfers / args.steps,
             "value_with_transfers": its / (wall + transfers), "pass_shape": shape,
+            "refreshes_per_step": refreshes / args.profile_steps,
+            "refresh_pairs_per_step": pairs / args.profile_steps,
+            "refresh_wait_ms_per_step": wait_ms / args.profile_steps,
+            "refresh_callback_ms_per_step": callback_ms / args.profile_steps,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

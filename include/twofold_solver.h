@@ -260,6 +260,12 @@ int tfs_read_profile(tfs_ctx* ctx, int family, uint64_t* launches, double* ms, d
  * (TFS_E_ARG for blocks outside 1..the resident limit or tail_rows above 256).
  */
 int tfs_autotune(tfs_ctx* ctx, uint32_t rounds);
+/* The passes with a grid barrier are launched as plain kernels by default: their grid is at most half
+ * of what the device holds at once and a stream runs one kernel at a time, so all their blocks are
+ * resident together when nothing else runs on the device. tfs_set_cooperative(ctx, 1) launches them
+ * with hipLaunchCooperativeKernel instead, which also holds when other work shares the device, at
+ * ~12 us per launch (no counterpart in the reference; the results are the same either way). */
+int tfs_set_cooperative(tfs_ctx* ctx, int on);
 int tfs_pass_shape(tfs_ctx* ctx, uint32_t shape[5]);
 int tfs_set_pass_shape(tfs_ctx* ctx, uint32_t blocks, uint32_t tail_rows, int made_axes);
 

@@ -1,6 +1,6 @@
-"""The pass shape (grid blocks, tail width, axes loaded or made) and its autotuning: every shape gives
-the same bodies, contacts and lists bit for bit, and tuning runs each candidate in turn, then keeps
-one of them. Scenes: an 8x8x8 box/sphere pile in walls over its first 180 steps (the drop and the
+"""The pass shape (grid blocks, tail width, axes loaded or made), its autotuning and the launch of the
+grid-barrier passes (plain or cooperative): every shape and launch gives the same bodies, contacts and
+lists bit for bit, and tuning runs each candidate in turn, then keeps one of them. Scenes: an 8x8x8 box/sphere pile in walls over its first 180 steps (the drop and the
 landing: many colours, refreshes, impacts) and the tumble scene over 120 (fast pairs, restitution)."""
 import numpy as np
 import pytest
@@ -10,9 +10,10 @@ from twofold_amd.world import Simulation
 
 pytestmark = pytest.mark.gpu
 
-def run(scene, shape=None, tune_rounds=0):
+def run(scene, shape=None, tune_rounds=0, cooperative=False):
     steps = 180 if scene.name.startswith("pile") else 120
     sim = Simulation(scene)
+    sim.solver.set_cooperative(cooperative)
     if shape is not None:
         sim.solver.set_pass_shape(*shape)
     seen = []
@@ -45,6 +46,11 @@ def test_every_shape_gives_the_same_bits(scene, default):
         got, seen = run(scene, shape)
         assert seen[-1]["blocks"] <= shape[0] and seen[-1]["tail_rows"] == shape[1] and seen[-1]["made_axes"] == shape[2]
         assert got == default[0], shape
+
+
+def test_cooperative_launch_gives_the_same_bits(scene, default):
+    got, _ = run(scene, cooperative=True)
+    assert got == default[0]
 
 
 def test_autotune_tries_every_candidate_then_keeps_one(scene, default):

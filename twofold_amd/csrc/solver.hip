@@ -2388,6 +2388,7 @@ struct tfs_ctx {
     uint32_t maxPassBlocks = 0;
     uint32_t passBlockLimit = 0;    // blocks of a cooperative pass that are resident at once (half the occupancy)
     bool usePass = true;
+    bool coopLaunch = false;        // grid-barrier kernels through hipLaunchCooperativeKernel (launchGrid)
     bool passTail = true;
     uint32_t tailSegRows = kTailSegRows;   // widest colour the tail takes
     bool madeAxes = false;
@@ -2639,6 +2640,22 @@ int runRows(tfs_ctx* ctx, int mode, const uint4* at, const uint32_t* starts, con
     return 0;
 }
 
+// A kernel with a grid barrier (k_pass, k_rest_levels, k_pass_adj): every block must be resident at
+// once. Its grid holds at most passBlockLimit blocks (half of what the device holds at once), and a
+// stream starts a kernel only when the one before it has finished, so when nothing else runs on the
+// device the blocks are resident together whatever the launch. The plain launch (the default) spares
+// the queue barriers the cooperative launch puts around the kernel (~6 us on each side: 5-7% of a
+// step); the cooperative one (tfs_set_cooperative, or TWOFOLD_COOP=on) also holds when other work
+// shares the device. A barrier whose blocks never all arrive ends at its spin limit with an error.
+int launchGrid(tfs_ctx* ctx, const void* fn, uint32_t grid, void** args) {
+    if (ctx->coopLaunch) {
+        HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kPassThreads), args, 0, ctx->stream));
+    } else {
+        HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(kPassThreads), args, 0, ctx->stream));
+    }
+    return 0;
+}
+
 // A pass over the listed row sets (active colours, then impact colours when `withImpact`) in one
 // cooperative launch.
 int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool withImpact, const Ctx& c,
@@ -2698,13 +2715,14 @@ int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool w
                                           : (rec ? (const void*)k_pass<kRelax, true, true> : (const void*)k_pass<kRelax, true, false>))
                          : (mode == kBias ? (rec ? (const void*)k_pass<kBias, false, true> : (const void*)k_pass<kBias, false, false>)
                                           : (rec ? (const void*)k_pass<kRelax, false, true> : (const void*)k_pass<kRelax, false, false>));
-    HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kPassThreads), args, 0, ctx->stream));
+    if (int e = launchGrid(ctx, fn, grid, args)) { return e; }
     advanceBarrier(ctx->barrierBase, grid, barriers);
     if (timed) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
     return 0;
 }
 
 int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32_t end, const Ctx& c);
+
 
 // One solve pass, biased or relax: the coloured active rows, the active rest in order, then the same
 // for the impact rows (src/world.cpp:1675-1680; src/lanes.cpp:419-424).
@@ -2848,7 +2866,7 @@ int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32
         const void* fn = mode == kBias ? (const void*)k_rest_levels<kBias, false>
                          : mode == kRelax ? (const void*)k_rest_levels<kRelax, false>
                                           : (const void*)k_rest_levels<kRestitute, false>;
-        HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kPassThreads), args, 0, ctx->stream));
+        if (int e = launchGrid(ctx, fn, grid, args)) { return e; }
         advanceBarrier(ctx->barrierBase, grid, wide - 1);
     }
     if (wide < levels) {
@@ -3235,7 +3253,7 @@ int passAdjoint(tfs_ctx* ctx, int mode, int slot, const Ctx& c) {
         const BarrierBase base = ctx->barrierBase;
         void* args[] = { &ctx->rows, &plan, (void*)&tape, (void*)&mass, (void*)&c, &ctx->adj, &ctx->bar, (void*)&base,
                          &ctx->cnt };
-        HIP_TRY(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kPassThreads), args, 0, ctx->stream));
+        if (int e = launchGrid(ctx, fn, grid, args)) { return e; }
         advanceBarrier(ctx->barrierBase, grid, plan.n - 1);
         return 0;
     }
@@ -3396,6 +3414,7 @@ int tfs_create(int device, uint32_t maxBodies, uint32_t maxContacts, tfs_ctx** o
         }
         if (const char* e = getenv("TWOFOLD_AXES")) { ctx->madeAxes = strcmp(e, "made") == 0; }
         if (const char* e = getenv("TWOFOLD_TAIL")) { ctx->passTail = strcmp(e, "off") != 0; }
+        if (const char* e = getenv("TWOFOLD_COOP")) { ctx->coopLaunch = strcmp(e, "off") != 0; }
     }
     if (!r) { r = allocBodies(ctx, maxBodies ? maxBodies : 1); }
     if (!r) { r = allocContacts(ctx, maxContacts ? maxContacts : 1); }
@@ -3566,6 +3585,12 @@ int tfs_set_pass_shape(tfs_ctx* ctx, uint32_t blocks, uint32_t tail_rows, int ma
     if (tail_rows > (uint32_t)kPassThreads) { return fail(ctx, TFS_E_ARG, "tail_rows must be at most %d", kPassThreads); }
     ctx->tuning = false;
     applyShape(ctx, tfs_ctx::PassShape{ blocks, tail_rows, made_axes != 0 });
+    return 0;
+}
+
+int tfs_set_cooperative(tfs_ctx* ctx, int on) {
+    if (!ctx) { return TFS_E_ARG; }
+    ctx->coopLaunch = on != 0;
     return 0;
 }
 

@@ -25,7 +25,7 @@ EXPORTS = ("tfs_abi_version", "tfs_sizeof_body", "tfs_sizeof_contact", "tfs_size
            "tfs_upload_contacts", "tfs_set_worlds", "tfs_step", "tfs_download_bodies", "tfs_download_contacts", "tfs_download_list",
            "tfs_last_step_ms", "tfs_last_step_counts", "tfs_set_profiling", "tfs_read_profile", "tfs_fixed_begin",
            "tfs_fixed_step", "tfs_adjoint_seed", "tfs_adjoint_step", "tfs_adjoint_read", "tfs_check_minmax",
-           "tfs_autotune", "tfs_pass_shape", "tfs_set_pass_shape")
+           "tfs_autotune", "tfs_pass_shape", "tfs_set_pass_shape", "tfs_set_cooperative")
 
 
 class SolverError(RuntimeError):
@@ -71,6 +71,7 @@ def load_library(path=LIB_PATH):
     lib.tfs_autotune.argtypes = [_vp, ctypes.c_uint32]
     lib.tfs_pass_shape.argtypes = [_vp, _vp]
     lib.tfs_set_pass_shape.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
+    lib.tfs_set_cooperative.argtypes = [_vp, ctypes.c_int]
     lib.tfs_read_profile.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_double)]
     if lib.tfs_abi_version() != 1:
@@ -248,6 +249,10 @@ class Solver:
         self._check(self.lib.tfs_pass_shape(self.h, _ptr(out)), "tfs_pass_shape")
         return dict(blocks=int(out[0]), tail_rows=int(out[1]), made_axes=bool(out[2]), tuning_steps_left=int(out[3]),
                     candidates=int(out[4]))
+
+    def set_cooperative(self, on):
+        """Launch the grid-barrier passes cooperatively (True) or as plain kernels (False, the default)."""
+        self._check(self.lib.tfs_set_cooperative(self.h, int(bool(on))), "tfs_set_cooperative")
 
     def set_pass_shape(self, blocks, tail_rows, made_axes):
         self._check(self.lib.tfs_set_pass_shape(self.h, blocks, tail_rows, int(bool(made_axes))), "tfs_set_pass_shape")
