@@ -605,11 +605,11 @@ struct LoadedAxes {
 // Packs the coloured contacts into rows in row order and builds their point axes from the bodies'
 // current masses, as prepareLanes does (src/lanes.cpp:66-110, 409-411).
 // The contact's points are read before anything is written, so one row waits for memory once.
-__global__ void __launch_bounds__(kThreads) k_pack(const uint32_t* __restrict__ list, uint32_t n, const tfs_contact* __restrict__ contacts,
-                       const uint8_t* __restrict__ moving, const float4* __restrict__ mass, Rows rows,
-                       uint32_t* __restrict__ rowOf) {
-    for (uint32_t r = gid(); r < n; r += gstride()) {
-        const uint32_t ci = list[r];
+// Row r from contact ci: header, directions, points and their three axes.
+__device__ __forceinline__ void packRow(uint32_t r, uint32_t ci, const tfs_contact* __restrict__ contacts,
+                                        const uint8_t* __restrict__ moving, const float4* __restrict__ mass,
+                                        const Rows& rows, uint32_t* __restrict__ rowOf) {
+    {
         const tfs_contact& c = contacts[ci];
         const int count = c.count;
         tfs_point pt[4];
@@ -637,6 +637,25 @@ __global__ void __launch_bounds__(kThreads) k_pack(const uint32_t* __restrict__ 
             for (int d = 0; d < 3; ++d) { storeAxis(rows, r, k, d, makeAxis(ma, mb, ld3(p.anchor_a), ld3(p.anchor_b), dir[d])); }
         }
         rowOf[ci] = r;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) k_pack(const uint32_t* __restrict__ list, uint32_t n, const tfs_contact* __restrict__ contacts,
+                       const uint8_t* __restrict__ moving, const float4* __restrict__ mass, Rows rows,
+                       uint32_t* __restrict__ rowOf) {
+    for (uint32_t r = gid(); r < n; r += gstride()) { packRow(r, list[r], contacts, moving, mass, rows, rowOf); }
+}
+
+// The rows of the contacts a refresh collided again, once their new points are in: the other rows
+// were packed while the host collided them.
+__global__ void __launch_bounds__(kThreads) k_pack_again(const uint32_t* __restrict__ again, uint32_t n,
+                                                         const uint8_t* __restrict__ colors,
+                                                         const tfs_contact* __restrict__ contacts,
+                                                         const uint8_t* __restrict__ moving, const float4* __restrict__ mass,
+                                                         Rows rows, uint32_t* __restrict__ rowOf) {
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        const uint32_t ci = again[k];
+        if (colors[ci] < TFS_COLORS) { packRow(rowOf[ci], ci, contacts, moving, mass, rows, rowOf); }
     }
 }
 
@@ -2933,9 +2952,15 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
     const uint32_t waves = std::min<uint32_t>(nAgain, 4u * 8192u);
     LAUNCH(k_gather_again, (waves + 3) / 4, kThreads, again, nAgain, ctx->contacts, ctx->bodies, ctx->motion,
            ctx->pinAgain, ctx->pinContacts, ctx->pinBodies);
+    HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
+    // Every row but those of the contacts collided again is final now (rebased, new inverse masses):
+    // they are packed while the host collides, and the others again after (k_pack_again), as
+    // prepareLanes packs all rows after a refresh.
+    LAUNCH(k_pack, blocksFor(laned), kThreads, ctx->rowsList, laned, ctx->contacts, ctx->moving, ctx->mass, ctx->rows,
+           ctx->rowOf);
     using Clock = std::chrono::steady_clock;
     const Clock::time_point t0 = Clock::now();
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipEventSynchronize(ctx->ev[3]));
     const Clock::time_point t1 = Clock::now();
     if (cb(user, substep, remaining, nAgain, ctx->pinAgain, ctx->pinContacts, ctx->pinBodies, ctx->pinFresh) != 0) {
         return fail(ctx, TFS_E_CALLBACK, "the refresh callback failed at substep %d", substep);
@@ -2952,8 +2977,13 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
     const float matchSq = ctx->settings.speculative * ctx->settings.speculative;
     LAUNCH(k_refresh_apply, blocksFor(nAgain), kThreads, again, nAgain, ctx->fresh,
            ctx->contacts, ctx->bodies, ctx->motion, ctx->mass, matchSq);
-    // A refreshed contact keeps its row; its points and axes are packed again, as prepareLanes does.
-    if (int e = packRows(ctx, laned, impactLaned, false)) { return e; }
+    // A refreshed contact keeps its row; its points and axes are packed again.
+    LAUNCH(k_pack_again, blocksFor(nAgain), kThreads, again, nAgain, ctx->colors, ctx->contacts, ctx->moving, ctx->mass,
+           ctx->rows, ctx->rowOf);
+    if (impactLaned > 0) {
+        LAUNCH(k_impact_rows, blocksFor(impactLaned), kThreads, ctx->impactRowsList, impactLaned, ctx->rowOf,
+               ctx->rows.hdr, ctx->impactHdr);
+    }
     if (nRest > 0) { LAUNCH(k_rest_pack, blocksFor(nRest), kThreads, ctx->restEntries[0], nRest, ctx->contacts, ctx->mass, restPack(ctx)); }
     return 0;
 }
