@@ -177,7 +177,8 @@ size_t tfs_sizeof_settings(void);
 void tfs_default_settings(tfs_settings* out);
 
 /* Creates a context on HIP device `device` with room for the given counts (they can grow later by
- * uploading more). Replaces the construction of World::Impl's solver state (src/world.cpp:145-165). */
+ * uploading more, up to 268,435,455 contacts: TFS_E_ARG beyond). Replaces the construction of
+ * World::Impl's solver state (src/world.cpp:145-165). */
 int tfs_create(int device, uint32_t max_bodies, uint32_t max_contacts, tfs_ctx** out);
 void tfs_destroy(tfs_ctx* ctx);
 const char* tfs_last_error(const tfs_ctx* ctx);

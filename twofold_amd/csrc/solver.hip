@@ -688,7 +688,7 @@ __global__ void k_unpack(uint32_t n, Rows rows, tfs_contact* contacts) {
 }
 
 // Velocity and impulse accesses of the rows. Within one launch per colour, kernel boundaries make
-// every write visible to the next colour, and plain loads and stores serve. In a cooperative pass
+// every write visible to the next colour, and plain loads and stores serve. In a grid-barrier pass
 // the colours follow each other inside one launch, across blocks on different XCDs; there the rows
 // read and write velocities and impulses with device-coherent (agent-scope) atomic loads and stores,
 // so that a barrier only has to order them, not flush caches.
@@ -856,7 +856,7 @@ __global__ void __launch_bounds__(kThreads) k_rows(Rows rows, const uint4* impac
     rowStep<MODE, false, MADE, true>(rows, impactHdr ? h.w : i, h, motion, mass, ctx, impactHdr ? Tape{} : tape);
 }
 
-// A whole pass in one cooperative launch: the colours of the active rows, then those of the impact
+// A whole pass in one launch with a grid barrier: the colours of the active rows, then those of the impact
 // rows, one after the other with a grid-wide barrier between them.
 constexpr int kPassThreads = 256;
 constexpr int kMaxSegments = 2 * TFS_COLORS;
@@ -868,7 +868,7 @@ struct PassPlan {
     uint32_t begin[kMaxSegments], end[kMaxSegments];
 };
 
-// The grid-wide barrier of a cooperative launch. Blocks arrive in kBarrierGroups groups (block b in
+// The grid-wide barrier of a pass launch (launchGrid). Blocks arrive in kBarrierGroups groups (block b in
 // group b % kBarrierGroups), each group on a counter of its own cache line, with an atomic add whose
 // result nobody waits for; then the first wave of every block reads all the group counters at once,
 // one lane per group, until every group is complete. Counters only grow: a launch is given each
@@ -1588,7 +1588,7 @@ __device__ __forceinline__ void rowAdjoint(const Rows& rows, uint32_t r, const T
     }
 }
 
-// The reverse of a pass in one cooperative launch: the plan lists the colours last to first.
+// The reverse of a pass in one launch with a grid barrier: the plan lists the colours last to first.
 template <bool BIAS, bool MADE>
 __global__ void __launch_bounds__(kPassThreads) k_pass_adj(Rows rows, PassPlan plan, Tape tape, const float4* mass,
                                                            Ctx ctx, Adjoint g, BarrierLines* bar, BarrierBase base,
@@ -1911,7 +1911,7 @@ __global__ void __launch_bounds__(kThreads) k_warm_rest_bodies(uint32_t nb, cons
 
 // The rest list level after level. ONE_BLOCK: one work-group walks the levels with __syncthreads
 // between them (narrow levels: the block barrier is cheap and one CU's accesses need no coherence
-// protocol); otherwise one cooperative launch over many blocks, velocities crossing blocks through
+// protocol); otherwise one launch over many blocks with the grid barrier, velocities crossing blocks through
 // coherent accesses as in k_pass. Each thread loads its next entry at the start of a level. A contact
 // is touched by one thread only in a launch, so plain accesses serve for it.
 constexpr int kRestBlockThreads = 512;
@@ -2405,7 +2405,7 @@ struct tfs_ctx {
     BarrierLines* bar = nullptr;   // grid barrier counters
     BarrierBase barrierBase{};
     uint32_t maxPassBlocks = 0;
-    uint32_t passBlockLimit = 0;    // blocks of a cooperative pass that are resident at once (half the occupancy)
+    uint32_t passBlockLimit = 0;    // blocks of a grid-barrier pass: half of what the device holds at once
     bool usePass = true;
     bool coopLaunch = false;        // grid-barrier kernels through hipLaunchCooperativeKernel (launchGrid)
     bool passTail = true;
@@ -2676,7 +2676,7 @@ int launchGrid(tfs_ctx* ctx, const void* fn, uint32_t grid, void** args) {
 }
 
 // A pass over the listed row sets (active colours, then impact colours when `withImpact`) in one
-// cooperative launch.
+// launch with a grid barrier between colours (launchGrid).
 int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool withImpact, const Ctx& c,
             const Tape& tape = Tape{}) {
     PassPlan plan{};
@@ -2741,7 +2741,6 @@ int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool w
 }
 
 int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32_t end, const Ctx& c);
-
 
 // One solve pass, biased or relax: the coloured active rows, the active rest in order, then the same
 // for the impact rows (src/world.cpp:1675-1680; src/lanes.cpp:419-424).
@@ -2848,7 +2847,7 @@ int buildRestLevels(tfs_ctx* ctx, int which, const uint32_t* list, uint32_t n) {
 }
 
 // A rest pass: the list's leading wide levels (more contacts than a rest work-group has threads) in
-// one cooperative launch over many blocks, the levels from the first narrow one on in one work-group.
+// one launch over many blocks with the grid barrier, the levels from the first narrow one on in one work-group.
 RestPack restPack(const tfs_ctx* ctx) {
     return RestPack{ ctx->restH, ctx->restP, ctx->restPeak, ctx->restAx, ctx->restPos, ctx->capC };
 }
@@ -3259,8 +3258,8 @@ int fixedStep(tfs_ctx* ctx, bool record) {
     return 0;
 }
 
-// The reverse of one pass: the colours last to first, in one cooperative launch (or one launch per
-// colour where cooperative launches are off).
+// The reverse of one pass: the colours last to first, in one launch with a grid barrier (or one launch per
+// colour where the pass launch is off).
 int passAdjoint(tfs_ctx* ctx, int mode, int slot, const Ctx& c) {
     const DevCounts& hc = ctx->fixedCounts;
     const Tape tape = tapeOf(ctx, slot);
@@ -3429,7 +3428,7 @@ int tfs_create(int device, uint32_t maxBodies, uint32_t maxContacts, tfs_ctx** o
             hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device) != hipSuccess) {
             r = TFS_E_DEVICE;
         }
-        // Every block of a cooperative pass must be resident at once (half the occupancy limit keeps well
+        // Every block of a grid-barrier pass must be resident at once (half the occupancy limit keeps well
         // inside that). The barrier's cost grows with the block count (about a microsecond at 64
         // blocks, 4 at 256), so the pass uses at most 128 blocks, one row per thread up to 32k rows
         // a colour.
