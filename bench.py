@@ -313,7 +313,7 @@ def run_batched(args):
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": None, "traffic_source": None,
-            "kernel": "k_pass<bias|relax> (one cooperative launch per solve pass over all worlds' rows)",
+            "kernel": "k_pass<bias|relax> (one grid-barrier launch per solve pass over all worlds' rows)",
             "launches": launches, "profiled_steps": args.batch_profile_steps,
             "bytes_per_launch": algo / max(launches, 1), "ms_per_launch": ms / max(launches, 1),
         },
@@ -568,8 +568,8 @@ def main():
             "traffic": K_PASS_TRAFFIC_BYTES if pile and args.side == 32 else None,
             "traffic_source": K_PASS_TRAFFIC_SOURCE if pile and args.side == 32 else None,
             "kernel": ("k_rest_levels<bias|relax> (one solve pass over a rest list: the leading wide levels in a "
-                       "cooperative launch, the rest in one work-group; bytes per planar contact 248 + 68n)" if planar
-                       else "k_pass<bias|relax> (one cooperative launch per solve pass)"),
+                       "grid-barrier launch, the rest in one work-group; bytes per planar contact 248 + 68n)" if planar
+                       else "k_pass<bias|relax> (one grid-barrier launch per solve pass)"),
             "launches": launches, "profiled_steps": args.profile_steps,
             "bytes_per_launch": algo / max(launches, 1), "ms_per_launch": ms / max(launches, 1),
         },

@@ -232,7 +232,8 @@ int tfs_last_step_ms(tfs_ctx* ctx, float* solve_ms, float* sleep_ms);
 
 /*
  * Measurement (no counterpart in the reference): with profiling on, every launch of the coloured-row
- * kernels is bracketed by HIP events on the context's stream, and per family (0 warm start, 1 solve:
+ * kernels is bracketed by HIP events on the context's stream, and per family (0 the substep start: velocities integrated and
+ * the warm start of every row, coloured and rest, in one launch; 1 solve:
  * biased and relax passes including the impact repeats, 2 restitution) the library sums the launch
  * count, the event time in milliseconds and the algorithmic bytes of the rows launched (per contact
  * with n points: warm start 224 + 244n, solve 224 + 296n, restitution 224 + 96n). Two more families
@@ -247,7 +248,7 @@ int tfs_read_profile(tfs_ctx* ctx, int family, uint64_t* launches, double* ms, d
 
 /*
  * Pass-shape autotuning (no counterpart in the reference). The coloured passes run as one
- * cooperative launch each; its shape is the number of blocks (one per compute unit at most; more
+ * grid-barrier launch each; its shape is the number of blocks (one per compute unit at most; more
  * blocks spread a colour's rows wider but make each grid barrier dearer), the widest colour that the
  * one-block tail takes at the end of a pass (0: no tail), and whether each row loads its packed
  * axes or makes them from the bodies' inverse masses. Results are the same, bit for bit, for every

@@ -1305,13 +1305,8 @@ __device__ __forceinline__ void warmApply(uint32_t e, const WarmSide& w, Motion&
 
 // Each body applies its rows in row order; a row's loads go out while the row before it is applied
 // (two rows in flight).
-__global__ void __launch_bounds__(kThreads) k_warm_bodies(uint32_t nb, const uint32_t* __restrict__ deg, const uint32_t* __restrict__ start,
-                              const uint32_t* __restrict__ ep, Rows rows, float4* __restrict__ motion) {
-    for (uint32_t x = gid(); x < nb; x += gstride()) {
-        const uint32_t d = deg[x];
-        if (d == 0) { continue; }
-        const uint32_t* list = ep + start[x];
-        Motion m = loadMotion(motion, x);
+__device__ __forceinline__ void warmBodyRows(uint32_t d, const uint32_t* __restrict__ list, const Rows& rows, Motion& m) {
+    {
         WarmSide a, b;
         uint32_t ea = list[0], eb = d > 1 ? list[1] : 0u;
         warmLoad(rows, ea, a);
@@ -1325,7 +1320,6 @@ __global__ void __launch_bounds__(kThreads) k_warm_bodies(uint32_t nb, const uin
             ea = ec;
             eb = ed;
         }
-        storeVelocity(motion, x, m);
     }
 }
 
@@ -1887,24 +1881,44 @@ __global__ void k_wrest_fill(const uint4* entries, uint32_t n, const uint32_t* s
     }
 }
 
-__global__ void __launch_bounds__(kThreads) k_warm_rest_bodies(uint32_t nb, const uint32_t* deg, const uint32_t* start,
-                                                               const uint32_t* ep, const uint4* entries, RestPack rp,
-                                                               float4* motion, const float4* mass) {
+__device__ __forceinline__ void warmBodyRest(uint32_t d, const uint32_t* __restrict__ list, const uint4* __restrict__ entries,
+                                             const RestPack& rp, const Mass& mx, Motion& m) {
+    for (uint32_t j = 0; j < d; ++j) {
+        const uint32_t en = list[j], k = en >> 1;
+        const bool sideB = en & 1u;
+        const uint32_t flags = entries[k].w;
+        tfs_contact c;
+        PlanarAxes ax[4];
+        loadRest(rp, k, c, ax);
+        warmRestSide(c, sideB, !(flags & (sideB ? 8u : 4u)), mx, m, ax);
+    }
+}
+
+// The start of a substep, one thread per body: integrateVelocities (src/world.cpp:1141-1151), then the
+// warm-start pushes of the body's coloured rows in row order (warmStartBatches), then those of its rest
+// contacts in list order (warmStart of the rest, src/solver.cpp:319-358; REST). A body's velocity after
+// them depends only on its own operations, in this order, so the three stages run as one.
+template <bool REST>
+__global__ void __launch_bounds__(kThreads)
+    k_substep_start(tfs_body* bodies, uint32_t nb, float4* __restrict__ motion, Ctx ctx, const uint32_t* __restrict__ deg,
+                    const uint32_t* __restrict__ start, const uint32_t* __restrict__ ep, Rows rows,
+                    const uint32_t* __restrict__ restDeg, const uint32_t* __restrict__ restStart,
+                    const uint32_t* __restrict__ restEp, const uint4* __restrict__ entries, RestPack rp,
+                    const float4* __restrict__ mass) {
     for (uint32_t x = gid(); x < nb; x += gstride()) {
+        tfs_body& body = bodies[x];
+        const bool integrate = body.alive && body.type == TFS_DYNAMIC && body.awake;
         const uint32_t d = deg[x];
-        if (d == 0) { continue; }
-        const uint32_t* list = ep + start[x];
+        const uint32_t dr = REST ? restDeg[x] : 0u;
+        if (!integrate && d == 0 && dr == 0) { continue; }
         Motion m = loadMotion(motion, x);
-        const Mass mx = loadMass(mass, x);
-        for (uint32_t j = 0; j < d; ++j) {
-            const uint32_t en = list[j], k = en >> 1;
-            const bool sideB = en & 1u;
-            const uint32_t flags = entries[k].w;
-            tfs_contact c;
-            PlanarAxes ax[4];
-            loadRest(rp, k, c, ax);
-            warmRestSide(c, sideB, !(flags & (sideB ? 8u : 4u)), mx, m, ax);
+        if (integrate) {
+            float kept = body.kept;
+            integrateVelocity(body, ctx, m.v, m.w, kept);
+            body.kept = kept;
         }
+        if (d > 0) { warmBodyRows(d, ep + start[x], rows, m); }
+        if (REST && dr > 0) { warmBodyRest(dr, restEp + restStart[x], entries, rp, loadMass(mass, x), m); }
         storeVelocity(motion, x, m);
     }
 }
@@ -1974,18 +1988,6 @@ __global__ void __launch_bounds__(ONE_BLOCK ? kRestBlockThreads : kPassThreads)
                 gridSync(bar, phase, cnt);
             }
         }
-    }
-}
-
-__global__ void k_integrate_velocities(tfs_body* bodies, uint32_t n, float4* motion, Ctx ctx) {
-    for (uint32_t i = gid(); i < n; i += gstride()) {
-        tfs_body& b = bodies[i];
-        if (!b.alive || b.type != TFS_DYNAMIC || !b.awake) { continue; }
-        Motion m = loadMotion(motion, i);
-        float kept = b.kept;
-        integrateVelocity(b, ctx, m.v, m.w, kept);
-        b.kept = kept;
-        storeVelocity(motion, i, m);
     }
 }
 
@@ -2770,24 +2772,6 @@ int solvePass(tfs_ctx* ctx, int mode, const DevCounts& hc, uint32_t laned, uint3
     return 0;
 }
 
-// The warm start of every coloured row, one launch over the bodies (see k_warm_bodies).
-int warmRows(tfs_ctx* ctx, uint32_t laned, uint32_t points) {
-    size_t ev = 0;
-    if (ctx->profiling) {
-        ev = ctx->pending.size() * 2;
-        while (ctx->evPool.size() < ev + 2) {
-            hipEvent_t x;
-            HIP_TRY(hipEventCreate(&x));
-            ctx->evPool.push_back(x);
-        }
-        ctx->pending.push_back({ 0, ev, rowBytes(kWarm, laned, points) });
-        HIP_TRY(hipEventRecord(ctx->evPool[ev], ctx->stream));
-    }
-    LAUNCH(k_warm_bodies, blocksFor(ctx->nb), kThreads, ctx->nb, ctx->bdeg, ctx->bstart, ctx->bep, ctx->rows, ctx->motion);
-    if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
-    return 0;
-}
-
 // Packs the rows and builds the impact rows and each body's list of rows.
 int packRows(tfs_ctx* ctx, uint32_t laned, uint32_t impactLaned, bool lists) {
     LAUNCH(k_pack, blocksFor(laned), kThreads, ctx->rowsList, laned, ctx->contacts, ctx->moving, ctx->mass, ctx->rows,
@@ -2850,6 +2834,33 @@ int buildRestLevels(tfs_ctx* ctx, int which, const uint32_t* list, uint32_t n) {
 // one launch over many blocks with the grid barrier, the levels from the first narrow one on in one work-group.
 RestPack restPack(const tfs_ctx* ctx) {
     return RestPack{ ctx->restH, ctx->restP, ctx->restPeak, ctx->restAx, ctx->restPos, ctx->capC };
+}
+
+// The start of a substep: velocities integrated and the warm start of every row, coloured and rest,
+// one launch over the bodies (k_substep_start). Profile family 0 times it, with the warm start's bytes.
+int substepStart(tfs_ctx* ctx, const Ctx& c, uint32_t laned, uint32_t points, bool rest) {
+    size_t ev = 0;
+    if (ctx->profiling) {
+        ev = ctx->pending.size() * 2;
+        while (ctx->evPool.size() < ev + 2) {
+            hipEvent_t x;
+            HIP_TRY(hipEventCreate(&x));
+            ctx->evPool.push_back(x);
+        }
+        ctx->pending.push_back({ 0, ev, rowBytes(kWarm, laned, points) });
+        HIP_TRY(hipEventRecord(ctx->evPool[ev], ctx->stream));
+    }
+    const uint32_t nb = ctx->nb;
+    if (rest) {
+        LAUNCH(k_substep_start<true>, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c, ctx->bdeg, ctx->bstart,
+               ctx->bep, ctx->rows, ctx->wrestDeg, ctx->wrestStart, ctx->wrestEp, ctx->restEntries[0], restPack(ctx),
+               ctx->mass);
+    } else {
+        LAUNCH(k_substep_start<false>, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c, ctx->bdeg, ctx->bstart,
+               ctx->bep, ctx->rows, nullptr, nullptr, nullptr, nullptr, RestPack{}, ctx->mass);
+    }
+    if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
+    return 0;
 }
 
 int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32_t end, const Ctx& c) {
@@ -3136,13 +3147,10 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
                 return r;
             }
         }
-        LAUNCH(k_integrate_velocities, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c);
-        if ((r = warmRows(ctx, laned, hc.pointsActive))) { return r; }
-        if (hc.nActive > laned) {
-            LAUNCH(k_warm_rest_bodies, blocksFor(nb), kThreads, nb, ctx->wrestDeg, ctx->wrestStart, ctx->wrestEp,
-                   ctx->restEntries[0], restPack(ctx), ctx->motion, ctx->mass);
+        if ((r = substepStart(ctx, c, laned, hc.pointsActive, hc.nActive > laned)) ||
+            (r = solvePass(ctx, kBias, hc, laned, impactLaned, c))) {
+            return r;
         }
-        if ((r = solvePass(ctx, kBias, hc, laned, impactLaned, c))) { return r; }
         LAUNCH(k_integrate_positions, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c);
         if ((r = solvePass(ctx, kRelax, hc, laned, impactLaned, c))) { return r; }
     }
@@ -3247,8 +3255,7 @@ int fixedStep(tfs_ctx* ctx, bool record) {
                             : runRows(ctx, mode, nullptr, hc.colorStart, hc.colorPoints, c, t);
     };
     for (int i = 0; i < n; ++i) {
-        LAUNCH(k_integrate_velocities, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c);
-        if ((r = warmRows(ctx, laned, hc.pointsActive)) || (r = pass(kBias, 2 * i))) { return r; }
+        if ((r = substepStart(ctx, c, laned, hc.pointsActive, false)) || (r = pass(kBias, 2 * i))) { return r; }
         LAUNCH(k_fx_positions, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c.h);
         if ((r = pass(kRelax, 2 * i + 1))) { return r; }
     }

@@ -227,8 +227,8 @@ class Solver:
         self._check(self.lib.tfs_set_profiling(self.h, 1 if on else 0), "tfs_set_profiling")
 
     def profile(self, family):
-        """(launches, milliseconds, algorithmic bytes) summed since profiling was turned on; family 0 warm
-        start, 1 solve, 2 restitution (event-timed); 3 waiting for the refresh's contacts on the host,
+        """(launches, milliseconds, algorithmic bytes) summed since profiling was turned on; family 0 the
+        substep start (velocities integrated, every row's warm start), 1 solve, 2 restitution (event-timed); 3 waiting for the refresh's contacts on the host,
         4 the refresh callback (host clock; bytes = pairs collided again); 5 the solve passes over the
         rest lists (event-timed)."""
         n, ms, by = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_double()
