@@ -231,7 +231,7 @@ def run_batched(args):
     sizes = scenes.batch_sizes(args.worlds)
     t0 = time.perf_counter()
     worlds = [scenes.batch_world(n, 0x5EED0000 + k) for k, n in enumerate(sizes)]
-    sim = Simulation(worlds)
+    sim = Simulation(worlds, device_refresh=args.refresh == "device")
     log(f"{len(worlds)} worlds, {len(sim.bodies)} bodies, built in {time.perf_counter() - t0:.1f} s; settling "
         f"{args.settle} steps")
     step = SolveSteps(sim)
@@ -255,8 +255,12 @@ def run_batched(args):
         step()
     sim.solver.set_profiling(False)
     launches, ms, algo = sim.solver.profile(1)
+    _, wait_ms, _ = sim.solver.profile(3)
+    refreshes, narrow_ms, pairs = sim.solver.profile(4)
+    family_ms = [sim.solver.profile(f)[1] / args.batch_profile_steps for f in range(6)]
     achieved = algo / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    log(f"solve passes: {launches} launches, {ms:.2f} ms, {algo / max(launches, 1) / 1e6:.2f} MB per launch")
+    log(f"solve passes: {launches} launches, {ms:.2f} ms, {algo / max(launches, 1) / 1e6:.2f} MB per launch; "
+        f"ms per step by profile family: {', '.join(f'{x:.2f}' for x in family_ms)}")
 
     # looping: the plain path on each sampled world alone, from the batch's current state
     rng = scenes.Rng(0x1005)
@@ -270,8 +274,11 @@ def run_batched(args):
             host = HostWorld(worlds[k], sim.settings)
             bodies = np.ascontiguousarray(sim.bodies[lo:hi])
             contacts_k, _ = host.collide(bodies.copy(), sim.dt)
+            solver.set_body_shapes(None)
             for timed in (False, True):
                 solver.upload(bodies, contacts_k)
+                if sim.device_refresh is not None:
+                    solver.set_body_shapes(host.body_shapes())
                 sync()
                 t = time.perf_counter()
                 solver.step_native(sim.dt, host.refresh_fn, host.h)
@@ -305,10 +312,16 @@ def run_batched(args):
         "config": {
             "workload": f"{len(worlds)} independent worlds of log-uniform [8, 1024] bodies ({len(sim.bodies)} "
                         f"bodies), after {args.settle} drop steps, dt 1/60, 4 substeps; per step: host narrowphase "
-                        f"+ upload (untimed), then one solve block + sleep pass over all worlds (timed)",
+                        f"+ upload (untimed), then one solve block + sleep pass over all worlds (timed; its mid-step "
+                        f"refreshes collide on the {'device' if sim.device_refresh is not None else 'host'})",
             "worlds": len(worlds), "bodies": len(sim.bodies), "contacts_per_step": contacts / args.steps,
             "looping_world_steps_per_s": looping, "looping_sample_worlds": len(sample),
             "batched_over_looping": batched / looping, "pass_shape": shape,
+            "refreshes_per_step": refreshes / args.batch_profile_steps,
+            "refresh_pairs_per_step": pairs / args.batch_profile_steps,
+            "refresh_wait_ms_per_step": wait_ms / args.batch_profile_steps,
+            "refresh_narrowphase_ms_per_step": narrow_ms / args.batch_profile_steps,
+            "refresh": "device" if sim.device_refresh is not None else "host callback",
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
@@ -448,6 +461,9 @@ def main():
     ap.add_argument("--batch-profile-steps", type=int, default=3, help="batched: event-timed steps for the roofline")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline leg")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--refresh", choices=("device", "host"), default="device",
+                    help="where mid-step refreshes collide their pairs: on the device (tfs_set_body_shapes; "
+                         "spheres, boxes and half-spaces) or through the host world's callback")
     ap.add_argument("--tune-rounds", type=int, default=2,
                     help="rounds of pass-shape autotuning over the last settle steps (0: the default shape)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -486,7 +502,8 @@ def main():
 
     scene = scenes.planar_pile() if planar else scenes.pile(side=args.side)
     t0 = time.perf_counter()
-    sim = Simulation(scene)
+    # the planar pile's prisms and cylinders are collided on the host only
+    sim = Simulation(scene, device_refresh=args.refresh == "device" and not planar)
     log(f"{scene.name}: {len(sim.bodies)} bodies built in {time.perf_counter() - t0:.1f} s; settling {args.settle} steps")
     for k in range(args.settle):
         tune(sim.solver, args, k)
@@ -519,8 +536,8 @@ def main():
     sim.solver.set_profiling(False)
     # the dominant kernel: the coloured passes on the pile, the rest passes (planar contacts) on the planar pile
     launches, ms, algo = sim.solver.profile(5 if planar else 1)
-    refreshes, wait_ms, _ = sim.solver.profile(3)
-    _, callback_ms, pairs = sim.solver.profile(4)
+    _, wait_ms, _ = sim.solver.profile(3)
+    refreshes, narrow_ms, pairs = sim.solver.profile(4)
     achieved = algo / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     log(f"timed {args.steps} solve steps: {wall:.3f} s; {'rest' if planar else 'solve'} passes: {launches} launches, {ms:.2f} ms, "
         f"{algo / max(launches, 1) / 1e6:.2f} MB per launch")
@@ -551,7 +568,8 @@ def main():
                          f"steps)" if planar else
                          f"settled {len(bodies0)}-body box/sphere pile in walls (after {args.settle} drop steps)")
                         + f", dt 1/60, {substeps} substeps; per step: host narrowphase + upload (untimed), then one "
-                        f"solve block + sleep pass on the GPU (timed)",
+                        f"solve block + sleep pass on the GPU (timed; its mid-step refreshes collide on the "
+                        f"{'device' if sim.device_refresh is not None else 'host'})",
             "bodies": len(bodies0), "contacts_per_step": contacts / args.steps, "active_points": counts["points"],
             "impact_points": counts["impact_points"], "colors": counts["colors"], "rest": counts["rest"],
             "rest_levels": counts["rest_levels"], "impact_rest_levels": counts["impact_rest_levels"],
@@ -561,7 +579,8 @@ def main():
             "refreshes_per_step": refreshes / args.profile_steps,
             "refresh_pairs_per_step": pairs / args.profile_steps,
             "refresh_wait_ms_per_step": wait_ms / args.profile_steps,
-            "refresh_callback_ms_per_step": callback_ms / args.profile_steps,
+            "refresh_narrowphase_ms_per_step": narrow_ms / args.profile_steps,
+            "refresh": "device" if sim.device_refresh is not None else "host callback",
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,

@@ -48,6 +48,10 @@ CONTACT_DTYPE = np.dtype(
     align=True,
 )
 
+# tfs_shape: one convex shape per body for the device refresh (kinds TFS_SHAPE_*)
+SHAPE_DTYPE = np.dtype([("kind", np.uint32), ("half", np.float32, 3), ("radius", np.float32)])
+SHAPE_SPHERE, SHAPE_BOX, SHAPE_HALF_SPACE = 0, 1, 4
+
 SETTINGS_DTYPE = np.dtype(
     [
         ("gravity", F, 3), ("length_unit", F), ("substeps", np.int32),

@@ -1,0 +1,423 @@
+// narrow.h - the host world's narrowphase for spheres, boxes and half-spaces, one source for the
+// host world (scene.cpp, g++) and the device refresh (solver.hip, hipcc). Both builds keep every
+// multiply and add separate and use IEEE division and square root, and nothing here depends on the
+// compiler beyond that, so a pair collided on either side gives the same manifold bit for bit.
+//
+// The manifolds follow the reference's conventions (normal from shape A to shape B, shapes ordered
+// by index, points at the midpoint between the two surfaces, a speculative margin grown by the
+// pair's travel: src/world.cpp:484-486, 509-511; anchors lifted onto each surface for separated
+// points: src/world.cpp:536-540) but not its bits: box-box uses separating axes and face clipping
+// of this file's own, where the reference clips polyhedra (src/collide3d.cpp).
+#pragma once
+
+#include <stdint.h>
+#include <string.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define TFN_HD __host__ __device__ __forceinline__
+#else
+#include <math.h>
+#define TFN_HD inline
+#endif
+
+#include "../../include/twofold_solver.h"
+
+namespace tfn {
+
+struct V {
+    float x, y, z;
+};
+TFN_HD V vec3(float x, float y, float z) { return V{ x, y, z }; }
+TFN_HD V operator+(V a, V b) { return vec3(a.x + b.x, a.y + b.y, a.z + b.z); }
+TFN_HD V operator-(V a, V b) { return vec3(a.x - b.x, a.y - b.y, a.z - b.z); }
+TFN_HD V operator*(V a, float s) { return vec3(a.x * s, a.y * s, a.z * s); }
+TFN_HD V operator-(V a) { return vec3(-a.x, -a.y, -a.z); }
+TFN_HD float dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+TFN_HD V cross(V a, V b) { return vec3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+TFN_HD float len(V a) { return sqrtf(dot(a, a)); }
+TFN_HD V unit(V a) {
+    const float l = len(a);
+    return l > 0 ? a * (1.0f / l) : vec3(0, 1, 0);
+}
+TFN_HD V ld(const float* p) { return vec3(p[0], p[1], p[2]); }
+TFN_HD void st(float* p, V v) {
+    p[0] = v.x;
+    p[1] = v.y;
+    p[2] = v.z;
+}
+// std::min / std::max as the standard defines them (the second operand wins only when it compares
+// less / greater), so that both builds pick the same operand for signed zeros and NaNs.
+TFN_HD float lo(float a, float b) { return b < a ? b : a; }
+TFN_HD float hi(float a, float b) { return a < b ? b : a; }
+
+struct Q {
+    float x, y, z, w;
+};
+TFN_HD V rotate(Q q, V v) {
+    const V u = vec3(q.x, q.y, q.z);
+    const V t = cross(u, v) * 2.0f;
+    return v + t * q.w + cross(u, t);
+}
+TFN_HD V unrotate(Q q, V v) { return rotate(Q{ -q.x, -q.y, -q.z, q.w }, v); }
+TFN_HD Q ldq(const float* p) { return Q{ p[0], p[1], p[2], p[3] }; }
+
+TFN_HD V perpendicular(V n) {
+    if (fabsf(n.x) > 0.57735f) { return unit(vec3(n.y, -n.x, 0.0f)); }
+    return unit(vec3(0.0f, n.z, -n.y));
+}
+
+TFN_HD float comp(V v, int i) { return i == 0 ? v.x : i == 1 ? v.y : v.z; }
+
+enum Kind { kSphere = 0, kBox = 1, kCylinder = 2, kPrism = 3, kHalfSpace = 4 };
+
+struct Point {
+    V pos;       // midpoint between the surfaces
+    float sep;   // along the normal, negative when overlapping
+};
+
+struct Manifold {
+    V n;
+    int count = 0;
+    Point p[4];
+};
+
+struct Pose {
+    V c;
+    Q q;
+    V ax[3];  // the shape's axes in the world
+};
+
+TFN_HD Pose poseOf(const float* com, const float* q) {
+    Pose p;
+    p.c = ld(com);
+    p.q = ldq(q);
+    p.ax[0] = rotate(p.q, vec3(1, 0, 0));
+    p.ax[1] = rotate(p.q, vec3(0, 1, 0));
+    p.ax[2] = rotate(p.q, vec3(0, 0, 1));
+    return p;
+}
+
+// Keeps at most four points: the deepest, the farthest from it, and the two farthest on either side
+// of the line between them.
+TFN_HD void reduce(const Point* pts, int n, V nrm, Manifold& m) {
+    if (n <= 4) {
+        m.count = n;
+        for (int i = 0; i < n; ++i) { m.p[i] = pts[i]; }
+        return;
+    }
+    int a = 0;
+    for (int i = 1; i < n; ++i) {
+        if (pts[i].sep < pts[a].sep) { a = i; }
+    }
+    int b = a == 0 ? 1 : 0;
+    float best = -1.0f;
+    for (int i = 0; i < n; ++i) {
+        const V d = pts[i].pos - pts[a].pos;
+        if (i != a && dot(d, d) > best) {
+            best = dot(d, d);
+            b = i;
+        }
+    }
+    const V ab = pts[b].pos - pts[a].pos;
+    int c = -1, d = -1;
+    float up = 0.0f, down = 0.0f;
+    for (int i = 0; i < n; ++i) {
+        if (i == a || i == b) { continue; }
+        const float s = dot(cross(ab, pts[i].pos - pts[a].pos), nrm);
+        if (s > up) {
+            up = s;
+            c = i;
+        }
+        if (s < down) {
+            down = s;
+            d = i;
+        }
+    }
+    const int keep[4] = { a, b, c, d };
+    m.count = 0;
+    for (int k = 0; k < 4; ++k) {
+        if (keep[k] >= 0) { m.p[m.count++] = pts[keep[k]]; }
+    }
+}
+
+TFN_HD void sphereSphere(V ca, float ra, V cb, float rb, float margin, Manifold& m) {
+    const V d = cb - ca;
+    const float dist = len(d);
+    const V n = dist > 1e-9f ? d * (1.0f / dist) : vec3(0, 1, 0);
+    const float sep = dist - ra - rb;
+    m.count = 0;
+    if (sep > margin) { return; }
+    m.n = n;
+    const V pa = ca + n * ra, pb = cb - n * rb;
+    m.p[0] = Point{ (pa + pb) * 0.5f, sep };
+    m.count = 1;
+}
+
+// Half-space A (below its local xy plane) against sphere B.
+TFN_HD void halfSpaceSphere(const Pose& a, V cb, float r, float margin, Manifold& m) {
+    const V n = a.ax[2];
+    const float offset = dot(n, a.c);
+    m.count = 0;
+    m.n = n;
+    const float sep = dot(n, cb) - offset - r;
+    if (sep > margin) { return; }
+    const V pb = cb - n * r;
+    m.p[0] = Point{ pb - n * (0.5f * sep), sep };
+    m.count = 1;
+}
+
+constexpr int kMaxHalfSpacePoints = 32;
+
+// Half-space A against the vertices of a convex shape: every vertex within the margin, reduced.
+TFN_HD void halfSpacePoints(const Pose& a, const V* verts, int nv, float margin, Manifold& m) {
+    const V n = a.ax[2];
+    const float offset = dot(n, a.c);
+    m.count = 0;
+    m.n = n;
+    Point pts[kMaxHalfSpacePoints];
+    int np = 0;
+    for (int i = 0; i < nv && np < kMaxHalfSpacePoints; ++i) {
+        const float sep = dot(n, verts[i]) - offset;
+        if (sep <= margin) { pts[np++] = Point{ verts[i] - n * (0.5f * sep), sep }; }
+    }
+    reduce(pts, np, n, m);
+}
+
+TFN_HD void boxVertices(const Pose& b, V half, V* out) {
+    for (int k = 0; k < 8; ++k) {
+        out[k] = b.c + b.ax[0] * ((k & 1) ? half.x : -half.x) + b.ax[1] * ((k & 2) ? half.y : -half.y) +
+                 b.ax[2] * ((k & 4) ? half.z : -half.z);
+    }
+}
+
+// Box A against sphere B; the normal points from the box to the sphere.
+TFN_HD void boxSphere(V h, const Pose& a, V cs, float r, float margin, Manifold& m) {
+    const V d = cs - a.c;
+    const V c = vec3(dot(d, a.ax[0]), dot(d, a.ax[1]), dot(d, a.ax[2]));
+    const V q = vec3(hi(-h.x, lo(h.x, c.x)), hi(-h.y, lo(h.y, c.y)), hi(-h.z, lo(h.z, c.z)));
+    V nl, onBox;
+    float sep;
+    const V off = c - q;
+    const float dist = len(off);
+    if (dist > 1e-7f) {
+        nl = off * (1.0f / dist);
+        sep = dist - r;
+        onBox = q;
+    } else {
+        int axis = 0;
+        float pen = h.x - fabsf(c.x);
+        for (int i = 1; i < 3; ++i) {
+            const float p = comp(h, i) - fabsf(comp(c, i));
+            if (p < pen) {
+                pen = p;
+                axis = i;
+            }
+        }
+        const float s = comp(c, axis) >= 0 ? 1.0f : -1.0f;
+        nl = vec3(axis == 0 ? s : 0.0f, axis == 1 ? s : 0.0f, axis == 2 ? s : 0.0f);
+        sep = -pen - r;
+        onBox = c;
+        if (axis == 0) { onBox.x = s * h.x; }
+        if (axis == 1) { onBox.y = s * h.y; }
+        if (axis == 2) { onBox.z = s * h.z; }
+    }
+    m.count = 0;
+    if (sep > margin) { return; }
+    const V n = a.ax[0] * nl.x + a.ax[1] * nl.y + a.ax[2] * nl.z;
+    const V pa = a.c + a.ax[0] * onBox.x + a.ax[1] * onBox.y + a.ax[2] * onBox.z;
+    const V pb = cs - n * r;
+    m.n = n;
+    m.p[0] = Point{ (pa + pb) * 0.5f, sep };
+    m.count = 1;
+}
+
+// A convex polygon clipped by planes: a quad by four planes has at most eight vertices; the room
+// beyond that only takes vertices that rounding adds near a plane, and both builds drop the same ones.
+constexpr int kMaxPolygon = 16;
+
+// Clips a convex polygon against the plane dot(n, x) <= d.
+TFN_HD void clip(V* poly, int& k, V n, float d) {
+    V out[kMaxPolygon];
+    int m = 0;
+    for (int i = 0; i < k; ++i) {
+        const V p = poly[i], q = poly[(i + 1) % k];
+        const float dp = dot(n, p) - d, dq = dot(n, q) - d;
+        if (dp <= 0 && m < kMaxPolygon) { out[m++] = p; }
+        if (((dp < 0 && dq > 0) || (dp > 0 && dq < 0)) && m < kMaxPolygon) { out[m++] = p + (q - p) * (dp / (dp - dq)); }
+    }
+    for (int i = 0; i < m; ++i) { poly[i] = out[i]; }
+    k = m;
+}
+
+TFN_HD float extentAlong(const Pose& p, const float* h, V l) {
+    return h[0] * fabsf(dot(p.ax[0], l)) + h[1] * fabsf(dot(p.ax[1], l)) + h[2] * fabsf(dot(p.ax[2], l));
+}
+
+// Box A against box B by separating axes: a face of one box clipped against the other's incident
+// face, or the closest points of two edges.
+TFN_HD void boxBox(V halfA, const Pose& a, V halfB, const Pose& b, float margin, Manifold& m) {
+    m.count = 0;
+    const V t = b.c - a.c;
+    const float ha[3] = { halfA.x, halfA.y, halfA.z }, hb[3] = { halfB.x, halfB.y, halfB.z };
+    float bestFace = -1e30f, bestEdge = -1e30f;
+    int faceAxis = -1, edgeI = -1, edgeJ = -1;
+    V faceN = vec3(0, 0, 0), edgeN = vec3(0, 0, 0);
+    for (int i = 0; i < 6; ++i) {
+        const V l = i < 3 ? a.ax[i] : b.ax[i - 3];
+        const float sep = fabsf(dot(t, l)) - extentAlong(a, ha, l) - extentAlong(b, hb, l);
+        if (sep > margin) { return; }
+        if (sep > bestFace) {
+            bestFace = sep;
+            faceAxis = i;
+            faceN = dot(t, l) >= 0 ? l : -l;
+        }
+    }
+    for (int i = 0; i < 3; ++i) {
+        for (int j = 0; j < 3; ++j) {
+            V l = cross(a.ax[i], b.ax[j]);
+            const float l2 = dot(l, l);
+            if (l2 < 1e-6f) { continue; }
+            l = l * (1.0f / sqrtf(l2));
+            const float sep = fabsf(dot(t, l)) - extentAlong(a, ha, l) - extentAlong(b, hb, l);
+            if (sep > margin) { return; }
+            if (sep > bestEdge) {
+                bestEdge = sep;
+                edgeI = i;
+                edgeJ = j;
+                edgeN = dot(t, l) >= 0 ? l : -l;
+            }
+        }
+    }
+    if (edgeI >= 0 && bestEdge > 0.95f * bestFace + 0.01f) {
+        // Two edges: the edge of A farthest along n and the edge of B farthest along -n.
+        V pa = a.c, pb = b.c;
+        for (int k = 0; k < 3; ++k) {
+            if (k != edgeI) { pa = pa + a.ax[k] * (dot(a.ax[k], edgeN) >= 0 ? ha[k] : -ha[k]); }
+            if (k != edgeJ) { pb = pb + b.ax[k] * (dot(b.ax[k], edgeN) >= 0 ? -hb[k] : hb[k]); }
+        }
+        const V da = a.ax[edgeI], db = b.ax[edgeJ];
+        const V r = pa - pb;
+        const float e = dot(db, db), f = dot(db, r), c = dot(da, r), bb = dot(da, db);
+        const float den = 1.0f - bb * bb;
+        float s = den > 1e-9f ? (bb * f - c * e) / den : 0.0f;
+        s = hi(-ha[edgeI], lo(ha[edgeI], s));
+        float u = (bb * s + f) / e;
+        u = hi(-hb[edgeJ], lo(hb[edgeJ], u));
+        const V qa = pa + da * s, qb = pb + db * u;
+        const float sep = dot(qb - qa, edgeN);
+        if (sep > margin) { return; }
+        m.n = edgeN;
+        m.p[0] = Point{ (qa + qb) * 0.5f, sep };
+        m.count = 1;
+        return;
+    }
+    const bool refA = faceAxis < 3;
+    const Pose& R = refA ? a : b;
+    const Pose& I = refA ? b : a;
+    const float* hr = refA ? ha : hb;
+    const float* hn = refA ? hb : ha;
+    const int ri = refA ? faceAxis : faceAxis - 3;
+    const V rn = refA ? faceN : -faceN;  // the reference face's outward normal, toward the other box
+    int ii = 0;
+    float most = -1.0f;
+    for (int k = 0; k < 3; ++k) {
+        const float d = fabsf(dot(I.ax[k], rn));
+        if (d > most) {
+            most = d;
+            ii = k;
+        }
+    }
+    const V in = dot(I.ax[ii], rn) > 0 ? -I.ax[ii] : I.ax[ii];  // incident face normal, facing the reference face
+    const int i1 = (ii + 1) % 3, i2 = (ii + 2) % 3;
+    const V fc = I.c + in * hn[ii];
+    V poly[kMaxPolygon] = { fc + I.ax[i1] * hn[i1] + I.ax[i2] * hn[i2], fc - I.ax[i1] * hn[i1] + I.ax[i2] * hn[i2],
+                            fc - I.ax[i1] * hn[i1] - I.ax[i2] * hn[i2], fc + I.ax[i1] * hn[i1] - I.ax[i2] * hn[i2] };
+    int k = 4;
+    for (int j = 0; j < 3 && k > 0; ++j) {
+        if (j == ri) { continue; }
+        clip(poly, k, R.ax[j], dot(R.ax[j], R.c) + hr[j]);
+        clip(poly, k, -R.ax[j], -dot(R.ax[j], R.c) + hr[j]);
+    }
+    Point pts[kMaxPolygon];
+    int np = 0;
+    const float plane = dot(rn, R.c) + hr[ri];
+    for (int i = 0; i < k; ++i) {
+        const float sep = dot(rn, poly[i]) - plane;
+        if (sep > margin) { continue; }
+        pts[np++] = Point{ poly[i] - rn * (0.5f * sep), sep };
+    }
+    m.n = faceN;
+    reduce(pts, np, faceN, m);
+}
+
+// The margin of a pair: the speculative distance grown by the pair's travel over `dt`, from the
+// bodies' velocities and extents.
+TFN_HD float pairMargin(V va, V wa, float extentA, V vb, V wb, float extentB, float dt, float speculative) {
+    const float travel = (len(vb - va) + len(wa) * extentA + len(wb) * extentB) * dt;
+    return speculative + travel;
+}
+
+// Sphere, box and half-space pairs, A before B: 1 with a manifold, 0 without, -1 for a pair this
+// file does not collide (another kind of shape, or two boxes that both stay extruded along the world
+// z, which the host world collides as a 2D problem).
+TFN_HD int collideBasic(int ka, V ha, float ra, const Pose& pa, int kb, V hb, float rb, const Pose& pb, bool bothInPlane,
+                        float margin, Manifold& m) {
+    const bool basicA = ka == kSphere || ka == kBox || ka == kHalfSpace;
+    const bool basicB = kb == kSphere || kb == kBox || kb == kHalfSpace;
+    if (!basicA || !basicB) { return -1; }
+    m.count = 0;
+    if (ka == kHalfSpace && kb != kHalfSpace) {
+        if (kb == kSphere) {
+            halfSpaceSphere(pa, pb.c, rb, margin, m);
+        } else {
+            V verts[8];
+            boxVertices(pb, hb, verts);
+            halfSpacePoints(pa, verts, 8, margin, m);
+        }
+    } else if (kb == kHalfSpace) {
+        return 0;
+    } else if (ka == kSphere && kb == kSphere) {
+        sphereSphere(pa.c, ra, pb.c, rb, margin, m);
+    } else if (kb == kSphere) {
+        boxSphere(ha, pa, pb.c, rb, margin, m);
+    } else if (ka == kSphere) {
+        boxSphere(hb, pb, pa.c, ra, margin, m);
+        m.n = -m.n;
+    } else {
+        if (bothInPlane) { return -1; }
+        boxBox(ha, pa, hb, pb, margin, m);
+    }
+    return m.count > 0 ? 1 : 0;
+}
+
+// A body that stays extruded along the world z: static, or locked to a plane whose normal is z
+// (a free body that happens to stand upright is not: it may meet others along z).
+TFN_HD bool inPlane(const tfs_body& b) {
+    if (b.type == TFS_STATIC) { return true; }
+    if ((b.locks & TFS_LOCK_PLANAR) != TFS_LOCK_PLANAR) { return false; }
+    return fabsf(rotate(ldq(b.lock_frame), vec3(0, 0, 1)).z) > 0.9999f;
+}
+
+// The geometric fields of the contact of a pair from its manifold: normal, tangents, count and per
+// point local_a, separation and anchors (src/world.cpp:920-950, 536-540 conventions).
+TFN_HD void contactGeometry(const tfs_body& a, const tfs_body& b, const Manifold& m, float speculative, tfs_contact& c) {
+    const V n = unit(m.n);
+    const V t1 = perpendicular(n);
+    st(c.normal, n);
+    st(c.tangent1, t1);
+    st(c.tangent2, cross(n, t1));
+    c.count = m.count;
+    const Q qa = ldq(a.q);
+    for (int i = 0; i < m.count; ++i) {
+        tfs_point& p = c.points[i];
+        const V pos = m.p[i].pos;
+        st(p.local_a, unrotate(qa, pos - ld(a.p)));
+        p.separation = m.p[i].sep;
+        const V up = p.separation > speculative ? n * (0.5f * p.separation) : vec3(0, 0, 0);
+        st(p.anchor_a, pos - up - ld(a.com));
+        st(p.anchor_b, pos + up - ld(b.com));
+    }
+}
+
+}  // namespace tfn

@@ -25,43 +25,12 @@
 #include <vector>
 
 #include "../../include/twofold_solver.h"
+#include "narrow.h"
 
 namespace {
 
-struct V {
-    float x, y, z;
-};
-V operator+(V a, V b) { return { a.x + b.x, a.y + b.y, a.z + b.z }; }
-V operator-(V a, V b) { return { a.x - b.x, a.y - b.y, a.z - b.z }; }
-V operator*(V a, float s) { return { a.x * s, a.y * s, a.z * s }; }
-V operator-(V a) { return { -a.x, -a.y, -a.z }; }
-float dot(V a, V b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-V cross(V a, V b) { return { a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x }; }
-float len(V a) { return std::sqrt(dot(a, a)); }
-V unit(V a) {
-    const float l = len(a);
-    return l > 0 ? a * (1.0f / l) : V{ 0, 1, 0 };
-}
-V ld(const float* p) { return { p[0], p[1], p[2] }; }
-void st(float* p, V v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+using namespace tfn;
 
-struct Q {
-    float x, y, z, w;
-};
-V rotate(Q q, V v) {
-    const V u{ q.x, q.y, q.z };
-    const V t = cross(u, v) * 2.0f;
-    return v + t * q.w + cross(u, t);
-}
-V unrotate(Q q, V v) { return rotate(Q{ -q.x, -q.y, -q.z, q.w }, v); }
-Q ldq(const float* p) { return { p[0], p[1], p[2], p[3] }; }
-
-V perpendicular(V n) {
-    if (std::fabs(n.x) > 0.57735f) { return unit(V{ n.y, -n.x, 0.0f }); }
-    return unit(V{ 0.0f, n.z, -n.y });
-}
-
-enum Kind { kSphere = 0, kBox = 1, kCylinder = 2, kPrism = 3, kHalfSpace = 4 };
 constexpr int kMaxPrismVertices = 16;
 
 struct Shape {
@@ -73,115 +42,25 @@ struct Shape {
     float density = 1.0f, friction = 0.5f, restitution = 0.0f, rolling = 0.0f;
 };
 
-struct Point {
-    V pos;       // midpoint between the surfaces
-    float sep;   // along the normal, negative when overlapping
-};
-
-struct Manifold {
-    V n;
-    int count = 0;
-    Point p[4];
-};
-
-struct Pose {
-    V c;
-    Q q;
-    V ax[3];  // box axes in the world
-};
-
-Pose poseOf(const tfs_body& b) {
-    Pose p;
-    p.c = ld(b.com);
-    p.q = ldq(b.q);
-    p.ax[0] = rotate(p.q, V{ 1, 0, 0 });
-    p.ax[1] = rotate(p.q, V{ 0, 1, 0 });
-    p.ax[2] = rotate(p.q, V{ 0, 0, 1 });
-    return p;
-}
-
-float comp(V v, int i) { return i == 0 ? v.x : i == 1 ? v.y : v.z; }
-
-// Keeps at most four points: the deepest, the farthest from it, and the two farthest on either side
-// of the line between them.
-void reduce(std::vector<Point>& pts, V n, Manifold& m) {
-    if (pts.size() <= 4) {
-        m.count = (int)pts.size();
-        for (size_t i = 0; i < pts.size(); ++i) { m.p[i] = pts[i]; }
-        return;
-    }
-    size_t a = 0;
-    for (size_t i = 1; i < pts.size(); ++i) {
-        if (pts[i].sep < pts[a].sep) { a = i; }
-    }
-    size_t b = a == 0 ? 1 : 0;
-    float best = -1.0f;
-    for (size_t i = 0; i < pts.size(); ++i) {
-        const V d = pts[i].pos - pts[a].pos;
-        if (i != a && dot(d, d) > best) {
-            best = dot(d, d);
-            b = i;
-        }
-    }
-    const V ab = pts[b].pos - pts[a].pos;
-    size_t c = SIZE_MAX, d = SIZE_MAX;
-    float hi = 0.0f, lo = 0.0f;
-    for (size_t i = 0; i < pts.size(); ++i) {
-        if (i == a || i == b) { continue; }
-        const float s = dot(cross(ab, pts[i].pos - pts[a].pos), n);
-        if (s > hi) {
-            hi = s;
-            c = i;
-        }
-        if (s < lo) {
-            lo = s;
-            d = i;
-        }
-    }
-    m.count = 0;
-    for (size_t i : { a, b, c, d }) {
-        if (i != SIZE_MAX) { m.p[m.count++] = pts[i]; }
-    }
-}
-
-void sphereSphere(V ca, float ra, V cb, float rb, float margin, Manifold& m) {
-    const V d = cb - ca;
-    const float dist = len(d);
-    const V n = dist > 1e-9f ? d * (1.0f / dist) : V{ 0, 1, 0 };
-    const float sep = dist - ra - rb;
-    m.count = 0;
-    if (sep > margin) { return; }
-    m.n = n;
-    const V pa = ca + n * ra, pb = cb - n * rb;
-    m.p[0] = { (pa + pb) * 0.5f, sep };
-    m.count = 1;
-}
+Pose poseOf(const tfs_body& b) { return tfn::poseOf(b.com, b.q); }
 
 // Half-space A (below its local xy plane) against a sphere, box, prism or cylinder B.
 void halfSpaceVs(const Pose& a, const Shape& sb, const Pose& b, float margin, Manifold& m) {
-    const V n = a.ax[2];
-    const float offset = dot(n, a.c);
-    m.count = 0;
-    m.n = n;
     if (sb.kind == kSphere) {
-        const float sep = dot(n, b.c) - offset - sb.radius;
-        if (sep > margin) { return; }
-        const V pb = b.c - n * sb.radius;
-        m.p[0] = { pb - n * (0.5f * sep), sep };
-        m.count = 1;
+        halfSpaceSphere(a, b.c, sb.radius, margin, m);
         return;
     }
-    std::vector<V> verts;
+    const V n = a.ax[2];
+    V verts[kMaxHalfSpacePoints];
+    int nv = 0;
     if (sb.kind == kBox) {
-        for (int k = 0; k < 8; ++k) {
-            verts.push_back(b.c + b.ax[0] * ((k & 1) ? sb.half.x : -sb.half.x) +
-                            b.ax[1] * ((k & 2) ? sb.half.y : -sb.half.y) + b.ax[2] * ((k & 4) ? sb.half.z : -sb.half.z));
-        }
+        boxVertices(b, sb.half, verts);
+        nv = 8;
     } else if (sb.kind == kPrism) {
         for (size_t i = 0; i + 1 < sb.poly.size(); i += 2) {
             const V xy = b.c + b.ax[0] * sb.poly[i] + b.ax[1] * sb.poly[i + 1];
-            verts.push_back(xy + b.ax[2] * sb.zMin);
-            verts.push_back(xy + b.ax[2] * sb.zMax);
+            verts[nv++] = xy + b.ax[2] * sb.zMin;
+            verts[nv++] = xy + b.ax[2] * sb.zMax;
         }
     } else if (sb.kind == kCylinder) {
         // each cap rim's lowest point along n, or four rim points of a cap lying flat on the plane
@@ -190,173 +69,13 @@ void halfSpaceVs(const Pose& a, const Shape& sb, const Pose& b, float margin, Ma
             const V down = -n - b.ax[2] * dot(-n, b.ax[2]);
             const float dl = len(down);
             if (dl > 1e-4f) {
-                verts.push_back(cap + down * (sb.radius / dl));
+                verts[nv++] = cap + down * (sb.radius / dl);
             } else {
-                for (int k = 0; k < 4; ++k) {
-                    verts.push_back(cap + b.ax[k & 1] * ((k & 2) ? -sb.radius : sb.radius));
-                }
+                for (int k = 0; k < 4; ++k) { verts[nv++] = cap + b.ax[k & 1] * ((k & 2) ? -sb.radius : sb.radius); }
             }
         }
     }
-    std::vector<Point> pts;
-    for (const V& v : verts) {
-        const float sep = dot(n, v) - offset;
-        if (sep <= margin) { pts.push_back({ v - n * (0.5f * sep), sep }); }
-    }
-    reduce(pts, n, m);
-}
-
-// Box A against sphere B; the normal points from the box to the sphere.
-void boxSphere(const Shape& sa, const Pose& a, V cs, float r, float margin, Manifold& m) {
-    const V d = cs - a.c;
-    const V c{ dot(d, a.ax[0]), dot(d, a.ax[1]), dot(d, a.ax[2]) };
-    const V h = sa.half;
-    const V q{ std::max(-h.x, std::min(h.x, c.x)), std::max(-h.y, std::min(h.y, c.y)), std::max(-h.z, std::min(h.z, c.z)) };
-    V nl, onBox;
-    float sep;
-    const V off = c - q;
-    const float dist = len(off);
-    if (dist > 1e-7f) {
-        nl = off * (1.0f / dist);
-        sep = dist - r;
-        onBox = q;
-    } else {
-        int axis = 0;
-        float pen = h.x - std::fabs(c.x);
-        for (int i = 1; i < 3; ++i) {
-            const float p = comp(h, i) - std::fabs(comp(c, i));
-            if (p < pen) {
-                pen = p;
-                axis = i;
-            }
-        }
-        const float s = comp(c, axis) >= 0 ? 1.0f : -1.0f;
-        nl = V{ axis == 0 ? s : 0.0f, axis == 1 ? s : 0.0f, axis == 2 ? s : 0.0f };
-        sep = -pen - r;
-        onBox = c;
-        if (axis == 0) { onBox.x = s * h.x; }
-        if (axis == 1) { onBox.y = s * h.y; }
-        if (axis == 2) { onBox.z = s * h.z; }
-    }
-    m.count = 0;
-    if (sep > margin) { return; }
-    const V n = a.ax[0] * nl.x + a.ax[1] * nl.y + a.ax[2] * nl.z;
-    const V pa = a.c + a.ax[0] * onBox.x + a.ax[1] * onBox.y + a.ax[2] * onBox.z;
-    const V pb = cs - n * r;
-    m.n = n;
-    m.p[0] = { (pa + pb) * 0.5f, sep };
-    m.count = 1;
-}
-
-// Clips a convex polygon against the plane dot(n, x) <= d.
-void clip(std::vector<V>& poly, V n, float d) {
-    std::vector<V> out;
-    const size_t k = poly.size();
-    for (size_t i = 0; i < k; ++i) {
-        const V p = poly[i], q = poly[(i + 1) % k];
-        const float dp = dot(n, p) - d, dq = dot(n, q) - d;
-        if (dp <= 0) { out.push_back(p); }
-        if ((dp < 0 && dq > 0) || (dp > 0 && dq < 0)) { out.push_back(p + (q - p) * (dp / (dp - dq))); }
-    }
-    poly.swap(out);
-}
-
-// Box A against box B by separating axes: a face of one box clipped against the other's
-// incident face, or the closest points of two edges.
-void boxBox(const Shape& sa, const Pose& a, const Shape& sb, const Pose& b, float margin, Manifold& m) {
-    m.count = 0;
-    const V t = b.c - a.c;
-    const float ha[3] = { sa.half.x, sa.half.y, sa.half.z }, hb[3] = { sb.half.x, sb.half.y, sb.half.z };
-    auto extentAlong = [](const Pose& p, const float* h, V l) {
-        return h[0] * std::fabs(dot(p.ax[0], l)) + h[1] * std::fabs(dot(p.ax[1], l)) + h[2] * std::fabs(dot(p.ax[2], l));
-    };
-    float bestFace = -1e30f, bestEdge = -1e30f;
-    int faceAxis = -1, edgeI = -1, edgeJ = -1;
-    V faceN{}, edgeN{};
-    for (int i = 0; i < 6; ++i) {
-        const V l = i < 3 ? a.ax[i] : b.ax[i - 3];
-        const float sep = std::fabs(dot(t, l)) - extentAlong(a, ha, l) - extentAlong(b, hb, l);
-        if (sep > margin) { return; }
-        if (sep > bestFace) {
-            bestFace = sep;
-            faceAxis = i;
-            faceN = dot(t, l) >= 0 ? l : -l;
-        }
-    }
-    for (int i = 0; i < 3; ++i) {
-        for (int j = 0; j < 3; ++j) {
-            V l = cross(a.ax[i], b.ax[j]);
-            const float l2 = dot(l, l);
-            if (l2 < 1e-6f) { continue; }
-            l = l * (1.0f / std::sqrt(l2));
-            const float sep = std::fabs(dot(t, l)) - extentAlong(a, ha, l) - extentAlong(b, hb, l);
-            if (sep > margin) { return; }
-            if (sep > bestEdge) {
-                bestEdge = sep;
-                edgeI = i;
-                edgeJ = j;
-                edgeN = dot(t, l) >= 0 ? l : -l;
-            }
-        }
-    }
-    if (edgeI >= 0 && bestEdge > 0.95f * bestFace + 0.01f) {
-        // Two edges: the edge of A farthest along n and the edge of B farthest along -n.
-        V pa = a.c, pb = b.c;
-        for (int k = 0; k < 3; ++k) {
-            if (k != edgeI) { pa = pa + a.ax[k] * (dot(a.ax[k], edgeN) >= 0 ? ha[k] : -ha[k]); }
-            if (k != edgeJ) { pb = pb + b.ax[k] * (dot(b.ax[k], edgeN) >= 0 ? -hb[k] : hb[k]); }
-        }
-        const V da = a.ax[edgeI], db = b.ax[edgeJ];
-        const V r = pa - pb;
-        const float e = dot(db, db), f = dot(db, r), c = dot(da, r), bb = dot(da, db);
-        const float den = 1.0f - bb * bb;
-        float s = den > 1e-9f ? (bb * f - c * e) / den : 0.0f;
-        s = std::max(-ha[edgeI], std::min(ha[edgeI], s));
-        float u = (bb * s + f) / e;
-        u = std::max(-hb[edgeJ], std::min(hb[edgeJ], u));
-        const V qa = pa + da * s, qb = pb + db * u;
-        const float sep = dot(qb - qa, edgeN);
-        if (sep > margin) { return; }
-        m.n = edgeN;
-        m.p[0] = { (qa + qb) * 0.5f, sep };
-        m.count = 1;
-        return;
-    }
-    const bool refA = faceAxis < 3;
-    const Pose& R = refA ? a : b;
-    const Pose& I = refA ? b : a;
-    const float* hr = refA ? ha : hb;
-    const float* hi = refA ? hb : ha;
-    const int ri = refA ? faceAxis : faceAxis - 3;
-    const V rn = refA ? faceN : -faceN;  // the reference face's outward normal, toward the other box
-    int ii = 0;
-    float most = -1.0f;
-    for (int k = 0; k < 3; ++k) {
-        const float d = std::fabs(dot(I.ax[k], rn));
-        if (d > most) {
-            most = d;
-            ii = k;
-        }
-    }
-    const V in = dot(I.ax[ii], rn) > 0 ? -I.ax[ii] : I.ax[ii];  // incident face normal, facing the reference face
-    const int i1 = (ii + 1) % 3, i2 = (ii + 2) % 3;
-    const V fc = I.c + in * hi[ii];
-    std::vector<V> poly = { fc + I.ax[i1] * hi[i1] + I.ax[i2] * hi[i2], fc - I.ax[i1] * hi[i1] + I.ax[i2] * hi[i2],
-                            fc - I.ax[i1] * hi[i1] - I.ax[i2] * hi[i2], fc + I.ax[i1] * hi[i1] - I.ax[i2] * hi[i2] };
-    for (int k = 0; k < 3 && !poly.empty(); ++k) {
-        if (k == ri) { continue; }
-        clip(poly, R.ax[k], dot(R.ax[k], R.c) + hr[k]);
-        clip(poly, -R.ax[k], -dot(R.ax[k], R.c) + hr[k]);
-    }
-    std::vector<Point> pts;
-    const float plane = dot(rn, R.c) + hr[ri];
-    for (const V& p : poly) {
-        const float sep = dot(rn, p) - plane;
-        if (sep > margin) { continue; }
-        pts.push_back({ p - rn * (0.5f * sep), sep });
-    }
-    m.n = faceN;
-    reduce(pts, faceN, m);
+    halfSpacePoints(a, verts, nv, margin, m);
 }
 
 // ---- z-extruded shapes as a 2D problem ----------------------------------------------------------
@@ -417,14 +136,6 @@ bool profileOf(const Shape& s, const Pose& p, Profile& f) {
     f.zlo = up > 0 ? p.c.z + z0 : p.c.z - z1;
     f.zhi = up > 0 ? p.c.z + z1 : p.c.z - z0;
     return true;
-}
-
-// A body that stays extruded along the world z: static, or locked to a plane whose normal is z
-// (a free body that happens to stand upright is not: it may meet others along z).
-bool inPlane(const tfs_body& b) {
-    if (b.type == TFS_STATIC) { return true; }
-    if ((b.locks & TFS_LOCK_PLANAR) != TFS_LOCK_PLANAR) { return false; }
-    return std::fabs(rotate(ldq(b.lock_frame), V{ 0, 0, 1 }).z) > 0.9999f;
 }
 
 // A 2D manifold: normal from A to B, up to two points (midpoints) with their separations.
@@ -711,12 +422,14 @@ float travelOf(const tfs_body& b, float dt) {
 
 // One manifold for the pair (a < b), or none.
 bool collide(const World& w, uint32_t ia, const tfs_body& a, uint32_t ib, const tfs_body& b, float dt, Manifold& m) {
-    const float travel =
-        (len(ld(b.v) - ld(a.v)) + len(ld(a.w)) * a.extent + len(ld(b.w)) * b.extent) * dt;
-    const float margin = w.settings.speculative + travel;
+    const float margin = pairMargin(ld(a.v), ld(a.w), a.extent, ld(b.v), ld(b.w), b.extent, dt, w.settings.speculative);
     const Shape& sa = w.shapes[ia];
     const Shape& sb = w.shapes[ib];
     const Pose pa = poseOf(a), pb = poseOf(b);
+    const bool bothInPlane = inPlane(a) && inPlane(b);
+    // spheres, boxes and half-spaces as the device refresh collides them (narrow.h)
+    const int basic = collideBasic(sa.kind, sa.half, sa.radius, pa, sb.kind, sb.half, sb.radius, pb, bothInPlane, margin, m);
+    if (basic >= 0) { return basic > 0; }
     m.count = 0;
     if (sa.kind == kHalfSpace && sb.kind != kHalfSpace) {
         halfSpaceVs(pa, sb, pb, margin, m);
@@ -726,23 +439,23 @@ bool collide(const World& w, uint32_t ia, const tfs_body& a, uint32_t ib, const 
         sphereSphere(pa.c, sa.radius, pb.c, sb.radius, margin, m);
     } else if (sb.kind == kSphere) {
         if (sa.kind == kBox) {
-            boxSphere(sa, pa, pb.c, sb.radius, margin, m);
+            boxSphere(sa.half, pa, pb.c, sb.radius, margin, m);
         } else {
             roundOrPrismSphere(sa, pa, pb.c, sb.radius, margin, m);
         }
     } else if (sa.kind == kSphere) {
         if (sb.kind == kBox) {
-            boxSphere(sb, pb, pa.c, sa.radius, margin, m);
+            boxSphere(sb.half, pb, pa.c, sa.radius, margin, m);
         } else {
             roundOrPrismSphere(sb, pb, pa.c, sa.radius, margin, m);
         }
         m.n = -m.n;
     } else {
         Profile fa, fb;
-        if (inPlane(a) && inPlane(b) && profileOf(sa, pa, fa) && profileOf(sb, pb, fb)) {
+        if (bothInPlane && profileOf(sa, pa, fa) && profileOf(sb, pb, fb)) {
             extruded(fa, fb, margin, m);
         } else if (sa.kind == kBox && sb.kind == kBox) {
-            boxBox(sa, pa, sb, pb, margin, m);
+            boxBox(sa.half, pa, sb.half, pb, margin, m);
         }  // a cylinder or prism against a shape not extruded along the world z: not supported
     }
     return m.count > 0;
@@ -755,27 +468,12 @@ void toContact(const World& w, uint32_t ia, const tfs_body& a, uint32_t ib, cons
     c.key = ((uint64_t)ia << 32) | ib;
     c.body_a = ia;
     c.body_b = ib;
-    const V n = unit(m.n);
-    const V t1 = perpendicular(n);
-    st(c.normal, n);
-    st(c.tangent1, t1);
-    st(c.tangent2, cross(n, t1));
     const Shape& sa = w.shapes[ia];
     const Shape& sb = w.shapes[ib];
     c.friction = std::sqrt(sa.friction * sb.friction);
     c.restitution = std::max(sa.restitution, sb.restitution);
     c.rolling_resistance = std::max(sa.rolling, sb.rolling);
-    c.count = m.count;
-    const Q qa = ldq(a.q);
-    for (int i = 0; i < m.count; ++i) {
-        tfs_point& p = c.points[i];
-        const V pos = m.p[i].pos;
-        st(p.local_a, unrotate(qa, pos - ld(a.p)));
-        p.separation = m.p[i].sep;
-        const V up = p.separation > w.settings.speculative ? n * (0.5f * p.separation) : V{ 0, 0, 0 };
-        st(p.anchor_a, pos - up - ld(a.com));
-        st(p.anchor_b, pos + up - ld(b.com));
-    }
+    contactGeometry(a, b, m, w.settings.speculative, c);
 }
 
 // carryImpulses: src/world.cpp:102-120.
@@ -1129,6 +827,23 @@ uint32_t tfw_collide(void* h, tfs_body* bodies, uint32_t nb, float dt, const tfs
     }
     if (woke) { *woke = wakes; }
     return n;
+}
+
+// The bodies' shapes for the device refresh (tfs_set_body_shapes), which collides spheres, boxes and
+// half-spaces with this world's narrowphase (narrow.h): the count, or -1 if a body's shape is a
+// cylinder or a prism. Writes at most `cap`.
+int tfw_body_shapes(void* h, tfs_shape* out, uint32_t cap) {
+    const World& w = *(const World*)h;
+    for (const Shape& s : w.shapes) {
+        if (s.kind != kSphere && s.kind != kBox && s.kind != kHalfSpace) { return -1; }
+    }
+    for (uint32_t i = 0; i < cap && i < w.shapes.size(); ++i) {
+        const Shape& s = w.shapes[i];
+        out[i].kind = (uint32_t)s.kind;
+        st(out[i].half, s.half);
+        out[i].radius = s.radius;
+    }
+    return (int)w.shapes.size();
 }
 
 // tfs_refresh_fn: collides the listed contacts' pairs again at the bodies' current poses.

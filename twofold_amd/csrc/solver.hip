@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "stages.h"
+#include "narrow.h"
 
 using namespace tf;
 
@@ -2129,6 +2130,38 @@ __global__ void k_gather_again(const uint32_t* again, uint32_t n, const tfs_cont
     }
 }
 
+// The narrowphase half of a refresh on the device: each listed pair collided again at its bodies'
+// current poses and velocities, with the host world's narrowphase (narrow.h), into fresh[k] as the
+// refresh callback would write it. A pair narrow.h does not collide raises error bit 16.
+__global__ void __launch_bounds__(kThreads) k_refresh_collide(const uint32_t* again, uint32_t n, const tfs_contact* contacts,
+                                                              const tfs_body* bodies, const float4* motion,
+                                                              const tfs_shape* shapes, float remaining, float speculative,
+                                                              tfs_contact* fresh, DevCounts* cnt) {
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        const tfs_contact& c = contacts[again[k]];
+        const uint32_t ia = c.body_a, ib = c.body_b;
+        const tfs_body& a = bodies[ia];
+        const tfs_body& b = bodies[ib];
+        const tfs_shape sa = shapes[ia], sb = shapes[ib];
+        const float4 va = motion[4 * ia], wa = motion[4 * ia + 1], vb = motion[4 * ib], wb = motion[4 * ib + 1];
+        const float margin = tfn::pairMargin(tfn::vec3(va.x, va.y, va.z), tfn::vec3(wa.x, wa.y, wa.z), a.extent,
+                                             tfn::vec3(vb.x, vb.y, vb.z), tfn::vec3(wb.x, wb.y, wb.z), b.extent, remaining,
+                                             speculative);
+        const tfn::Pose pa = tfn::poseOf(a.com, a.q), pb = tfn::poseOf(b.com, b.q);
+        const bool bothInPlane = tfn::inPlane(a) && tfn::inPlane(b);
+        tfn::Manifold m;
+        const int got = tfn::collideBasic((int)sa.kind, tfn::ld(sa.half), sa.radius, pa, (int)sb.kind, tfn::ld(sb.half),
+                                          sb.radius, pb, bothInPlane, margin, m);
+        tfs_contact& f = fresh[k];
+        f.count = 0;
+        if (got < 0) {
+            atomicOr(&cnt->error, 16u);
+        } else if (got > 0) {
+            tfn::contactGeometry(a, b, m, speculative, f);
+        }
+    }
+}
+
 // The solver side of each refreshed fast contact (src/world.cpp:1108-1138).
 __global__ void __launch_bounds__(kThreads) k_refresh_apply(const uint32_t* again, uint32_t n, const tfs_contact* fresh, tfs_contact* contacts,
                                 const tfs_body* bodies, const float4* motion, const float4* mass, float matchSq) {
@@ -2439,6 +2472,8 @@ struct tfs_ctx {
     tfs_contact* pinContacts = nullptr;
     uint32_t* pinAgain = nullptr;
     tfs_contact* pinFresh = nullptr;
+    tfs_shape* shapes = nullptr;    // one per body: the device refresh (tfs_set_body_shapes), or none
+    uint32_t nShapes = 0, shapesCap = 0;
     uint32_t pinCap = 0;
     // the fixed-contact model (tfs_fixed_*) and its adjoint (tfs_adjoint_*): the counts of the step
     // that fixed the contacts, one checkpoint per step taken, one step's tape, the adjoint state
@@ -2911,10 +2946,19 @@ int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32
     return 0;
 }
 
+// The error bits kernels raise: 16 a pair the device narrowphase does not collide, the others a
+// bounded wait that ran out.
+int kernelFail(tfs_ctx* ctx, uint32_t bits) {
+    if (bits & 16u) {
+        return fail(ctx, TFS_E_KERNEL, "the device refresh met two boxes that both stay extruded along z (error %u)", bits);
+    }
+    return fail(ctx, TFS_E_KERNEL, "a kernel's bounded wait ran out (error %u)", bits);
+}
+
 int syncCounts(tfs_ctx* ctx) {
     HIP_TRY(hipMemcpyAsync(ctx->hostCnt, ctx->cnt, sizeof(DevCounts), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (ctx->hostCnt->error) { return fail(ctx, TFS_E_KERNEL, "a kernel's bounded wait ran out (error %u)", ctx->hostCnt->error); }
+    if (ctx->hostCnt->error) { return kernelFail(ctx, ctx->hostCnt->error); }
     return 0;
 }
 
@@ -2925,6 +2969,8 @@ int growPinned(tfs_ctx* ctx, T*& p, size_t n) {
     HIP_TRY(hipHostMalloc((void**)&p, sizeof(T) * (n ? n : 1), hipHostMallocDefault));
     return 0;
 }
+
+int refreshApply(tfs_ctx* ctx, const uint32_t* again, uint32_t nAgain, uint32_t laned, uint32_t impactLaned, uint32_t nRest);
 
 int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaining, tfs_refresh_fn cb, void* user,
             const DevCounts& hc) {
@@ -2959,6 +3005,26 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
         }
         ctx->pinCap = cap;
     }
+    if (ctx->nShapes) {
+        // the pairs collided on the device: no transfer, no callback
+        LAUNCH(k_pack, blocksFor(laned), kThreads, ctx->rowsList, laned, ctx->contacts, ctx->moving, ctx->mass, ctx->rows,
+               ctx->rowOf);
+        size_t ev = 0;
+        if (ctx->profiling) {
+            ev = ctx->pending.size() * 2;
+            while (ctx->evPool.size() < ev + 2) {
+                hipEvent_t x;
+                HIP_TRY(hipEventCreate(&x));
+                ctx->evPool.push_back(x);
+            }
+            ctx->pending.push_back({ 4, ev, (double)nAgain });
+            HIP_TRY(hipEventRecord(ctx->evPool[ev], ctx->stream));
+        }
+        LAUNCH(k_refresh_collide, blocksFor(nAgain), kThreads, again, nAgain, ctx->contacts, ctx->bodies, ctx->motion,
+               ctx->shapes, remaining, ctx->settings.speculative, ctx->fresh, ctx->cnt);
+        if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
+        return refreshApply(ctx, again, nAgain, laned, impactLaned, nRest);
+    }
     const uint32_t waves = std::min<uint32_t>(nAgain, 4u * 8192u);
     LAUNCH(k_gather_again, (waves + 3) / 4, kThreads, again, nAgain, ctx->contacts, ctx->bodies, ctx->motion,
            ctx->pinAgain, ctx->pinContacts, ctx->pinBodies);
@@ -2984,6 +3050,11 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
         ctx->profBytes[4] += nAgain;
     }
     HIP_TRY(hipMemcpyAsync(ctx->fresh, ctx->pinFresh, sizeof(tfs_contact) * nAgain, hipMemcpyHostToDevice, ctx->stream));
+    return refreshApply(ctx, again, nAgain, laned, impactLaned, nRest);
+}
+
+// After the narrowphase of a refresh: the new manifolds into the contacts, their rows packed again.
+int refreshApply(tfs_ctx* ctx, const uint32_t* again, uint32_t nAgain, uint32_t laned, uint32_t impactLaned, uint32_t nRest) {
     const float matchSq = ctx->settings.speculative * ctx->settings.speculative;
     LAUNCH(k_refresh_apply, blocksFor(nAgain), kThreads, again, nAgain, ctx->fresh,
            ctx->contacts, ctx->bodies, ctx->motion, ctx->mass, matchSq);
@@ -3082,6 +3153,9 @@ void tuneRecord(tfs_ctx* ctx, float ms) {
 }
 
 int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
+    if (ctx->nShapes && ctx->nShapes != ctx->nb) {
+        return fail(ctx, TFS_E_STATE, "%u body shapes for %u bodies", ctx->nShapes, ctx->nb);
+    }
     if (ctx->tuning) { applyShape(ctx, ctx->tuneShapes[tuneCandidate(ctx)]); }
     const uint32_t nb = ctx->nb, nc = ctx->nc;
     const tfs_settings& s = ctx->settings;
@@ -3092,7 +3166,7 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     int r;
     if ((r = beginStep(ctx, dt, hc))) { return r; }
     const uint32_t laned = hc.colorStart[TFS_COLORS], impactLaned = hc.impactColorStart[TFS_COLORS];
-    if (n > 1 && !cb) {
+    if (n > 1 && !cb && !ctx->nShapes) {
         bool needs = false;
         for (int i = 1; i < n; ++i) { needs = needs || (i == n / 2 ? hc.nFast : hc.nFastSolid) > 0; }
         if (needs) { return fail(ctx, TFS_E_CALLBACK, "fast contacts need a refresh callback"); }
@@ -3111,7 +3185,7 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
                                hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        if (ctx->hostCnt->error) { return fail(ctx, TFS_E_KERNEL, "a kernel's bounded wait ran out (error %u)", ctx->hostCnt->error); }
+        if (ctx->hostCnt->error) { return kernelFail(ctx, ctx->hostCnt->error); }
         for (int w = 0; w < 2; ++w) {
             ctx->restLevels[w] = ctx->hostCnt->restLevels[w];
             const uint32_t narrow = ~ctx->hostCnt->restNarrow[w];  // ~0 -> no narrow level
@@ -3197,7 +3271,7 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
     HIP_TRY(hipEventSynchronize(ctx->ev[2]));
-    if (ctx->hostCnt->error) { return fail(ctx, TFS_E_KERNEL, "a kernel's bounded wait ran out (error %u)", ctx->hostCnt->error); }
+    if (ctx->hostCnt->error) { return kernelFail(ctx, ctx->hostCnt->error); }
     HIP_TRY(hipEventElapsedTime(&ctx->solveMs, ctx->ev[0], ctx->ev[1]));
     HIP_TRY(hipEventElapsedTime(&ctx->sleepMs, ctx->ev[1], ctx->ev[2]));
     if ((r = collectProfile(ctx))) { return r; }
@@ -3310,7 +3384,7 @@ int passAdjoint(tfs_ctx* ctx, int mode, int slot, const Ctx& c) {
 int checkErrors(tfs_ctx* ctx) {
     HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (ctx->hostCnt->error) { return fail(ctx, TFS_E_KERNEL, "a kernel's bounded wait ran out (error %u)", ctx->hostCnt->error); }
+    if (ctx->hostCnt->error) { return kernelFail(ctx, ctx->hostCnt->error); }
     return 0;
 }
 
@@ -3467,7 +3541,7 @@ void tfs_destroy(tfs_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) { (void)hipStreamSynchronize(ctx->stream); }
     void* ptrs[] = { ctx->bodies, ctx->motion, ctx->mass, ctx->moving, ctx->struck, ctx->deg, ctx->epStart, ctx->cursor,
-                     ctx->word, ctx->parent, ctx->islandTime, ctx->contacts, ctx->fresh, ctx->fActive, ctx->fFast,
+                     ctx->word, ctx->parent, ctx->islandTime, ctx->contacts, ctx->fresh, ctx->shapes, ctx->fActive, ctx->fFast,
                      ctx->fFastSolid, ctx->active, ctx->fast, ctx->fastSolid, ctx->impactFlag, ctx->impactPos,
                      ctx->impact, ctx->rowsList, ctx->impactRowsList, ctx->colors, ctx->ep, ctx->rank, ctx->rowOf,
                      ctx->hist, ctx->status, ctx->cnt, ctx->bar, ctx->rows.hdr, ctx->rows.nrm,
@@ -3627,6 +3701,28 @@ int tfs_set_pass_shape(tfs_ctx* ctx, uint32_t blocks, uint32_t tail_rows, int ma
 int tfs_set_cooperative(tfs_ctx* ctx, int on) {
     if (!ctx) { return TFS_E_ARG; }
     ctx->coopLaunch = on != 0;
+    return 0;
+}
+
+int tfs_set_body_shapes(tfs_ctx* ctx, uint32_t n, const tfs_shape* shapes) {
+    if (!ctx || (n && !shapes)) { return TFS_E_ARG; }
+    for (uint32_t i = 0; i < n; ++i) {
+        const tfs_shape& x = shapes[i];
+        const bool ok = x.kind == TFS_SHAPE_HALF_SPACE ||
+                        (x.kind == TFS_SHAPE_SPHERE && x.radius > 0.0f && x.radius <= 3.4e38f) ||
+                        (x.kind == TFS_SHAPE_BOX && x.half[0] > 0.0f && x.half[0] <= 3.4e38f && x.half[1] > 0.0f &&
+                         x.half[1] <= 3.4e38f && x.half[2] > 0.0f && x.half[2] <= 3.4e38f);
+        if (!ok) { return fail(ctx, TFS_E_ARG, "shape %u: kind %u with these sizes is not a sphere, box or half-space", i, x.kind); }
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) { return fail(ctx, TFS_E_DEVICE, "hipSetDevice failed"); }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (n > ctx->shapesCap) {
+        int r;
+        if ((r = grow(ctx, ctx->shapes, n))) { return r; }
+        ctx->shapesCap = n;
+    }
+    if (n) { HIP_TRY(hipMemcpy(ctx->shapes, shapes, sizeof(tfs_shape) * n, hipMemcpyHostToDevice)); }
+    ctx->nShapes = n;
     return 0;
 }
 

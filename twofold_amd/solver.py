@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 from . import _abi
-from ._abi import BODY_DTYPE, CONTACT_DTYPE, SETTINGS_DTYPE
+from ._abi import BODY_DTYPE, CONTACT_DTYPE, SETTINGS_DTYPE, SHAPE_DTYPE
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtwofold_hip.so")
 
@@ -25,7 +25,7 @@ EXPORTS = ("tfs_abi_version", "tfs_sizeof_body", "tfs_sizeof_contact", "tfs_size
            "tfs_upload_contacts", "tfs_set_worlds", "tfs_step", "tfs_download_bodies", "tfs_download_contacts", "tfs_download_list",
            "tfs_last_step_ms", "tfs_last_step_counts", "tfs_set_profiling", "tfs_read_profile", "tfs_fixed_begin",
            "tfs_fixed_step", "tfs_adjoint_seed", "tfs_adjoint_step", "tfs_adjoint_read", "tfs_check_minmax",
-           "tfs_autotune", "tfs_pass_shape", "tfs_set_pass_shape", "tfs_set_cooperative")
+           "tfs_autotune", "tfs_pass_shape", "tfs_set_pass_shape", "tfs_set_cooperative", "tfs_set_body_shapes")
 
 
 class SolverError(RuntimeError):
@@ -72,6 +72,7 @@ def load_library(path=LIB_PATH):
     lib.tfs_pass_shape.argtypes = [_vp, _vp]
     lib.tfs_set_pass_shape.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
     lib.tfs_set_cooperative.argtypes = [_vp, ctypes.c_int]
+    lib.tfs_set_body_shapes.argtypes = [_vp, ctypes.c_uint32, _vp]
     lib.tfs_read_profile.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_double)]
     if lib.tfs_abi_version() != 1:
@@ -253,6 +254,16 @@ class Solver:
     def set_cooperative(self, on):
         """Launch the grid-barrier passes cooperatively (True) or as plain kernels (False, the default)."""
         self._check(self.lib.tfs_set_cooperative(self.h, int(bool(on))), "tfs_set_cooperative")
+
+    def set_body_shapes(self, shapes):
+        """One shape per resident body (SHAPE_DTYPE): every refresh then collides its pairs on the
+        device, with the host world's sphere/box/half-space narrowphase, and no callback is called.
+        None or an empty array turns it off."""
+        if shapes is None or len(shapes) == 0:
+            self._check(self.lib.tfs_set_body_shapes(self.h, 0, None), "tfs_set_body_shapes")
+            return
+        shapes = np.ascontiguousarray(shapes, SHAPE_DTYPE)
+        self._check(self.lib.tfs_set_body_shapes(self.h, len(shapes), shapes.ctypes.data_as(_vp)), "tfs_set_body_shapes")
 
     def set_pass_shape(self, blocks, tail_rows, made_axes):
         self._check(self.lib.tfs_set_pass_shape(self.h, blocks, tail_rows, int(bool(made_axes))), "tfs_set_pass_shape")

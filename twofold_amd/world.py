@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 from . import _abi
-from ._abi import BODY_DTYPE, CONTACT_DTYPE, SETTINGS_DTYPE
+from ._abi import BODY_DTYPE, CONTACT_DTYPE, SETTINGS_DTYPE, SHAPE_DTYPE
 from .solver import REFRESH_FN, Solver, SolverError
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtwofold_scene.so")
@@ -44,6 +44,8 @@ def _load():
         lib.tfw_collide.restype = ctypes.c_uint32
         lib.tfw_collide.argtypes = [_vp, _vp, ctypes.c_uint32, ctypes.c_float, _vp, ctypes.c_uint32, _vp,
                                     ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        lib.tfw_body_shapes.restype = ctypes.c_int
+        lib.tfw_body_shapes.argtypes = [_vp, _vp, ctypes.c_uint32]
         _lib = lib
     return _lib
 
@@ -98,6 +100,16 @@ class HostWorld:
         self.lib.tfw_bodies(self.h, out.ctypes.data_as(_vp), n)
         return out
 
+    def body_shapes(self):
+        """The bodies' shapes (SHAPE_DTYPE) for the device refresh, or None if a body's shape is a
+        cylinder or a prism, which only this host world collides."""
+        n = self.lib.tfw_body_shapes(self.h, None, 0)
+        if n < 0:
+            return None
+        out = np.zeros(n, SHAPE_DTYPE)
+        self.lib.tfw_body_shapes(self.h, out.ctypes.data_as(_vp), n)
+        return out
+
     def collide(self, bodies, dt, prev=None):
         """Contacts at the bodies' poses, in key order, impulses carried from prev. May wake bodies in
         place; returns (contacts, woken)."""
@@ -118,7 +130,7 @@ class Simulation:
     """A scene (or a list of scenes: independent worlds solved together) stepped with the host
     narrowphase and the GPU solver."""
 
-    def __init__(self, scene, settings=None, device=0):
+    def __init__(self, scene, settings=None, device=0, device_refresh=False):
         self.world = HostWorld(scene, settings)
         self.settings = self.world.settings
         self.dt = self.world.dt
@@ -127,6 +139,12 @@ class Simulation:
         self.solver = Solver(len(self.bodies), max(4 * len(self.bodies), 64), device=device, settings=self.settings)
         self.upload(self.bodies, self.contacts)
         self.steps = 0
+        # device_refresh: mid-step refreshes collide on the device with the same narrowphase
+        # (spheres, boxes and half-spaces only; None keeps the host callback)
+        self.device_refresh = self.world.body_shapes() if device_refresh else None
+        if device_refresh and self.device_refresh is None:
+            raise SolverError("the device refresh collides spheres, boxes and half-spaces only")
+        self.solver.set_body_shapes(self.device_refresh)
 
     def upload(self, bodies, contacts):
         if self.world.worlds > 1:
