@@ -267,9 +267,10 @@ int tfs_last_step_ms(tfs_ctx* ctx, float* solve_ms, float* sleep_ms);
  * biased and relax passes including the impact repeats, 2 restitution) the library sums the launch
  * count, the event time in milliseconds and the algorithmic bytes of the rows launched (per contact
  * with n points: warm start 224 + 244n, solve 224 + 296n, restitution 224 + 96n). Two more families
- * time the host side of each refresh with the host clock: 3 the wait for the contacts collided again
- * to reach host memory, 4 the refresh callback itself (its "bytes" sum counts the pairs collided
- * again). Family 5 is the biased and relax passes over the contacts outside the colours (the rest
+ * time each refresh's narrowphase: through the callback, with the host clock, 3 the wait for the
+ * contacts collided again to reach host memory and 4 the callback itself; on the device
+ * (tfs_set_body_shapes), 4 the event time of the collision kernel and 3 nothing. Family 4's "bytes"
+ * sum counts the pairs collided again. Family 5 is the biased and relax passes over the contacts outside the colours (the rest
  * lists, active and impact), event-timed per pass over one list, with bytes per contact of 248 + 68n
  * (planar) or 224 + 296n (3D). Turning it on clears the sums; turning it off keeps them for reading.
  */
