@@ -996,6 +996,78 @@ __device__ __forceinline__ void loadRowData(const Rows& rows, uint32_t r, uint4 
     d.dqB = motion[4 * h.y + 3];
 }
 
+// rowStep over data already loaded (loadRowData): only what rows of the pass write (velocities,
+// impulses) is loaded here, coherently. The same operations on the same values as rowStep.
+template <int MODE, bool MADE, bool TAPE>
+__device__ __forceinline__ void rowSolveLoaded(const Rows& rows, uint32_t r, uint4 h, const RowData<MADE>& d, float4* motion,
+                                               const float4* mass, const Ctx& ctx, const Tape& tape) {
+    using X = Access<true>;
+    const int count = (int)(h.z & kCountMask);
+    if (count == 0) { return; }
+    const Row row{ h.z, vec(d.n.x, d.n.y, d.n.z), vec(d.t1.x, d.t1.y, d.t1.z), vec(d.t2.x, d.t2.y, d.t2.z), d.n.w, d.t1.w };
+    RowPoint p[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k < count) {
+            const uint32_t j = (uint32_t)k * rows.cap + r;
+            const float4 im = X::ld(&rows.imp[j]);
+            p[k].ra = vec(d.pa[k].x, d.pa[k].y, d.pa[k].z);
+            p[k].adj = d.pa[k].w;
+            p[k].rb = vec(d.pb[k].x, d.pb[k].y, d.pb[k].z);
+            p[k].rv = d.pb[k].w;
+            p[k].ni = im.x;
+            p[k].ti = vec(im.y, im.z, im.w);
+            p[k].nimax = X::ld1(&rows.peak[j]);
+        }
+    }
+    const float4 va = X::ld(&motion[4 * h.x]), wa = X::ld(&motion[4 * h.x + 1]);
+    const float4 vb = X::ld(&motion[4 * h.y]), wb = X::ld(&motion[4 * h.y + 1]);
+    Motion A{ vec(va.x, va.y, va.z), vec(wa.x, wa.y, wa.z), vec(d.dpA.x, d.dpA.y, d.dpA.z),
+              quat(d.dqA.x, d.dqA.y, d.dqA.z, d.dqA.w) };
+    Motion B{ vec(vb.x, vb.y, vb.z), vec(wb.x, wb.y, wb.z), vec(d.dpB.x, d.dpB.y, d.dpB.z),
+              quat(d.dqB.x, d.dqB.y, d.dqB.z, d.dqB.w) };
+    if (TAPE && tape.v) {
+        const uint32_t c = tape.cap;
+        tape.v[r] = va;
+        tape.v[c + r] = wa;
+        tape.v[2 * c + r] = vb;
+        tape.v[3 * c + r] = wb;
+        const V3 dp = B.dp - A.dp;
+        float sp[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (k < count) {
+                tape.imp[(uint32_t)k * c + r] = make_float4(p[k].ni, p[k].ti.x, p[k].ti.y, p[k].ti.z);
+                sp[k] = pointSeparation(A, B, dp, p[k], row.n);
+            }
+        }
+        tape.sep[r] = make_float4(sp[0], sp[1], sp[2], sp[3]);
+    }
+    if (MADE) {
+        const RowMadeAxes ax{ loadMass(mass, h.x), loadMass(mass, h.y), row, p };
+        tf::rows<MODE>(row, p, ax, A, B, ctx);
+    } else {
+        const RegAxes<MADE ? 1 : 15> ax{ d.axv };
+        tf::rows<MODE>(row, p, ax, A, B, ctx);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k < count) {
+            const uint32_t j = (uint32_t)k * rows.cap + r;
+            X::st(&rows.imp[j], make_float4(p[k].ni, p[k].ti.x, p[k].ti.y, p[k].ti.z));
+            X::st1(&rows.peak[j], p[k].nimax);
+        }
+    }
+    if (h.z & kMoveA) {
+        X::st(&motion[4 * h.x], make_float4(A.v.x, A.v.y, A.v.z, 0.0f));
+        X::st(&motion[4 * h.x + 1], make_float4(A.w.x, A.w.y, A.w.z, 0.0f));
+    }
+    if (h.z & kMoveB) {
+        X::st(&motion[4 * h.y], make_float4(B.v.x, B.v.y, B.v.z, 0.0f));
+        X::st(&motion[4 * h.y + 1], make_float4(B.w.x, B.w.y, B.w.z, 0.0f));
+    }
+}
+
 __device__ __forceinline__ uint32_t tailSlot(uint32_t* key, uint32_t body) {
     uint32_t k = (body * 2654435761u) >> 23;   // 9 bits: kTailSlots = 512
     for (uint32_t probe = 0; probe < kTailSlots; ++probe, k = (k + 1) & (kTailSlots - 1)) {
@@ -1169,8 +1241,9 @@ __device__ __forceinline__ void passTail(const Rows& rows, const uint4* impactHd
     }
 }
 
-// Headers do not change during a pass, so each thread loads the header of its first row in the
-// next segment before the barrier; after the barrier every other load of that row goes out at once.
+// Headers, directions, points, axes and the bodies' dp and dq do not change during a pass, so each
+// thread loads them for its first row of the next segment while it waits at the barrier; after the
+// barrier only the velocities and impulses that earlier segments wrote are loaded.
 // Segments from plan.tailFrom on are the tail (passTail).
 template <int MODE, bool MADE, bool TAPE>
 __global__ void __launch_bounds__(kPassThreads) k_pass(Rows rows, const uint4* impactHdr, PassPlan plan,
@@ -1182,18 +1255,31 @@ __global__ void __launch_bounds__(kPassThreads) k_pass(Rows rows, const uint4* i
     auto header = [&](uint32_t s, uint32_t i) { return s >= plan.impactFrom ? impactHdr[i] : rows.hdr[i]; };
     const uint32_t grid = plan.tailFrom;
     uint4 next = make_uint4(0u, 0u, 0u, 0u);
-    if (grid > 0 && plan.begin[0] + g < plan.end[0]) { next = header(0, plan.begin[0] + g); }
+    RowData<MADE> nd;
+    if (grid > 0 && plan.begin[0] + g < plan.end[0]) {
+        next = header(0, plan.begin[0] + g);
+        loadRowData<MADE>(rows, 0 >= plan.impactFrom ? next.w : plan.begin[0] + g, next, motion, nd);
+    }
     for (uint32_t s = 0; s < grid; ++s) {
         const bool impact = s >= plan.impactFrom;
         const uint32_t first = plan.begin[s] + g;
         const uint4 firstHdr = next;
-        if (s + 1 < grid && plan.begin[s + 1] + g < plan.end[s + 1]) { next = header(s + 1, plan.begin[s + 1] + g); }
+        const bool more = s + 1 < grid && plan.begin[s + 1] + g < plan.end[s + 1];
+        if (more) { next = header(s + 1, plan.begin[s + 1] + g); }
         for (uint32_t i = first; i < plan.end[s]; i += stride) {
-            const uint4 h = i == first ? firstHdr : header(s, i);
-            rowStep<MODE, true, MADE, TAPE>(rows, impact ? h.w : i, h, motion, mass, ctx, impact ? Tape{} : tape);
+            uint4 h = firstHdr;
+            if (i != first) {
+                h = header(s, i);
+                loadRowData<MADE>(rows, impact ? h.w : i, h, motion, nd);
+            }
+            rowSolveLoaded<MODE, MADE, TAPE>(rows, impact ? h.w : i, h, nd, motion, mass, ctx, impact ? Tape{} : tape);
         }
         if (s + 1 < grid) {
-            gridSync(bar, phase, cnt);
+            // The next segment's first row: what no row of the pass writes is loaded between arriving
+            // at the barrier (which waits for this thread's memory operations) and waiting at it.
+            gridArrive(bar, phase);
+            if (more) { loadRowData<MADE>(rows, s + 1 >= plan.impactFrom ? next.w : plan.begin[s + 1] + g, next, motion, nd); }
+            gridWait(bar, phase, cnt);
         } else if (grid < plan.n) {
             gridArrive(bar, phase);
         }
