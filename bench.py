@@ -257,7 +257,7 @@ def run_batched(args):
     launches, ms, algo = sim.solver.profile(1)
     _, wait_ms, _ = sim.solver.profile(3)
     refreshes, narrow_ms, pairs = sim.solver.profile(4)
-    family_ms = [sim.solver.profile(f)[1] / args.batch_profile_steps for f in range(6)]
+    family_ms = [sim.solver.profile(f)[1] / max(args.batch_profile_steps, 1) for f in range(6)]
     achieved = algo / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     log(f"solve passes: {launches} launches, {ms:.2f} ms, {algo / max(launches, 1) / 1e6:.2f} MB per launch; "
         f"ms per step by profile family: {', '.join(f'{x:.2f}' for x in family_ms)}")
@@ -317,10 +317,10 @@ def run_batched(args):
             "worlds": len(worlds), "bodies": len(sim.bodies), "contacts_per_step": contacts / args.steps,
             "looping_world_steps_per_s": looping, "looping_sample_worlds": len(sample),
             "batched_over_looping": batched / looping, "pass_shape": shape,
-            "refreshes_per_step": refreshes / args.batch_profile_steps,
-            "refresh_pairs_per_step": pairs / args.batch_profile_steps,
-            "refresh_wait_ms_per_step": wait_ms / args.batch_profile_steps,
-            "refresh_narrowphase_ms_per_step": narrow_ms / args.batch_profile_steps,
+            "refreshes_per_step": refreshes / max(args.batch_profile_steps, 1),
+            "refresh_pairs_per_step": pairs / max(args.batch_profile_steps, 1),
+            "refresh_wait_ms_per_step": wait_ms / max(args.batch_profile_steps, 1),
+            "refresh_narrowphase_ms_per_step": narrow_ms / max(args.batch_profile_steps, 1),
             "refresh": "device" if sim.device_refresh is not None else "host callback",
         },
         "roofline": {
@@ -576,10 +576,10 @@ def main():
             "world_steps_per_s": args.steps / wall, "gpu_solve_ms_per_step": solve_ms / args.steps,
             "transfer_ms_per_step": 1e3 * transfers / args.steps,
             "value_with_transfers": its / (wall + transfers), "pass_shape": shape,
-            "refreshes_per_step": refreshes / args.profile_steps,
-            "refresh_pairs_per_step": pairs / args.profile_steps,
-            "refresh_wait_ms_per_step": wait_ms / args.profile_steps,
-            "refresh_narrowphase_ms_per_step": narrow_ms / args.profile_steps,
+            "refreshes_per_step": refreshes / max(args.profile_steps, 1),
+            "refresh_pairs_per_step": pairs / max(args.profile_steps, 1),
+            "refresh_wait_ms_per_step": wait_ms / max(args.profile_steps, 1),
+            "refresh_narrowphase_ms_per_step": narrow_ms / max(args.profile_steps, 1),
             "refresh": "device" if sim.device_refresh is not None else "host callback",
         },
         "roofline": {

@@ -16,9 +16,11 @@
 #if defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define TFN_HD __host__ __device__ __forceinline__
+#define TFN_UNROLL _Pragma("unroll")
 #else
 #include <math.h>
 #define TFN_HD inline
+#define TFN_UNROLL
 #endif
 
 #include "../../include/twofold_solver.h"
@@ -98,46 +100,101 @@ TFN_HD Pose poseOf(const float* com, const float* q) {
     return p;
 }
 
-// Keeps at most four points: the deepest, the farthest from it, and the two farthest on either side
-// of the line between them.
-TFN_HD void reduce(const Point* pts, int n, V nrm, Manifold& m) {
+// Choices between values, made on the values: a conditional between two named objects would pick an
+// address, which puts both objects in memory on the device.
+TFN_HD float pickf(bool c, float x, float y) { return c ? x : y; }
+TFN_HD V pickv(bool c, V x, V y) { return vec3(pickf(c, x.x, y.x), pickf(c, x.y, y.y), pickf(c, x.z, y.z)); }
+TFN_HD Point pickp(bool c, Point x, Point y) { return Point{ pickv(c, x.pos, y.pos), pickf(c, x.sep, y.sep) }; }
+TFN_HD Pose pickpose(bool c, const Pose& x, const Pose& y) {
+    Pose r;
+    r.c = pickv(c, x.c, y.c);
+    r.q = Q{ pickf(c, x.q.x, y.q.x), pickf(c, x.q.y, y.q.y), pickf(c, x.q.z, y.q.z), pickf(c, x.q.w, y.q.w) };
+    r.ax[0] = pickv(c, x.ax[0], y.ax[0]);
+    r.ax[1] = pickv(c, x.ax[1], y.ax[1]);
+    r.ax[2] = pickv(c, x.ax[2], y.ax[2]);
+    return r;
+}
+// Element i of three.
+TFN_HD V of3(const V (&a)[3], int i) { return pickv(i == 0, a[0], pickv(i == 1, a[1], a[2])); }
+TFN_HD float of3(const float (&a)[3], int i) { return pickf(i == 0, a[0], pickf(i == 1, a[1], a[2])); }
+
+// Small arrays stay in registers on the device when every index is a constant of an unrolled loop:
+// writes at a running count go through put, and loops run to the array's size with a guard.
+template <class T, int N>
+TFN_HD void put(T (&a)[N], int i, const T& v) {
+    TFN_UNROLL
+    for (int j = 0; j < N; ++j) {
+        if (j == i) { a[j] = v; }
+    }
+}
+
+// Keeps at most four points of the first n: the deepest, the farthest from it, and the two farthest
+// on either side of the line between them.
+template <int N>
+TFN_HD void reduce(const Point (&pts)[N], int n, V nrm, Manifold& m) {
     if (n <= 4) {
         m.count = n;
-        for (int i = 0; i < n; ++i) { m.p[i] = pts[i]; }
+        TFN_UNROLL
+        for (int i = 0; i < 4; ++i) {
+            if (i < n) { m.p[i] = pts[i]; }
+        }
         return;
     }
     int a = 0;
-    for (int i = 1; i < n; ++i) {
-        if (pts[i].sep < pts[a].sep) { a = i; }
+    Point pa = pts[0];
+    TFN_UNROLL
+    for (int i = 1; i < N; ++i) {
+        if (i < n && pts[i].sep < pa.sep) {
+            a = i;
+            pa = pts[i];
+        }
     }
     int b = a == 0 ? 1 : 0;
+    Point pb = pickp(a == 0, pts[1], pts[0]);
     float best = -1.0f;
-    for (int i = 0; i < n; ++i) {
-        const V d = pts[i].pos - pts[a].pos;
+    TFN_UNROLL
+    for (int i = 0; i < N; ++i) {
+        if (i >= n) { continue; }
+        const V d = pts[i].pos - pa.pos;
         if (i != a && dot(d, d) > best) {
             best = dot(d, d);
             b = i;
+            pb = pts[i];
         }
     }
-    const V ab = pts[b].pos - pts[a].pos;
+    const V ab = pb.pos - pa.pos;
     int c = -1, d = -1;
+    Point pc = pa, pd = pa;
     float up = 0.0f, down = 0.0f;
-    for (int i = 0; i < n; ++i) {
-        if (i == a || i == b) { continue; }
-        const float s = dot(cross(ab, pts[i].pos - pts[a].pos), nrm);
+    TFN_UNROLL
+    for (int i = 0; i < N; ++i) {
+        if (i >= n || i == a || i == b) { continue; }
+        const float s = dot(cross(ab, pts[i].pos - pa.pos), nrm);
         if (s > up) {
             up = s;
             c = i;
+            pc = pts[i];
         }
         if (s < down) {
             down = s;
             d = i;
+            pd = pts[i];
         }
     }
-    const int keep[4] = { a, b, c, d };
-    m.count = 0;
-    for (int k = 0; k < 4; ++k) {
-        if (keep[k] >= 0) { m.p[m.count++] = pts[keep[k]]; }
+    m.p[0] = pa;
+    m.p[1] = pb;
+    m.count = 2;
+    if (c >= 0) {
+        m.p[2] = pc;
+        m.count = 3;
+    }
+    if (d >= 0) {
+        if (m.count == 3) {
+            m.p[3] = pd;
+        } else {
+            m.p[2] = pd;
+        }
+        m.count += 1;
     }
 }
 
@@ -169,22 +226,30 @@ TFN_HD void halfSpaceSphere(const Pose& a, V cb, float r, float margin, Manifold
 
 constexpr int kMaxHalfSpacePoints = 32;
 
-// Half-space A against the vertices of a convex shape: every vertex within the margin, reduced.
-TFN_HD void halfSpacePoints(const Pose& a, const V* verts, int nv, float margin, Manifold& m) {
+// Half-space A against the first nv vertices of a convex shape: every vertex within the margin,
+// reduced.
+template <int NV>
+TFN_HD void halfSpacePoints(const Pose& a, const V (&verts)[NV], int nv, float margin, Manifold& m) {
     const V n = a.ax[2];
     const float offset = dot(n, a.c);
     m.count = 0;
     m.n = n;
-    Point pts[kMaxHalfSpacePoints];
+    Point pts[NV];
     int np = 0;
-    for (int i = 0; i < nv && np < kMaxHalfSpacePoints; ++i) {
+    TFN_UNROLL
+    for (int i = 0; i < NV; ++i) {
+        if (i >= nv) { continue; }
         const float sep = dot(n, verts[i]) - offset;
-        if (sep <= margin) { pts[np++] = Point{ verts[i] - n * (0.5f * sep), sep }; }
+        if (sep <= margin) {
+            put(pts, np, Point{ verts[i] - n * (0.5f * sep), sep });
+            ++np;
+        }
     }
     reduce(pts, np, n, m);
 }
 
-TFN_HD void boxVertices(const Pose& b, V half, V* out) {
+TFN_HD void boxVertices(const Pose& b, V half, V (&out)[8]) {
+    TFN_UNROLL
     for (int k = 0; k < 8; ++k) {
         out[k] = b.c + b.ax[0] * ((k & 1) ? half.x : -half.x) + b.ax[1] * ((k & 2) ? half.y : -half.y) +
                  b.ax[2] * ((k & 4) ? half.z : -half.z);
@@ -236,17 +301,26 @@ TFN_HD void boxSphere(V h, const Pose& a, V cs, float r, float margin, Manifold&
 // beyond that only takes vertices that rounding adds near a plane, and both builds drop the same ones.
 constexpr int kMaxPolygon = 16;
 
-// Clips a convex polygon against the plane dot(n, x) <= d.
-TFN_HD void clip(V* poly, int& k, V n, float d) {
-    V out[kMaxPolygon];
+// Clips the convex polygon of the first k vertices against the plane dot(n, x) <= d.
+TFN_HD void clip(V (&poly)[kMaxPolygon], int& k, V n, float d) {
+    V out[kMaxPolygon] = {};
     int m = 0;
-    for (int i = 0; i < k; ++i) {
-        const V p = poly[i], q = poly[(i + 1) % k];
+    TFN_UNROLL
+    for (int i = 0; i < kMaxPolygon; ++i) {
+        if (i >= k) { continue; }
+        const V p = poly[i], q = pickv(i + 1 < k, poly[(i + 1) % kMaxPolygon], poly[0]);
         const float dp = dot(n, p) - d, dq = dot(n, q) - d;
-        if (dp <= 0 && m < kMaxPolygon) { out[m++] = p; }
-        if (((dp < 0 && dq > 0) || (dp > 0 && dq < 0)) && m < kMaxPolygon) { out[m++] = p + (q - p) * (dp / (dp - dq)); }
+        if (dp <= 0 && m < kMaxPolygon) {
+            put(out, m, p);
+            ++m;
+        }
+        if (((dp < 0 && dq > 0) || (dp > 0 && dq < 0)) && m < kMaxPolygon) {
+            put(out, m, p + (q - p) * (dp / (dp - dq)));
+            ++m;
+        }
     }
-    for (int i = 0; i < m; ++i) { poly[i] = out[i]; }
+    TFN_UNROLL
+    for (int i = 0; i < kMaxPolygon; ++i) { poly[i] = out[i]; }
     k = m;
 }
 
@@ -263,8 +337,9 @@ TFN_HD void boxBox(V halfA, const Pose& a, V halfB, const Pose& b, float margin,
     float bestFace = -1e30f, bestEdge = -1e30f;
     int faceAxis = -1, edgeI = -1, edgeJ = -1;
     V faceN = vec3(0, 0, 0), edgeN = vec3(0, 0, 0);
+    TFN_UNROLL
     for (int i = 0; i < 6; ++i) {
-        const V l = i < 3 ? a.ax[i] : b.ax[i - 3];
+        const V l = i < 3 ? of3(a.ax, i) : of3(b.ax, i - 3);
         const float sep = fabsf(dot(t, l)) - extentAlong(a, ha, l) - extentAlong(b, hb, l);
         if (sep > margin) { return; }
         if (sep > bestFace) {
@@ -273,9 +348,11 @@ TFN_HD void boxBox(V halfA, const Pose& a, V halfB, const Pose& b, float margin,
             faceN = dot(t, l) >= 0 ? l : -l;
         }
     }
+    TFN_UNROLL
     for (int i = 0; i < 3; ++i) {
+        TFN_UNROLL
         for (int j = 0; j < 3; ++j) {
-            V l = cross(a.ax[i], b.ax[j]);
+            V l = cross(of3(a.ax, i), of3(b.ax, j));
             const float l2 = dot(l, l);
             if (l2 < 1e-6f) { continue; }
             l = l * (1.0f / sqrtf(l2));
@@ -292,18 +369,20 @@ TFN_HD void boxBox(V halfA, const Pose& a, V halfB, const Pose& b, float margin,
     if (edgeI >= 0 && bestEdge > 0.95f * bestFace + 0.01f) {
         // Two edges: the edge of A farthest along n and the edge of B farthest along -n.
         V pa = a.c, pb = b.c;
+        TFN_UNROLL
         for (int k = 0; k < 3; ++k) {
-            if (k != edgeI) { pa = pa + a.ax[k] * (dot(a.ax[k], edgeN) >= 0 ? ha[k] : -ha[k]); }
-            if (k != edgeJ) { pb = pb + b.ax[k] * (dot(b.ax[k], edgeN) >= 0 ? -hb[k] : hb[k]); }
+            if (k != edgeI) { pa = pa + of3(a.ax, k) * (dot(of3(a.ax, k), edgeN) >= 0 ? of3(ha, k) : -of3(ha, k)); }
+            if (k != edgeJ) { pb = pb + of3(b.ax, k) * (dot(of3(b.ax, k), edgeN) >= 0 ? -of3(hb, k) : of3(hb, k)); }
         }
-        const V da = a.ax[edgeI], db = b.ax[edgeJ];
+        const V da = of3(a.ax, edgeI), db = of3(b.ax, edgeJ);
         const V r = pa - pb;
         const float e = dot(db, db), f = dot(db, r), c = dot(da, r), bb = dot(da, db);
         const float den = 1.0f - bb * bb;
         float s = den > 1e-9f ? (bb * f - c * e) / den : 0.0f;
-        s = hi(-ha[edgeI], lo(ha[edgeI], s));
+        const float hae = of3(ha, edgeI), hbe = of3(hb, edgeJ);
+        s = hi(-hae, lo(hae, s));
         float u = (bb * s + f) / e;
-        u = hi(-hb[edgeJ], lo(hb[edgeJ], u));
+        u = hi(-hbe, lo(hbe, u));
         const V qa = pa + da * s, qb = pb + db * u;
         const float sep = dot(qb - qa, edgeN);
         if (sep > margin) { return; }
@@ -313,39 +392,50 @@ TFN_HD void boxBox(V halfA, const Pose& a, V halfB, const Pose& b, float margin,
         return;
     }
     const bool refA = faceAxis < 3;
-    const Pose& R = refA ? a : b;
-    const Pose& I = refA ? b : a;
-    const float* hr = refA ? ha : hb;
-    const float* hn = refA ? hb : ha;
+    // (copies, not references: a reference chosen at run time would put both boxes in memory)
+    const Pose R = pickpose(refA, a, b);
+    const Pose I = pickpose(refA, b, a);
+    const float hr[3] = { pickf(refA, ha[0], hb[0]), pickf(refA, ha[1], hb[1]), pickf(refA, ha[2], hb[2]) };
+    const float hn[3] = { pickf(refA, hb[0], ha[0]), pickf(refA, hb[1], ha[1]), pickf(refA, hb[2], ha[2]) };
     const int ri = refA ? faceAxis : faceAxis - 3;
     const V rn = refA ? faceN : -faceN;  // the reference face's outward normal, toward the other box
     int ii = 0;
     float most = -1.0f;
+    TFN_UNROLL
     for (int k = 0; k < 3; ++k) {
-        const float d = fabsf(dot(I.ax[k], rn));
+        const float d = fabsf(dot(of3(I.ax, k), rn));
         if (d > most) {
             most = d;
             ii = k;
         }
     }
-    const V in = dot(I.ax[ii], rn) > 0 ? -I.ax[ii] : I.ax[ii];  // incident face normal, facing the reference face
+    const V iax = of3(I.ax, ii);
+    const V in = dot(iax, rn) > 0 ? -iax : iax;  // incident face normal, facing the reference face
     const int i1 = (ii + 1) % 3, i2 = (ii + 2) % 3;
-    const V fc = I.c + in * hn[ii];
-    V poly[kMaxPolygon] = { fc + I.ax[i1] * hn[i1] + I.ax[i2] * hn[i2], fc - I.ax[i1] * hn[i1] + I.ax[i2] * hn[i2],
-                            fc - I.ax[i1] * hn[i1] - I.ax[i2] * hn[i2], fc + I.ax[i1] * hn[i1] - I.ax[i2] * hn[i2] };
+    const V ax1 = of3(I.ax, i1), ax2 = of3(I.ax, i2);
+    const float h1 = of3(hn, i1), h2 = of3(hn, i2);
+    const V fc = I.c + in * of3(hn, ii);
+    V poly[kMaxPolygon] = { fc + ax1 * h1 + ax2 * h2, fc - ax1 * h1 + ax2 * h2, fc - ax1 * h1 - ax2 * h2,
+                            fc + ax1 * h1 - ax2 * h2 };
+    // the side planes of the reference face: its two other axes in order, each both ways
     int k = 4;
-    for (int j = 0; j < 3 && k > 0; ++j) {
-        if (j == ri) { continue; }
-        clip(poly, k, R.ax[j], dot(R.ax[j], R.c) + hr[j]);
-        clip(poly, k, -R.ax[j], -dot(R.ax[j], R.c) + hr[j]);
-    }
+    const int j0 = ri == 0 ? 1 : 0, j1 = ri == 2 ? 1 : 2;
+    const V r0 = of3(R.ax, j0), r1 = of3(R.ax, j1);
+    const float h0 = of3(hr, j0), h1r = of3(hr, j1);
+    clip(poly, k, r0, dot(r0, R.c) + h0);
+    clip(poly, k, -r0, -dot(r0, R.c) + h0);
+    clip(poly, k, r1, dot(r1, R.c) + h1r);
+    clip(poly, k, -r1, -dot(r1, R.c) + h1r);
     Point pts[kMaxPolygon];
     int np = 0;
-    const float plane = dot(rn, R.c) + hr[ri];
-    for (int i = 0; i < k; ++i) {
+    const float plane = dot(rn, R.c) + of3(hr, ri);
+    TFN_UNROLL
+    for (int i = 0; i < kMaxPolygon; ++i) {
+        if (i >= k) { continue; }
         const float sep = dot(rn, poly[i]) - plane;
         if (sep > margin) { continue; }
-        pts[np++] = Point{ poly[i] - rn * (0.5f * sep), sep };
+        put(pts, np, Point{ poly[i] - rn * (0.5f * sep), sep });
+        ++np;
     }
     m.n = faceN;
     reduce(pts, np, faceN, m);
@@ -409,7 +499,9 @@ TFN_HD void contactGeometry(const tfs_body& a, const tfs_body& b, const Manifold
     st(c.tangent2, cross(n, t1));
     c.count = m.count;
     const Q qa = ldq(a.q);
-    for (int i = 0; i < m.count; ++i) {
+    TFN_UNROLL
+    for (int i = 0; i < 4; ++i) {
+        if (i >= m.count) { continue; }
         tfs_point& p = c.points[i];
         const V pos = m.p[i].pos;
         st(p.local_a, unrotate(qa, pos - ld(a.p)));

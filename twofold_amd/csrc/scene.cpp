@@ -54,8 +54,9 @@ void halfSpaceVs(const Pose& a, const Shape& sb, const Pose& b, float margin, Ma
     V verts[kMaxHalfSpacePoints];
     int nv = 0;
     if (sb.kind == kBox) {
-        boxVertices(b, sb.half, verts);
-        nv = 8;
+        V box[8];
+        boxVertices(b, sb.half, box);
+        for (const V& v : box) { verts[nv++] = v; }
     } else if (sb.kind == kPrism) {
         for (size_t i = 0; i + 1 < sb.poly.size(); i += 2) {
             const V xy = b.c + b.ax[0] * sb.poly[i] + b.ax[1] * sb.poly[i + 1];
