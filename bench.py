@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 # Memory-side bytes per k_pass launch on this workload, from two rocprofv3 --pmc runs of this script
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 per dispatch, the factors from a 256 MiB copy measured in the same
 # runs, --copy-mib 256); counters cannot be read from inside the process, so the figure is carried here.
-K_PASS_TRAFFIC_BYTES = 100.6e6
+K_PASS_TRAFFIC_BYTES = 102.4e6
 K_PASS_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, mean of the last 32 k_pass dispatches, scaled by a "
                          "256 MiB copy (profiles/k_pass_pile32k_traffic.md)")
 
