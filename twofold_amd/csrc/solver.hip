@@ -41,6 +41,7 @@ constexpr int kProfileFamilies = 6;
 constexpr int kKeys = 5 * TFS_COLORS + 1; // row-order buckets: colour x point count, then the rest
 constexpr uint32_t kSpinLimit = 1u << 26;
 constexpr int kIslandRounds = 64;      // hook-and-jump rounds of the island pass before it gives up
+constexpr int kIslandAhead = 6;        // rounds launched before the host checks the first time
 
 struct DevCounts {
     uint32_t nActive, nFast, nFastSolid, nImpact;
@@ -2347,7 +2348,10 @@ __device__ __forceinline__ uint32_t rootOf(const uint32_t* parent, uint32_t x) {
 // Links the islands of two awake dynamic bodies in contact. Every link points from a larger index
 // to a smaller one, so each island ends with its lowest body index as root, as the reference's
 // union by the smaller root gives (src/world.cpp:1193-1194, include/twofold/world.h:464-465).
-__global__ void k_sleep_hook(const uint2* edges, uint32_t nc, uint32_t* parent, uint32_t* changed) {
+// A round given `prev` runs only if the round before it linked something (*prev != 0): rounds after
+// one that links nothing change nothing, so they can be launched ahead of the check.
+__global__ void k_sleep_hook(const uint2* edges, uint32_t nc, uint32_t* parent, uint32_t* changed, const uint32_t* prev) {
+    if (prev && *prev == 0u) { return; }
     bool linked = false;
     for (uint32_t k = gid(); k < nc; k += gstride()) {
         const uint2 e = edges[k];
@@ -2360,13 +2364,16 @@ __global__ void k_sleep_hook(const uint2* edges, uint32_t nc, uint32_t* parent, 
     if (__ballot(linked) != 0ull && (threadIdx.x % 64) == 0) { *changed = 1; }
 }
 
-__global__ void k_sleep_jump(uint32_t* parent, uint32_t n) {
+__global__ void k_sleep_jump(uint32_t* parent, uint32_t n, const uint32_t* prev) {
+    if (prev && *prev == 0u) { return; }
     for (uint32_t i = gid(); i < n; i += gstride()) { parent[i] = rootOf(parent, i); }
 }
 
 // Island ids are the lowest body index of the island; with several worlds, local to the body's world.
+// Given `unsettled`, both run only if that round linked nothing (*unsettled == 0).
 __global__ void k_island_time(tfs_body* bodies, uint32_t n, const uint32_t* parent, uint32_t* islandTime,
-                              const uint32_t* bodyWorld, const uint32_t* worldStart) {
+                              const uint32_t* bodyWorld, const uint32_t* worldStart, const uint32_t* unsettled) {
+    if (unsettled && *unsettled != 0u) { return; }
     // Sleep times are never negative, so their bits order as the floats do. A large island fills whole
     // waves: its lanes take their minimum across the wave first and make one atomic between them.
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gstride()) {
@@ -2394,7 +2401,8 @@ __global__ void k_island_time(tfs_body* bodies, uint32_t n, const uint32_t* pare
 }
 
 __global__ void k_sleep_apply(tfs_body* bodies, uint32_t n, const uint32_t* parent, const uint32_t* islandTime,
-                              float timeToSleep, float4* motion, float4* mass) {
+                              float timeToSleep, float4* motion, float4* mass, const uint32_t* unsettled) {
+    if (unsettled && *unsettled != 0u) { return; }
     for (uint32_t i = gid(); i < n; i += gstride()) {
         tfs_body& b = bodies[i];
         if (!b.alive || !awakeDynamic(b)) { continue; }
@@ -3333,30 +3341,49 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     LAUNCH(k_sleep_time, blocksFor(nb), kThreads, ctx->bodies, nb, dt, s.sleep_threshold * s.length_unit, ctx->parent,
            ctx->islandTime);
     LAUNCH(k_sleep_kinematic, blocksFor(nc), kThreads, ctx->contacts, nc, ctx->bodies, ctx->edges);
-    // Hook-and-jump rounds until one links nothing; rounds after that change nothing, so they are
-    // launched a few at a time and only the last round of each batch is checked on the host.
-    for (int it = 0;;) {
-        const int batch = it == 0 ? 4 : 2;
-        if (it + batch > kIslandRounds) {
-            return fail(ctx, TFS_E_KERNEL, "the island pass did not settle in %d rounds", kIslandRounds);
-        }
-        for (int j = 0; j < batch; ++j, ++it) {
-            LAUNCH(k_sleep_hook, blocksFor(nc), kThreads, ctx->edges, nc, ctx->parent, &ctx->cnt->changed[it]);
-            LAUNCH(k_sleep_jump, blocksFor(nb), kThreads, ctx->parent, nb);
-        }
-        HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->changed[it - 1], &ctx->cnt->changed[it - 1], sizeof(uint32_t),
-                               hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        if (!ctx->hostCnt->changed[it - 1]) { break; }
-    }
+    // Hook-and-jump rounds until one links nothing. Four rounds, then two that run only if the round
+    // before them linked something, then the island times and the sleep, which run only if the last
+    // round linked nothing: no host check in between. In the rare step whose islands need more rounds,
+    // the host sees that at the step's end and goes on two rounds at a time, checking each pair.
     const bool worlds = ctx->nWorlds > 1;
-    LAUNCH(k_island_time, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime,
-           worlds ? ctx->bodyWorld : nullptr, worlds ? ctx->worldStart : nullptr);
+    const uint32_t* bw = worlds ? ctx->bodyWorld : nullptr;
+    const uint32_t* ws = worlds ? ctx->worldStart : nullptr;
+    int it = 0;
+    for (; it < kIslandAhead; ++it) {
+        const uint32_t* prev = it >= 4 ? &ctx->cnt->changed[it - 1] : nullptr;
+        LAUNCH(k_sleep_hook, blocksFor(nc), kThreads, ctx->edges, nc, ctx->parent, &ctx->cnt->changed[it], prev);
+        LAUNCH(k_sleep_jump, blocksFor(nb), kThreads, ctx->parent, nb, prev);
+    }
+    const uint32_t* last = &ctx->cnt->changed[kIslandAhead - 1];
+    LAUNCH(k_island_time, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime, bw, ws, last);
     LAUNCH(k_sleep_apply, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime, s.time_to_sleep,
-           ctx->motion, ctx->mass);
+           ctx->motion, ctx->mass, last);
+    HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->changed[kIslandAhead - 1], last, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           ctx->stream));
     HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
     HIP_TRY(hipEventSynchronize(ctx->ev[2]));
+    if (ctx->hostCnt->changed[kIslandAhead - 1]) {
+        for (;;) {
+            if (it + 2 > kIslandRounds) {
+                return fail(ctx, TFS_E_KERNEL, "the island pass did not settle in %d rounds", kIslandRounds);
+            }
+            for (int j = 0; j < 2; ++j, ++it) {
+                LAUNCH(k_sleep_hook, blocksFor(nc), kThreads, ctx->edges, nc, ctx->parent, &ctx->cnt->changed[it], nullptr);
+                LAUNCH(k_sleep_jump, blocksFor(nb), kThreads, ctx->parent, nb, nullptr);
+            }
+            HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->changed[it - 1], &ctx->cnt->changed[it - 1], sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            if (!ctx->hostCnt->changed[it - 1]) { break; }
+        }
+        LAUNCH(k_island_time, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime, bw, ws, nullptr);
+        LAUNCH(k_sleep_apply, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime, s.time_to_sleep,
+               ctx->motion, ctx->mass, nullptr);
+        HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
+        HIP_TRY(hipEventSynchronize(ctx->ev[2]));
+    }
     if (ctx->hostCnt->error) { return kernelFail(ctx, ctx->hostCnt->error); }
     HIP_TRY(hipEventElapsedTime(&ctx->solveMs, ctx->ev[0], ctx->ev[1]));
     HIP_TRY(hipEventElapsedTime(&ctx->sleepMs, ctx->ev[1], ctx->ev[2]));
