@@ -42,6 +42,9 @@ constexpr int kKeys = 5 * TFS_COLORS + 1; // row-order buckets: colour x point c
 constexpr uint32_t kSpinLimit = 1u << 26;
 constexpr int kIslandRounds = 64;      // hook-and-jump rounds of the island pass before it gives up
 constexpr int kIslandAhead = 6;        // rounds launched before the host checks the first time
+// The per-body kernels of a substep run a wave per block: a 32k-body pile is then 513 blocks over the
+// 256 compute units rather than 129 blocks of four waves on half of them.
+constexpr uint32_t kBodyThreads = 64;
 
 struct DevCounts {
     uint32_t nActive, nFast, nFastSolid, nImpact;
@@ -2981,11 +2984,11 @@ int substepStart(tfs_ctx* ctx, const Ctx& c, uint32_t laned, uint32_t points, bo
     }
     const uint32_t nb = ctx->nb;
     if (rest) {
-        LAUNCH(k_substep_start<true>, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c, ctx->bdeg, ctx->bstart,
+        LAUNCH(k_substep_start<true>, blocksFor(nb, kBodyThreads), kBodyThreads, ctx->bodies, nb, ctx->motion, c, ctx->bdeg, ctx->bstart,
                ctx->bep, ctx->rows, ctx->wrestDeg, ctx->wrestStart, ctx->wrestEp, ctx->restEntries[0], restPack(ctx),
                ctx->mass);
     } else {
-        LAUNCH(k_substep_start<false>, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c, ctx->bdeg, ctx->bstart,
+        LAUNCH(k_substep_start<false>, blocksFor(nb, kBodyThreads), kBodyThreads, ctx->bodies, nb, ctx->motion, c, ctx->bdeg, ctx->bstart,
                ctx->bep, ctx->rows, nullptr, nullptr, nullptr, nullptr, RestPack{}, ctx->mass);
     }
     if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
@@ -3087,7 +3090,7 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
     }
     LAUNCH(k_refresh_pre, blocksFor(hc.nActive), kThreads, ctx->active, ctx->cnt, againFlag, ctx->contacts, ctx->rows,
            ctx->rowOf, ctx->colors, ctx->motion, refreshing, bodyWorld);
-    LAUNCH(k_refresh_bodies, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, ctx->mass, refreshing, bodyWorld);
+    LAUNCH(k_refresh_bodies, blocksFor(nb, kBodyThreads), kBodyThreads, ctx->bodies, nb, ctx->motion, ctx->mass, refreshing, bodyWorld);
     // Only the contacts collided again and their bodies go to the host; the callback reads no others.
     // Grown by half again whenever a longer list comes, so that steady stepping does not allocate.
     int e;
@@ -3319,7 +3322,7 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
             (r = solvePass(ctx, kBias, hc, laned, impactLaned, c))) {
             return r;
         }
-        LAUNCH(k_integrate_positions, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c);
+        LAUNCH(k_integrate_positions, blocksFor(nb, kBodyThreads), kBodyThreads, ctx->bodies, nb, ctx->motion, c);
         if ((r = solvePass(ctx, kRelax, hc, laned, impactLaned, c))) { return r; }
     }
     // restituteBatches + applyRestitution (src/world.cpp:1682-1683): nothing to do unless some active
