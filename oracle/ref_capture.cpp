@@ -410,6 +410,16 @@ int tfr_add_shape(void* h, int body, int kind, const float* dims, int ndims, con
     return (int)wd.addShape(id, sd).index;
 }
 
+// Adds a force and a torque to a body for the next step through the public API (World::addForce,
+// World::addTorque, src/world.cpp:2064-2088), without waking it: a sleeping body ignores them.
+void tfr_add_force(void* h, int body, const float* force, const float* torque) {
+    World& wd = ((Handle*)h)->world;
+    if (body < 0 || (size_t)body >= wd.impl_->bodies.size()) { return; }
+    const BodyId id{ (uint32_t)body, wd.impl_->bodies[(size_t)body].generation };
+    wd.addForce(id, vec3(force), false);
+    wd.addTorque(id, vec3(torque), false);
+}
+
 // Steps the world, recording the solve block when `record` is set. Returns 1 when the step reached
 // the solve block, 0 otherwise.
 int tfr_step(void* h, float dt, int record) {

@@ -55,6 +55,7 @@ def _load_ref():
         lib.tfr_profile.argtypes = [_vp, _fp]
         lib.tfr_counts.argtypes = [_vp, _vp]
         lib.tfr_set_bodies.argtypes = [_vp, _vp, ctypes.c_uint32]
+        lib.tfr_add_force.argtypes = [_vp, ctypes.c_int, _fp, _fp]
         for name in ("tfr_num_bodies", "tfr_num_contacts", "tfr_cap_num_refreshes"):
             getattr(lib, name).restype = ctypes.c_uint32
             getattr(lib, name).argtypes = [_vp]
@@ -87,8 +88,10 @@ class RefWorld:
 
     def __init__(self, scene, settings=None, continuous=True):
         self.lib = _load_ref()
-        s = settings if settings is not None else _abi.default_settings(substeps=scene.substeps)
+        s = settings if settings is not None else _abi.default_settings(substeps=scene.substeps,
+                                                                        **getattr(scene, "settings", {}))
         self.settings = s
+        self.forces = scene.forces() if hasattr(scene, "forces") else []
         self.h = self.lib.tfr_create(s.ctypes.data_as(_vp))
         self.dt = scene.dt
         if not continuous:
@@ -122,6 +125,8 @@ class RefWorld:
 
     def step(self, dt=None, record=True):
         dt = self.dt if dt is None else np.float32(dt)
+        for body, force, torque in self.forces:
+            self.lib.tfr_add_force(self.h, body, _f(force), _f(torque))
         solved = self.lib.tfr_step(self.h, dt, 1 if record else 2)
         if not record:
             return None

@@ -69,7 +69,7 @@ def run_against_reference(solver, scene, steps=None):
 
 @needs_ref
 @pytest.mark.parametrize("make", [scenes.pyramid, scenes.stack, scenes.bounce, scenes.rolling, scenes.planar_mix,
-                                  scenes.tumble, scenes.hub, scenes.planar_pile_small],
+                                  scenes.tumble, scenes.hub, scenes.planar_pile_small, scenes.edges],
                          ids=lambda f: f.__name__)
 def test_gpu_matches_reference_every_step(solver, make):
     scene = make()
@@ -163,5 +163,22 @@ def test_gpu_bodies_without_contacts_match_oracle():
             got = s.bodies()
             assert same_bits(want, got) == [], f"step {step}"
             bodies = got
+    finally:
+        s.close()
+
+
+@needs_ref
+@pytest.mark.parametrize("ahead", [1, 2])
+def test_gpu_island_rounds_checked_by_the_host(monkeypatch, ahead):
+    """The island pass launches a few hook-and-jump rounds before the host first checks whether they
+    settled; a pile whose islands need more rounds goes on under the host's control. With only one or
+    two rounds launched ahead (TWOFOLD_ISLAND_AHEAD) every step takes that path, and the island ids,
+    sleep flags and velocities must still match the reference bit for bit (src/world.cpp:1174-1247)."""
+    monkeypatch.setenv("TWOFOLD_ISLAND_AHEAD", str(ahead))
+    s = Solver(1024, 4096)
+    try:
+        for scene in (scenes.pyramid(rows=12, steps=120), scenes.pile(side=6, half_room=4.0, steps=120)):
+            print(f"island rounds ahead {ahead}: {scene.name}", flush=True)
+            assert run_against_reference(s, scene) > 0
     finally:
         s.close()

@@ -23,7 +23,7 @@ needs_ref = pytest.mark.skipif(not refworld.ref_available(), reason="reference c
 
 @needs_ref
 @pytest.mark.parametrize("make", [scenes.pyramid, scenes.stack, scenes.bounce, scenes.rolling, scenes.planar_mix,
-                                  scenes.tumble, scenes.hub, scenes.planar_pile_small],
+                                  scenes.tumble, scenes.hub, scenes.planar_pile_small, scenes.edges],
                          ids=lambda f: f.__name__)
 def test_oracle_matches_reference_every_step(make):
     scene = make()
@@ -203,3 +203,40 @@ def test_invariants_hold_and_catch_planted_mistakes():
     bad_b = b.copy()
     bad_b["island"][awake[-1]] += 1
     assert any("island" in m for m in parity.invariants(rec.pre_bodies, rec.pre_contacts, c, bad_b, lists))
+
+
+@needs_ref
+def test_edges_scene_reaches_its_branches():
+    """The edge scene exercises what the other scenes leave alone (its recipe says which lines): per
+    step, the reference's own solve-block records show damped, scaled and forced bodies, speeds
+    clamped (`kept` below 1), kinematic bodies touching dynamic ones, bodies carried below the sleep
+    threshold by a kinematic platform that keep a zero sleep time, and restitution applied to planar
+    and to 3D contacts outside the colours."""
+    scene = scenes.edges()
+    w = refworld.RefWorld(scene)
+    seen = dict(kept=0, kinematic=0, planar_bounce=0, rest_bounce=0, forced=0, damped=0, scaled=0)
+    carried = [i for i, b in enumerate(scene.bodies) if b["type"] == scenes.DYNAMIC and b["pos"][2] == 3 and b["pos"][0] < -5]
+    for _ in range(scene.steps):
+        rec = w.step()
+        if rec is None:
+            continue
+        pb, pc = rec.pre_bodies, rec.pre_contacts
+        awake = (pb["type"] == scenes.DYNAMIC) & (pb["awake"] == 1)
+        seen["kept"] += int((rec.post_bodies["kept"] < 1).sum())
+        seen["forced"] += int((awake & (np.abs(pb["force"]).sum(1) > 0) & (np.abs(pb["torque"]).sum(1) > 0)).sum())
+        seen["damped"] += int((awake & (pb["linear_damping"] > 0)).sum())
+        seen["scaled"] += int((awake & (pb["gravity_scale"] != 1)).sum())
+        for ci in rec.lists["active"]:
+            if pc["count"][ci] > 0 and scenes.KINEMATIC in (pb["type"][pc["body_a"][ci]], pb["type"][pc["body_b"][ci]]):
+                seen["kinematic"] += 1
+        post = rec.post_contacts
+        for ci in rec.lists["rest"]:
+            n = post["count"][ci]
+            p = post["points"][ci][:n]
+            if post["restitution"][ci] > 0 and (p["relative_velocity"] < -1).any() and (p["max_normal_impulse"] != 0).any():
+                seen["planar_bounce" if post["planar"][ci] else "rest_bounce"] += 1
+        for i in carried:
+            sb = rec.step_bodies[i]
+            if sb["awake"] and np.linalg.norm(sb["v"]) < 0.05:
+                assert sb["sleep_time"] == 0, "a body carried by a moving kinematic body kept its sleep timer"
+    assert all(v > 0 for v in seen.values()), seen

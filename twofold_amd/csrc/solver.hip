@@ -41,7 +41,7 @@ constexpr int kProfileFamilies = 6;
 constexpr int kKeys = 5 * TFS_COLORS + 1; // row-order buckets: colour x point count, then the rest
 constexpr uint32_t kSpinLimit = 1u << 26;
 constexpr int kIslandRounds = 64;      // hook-and-jump rounds of the island pass before it gives up
-constexpr int kIslandAhead = 6;        // rounds launched before the host checks the first time
+constexpr int kIslandAhead = 6;        // rounds launched before the host checks the first time (at most)
 // The per-body kernels of a substep run a wave per block: a 32k-body pile is then 513 blocks over the
 // 256 compute units rather than 129 blocks of four waves on half of them.
 constexpr uint32_t kBodyThreads = 64;
@@ -2462,6 +2462,9 @@ struct tfs_ctx {
     std::vector<uint32_t> hostBodyWorld;
     bool haveBodies = false, haveContacts = false;
     float solveMs = 0.0f, sleepMs = 0.0f;
+    // hook-and-jump rounds of the island pass launched before the host first checks (1..kIslandAhead;
+    // TWOFOLD_ISLAND_AHEAD lowers it, so that tests reach the host-driven rounds)
+    int islandAhead = kIslandAhead;
     // Measurement: HIP events around each row launch, summed per family (0 warm start, 1 solve,
     // 2 restitution), with the algorithmic bytes of the rows each launch covers.
     bool profiling = false;
@@ -3352,21 +3355,22 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     const uint32_t* bw = worlds ? ctx->bodyWorld : nullptr;
     const uint32_t* ws = worlds ? ctx->worldStart : nullptr;
     int it = 0;
-    for (; it < kIslandAhead; ++it) {
+    const int ahead = ctx->islandAhead;
+    for (; it < ahead; ++it) {
         const uint32_t* prev = it >= 4 ? &ctx->cnt->changed[it - 1] : nullptr;
         LAUNCH(k_sleep_hook, blocksFor(nc), kThreads, ctx->edges, nc, ctx->parent, &ctx->cnt->changed[it], prev);
         LAUNCH(k_sleep_jump, blocksFor(nb), kThreads, ctx->parent, nb, prev);
     }
-    const uint32_t* last = &ctx->cnt->changed[kIslandAhead - 1];
+    const uint32_t* last = &ctx->cnt->changed[ahead - 1];
     LAUNCH(k_island_time, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime, bw, ws, last);
     LAUNCH(k_sleep_apply, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime, s.time_to_sleep,
            ctx->motion, ctx->mass, last);
-    HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->changed[kIslandAhead - 1], last, sizeof(uint32_t), hipMemcpyDeviceToHost,
+    HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->changed[ahead - 1], last, sizeof(uint32_t), hipMemcpyDeviceToHost,
                            ctx->stream));
     HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
     HIP_TRY(hipEventSynchronize(ctx->ev[2]));
-    if (ctx->hostCnt->changed[kIslandAhead - 1]) {
+    if (ctx->hostCnt->changed[ahead - 1]) {
         for (;;) {
             if (it + 2 > kIslandRounds) {
                 return fail(ctx, TFS_E_KERNEL, "the island pass did not settle in %d rounds", kIslandRounds);
@@ -3632,11 +3636,15 @@ int tfs_create(int device, uint32_t maxBodies, uint32_t maxContacts, tfs_ctx** o
         ctx->passBlockLimit = (uint32_t)std::max(1, perCu * cus / 2);
         ctx->maxPassBlocks = std::min<uint32_t>(ctx->passBlockLimit, 128u);
         ctx->usePass = coop != 0 && perCu > 0;
-        // Tuning knobs, read once: TWOFOLD_PASS=launch|coop, TWOFOLD_PASS_BLOCKS=n, TWOFOLD_AXES=packed|made,
-        // TWOFOLD_TAIL=on|off.
+        // Tuning knobs, read once: TWOFOLD_PASS=launch|coop, TWOFOLD_PASS_BLOCKS=n (at most the resident
+        // limit of a grid-barrier launch), TWOFOLD_AXES=packed|made, TWOFOLD_TAIL=on|off,
+        // TWOFOLD_ISLAND_AHEAD=n (1..6).
         if (const char* e = getenv("TWOFOLD_PASS")) { ctx->usePass = ctx->usePass && strcmp(e, "launch") != 0; }
         if (const char* e = getenv("TWOFOLD_PASS_BLOCKS")) {
-            ctx->maxPassBlocks = (uint32_t)std::max(1, std::min(atoi(e), perCu * cus));
+            ctx->maxPassBlocks = (uint32_t)std::max(1, std::min(atoi(e), (int)ctx->passBlockLimit));
+        }
+        if (const char* e = getenv("TWOFOLD_ISLAND_AHEAD")) {
+            ctx->islandAhead = std::max(1, std::min(atoi(e), kIslandAhead));
         }
         if (const char* e = getenv("TWOFOLD_AXES")) { ctx->madeAxes = strcmp(e, "made") == 0; }
         if (const char* e = getenv("TWOFOLD_TAIL")) { ctx->passTail = strcmp(e, "off") != 0; }

@@ -90,22 +90,32 @@ UP = from_to((0, 0, 1), (0, 1, 0))  # the ground's rotation in the reference tes
 class Scene:
     """Bodies and shapes of one world, in creation order."""
 
-    def __init__(self, name, substeps=4, dt=1.0 / 60.0, steps=1):
+    def __init__(self, name, substeps=4, dt=1.0 / 60.0, steps=1, settings=None):
         self.name = name
         self.substeps = substeps
         self.dt = f32(dt)
         self.steps = steps
         self.bodies = []
         self.shapes = []
+        # WorldSettings that differ from the defaults (tfs_settings field names)
+        self.settings = dict(settings or {})
 
     def body(self, type=DYNAMIC, pos=(0, 0, 0), rot=IDENTITY, v=(0, 0, 0), w=(0, 0, 0), locks=0,
-             lock_frame=IDENTITY, awake=True, linear_damping=0.0, angular_damping=0.0, gravity_scale=1.0):
+             lock_frame=IDENTITY, awake=True, linear_damping=0.0, angular_damping=0.0, gravity_scale=1.0,
+             force=(0, 0, 0), torque=(0, 0, 0)):
+        """A body; `force` and `torque` are applied before every step (World::addForce / addTorque
+        without waking, src/world.cpp:2064-2088: they last one step and a sleeping body ignores them)."""
         self.bodies.append(dict(
             type=type, pos=np.asarray(pos, f32), rot=np.asarray(rot, f32), v=np.asarray(v, f32),
             w=np.asarray(w, f32), locks=locks, lock_frame=np.asarray(lock_frame, f32), awake=awake,
             linear_damping=f32(linear_damping), angular_damping=f32(angular_damping),
-            gravity_scale=f32(gravity_scale)))
+            gravity_scale=f32(gravity_scale), force=np.asarray(force, f32), torque=np.asarray(torque, f32)))
         return len(self.bodies) - 1
+
+    def forces(self):
+        """(body, force, torque) of every body with a force or torque applied each step."""
+        return [(i, b["force"], b["torque"]) for i, b in enumerate(self.bodies)
+                if b["force"].any() or b["torque"].any()]
 
     def shape(self, body, kind, dims=(), pos=(0, 0, 0), rot=IDENTITY, density=1.0, friction=0.5,
               restitution=0.0, rolling=0.0):
@@ -296,6 +306,75 @@ def hub(n=7, steps=90):
             z = (f32(k) - f32(n - 1) / f32(2)) * f32(0.5)
             b = s.body(DYNAMIC, pos=(x, 0.7, z))
             s.shape(b, SPHERE, (0.2,), density=1000.0, friction=0.5)
+    return s
+
+
+def edges(steps=240):
+    """The branches of the path that the other scenes leave alone, in one world:
+    - damping, gravity scale, and a force and a torque applied every step (integrateVelocities,
+      src/world.cpp:1141-1151);
+    - a lowered speed cap (7 m/s, 9 rad/s) that thrown and falling bodies reach, so limitSpeed clamps
+      and `kept` drops below 1 (src/solver.cpp:30-41; src/world.cpp:1150, 1162, 1684);
+    - a kinematic platform sliding at 3 cm/s (below the sleep threshold) carrying two boxes, and a
+      kinematic turntable carrying a sphere: kinematic-dynamic contacts with static softness, and the
+      kinematic wake of updateSleep (src/world.cpp:1204-1207);
+    - restitution on contacts outside the colours: planar boxes and a planar disc dropped with
+      restitution 0.5 (solvePlanar's list), a ball and a box with rolling resistance dropped with
+      restitution (the 3D rest list), and a slab under 7 x 7 dropped spheres (more contacts than
+      colours), all landing faster than the restitution threshold (applyRestitution,
+      src/solver.cpp:423-460).
+    The ground has restitution 0.4 and friction 0.6."""
+    s = Scene("edges", steps=steps, settings=dict(max_linear_speed=7.0, max_angular_speed=9.0))
+    s.ground(0.6, restitution=0.4)
+    # damping, gravity scale, forces and torques
+    b = s.body(DYNAMIC, pos=(-6, 2.0, -4), v=(1.0, 0, 0), w=(0, 3.0, 0), linear_damping=0.8, angular_damping=0.5)
+    s.shape(b, BOX, (0.4, 0.3, 0.5), density=800.0, friction=0.5)
+    b = s.body(DYNAMIC, pos=(-4, 3.0, -4), gravity_scale=0.25, angular_damping=0.3)
+    s.shape(b, SPHERE, (0.35,), density=900.0, friction=0.7)
+    b = s.body(DYNAMIC, pos=(-2, 1.5, -4), gravity_scale=1.7, linear_damping=0.1)
+    s.shape(b, BOX, (0.3, 0.3, 0.3), density=1000.0, friction=0.6)
+    b = s.body(DYNAMIC, pos=(0, 0.6, -4), force=(400.0, 0.0, -150.0), torque=(0.0, 60.0, 0.0))
+    s.shape(b, BOX, (0.4, 0.4, 0.4), density=1000.0, friction=0.4)
+    b = s.body(DYNAMIC, pos=(2, 0.8, -4), force=(-90.0, 250.0, 40.0), torque=(35.0, 0.0, -20.0), linear_damping=0.2)
+    s.shape(b, SPHERE, (0.3,), density=1000.0, friction=0.6)
+    # the speed cap: thrown and falling bodies
+    b = s.body(DYNAMIC, pos=(4, 1.0, -4), v=(12.0, 3.0, 0.0), w=(0.0, 15.0, 4.0))
+    s.shape(b, BOX, (0.3, 0.3, 0.3), density=1000.0, friction=0.6)
+    b = s.body(DYNAMIC, pos=(6, 6.0, -4), w=(20.0, 0.0, 0.0))
+    s.shape(b, SPHERE, (0.3,), density=1000.0, friction=0.6, restitution=0.3)
+    # a sliding kinematic platform and a turntable
+    plat = s.body(KINEMATIC, pos=(-6, 0.25, 3), v=(0.03, 0.0, 0.0))
+    s.shape(plat, BOX, (1.5, 0.25, 1.5), friction=0.8)
+    for k in range(2):
+        b = s.body(DYNAMIC, pos=(f32(-6.6) + f32(1.2) * f32(k), 0.8, 3))
+        s.shape(b, BOX, (0.3, 0.3, 0.3), density=1000.0, friction=0.8)
+    table = s.body(KINEMATIC, pos=(-2, 0.2, 3), w=(0.0, 0.8, 0.0))
+    s.shape(table, BOX, (1.0, 0.2, 1.0), friction=0.9)
+    b = s.body(DYNAMIC, pos=(-1.4, 0.75, 3))
+    s.shape(b, SPHERE, (0.35,), density=1000.0, friction=0.9, rolling=0.05)
+    # restitution on the planar rest list
+    for k in range(3):
+        b = s.body(DYNAMIC, pos=(f32(1.5) + f32(1.3) * f32(k), f32(1.2) + f32(0.4) * f32(k), 3), locks=LOCK_PLANAR,
+                   w=(0, 0, f32(0.5) * f32(k)))
+        s.shape(b, BOX, (0.5, 0.4, 0.5), density=1000.0, friction=0.6, restitution=0.5)
+    disc = s.body(DYNAMIC, pos=(6, 1.6, 3), locks=LOCK_PLANAR, w=(0, 0, -3.0))
+    s.shape(disc, CYLINDER, (0.5, 0.25), density=1000.0, friction=0.6, restitution=0.5, rolling=0.05)
+    # restitution on the 3D rest list: rolling resistance
+    b = s.body(DYNAMIC, pos=(-6, 2.0, 8), v=(2.0, 0, 0), w=(0, 0, -4.0))
+    s.shape(b, SPHERE, (0.5,), density=1000.0, friction=0.6, restitution=0.6, rolling=0.1)
+    b = s.body(DYNAMIC, pos=(-3, 1.5, 8), v=(1.5, 0, 0))
+    s.shape(b, BOX, (0.4, 0.4, 0.4), density=1000.0, friction=0.6, restitution=0.5, rolling=0.1)
+    # restitution past the colours: a slab under 49 dropped spheres
+    n = 7
+    half = f32(0.25) * f32(n) + f32(0.05)
+    slab = s.body(DYNAMIC, pos=(3, 0.25, 8))
+    s.shape(slab, BOX, (half, 0.25, half), density=500.0, friction=0.5, restitution=0.6)
+    for i in range(n):
+        for k in range(n):
+            x = f32(3) + (f32(i) - f32(n - 1) / f32(2)) * f32(0.5)
+            z = f32(8) + (f32(k) - f32(n - 1) / f32(2)) * f32(0.5)
+            b = s.body(DYNAMIC, pos=(x, 1.3, z))
+            s.shape(b, SPHERE, (0.2,), density=1000.0, friction=0.5, restitution=0.6)
     return s
 
 

@@ -6,8 +6,11 @@ block and the sleep pass on the GPU, and hands back bodies, contacts and the int
 (lists, colours, row order). Errors raise SolverError with the library's message; there is no CPU
 fallback.
 """
+import atexit
 import ctypes
 import os
+import sys
+import weakref
 
 import numpy as np
 
@@ -102,6 +105,19 @@ def _ptr(a):
     return a.ctypes.data_as(_vp)
 
 
+# Solvers still open. Their device memory is released before the interpreter tears anything down:
+# at interpreter exit an object's __del__ can run after other libraries (torch, a profiler) have
+# already shut the HIP runtime down, and a hipFree then crashes inside exit().
+_live = weakref.WeakSet()
+
+
+@atexit.register
+def close_all():
+    """Closes every solver still open (run at interpreter exit, before module teardown)."""
+    for s in list(_live):
+        s.close()
+
+
 class Solver:
     """One world's solver state, resident on a GPU."""
 
@@ -115,15 +131,28 @@ class Solver:
         self.nb = 0
         self.nc = 0
         self._cb = None
+        _live.add(self)
         if settings is not None:
             self.set_settings(settings)
 
     def close(self):
+        """Releases the device state (tfs_destroy). Safe to call more than once."""
         if getattr(self, "h", None):
             self.lib.tfs_destroy(self.h)
             self.h = None
+        _live.discard(self)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
     def __del__(self):
+        # during interpreter shutdown the runtime may already be gone: leave the memory to the
+        # process's end rather than call into it (close_all has closed every solver before that)
+        if sys.is_finalizing():
+            return
         self.close()
 
     def _check(self, rc, what):

@@ -171,3 +171,14 @@ class Simulation:
     def step(self):
         self.collide()
         self.solve()
+
+    def close(self):
+        """Releases the solver's device state and the host world."""
+        self.solver.close()
+        self.world.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
