@@ -30,7 +30,7 @@ sys.path.insert(0, ROOT)
 import numpy as np  # noqa: E402
 
 from twofold_amd import scenes  # noqa: E402
-from twofold_amd.solver import Solver  # noqa: E402
+from twofold_amd.solver import Solver, device_synchronize  # noqa: E402
 from twofold_amd.world import HostWorld, Simulation  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
@@ -163,12 +163,8 @@ def cpu_port(sim, bodies, contacts, budget_s):
 
 
 def device_sync():
-    try:
-        import torch
-
-        return torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except ImportError:
-        return lambda: None
+    """hipDeviceSynchronize through the library's own HIP runtime."""
+    return device_synchronize
 
 
 class SolveSteps:

@@ -33,7 +33,8 @@ enum {
     TFS_E_DEVICE = -2,   /* a HIP call failed */
     TFS_E_STATE = -3,    /* call out of order (e.g. step before upload) */
     TFS_E_CALLBACK = -4, /* the refresh callback returned an error */
-    TFS_E_KERNEL = -5    /* a kernel reported an internal error (e.g. a bounded wait ran out) */
+    TFS_E_KERNEL = -5,   /* a kernel reported an internal error (e.g. a bounded wait ran out) */
+    TFS_E_CAPACITY = -6  /* tfs_step_noalloc: a world larger than its fixed scratch holds */
 };
 
 /* Motion types, as twofold::MotionType (include/twofold/world.h:37). */
@@ -241,6 +242,31 @@ int tfs_step(tfs_ctx* ctx, float dt, tfs_refresh_fn refresh, void* user);
  * (static, or locked to a plane with normal z), which the host world collides as a 2D problem.
  */
 int tfs_set_body_shapes(tfs_ctx* ctx, uint32_t n, const tfs_shape* shapes);
+
+/*
+ * One step that allocates nothing. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). It is tfs_step (upload, step, download) over a world that the caller keeps in its own
+ * device memory: `bodies` (n_bodies) and `contacts` (n_contacts, key order) are device pointers, read
+ * at the call and overwritten with the step's results; `shapes` (device pointer, one per body, or NULL)
+ * turns on the device refresh as tfs_set_body_shapes does, and without it a step that needs a refresh
+ * fails with TFS_E_CALLBACK. Work is ordered on `stream` (a hipStream_t, NULL for the device's default
+ * stream) and the call returns once the results are in the caller's arrays.
+ *
+ * Every scratch buffer the step uses (copies of the world, rows, lists, colouring and island scratch,
+ * counters, the grid barrier) is carved from one fixed array of 24 MiB that the library holds on each
+ * device it runs on until the process ends; a world of more than TFS_NOALLOC_MAX_BODIES bodies or
+ * TFS_NOALLOC_MAX_CONTACTS contacts is refused with TFS_E_CAPACITY. The first call on a device sets
+ * that device's array up (devices 0-15). All calls on one device share its array and its state, and
+ * each call leaves its contents behind: calls that overlap on one device (from two threads, or on two
+ * streams) race. Results are the same bits as tfs_step's on the same input. Errors: TFS_E_ARG for a
+ * null pointer, substeps outside 1..64, or a contact or shape that names a body out of range or is
+ * malformed (checked on the device before any step kernel runs); no message is kept.
+ */
+#define TFS_NOALLOC_MAX_BODIES 2048
+#define TFS_NOALLOC_MAX_CONTACTS 8192
+int tfs_step_noalloc(int device, tfs_body* bodies, uint32_t n_bodies, tfs_contact* contacts, uint32_t n_contacts,
+                     const tfs_shape* shapes, const tfs_settings* settings, float dt, void* stream);
 
 int tfs_download_bodies(tfs_ctx* ctx, tfs_body* out, uint32_t n);
 int tfs_download_contacts(tfs_ctx* ctx, tfs_contact* out, uint32_t n);

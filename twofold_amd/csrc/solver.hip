@@ -2566,6 +2566,10 @@ struct tfs_ctx {
     uint32_t* bep = nullptr;
     DevCounts* cnt = nullptr;
     DevCounts* hostCnt = nullptr;
+    // Device memory carved from a fixed array instead of allocated (the context of tfs_step_noalloc):
+    // every buffer comes from [arena, arena + arenaCap) once, at set-up, and nothing is ever freed.
+    char* arena = nullptr;
+    size_t arenaCap = 0, arenaUsed = 0;
     // pinned host arrays of the refresh callback, compact in the order of the list: the list, its
     // contacts, their bodies (two per contact), and the fresh manifolds
     tfs_body* pinBodies = nullptr;
@@ -2614,8 +2618,21 @@ int fail(tfs_ctx* c, int code, const char* fmt, ...) {
         if (e_ != hipSuccess) { return fail(ctx, TFS_E_DEVICE, "%s: %s", #kernel, hipGetErrorString(e_)); } \
     } while (0)
 
+// Takes `bytes` from the context's fixed array (256-byte aligned), for a context that never allocates.
+int carve(tfs_ctx* ctx, void** p, size_t bytes) {
+    const size_t at = (ctx->arenaUsed + 255) & ~(size_t)255;
+    if (at + bytes > ctx->arenaCap) {
+        return fail(ctx, TFS_E_CAPACITY, "the fixed scratch of %zu bytes is too small (%zu more needed)", ctx->arenaCap,
+                    at + bytes - ctx->arenaCap);
+    }
+    *p = ctx->arena + at;
+    ctx->arenaUsed = at + bytes;
+    return 0;
+}
+
 template <class T>
 int grow(tfs_ctx* ctx, T*& p, size_t n) {
+    if (ctx->arena) { return carve(ctx, (void**)&p, sizeof(T) * (n ? n : 1)); }
     if (p) { HIP_TRY(hipFree(p)); }
     p = nullptr;
     HIP_TRY(hipMalloc((void**)&p, sizeof(T) * (n ? n : 1)));
@@ -3052,6 +3069,10 @@ int kernelFail(tfs_ctx* ctx, uint32_t bits) {
     if (bits & 16u) {
         return fail(ctx, TFS_E_KERNEL, "the device refresh met two boxes that both stay extruded along z (error %u)", bits);
     }
+    if (bits & 32u) {
+        return fail(ctx, TFS_E_ARG, "a contact names a body out of range or has more than 4 points, or a shape is not "
+                                    "a sphere, box or half-space of finite positive size (error %u)", bits);
+    }
     return fail(ctx, TFS_E_KERNEL, "a kernel's bounded wait ran out (error %u)", bits);
 }
 
@@ -3094,17 +3115,6 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
     LAUNCH(k_refresh_pre, blocksFor(hc.nActive), kThreads, ctx->active, ctx->cnt, againFlag, ctx->contacts, ctx->rows,
            ctx->rowOf, ctx->colors, ctx->motion, refreshing, bodyWorld);
     LAUNCH(k_refresh_bodies, blocksFor(nb, kBodyThreads), kBodyThreads, ctx->bodies, nb, ctx->motion, ctx->mass, refreshing, bodyWorld);
-    // Only the contacts collided again and their bodies go to the host; the callback reads no others.
-    // Grown by half again whenever a longer list comes, so that steady stepping does not allocate.
-    int e;
-    if (nAgain > ctx->pinCap) {
-        const uint32_t cap = std::max<uint32_t>(std::max<uint32_t>(nAgain, ctx->pinCap + ctx->pinCap / 2), 1024u);
-        if ((e = growPinned(ctx, ctx->pinAgain, cap)) || (e = growPinned(ctx, ctx->pinContacts, cap)) ||
-            (e = growPinned(ctx, ctx->pinBodies, 2 * (size_t)cap)) || (e = growPinned(ctx, ctx->pinFresh, cap))) {
-            return e;
-        }
-        ctx->pinCap = cap;
-    }
     if (ctx->nShapes) {
         // the pairs collided on the device: no transfer, no callback
         LAUNCH(k_pack, blocksFor(laned), kThreads, ctx->rowsList, laned, ctx->contacts, ctx->moving, ctx->mass, ctx->rows,
@@ -3124,6 +3134,17 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
                ctx->shapes, remaining, ctx->settings.speculative, ctx->fresh, ctx->cnt);
         if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
         return refreshApply(ctx, again, nAgain, laned, impactLaned, nRest);
+    }
+    // Only the contacts collided again and their bodies go to the host; the callback reads no others.
+    // Grown by half again whenever a longer list comes, so that steady stepping does not allocate.
+    int e;
+    if (nAgain > ctx->pinCap) {
+        const uint32_t cap = std::max<uint32_t>(std::max<uint32_t>(nAgain, ctx->pinCap + ctx->pinCap / 2), 1024u);
+        if ((e = growPinned(ctx, ctx->pinAgain, cap)) || (e = growPinned(ctx, ctx->pinContacts, cap)) ||
+            (e = growPinned(ctx, ctx->pinBodies, 2 * (size_t)cap)) || (e = growPinned(ctx, ctx->pinFresh, cap))) {
+            return e;
+        }
+        ctx->pinCap = cap;
     }
     const uint32_t waves = std::min<uint32_t>(nAgain, 4u * 8192u);
     LAUNCH(k_gather_again, (waves + 3) / 4, kThreads, again, nAgain, ctx->contacts, ctx->bodies, ctx->motion,
@@ -3261,7 +3282,9 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     const tfs_settings& s = ctx->settings;
     const Ctx c = makeCtx(s, dt);
     const int n = s.substeps > 1 ? s.substeps : 1;
-    HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
+    // a context without events (tfs_step_noalloc's) does not time its steps
+    const bool timed = ctx->ev[0] != nullptr;
+    if (timed) { HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream)); }
     DevCounts hc{};
     int r;
     if ((r = beginStep(ctx, dt, hc))) { return r; }
@@ -3341,7 +3364,7 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
                restPack(ctx), ctx->contacts);
     }
     LAUNCH(k_writeback, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion);
-    HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
+    if (timed) { HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream)); }
 
     // sleep and islands
     LAUNCH(k_sleep_time, blocksFor(nb), kThreads, ctx->bodies, nb, dt, s.sleep_threshold * s.length_unit, ctx->parent,
@@ -3368,8 +3391,12 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->changed[ahead - 1], last, sizeof(uint32_t), hipMemcpyDeviceToHost,
                            ctx->stream));
     HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
-    HIP_TRY(hipEventSynchronize(ctx->ev[2]));
+    if (timed) {
+        HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
+        HIP_TRY(hipEventSynchronize(ctx->ev[2]));
+    } else {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
     if (ctx->hostCnt->changed[ahead - 1]) {
         for (;;) {
             if (it + 2 > kIslandRounds) {
@@ -3388,12 +3415,18 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
         LAUNCH(k_sleep_apply, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime, s.time_to_sleep,
                ctx->motion, ctx->mass, nullptr);
         HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
-        HIP_TRY(hipEventSynchronize(ctx->ev[2]));
+        if (timed) {
+            HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
+            HIP_TRY(hipEventSynchronize(ctx->ev[2]));
+        } else {
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+        }
     }
     if (ctx->hostCnt->error) { return kernelFail(ctx, ctx->hostCnt->error); }
-    HIP_TRY(hipEventElapsedTime(&ctx->solveMs, ctx->ev[0], ctx->ev[1]));
-    HIP_TRY(hipEventElapsedTime(&ctx->sleepMs, ctx->ev[1], ctx->ev[2]));
+    if (timed) {
+        HIP_TRY(hipEventElapsedTime(&ctx->solveMs, ctx->ev[0], ctx->ev[1]));
+        HIP_TRY(hipEventElapsedTime(&ctx->sleepMs, ctx->ev[1], ctx->ev[2]));
+    }
     if ((r = collectProfile(ctx))) { return r; }
     if (ctx->tuning) {
         if (ctx->tunePassMs > 0.0f) { tuneRecord(ctx, ctx->tunePassMs); }
@@ -3543,6 +3576,81 @@ int fixedBegin(tfs_ctx* ctx, float dt, uint32_t maxSteps) {
     ctx->fixedMax = maxSteps;
     ctx->fixedOn = true;
     return checkErrors(ctx);
+}
+
+}  // namespace
+
+// ---- the step that allocates nothing (tfs_step_noalloc) ------------------------------------------
+//
+// Its scratch: every device buffer a step of up to kNoAllocBodies bodies and kNoAllocContacts contacts
+// needs (the body and contact arrays the step works on, rows, lists, colouring and island scratch,
+// scan status words, counters, the grid barrier's lines), carved once per device out of this array.
+// Host code never names the array itself: it uses the address hipGetSymbolAddress gives for the
+// current device.
+constexpr uint32_t kNoAllocBodies = TFS_NOALLOC_MAX_BODIES;
+constexpr uint32_t kNoAllocContacts = TFS_NOALLOC_MAX_CONTACTS;
+constexpr size_t kNoAllocScratchBytes = (size_t)24 << 20;
+constexpr int kNoAllocDevices = 16;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// Rejects what would send a kernel out of bounds: a contact naming a body out of range or holding
+// more than 4 points, a shape that is not a sphere, box or half-space of finite positive size.
+__global__ void k_check_input(const tfs_contact* contacts, uint32_t nc, uint32_t nb, const tfs_shape* shapes,
+                              uint32_t ns, DevCounts* cnt) {
+    for (uint32_t i = gid(); i < nc || i < ns; i += gstride()) {
+        bool bad = false;
+        if (i < nc) {
+            const tfs_contact& c = contacts[i];
+            bad = c.body_a >= nb || c.body_b >= nb || c.count < 0 || c.count > 4;
+        }
+        if (i < ns) {
+            const tfs_shape& x = shapes[i];
+            const bool ok = x.kind == TFS_SHAPE_HALF_SPACE ||
+                            (x.kind == TFS_SHAPE_SPHERE && x.radius > 0.0f && x.radius <= 3.4e38f) ||
+                            (x.kind == TFS_SHAPE_BOX && x.half[0] > 0.0f && x.half[0] <= 3.4e38f && x.half[1] > 0.0f &&
+                             x.half[1] <= 3.4e38f && x.half[2] > 0.0f && x.half[2] <= 3.4e38f);
+            bad = bad || !ok;
+        }
+        if (bad) { atomicOr(&cnt->error, 32u); }
+    }
+}
+
+namespace {
+
+// One context per device for tfs_step_noalloc, set up on its first call on that device, its device
+// memory all carved from that device's copy of `scratch`; counts come back through plain host memory.
+tfs_ctx g_noalloc[kNoAllocDevices];
+DevCounts g_noallocHostCnt[kNoAllocDevices];
+bool g_noallocReady[kNoAllocDevices];
+
+int noallocSetUp(int device) {
+    tfs_ctx* ctx = &g_noalloc[device];
+    ctx->error.reserve(512);   // error messages then fit without allocating
+    tfs_default_settings(&ctx->settings);
+    ctx->device = device;
+    ctx->stream = nullptr;
+    void* base = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&base, HIP_SYMBOL(scratch)));
+    if (!base) { return fail(ctx, TFS_E_DEVICE, "no address for the scratch array on device %d", device); }
+    ctx->arena = (char*)base;
+    ctx->arenaCap = kNoAllocScratchBytes;
+    ctx->arenaUsed = 0;
+    int r;
+    if ((r = carve(ctx, (void**)&ctx->cnt, sizeof(DevCounts))) || (r = carve(ctx, (void**)&ctx->bar, sizeof(BarrierLines)))) {
+        return r;
+    }
+    ctx->hostCnt = &g_noallocHostCnt[device];
+    int perCu = 0, cus = 0, coop = 0;
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, (const void*)k_pass<kRelax, false, true>, kPassThreads, 0));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, device));
+    ctx->passBlockLimit = (uint32_t)std::max(1, perCu * cus / 2);
+    ctx->maxPassBlocks = std::min<uint32_t>(ctx->passBlockLimit, 128u);
+    ctx->usePass = coop != 0 && perCu > 0;
+    if ((r = allocBodies(ctx, kNoAllocBodies)) || (r = allocContacts(ctx, kNoAllocContacts))) { return r; }
+    HIP_TRY(hipStreamSynchronize(nullptr));
+    g_noallocReady[device] = true;
+    return 0;
 }
 
 }  // namespace
@@ -3780,6 +3888,47 @@ int tfs_step(tfs_ctx* ctx, float dt, tfs_refresh_fn refresh, void* user) {
     if (hipSetDevice(ctx->device) != hipSuccess) { return fail(ctx, TFS_E_DEVICE, "hipSetDevice failed"); }
     ctx->fixedOn = false;
     return step(ctx, dt, refresh, user);
+}
+
+int tfs_step_noalloc(int device, tfs_body* bodies, uint32_t n_bodies, tfs_contact* contacts, uint32_t n_contacts,
+                     const tfs_shape* shapes, const tfs_settings* settings, float dt, void* stream) {
+    if (device < 0 || device >= kNoAllocDevices || !settings || (n_bodies && !bodies) || (n_contacts && !contacts)) {
+        return TFS_E_ARG;
+    }
+    if (n_bodies > kNoAllocBodies || n_contacts > kNoAllocContacts) { return TFS_E_CAPACITY; }
+    if (settings->substeps < 1 || settings->substeps > 64) { return TFS_E_ARG; }
+    if (!(dt > 0.0f) || !(dt <= 3.4e38f)) { return 0; }
+    if (hipSetDevice(device) != hipSuccess) { return TFS_E_DEVICE; }
+    int r;
+    if (!g_noallocReady[device] && (r = noallocSetUp(device))) { return r; }
+    tfs_ctx* ctx = &g_noalloc[device];
+    ctx->stream = (hipStream_t)stream;
+    ctx->settings = *settings;
+    ctx->nb = n_bodies;
+    ctx->nc = n_contacts;
+    ctx->nWorlds = 1;
+    ctx->haveBodies = ctx->haveContacts = true;
+    ctx->fixedOn = false;
+    ctx->shapes = const_cast<tfs_shape*>(shapes);
+    ctx->nShapes = shapes ? n_bodies : 0;
+    // the caller's world into the scratch's arrays, checked before any kernel indexes with it
+    if (n_bodies) { HIP_TRY(hipMemcpyAsync(ctx->bodies, bodies, sizeof(tfs_body) * n_bodies, hipMemcpyDeviceToDevice, ctx->stream)); }
+    if (n_contacts) {
+        HIP_TRY(hipMemcpyAsync(ctx->contacts, contacts, sizeof(tfs_contact) * n_contacts, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    HIP_TRY(hipMemsetAsync(ctx->cnt, 0, sizeof(DevCounts), ctx->stream));
+    if (n_contacts || ctx->nShapes) {
+        LAUNCH(k_check_input, blocksFor(std::max(n_contacts, ctx->nShapes)), kThreads, ctx->contacts, n_contacts, n_bodies,
+               ctx->shapes, ctx->nShapes, ctx->cnt);
+    }
+    if ((r = checkErrors(ctx))) { return r; }
+    if ((r = step(ctx, dt, nullptr, nullptr))) { return r; }
+    if (n_bodies) { HIP_TRY(hipMemcpyAsync(bodies, ctx->bodies, sizeof(tfs_body) * n_bodies, hipMemcpyDeviceToDevice, ctx->stream)); }
+    if (n_contacts) {
+        HIP_TRY(hipMemcpyAsync(contacts, ctx->contacts, sizeof(tfs_contact) * n_contacts, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
 }
 
 int tfs_autotune(tfs_ctx* ctx, uint32_t rounds) {

@@ -28,7 +28,8 @@ EXPORTS = ("tfs_abi_version", "tfs_sizeof_body", "tfs_sizeof_contact", "tfs_size
            "tfs_upload_contacts", "tfs_set_worlds", "tfs_step", "tfs_download_bodies", "tfs_download_contacts", "tfs_download_list",
            "tfs_last_step_ms", "tfs_last_step_counts", "tfs_set_profiling", "tfs_read_profile", "tfs_fixed_begin",
            "tfs_fixed_step", "tfs_adjoint_seed", "tfs_adjoint_step", "tfs_adjoint_read", "tfs_check_minmax",
-           "tfs_autotune", "tfs_pass_shape", "tfs_set_pass_shape", "tfs_set_cooperative", "tfs_set_body_shapes")
+           "tfs_autotune", "tfs_pass_shape", "tfs_set_pass_shape", "tfs_set_cooperative", "tfs_set_body_shapes",
+           "tfs_step_noalloc")
 
 
 class SolverError(RuntimeError):
@@ -76,6 +77,8 @@ def load_library(path=LIB_PATH):
     lib.tfs_set_pass_shape.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
     lib.tfs_set_cooperative.argtypes = [_vp, ctypes.c_int]
     lib.tfs_set_body_shapes.argtypes = [_vp, ctypes.c_uint32, _vp]
+    lib.tfs_step_noalloc.argtypes = [ctypes.c_int, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp, ctypes.c_float,
+                                     _vp]
     lib.tfs_read_profile.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_double)]
     if lib.tfs_abi_version() != 1:
@@ -103,6 +106,68 @@ def device_minmax(a, b, device=0):
 
 def _ptr(a):
     return a.ctypes.data_as(_vp)
+
+
+_hip = None
+
+
+def hip_runtime():
+    """The HIP runtime the library itself links (libamdhip64 from ROCm), loaded once. Callers that need
+    device memory or a device-wide sync beside the library use this one: a second HIP runtime in the
+    process (torch bundles its own) may not see the GPU once the first is up."""
+    global _hip
+    if _hip is None:
+        load_library()
+        hip = ctypes.CDLL("libamdhip64.so.7")
+        hip.hipMalloc.argtypes = [ctypes.POINTER(_vp), ctypes.c_size_t]
+        hip.hipFree.argtypes = [_vp]
+        hip.hipMemcpy.argtypes = [_vp, _vp, ctypes.c_size_t, ctypes.c_int]
+        hip.hipDeviceSynchronize.argtypes = []
+        _hip = hip
+    return _hip
+
+
+def device_synchronize():
+    """Waits for all work on the current device (hipDeviceSynchronize)."""
+    rc = hip_runtime().hipDeviceSynchronize()
+    if rc != 0:
+        raise SolverError(f"hipDeviceSynchronize failed ({rc})")
+
+
+class DeviceArray:
+    """A numpy structured array copied into device memory of its own (hipMalloc), for the entry points
+    that take device pointers; `get()` copies it back."""
+
+    def __init__(self, a):
+        self.hip = hip_runtime()
+        a = np.ascontiguousarray(a)
+        self.dtype, self.n = a.dtype, len(a)
+        self.ptr = _vp()
+        rc = self.hip.hipMalloc(ctypes.byref(self.ptr), max(a.nbytes, 1))
+        if rc != 0:
+            raise SolverError(f"hipMalloc failed ({rc})")
+        if a.nbytes:
+            self._copy(self.ptr, a.ctypes.data, a.nbytes, 1)
+
+    def _copy(self, dst, src, n, kind):
+        rc = self.hip.hipMemcpy(dst, src, n, kind)
+        if rc != 0:
+            raise SolverError(f"hipMemcpy failed ({rc})")
+
+    def get(self):
+        out = np.zeros(self.n, self.dtype)
+        if out.nbytes:
+            self._copy(out.ctypes.data, self.ptr, out.nbytes, 2)
+        return out
+
+    def close(self):
+        if self.ptr:
+            self.hip.hipFree(self.ptr)
+            self.ptr = _vp()
+
+    def __del__(self):
+        if not sys.is_finalizing():
+            self.close()
 
 
 # Solvers still open. Their device memory is released before the interpreter tears anything down:
