@@ -228,6 +228,13 @@ def run_batched(args):
     t0 = time.perf_counter()
     worlds = [scenes.batch_world(n, 0x5EED0000 + k) for k, n in enumerate(sizes)]
     sim = Simulation(worlds, device_refresh=args.refresh == "device")
+    try:
+        return _run_batched(args, worlds, sim, t0)
+    finally:
+        sim.close()
+
+
+def _run_batched(args, worlds, sim, t0):
     log(f"{len(worlds)} worlds, {len(sim.bodies)} bodies, built in {time.perf_counter() - t0:.1f} s; settling "
         f"{args.settle} steps")
     step = SolveSteps(sim)
@@ -304,7 +311,8 @@ def run_batched(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: seeded per-world pile recipes, stepped on the GPU with the host narrowphase",
+        "data": "synthetic: seeded per-world pile recipes; host narrowphase between steps, mid-step refreshes "
+                f"collided on the {'device' if sim.device_refresh is not None else 'host'}",
         "config": {
             "workload": f"{len(worlds)} independent worlds of log-uniform [8, 1024] bodies ({len(sim.bodies)} "
                         f"bodies), after {args.settle} drop steps, dt 1/60, 4 substeps; per step: host narrowphase "
@@ -373,8 +381,22 @@ def run_adjoint(args):
             log(f"round {rep}: forward {1e3 * forward / args.steps:.3f} ms/step, adjoint {1e3 * backward / args.steps:.3f} "
                 f"ms/step, loss {loss:.6g}, |dL/dv0| {np.linalg.norm(gv):.6g}")
         counts = solver.counts()
+        # one more round event-timed: the reverse sweep's passes (profile family 6) for the roofline
+        solver.upload_worlds(bodies, body_start, contacts)
+        solver.fixed_begin(host.dt, args.steps)
+        for _ in range(args.steps):
+            solver.fixed_step()
+        solver.adjoint_seed(dl)
+        solver.set_profiling(True)
+        for _ in range(args.steps):
+            solver.adjoint_step()
+        solver.set_profiling(False)
+        launches, ms, algo = solver.profile(6)
+        achieved = algo / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        log(f"reverse passes: {launches} launches, {ms:.2f} ms, {algo / max(launches, 1) / 1e6:.2f} MB per launch")
     finally:
         solver.close()
+        host.close()
     ratio = backward / forward
     cpu = None
     if not args.no_cpu:
@@ -401,6 +423,14 @@ def run_adjoint(args):
             "forward_world_steps_per_s": len(worlds) * args.steps / forward,
             "adjoint_world_steps_per_s": len(worlds) * args.steps / backward,
             "colors": counts["colors"], "points": counts["points"],
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None, "traffic_source": None,
+            "kernel": "k_pass_adj<bias|relax> (one grid-barrier launch per reverse pass over all worlds' rows; bytes "
+                      "per row 440 + 348n)",
+            "launches": launches, "profiled_steps": args.steps,
+            "bytes_per_launch": algo / max(launches, 1), "ms_per_launch": ms / max(launches, 1),
         },
         "cpu_baseline": cpu,
     }
@@ -437,115 +467,93 @@ def cpu_adjoint(worlds, args):
             "sample": f"{len(worlds)} worlds x {steps} steps of the float64 numpy model (oracle/fixed_contacts.py)"}
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
-    ap.add_argument("--workload", choices=("pile", "batched", "planar", "adjoint"), default="pile",
-                    help="pile: the 32k-body pile (the headline metric); batched: independent worlds; planar: 65,536 "
-                         "bodies locked to a plane with free spheres; adjoint: the fixed-contact model's reverse sweep "
-                         "on 1,024 throw worlds")
-    ap.add_argument("--worlds", type=int, default=None, help="batched: number of worlds (4096); adjoint (1024)")
-    ap.add_argument("--cpu-worlds", type=int, default=4, help="adjoint: worlds of the CPU baseline sample")
-    ap.add_argument("--cpu-steps", type=int, default=4, help="adjoint: steps of the CPU baseline sample")
-    ap.add_argument("--loop-sample", type=int, default=64, help="batched: worlds timed alone for the looping rate")
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None, help="timed steps (pile 60, batched 10)")
-    ap.add_argument("--warmup", type=int, default=None, help="untimed steps before them (pile 5, batched 2)")
-    ap.add_argument("--settle", type=int, default=None,
-                    help="steps of drop-and-settle before the warm-up (pile 240, batched 60)")
-    ap.add_argument("--side", type=int, default=32, help="bodies per side of the grid (32 -> 32,768)")
-    ap.add_argument("--profile-steps", type=int, default=10, help="event-timed steps for the roofline")
-    ap.add_argument("--batch-profile-steps", type=int, default=3, help="batched: event-timed steps for the roofline")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline leg")
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--refresh", choices=("device", "host"), default="device",
-                    help="where mid-step refreshes collide their pairs: on the device (tfs_set_body_shapes; "
-                         "spheres, boxes and half-spaces) or through the host world's callback")
-    ap.add_argument("--tune-rounds", type=int, default=2,
-                    help="rounds of pass-shape autotuning over the last settle steps (0: the default shape)")
-    ap.add_argument("--copy-mib", type=int, default=0,
-                    help="first run one device-to-device copy of this many MiB: the reference reading for scaling "
-                         "the FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run")
-    args = ap.parse_args()
-    pile = args.workload == "pile"
-    planar = args.workload == "planar"
-    adjoint = args.workload == "adjoint"
-    args.worlds = args.worlds if args.worlds is not None else (1024 if adjoint else 4096)
-    args.steps = args.steps if args.steps is not None else (60 if pile else 32 if adjoint else 10 if not planar else 20)
-    args.warmup = args.warmup if args.warmup is not None else (5 if pile else 2)
-    args.settle = args.settle if args.settle is not None else (240 if pile else 60 if not planar else 120)
-    world_size = int(os.environ.get("WORLD_SIZE", "1"))
-    if args.gpus != 1 or world_size != 1:
-        print("bench.py runs on one GPU only: this build measures the solve path on a single MI355X "
-              "(--gpus 1); multi-GPU runs are not part of it.", file=sys.stderr)
-        sys.exit(2)
-
-    if args.copy_mib > 0:
-        import torch
-
-        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"copied {args.copy_mib} MiB device to device")
-        del src, dst
-
-    if args.workload == "batched":
-        print(json.dumps(run_batched(args)), flush=True)
-        return
-    if adjoint:
-        print(json.dumps(run_adjoint(args)), flush=True)
-        return
-
+def run_pile(args, planar=False):
+    """The 32,768-body pile (the headline) or, with `planar`, the planar pile: drop and settle with every
+    step's solve timed (the whole-run rate), then the timed window, then (pile) a leg with the mid-step
+    refreshes through the host callback, then event-timed steps for the roofline, then the CPU baseline
+    on the state at the end of the settle."""
     scene = scenes.planar_pile() if planar else scenes.pile(side=args.side)
     t0 = time.perf_counter()
     # the planar pile's prisms and cylinders are collided on the host only
     sim = Simulation(scene, device_refresh=args.refresh == "device" and not planar)
     log(f"{scene.name}: {len(sim.bodies)} bodies built in {time.perf_counter() - t0:.1f} s; settling {args.settle} steps")
-    for k in range(args.settle):
-        tune(sim.solver, args, k)
-        sim.step()
-        if (k + 1) % 20 == 0:
-            c = sim.solver.counts()
-            log(f"settle step {k + 1}: {len(sim.contacts)} contacts, {c['points']} points, {c['colors']} colours, "
-                f"{time.perf_counter() - t0:.1f} s")
-    shape = finish_tuning(sim.solver, sim.step)
-    bodies0 = sim.solver.bodies()
     substeps = int(sim.settings["substeps"])
     solve_step = SolveSteps(sim)
-    for _ in range(args.warmup):
-        solve_step()
-    solve_step.transfer = 0.0
-    its = 0
-    solve_ms = 0.0
-    wall = 0.0
-    contacts = 0
-    for _ in range(args.steps):
-        wall += solve_step()
-        counts = sim.solver.counts()
-        its += iterations(counts, substeps)
-        solve_ms += sim.solver.last_ms()[0]
-        contacts += len(sim.contacts)
-    transfers = solve_step.transfer
-    sim.solver.set_profiling(True)
-    for _ in range(args.profile_steps):
-        solve_step()
-    sim.solver.set_profiling(False)
-    # the dominant kernel: the coloured passes on the pile, the rest passes (planar contacts) on the planar pile
-    launches, ms, algo = sim.solver.profile(5 if planar else 1)
-    _, wait_ms, _ = sim.solver.profile(3)
-    refreshes, narrow_ms, pairs = sim.solver.profile(4)
-    achieved = algo / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    log(f"timed {args.steps} solve steps: {wall:.3f} s; {'rest' if planar else 'solve'} passes: {launches} launches, {ms:.2f} ms, "
-        f"{algo / max(launches, 1) / 1e6:.2f} MB per launch")
+    run = dict(its=0, wall=0.0, steps=0)
 
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_reference(scene, bodies0, substeps, args.cpu_seconds)
-        if cpu is None:
-            cpu = cpu_port(sim, bodies0, sim.world.collide(bodies0.copy(), sim.dt, sim.contacts)[0], args.cpu_seconds)
-        log(f"cpu baseline ({cpu['kind']}): {cpu['value']:.3e} iterations/s ({cpu['ms_per_step']:.0f} ms/step)")
+    def whole(spent):
+        run["its"] += iterations(sim.solver.counts(), substeps)
+        run["wall"] += spent
+        run["steps"] += 1
 
-    out = {
+    try:
+        for k in range(args.settle):
+            tune(sim.solver, args, k)
+            whole(solve_step())
+            if (k + 1) % args.log_every == 0:
+                c = sim.solver.counts()
+                v = np.linalg.norm(sim.bodies["v"], axis=1)
+                log(f"settle step {k + 1}: {len(sim.contacts)} contacts, {c['points']} points, {c['colors']} colours, "
+                    f"{c['rest']} rest, {c['rest_levels']} rest levels, speed mean {v.mean():.3f} max {v.max():.2f} m/s, "
+                    f"{time.perf_counter() - t0:.1f} s")
+        shape = finish_tuning(sim.solver, solve_step)
+        bodies0 = sim.solver.bodies()
+        for _ in range(args.warmup):
+            whole(solve_step())
+        solve_step.transfer = 0.0
+        its = 0
+        solve_ms = 0.0
+        wall = 0.0
+        contacts = 0
+        first = run["steps"] + 1
+        for _ in range(args.steps):
+            spent = solve_step()
+            wall += spent
+            counts = sim.solver.counts()
+            its += iterations(counts, substeps)
+            solve_ms += sim.solver.last_ms()[0]
+            contacts += len(sim.contacts)
+            whole(spent)
+        window = f"{first}-{run['steps']}"
+        transfers = solve_step.transfer
+        while run["steps"] < args.whole_steps:
+            whole(solve_step())
+        host_leg = None
+        if not planar and sim.device_refresh is not None and args.host_refresh_steps > 0:
+            # the drop-in configuration: the same steps with the refreshes through the host callback
+            sim.solver.set_body_shapes(None)
+            h_its, h_wall = 0, 0.0
+            for _ in range(args.host_refresh_steps):
+                h_wall += solve_step()
+                h_its += iterations(sim.solver.counts(), substeps)
+            sim.solver.set_body_shapes(sim.device_refresh)
+            host_leg = {"value": h_its / h_wall, "unit": "iterations/s", "steps": args.host_refresh_steps,
+                        "ms_per_step": 1e3 * h_wall / args.host_refresh_steps,
+                        "window": f"{run['steps'] + 1}-{run['steps'] + args.host_refresh_steps}",
+                        "refresh": "host callback (twofold_amd/csrc/scene.cpp tfw_refresh)"}
+            log(f"host-refresh leg: {host_leg['value']:.4g} iterations/s")
+        sim.solver.set_profiling(True)
+        for _ in range(args.profile_steps):
+            solve_step()
+        sim.solver.set_profiling(False)
+        # the dominant kernel: the coloured passes on the pile, the rest passes (planar contacts) on the planar pile
+        launches, ms, algo = sim.solver.profile(5 if planar else 1)
+        _, wait_ms, _ = sim.solver.profile(3)
+        refreshes, narrow_ms, pairs = sim.solver.profile(4)
+        achieved = algo / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        log(f"timed {args.steps} solve steps: {wall:.3f} s; {'rest' if planar else 'solve'} passes: {launches} launches, "
+            f"{ms:.2f} ms, {algo / max(launches, 1) / 1e6:.2f} MB per launch")
+        cpu = None
+        if not args.no_cpu:
+            cpu = cpu_reference(scene, bodies0, substeps, args.cpu_seconds)
+            if cpu is None:
+                cpu = cpu_port(sim, bodies0, sim.world.collide(bodies0.copy(), sim.dt, sim.contacts)[0], args.cpu_seconds)
+            log(f"cpu baseline ({cpu['kind']}): {cpu['value']:.3e} iterations/s ({cpu['ms_per_step']:.0f} ms/step)")
+        refresh_where = "device" if sim.device_refresh is not None else "host"
+    finally:
+        sim.close()
+    traffic = pile_traffic(args) if not planar else None
+    return {
         "metric": "contact constraint-iterations/s",
         "value": its / wall,
         "unit": "iterations/s",
@@ -557,31 +565,36 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": f"synthetic: seeded {scene.name} recipe, stepped on the GPU with the host narrowphase",
+        "data": f"synthetic: seeded {scene.name} recipe; host narrowphase between steps, mid-step refreshes "
+                f"collided on the {'device (tfs_set_body_shapes)' if refresh_where == 'device' else 'host (refresh callback)'}",
         "config": {
             "workload": (f"{len(bodies0)} bodies: 65,536 bodies locked to the xy plane (40% boxes, 40% prisms of 3-8 "
-                         f"sides, 20% cylinders) and 4,096 free spheres in a U container (after {args.settle} drop "
-                         f"steps)" if planar else
-                         f"settled {len(bodies0)}-body box/sphere pile in walls (after {args.settle} drop steps)")
-                        + f", dt 1/60, {substeps} substeps; per step: host narrowphase + upload (untimed), then one "
-                        f"solve block + sleep pass on the GPU (timed; its mid-step refreshes collide on the "
-                        f"{'device' if sim.device_refresh is not None else 'host'})",
+                         f"sides, 20% cylinders) and 4,096 free spheres in a U container" if planar else
+                         f"{len(bodies0)}-body box/sphere pile in walls")
+                        + f", timed over steps {window} after {args.settle} drop-and-settle steps, dt 1/60, {substeps} "
+                        f"substeps; per step: host narrowphase + upload (untimed), then one solve block + sleep pass "
+                        f"on the GPU (timed; its mid-step refreshes collide on the {refresh_where})",
+            "timed_steps": window,
             "bodies": len(bodies0), "contacts_per_step": contacts / args.steps, "active_points": counts["points"],
             "impact_points": counts["impact_points"], "colors": counts["colors"], "rest": counts["rest"],
             "rest_levels": counts["rest_levels"], "impact_rest_levels": counts["impact_rest_levels"],
             "world_steps_per_s": args.steps / wall, "gpu_solve_ms_per_step": solve_ms / args.steps,
             "transfer_ms_per_step": 1e3 * transfers / args.steps,
             "value_with_transfers": its / (wall + transfers), "pass_shape": shape,
+            "whole_run": {"steps": f"1-{run['steps']}", "value": run["its"] / run["wall"], "unit": "iterations/s",
+                          "ms_per_step": 1e3 * run["wall"] / run["steps"],
+                          "note": "every step's solve timed the same way from the first drop step on, the pass-shape "
+                                  "tuning steps included"},
+            "value_host_refresh": host_leg,
             "refreshes_per_step": refreshes / max(args.profile_steps, 1),
             "refresh_pairs_per_step": pairs / max(args.profile_steps, 1),
             "refresh_wait_ms_per_step": wait_ms / max(args.profile_steps, 1),
             "refresh_narrowphase_ms_per_step": narrow_ms / max(args.profile_steps, 1),
-            "refresh": "device" if sim.device_refresh is not None else "host callback",
+            "refresh": "device" if refresh_where == "device" else "host callback",
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": K_PASS_TRAFFIC_BYTES if pile and args.side == 32 else None,
-            "traffic_source": K_PASS_TRAFFIC_SOURCE if pile and args.side == 32 else None,
+            "traffic": traffic[0] if traffic else None, "traffic_source": traffic[1] if traffic else None,
             "kernel": ("k_rest_levels<bias|relax> (one solve pass over a rest list: the leading wide levels in a "
                        "grid-barrier launch, the rest in one work-group; bytes per planar contact 248 + 68n)" if planar
                        else "k_pass<bias|relax> (one grid-barrier launch per solve pass)"),
@@ -590,7 +603,119 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+
+
+def pile_traffic(args):
+    """The k_pass traffic figure of the 32k pile (carried from a rocprofv3 run), or None for other sizes."""
+    if args.side != 32:
+        return None
+    return K_PASS_TRAFFIC_BYTES, K_PASS_TRAFFIC_SOURCE
+
+
+DEFAULTS = {  # steps, warmup, settle per workload when it runs alone or as a part of the full run
+    "pile": (60, 5, 240),
+    "planar": (20, 2, 720),
+    "batched": (10, 2, 60),
+    "adjoint": (32, 2, 0),
+}
+
+
+def sub_args(args, workload):
+    """The arguments of a workload run as a part of the full run: its own defaults."""
+    a = argparse.Namespace(**vars(args))
+    a.workload = workload
+    a.steps, a.warmup, a.settle = DEFAULTS[workload]
+    a.worlds = 1024 if workload == "adjoint" else 4096
+    a.cpu_seconds = args.sub_cpu_seconds
+    return a
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--workload", choices=("all", "pile", "batched", "planar", "adjoint"), default="all",
+                    help="all (default): the 32k-body pile (the headline) with the planar, batched and adjoint "
+                         "workloads in sub-objects of the same line; or one of them alone")
+    ap.add_argument("--worlds", type=int, default=None, help="batched: number of worlds (4096); adjoint (1024)")
+    ap.add_argument("--cpu-worlds", type=int, default=4, help="adjoint: worlds of the CPU baseline sample")
+    ap.add_argument("--cpu-steps", type=int, default=4, help="adjoint: steps of the CPU baseline sample")
+    ap.add_argument("--loop-sample", type=int, default=64, help="batched: worlds timed alone for the looping rate")
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (pile 60, planar 20, batched 10, adjoint 32)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps before them (pile 5, others 2)")
+    ap.add_argument("--settle", type=int, default=None,
+                    help="steps of drop-and-settle before the warm-up (pile 240, planar 720, batched 60)")
+    ap.add_argument("--whole-steps", type=int, default=300, help="pile: steps of the whole-run rate (1 to this)")
+    ap.add_argument("--host-refresh-steps", type=int, default=10,
+                    help="pile: steps timed with the refreshes through the host callback (the drop-in form)")
+    ap.add_argument("--side", type=int, default=32, help="bodies per side of the grid (32 -> 32,768)")
+    ap.add_argument("--profile-steps", type=int, default=10, help="event-timed steps for the roofline")
+    ap.add_argument("--batch-profile-steps", type=int, default=3, help="batched: event-timed steps for the roofline")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the headline's CPU baseline leg")
+    ap.add_argument("--sub-cpu-seconds", type=float, default=8.0, help="budget of the other workloads' CPU legs")
+    ap.add_argument("--time-budget", type=float, default=420.0,
+                    help="all: seconds after which the workloads not yet run are skipped (noted in the line)")
+    ap.add_argument("--log-every", type=int, default=20, help="settle steps between progress lines")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--refresh", choices=("device", "host"), default="device",
+                    help="where mid-step refreshes collide their pairs: on the device (tfs_set_body_shapes; "
+                         "spheres, boxes and half-spaces) or through the host world's callback")
+    ap.add_argument("--tune-rounds", type=int, default=2,
+                    help="rounds of pass-shape autotuning over the last settle steps (0: the default shape)")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB: the reference reading for scaling "
+                         "the FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run")
+    args = ap.parse_args()
+    world_size = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world_size != 1:
+        print("bench.py runs on one GPU only: this build measures the solve path on a single MI355X "
+              "(--gpus 1); multi-GPU runs are not part of it.", file=sys.stderr)
+        sys.exit(2)
+    t_start = time.perf_counter()
+    one = args.workload if args.workload != "all" else "pile"
+    d_steps, d_warmup, d_settle = DEFAULTS[one]
+    args.worlds = args.worlds if args.worlds is not None else (1024 if one == "adjoint" else 4096)
+    args.steps = args.steps if args.steps is not None else d_steps
+    args.warmup = args.warmup if args.warmup is not None else d_warmup
+    args.settle = args.settle if args.settle is not None else d_settle
+
+    if args.copy_mib > 0:
+        copy_reference(args.copy_mib)
+
+    if one == "batched":
+        out = run_batched(args)
+    elif one == "adjoint":
+        out = run_adjoint(args)
+    else:
+        out = run_pile(args, planar=one == "planar")
+    if args.workload == "all":
+        for name, fn in (("adjoint", run_adjoint), ("planar", lambda a: run_pile(a, planar=True)),
+                         ("batched", run_batched)):
+            spent = time.perf_counter() - t_start
+            if spent > args.time_budget:
+                out[name] = {"skipped": f"the run had spent {spent:.0f} s of its {args.time_budget:.0f} s budget"}
+                log(f"{name}: skipped ({spent:.0f} s spent)")
+                continue
+            log(f"---- {name} ({spent:.0f} s into the run)")
+            sub = fn(sub_args(args, name))
+            for k in ("n_gpus", "scaling", "vs_baseline", "dtype"):
+                sub.pop(k, None)
+            out[name] = sub
+        out["config"]["bench_seconds"] = time.perf_counter() - t_start
     print(json.dumps(out), flush=True)
+
+
+def copy_reference(mib):
+    """One device-to-device copy of `mib` MiB (hipMemcpy through the library's HIP runtime), the known
+    size that the FETCH_SIZE / WRITE_SIZE counters of a rocprofv3 --pmc run are scaled by."""
+    from twofold_amd.solver import DeviceArray
+
+    a = DeviceArray(np.ones(mib << 18, np.float32))
+    b = DeviceArray(np.zeros(mib << 18, np.float32))
+    rc = a.hip.hipMemcpy(b.ptr, a.ptr, mib << 20, 3)
+    device_synchronize()
+    a.close()
+    b.close()
+    log(f"copied {mib} MiB device to device ({rc})")
 
 
 if __name__ == "__main__":

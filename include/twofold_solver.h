@@ -298,7 +298,9 @@ int tfs_last_step_ms(tfs_ctx* ctx, float* solve_ms, float* sleep_ms);
  * (tfs_set_body_shapes), 4 the event time of the collision kernel and 3 nothing. Family 4's "bytes"
  * sum counts the pairs collided again. Family 5 is the biased and relax passes over the contacts outside the colours (the rest
  * lists, active and impact), event-timed per pass over one list, with bytes per contact of 248 + 68n
- * (planar) or 224 + 296n (3D). Turning it on clears the sums; turning it off keeps them for reading.
+ * (planar) or 224 + 296n (3D). Family 6 is the reverse sweep's passes (tfs_adjoint_step), event-timed per
+ * pass, with 440 + 348n bytes per row (the solve's row, its record, the velocity and impulse adjoints).
+ * Turning it on clears the sums; turning it off keeps them for reading.
  */
 int tfs_set_profiling(tfs_ctx* ctx, int on);
 int tfs_read_profile(tfs_ctx* ctx, int family, uint64_t* launches, double* ms, double* bytes);

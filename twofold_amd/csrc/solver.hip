@@ -37,7 +37,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kScanSlots = 16;         // scans in one step
-constexpr int kProfileFamilies = 6;
+constexpr int kProfileFamilies = 7;
 constexpr int kKeys = 5 * TFS_COLORS + 1; // row-order buckets: colour x point count, then the rest
 constexpr uint32_t kSpinLimit = 1u << 26;
 constexpr int kIslandRounds = 64;      // hook-and-jump rounds of the island pass before it gives up
@@ -3509,6 +3509,25 @@ int passAdjoint(tfs_ctx* ctx, int mode, int slot, const Ctx& c) {
         ++plan.n;
     }
     if (plan.n == 0) { return 0; }
+    size_t ev = 0;
+    if (ctx->profiling) {
+        // bytes per row of the reverse sweep: the solve's row (224 + 296n), its record (64 + 16 + 16n), the
+        // two bodies' velocity adjoints read and written (128), per point the impulse adjoints read and
+        // written (32) and the separation adjoint (4), the friction adjoint (8)
+        uint32_t rowsIn = 0, pointsIn = 0;
+        for (int col = 0; col < TFS_COLORS; ++col) {
+            rowsIn += hc.colorStart[col + 1] - hc.colorStart[col];
+            pointsIn += hc.colorPoints[col];
+        }
+        ev = ctx->pending.size() * 2;
+        while (ctx->evPool.size() < ev + 2) {
+            hipEvent_t x;
+            HIP_TRY(hipEventCreate(&x));
+            ctx->evPool.push_back(x);
+        }
+        ctx->pending.push_back({ 6, ev, 440.0 * rowsIn + 348.0 * pointsIn });
+        HIP_TRY(hipEventRecord(ctx->evPool[ev], ctx->stream));
+    }
     const void* fn = ctx->madeAxes ? (biased ? (const void*)k_pass_adj<true, true> : (const void*)k_pass_adj<false, true>)
                                    : (biased ? (const void*)k_pass_adj<true, false> : (const void*)k_pass_adj<false, false>);
     if (ctx->usePass) {
@@ -3518,6 +3537,7 @@ int passAdjoint(tfs_ctx* ctx, int mode, int slot, const Ctx& c) {
                          &ctx->cnt };
         if (int e = launchGrid(ctx, fn, grid, args)) { return e; }
         advanceBarrier(ctx->barrierBase, grid, plan.n - 1);
+        if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
         return 0;
     }
     for (uint32_t k = 0; k < plan.n; ++k) {
@@ -3531,6 +3551,7 @@ int passAdjoint(tfs_ctx* ctx, int mode, int slot, const Ctx& c) {
                          &ctx->cnt };
         HIP_TRY(hipLaunchKernel(fn, dim3(grid), dim3(kPassThreads), args, 0, ctx->stream));
     }
+    if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
     return 0;
 }
 
@@ -4019,7 +4040,8 @@ int tfs_fixed_step(tfs_ctx* ctx) {
     if ((r = fixedStep(ctx, false))) { return r; }
     ++ctx->fixedSteps;
     ctx->fixedLeft = 0;
-    return checkErrors(ctx);
+    if ((r = checkErrors(ctx))) { return r; }
+    return collectProfile(ctx);
 }
 
 int tfs_adjoint_seed(tfs_ctx* ctx, const float* dl_dcom) {
@@ -4055,7 +4077,8 @@ int tfs_adjoint_step(tfs_ctx* ctx) {
         LAUNCH(k_adj_warm, blocksFor(nr), kThreads, ctx->rows, nr, ctx->adj);
     }
     ctx->fixedLeft = k;
-    return checkErrors(ctx);
+    if ((r = checkErrors(ctx))) { return r; }
+    return collectProfile(ctx);
 }
 
 int tfs_adjoint_read(tfs_ctx* ctx, float* dl_dv, float* dl_dw, float* dl_dfriction) {

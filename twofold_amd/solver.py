@@ -325,7 +325,7 @@ class Solver:
         """(launches, milliseconds, algorithmic bytes) summed since profiling was turned on; family 0 the
         substep start (velocities integrated, every row's warm start), 1 solve, 2 restitution (event-timed); 3 waiting for the refresh's contacts on the host,
         4 the refresh callback (host clock; bytes = pairs collided again); 5 the solve passes over the
-        rest lists (event-timed)."""
+        rest lists (event-timed); 6 the reverse sweep's passes of tfs_adjoint_step (event-timed)."""
         n, ms, by = ctypes.c_uint64(), ctypes.c_double(), ctypes.c_double()
         self._check(self.lib.tfs_read_profile(self.h, family, ctypes.byref(n), ctypes.byref(ms), ctypes.byref(by)),
                     "tfs_read_profile")
