@@ -1,0 +1,160 @@
+"""The planar, batched and adjoint workloads at their full sizes on the GPU.
+
+- The planar pile (65,536 bodies locked to the plane and 4,096 free spheres): dropped on the GPU for
+  420 steps (when it holds the most contacts), then two steps of the reference from that state (its
+  own narrowphase and refresh manifolds) replayed on the GPU, bit for bit, with the results'
+  size-independent properties.
+- 4,096 batched worlds of log-uniform [8, 1024] bodies: dropped together on the GPU for 30 steps, then
+  every world put in the reference alone at its state and stepped once; the GPU solves all 4,096
+  recorded inputs in one step and every world must match its own reference step bit for bit.
+- 1,024 adjoint throw worlds: the fixed-contact model forward 32 steps and the reverse sweep over all
+  of them on the GPU; for a seeded sample of the worlds the centres of mass and the gradients must
+  match the float64 model of that world alone (the tolerances of test_adjoint_gpu.py), worlds whose
+  float64 gradient is itself ill-conditioned at float32's drift left out."""
+import numpy as np
+import pytest
+
+from twofold_amd import scenes
+from twofold_amd.solver import Solver
+from twofold_amd.world import HostWorld, Simulation
+import parity
+import refworld
+from test_batched import concatenate, world_view
+
+pytestmark = pytest.mark.gpu
+
+needs_ref = pytest.mark.skipif(not refworld.ref_available(), reason="reference checker not built (make -C oracle ref)")
+
+
+def recorded(refreshes):
+    by = {r["substep"]: r for r in refreshes}
+
+    def fresh(substep, remaining, again, contacts, bodies):
+        r = by[substep]
+        assert np.array_equal(again, r["again"]), "the GPU asked to refresh other contacts"
+        return r["fresh"]
+
+    return fresh
+
+
+@needs_ref
+def test_full_planar_pile_matches_reference():
+    scene = scenes.planar_pile()
+    print("planar pile: dropping 420 steps", flush=True)
+    with Simulation(scene) as sim:
+        for k in range(420):
+            sim.step()
+            if (k + 1) % 60 == 0:
+                print(f"planar pile: step {k + 1}, {len(sim.contacts)} contacts", flush=True)
+        bodies = sim.solver.bodies()
+    w = refworld.RefWorld(scene)
+    w.set_bodies(bodies)
+    with Solver(len(bodies), 4 * len(bodies)) as s:
+        for k in range(2):
+            rec = w.step()
+            assert rec is not None
+            case = parity.record_dict(rec)
+            print(f"planar pile: reference step {k}, {len(case['pre_contacts'])} contacts", flush=True)
+            s.set_settings(w.settings)
+            s.upload(case["pre_bodies"], case["pre_contacts"])
+            s.step(case["dt"], recorded(case["refreshes"]) if case["refreshes"] else None)
+            b, c, lists = s.bodies(), s.contacts(), s.lists()
+            assert parity.compare(case, b, c, lists) == []
+            assert parity.invariants(case["pre_bodies"], case["pre_contacts"], c, b, lists) == []
+            counts = s.counts()
+            assert counts["rest"] > 50000 and counts["rest_levels"] > 300, counts
+
+
+@needs_ref
+def test_all_4096_batched_worlds_match_each_world_alone():
+    sizes = scenes.batch_sizes(4096)
+    worlds = [scenes.batch_world(n, 0x5EED0000 + k) for k, n in enumerate(sizes)]
+    print(f"batched: {len(worlds)} worlds, {sum(sizes)} dynamic bodies; dropping 30 steps", flush=True)
+    with Simulation(worlds, device_refresh=True) as sim:
+        for _ in range(30):
+            sim.step()
+        state = sim.solver.bodies()
+        start = sim.world.body_start
+    recs = []
+    for k, sc in enumerate(worlds):
+        w = refworld.RefWorld(sc)
+        w.set_bodies(state[start[k]:start[k + 1]])
+        rec = w.step()
+        assert rec is not None
+        recs.append(rec)
+        del w
+    bodies, bs, contacts, cs, refresh = concatenate(recs)
+    print(f"batched: {len(bodies)} bodies, {len(contacts)} contacts in one step", flush=True)
+    with Solver(len(bodies), len(contacts), settings=refworld.RefWorld(worlds[0]).settings) as s:
+        s.upload_worlds(bodies, bs, contacts)
+        s.step(np.float32(worlds[0].dt), refresh)
+        got_b, got_c, lists = s.bodies(), s.contacts(), s.lists()
+    refreshed = 0
+    for k, rec in enumerate(recs):
+        b, c, l = world_view(k, bs, cs, got_b, got_c, lists)
+        bad = parity.compare(parity.record_dict(rec), b, c, l)
+        assert bad == [], f"world {k} ({len(rec.pre_bodies)} bodies): {bad[:5]}"
+        refreshed += len(rec.refreshes) > 0
+    assert len(contacts) > 100000 and refreshed > 100, (len(contacts), refreshed)
+
+
+def test_adjoint_1024_worlds_match_the_model_on_a_sample():
+    from adjoint_cases import Case
+
+    seeds = [0xAD000000 + k for k in range(1024)]
+    steps = 32
+    host = HostWorld([scenes.throw_world(sd, steps=steps) for sd in seeds])
+    bodies = host.bodies()
+    contacts, _ = host.collide(bodies, host.dt)
+    body_start = np.asarray(host.body_start, np.uint32)
+    settings, dt = host.settings, host.dt
+    host.close()
+    rng = np.random.default_rng(0xAD)
+    sample = sorted(rng.choice(1024, size=8, replace=False).tolist())
+    cases = {k: Case([seeds[k]], steps=steps) for k in sample}
+    target = bodies["com"].astype(np.float64)
+    for k in sample:
+        target[body_start[k]:body_start[k + 1]] = cases[k].target
+    moving = bodies["type"] == 2
+    print(f"adjoint: {len(bodies)} bodies, {len(contacts)} contacts, {steps} steps; sample {sample}", flush=True)
+    with Solver(len(bodies), len(contacts), settings=settings) as s:
+        s.upload_worlds(bodies, body_start, contacts)
+        s.fixed_begin(dt, steps)
+        for _ in range(steps):
+            s.fixed_step()
+        com = s.bodies()["com"].astype(np.float64)
+        dl = np.zeros_like(com)
+        dl[moving] = 2.0 * (com[moving] - target[moving])
+        s.adjoint_seed(dl)
+        for _ in range(steps):
+            s.adjoint_step()
+        gv, gw, gmu = s.adjoint_read()
+    checked = 0
+    for k in sample:
+        case = cases[k]
+        lo, hi = body_start[k], body_start[k + 1]
+        nc = len(case.contacts)
+        c0 = int(np.searchsorted(contacts["body_a"], lo))
+        assert contacts["body_a"][c0 + nc - 1] < hi and (c0 + nc == len(contacts) or contacts["body_a"][c0 + nc] >= hi)
+        m = case.model
+        # float32 rounding moves the GPU's trajectory off the float64 one by about 1e-5 relative over
+        # the 32 steps: a world whose float64 gradient moves by more than half the tolerance when v0
+        # moves that much is ill-conditioned in the model itself (clamps flip, the rollout is
+        # chaotic), and is not compared
+        g1 = m.adjoint(case.target, steps=steps)
+        g2 = m.adjoint(case.target, v0=m.v0 * (1 + 1e-5), steps=steps)
+        mv = m.moving
+        if np.linalg.norm(g1[0][mv] - g2[0][mv]) > 0.5e-3 * np.linalg.norm(g1[0][mv]):
+            print(f"adjoint: world {k} ill-conditioned in the model, skipped", flush=True)
+            continue
+        want_com = m.forward(steps=steps)
+        moved = np.linalg.norm(want_com - m.com0)
+        assert np.linalg.norm(com[lo:hi] - want_com) <= 1e-4 * moved, k
+        rel = lambda a, b: np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30)
+        ev = rel(gv[lo:hi][mv], g1[0][mv])
+        ew = rel(gw[lo:hi][mv], g1[1][mv])
+        emu = rel(np.sum(gmu[c0:c0 + nc]), np.sum(g1[2]))
+        print(f"adjoint: world {k}: v {ev:.2e} w {ew:.2e} friction {emu:.2e}", flush=True)
+        assert ev <= 1e-3 and emu <= 1e-3 and ew <= 5e-3, (k, ev, ew, emu)
+        checked += 1
+    assert checked >= 5
