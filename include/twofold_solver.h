@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define TFS_ABI_VERSION 1
+#define TFS_ABI_VERSION 2
 
 enum {
     TFS_OK = 0,
@@ -127,7 +127,9 @@ typedef struct tfs_contact {
     int32_t count;
     uint8_t planar;             /* out */
     uint8_t laned;              /* out */
-    uint8_t pad_[2];
+    uint8_t touched;            /* the pair was touching after the last step, as contact events report it
+                                   (Contact::touched, src/internal.h:492-493); tfs_step updates it */
+    uint8_t pad_;
     tfs_point points[4];
 } tfs_contact;
 
@@ -145,8 +147,19 @@ typedef struct tfs_settings {
     float restitution_threshold;
     float sleep_threshold;
     float time_to_sleep;
-    float speculative;          /* Tolerances::speculative */
+    float speculative;          /* Tolerances::speculative (4 x Tolerances::linearSlop) */
+    float hit_threshold;        /* approach speed above which a touching point is a hit */
 } tfs_settings;
+
+/* A hit: the fastest approaching point of a pair that touches in a step, when it approached faster
+ * than hit_threshold x length_unit (HitEvent, include/twofold/world.h:278-288). */
+typedef struct tfs_hit {
+    uint32_t contact;           /* index of the contact holding the point */
+    uint32_t point;             /* the point's index in it */
+    float position[3];          /* the point on body A: its local_a through body A's pose */
+    float normal[3];            /* from body A to body B */
+    float speed;                /* approach speed, -relative_velocity */
+} tfs_hit;
 
 /*
  * The refresh callback. Before substep i > 0 of a step, when the list `again` (fast contacts at
@@ -254,7 +267,7 @@ int tfs_set_body_shapes(tfs_ctx* ctx, uint32_t n, const tfs_shape* shapes);
  * stream) and the call returns once the results are in the caller's arrays.
  *
  * Every scratch buffer the step uses (copies of the world, rows, lists, colouring and island scratch,
- * counters, the grid barrier) is carved from one fixed array of 24 MiB that the library holds on each
+ * counters, the grid barrier) is carved from one fixed array of 28 MiB that the library holds on each
  * device it runs on until the process ends; a world of more than TFS_NOALLOC_MAX_BODIES bodies or
  * TFS_NOALLOC_MAX_CONTACTS contacts is refused with TFS_E_CAPACITY. The first call on a device sets
  * that device's array up (devices 0-15). All calls on one device share its array and its state, and
@@ -281,6 +294,21 @@ int tfs_download_contacts(tfs_ctx* ctx, tfs_contact* out, uint32_t n);
  * Writes up to `cap` entries (bytes for kind 6, uint32 otherwise) and the full count to *count.
  */
 int tfs_download_list(tfs_ctx* ctx, int kind, void* out, uint32_t cap, uint32_t* count);
+
+/*
+ * Touch events of the last step: reportTouches (src/world.cpp:1424-1468), which the reference runs
+ * right after the solve block, run by tfs_step on the device at the same point (before the sleep pass)
+ * over the contacts the solve left. A pair (the consecutive contacts of one key) touches when a point
+ * carries a largest normal impulse above zero or lies closer than 0 (or, for a pair that touched
+ * before, closer than the linear slop, speculative / 4); each contact's `touched` is updated, and a pair
+ * whose state changed gives an event, events[k] = (index of its first contact << 1) | began, in
+ * contact order. A pair solved this step (a body dynamic and awake) whose fastest approaching pushed
+ * point approached faster than hit_threshold x length_unit gives a hit, in contact order. `touching`
+ * is the number of contacts with a point overlapping or pushed (StepProfile::touchingContacts). Writes up
+ * to the caps and the full counts; any pointer may be NULL.
+ */
+int tfs_download_touches(tfs_ctx* ctx, uint32_t* events, uint32_t events_cap, uint32_t* n_events, tfs_hit* hits,
+                         uint32_t hits_cap, uint32_t* n_hits, uint32_t* touching);
 
 /* Solve-stage timing of the last step in milliseconds, measured with HIP events on the stream,
  * as the reference's StepProfile::solver brackets it (src/world.cpp:1644-1692). */

@@ -1023,7 +1023,63 @@ static void solveRows(World* w, const uint32_t* rows, uint32_t nLaned, uint32_t 
     }
 }
 
-/* src/world.cpp:1564-1691 from the step state on, then updateSleep (1724). */
+/* reportTouches: src/world.cpp:1424-1468. The pairs are the runs of contacts with one key; each run's
+ * state changes when a point was pushed (largest normal impulse above zero) or lies closer than zero
+ * (the linear slop, speculative / 4, for a pair that touched before); the fastest approaching pushed
+ * point of a pair solved this step is its hit when faster than the hit threshold. */
+static void reportTouches(World* w, tfo_lists* L) {
+    const tfs_settings* s = w->s;
+    const float slop = 0.25f * s->speculative;
+    L->n_touch_events = L->n_hits = L->touching = 0;
+    uint32_t last = 0;
+    for (uint32_t first = 0; first < w->nc; first = last) {
+        const uint64_t key = w->contacts[first].key;
+        const int was = w->contacts[first].touched != 0;
+        const float apart = was ? slop : 0.0f;
+        int touching = 0;
+        float fastest = s->hit_threshold * s->length_unit;
+        int struck = -1, point = 0;
+        for (last = first; last < w->nc && w->contacts[last].key == key; ++last) {
+            const tfs_contact* c = &w->contacts[last];
+            const tfs_body* a = &w->bodies[c->body_a];
+            const tfs_body* b = &w->bodies[c->body_b];
+            int overlapping = 0;
+            for (int i = 0; i < c->count; ++i) {                 /* Contact::touching, internal.h:497-502 */
+                if (c->points[i].separation < 0.0f || c->points[i].normal_impulse > 0.0f) { overlapping = 1; }
+            }
+            L->touching += (uint32_t)overlapping;
+            for (int i = 0; i < c->count; ++i) {
+                const tfs_point* p = &c->points[i];
+                if (p->max_normal_impulse > 0.0f) {
+                    touching = 1;
+                    if (-p->relative_velocity > fastest && (moving(a) || moving(b))) {
+                        fastest = -p->relative_velocity;
+                        struck = (int)last;
+                        point = i;
+                    }
+                } else if (p->separation < apart) {
+                    touching = 1;
+                }
+            }
+        }
+        if (touching != was) {
+            L->touch_events[L->n_touch_events++] = (first << 1) | (uint32_t)touching;
+            for (uint32_t k = first; k < last; ++k) { w->contacts[k].touched = (uint8_t)touching; }
+        }
+        if (struck >= 0) {
+            const tfs_contact* c = &w->contacts[struck];
+            const tfs_body* a = &w->bodies[c->body_a];
+            tfs_hit* h = &L->hits[L->n_hits++];
+            h->contact = (uint32_t)struck;
+            h->point = (uint32_t)point;
+            st3(h->position, add(v3(a->p), rotate(q4(a->q), v3(c->points[point].local_a))));   /* math.h:236 */
+            for (int d = 0; d < 3; ++d) { h->normal[d] = c->normal[d]; }
+            h->speed = fastest;
+        }
+    }
+}
+
+/* src/world.cpp:1564-1691 from the step state on, reportTouches (1693), then updateSleep (1724). */
 int tfo_step(tfs_body* bodies, uint32_t nb, tfs_contact* contacts, uint32_t nc, const tfs_settings* s, float dt,
              tfs_refresh_fn cb, void* user, tfo_lists* L) {
     if (!(dt > 0.0f) || !isfinite(dt)) { return 0; }
@@ -1138,6 +1194,7 @@ int tfo_step(tfs_body* bodies, uint32_t nb, tfs_contact* contacts, uint32_t nc, 
     }
     limitSpeeds(&w);
 
+    if (L->touch_events) { reportTouches(&w, L); }
     updateSleep(&w, dt);
     return 0;
 }

@@ -30,6 +30,11 @@ typedef struct tfo_lists {
     uint32_t n_rest, n_impact_rest; /* trailing entries of by_color / impact_by_color */
     uint32_t n_colors;              /* highest colour used + 1 */
     uint32_t points_active, points_impact;
+    /* reportTouches (src/world.cpp:1424-1468), when touch_events is not NULL: events as
+     * (first contact of the pair << 1) | began, hits, and the touching contacts' count */
+    uint32_t* touch_events;
+    tfs_hit* hits;
+    uint32_t n_touch_events, n_hits, touching;
 } tfo_lists;
 
 /* One step; returns 0, or TFS_E_CALLBACK when a refresh is due and refresh is NULL or fails. */

@@ -134,6 +134,7 @@ tfs_contact toContact(const Contact& c) {
     o.count = c.count;
     o.planar = c.planar ? 1 : 0;
     o.laned = c.laned ? 1 : 0;
+    o.touched = c.touched ? 1 : 0;
     for (int i = 0; i < 4; ++i) {
         const ContactPoint& p = c.points[i];
         tfs_point& q = o.points[i];
@@ -344,11 +345,32 @@ void* tfr_create(const tfs_settings* s) {
         ws.restitutionThreshold = s->restitution_threshold;
         ws.sleepThreshold = s->sleep_threshold;
         ws.timeToSleep = s->time_to_sleep;
+        ws.hitThreshold = s->hit_threshold;
     }
     return new Handle(ws);
 }
 
 void tfr_destroy(void* h) { delete (Handle*)h; }
+
+// A default WorldSettings and the Tolerances a World derives from it, as tfs_settings.
+void tfr_default_settings(tfs_settings* o) {
+    std::memset(o, 0, sizeof *o);
+    const World w{ WorldSettings{} };
+    const WorldSettings& s = w.settings();
+    put3(o->gravity, s.gravity);
+    o->length_unit = s.lengthUnit;
+    o->substeps = s.substeps;
+    o->contact_hertz = s.contactHertz;
+    o->contact_damping_ratio = s.contactDampingRatio;
+    o->max_pushout_speed = s.maxPushoutSpeed;
+    o->max_linear_speed = s.maxLinearSpeed;
+    o->max_angular_speed = s.maxAngularSpeed;
+    o->restitution_threshold = s.restitutionThreshold;
+    o->sleep_threshold = s.sleepThreshold;
+    o->time_to_sleep = s.timeToSleep;
+    o->speculative = w.impl_->tol.speculative;
+    o->hit_threshold = s.hitThreshold;
+}
 
 // Makes `h` a copy of `from` (World's copy is a full snapshot, src/world.cpp:1732-1737).
 void tfr_copy(void* h, const void* from) { ((Handle*)h)->world = ((const Handle*)from)->world; }
@@ -461,7 +483,8 @@ void tfr_set_bodies(void* h, const tfs_body* b, uint32_t n) {
     }
 }
 
-// StepProfile of the last step: total, broadphase, narrowphase, solver, continuous, sleep (ms).
+// StepProfile of the last step: total, broadphase, narrowphase, solver, continuous, sleep (ms), and
+// touching contacts.
 void tfr_profile(void* h, float* out) {
     const StepProfile& p = ((Handle*)h)->world.profile();
     out[0] = p.total;
@@ -470,6 +493,39 @@ void tfr_profile(void* h, float* out) {
     out[3] = p.solver;
     out[4] = p.continuous;
     out[5] = p.sleep;
+    out[6] = (float)p.touchingContacts;
+}
+
+// The hit events of the last step (reportTouches, src/world.cpp:1424-1468) as tfs_hit: the contact is
+// the first of the pair's key (the scenes here have one contact per pair), the point is the one whose
+// position the event holds (UINT32_MAX if none matches).
+uint32_t tfr_hits(void* h, tfs_hit* out, uint32_t cap) {
+    World::Impl& w = *((Handle*)h)->world.impl_;
+    std::vector<tfs_hit> v;
+    for (const HitEvent& e : w.hitEvents) {
+        tfs_hit x;
+        std::memset(&x, 0, sizeof x);
+        const uint64_t key = ((uint64_t)e.shapeA.index << 32) | e.shapeB.index;
+        auto it = std::lower_bound(w.contacts.begin(), w.contacts.end(), key,
+                                   [](const Contact& c, uint64_t k) { return c.key < k; });
+        x.contact = it != w.contacts.end() && it->key == key ? (uint32_t)(it - w.contacts.begin()) : UINT32_MAX;
+        x.point = UINT32_MAX;
+        if (x.contact != UINT32_MAX) {
+            const Body& a = w.bodies[it->bodyA];
+            for (int i = 0; i < it->count; ++i) {
+                const Vec3 p = a.xf.apply(it->points[i].localA);
+                if (p.x == e.point.x && p.y == e.point.y && p.z == e.point.z) {
+                    x.point = (uint32_t)i;
+                    break;
+                }
+            }
+        }
+        put3(x.position, e.point);
+        put3(x.normal, e.normal);
+        x.speed = e.speed;
+        v.push_back(x);
+    }
+    return copyOut(v, out, cap);
 }
 
 uint32_t tfr_num_bodies(void* h) { return (uint32_t)((Handle*)h)->world.impl_->bodies.size(); }

@@ -5,6 +5,8 @@ import os
 
 import numpy as np
 
+from twofold_amd import _abi
+
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 LISTS = ("active", "fast", "fast_solid", "impact", "rows", "rest", "impact_rows", "impact_rest")
 
@@ -24,16 +26,33 @@ def save(path, scene, step, rec, settings):
     np.savez_compressed(path, **arrays)
 
 
+def _settings(old):
+    """Settings as the current layout has them: fields the fixture predates take their defaults."""
+    s = _abi.default_settings()
+    for name in old.dtype.names:
+        s[name] = old[name]
+    return s
+
+
+def _contacts(a):
+    """Contacts as the current layout has them. Fixtures recorded before the `touched` byte took one of
+    the two pad bytes hold zero there: the same bytes, read with the current field names."""
+    if a.dtype == _abi.CONTACT_DTYPE:
+        return a
+    assert a.dtype.itemsize == _abi.CONTACT_DTYPE.itemsize
+    return np.frombuffer(np.ascontiguousarray(a).tobytes(), _abi.CONTACT_DTYPE).copy()
+
+
 def _case(path):
     z = np.load(path, allow_pickle=False)
-    case = dict(scene=str(z["scene"]), step=int(z["step"]), dt=np.float32(z["dt"]), settings=z["settings"],
-                pre_bodies=z["pre_bodies"], pre_contacts=z["pre_contacts"], post_bodies=z["post_bodies"],
-                post_contacts=z["post_contacts"], step_bodies=z["step_bodies"], colors=z["colors"],
+    case = dict(scene=str(z["scene"]), step=int(z["step"]), dt=np.float32(z["dt"]), settings=_settings(z["settings"]),
+                pre_bodies=z["pre_bodies"], pre_contacts=_contacts(z["pre_contacts"]), post_bodies=z["post_bodies"],
+                post_contacts=_contacts(z["post_contacts"]), step_bodies=z["step_bodies"], colors=z["colors"],
                 lists={k: z["list_" + k] for k in LISTS}, refreshes=[])
     for i in range(int(z["n_refresh"])):
         case["refreshes"].append(dict(substep=int(z[f"refresh{i}_substep"]),
                                       remaining=np.float32(z[f"refresh{i}_remaining"]),
-                                      again=z[f"refresh{i}_again"], fresh=z[f"refresh{i}_fresh"]))
+                                      again=z[f"refresh{i}_again"], fresh=_contacts(z[f"refresh{i}_fresh"])))
     return case
 
 

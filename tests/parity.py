@@ -126,3 +126,35 @@ def invariants(pre_bodies, pre_contacts, contacts, bodies, lists):
     if len(wrong):
         bad.append(f"island ids at {wrong[:5].tolist()}")
     return bad
+
+
+def compare_touches(rec, got_contacts, events, hits, touching):
+    """reportTouches (src/world.cpp:1424-1468) against the reference's step: every contact's touched flag
+    after the step, the events (the pairs whose flag changed, in contact order, with their new state),
+    the hits (contact, position, normal and speed, bit for bit; the point where the reference's position
+    names one) and the touching contacts' count. Returns a list of mismatches."""
+    bad = []
+    pre, after = rec.pre_contacts, rec.after_contacts
+    if not np.array_equal(after["touched"], np.asarray(got_contacts["touched"])):
+        d = np.nonzero(after["touched"] != np.asarray(got_contacts["touched"]))[0]
+        bad.append(f"touched at {d[:5].tolist()}")
+    keys = pre["key"]
+    head = np.ones(len(keys), bool)
+    head[1:] = keys[1:] != keys[:-1]
+    changed = np.nonzero(head & (pre["touched"] != after["touched"]))[0].astype(np.uint32)
+    want = (changed << 1) | after["touched"][changed].astype(np.uint32)
+    if not np.array_equal(want, np.asarray(events, np.uint32)):
+        bad.append(f"events: want {len(want)}, got {len(events)}")
+    w, g = rec.hits, np.asarray(hits)
+    if len(w) != len(g):
+        bad.append(f"hits: want {len(w)}, got {len(g)}")
+    else:
+        for f in ("contact", "position", "normal", "speed"):
+            if _bits(w[f]).tobytes() != _bits(g[f]).tobytes():
+                bad.append(f"hit {f}")
+        known = w["point"] != 0xFFFFFFFF
+        if not np.array_equal(w["point"][known], g["point"][known]):
+            bad.append("hit point")
+    if touching != rec.touching:
+        bad.append(f"touching contacts: want {rec.touching}, got {touching}")
+    return bad

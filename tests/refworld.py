@@ -68,6 +68,8 @@ def _load_ref():
         lib.tfr_cap_colors.restype = ctypes.c_uint32
         lib.tfr_cap_colors.argtypes = [_vp, _vp, ctypes.c_uint32]
         lib.tfr_cap_ctx.argtypes = [_vp, _fp]
+        lib.tfr_hits.restype = ctypes.c_uint32
+        lib.tfr_hits.argtypes = [_vp, _vp, ctypes.c_uint32]
         lib.tfr_cap_refresh.restype = ctypes.c_uint32
         lib.tfr_cap_refresh.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int), _fp, _vp, _vp,
                                         ctypes.c_uint32]
@@ -80,7 +82,7 @@ class StepRecord:
     """What the reference's solve block saw and did in one step."""
 
     __slots__ = ("pre_bodies", "pre_contacts", "post_bodies", "post_contacts", "step_bodies", "lists",
-                 "colors", "ctx", "refreshes", "dt")
+                 "colors", "ctx", "refreshes", "dt", "after_contacts", "hits", "touching")
 
 
 class RefWorld:
@@ -135,9 +137,16 @@ class RefWorld:
         return self._record(dt)
 
     def profile(self):
-        out = np.zeros(6, np.float32)
+        out = np.zeros(7, np.float32)
         self.lib.tfr_profile(self.h, out.ctypes.data_as(_fp))
-        return dict(zip(("total", "broadphase", "narrowphase", "solver", "continuous", "sleep"), out.tolist()))
+        return dict(zip(("total", "broadphase", "narrowphase", "solver", "continuous", "sleep", "touching"),
+                        out.tolist()))
+
+    def hits(self):
+        n = self.lib.tfr_hits(self.h, None, 0)
+        out = np.zeros(n, _abi.HIT_DTYPE)
+        self.lib.tfr_hits(self.h, _ptr(out), n)
+        return out
 
     def bodies(self):
         n = self.lib.tfr_num_bodies(self.h)
@@ -167,6 +176,10 @@ class RefWorld:
             return a
         r.pre_bodies, r.post_bodies, r.step_bodies = bodies(0), bodies(1), bodies(2)
         r.pre_contacts, r.post_contacts = contacts(0), contacts(1)
+        # after the whole step: the contacts' touched flags as reportTouches left them, its hits and count
+        r.after_contacts = self.contacts()
+        r.hits = self.hits()
+        r.touching = int(self.profile()["touching"])
         names = ("active", "fast", "fast_solid", "impact", "rows", "rest", "impact_rows", "impact_rest")
         r.lists = {}
         for k, name in enumerate(names):
@@ -201,7 +214,9 @@ class _Lists(ctypes.Structure):
                 ("active", "fast", "fast_solid", "impact", "by_color", "impact_by_color")] + [
         ("colors", ctypes.POINTER(ctypes.c_uint8))] + [
         (n, ctypes.c_uint32) for n in ("n_active", "n_fast", "n_fast_solid", "n_impact", "n_rest", "n_impact_rest",
-                                       "n_colors", "points_active", "points_impact")]
+                                       "n_colors", "points_active", "points_impact")] + [
+        ("touch_events", ctypes.POINTER(ctypes.c_uint32)), ("hits", ctypes.c_void_p)] + [
+        (n, ctypes.c_uint32) for n in ("n_touch_events", "n_hits", "touching")]
 
 
 _oracle = None
@@ -257,6 +272,10 @@ def oracle_step(bodies, contacts, settings, dt, refresh=None):
     for k, a in arrays.items():
         setattr(L, k, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
     L.colors = colors.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    events = np.zeros(n, np.uint32)
+    hits = np.zeros(n, _abi.HIT_DTYPE)
+    L.touch_events = events.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+    L.hits = hits.ctypes.data
     cb = REFRESH_FN(refresh) if refresh is not None else REFRESH_FN()
     rc = lib.tfo_step(_ptr(b), len(b), _ptr(c), len(c), settings.ctypes.data_as(_vp), np.float32(dt), cb, None,
                       ctypes.byref(L))
@@ -269,5 +288,6 @@ def oracle_step(bodies, contacts, settings, dt, refresh=None):
         impact_rows=arrays["impact_by_color"][:L.n_impact],
         impact_rest=arrays["impact_by_color"][L.n_impact - L.n_impact_rest:L.n_impact],
         colors=colors[:len(contacts)], n_colors=L.n_colors, points_active=L.points_active,
-        points_impact=L.points_impact)
+        points_impact=L.points_impact, touch_events=events[:L.n_touch_events], hits=hits[:L.n_hits],
+        touching=L.touching)
     return b, c, lists

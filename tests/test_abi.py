@@ -43,8 +43,27 @@ def test_default_settings_are_the_reference_defaults():
         np.testing.assert_array_equal(s[name], want[name], err_msg=name)
 
 
+def test_default_settings_are_the_compiled_references():
+    """tfs_default_settings against a default WorldSettings and the Tolerances a World derives from it
+    (include/twofold/world.h:216-241, src/world.cpp:224-225), read from the reference built here."""
+    import refworld
+
+    if not refworld.ref_available():
+        import pytest
+
+        pytest.skip("reference checker not built (make -C oracle ref)")
+    ref = refworld._load_ref()
+    ref.tfr_default_settings.argtypes = [ctypes.c_void_p]
+    want = np.zeros((), _abi.SETTINGS_DTYPE)
+    ref.tfr_default_settings(want.ctypes.data)
+    got = np.zeros((), _abi.SETTINGS_DTYPE)
+    solver.load_library().tfs_default_settings(got.ctypes.data)
+    for name in _abi.SETTINGS_DTYPE.names:
+        np.testing.assert_array_equal(got[name], want[name], err_msg=name)
+
+
 def test_argument_errors_without_a_device():
     lib = solver.load_library()
     assert lib.tfs_create(0, 1, 1, None) == -1  # TFS_E_ARG, checked before any device call
     assert lib.tfs_last_error(None) == b"no context"
-    assert lib.tfs_abi_version() == 1
+    assert lib.tfs_abi_version() == solver.ABI_VERSION

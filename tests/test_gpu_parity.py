@@ -62,6 +62,7 @@ def run_against_reference(solver, scene, steps=None):
         case = parity.record_dict(rec)
         b, c, lists = gpu_step(solver, w.settings, case)
         bad = parity.compare(case, b, c, lists)
+        bad += parity.compare_touches(rec, c, *solver.touches())
         assert bad == [], f"{scene.name} step {k}: {bad[:5]}"
         checked += 1
     return checked

@@ -36,6 +36,7 @@ def test_oracle_matches_reference_every_step(make):
         b, c, lists = refworld.oracle_step(rec.pre_bodies, rec.pre_contacts, w.settings, scene.dt,
                                            refworld.recorded_refresh(rec.refreshes))
         bad = parity.compare(parity.record_dict(rec), b, c, lists)
+        bad += parity.compare_touches(rec, c, lists["touch_events"], lists["hits"], lists["touching"])
         assert bad == [], f"{scene.name} step {k}: {bad[:5]}"
         checked += 1
     assert checked > 0
@@ -56,6 +57,7 @@ def test_oracle_matches_reference_small_pile():
         b, c, lists = refworld.oracle_step(rec.pre_bodies, rec.pre_contacts, w.settings, scene.dt,
                                            refworld.recorded_refresh(rec.refreshes))
         bad = parity.compare(parity.record_dict(rec), b, c, lists)
+        bad += parity.compare_touches(rec, c, lists["touch_events"], lists["hits"], lists["touching"])
         assert bad == [], f"step {k}: {bad[:5]}"
     assert refreshed > 0 and impacts > 0
 

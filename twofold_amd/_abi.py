@@ -42,11 +42,14 @@ CONTACT_DTYPE = np.dtype(
         ("normal", F, 3), ("tangent1", F, 3), ("tangent2", F, 3),
         ("friction", F), ("restitution", F), ("rolling_resistance", F), ("rolling_radius", F),
         ("rolling_impulse", F, 3), ("plane", F, 3), ("count", np.int32),
-        ("planar", U8), ("laned", U8), ("pad_", U8, 2),
+        ("planar", U8), ("laned", U8), ("touched", U8), ("pad_", U8),
         ("points", POINT_DTYPE, 4),
     ],
     align=True,
 )
+
+# tfs_hit: a hit event of tfs_download_touches
+HIT_DTYPE = np.dtype([("contact", U32), ("point", U32), ("position", F, 3), ("normal", F, 3), ("speed", F)])
 
 # tfs_shape: one convex shape per body for the device refresh (kinds TFS_SHAPE_*)
 SHAPE_DTYPE = np.dtype([("kind", np.uint32), ("half", np.float32, 3), ("radius", np.float32)])
@@ -57,7 +60,7 @@ SETTINGS_DTYPE = np.dtype(
         ("gravity", F, 3), ("length_unit", F), ("substeps", np.int32),
         ("contact_hertz", F), ("contact_damping_ratio", F), ("max_pushout_speed", F),
         ("max_linear_speed", F), ("max_angular_speed", F), ("restitution_threshold", F),
-        ("sleep_threshold", F), ("time_to_sleep", F), ("speculative", F),
+        ("sleep_threshold", F), ("time_to_sleep", F), ("speculative", F), ("hit_threshold", F),
     ],
     align=True,
 )
@@ -85,6 +88,7 @@ def default_settings(**overrides):
     s["sleep_threshold"] = 0.05
     s["time_to_sleep"] = 0.5
     s["speculative"] = 0.02
+    s["hit_threshold"] = 1.0
     for k, v in overrides.items():
         s[k] = v
     return s

@@ -806,7 +806,10 @@ uint32_t tfw_collide(void* h, tfs_body* bodies, uint32_t nb, float dt, const tfs
         toContact(w, ia, bodies[ia], ib, bodies[ib], m, c);
         const tfs_contact* it = std::lower_bound(prev, prev + nprev, c.key,
                                                  [](const tfs_contact& x, uint64_t key) { return x.key < key; });
-        if (it != prev + nprev && it->key == c.key) { carry(*it, c, matchSq); }
+        if (it != prev + nprev && it->key == c.key) {
+            carry(*it, c, matchSq);
+            c.touched = it->touched;   // as updateContacts keeps it for the pair (src/world.cpp:895, 927)
+        }
         hit[(size_t)k] = 1;
     }
     uint32_t n = 0, wakes = 0;

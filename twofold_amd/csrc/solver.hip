@@ -59,6 +59,7 @@ struct DevCounts {
     uint32_t restShape[2][4]; // per rest list: planar contacts, their points, 3D contacts, their points
     uint32_t scanTicket[kScanSlots];
     uint32_t changed[kIslandRounds];
+    uint32_t nTouchEvents, nHits, touching;   // reportTouches of the step
     uint32_t colorStart[TFS_COLORS + 1];
     uint32_t impactColorStart[TFS_COLORS + 1];
     uint32_t colorPoints[TFS_COLORS + 1];
@@ -1414,6 +1415,89 @@ __device__ __forceinline__ void warmBodyRows(uint32_t d, const uint32_t* __restr
     }
 }
 
+// ---- touch events: reportTouches (src/world.cpp:1424-1468) -------------------------------------------
+//
+// One thread per pair: the thread of a pair's first contact (a contact whose key differs from the one
+// before it) walks the pair's contacts in order, as the reference's loop does, decides whether the pair
+// touches and which pushed point approached fastest, flags an event when the pair's state changed
+// (and writes the new state to all its contacts) and a hit when a point was fast enough. Every thread
+// counts its own contact's touching() (src/internal.h:497-502). Flags are compacted by a scan.
+__global__ void k_touches(tfs_contact* contacts, uint32_t nc, const tfs_body* bodies, float hitThreshold, float linearSlop,
+                          uint32_t* touchFlag, uint32_t* hitFlag, tfs_hit* hitData, uint32_t* touching) {
+    uint32_t count = 0;
+    for (uint32_t i = gid(); i < nc; i += gstride()) {
+        const tfs_contact& self = contacts[i];
+        bool overlapping = false;
+        TF_POINTS(k, self.count) {
+            overlapping = overlapping || self.points[k].separation < 0.0f || self.points[k].normal_impulse > 0.0f;
+        }
+        count += overlapping ? 1u : 0u;
+        uint32_t ev = 0, hit = 0;
+        if (i == 0 || contacts[i - 1].key != self.key) {
+            const bool was = self.touched != 0;
+            const float apart = was ? linearSlop : 0.0f;
+            bool touch = false;
+            float fastest = hitThreshold;
+            uint32_t struck = kNoSlot;
+            int point = 0;
+            uint32_t last = i;
+            for (; last < nc && contacts[last].key == self.key; ++last) {
+                const tfs_contact& c = contacts[last];
+                const tfs_body& a = bodies[c.body_a];
+                const tfs_body& b = bodies[c.body_b];
+                const bool solved = movingBody(a) || movingBody(b);
+                for (int k = 0; k < c.count && k < 4; ++k) {
+                    const tfs_point& p = c.points[k];
+                    if (p.max_normal_impulse > 0.0f) {
+                        touch = true;
+                        if (-p.relative_velocity > fastest && solved) {
+                            fastest = -p.relative_velocity;
+                            struck = last;
+                            point = k;
+                        }
+                    } else if (p.separation < apart) {
+                        touch = true;
+                    }
+                }
+            }
+            if (touch != was) {
+                ev = 1;
+                for (uint32_t k = i; k < last; ++k) { contacts[k].touched = touch ? 1 : 0; }
+            }
+            if (struck != kNoSlot) {
+                hit = 1;
+                const tfs_contact& c = contacts[struck];
+                const tfs_body& a = bodies[c.body_a];
+                const V3 at = ld3(a.p) + rotate(ld4(a.q), ld3(c.points[point].local_a));
+                tfs_hit h;
+                h.contact = struck;
+                h.point = (uint32_t)point;
+                st3(h.position, at);
+                for (int d = 0; d < 3; ++d) { h.normal[d] = c.normal[d]; }
+                h.speed = fastest;
+                hitData[i] = h;
+            }
+        }
+        touchFlag[i] = ev;
+        hitFlag[i] = hit;
+    }
+    waveAdd(touching, count);
+}
+
+// Each event gets its pair's new state: (first contact << 1) | began.
+__global__ void k_touch_events(uint32_t* list, const uint32_t* nPtr, const tfs_contact* contacts) {
+    const uint32_t n = *nPtr;
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        const uint32_t i = list[k];
+        list[k] = (i << 1) | (contacts[i].touched ? 1u : 0u);
+    }
+}
+
+__global__ void k_gather_hits(const uint32_t* list, const uint32_t* nPtr, const tfs_hit* hitData, tfs_hit* out) {
+    const uint32_t n = *nPtr;
+    for (uint32_t k = gid(); k < n; k += gstride()) { out[k] = hitData[list[k]]; }
+}
+
 // ---- the fixed-contact model and its adjoint ---------------------------------------------------------
 //
 // The solve block run for several steps over one contact set, fixed at the first step: no
@@ -2529,6 +2613,15 @@ struct tfs_ctx {
     uint32_t* ep = nullptr;
     uint32_t* rank = nullptr;
     uint32_t* rowOf = nullptr;
+    // reportTouches: per contact, the pair's event and hit flags (set on its first contact) and its hit;
+    // the compacted event list and hits
+    uint32_t* touchFlag = nullptr;
+    uint32_t* hitFlag = nullptr;
+    uint32_t* touchList = nullptr;
+    uint32_t* hitList = nullptr;
+    tfs_hit* hitData = nullptr;
+    tfs_hit* hits = nullptr;
+    uint32_t touchCounts[3] = {};   // events, hits, touching contacts of the last step
     uint32_t* hist = nullptr;
     size_t histCap = 0;
     unsigned long long* status = nullptr;  // look-back scan status words
@@ -2690,7 +2783,9 @@ int allocContacts(tfs_ctx* ctx, uint32_t cap) {
         (r = grow(ctx, ctx->restLevelStart[0], (size_t)cap + 1)) ||
         (r = grow(ctx, ctx->restLevelStart[1], (size_t)cap + 1)) || (r = grow(ctx, ctx->restH, 5 * (size_t)cap)) ||
         (r = grow(ctx, ctx->restP, 12 * (size_t)cap)) || (r = grow(ctx, ctx->restPeak, 4 * (size_t)cap)) || (r = grow(ctx, ctx->restAx, 8 * (size_t)cap)) ||
-        (r = grow(ctx, ctx->restPos, cap)) || (r = grow(ctx, ctx->wrestEp, 2 * (size_t)cap))) {
+        (r = grow(ctx, ctx->restPos, cap)) || (r = grow(ctx, ctx->wrestEp, 2 * (size_t)cap)) ||
+        (r = grow(ctx, ctx->touchFlag, cap)) || (r = grow(ctx, ctx->hitFlag, cap)) || (r = grow(ctx, ctx->touchList, cap)) ||
+        (r = grow(ctx, ctx->hitList, cap)) || (r = grow(ctx, ctx->hitData, cap)) || (r = grow(ctx, ctx->hits, cap))) {
         return r;
     }
     if ((r = allocRows(ctx, ctx->rows, cap)) || (r = grow(ctx, ctx->impactHdr, cap)) || (r = grow(ctx, ctx->bep, 2 * (size_t)cap))) {
@@ -3366,6 +3461,18 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     LAUNCH(k_writeback, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion);
     if (timed) { HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream)); }
 
+    // reportTouches (src/world.cpp:1693, 1424-1468): on the contacts and poses the solve block left, before
+    // the sleep pass changes which bodies are awake
+    if (nc > 0) {
+        LAUNCH(k_touches, blocksFor(nc), kThreads, ctx->contacts, nc, ctx->bodies, s.hit_threshold * s.length_unit,
+               0.25f * s.speculative, ctx->touchFlag, ctx->hitFlag, ctx->hitData, &ctx->cnt->touching);
+        const ScanJob touch[2] = { { ctx->touchFlag, nullptr, nc, nullptr, ctx->touchList, &ctx->cnt->nTouchEvents },
+                                   { ctx->hitFlag, nullptr, nc, nullptr, ctx->hitList, &ctx->cnt->nHits } };
+        if ((r = scanJobs(ctx, touch, 2, nc))) { return r; }
+        LAUNCH(k_gather_hits, blocksFor(nc), kThreads, ctx->hitList, &ctx->cnt->nHits, ctx->hitData, ctx->hits);
+        LAUNCH(k_touch_events, blocksFor(nc), kThreads, ctx->touchList, &ctx->cnt->nTouchEvents, ctx->contacts);
+    }
+
     // sleep and islands
     LAUNCH(k_sleep_time, blocksFor(nb), kThreads, ctx->bodies, nb, dt, s.sleep_threshold * s.length_unit, ctx->parent,
            ctx->islandTime);
@@ -3391,12 +3498,20 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->changed[ahead - 1], last, sizeof(uint32_t), hipMemcpyDeviceToHost,
                            ctx->stream));
     HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    static_assert(offsetof(DevCounts, nHits) == offsetof(DevCounts, nTouchEvents) + sizeof(uint32_t) &&
+                      offsetof(DevCounts, touching) == offsetof(DevCounts, nTouchEvents) + 2 * sizeof(uint32_t),
+                  "the touch counts are read back together");
+    HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->nTouchEvents, &ctx->cnt->nTouchEvents, 3 * sizeof(uint32_t),
+                           hipMemcpyDeviceToHost, ctx->stream));
     if (timed) {
         HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
         HIP_TRY(hipEventSynchronize(ctx->ev[2]));
     } else {
         HIP_TRY(hipStreamSynchronize(ctx->stream));
     }
+    ctx->touchCounts[0] = ctx->hostCnt->nTouchEvents;
+    ctx->touchCounts[1] = ctx->hostCnt->nHits;
+    ctx->touchCounts[2] = ctx->hostCnt->touching;
     if (ctx->hostCnt->changed[ahead - 1]) {
         for (;;) {
             if (it + 2 > kIslandRounds) {
@@ -3610,7 +3725,7 @@ int fixedBegin(tfs_ctx* ctx, float dt, uint32_t maxSteps) {
 // current device.
 constexpr uint32_t kNoAllocBodies = TFS_NOALLOC_MAX_BODIES;
 constexpr uint32_t kNoAllocContacts = TFS_NOALLOC_MAX_CONTACTS;
-constexpr size_t kNoAllocScratchBytes = (size_t)24 << 20;
+constexpr size_t kNoAllocScratchBytes = (size_t)28 << 20;
 constexpr int kNoAllocDevices = 16;
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
@@ -3732,6 +3847,7 @@ void tfs_default_settings(tfs_settings* s) {
     s->sleep_threshold = 0.05f;
     s->time_to_sleep = 0.5f;
     s->speculative = 0.02f;
+    s->hit_threshold = 1.0f;
 }
 
 int tfs_create(int device, uint32_t maxBodies, uint32_t maxContacts, tfs_ctx** out) {
@@ -3804,7 +3920,8 @@ void tfs_destroy(tfs_ctx* ctx) {
                      ctx->restDeg, ctx->restStart, ctx->restCursor, ctx->restEp, ctx->restRank, ctx->restLevel,
                      ctx->restCount, ctx->restWord, ctx->restEntries[0], ctx->restEntries[1], ctx->restLevelStart[0],
                      ctx->restLevelStart[1], ctx->restH, ctx->restP, ctx->restPeak, ctx->restAx, ctx->restPos, ctx->wrestDeg,
-                     ctx->wrestStart, ctx->wrestCursor, ctx->wrestEp, ctx->ckBody, ctx->ckRow, ctx->tapeBuf, ctx->adj.bar, ctx->adj.dpBar,
+                     ctx->wrestStart, ctx->wrestCursor, ctx->wrestEp, ctx->touchFlag, ctx->hitFlag, ctx->touchList,
+                     ctx->hitList, ctx->hitData, ctx->hits, ctx->ckBody, ctx->ckRow, ctx->tapeBuf, ctx->adj.bar, ctx->adj.dpBar,
                      ctx->adj.comBar, ctx->adj.impBar, ctx->adj.sepBar, ctx->adj.muBar, ctx->dlcom, ctx->muContact };
     for (void* p : ptrs) {
         if (p) { (void)hipFree(p); }
@@ -4153,6 +4270,20 @@ int tfs_download_list(tfs_ctx* ctx, int kind, void* out, uint32_t cap, uint32_t*
     *count = n;
     const uint32_t m = n < cap ? n : cap;
     if (m && out) { HIP_TRY(hipMemcpy(out, src, size * m, hipMemcpyDeviceToHost)); }
+    return 0;
+}
+
+int tfs_download_touches(tfs_ctx* ctx, uint32_t* events, uint32_t events_cap, uint32_t* n_events, tfs_hit* hits,
+                         uint32_t hits_cap, uint32_t* n_hits, uint32_t* touching) {
+    if (!ctx) { return TFS_E_ARG; }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const uint32_t ne = ctx->touchCounts[0], nh = ctx->touchCounts[1];
+    if (events && ne) { HIP_TRY(hipMemcpy(events, ctx->touchList, sizeof(uint32_t) * std::min(ne, events_cap), hipMemcpyDeviceToHost)); }
+    if (hits && nh) { HIP_TRY(hipMemcpy(hits, ctx->hits, sizeof(tfs_hit) * std::min(nh, hits_cap), hipMemcpyDeviceToHost)); }
+    if (n_events) { *n_events = ne; }
+    if (n_hits) { *n_hits = nh; }
+    if (touching) { *touching = ctx->touchCounts[2]; }
     return 0;
 }
 

@@ -15,11 +15,12 @@ import weakref
 import numpy as np
 
 from . import _abi
-from ._abi import BODY_DTYPE, CONTACT_DTYPE, SETTINGS_DTYPE, SHAPE_DTYPE
+from ._abi import BODY_DTYPE, CONTACT_DTYPE, HIT_DTYPE, SETTINGS_DTYPE, SHAPE_DTYPE
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtwofold_hip.so")
 
 _vp = ctypes.c_void_p
+ABI_VERSION = 2  # TFS_ABI_VERSION of include/twofold_solver.h this binding mirrors
 REFRESH_FN = ctypes.CFUNCTYPE(ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, ctypes.c_uint32,
                               ctypes.POINTER(ctypes.c_uint32), _vp, _vp, _vp)
 
@@ -29,7 +30,7 @@ EXPORTS = ("tfs_abi_version", "tfs_sizeof_body", "tfs_sizeof_contact", "tfs_size
            "tfs_last_step_ms", "tfs_last_step_counts", "tfs_set_profiling", "tfs_read_profile", "tfs_fixed_begin",
            "tfs_fixed_step", "tfs_adjoint_seed", "tfs_adjoint_step", "tfs_adjoint_read", "tfs_check_minmax",
            "tfs_autotune", "tfs_pass_shape", "tfs_set_pass_shape", "tfs_set_cooperative", "tfs_set_body_shapes",
-           "tfs_step_noalloc")
+           "tfs_step_noalloc", "tfs_download_touches")
 
 
 class SolverError(RuntimeError):
@@ -77,11 +78,13 @@ def load_library(path=LIB_PATH):
     lib.tfs_set_pass_shape.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
     lib.tfs_set_cooperative.argtypes = [_vp, ctypes.c_int]
     lib.tfs_set_body_shapes.argtypes = [_vp, ctypes.c_uint32, _vp]
+    lib.tfs_download_touches.argtypes = [_vp, _vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), _vp,
+                                         ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
     lib.tfs_step_noalloc.argtypes = [ctypes.c_int, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp, ctypes.c_float,
                                      _vp]
     lib.tfs_read_profile.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_double)]
-    if lib.tfs_abi_version() != 1:
+    if lib.tfs_abi_version() != ABI_VERSION:
         raise SolverError("ABI version mismatch")
     _abi.check_sizes(lib, "tfs")
     if lib.tfs_sizeof_settings() != SETTINGS_DTYPE.itemsize:
@@ -389,6 +392,19 @@ class Solver:
         gm = np.zeros(max(self.nc, 1), np.float32)
         self._check(self.lib.tfs_adjoint_read(self.h, _ptr(gv), _ptr(gw), _ptr(gm)), "tfs_adjoint_read")
         return gv, gw, gm[:self.nc]
+
+    def touches(self):
+        """Touch events of the last step (reportTouches, src/world.cpp:1424-1468): events as
+        (first contact of the pair << 1) | began, hits (HIT_DTYPE), and the touching contacts' count."""
+        ne, nh, touching = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        self._check(self.lib.tfs_download_touches(self.h, None, 0, ctypes.byref(ne), None, 0, ctypes.byref(nh),
+                                                  ctypes.byref(touching)), "tfs_download_touches")
+        events = np.zeros(ne.value, np.uint32)
+        hits = np.zeros(nh.value, HIT_DTYPE)
+        self._check(self.lib.tfs_download_touches(self.h, _ptr(events), ne.value, ctypes.byref(ne), _ptr(hits),
+                                                  nh.value, ctypes.byref(nh), ctypes.byref(touching)),
+                    "tfs_download_touches")
+        return events, hits, touching.value
 
     def counts(self):
         """active contacts, their points, impact contacts, their points, colours used, rest contacts, and
