@@ -216,7 +216,7 @@ def finish_tuning(solver, step):
         step()
     solver.autotune(0)
     x = solver.pass_shape()
-    return {"blocks": x["blocks"], "tail_rows": x["tail_rows"], "made_axes": x["made_axes"]}
+    return {"blocks": x["blocks"], "tail_rows": x["tail_rows"], "made_axes": x["made_axes"], "fused_substep": x["fused"]}
 
 
 def run_batched(args):
@@ -244,6 +244,8 @@ def _run_batched(args, worlds, sim, t0):
         if (k + 1) % 10 == 0:
             log(f"settle step {k + 1}: {len(sim.contacts)} contacts, {time.perf_counter() - t0:.1f} s")
     shape = finish_tuning(sim.solver, step)
+    # the substep loop of each step as one HIP graph (the refreshes collide on the device)
+    sim.solver.set_graph(args.graph)
     for _ in range(args.warmup):
         step()
     wall = 0.0
@@ -252,7 +254,20 @@ def _run_batched(args, worlds, sim, t0):
         wall += step()
         contacts += len(sim.contacts)
     batched = len(worlds) * args.steps / wall
-    log(f"batched: {args.steps} steps in {wall:.3f} s of solve, {batched:.4g} world-steps/s")
+    log(f"batched: {args.steps} steps in {wall:.3f} s of solve, {batched:.4g} world-steps/s (graph {args.graph}, "
+        f"shape {shape})")
+    # the other launch forms, for comparison: the tuned shape with the other graph setting, and the
+    # fused substep kernel in a graph
+    sim.solver.set_graph(not args.graph)
+    other = sum(step() for _ in range(args.compare_steps)) / max(args.compare_steps, 1)
+    tuned = sim.solver.pass_shape()
+    sim.solver.set_pass_shape(min(tuned["blocks"], 128), 64, tuned["made_axes"], True)
+    sim.solver.set_graph(True)
+    fused_graph = sum(step() for _ in range(args.compare_steps)) / max(args.compare_steps, 1)
+    sim.solver.set_pass_shape(tuned["blocks"], tuned["tail_rows"], tuned["made_axes"], tuned["fused"])
+    sim.solver.set_graph(args.graph)
+    log(f"batched with graph {not args.graph}: {1e3 * other:.3f} ms per step; fused substeps in a graph: "
+        f"{1e3 * fused_graph:.3f} ms per step")
     sim.solver.set_profiling(True)
     for _ in range(args.batch_profile_steps):
         step()
@@ -321,6 +336,12 @@ def _run_batched(args, worlds, sim, t0):
             "worlds": len(worlds), "bodies": len(sim.bodies), "contacts_per_step": contacts / args.steps,
             "looping_world_steps_per_s": looping, "looping_sample_worlds": len(sample),
             "batched_over_looping": batched / looping, "pass_shape": shape,
+            "graph": bool(args.graph),
+            ("ms_per_step_without_graph" if args.graph else "ms_per_step_with_graph"): 1e3 * other,
+            "ms_per_step_fused_substeps_in_graph": 1e3 * fused_graph,
+            "launch_forms_note": "the tuned shape (the autotuner's candidates include the fused substep kernel) "
+                                 "without and with the per-step HIP graph of the substep loop, and the fused "
+                                 "substep kernel in a graph; each timed on its own steps of the same run",
             "refreshes_per_step": refreshes / max(args.batch_profile_steps, 1),
             "refresh_pairs_per_step": pairs / max(args.batch_profile_steps, 1),
             "refresh_wait_ms_per_step": wait_ms / max(args.batch_profile_steps, 1),
@@ -656,6 +677,9 @@ def main():
                     help="all: seconds after which the workloads not yet run are skipped (noted in the line)")
     ap.add_argument("--log-every", type=int, default=20, help="settle steps between progress lines")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", type=int, default=0,
+                    help="batched: launch each step's substep loop as one HIP graph (1/0; the other form is timed too)")
+    ap.add_argument("--compare-steps", type=int, default=4, help="batched: steps timed with the other launch form")
     ap.add_argument("--refresh", choices=("device", "host"), default="device",
                     help="where mid-step refreshes collide their pairs: on the device (tfs_set_body_shapes; "
                          "spheres, boxes and half-spaces) or through the host world's callback")

@@ -328,6 +328,8 @@ int tfs_last_step_ms(tfs_ctx* ctx, float* solve_ms, float* sleep_ms);
  * lists, active and impact), event-timed per pass over one list, with bytes per contact of 248 + 68n
  * (planar) or 224 + 296n (3D). Family 6 is the reverse sweep's passes (tfs_adjoint_step), event-timed per
  * pass, with 440 + 348n bytes per row (the solve's row, its record, the velocity and impulse adjoints).
+ * Family 7 is the fused substeps (tfs_set_pass_shape), event-timed per launch, with the warm start's and
+ * both solve passes' bytes.
  * Turning it on clears the sums; turning it off keeps them for reading.
  */
 int tfs_set_profiling(tfs_ctx* ctx, int on);
@@ -344,19 +346,31 @@ int tfs_read_profile(tfs_ctx* ctx, int family, uint64_t* launches, double* ms, d
  * each run one candidate, in rotating order, and then keeps the shape whose steps spent the least
  * time in their solve passes (coloured and rest, event-timed; median over its rounds); a step with
  * no solve pass does not count. rounds = 0 stops tuning and keeps the current shape. The fixed-contact
- * model's steps run the current shape and do not count. tfs_pass_shape reads shape[5]: blocks, tail rows (0: off), made axes (0/1), steps
- * of tuning left, candidates (0 when not tuning). tfs_set_pass_shape sets a shape and stops tuning
- * (TFS_E_ARG for blocks outside 1..the resident limit or tail_rows above 256).
+ * model's steps run the current shape and do not count. A candidate's time is that of the step's substep loop
+ * (refreshes, substep starts, passes, positions), event-timed. The block count sets how many rows
+ * each wavefront solves per colour (a colour of R rows over B blocks of 4 wavefronts: 64 x
+ * ceil(R / 256B)). tfs_pass_shape reads shape[7]: blocks, tail rows (0: off), made axes (0/1), steps
+ * of tuning left, candidates (0 when not tuning), fused substep (0/1), graph (0/1). tfs_set_pass_shape sets a shape and stops tuning
+ * (TFS_E_ARG for blocks outside 1..the resident limit, beyond the fused launch's when fused, or
+ * tail_rows above 256).
  */
 int tfs_autotune(tfs_ctx* ctx, uint32_t rounds);
+/* The batched form's launch (no counterpart in the reference). A substep with no rest list can run as
+ * one persistent launch (the fused substep: velocities integrated and every coloured row's warm start,
+ * the biased pass, the positions, the relax pass, with a grid barrier between the phases); it is a
+ * fourth part of the pass shape, which tfs_autotune also tries (each block count with loaded and made
+ * axes, tail 64). tfs_set_graph(ctx, 1) captures each step's substep loop (its refreshes, substep starts,
+ * passes and positions) in a HIP graph and launches it as one, when no refresh of the step needs the
+ * host callback and neither profiling nor tuning is on. Results are the same bits either way. */
+int tfs_set_graph(tfs_ctx* ctx, int on);
 /* The passes with a grid barrier are launched as plain kernels by default: their grid is at most half
  * of what the device holds at once and a stream runs one kernel at a time, so all their blocks are
  * resident together when nothing else runs on the device. tfs_set_cooperative(ctx, 1) launches them
  * with hipLaunchCooperativeKernel instead, which also holds when other work shares the device, at
  * ~12 us per launch (no counterpart in the reference; the results are the same either way). */
 int tfs_set_cooperative(tfs_ctx* ctx, int on);
-int tfs_pass_shape(tfs_ctx* ctx, uint32_t shape[5]);
-int tfs_set_pass_shape(tfs_ctx* ctx, uint32_t blocks, uint32_t tail_rows, int made_axes);
+int tfs_pass_shape(tfs_ctx* ctx, uint32_t shape[7]);
+int tfs_set_pass_shape(tfs_ctx* ctx, uint32_t blocks, uint32_t tail_rows, int made_axes, int fused);
 
 /* Counts of the last step: active contacts, their points, impact contacts and their points,
  * colours used, rest contacts, and the dependency levels of the active and of the impact rest list

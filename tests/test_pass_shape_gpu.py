@@ -10,10 +10,11 @@ from twofold_amd.world import Simulation
 
 pytestmark = pytest.mark.gpu
 
-def run(scene, shape=None, tune_rounds=0, cooperative=False):
+def run(scene, shape=None, tune_rounds=0, cooperative=False, device_refresh=False, graph=False):
     steps = 180 if scene.name.startswith("pile") else 120
-    sim = Simulation(scene)
+    sim = Simulation(scene, device_refresh=device_refresh)
     sim.solver.set_cooperative(cooperative)
+    sim.solver.set_graph(graph)
     if shape is not None:
         sim.solver.set_pass_shape(*shape)
     seen = []
@@ -63,3 +64,17 @@ def test_autotune_tries_every_candidate_then_keeps_one(scene, default):
     final = seen[-1]
     assert final["blocks"] in (64, 128, 256) and final["tail_rows"] in (0, 64, 128)
     assert got == default[0]
+
+
+def test_fused_substeps_and_graphs_give_the_same_bits(scene):
+    """The fused substep launch (one persistent kernel per substep) and the substep loop captured in a
+    HIP graph, launched once per step with the eager steps' synchronisations between them, against the
+    separate launches, step for step to the end of the run: with the refreshes on the device, so that
+    no host callback keeps the loop out of the graph."""
+    base, _ = run(scene, device_refresh=True)
+    for shape, graph in (((128, 64, False, True), False), ((64, 64, True, True), False), ((128, 64, False, False), True),
+                         ((128, 64, False, True), True)):
+        print(f"{scene.name}: shape {shape} graph {graph}", flush=True)
+        got, seen = run(scene, shape, device_refresh=True, graph=graph)
+        assert seen[-1]["fused"] == shape[3] and seen[-1]["graph"] == graph
+        assert got == base, (shape, graph)

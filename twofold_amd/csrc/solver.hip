@@ -37,7 +37,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kScanSlots = 16;         // scans in one step
-constexpr int kProfileFamilies = 7;
+constexpr int kProfileFamilies = 8;
 constexpr int kKeys = 5 * TFS_COLORS + 1; // row-order buckets: colour x point count, then the rest
 constexpr uint32_t kSpinLimit = 1u << 26;
 constexpr int kIslandRounds = 64;      // hook-and-jump rounds of the island pass before it gives up
@@ -977,7 +977,9 @@ struct RowData {
     float4 dpA, dqA, dpB, dqB;
 };
 
-template <bool MADE>
+// DPQ: the bodies' dp and dq were written earlier in the same launch (the fused substep kernel), by
+// other blocks: they are loaded coherently.
+template <bool MADE, bool DPQ = false>
 __device__ __forceinline__ void loadRowData(const Rows& rows, uint32_t r, uint4 h, const float4* motion, RowData<MADE>& d) {
     const int count = (int)(h.z & kCountMask);
     d.n = rows.nrm[r];
@@ -995,10 +997,11 @@ __device__ __forceinline__ void loadRowData(const Rows& rows, uint32_t r, uint4 
             }
         }
     }
-    d.dpA = motion[4 * h.x + 2];
-    d.dqA = motion[4 * h.x + 3];
-    d.dpB = motion[4 * h.y + 2];
-    d.dqB = motion[4 * h.y + 3];
+    using X = Access<DPQ>;
+    d.dpA = X::ld(&motion[4 * h.x + 2]);
+    d.dqA = X::ld(&motion[4 * h.x + 3]);
+    d.dpB = X::ld(&motion[4 * h.y + 2]);
+    d.dqB = X::ld(&motion[4 * h.y + 3]);
 }
 
 // rowStep over data already loaded (loadRowData): only what rows of the pass write (velocities,
@@ -1085,7 +1088,9 @@ __device__ __forceinline__ uint32_t tailSlot(uint32_t* key, uint32_t body) {
 // Block 0's part of a pass from segment plan.tailFrom on. With `wait`, the grid phase before it
 // ended with gridArrive: data that no row of the pass writes is loaded before gridWait, velocities and
 // impulses after it.
-template <int MODE, bool MADE, bool TAPE>
+// FUSED: the pass runs inside the fused substep kernel, whose later phases read what the tail writes
+// from other blocks, so the tail's results go back with coherent stores.
+template <int MODE, bool MADE, bool TAPE, bool FUSED = false>
 __device__ __forceinline__ void passTail(const Rows& rows, const uint4* impactHdr, const PassPlan& plan, float4* motion,
                                          const float4* mass, const Ctx& ctx, const Tape& tape, BarrierLines* bar,
                                          const BarrierPhase& phase, DevCounts* cnt, bool wait) {
@@ -1134,7 +1139,7 @@ __device__ __forceinline__ void passTail(const Rows& rows, const uint4* impactHd
     RowData<MADE> d;
     uint32_t sa = kNoSlot, sb = kNoSlot;
     if (row != kNoSlot) {
-        loadRowData<MADE>(rows, row, h, motion, d);
+        loadRowData<MADE, FUSED>(rows, row, h, motion, d);
         sa = tailSlot(slotKey, h.x);
         sb = tailSlot(slotKey, h.y);
         if (sa == kNoSlot || sb == kNoSlot) { atomicOr(&cnt->error, 4u); }
@@ -1229,19 +1234,20 @@ __device__ __forceinline__ void passTail(const Rows& rows, const uint4* impactHd
         __syncthreads();
     }
     // Back to memory: the impulses of every tail row, the velocities of every slot.
+    using Y = Access<FUSED>;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (owner && k < count) {
             const uint32_t j = (uint32_t)k * rows.cap + row;
-            rows.imp[j] = im[k];
-            rows.peak[j] = peak[k];
+            Y::st(&rows.imp[j], im[k]);
+            Y::st1(&rows.peak[j], peak[k]);
         }
     }
     for (uint32_t k = t; k < kTailSlots; k += blockDim.x) {
         const uint32_t b = slotKey[k];
         if (b != kNoSlot) {
-            motion[4 * b] = slotV[k];
-            motion[4 * b + 1] = slotW[k];
+            Y::st(&motion[4 * b], slotV[k]);
+            Y::st(&motion[4 * b + 1], slotW[k]);
         }
     }
 }
@@ -1250,12 +1256,12 @@ __device__ __forceinline__ void passTail(const Rows& rows, const uint4* impactHd
 // thread loads them for its first row of the next segment while it waits at the barrier; after the
 // barrier only the velocities and impulses that earlier segments wrote are loaded.
 // Segments from plan.tailFrom on are the tail (passTail).
-template <int MODE, bool MADE, bool TAPE>
-__global__ void __launch_bounds__(kPassThreads) k_pass(Rows rows, const uint4* impactHdr, PassPlan plan,
-                                                       float4* motion, const float4* mass, Ctx ctx,
-                                                       BarrierLines* bar, BarrierBase base, DevCounts* cnt,
-                                                       Tape tape) {
-    BarrierPhase phase(base);
+// The pass as a device function: k_pass launches it alone; the fused substep kernel (k_substep_fused)
+// runs it between its other phases (FUSED), with the grid barrier's phase carried across them.
+template <int MODE, bool MADE, bool TAPE, bool FUSED>
+__device__ __forceinline__ void passRun(const Rows& rows, const uint4* impactHdr, const PassPlan& plan, float4* motion,
+                                        const float4* mass, const Ctx& ctx, BarrierLines* bar, BarrierPhase& phase,
+                                        DevCounts* cnt, const Tape& tape) {
     const uint32_t g = passSlot(), stride = gstride();
     auto header = [&](uint32_t s, uint32_t i) { return s >= plan.impactFrom ? impactHdr[i] : rows.hdr[i]; };
     const uint32_t grid = plan.tailFrom;
@@ -1263,7 +1269,7 @@ __global__ void __launch_bounds__(kPassThreads) k_pass(Rows rows, const uint4* i
     RowData<MADE> nd;
     if (grid > 0 && plan.begin[0] + g < plan.end[0]) {
         next = header(0, plan.begin[0] + g);
-        loadRowData<MADE>(rows, 0 >= plan.impactFrom ? next.w : plan.begin[0] + g, next, motion, nd);
+        loadRowData<MADE, FUSED>(rows, 0 >= plan.impactFrom ? next.w : plan.begin[0] + g, next, motion, nd);
     }
     for (uint32_t s = 0; s < grid; ++s) {
         const bool impact = s >= plan.impactFrom;
@@ -1275,7 +1281,7 @@ __global__ void __launch_bounds__(kPassThreads) k_pass(Rows rows, const uint4* i
             uint4 h = firstHdr;
             if (i != first) {
                 h = header(s, i);
-                loadRowData<MADE>(rows, impact ? h.w : i, h, motion, nd);
+                loadRowData<MADE, FUSED>(rows, impact ? h.w : i, h, motion, nd);
             }
             rowSolveLoaded<MODE, MADE, TAPE>(rows, impact ? h.w : i, h, nd, motion, mass, ctx, impact ? Tape{} : tape);
         }
@@ -1283,15 +1289,30 @@ __global__ void __launch_bounds__(kPassThreads) k_pass(Rows rows, const uint4* i
             // The next segment's first row: what no row of the pass writes is loaded between arriving
             // at the barrier (which waits for this thread's memory operations) and waiting at it.
             gridArrive(bar, phase);
-            if (more) { loadRowData<MADE>(rows, s + 1 >= plan.impactFrom ? next.w : plan.begin[s + 1] + g, next, motion, nd); }
+            if (more) { loadRowData<MADE, FUSED>(rows, s + 1 >= plan.impactFrom ? next.w : plan.begin[s + 1] + g, next, motion, nd); }
             gridWait(bar, phase, cnt);
         } else if (grid < plan.n) {
             gridArrive(bar, phase);
         }
     }
     if (grid < plan.n && blockIdx.x == 0) {
-        passTail<MODE, MADE, TAPE>(rows, impactHdr, plan, motion, mass, ctx, tape, bar, phase, cnt, grid > 0);
+        passTail<MODE, MADE, TAPE, FUSED>(rows, impactHdr, plan, motion, mass, ctx, tape, bar, phase, cnt, grid > 0);
     }
+}
+
+template <int MODE, bool MADE, bool TAPE>
+__global__ void __launch_bounds__(kPassThreads) k_pass(Rows rows, const uint4* impactHdr, PassPlan plan,
+                                                       float4* motion, const float4* mass, Ctx ctx,
+                                                       BarrierLines* bar, BarrierBase base, DevCounts* cnt,
+                                                       Tape tape) {
+    BarrierPhase phase(base);
+    passRun<MODE, MADE, TAPE, false>(rows, impactHdr, plan, motion, mass, ctx, bar, phase, cnt, tape);
+}
+
+// The barriers a pass launch of `plan` arrives at.
+__host__ __device__ __forceinline__ uint32_t passBarriers(const PassPlan& plan) {
+    if (plan.n == 0) { return 0; }
+    return plan.tailFrom == plan.n ? plan.n - 1 : plan.tailFrom;
 }
 
 // ---- warm start, one body at a time ----------------------------------------------------------------
@@ -2098,6 +2119,61 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
+// ---- the fused substep -------------------------------------------------------------------------------
+//
+// A whole substep without a refresh in one persistent launch (src/world.cpp:1671-1680): the substep
+// start over the bodies (integrateVelocities and the warm start of every coloured row, as
+// k_substep_start), the biased pass, the positions (as k_integrate_positions), the relax pass, with a
+// grid barrier between the phases. Data that a phase writes and a later phase reads from another block
+// (velocities, dp and dq, impulses) goes through coherent accesses, as within a pass; each body stays
+// with one thread in both body phases. It needs every active contact in a colour (no rest list), and
+// all its blocks resident at once, as the pass launches do. The operations and their order are those
+// of the separate launches, so the results are the same bits.
+template <bool MADE>
+__global__ void __launch_bounds__(kPassThreads)
+    k_substep_fused(tfs_body* bodies, uint32_t nb, Rows rows, const uint4* impactHdr, PassPlan bias, PassPlan relax,
+                    float4* motion, const float4* mass, Ctx ctx, const uint32_t* __restrict__ deg,
+                    const uint32_t* __restrict__ start, const uint32_t* __restrict__ ep, BarrierLines* bar,
+                    BarrierBase base, DevCounts* cnt) {
+    using X = Access<true>;
+    BarrierPhase phase(base);
+    for (uint32_t x = gid(); x < nb; x += gstride()) {
+        tfs_body& body = bodies[x];
+        const bool integrate = body.alive && body.type == TFS_DYNAMIC && body.awake;
+        const uint32_t d = deg[x];
+        if (!integrate && d == 0) { continue; }
+        Motion m = loadMotion(motion, x);
+        if (integrate) {
+            float kept = body.kept;
+            integrateVelocity(body, ctx, m.v, m.w, kept);
+            body.kept = kept;
+        }
+        if (d > 0) { warmBodyRows(d, ep + start[x], rows, m); }
+        X::st(&motion[4 * x], make_float4(m.v.x, m.v.y, m.v.z, 0.0f));
+        X::st(&motion[4 * x + 1], make_float4(m.w.x, m.w.y, m.w.z, 0.0f));
+    }
+    gridSync(bar, phase, cnt);
+    passRun<kBias, MADE, false, true>(rows, impactHdr, bias, motion, mass, ctx, bar, phase, cnt, Tape{});
+    gridSync(bar, phase, cnt);
+    for (uint32_t x = gid(); x < nb; x += gstride()) {
+        tfs_body& b = bodies[x];
+        if (!b.alive) { continue; }
+        const float4 v = X::ld(&motion[4 * x]), w = X::ld(&motion[4 * x + 1]);
+        const float4 dp = motion[4 * x + 2], dq = motion[4 * x + 3];
+        Motion m{ vec(v.x, v.y, v.z), vec(w.x, w.y, w.z), vec(dp.x, dp.y, dp.z), quat(dq.x, dq.y, dq.z, dq.w) };
+        float kept = b.kept;
+        limitSpeed(b, m.v, m.w, kept, ctx.maxSpeed, ctx.maxSpin);
+        b.kept = kept;
+        if (b.type != TFS_STATIC && !(b.type == TFS_DYNAMIC && !b.awake)) { integratePosition(b, m, ctx.h); }
+        X::st(&motion[4 * x], make_float4(m.v.x, m.v.y, m.v.z, 0.0f));
+        X::st(&motion[4 * x + 1], make_float4(m.w.x, m.w.y, m.w.z, 0.0f));
+        X::st(&motion[4 * x + 2], make_float4(m.dp.x, m.dp.y, m.dp.z, 0.0f));
+        X::st(&motion[4 * x + 3], make_float4(m.dq.x, m.dq.y, m.dq.z, m.dq.w));
+    }
+    gridSync(bar, phase, cnt);
+    passRun<kRelax, MADE, false, true>(rows, impactHdr, relax, motion, mass, ctx, bar, phase, cnt, Tape{});
+}
+
 // The rest list level after level. ONE_BLOCK: one work-group walks the levels with __syncthreads
 // between them (narrow levels: the block barrier is cheap and one CU's accesses need no coherence
 // protocol); otherwise one launch over many blocks with the grid barrier, velocities crossing blocks through
@@ -2533,7 +2609,7 @@ struct Buf {
 struct tfs_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {};
+    hipEvent_t ev[6] = {};   // step start, solve end, sleep end, refresh gather, substep loop start / end
     std::string error;
     tfs_settings settings{};
     uint32_t nb = 0, nc = 0, capB = 0, capC = 0;
@@ -2639,6 +2715,12 @@ struct tfs_ctx {
     bool passTail = true;
     uint32_t tailSegRows = kTailSegRows;   // widest colour the tail takes
     bool madeAxes = false;
+    // A substep without a refresh and without rest lists in one launch (k_substep_fused), and the
+    // substep loop of a step captured in a HIP graph and launched as one (both part of the pass shape)
+    bool fused = false;
+    bool useGraph = false;
+    uint32_t fusedBlockLimit = 0;   // blocks of the fused launch: half of what the device holds at once
+    hipGraphExec_t graphExec = nullptr;
     // Autotuning of the pass shape (grid blocks, tail colour width, axes loaded or made): while on,
     // tuning step s runs candidate (s + s / C) % C of the C shapes, and after `tuneRounds` rounds the
     // shape whose steps spent the least time in their solve passes (median) is kept. A step without
@@ -2646,6 +2728,7 @@ struct tfs_ctx {
     struct PassShape {
         uint32_t blocks, tailRows;
         bool made;
+        bool fused;
     };
     std::vector<PassShape> tuneShapes;
     std::vector<std::vector<float>> tuneMs;
@@ -2926,11 +3009,13 @@ int launchGrid(tfs_ctx* ctx, const void* fn, uint32_t grid, void** args) {
 
 // A pass over the listed row sets (active colours, then impact colours when `withImpact`) in one
 // launch with a grid barrier between colours (launchGrid).
-int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool withImpact, const Ctx& c,
-            const Tape& tape = Tape{}) {
-    PassPlan plan{};
+// The plan of one pass over the listed row sets (active colours, then impact colours when
+// `withImpact`), its grid and its algorithmic bytes; plan.n == 0 when there are no rows.
+void planPass(const tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool withImpact, PassPlan& plan,
+              uint32_t& grid, double& bytes) {
+    plan = PassPlan{};
     uint32_t widest = 0;
-    double bytes = 0.0;
+    bytes = 0.0;
     auto add = [&](const uint32_t* starts, const uint32_t* points) {
         for (int col = 0; col < TFS_COLORS; ++col) {
             if (starts[col + 1] <= starts[col]) { continue; }
@@ -2944,7 +3029,8 @@ int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool w
     if (withActive) { add(hc.colorStart, hc.colorPoints); }
     plan.impactFrom = plan.n;
     if (withImpact) { add(hc.impactColorStart, hc.impactColorPoints); }
-    if (plan.n == 0) { return 0; }
+    grid = 1;
+    if (plan.n == 0) { return; }
     // The tail: the trailing colours of at most one wave of rows each, at most one block of rows in
     // all (the active rows and the impact visits), when there are at least two of them.
     uint32_t tailFrom = plan.n, tailRows = 0;
@@ -2958,10 +3044,19 @@ int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool w
     plan.tailFrom = tailFrom;
     widest = 0;
     for (uint32_t s = 0; s < tailFrom; ++s) { widest = std::max(widest, plan.end[s] - plan.begin[s]); }
-    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((widest + 63) / 64, ctx->maxPassBlocks));
-    const uint32_t barriers = tailFrom == plan.n ? plan.n - 1 : tailFrom;
+    grid = std::max<uint32_t>(1, std::min<uint32_t>((widest + 63) / 64, ctx->maxPassBlocks));
+}
+
+int runPass(tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive, bool withImpact, const Ctx& c,
+            const Tape& tape = Tape{}) {
+    PassPlan plan;
+    uint32_t grid = 1;
+    double bytes = 0.0;
+    planPass(ctx, mode, hc, withActive, withImpact, plan, grid, bytes);
+    if (plan.n == 0) { return 0; }
+    const uint32_t barriers = passBarriers(plan);
     size_t ev = 0;
-    const bool timed = ctx->profiling || (ctx->tuning && !ctx->fixedOn);
+    const bool timed = ctx->profiling;
     if (timed) {
         ev = ctx->pending.size() * 2;
         while (ctx->evPool.size() < ev + 2) {
@@ -3041,7 +3136,6 @@ int collectProfile(tfs_ctx* ctx) {
     for (const auto& p : ctx->pending) {
         float ms = 0.0f;
         HIP_TRY(hipEventElapsedTime(&ms, ctx->evPool[p.ev], ctx->evPool[p.ev + 1]));
-        if (ctx->tuning && (p.family == 1 || p.family == 5)) { ctx->tunePassMs += ms; }
         if (!ctx->profiling) { continue; }
         ctx->profLaunches[p.family] += 1;
         ctx->profMs[p.family] += ms;
@@ -3124,7 +3218,7 @@ int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32
     const RestPack rp = restPack(ctx);
     const bool viaPos = which == 1;
     size_t ev = 0;
-    const bool timed = (ctx->profiling || (ctx->tuning && !ctx->fixedOn)) && (mode == kBias || mode == kRelax);
+    const bool timed = ctx->profiling && (mode == kBias || mode == kRelax);
     if (timed) {
         ev = ctx->pending.size() * 2;
         while (ctx->evPool.size() < ev + 2) {
@@ -3341,6 +3435,40 @@ void applyShape(tfs_ctx* ctx, const tfs_ctx::PassShape& x) {
     ctx->passTail = x.tailRows > 0;
     ctx->tailSegRows = x.tailRows > 0 ? x.tailRows : kTailSegRows;
     ctx->madeAxes = x.made;
+    ctx->fused = x.fused;
+}
+
+// One substep in the fused launch (k_substep_fused): the same plans as the separate passes, one grid
+// of at most fusedBlockLimit blocks for all four phases.
+int substepFused(tfs_ctx* ctx, const Ctx& c, const DevCounts& hc) {
+    PassPlan pb, pr;
+    uint32_t unusedGrid = 0;
+    double bytesB = 0.0, bytesR = 0.0;
+    planPass(ctx, kBias, hc, true, hc.nImpact > 0, pb, unusedGrid, bytesB);
+    planPass(ctx, kRelax, hc, true, hc.nImpact > 0, pr, unusedGrid, bytesR);
+    const uint32_t grid = std::max<uint32_t>(1, std::min(ctx->maxPassBlocks, ctx->fusedBlockLimit));
+    size_t ev = 0;
+    if (ctx->profiling) {
+        ev = ctx->pending.size() * 2;
+        while (ctx->evPool.size() < ev + 2) {
+            hipEvent_t x;
+            HIP_TRY(hipEventCreate(&x));
+            ctx->evPool.push_back(x);
+        }
+        ctx->pending.push_back({ 7, ev, rowBytes(kWarm, hc.colorStart[TFS_COLORS], hc.pointsActive) + bytesB + bytesR });
+        HIP_TRY(hipEventRecord(ctx->evPool[ev], ctx->stream));
+    }
+    const BarrierBase base = ctx->barrierBase;
+    const float4* mass = ctx->mass;
+    const uint4* at = ctx->impactHdr;
+    uint32_t nb = ctx->nb;
+    void* args[] = { &ctx->bodies, &nb, &ctx->rows, (void*)&at, &pb, &pr, &ctx->motion, (void*)&mass, (void*)&c, &ctx->bdeg,
+                     &ctx->bstart, &ctx->bep, &ctx->bar, (void*)&base, &ctx->cnt };
+    const void* fn = ctx->madeAxes ? (const void*)k_substep_fused<true> : (const void*)k_substep_fused<false>;
+    if (int e = launchGrid(ctx, fn, grid, args)) { return e; }
+    advanceBarrier(ctx->barrierBase, grid, 3 + passBarriers(pb) + passBarriers(pr));
+    if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
+    return 0;
 }
 
 // The candidate a tuning step runs, and the bookkeeping after it.
@@ -3430,22 +3558,70 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
         }
     }
 
-    // substeps: src/world.cpp:1658-1681
-    for (int i = 0; i < n; ++i) {
-        const bool half = i == n / 2;
-        const uint32_t nAgain = half ? hc.nFast : hc.nFastSolid;
-        if (i > 0 && nAgain > 0) {
-            if ((r = refresh(ctx, i, half, nAgain, (float)(n - i) * c.h, cb, user, hc))) {
-                return r;
+    // substeps: src/world.cpp:1658-1681. A substep with no rest list may run as one fused launch; with
+    // no host callback to wait for, the whole loop may go out as one HIP graph.
+    const bool fuse = ctx->fused && ctx->usePass && hc.nActive == laned && hc.nImpact == impactLaned;
+    bool hostRefresh = false;
+    for (int i = 1; i < n; ++i) { hostRefresh = hostRefresh || (!ctx->nShapes && (i == n / 2 ? hc.nFast : hc.nFastSolid) > 0); }
+    const bool graph = ctx->useGraph && timed && !ctx->profiling && !ctx->tuning && !ctx->coopLaunch && !hostRefresh;
+    auto loop = [&]() -> int {
+        for (int i = 0; i < n; ++i) {
+            const bool half = i == n / 2;
+            const uint32_t nAgain = half ? hc.nFast : hc.nFastSolid;
+            int e;
+            if (i > 0 && nAgain > 0) {
+                if ((e = refresh(ctx, i, half, nAgain, (float)(n - i) * c.h, cb, user, hc))) { return e; }
+            }
+            if (fuse) {
+                if ((e = substepFused(ctx, c, hc))) { return e; }
+                continue;
+            }
+            if ((e = substepStart(ctx, c, laned, hc.pointsActive, hc.nActive > laned)) ||
+                (e = solvePass(ctx, kBias, hc, laned, impactLaned, c))) {
+                return e;
+            }
+            LAUNCH(k_integrate_positions, blocksFor(nb, kBodyThreads), kBodyThreads, ctx->bodies, nb, ctx->motion, c);
+            if ((e = solvePass(ctx, kRelax, hc, laned, impactLaned, c))) { return e; }
+        }
+        return 0;
+    };
+    if (ctx->tuning) { HIP_TRY(hipEventRecord(ctx->ev[4], ctx->stream)); }
+    if (graph) {
+        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed));
+        r = loop();
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+        if (r || ec != hipSuccess) {
+            if (g) { (void)hipGraphDestroy(g); }
+            return r ? r : fail(ctx, TFS_E_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+        }
+        // The last step's executable graph takes this step's parameters when its topology is the same
+        // (the same launches in the same order); otherwise it is made again.
+        bool updated = false;
+        if (ctx->graphExec) {
+            hipGraphExecUpdateResult how;
+            hipGraphNode_t errNode = nullptr;
+            updated = hipGraphExecUpdate(ctx->graphExec, g, &errNode, &how) == hipSuccess && how == hipGraphExecUpdateSuccess;
+            if (!updated) {
+                (void)hipGetLastError();
+                HIP_TRY(hipGraphExecDestroy(ctx->graphExec));
+                ctx->graphExec = nullptr;
             }
         }
-        if ((r = substepStart(ctx, c, laned, hc.pointsActive, hc.nActive > laned)) ||
-            (r = solvePass(ctx, kBias, hc, laned, impactLaned, c))) {
-            return r;
+        if (!updated) {
+            const hipError_t ei = hipGraphInstantiate(&ctx->graphExec, g, nullptr, nullptr, 0);
+            if (ei != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                ctx->graphExec = nullptr;
+                return fail(ctx, TFS_E_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+            }
         }
-        LAUNCH(k_integrate_positions, blocksFor(nb, kBodyThreads), kBodyThreads, ctx->bodies, nb, ctx->motion, c);
-        if ((r = solvePass(ctx, kRelax, hc, laned, impactLaned, c))) { return r; }
+        (void)hipGraphDestroy(g);
+        HIP_TRY(hipGraphLaunch(ctx->graphExec, ctx->stream));
+    } else if ((r = loop())) {
+        return r;
     }
+    if (ctx->tuning) { HIP_TRY(hipEventRecord(ctx->ev[5], ctx->stream)); }
     // restituteBatches + applyRestitution (src/world.cpp:1682-1683): nothing to do unless some active
     // contact has restitution.
     if (hc.anyRestitution && ((r = runRows(ctx, kRestitute, nullptr, hc.colorStart, hc.colorPoints, c)) ||
@@ -3544,7 +3720,11 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     }
     if ((r = collectProfile(ctx))) { return r; }
     if (ctx->tuning) {
-        if (ctx->tunePassMs > 0.0f) { tuneRecord(ctx, ctx->tunePassMs); }
+        // the candidate's time: the whole substep loop (refreshes, substep starts, passes, positions), per
+        // contact point, so that steps of a pile still filling up compare fairly
+        HIP_TRY(hipEventElapsedTime(&ctx->tunePassMs, ctx->ev[4], ctx->ev[5]));
+        const uint32_t points = hc.pointsActive + hc.pointsImpact;
+        if (hc.nActive > 0 && points > 0) { tuneRecord(ctx, ctx->tunePassMs / (float)points); }
         ctx->tunePassMs = 0.0f;
     }
     uint32_t nColors = 0;
@@ -3881,6 +4061,12 @@ int tfs_create(int device, uint32_t maxBodies, uint32_t maxContacts, tfs_ctx** o
         ctx->passBlockLimit = (uint32_t)std::max(1, perCu * cus / 2);
         ctx->maxPassBlocks = std::min<uint32_t>(ctx->passBlockLimit, 128u);
         ctx->usePass = coop != 0 && perCu > 0;
+        int perCuFused = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCuFused, (const void*)k_substep_fused<false>, kPassThreads, 0) !=
+            hipSuccess) {
+            r = TFS_E_DEVICE;
+        }
+        ctx->fusedBlockLimit = (uint32_t)std::max(0, perCuFused * cus / 2);
         // Tuning knobs, read once: TWOFOLD_PASS=launch|coop, TWOFOLD_PASS_BLOCKS=n (at most the resident
         // limit of a grid-barrier launch), TWOFOLD_AXES=packed|made, TWOFOLD_TAIL=on|off,
         // TWOFOLD_ISLAND_AHEAD=n (1..6).
@@ -3891,6 +4077,8 @@ int tfs_create(int device, uint32_t maxBodies, uint32_t maxContacts, tfs_ctx** o
         if (const char* e = getenv("TWOFOLD_ISLAND_AHEAD")) {
             ctx->islandAhead = std::max(1, std::min(atoi(e), kIslandAhead));
         }
+        if (const char* e = getenv("TWOFOLD_FUSED")) { ctx->fused = strcmp(e, "on") == 0 && ctx->fusedBlockLimit > 0; }
+        if (const char* e = getenv("TWOFOLD_GRAPH")) { ctx->useGraph = strcmp(e, "on") == 0; }
         if (const char* e = getenv("TWOFOLD_AXES")) { ctx->madeAxes = strcmp(e, "made") == 0; }
         if (const char* e = getenv("TWOFOLD_TAIL")) { ctx->passTail = strcmp(e, "off") != 0; }
         if (const char* e = getenv("TWOFOLD_COOP")) { ctx->coopLaunch = strcmp(e, "off") != 0; }
@@ -3934,6 +4122,7 @@ void tfs_destroy(tfs_ctx* ctx) {
         if (e) { (void)hipEventDestroy(e); }
     }
     for (hipEvent_t e : ctx->evPool) { (void)hipEventDestroy(e); }
+    if (ctx->graphExec) { (void)hipGraphExecDestroy(ctx->graphExec); }
     if (ctx->stream) { (void)hipStreamDestroy(ctx->stream); }
     delete ctx;
 }
@@ -4077,10 +4266,13 @@ int tfs_autotune(tfs_ctx* ctx, uint32_t rounds) {
     for (uint32_t blocks : { 64u, 128u, 256u }) {
         if (blocks > ctx->passBlockLimit) { continue; }
         for (uint32_t tail : { 0u, 64u, 128u }) {
-            for (bool made : { false, true }) { ctx->tuneShapes.push_back(tfs_ctx::PassShape{ blocks, tail, made }); }
+            for (bool made : { false, true }) { ctx->tuneShapes.push_back(tfs_ctx::PassShape{ blocks, tail, made, false }); }
+        }
+        if (blocks <= ctx->fusedBlockLimit) {
+            for (bool made : { false, true }) { ctx->tuneShapes.push_back(tfs_ctx::PassShape{ blocks, 64u, made, true }); }
         }
     }
-    if (ctx->tuneShapes.empty()) { ctx->tuneShapes.push_back(tfs_ctx::PassShape{ ctx->passBlockLimit, 64u, false }); }
+    if (ctx->tuneShapes.empty()) { ctx->tuneShapes.push_back(tfs_ctx::PassShape{ ctx->passBlockLimit, 64u, false, false }); }
     ctx->tuneMs.assign(ctx->tuneShapes.size(), std::vector<float>());
     ctx->tuneStep = 0;
     ctx->tuneRounds = rounds;
@@ -4088,24 +4280,35 @@ int tfs_autotune(tfs_ctx* ctx, uint32_t rounds) {
     return 0;
 }
 
-int tfs_pass_shape(tfs_ctx* ctx, uint32_t shape[5]) {
+int tfs_pass_shape(tfs_ctx* ctx, uint32_t shape[7]) {
     if (!ctx || !shape) { return TFS_E_ARG; }
     shape[0] = ctx->maxPassBlocks;
     shape[1] = ctx->passTail ? ctx->tailSegRows : 0u;
     shape[2] = ctx->madeAxes ? 1u : 0u;
     shape[3] = ctx->tuning ? (uint32_t)(ctx->tuneRounds * ctx->tuneShapes.size() - ctx->tuneStep) : 0u;
     shape[4] = ctx->tuning ? (uint32_t)ctx->tuneShapes.size() : 0u;
+    shape[5] = ctx->fused ? 1u : 0u;
+    shape[6] = ctx->useGraph ? 1u : 0u;
     return 0;
 }
 
-int tfs_set_pass_shape(tfs_ctx* ctx, uint32_t blocks, uint32_t tail_rows, int made_axes) {
+int tfs_set_pass_shape(tfs_ctx* ctx, uint32_t blocks, uint32_t tail_rows, int made_axes, int fused) {
     if (!ctx) { return TFS_E_ARG; }
     if (blocks == 0 || blocks > ctx->passBlockLimit) {
         return fail(ctx, TFS_E_ARG, "blocks must be in 1..%u (the blocks resident at once)", ctx->passBlockLimit);
     }
     if (tail_rows > (uint32_t)kPassThreads) { return fail(ctx, TFS_E_ARG, "tail_rows must be at most %d", kPassThreads); }
+    if (fused && blocks > ctx->fusedBlockLimit) {
+        return fail(ctx, TFS_E_ARG, "a fused substep takes at most %u blocks", ctx->fusedBlockLimit);
+    }
     ctx->tuning = false;
-    applyShape(ctx, tfs_ctx::PassShape{ blocks, tail_rows, made_axes != 0 });
+    applyShape(ctx, tfs_ctx::PassShape{ blocks, tail_rows, made_axes != 0, fused != 0 });
+    return 0;
+}
+
+int tfs_set_graph(tfs_ctx* ctx, int on) {
+    if (!ctx) { return TFS_E_ARG; }
+    ctx->useGraph = on != 0;
     return 0;
 }
 

@@ -30,7 +30,7 @@ EXPORTS = ("tfs_abi_version", "tfs_sizeof_body", "tfs_sizeof_contact", "tfs_size
            "tfs_last_step_ms", "tfs_last_step_counts", "tfs_set_profiling", "tfs_read_profile", "tfs_fixed_begin",
            "tfs_fixed_step", "tfs_adjoint_seed", "tfs_adjoint_step", "tfs_adjoint_read", "tfs_check_minmax",
            "tfs_autotune", "tfs_pass_shape", "tfs_set_pass_shape", "tfs_set_cooperative", "tfs_set_body_shapes",
-           "tfs_step_noalloc", "tfs_download_touches")
+           "tfs_step_noalloc", "tfs_download_touches", "tfs_set_graph")
 
 
 class SolverError(RuntimeError):
@@ -75,7 +75,8 @@ def load_library(path=LIB_PATH):
     lib.tfs_check_minmax.argtypes = [ctypes.c_int, _vp, _vp, ctypes.c_uint32, _vp]
     lib.tfs_autotune.argtypes = [_vp, ctypes.c_uint32]
     lib.tfs_pass_shape.argtypes = [_vp, _vp]
-    lib.tfs_set_pass_shape.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
+    lib.tfs_set_pass_shape.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
+    lib.tfs_set_graph.argtypes = [_vp, ctypes.c_int]
     lib.tfs_set_cooperative.argtypes = [_vp, ctypes.c_int]
     lib.tfs_set_body_shapes.argtypes = [_vp, ctypes.c_uint32, _vp]
     lib.tfs_download_touches.argtypes = [_vp, _vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), _vp,
@@ -343,10 +344,10 @@ class Solver:
     def pass_shape(self):
         """The pass shape in use: dict of blocks, tail_rows (0 = no tail), made_axes, tuning_steps_left,
         candidates (0 when not tuning)."""
-        out = np.zeros(5, np.uint32)
+        out = np.zeros(7, np.uint32)
         self._check(self.lib.tfs_pass_shape(self.h, _ptr(out)), "tfs_pass_shape")
         return dict(blocks=int(out[0]), tail_rows=int(out[1]), made_axes=bool(out[2]), tuning_steps_left=int(out[3]),
-                    candidates=int(out[4]))
+                    candidates=int(out[4]), fused=bool(out[5]), graph=bool(out[6]))
 
     def set_cooperative(self, on):
         """Launch the grid-barrier passes cooperatively (True) or as plain kernels (False, the default)."""
@@ -362,8 +363,13 @@ class Solver:
         shapes = np.ascontiguousarray(shapes, SHAPE_DTYPE)
         self._check(self.lib.tfs_set_body_shapes(self.h, len(shapes), shapes.ctypes.data_as(_vp)), "tfs_set_body_shapes")
 
-    def set_pass_shape(self, blocks, tail_rows, made_axes):
-        self._check(self.lib.tfs_set_pass_shape(self.h, blocks, tail_rows, int(bool(made_axes))), "tfs_set_pass_shape")
+    def set_pass_shape(self, blocks, tail_rows, made_axes, fused=False):
+        self._check(self.lib.tfs_set_pass_shape(self.h, blocks, tail_rows, int(bool(made_axes)), int(bool(fused))),
+                    "tfs_set_pass_shape")
+
+    def set_graph(self, on):
+        """Launch each step's substep loop as one captured HIP graph (when no host refresh is needed)."""
+        self._check(self.lib.tfs_set_graph(self.h, int(bool(on))), "tfs_set_graph")
 
     # ---- the fixed-contact model and its adjoint ----
 
