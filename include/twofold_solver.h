@@ -149,7 +149,48 @@ typedef struct tfs_settings {
     float time_to_sleep;
     float speculative;          /* Tolerances::speculative (4 x Tolerances::linearSlop) */
     float hit_threshold;        /* approach speed above which a touching point is a hit */
+    float contact_reuse_distance;  /* a pair moved less than this relative to itself keeps its points */
+    float contact_reuse_angle;     /* ... and turned less than this (radians) */
 } tfs_settings;
+
+/* What a contact keeps for its reuse in the next step (src/internal.h:463-503: localNormal,
+ * baseOffset, baseTurn, reusable, part, and per point surfaceA, surfaceB, src/internal.h:428-440), one
+ * per contact beside tfs_contact. tfs_stream_contacts keeps it on the device. */
+typedef struct tfs_contact_ext {
+    float local_normal[3];      /* the normal in body A's frame when the points were collided */
+    float base_offset[3];       /* body B's centre of mass from body A's, in body A's frame, then */
+    float base_turn[4];         /* body B's rotation relative to body A's, then */
+    float surface_a[4][3];      /* each point on body A's surface, in body A's frame */
+    float surface_b[4][3];      /* and on body B's, in body B's frame */
+    uint32_t part;              /* which of the pair's manifolds (a mesh pair has several) */
+    uint8_t reusable;
+    uint8_t pad_[3];
+} tfs_contact_ext;
+
+/* A pair of shapes the broadphase found this step whose bodies are not both resting: the candidates of
+ * updateContacts (src/world.cpp:752-869), in key order, with the contact materials the pair's shapes
+ * mix to (friction, restitution, rolling resistance, src/world.cpp:951-955) and the narrowphase's
+ * manifolds for it, manifolds[first .. first + count) (count 0: nothing found). */
+typedef struct tfs_pair {
+    uint64_t key;
+    uint32_t body_a;
+    uint32_t body_b;
+    float friction;
+    float restitution;
+    float rolling_resistance;
+    uint32_t first;
+    uint32_t count;
+    uint32_t pad_;
+} tfs_pair;
+
+/* One manifold of the narrowphase (Manifold, src/internal.h:327-346): the normal from A to B and up to
+ * four points, each the midpoint of the two surfaces with their separation along the normal. */
+typedef struct tfs_manifold {
+    float normal[3];
+    int32_t count;
+    float position[4][3];
+    float separation[4];
+} tfs_manifold;
 
 /* A hit: the fastest approaching point of a pair that touches in a step, when it approached faster
  * than hit_threshold x length_unit (HitEvent, include/twofold/world.h:278-288). */
@@ -220,6 +261,30 @@ int tfs_set_settings(tfs_ctx* ctx, const tfs_settings* settings);
  * (src/world.cpp:752-993). A contact's body indices must be < n_bodies. */
 int tfs_upload_bodies(tfs_ctx* ctx, const tfs_body* bodies, uint32_t n);
 int tfs_upload_contacts(tfs_ctx* ctx, const tfs_contact* contacts, uint32_t n);
+
+/*
+ * The contact stream: the solver half of updateContacts (src/world.cpp:880-989) on the device, so that
+ * the contacts stay resident from step to step. Before a step, over the resident bodies and the
+ * resident contacts of the last step, for each pair (key order): when the pair had contacts and every
+ * one still fits (stillFits, src/world.cpp:1002-1009: the bodies moved and turned relative to each
+ * other less than the reuse distance and angle since the points were collided), the old contacts are
+ * kept with their points measured again (measureAgain, 1013-1025); otherwise each manifold becomes a
+ * contact (its points on both surfaces, anchors lifted apart when far, 927-943), remembers the pair's
+ * pose (remember, 995-1000) and takes the impulses of the old contact with the closest normal, point by
+ * point (closestNormal, carryImpulses, 543-549, 102-120), and the touched flag. The pair's materials go
+ * to all its contacts. The contacts of resting pairs (no body dynamic and awake, or a moving kinematic
+ * one) stay as they were (969-980), and all are merged in key order; they replace the resident
+ * contacts. The caller collides every candidate pair (a kept pair's manifold is not read) and wakes
+ * bodies as updateContacts does (957-958) before tfs_step. Contacts and their reuse data then never
+ * cross PCIe. TFS_E_ARG for a pair out of key order, a body out of range or a manifold range outside
+ * `manifolds`.
+ */
+int tfs_stream_contacts(tfs_ctx* ctx, const tfs_pair* pairs, uint32_t n_pairs, const tfs_manifold* manifolds,
+                        uint32_t n_manifolds);
+/* The reuse data of the resident contacts (tfs_contact_ext), one per contact in the same order. An upload
+ * sets it for the contacts last uploaded (tfs_upload_contacts leaves every contact not reusable). */
+int tfs_upload_contact_ext(tfs_ctx* ctx, const tfs_contact_ext* ext, uint32_t n);
+int tfs_download_contact_ext(tfs_ctx* ctx, tfs_contact_ext* out, uint32_t n);
 
 /*
  * Batched worlds (no counterpart in the reference, which steps one World at a time): declares that

@@ -705,6 +705,7 @@ typedef struct {
     uint32_t nc;
     const tfs_settings* s;
     Ctx ctx;
+    tfs_contact_ext* ext;   /* the contacts' reuse data, or NULL */
 } World;
 
 /* relativeTravel: src/world.cpp:484-486. */
@@ -913,6 +914,14 @@ static int refreshContacts(World* w, const uint32_t* active, uint32_t na, const 
         const tfs_contact old = *c;
         const tfs_body* a = &w->bodies[c->body_a];
         const tfs_body* b = &w->bodies[c->body_b];
+        if (w->ext) {                                             /* world.cpp:1110, 1117 */
+            tfs_contact_ext* e = &w->ext[again[k]];
+            e->reusable = 0;
+            for (int i = 0; i < (f->count > 4 ? 4 : f->count); ++i) {
+                memset(e->surface_a[i], 0, sizeof e->surface_a[i]);
+                memset(e->surface_b[i], 0, sizeof e->surface_b[i]);
+            }
+        }
         memcpy(c->normal, f->normal, sizeof c->normal);
         memcpy(c->tangent1, f->tangent1, sizeof c->tangent1);
         memcpy(c->tangent2, f->tangent2, sizeof c->tangent2);
@@ -1023,6 +1032,168 @@ static void solveRows(World* w, const uint32_t* rows, uint32_t nLaned, uint32_t 
     }
 }
 
+/* ---- the contact stream: updateContacts' solver half, src/world.cpp:880-989 ---------------------- */
+
+static V3 unrotate(Q4 q, V3 v) { return rotate(conjugate(q), v); }                       /* math.h:176 */
+static float qdot(Q4 a, Q4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; } /* math.h:189 */
+static V3 perpendicular(V3 n) {                                                           /* math.h:70-73 */
+    if (fabsf(n.x) > 0.57735f) { return normalize(vec(n.y, -n.x, 0.0f)); }
+    return normalize(vec(0.0f, n.z, -n.y));
+}
+static V3 apply(const tfs_body* b, V3 v) { return add(v3(b->p), rotate(q4(b->q), v)); }          /* math.h:236 */
+static V3 applyInverse(const tfs_body* b, V3 v) { return unrotate(q4(b->q), sub(v, v3(b->p))); } /* math.h:237 */
+
+static V3 lift(const tfs_body* a, const tfs_body* b, V3 n, float sep, float speculative) {       /* world.cpp:537-540 */
+    if (isPlanar(a) || isPlanar(b)) { return vec(0, 0, 0); }
+    return sep > speculative ? mul(n, 0.5f * sep) : vec(0, 0, 0);
+}
+
+static int stillFits(const tfs_contact_ext* c, const tfs_body* a, const tfs_body* b, const tfs_settings* s) {  /* 1002-1009 */
+    if (!c->reusable) { return 0; }
+    const float reach = s->contact_reuse_distance * s->length_unit;
+    const V3 off = sub(unrotate(q4(a->q), sub(v3(b->com), v3(a->com))), v3(c->base_offset));
+    if (dot(off, off) > reach * reach) { return 0; }
+    const Q4 turn = qmul(conjugate(q4(a->q)), q4(b->q));
+    return fabsf(qdot(turn, q4(c->base_turn))) >= cosf(0.5f * s->contact_reuse_angle);
+}
+
+static void measureAgain(const tfs_body* a, const tfs_body* b, tfs_contact* c, const tfs_contact_ext* e,
+                         float speculative) {                                                  /* 1013-1025 */
+    const V3 n = rotate(q4(a->q), v3(e->local_normal));
+    const V3 t1 = perpendicular(n);
+    st3(c->normal, n);
+    st3(c->tangent1, t1);
+    st3(c->tangent2, cross(n, t1));
+    for (int i = 0; i < c->count; ++i) {
+        tfs_point* p = &c->points[i];
+        const V3 pa = apply(a, v3(e->surface_a[i])), pb = apply(b, v3(e->surface_b[i]));
+        const V3 mid = mul(add(pa, pb), 0.5f);
+        p->separation = dot(sub(pb, pa), n);
+        const V3 up = lift(a, b, n, p->separation, speculative);
+        st3(p->anchor_a, sub(sub(mid, up), v3(a->com)));
+        st3(p->anchor_b, sub(add(mid, up), v3(b->com)));
+        const V3 t = v3(p->tangent_impulse);
+        st3(p->tangent_impulse, sub(t, mul(n, dot(n, t))));
+    }
+}
+
+/* One pair's contacts when it is collided again: 919-949 (remember: 995-1000). */
+static void fromManifold(const tfs_body* a, const tfs_body* b, const tfs_manifold* m, tfs_contact* c, tfs_contact_ext* e,
+                         float speculative) {
+    const V3 n = v3(m->normal);
+    const V3 t1 = perpendicular(n);
+    st3(c->normal, n);
+    st3(c->tangent1, t1);
+    st3(c->tangent2, cross(n, t1));
+    c->count = m->count;
+    for (int i = 0; i < m->count; ++i) {
+        tfs_point* p = &c->points[i];
+        const V3 pos = v3(m->position[i]);
+        const float sep = m->separation[i];
+        const V3 half = mul(n, 0.5f * sep);
+        st3(e->surface_a[i], applyInverse(a, sub(pos, half)));
+        st3(e->surface_b[i], applyInverse(b, add(pos, half)));
+        st3(p->local_a, applyInverse(a, pos));
+        p->separation = sep;
+        const V3 up = lift(a, b, n, sep, speculative);
+        st3(p->anchor_a, sub(sub(pos, up), v3(a->com)));
+        st3(p->anchor_b, sub(add(pos, up), v3(b->com)));
+    }
+    st3(e->local_normal, unrotate(q4(a->q), n));
+    st3(e->base_offset, unrotate(q4(a->q), sub(v3(b->com), v3(a->com))));
+    stq(e->base_turn, qmul(conjugate(q4(a->q)), q4(b->q)));
+    e->reusable = 1;
+}
+
+static int movingAwake(const tfs_body* b);
+
+uint32_t tfo_stream_contacts(const tfs_body* bodies, const tfs_contact* old, const tfs_contact_ext* oldExt, uint32_t nOld,
+                             const tfs_pair* pairs, uint32_t np, const tfs_manifold* manifolds, const tfs_settings* s,
+                             tfs_contact* out, tfs_contact_ext* outExt) {
+    const float speculative = s->speculative;
+    const float matchSq = speculative * speculative;
+    uint32_t n = 0, o = 0;
+    for (uint32_t k = 0; k < np; ++k) {                           /* candidates, key order: 883-959 */
+        const tfs_pair* x = &pairs[k];
+        const tfs_body* a = &bodies[x->body_a];
+        const tfs_body* b = &bodies[x->body_b];
+        while (o < nOld && old[o].key < x->key) { ++o; }
+        uint32_t end = o;
+        while (end < nOld && old[end].key == x->key) { ++end; }
+        int keep = end > o;
+        for (uint32_t i = o; i < end && keep; ++i) { keep = stillFits(&oldExt[i], a, b, s); }
+        const uint8_t touched = end > o ? old[o].touched : 0;
+        const uint32_t first = n;
+        if (keep) {
+            for (uint32_t i = o; i < end; ++i) {
+                out[n] = old[i];
+                outExt[n] = oldExt[i];
+                measureAgain(a, b, &out[n], &outExt[n], speculative);
+                ++n;
+            }
+        } else {
+            for (uint32_t j = 0; j < x->count; ++j) {
+                tfs_contact* c = &out[n];
+                tfs_contact_ext* e = &outExt[n];
+                memset(c, 0, sizeof *c);
+                memset(e, 0, sizeof *e);
+                c->key = x->key;
+                c->body_a = x->body_a;
+                c->body_b = x->body_b;
+                c->touched = touched;
+                e->part = j;
+                fromManifold(a, b, &manifolds[x->first + j], c, e, speculative);
+                const tfs_contact* from = NULL;                      /* closestNormal: 543-549 */
+                for (uint32_t i = o; i < end; ++i) {
+                    if (old[i].count > 0 && (!from || dot(v3(old[i].normal), v3(c->normal)) > dot(v3(from->normal), v3(c->normal)))) {
+                        from = &old[i];
+                    }
+                }
+                if (from) { carryImpulses(from, c, matchSq); }
+                ++n;
+            }
+        }
+        for (uint32_t i = first; i < n; ++i) {                    /* 951-955 */
+            out[i].friction = x->friction;
+            out[i].restitution = x->restitution;
+            out[i].rolling_resistance = x->rolling_resistance;
+        }
+    }
+    /* resting pairs stay (969-980), merged in key order (981-983); the two key sets are disjoint */
+    const uint32_t collided = n;
+    tfs_contact* rest = (tfs_contact*)malloc(sizeof(tfs_contact) * (nOld ? nOld : 1));
+    tfs_contact_ext* restExt = (tfs_contact_ext*)malloc(sizeof(tfs_contact_ext) * (nOld ? nOld : 1));
+    uint32_t nr = 0;
+    for (uint32_t i = 0; i < nOld; ++i) {
+        const tfs_body* a = &bodies[old[i].body_a];
+        const tfs_body* b = &bodies[old[i].body_b];
+        if (movingAwake(a) || movingAwake(b)) { continue; }
+        if (a->type != TFS_DYNAMIC && b->type != TFS_DYNAMIC) { continue; }
+        rest[nr] = old[i];
+        restExt[nr] = oldExt[i];
+        ++nr;
+    }
+    tfs_contact* merged = (tfs_contact*)malloc(sizeof(tfs_contact) * (collided + nr ? collided + nr : 1));
+    tfs_contact_ext* mergedExt = (tfs_contact_ext*)malloc(sizeof(tfs_contact_ext) * (collided + nr ? collided + nr : 1));
+    uint32_t i = 0, j = 0, m = 0;
+    while (i < collided || j < nr) {
+        if (j >= nr || (i < collided && out[i].key < rest[j].key)) {
+            merged[m] = out[i];
+            mergedExt[m++] = outExt[i++];
+        } else {
+            merged[m] = rest[j];
+            mergedExt[m++] = restExt[j++];
+        }
+    }
+    memcpy(out, merged, sizeof(tfs_contact) * m);
+    memcpy(outExt, mergedExt, sizeof(tfs_contact_ext) * m);
+    free(rest);
+    free(restExt);
+    free(merged);
+    free(mergedExt);
+    return m;
+}
+
 /* reportTouches: src/world.cpp:1424-1468. The pairs are the runs of contacts with one key; each run's
  * state changes when a point was pushed (largest normal impulse above zero) or lies closer than zero
  * (the linear slop, speculative / 4, for a pair that touched before); the fastest approaching pushed
@@ -1089,6 +1260,7 @@ int tfo_step(tfs_body* bodies, uint32_t nb, tfs_contact* contacts, uint32_t nc, 
     w.contacts = contacts;
     w.nc = nc;
     w.s = s;
+    w.ext = L->ext;
     const int n = s->substeps > 1 ? s->substeps : 1;
     const float h = dt / (float)n;
 

@@ -35,12 +35,21 @@ typedef struct tfo_lists {
     uint32_t* touch_events;
     tfs_hit* hits;
     uint32_t n_touch_events, n_hits, touching;
+    /* the contacts' reuse data, or NULL: a refresh leaves the contacts it collided again not reusable */
+    tfs_contact_ext* ext;
 } tfo_lists;
 
 /* One step; returns 0, or TFS_E_CALLBACK when a refresh is due and refresh is NULL or fails. */
 int tfo_step(tfs_body* bodies, uint32_t n_bodies, tfs_contact* contacts, uint32_t n_contacts,
              const tfs_settings* settings, float dt, tfs_refresh_fn refresh, void* user,
              tfo_lists* lists);
+
+/* The contact stream (src/world.cpp:880-989): the contacts and reuse data updateContacts leaves, from
+ * the old ones, the bodies, the candidate pairs and their manifolds. `out` and `out_ext` need room for
+ * n_old + the manifolds' count entries. Returns the number of contacts. */
+uint32_t tfo_stream_contacts(const tfs_body* bodies, const tfs_contact* old, const tfs_contact_ext* old_ext, uint32_t n_old,
+                             const tfs_pair* pairs, uint32_t n_pairs, const tfs_manifold* manifolds,
+                             const tfs_settings* settings, tfs_contact* out, tfs_contact_ext* out_ext);
 
 /* The pieces, exposed for focused tests. */
 void tfo_update_inverse_mass(tfs_body* b);

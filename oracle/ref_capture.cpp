@@ -151,6 +151,21 @@ tfs_contact toContact(const Contact& c) {
     return o;
 }
 
+tfs_contact_ext toExt(const Contact& c) {
+    tfs_contact_ext o;
+    std::memset(&o, 0, sizeof o);
+    put3(o.local_normal, c.localNormal);
+    put3(o.base_offset, c.baseOffset);
+    put4(o.base_turn, c.baseTurn);
+    for (int i = 0; i < 4; ++i) {
+        put3(o.surface_a[i], c.points[i].surfaceA);
+        put3(o.surface_b[i], c.points[i].surfaceB);
+    }
+    o.part = c.part;
+    o.reusable = c.reusable ? 1 : 0;
+    return o;
+}
+
 struct Refresh {
     int substep = 0;
     float remaining = 0.0f;
@@ -168,6 +183,13 @@ struct Capture {
     std::vector<uint8_t> colors;
     std::vector<Refresh> refreshes;
     float ctx[12] = {};
+    // updateContacts (src/world.cpp:752-989): the state it started from, its candidate pairs with their
+    // mixed materials and the narrowphase's manifolds for each, and the reuse data it left
+    std::vector<tfs_body> oldBodies;
+    std::vector<tfs_contact> oldContacts;
+    std::vector<tfs_contact_ext> oldExt, preExt;
+    std::vector<tfs_pair> pairs;
+    std::vector<tfs_manifold> manifolds;
 };
 
 struct Handle {
@@ -188,6 +210,7 @@ int g_batchCalls = 0;
 int g_lanesCalls = 0;
 bool g_final = false;
 size_t g_limitCount = 0, g_alive = 0;
+float g_dt = 0.0f;
 
 }
 
@@ -204,6 +227,44 @@ void tfcap_prepareContacts(SolverData& s, const std::vector<uint32_t>& active) {
         c.active = active;
         c.fast = g_impl->fastContacts;
         c.fastSolid = g_impl->fastSolidContacts;
+        c.preExt.clear();
+        for (const Contact& x : s.contacts) { c.preExt.push_back(toExt(x)); }
+        // the candidates of this step's updateContacts, each collided again now: collidePair depends on the
+        // bodies' poses and velocities only, which nothing has changed since
+        World::Impl& w = *g_impl;
+        c.pairs.clear();
+        c.manifolds.clear();
+        std::vector<Manifold> found;
+        for (uint64_t key : w.candidatePairs) {
+            const uint32_t ia = (uint32_t)(key >> 32), ib = (uint32_t)(key & 0xffffffffu);
+            const Shape& sa = w.shapes[ia];
+            const Shape& sb = w.shapes[ib];
+            const Body& ba = w.bodies[sa.body];
+            const Body& bb = w.bodies[sb.body];
+            tfs_pair p;
+            std::memset(&p, 0, sizeof p);
+            p.key = key;
+            p.body_a = sa.body;
+            p.body_b = sb.body;
+            p.friction = w.frictionMix({ sa.friction, ba.type }, { sb.friction, bb.type });
+            p.restitution = w.restitutionMix({ sa.restitution, ba.type }, { sb.restitution, bb.type });
+            p.rolling_resistance = maxf(sa.rollingResistance, sb.rollingResistance);
+            w.collidePair(sa, sb, g_dt, found);
+            p.first = (uint32_t)c.manifolds.size();
+            p.count = (uint32_t)found.size();
+            for (const Manifold& m : found) {
+                tfs_manifold x;
+                std::memset(&x, 0, sizeof x);
+                put3(x.normal, m.normal);
+                x.count = m.count;
+                for (int i = 0; i < m.count; ++i) {
+                    put3(x.position[i], m.points[i].position);
+                    x.separation[i] = m.points[i].separation;
+                }
+                c.manifolds.push_back(x);
+            }
+            c.pairs.push_back(p);
+        }
         g_contactBase = s.contacts.data();
         g_alive = 0;
         for (const Body& b : s.bodies) { g_alive += b.alive ? 1 : 0; }
@@ -346,6 +407,8 @@ void* tfr_create(const tfs_settings* s) {
         ws.sleepThreshold = s->sleep_threshold;
         ws.timeToSleep = s->time_to_sleep;
         ws.hitThreshold = s->hit_threshold;
+        ws.contactReuseDistance = s->contact_reuse_distance;
+        ws.contactReuseAngle = s->contact_reuse_angle;
     }
     return new Handle(ws);
 }
@@ -370,6 +433,8 @@ void tfr_default_settings(tfs_settings* o) {
     o->time_to_sleep = s.timeToSleep;
     o->speculative = w.impl_->tol.speculative;
     o->hit_threshold = s.hitThreshold;
+    o->contact_reuse_distance = s.contactReuseDistance;
+    o->contact_reuse_angle = s.contactReuseAngle;
 }
 
 // Makes `h` a copy of `from` (World's copy is a full snapshot, src/world.cpp:1732-1737).
@@ -448,6 +513,14 @@ int tfr_step(void* h, float dt, int record) {
     Handle& hd = *(Handle*)h;
     hd.cap = Capture{};
     g_cap = record == 1 ? &hd.cap : nullptr;
+    g_dt = dt;
+    if (g_cap) {
+        for (const Body& b : hd.world.impl_->bodies) { g_cap->oldBodies.push_back(toBody(b)); }
+        for (const Contact& x : hd.world.impl_->contacts) {
+            g_cap->oldContacts.push_back(toContact(x));
+            g_cap->oldExt.push_back(toExt(x));
+        }
+    }
     std::memset(hd.counts, 0, sizeof hd.counts);
     g_counts = hd.counts;
     g_impl = hd.world.impl_.get();
@@ -550,11 +623,24 @@ uint32_t tfr_cap_bodies(void* h, int which, tfs_body* out, uint32_t cap) {
     return copyOut(which == 0 ? c.preBodies : which == 1 ? c.postBodies : c.stepBodies, out, cap);
 }
 
-// Recorded contacts: which 0 = at the start of the solve block, 1 = after it.
+// Recorded contacts: which 0 = at the start of the solve block, 1 = after it, 2 = before the step.
 uint32_t tfr_cap_contacts(void* h, int which, tfs_contact* out, uint32_t cap) {
     const Capture& c = ((Handle*)h)->cap;
-    return copyOut(which == 0 ? c.preContacts : c.postContacts, out, cap);
+    return copyOut(which == 0 ? c.preContacts : which == 1 ? c.postContacts : c.oldContacts, out, cap);
 }
+
+// Recorded reuse data: which 0 = at the start of the solve block, 2 = before the step.
+uint32_t tfr_cap_ext(void* h, int which, tfs_contact_ext* out, uint32_t cap) {
+    const Capture& c = ((Handle*)h)->cap;
+    return copyOut(which == 0 ? c.preExt : c.oldExt, out, cap);
+}
+
+// Bodies before the step.
+uint32_t tfr_cap_old_bodies(void* h, tfs_body* out, uint32_t cap) { return copyOut(((Handle*)h)->cap.oldBodies, out, cap); }
+
+// The step's candidate pairs and their manifolds.
+uint32_t tfr_cap_pairs(void* h, tfs_pair* out, uint32_t cap) { return copyOut(((Handle*)h)->cap.pairs, out, cap); }
+uint32_t tfr_cap_manifolds(void* h, tfs_manifold* out, uint32_t cap) { return copyOut(((Handle*)h)->cap.manifolds, out, cap); }
 
 // Recorded lists: 0 active, 1 fast, 2 fastSolid, 3 impact, 4 byColor, 5 rest, 6 impact byColor,
 // 7 impact rest.

@@ -158,3 +158,39 @@ def compare_touches(rec, got_contacts, events, hits, touching):
     if touching != rec.touching:
         bad.append(f"touching contacts: want {rec.touching}, got {touching}")
     return bad
+
+
+def compare_stream(rec, contacts, ext):
+    """The contact stream's result against the contacts and reuse data the reference's updateContacts
+    left (src/world.cpp:880-989), field for field and bit for bit (pad bytes aside)."""
+    bad = []
+    want_c, want_e = rec.pre_contacts, rec.pre_ext
+    if len(want_c) != len(contacts):
+        return [f"contacts: want {len(want_c)}, got {len(contacts)}"]
+    for f in want_c.dtype.names:
+        if f in ("pad_", "points"):
+            continue
+        d = _rows_differ(want_c[f], contacts[f])
+        if len(d):
+            bad.append(f"contact {f} at {d[:5].tolist()}")
+    for i in range(4):
+        m = want_c["count"] > i
+        for f in want_c["points"].dtype.names:
+            d = _rows_differ(want_c["points"][f][:, i][m], contacts["points"][f][:, i][m])
+            if len(d):
+                bad.append(f"point {i} {f} at {np.nonzero(m)[0][d][:5].tolist()}")
+    for f in want_e.dtype.names:
+        if f == "pad_":
+            continue
+        w, g = want_e[f], ext[f]
+        if f in ("surface_a", "surface_b"):
+            for i in range(4):
+                m = want_c["count"] > i
+                d = _rows_differ(w[:, i][m], g[:, i][m])
+                if len(d):
+                    bad.append(f"{f} {i} at {np.nonzero(m)[0][d][:5].tolist()}")
+            continue
+        d = _rows_differ(w, g)
+        if len(d):
+            bad.append(f"ext {f} at {d[:5].tolist()}")
+    return bad

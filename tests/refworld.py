@@ -68,6 +68,11 @@ def _load_ref():
         lib.tfr_cap_colors.restype = ctypes.c_uint32
         lib.tfr_cap_colors.argtypes = [_vp, _vp, ctypes.c_uint32]
         lib.tfr_cap_ctx.argtypes = [_vp, _fp]
+        for name in ("tfr_cap_old_bodies", "tfr_cap_pairs", "tfr_cap_manifolds"):
+            getattr(lib, name).restype = ctypes.c_uint32
+            getattr(lib, name).argtypes = [_vp, _vp, ctypes.c_uint32]
+        lib.tfr_cap_ext.restype = ctypes.c_uint32
+        lib.tfr_cap_ext.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_uint32]
         lib.tfr_hits.restype = ctypes.c_uint32
         lib.tfr_hits.argtypes = [_vp, _vp, ctypes.c_uint32]
         lib.tfr_cap_refresh.restype = ctypes.c_uint32
@@ -82,7 +87,8 @@ class StepRecord:
     """What the reference's solve block saw and did in one step."""
 
     __slots__ = ("pre_bodies", "pre_contacts", "post_bodies", "post_contacts", "step_bodies", "lists",
-                 "colors", "ctx", "refreshes", "dt", "after_contacts", "hits", "touching")
+                 "colors", "ctx", "refreshes", "dt", "after_contacts", "hits", "touching", "old_bodies",
+                 "old_contacts", "old_ext", "pre_ext", "pairs", "manifolds")
 
 
 class RefWorld:
@@ -176,6 +182,18 @@ class RefWorld:
             return a
         r.pre_bodies, r.post_bodies, r.step_bodies = bodies(0), bodies(1), bodies(2)
         r.pre_contacts, r.post_contacts = contacts(0), contacts(1)
+        # updateContacts: the state it started from, its candidate pairs and their manifolds, the reuse data
+        def grab(fn, dtype, *which):
+            n = fn(h, *which, None, 0)
+            a = np.zeros(n, dtype)
+            fn(h, *which, _ptr(a), n)
+            return a
+        r.old_bodies = grab(lib.tfr_cap_old_bodies, BODY_DTYPE)
+        r.old_contacts = grab(lib.tfr_cap_contacts, CONTACT_DTYPE, 2)
+        r.old_ext = grab(lib.tfr_cap_ext, _abi.EXT_DTYPE, 2)
+        r.pre_ext = grab(lib.tfr_cap_ext, _abi.EXT_DTYPE, 0)
+        r.pairs = grab(lib.tfr_cap_pairs, _abi.PAIR_DTYPE)
+        r.manifolds = grab(lib.tfr_cap_manifolds, _abi.MANIFOLD_DTYPE)
         # after the whole step: the contacts' touched flags as reportTouches left them, its hits and count
         r.after_contacts = self.contacts()
         r.hits = self.hits()
@@ -216,7 +234,7 @@ class _Lists(ctypes.Structure):
         (n, ctypes.c_uint32) for n in ("n_active", "n_fast", "n_fast_solid", "n_impact", "n_rest", "n_impact_rest",
                                        "n_colors", "points_active", "points_impact")] + [
         ("touch_events", ctypes.POINTER(ctypes.c_uint32)), ("hits", ctypes.c_void_p)] + [
-        (n, ctypes.c_uint32) for n in ("n_touch_events", "n_hits", "touching")]
+        (n, ctypes.c_uint32) for n in ("n_touch_events", "n_hits", "touching")] + [("ext", ctypes.c_void_p)]
 
 
 _oracle = None
@@ -259,8 +277,23 @@ def recorded_refresh(refreshes):
     return cb
 
 
-def oracle_step(bodies, contacts, settings, dt, refresh=None):
-    """Runs tfo_step in place on copies; returns (bodies, contacts, lists dict)."""
+def oracle_stream(bodies, old, old_ext, pairs, manifolds, settings):
+    """tfo_stream_contacts: the contacts and reuse data the contact stream leaves."""
+    lib = _load_oracle()
+    lib.tfo_stream_contacts.restype = ctypes.c_uint32
+    lib.tfo_stream_contacts.argtypes = [_vp, _vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp, _vp, _vp]
+    cap = max(len(old) + len(manifolds), 1)
+    out = np.zeros(cap, CONTACT_DTYPE)
+    ext = np.zeros(cap, _abi.EXT_DTYPE)
+    args = [np.ascontiguousarray(x) for x in (bodies, old, old_ext, pairs, manifolds)]
+    n = lib.tfo_stream_contacts(_ptr(args[0]), _ptr(args[1]), _ptr(args[2]), len(old), _ptr(args[3]), len(pairs),
+                                _ptr(args[4]), settings.ctypes.data_as(_vp), _ptr(out), _ptr(ext))
+    return out[:n], ext[:n]
+
+
+def oracle_step(bodies, contacts, settings, dt, refresh=None, ext=None):
+    """Runs tfo_step in place on copies; returns (bodies, contacts, lists dict). `ext` (the contacts'
+    reuse data) is updated in place when given."""
     lib = _load_oracle()
     b = np.ascontiguousarray(bodies.copy())
     c = np.ascontiguousarray(contacts.copy())
@@ -276,6 +309,7 @@ def oracle_step(bodies, contacts, settings, dt, refresh=None):
     hits = np.zeros(n, _abi.HIT_DTYPE)
     L.touch_events = events.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
     L.hits = hits.ctypes.data
+    L.ext = ext.ctypes.data if ext is not None else None
     cb = REFRESH_FN(refresh) if refresh is not None else REFRESH_FN()
     rc = lib.tfo_step(_ptr(b), len(b), _ptr(c), len(c), settings.ctypes.data_as(_vp), np.float32(dt), cb, None,
                       ctypes.byref(L))

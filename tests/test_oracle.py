@@ -242,3 +242,34 @@ def test_edges_scene_reaches_its_branches():
             if sb["awake"] and np.linalg.norm(sb["v"]) < 0.05:
                 assert sb["sleep_time"] == 0, "a body carried by a moving kinematic body kept its sleep timer"
     assert all(v > 0 for v in seen.values()), seen
+
+
+@needs_ref
+@pytest.mark.parametrize("make", [lambda: scenes.pile(side=8, half_room=6.0, steps=150), scenes.tumble, scenes.edges,
+                                  lambda: scenes.pyramid(rows=12, steps=150)],
+                         ids=["pile8", "tumble", "edges", "pyramid12"])
+def test_oracle_contact_stream_matches_reference(make):
+    """The contact stream (reuse of the contacts that still fit, points measured again, new manifolds
+    with impulses carried from the closest old contact, resting pairs kept, all merged in key order)
+    against what the reference's updateContacts left, every step, chained: each step streams onto the
+    contacts and reuse data the oracle's own last step left (a mid-step refresh leaves the contacts it
+    collided again not reusable, src/world.cpp:1110, 1117)."""
+    scene = make()
+    w = refworld.RefWorld(scene)
+    kept = fresh = 0
+    prev = None
+    for k in range(scene.steps):
+        rec = w.step()
+        if rec is None:
+            prev = None
+            continue
+        old, old_ext = (rec.old_contacts, rec.old_ext) if prev is None else prev
+        c, e = refworld.oracle_stream(rec.old_bodies, old, old_ext, rec.pairs, rec.manifolds, w.settings)
+        bad = parity.compare_stream(rec, c, e)
+        assert bad == [], f"{scene.name} step {k}: {bad[:5]}"
+        fresh += len(rec.manifolds)
+        kept += int(rec.pre_ext["reusable"].sum())
+        e = e.copy()
+        _, c2, _ = refworld.oracle_step(rec.pre_bodies, c, w.settings, rec.dt, refworld.recorded_refresh(rec.refreshes), ext=e)
+        prev = (c2, e)
+    assert fresh > 0 and kept > 0

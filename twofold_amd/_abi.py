@@ -48,6 +48,14 @@ CONTACT_DTYPE = np.dtype(
     align=True,
 )
 
+# tfs_contact_ext, tfs_pair, tfs_manifold: the contact stream (tfs_stream_contacts)
+EXT_DTYPE = np.dtype([("local_normal", F, 3), ("base_offset", F, 3), ("base_turn", F, 4), ("surface_a", F, (4, 3)),
+                      ("surface_b", F, (4, 3)), ("part", U32), ("reusable", U8), ("pad_", U8, 3)], align=True)
+PAIR_DTYPE = np.dtype([("key", np.uint64), ("body_a", U32), ("body_b", U32), ("friction", F), ("restitution", F),
+                       ("rolling_resistance", F), ("first", U32), ("count", U32), ("pad_", U32)], align=True)
+MANIFOLD_DTYPE = np.dtype([("normal", F, 3), ("count", np.int32), ("position", F, (4, 3)), ("separation", F, 4)],
+                          align=True)
+
 # tfs_hit: a hit event of tfs_download_touches
 HIT_DTYPE = np.dtype([("contact", U32), ("point", U32), ("position", F, 3), ("normal", F, 3), ("speed", F)])
 
@@ -61,6 +69,7 @@ SETTINGS_DTYPE = np.dtype(
         ("contact_hertz", F), ("contact_damping_ratio", F), ("max_pushout_speed", F),
         ("max_linear_speed", F), ("max_angular_speed", F), ("restitution_threshold", F),
         ("sleep_threshold", F), ("time_to_sleep", F), ("speculative", F), ("hit_threshold", F),
+        ("contact_reuse_distance", F), ("contact_reuse_angle", F),
     ],
     align=True,
 )
@@ -89,6 +98,8 @@ def default_settings(**overrides):
     s["time_to_sleep"] = 0.5
     s["speculative"] = 0.02
     s["hit_threshold"] = 1.0
+    s["contact_reuse_distance"] = 0.0005
+    s["contact_reuse_angle"] = 0.035
     for k, v in overrides.items():
         s[k] = v
     return s

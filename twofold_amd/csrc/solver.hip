@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -60,6 +61,7 @@ struct DevCounts {
     uint32_t scanTicket[kScanSlots];
     uint32_t changed[kIslandRounds];
     uint32_t nTouchEvents, nHits, touching;   // reportTouches of the step
+    uint32_t streamPairs, streamRest;         // contacts the stream's pairs and resting contacts leave
     uint32_t colorStart[TFS_COLORS + 1];
     uint32_t impactColorStart[TFS_COLORS + 1];
     uint32_t colorPoints[TFS_COLORS + 1];
@@ -1519,6 +1521,144 @@ __global__ void k_gather_hits(const uint32_t* list, const uint32_t* nPtr, const 
     for (uint32_t k = gid(); k < n; k += gstride()) { out[k] = hitData[list[k]]; }
 }
 
+// ---- the contact stream: updateContacts' solver half (src/world.cpp:880-989) -----------------------
+//
+// Pairs and old contacts are both in key order. Per pair: its old contacts by binary search, whether
+// they are all kept, and how many contacts it leaves; per old contact: whether its pair rests (kept
+// as it is). Scans place both lists, and the two merge by key: a pair's outputs go after the resting
+// contacts with smaller keys, and a resting contact after the outputs of the pairs with smaller keys
+// (the two key sets are disjoint: a pair has a body dynamic and awake or a moving kinematic one, a
+// resting contact has none).
+
+__device__ __forceinline__ uint32_t lowerKey(const tfs_contact* c, uint32_t n, uint64_t key) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (c[mid].key < key) { lo = mid + 1; } else { hi = mid; }
+    }
+    return lo;
+}
+
+__global__ void k_stream_match(const tfs_pair* pairs, uint32_t np, const tfs_contact* old, const tfs_contact_ext* oldExt,
+                               uint32_t nOld, const tfs_body* bodies, float reach, float reuseCos, uint32_t* lo, uint32_t* hi,
+                               uint32_t* keep, uint32_t* count) {
+    for (uint32_t p = gid(); p < np; p += gstride()) {
+        const tfs_pair& x = pairs[p];
+        const uint32_t l = lowerKey(old, nOld, x.key);
+        uint32_t h = l;
+        while (h < nOld && old[h].key == x.key) { ++h; }
+        const tfs_body& a = bodies[x.body_a];
+        const tfs_body& b = bodies[x.body_b];
+        bool k = h > l;
+        for (uint32_t i = l; i < h && k; ++i) { k = stillFits(oldExt[i], a, b, reach, reuseCos); }
+        lo[p] = l;
+        hi[p] = h;
+        keep[p] = k ? 1u : 0u;
+        count[p] = k ? h - l : x.count;
+    }
+}
+
+// src/world.cpp:969-980: an old contact whose bodies are neither dynamic and awake nor moving kinematic,
+// one of them dynamic, stays as it is.
+__global__ void k_stream_resting(const tfs_contact* old, uint32_t nOld, const tfs_body* bodies, uint32_t* flag) {
+    for (uint32_t i = gid(); i < nOld; i += gstride()) {
+        const tfs_body& a = bodies[old[i].body_a];
+        const tfs_body& b = bodies[old[i].body_b];
+        const bool rest = !movingAwake(a, ld3(a.v), ld3(a.w)) && !movingAwake(b, ld3(b.v), ld3(b.w)) &&
+                          (a.type == TFS_DYNAMIC || b.type == TFS_DYNAMIC);
+        flag[i] = rest ? 1u : 0u;
+    }
+}
+
+// Resting contacts (listed) with keys below `key`.
+__device__ __forceinline__ uint32_t restingBelow(const tfs_contact* old, const uint32_t* list, uint32_t n, uint64_t key) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (old[list[mid]].key < key) { lo = mid + 1; } else { hi = mid; }
+    }
+    return lo;
+}
+
+__global__ void k_stream_fill(const tfs_pair* pairs, uint32_t np, const uint32_t* lo, const uint32_t* hi, const uint32_t* keep,
+                              const uint32_t* start, const tfs_manifold* mf, const tfs_contact* old,
+                              const tfs_contact_ext* oldExt, const uint32_t* restList, const uint32_t* nRestPtr,
+                              const tfs_body* bodies, float speculative, float matchSq, tfs_contact* out,
+                              tfs_contact_ext* outExt) {
+    const uint32_t nRest = *nRestPtr;
+    for (uint32_t p = gid(); p < np; p += gstride()) {
+        const tfs_pair& x = pairs[p];
+        const uint32_t base = start[p] + restingBelow(old, restList, nRest, x.key);
+        const tfs_body& a = bodies[x.body_a];
+        const tfs_body& b = bodies[x.body_b];
+        const uint32_t l = lo[p], h = hi[p];
+        uint32_t made = 0;
+        if (keep[p]) {
+            for (uint32_t i = l; i < h; ++i, ++made) {
+                tfs_contact c = old[i];
+                const tfs_contact_ext e = oldExt[i];
+                measureAgain(a, b, c, e, speculative);
+                c.friction = x.friction;
+                c.restitution = x.restitution;
+                c.rolling_resistance = x.rolling_resistance;
+                out[base + made] = c;
+                outExt[base + made] = e;
+            }
+            continue;
+        }
+        const uint8_t touched = h > l ? old[l].touched : 0;
+        for (uint32_t j = 0; j < x.count; ++j, ++made) {
+            tfs_contact c;
+            memset(&c, 0, sizeof c);
+            tfs_contact_ext e;
+            memset(&e, 0, sizeof e);
+            c.key = x.key;
+            c.body_a = x.body_a;
+            c.body_b = x.body_b;
+            c.touched = touched;
+            e.part = j;
+            contactFromManifold(a, b, mf[x.first + j], c, e, speculative);
+            // closestNormal (src/world.cpp:543-549): the old contact with points whose normal is closest
+            uint32_t best = kNoSlot;
+            const V3 n = ld3(c.normal);
+            for (uint32_t i = l; i < h; ++i) {
+                if (old[i].count > 0 && (best == kNoSlot || dot(ld3(old[i].normal), n) > dot(ld3(old[best].normal), n))) {
+                    best = i;
+                }
+            }
+            if (best != kNoSlot) { carryImpulses(old[best], c, matchSq); }
+            c.friction = x.friction;
+            c.restitution = x.restitution;
+            c.rolling_resistance = x.rolling_resistance;
+            out[base + made] = c;
+            outExt[base + made] = e;
+        }
+    }
+}
+
+// Pairs with keys below `key`.
+__device__ __forceinline__ uint32_t pairsBelow(const tfs_pair* pairs, uint32_t n, uint64_t key) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (pairs[mid].key < key) { lo = mid + 1; } else { hi = mid; }
+    }
+    return lo;
+}
+
+__global__ void k_stream_rest_fill(const uint32_t* restList, const uint32_t* nRestPtr, const tfs_contact* old,
+                                   const tfs_contact_ext* oldExt, const tfs_pair* pairs, uint32_t np, const uint32_t* start,
+                                   const uint32_t* pairTotal, tfs_contact* out, tfs_contact_ext* outExt) {
+    const uint32_t nRest = *nRestPtr;
+    for (uint32_t k = gid(); k < nRest; k += gstride()) {
+        const uint32_t i = restList[k];
+        const uint32_t p = pairsBelow(pairs, np, old[i].key);
+        const uint32_t before = p < np ? start[p] : *pairTotal;
+        out[k + before] = old[i];
+        outExt[k + before] = oldExt[i];
+    }
+}
+
 // ---- the fixed-contact model and its adjoint ---------------------------------------------------------
 //
 // The solve block run for several steps over one contact set, fixed at the first step: no
@@ -2413,14 +2553,28 @@ __global__ void __launch_bounds__(kThreads) k_refresh_collide(const uint32_t* ag
 }
 
 // The solver side of each refreshed fast contact (src/world.cpp:1108-1138).
+// A contact collided again is no longer reusable and its points' surface points are cleared
+// (src/world.cpp:1110, 1117: the points start over), in its reuse data `ext`.
 __global__ void __launch_bounds__(kThreads) k_refresh_apply(const uint32_t* again, uint32_t n, const tfs_contact* fresh, tfs_contact* contacts,
-                                const tfs_body* bodies, const float4* motion, const float4* mass, float matchSq) {
+                                const tfs_body* bodies, const float4* motion, const float4* mass, float matchSq,
+                                tfs_contact_ext* ext) {
     for (uint32_t k = gid(); k < n; k += gstride()) {
         tfs_contact& c = contacts[again[k]];
         const tfs_contact& f = fresh[k];
         if (f.count <= 0) {
             c.count = 0;
             continue;
+        }
+        {
+            tfs_contact_ext& e = ext[again[k]];
+            e.reusable = 0;
+            const int m = f.count > 4 ? 4 : f.count;
+            for (int i = 0; i < m; ++i) {
+                for (int j = 0; j < 3; ++j) {
+                    e.surface_a[i][j] = 0.0f;
+                    e.surface_b[i][j] = 0.0f;
+                }
+            }
         }
         // The contact as it was (read in place: c is written once, at the end) and as it is refreshed.
         const tfs_contact& old = c;
@@ -2698,6 +2852,17 @@ struct tfs_ctx {
     tfs_hit* hitData = nullptr;
     tfs_hit* hits = nullptr;
     uint32_t touchCounts[3] = {};   // events, hits, touching contacts of the last step
+    // the contact stream (tfs_stream_contacts): each contact's reuse data, and the stream's inputs, its
+    // per-pair scratch and its output before it replaces the resident contacts
+    tfs_contact_ext* ext = nullptr;
+    tfs_contact* contactsNext = nullptr;
+    tfs_contact_ext* extNext = nullptr;
+    uint32_t streamCap = 0;
+    tfs_pair* pairs = nullptr;
+    uint32_t *pairLo = nullptr, *pairHi = nullptr, *pairKeep = nullptr, *pairStart = nullptr;
+    uint32_t pairCap = 0;
+    tfs_manifold* manifolds = nullptr;
+    uint32_t manifoldCap = 0;
     uint32_t* hist = nullptr;
     size_t histCap = 0;
     unsigned long long* status = nullptr;  // look-back scan status words
@@ -2868,7 +3033,8 @@ int allocContacts(tfs_ctx* ctx, uint32_t cap) {
         (r = grow(ctx, ctx->restP, 12 * (size_t)cap)) || (r = grow(ctx, ctx->restPeak, 4 * (size_t)cap)) || (r = grow(ctx, ctx->restAx, 8 * (size_t)cap)) ||
         (r = grow(ctx, ctx->restPos, cap)) || (r = grow(ctx, ctx->wrestEp, 2 * (size_t)cap)) ||
         (r = grow(ctx, ctx->touchFlag, cap)) || (r = grow(ctx, ctx->hitFlag, cap)) || (r = grow(ctx, ctx->touchList, cap)) ||
-        (r = grow(ctx, ctx->hitList, cap)) || (r = grow(ctx, ctx->hitData, cap)) || (r = grow(ctx, ctx->hits, cap))) {
+        (r = grow(ctx, ctx->hitList, cap)) || (r = grow(ctx, ctx->hitData, cap)) || (r = grow(ctx, ctx->hits, cap)) ||
+        (r = grow(ctx, ctx->ext, cap))) {
         return r;
     }
     if ((r = allocRows(ctx, ctx->rows, cap)) || (r = grow(ctx, ctx->impactHdr, cap)) || (r = grow(ctx, ctx->bep, 2 * (size_t)cap))) {
@@ -3367,7 +3533,7 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
 int refreshApply(tfs_ctx* ctx, const uint32_t* again, uint32_t nAgain, uint32_t laned, uint32_t impactLaned, uint32_t nRest) {
     const float matchSq = ctx->settings.speculative * ctx->settings.speculative;
     LAUNCH(k_refresh_apply, blocksFor(nAgain), kThreads, again, nAgain, ctx->fresh,
-           ctx->contacts, ctx->bodies, ctx->motion, ctx->mass, matchSq);
+           ctx->contacts, ctx->bodies, ctx->motion, ctx->mass, matchSq, ctx->ext);
     // A refreshed contact keeps its row; its points and axes are packed again.
     LAUNCH(k_pack_again, blocksFor(nAgain), kThreads, again, nAgain, ctx->colors, ctx->contacts, ctx->moving, ctx->mass,
            ctx->rows, ctx->rowOf);
@@ -4028,6 +4194,8 @@ void tfs_default_settings(tfs_settings* s) {
     s->time_to_sleep = 0.5f;
     s->speculative = 0.02f;
     s->hit_threshold = 1.0f;
+    s->contact_reuse_distance = 0.0005f;
+    s->contact_reuse_angle = 0.035f;
 }
 
 int tfs_create(int device, uint32_t maxBodies, uint32_t maxContacts, tfs_ctx** out) {
@@ -4109,7 +4277,8 @@ void tfs_destroy(tfs_ctx* ctx) {
                      ctx->restCount, ctx->restWord, ctx->restEntries[0], ctx->restEntries[1], ctx->restLevelStart[0],
                      ctx->restLevelStart[1], ctx->restH, ctx->restP, ctx->restPeak, ctx->restAx, ctx->restPos, ctx->wrestDeg,
                      ctx->wrestStart, ctx->wrestCursor, ctx->wrestEp, ctx->touchFlag, ctx->hitFlag, ctx->touchList,
-                     ctx->hitList, ctx->hitData, ctx->hits, ctx->ckBody, ctx->ckRow, ctx->tapeBuf, ctx->adj.bar, ctx->adj.dpBar,
+                     ctx->hitList, ctx->hitData, ctx->hits, ctx->ext, ctx->contactsNext, ctx->extNext, ctx->pairs,
+                     ctx->pairLo, ctx->pairHi, ctx->pairKeep, ctx->pairStart, ctx->manifolds, ctx->ckBody, ctx->ckRow, ctx->tapeBuf, ctx->adj.bar, ctx->adj.dpBar,
                      ctx->adj.comBar, ctx->adj.impBar, ctx->adj.sepBar, ctx->adj.muBar, ctx->dlcom, ctx->muContact };
     for (void* p : ptrs) {
         if (p) { (void)hipFree(p); }
@@ -4202,8 +4371,121 @@ int tfs_upload_contacts(tfs_ctx* ctx, const tfs_contact* contacts, uint32_t n) {
         int r = allocContacts(ctx, n);
         if (r) { return r; }
     }
-    if (n) { HIP_TRY(hipMemcpy(ctx->contacts, contacts, sizeof(tfs_contact) * n, hipMemcpyHostToDevice)); }
+    if (n) {
+        HIP_TRY(hipMemcpy(ctx->contacts, contacts, sizeof(tfs_contact) * n, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemset(ctx->ext, 0, sizeof(tfs_contact_ext) * n));
+    }
     ctx->nc = n;
+    ctx->haveContacts = true;
+    return 0;
+}
+
+int tfs_upload_contact_ext(tfs_ctx* ctx, const tfs_contact_ext* ext, uint32_t n) {
+    if (!ctx || (!ext && n)) { return fail(ctx, TFS_E_ARG, "null argument"); }
+    if (n != ctx->nc) { return fail(ctx, TFS_E_ARG, "%u reuse records for %u contacts", n, ctx->nc); }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (n) { HIP_TRY(hipMemcpy(ctx->ext, ext, sizeof(tfs_contact_ext) * n, hipMemcpyHostToDevice)); }
+    return 0;
+}
+
+int tfs_download_contact_ext(tfs_ctx* ctx, tfs_contact_ext* out, uint32_t n) {
+    if (!ctx || (!out && n)) { return fail(ctx, TFS_E_ARG, "null argument"); }
+    if (n > ctx->nc) { return fail(ctx, TFS_E_ARG, "%u reuse records asked for, %u contacts resident", n, ctx->nc); }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (n) { HIP_TRY(hipMemcpy(out, ctx->ext, sizeof(tfs_contact_ext) * n, hipMemcpyDeviceToHost)); }
+    return 0;
+}
+
+int tfs_stream_contacts(tfs_ctx* ctx, const tfs_pair* pairs, uint32_t n_pairs, const tfs_manifold* manifolds,
+                        uint32_t n_manifolds) {
+    if (!ctx || (!pairs && n_pairs) || (!manifolds && n_manifolds)) { return fail(ctx, TFS_E_ARG, "null argument"); }
+    if (!ctx->haveBodies) { return fail(ctx, TFS_E_STATE, "upload bodies before streaming contacts"); }
+    ctx->fixedOn = false;
+    const uint32_t nb = ctx->nb, nOld = ctx->haveContacts ? ctx->nc : 0u;
+    for (uint32_t p = 0; p < n_pairs; ++p) {
+        const tfs_pair& x = pairs[p];
+        if (p > 0 && !(pairs[p - 1].key < x.key)) { return fail(ctx, TFS_E_ARG, "pair %u is out of key order", p); }
+        if (x.body_a >= nb || x.body_b >= nb) { return fail(ctx, TFS_E_ARG, "pair %u names body %u/%u of %u", p, x.body_a, x.body_b, nb); }
+        if ((uint64_t)x.first + x.count > n_manifolds) { return fail(ctx, TFS_E_ARG, "pair %u's manifolds are out of range", p); }
+        if (ctx->nWorlds > 1 && ctx->hostBodyWorld[x.body_a] != ctx->hostBodyWorld[x.body_b]) {
+            return fail(ctx, TFS_E_ARG, "pair %u joins bodies of two worlds", p);
+        }
+        for (uint32_t j = 0; j < x.count; ++j) {
+            const int32_t c = manifolds[x.first + j].count;
+            if (c < 1 || c > 4) { return fail(ctx, TFS_E_ARG, "pair %u's manifold %u has %d points", p, j, c); }
+        }
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    int r;
+    const uint32_t most = nOld + n_manifolds;
+    if (most > ctx->streamCap) {
+        if ((r = grow(ctx, ctx->contactsNext, most)) || (r = grow(ctx, ctx->extNext, most))) { return r; }
+        ctx->streamCap = most;
+    }
+    if (n_pairs > ctx->pairCap) {
+        if ((r = grow(ctx, ctx->pairs, n_pairs)) || (r = grow(ctx, ctx->pairLo, n_pairs)) || (r = grow(ctx, ctx->pairHi, n_pairs)) ||
+            (r = grow(ctx, ctx->pairKeep, n_pairs)) || (r = grow(ctx, ctx->pairStart, n_pairs))) {
+            return r;
+        }
+        ctx->pairCap = n_pairs;
+    }
+    if (n_manifolds > ctx->manifoldCap) {
+        if ((r = grow(ctx, ctx->manifolds, n_manifolds))) { return r; }
+        ctx->manifoldCap = n_manifolds;
+    }
+    // the scans' status words cover the larger of the two lists
+    if ((size_t)3 * std::max<size_t>(1, ((size_t)std::max(n_pairs, nOld) + kScanTile - 1) / kScanTile) > ctx->statusCap) {
+        const uint32_t keepB = ctx->capB;
+        ctx->capB = std::max(ctx->capB, std::max(n_pairs, nOld));
+        r = ensureScratch(ctx);
+        ctx->capB = keepB;
+        if (r) { return r; }
+    }
+    if (n_pairs) { HIP_TRY(hipMemcpyAsync(ctx->pairs, pairs, sizeof(tfs_pair) * n_pairs, hipMemcpyHostToDevice, ctx->stream)); }
+    if (n_manifolds) {
+        HIP_TRY(hipMemcpyAsync(ctx->manifolds, manifolds, sizeof(tfs_manifold) * n_manifolds, hipMemcpyHostToDevice, ctx->stream));
+    }
+    HIP_TRY(hipMemsetAsync(ctx->cnt, 0, sizeof(DevCounts), ctx->stream));
+    ctx->scanSlot = 0;
+    const tfs_settings& s = ctx->settings;
+    const float reach = s.contact_reuse_distance * s.length_unit;
+    const float reuseCos = std::cos(0.5f * s.contact_reuse_angle);
+    const float matchSq = s.speculative * s.speculative;
+    if (n_pairs) {
+        LAUNCH(k_stream_match, blocksFor(n_pairs), kThreads, ctx->pairs, n_pairs, ctx->contacts, ctx->ext, nOld, ctx->bodies, reach,
+               reuseCos, ctx->pairLo, ctx->pairHi, ctx->pairKeep, ctx->pairStart);
+    }
+    if (nOld) { LAUNCH(k_stream_resting, blocksFor(nOld), kThreads, ctx->contacts, nOld, ctx->bodies, ctx->fActive); }
+    const ScanJob jobs[2] = { { ctx->pairStart, nullptr, n_pairs, ctx->pairStart, nullptr, &ctx->cnt->streamPairs },
+                              { ctx->fActive, nullptr, nOld, nullptr, ctx->active, &ctx->cnt->streamRest } };
+    if ((r = scanJobs(ctx, jobs, 2, std::max(n_pairs, nOld)))) { return r; }
+    if (n_pairs) {
+        LAUNCH(k_stream_fill, blocksFor(n_pairs), kThreads, ctx->pairs, n_pairs, ctx->pairLo, ctx->pairHi, ctx->pairKeep,
+               ctx->pairStart, ctx->manifolds, ctx->contacts, ctx->ext, ctx->active, &ctx->cnt->streamRest, ctx->bodies,
+               s.speculative, matchSq, ctx->contactsNext, ctx->extNext);
+    }
+    if (nOld) {
+        LAUNCH(k_stream_rest_fill, blocksFor(nOld), kThreads, ctx->active, &ctx->cnt->streamRest, ctx->contacts, ctx->ext,
+               ctx->pairs, n_pairs, ctx->pairStart, &ctx->cnt->streamPairs, ctx->contactsNext, ctx->extNext);
+    }
+    static_assert(offsetof(DevCounts, streamRest) == offsetof(DevCounts, streamPairs) + sizeof(uint32_t),
+                  "the stream's totals are read back together");
+    HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->streamPairs, &ctx->cnt->streamPairs, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->hostCnt->error) { return kernelFail(ctx, ctx->hostCnt->error); }
+    const uint32_t total = (n_pairs ? ctx->hostCnt->streamPairs : 0u) + (nOld ? ctx->hostCnt->streamRest : 0u);
+    if (total > ctx->capC && (r = allocContacts(ctx, total))) { return r; }
+    if (total) {
+        HIP_TRY(hipMemcpyAsync(ctx->contacts, ctx->contactsNext, sizeof(tfs_contact) * total, hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(ctx->ext, ctx->extNext, sizeof(tfs_contact_ext) * total, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->nc = total;
     ctx->haveContacts = true;
     return 0;
 }

@@ -940,4 +940,89 @@ TF_HD void carryImpulses(const tfs_contact& old, tfs_contact& c, float matchSq) 
     c.rolling_impulse[2] = old.rolling_impulse[2];
 }
 
+// ---- the contact stream: the solver half of updateContacts (src/world.cpp:880-1025) ----------------
+
+// Quat::unrotate: include/twofold/math.h:176. Quaternion dot: math.h:189.
+TF_HD V3 unrotate(Q4 q, V3 v) { return rotate(conjugate(q), v); }
+TF_HD float qdot(Q4 a, Q4 b) { return a.x * b.x + a.y * b.y + a.z * b.z + a.w * b.w; }
+// perpendicular: include/twofold/math.h:70-73.
+TF_HD V3 perpendicular(V3 n) {
+    if (fabs_(n.x) > 0.57735f) { return normalize(vec(n.y, -n.x, 0.0f)); }
+    return normalize(vec(0.0f, n.z, -n.y));
+}
+// Transform::apply / applyInverse: include/twofold/math.h:236-237, for a body's pose.
+TF_HD V3 applyPose(const tfs_body& b, V3 v) { return ld3(b.p) + rotate(ld4(b.q), v); }
+TF_HD V3 applyInversePose(const tfs_body& b, V3 v) { return unrotate(ld4(b.q), v - ld3(b.p)); }
+
+// lift: src/world.cpp:537-540. Points further apart than the speculative distance anchor on each surface.
+TF_HD V3 lift(const tfs_body& a, const tfs_body& b, V3 normal, float separation, float speculative) {
+    if (isPlanarLocks(a.locks) || isPlanarLocks(b.locks)) { return vec(0.0f, 0.0f, 0.0f); }
+    return separation > speculative ? normal * (0.5f * separation) : vec(0.0f, 0.0f, 0.0f);
+}
+
+// stillFits: src/world.cpp:1002-1009. `reach` = contactReuseDistance x lengthUnit, `reuseCos` =
+// cos(contactReuseAngle / 2) as the host's cosf gives it.
+TF_HD bool stillFits(const tfs_contact_ext& c, const tfs_body& a, const tfs_body& b, float reach, float reuseCos) {
+    if (!c.reusable) { return false; }
+    const V3 off = unrotate(ld4(a.q), ld3(b.com) - ld3(a.com)) - ld3(c.base_offset);
+    if (dot(off, off) > reach * reach) { return false; }
+    const Q4 turn = conjugate(ld4(a.q)) * ld4(b.q);
+    return fabs_(qdot(turn, ld4(c.base_turn))) >= reuseCos;
+}
+
+// remember: src/world.cpp:995-1000.
+TF_HD void remember(const tfs_body& a, const tfs_body& b, const tfs_contact& c, tfs_contact_ext& e) {
+    const Q4 qa = ld4(a.q);
+    st3(e.local_normal, unrotate(qa, ld3(c.normal)));
+    st3(e.base_offset, unrotate(qa, ld3(b.com) - ld3(a.com)));
+    st4(e.base_turn, conjugate(qa) * ld4(b.q));
+    e.reusable = 1;
+}
+
+// measureAgain: src/world.cpp:1013-1025. Kept points stay where they were on each body; the normal
+// turns with body A.
+TF_HD void measureAgain(const tfs_body& a, const tfs_body& b, tfs_contact& c, const tfs_contact_ext& e, float speculative) {
+    const V3 n = rotate(ld4(a.q), ld3(e.local_normal));
+    const V3 t1 = perpendicular(n);
+    st3(c.normal, n);
+    st3(c.tangent1, t1);
+    st3(c.tangent2, cross(n, t1));
+    TF_POINTS(i, c.count) {
+        tfs_point& p = c.points[i];
+        const V3 pa = applyPose(a, ld3(e.surface_a[i])), pb = applyPose(b, ld3(e.surface_b[i]));
+        const V3 mid = (pa + pb) * 0.5f;
+        p.separation = dot(pb - pa, n);
+        const V3 up = lift(a, b, n, p.separation, speculative);
+        st3(p.anchor_a, mid - up - ld3(a.com));
+        st3(p.anchor_b, mid + up - ld3(b.com));
+        const V3 t = ld3(p.tangent_impulse);
+        st3(p.tangent_impulse, t - n * dot(n, t));
+    }
+}
+
+// A new contact from a manifold: src/world.cpp:919-944 (the fields the solver path reads), then remember.
+TF_HD void contactFromManifold(const tfs_body& a, const tfs_body& b, const tfs_manifold& m, tfs_contact& c,
+                               tfs_contact_ext& e, float speculative) {
+    const V3 n = ld3(m.normal);
+    const V3 t1 = perpendicular(n);
+    st3(c.normal, n);
+    st3(c.tangent1, t1);
+    st3(c.tangent2, cross(n, t1));
+    c.count = m.count;
+    TF_POINTS(i, m.count) {
+        tfs_point& p = c.points[i];
+        const V3 pos = ld3(m.position[i]);
+        const float sep = m.separation[i];
+        const V3 half = n * (0.5f * sep);
+        st3(e.surface_a[i], applyInversePose(a, pos - half));
+        st3(e.surface_b[i], applyInversePose(b, pos + half));
+        st3(p.local_a, applyInversePose(a, pos));
+        p.separation = sep;
+        const V3 up = lift(a, b, n, sep, speculative);
+        st3(p.anchor_a, pos - up - ld3(a.com));
+        st3(p.anchor_b, pos + up - ld3(b.com));
+    }
+    remember(a, b, c, e);
+}
+
 }  // namespace tf

@@ -15,7 +15,7 @@ import weakref
 import numpy as np
 
 from . import _abi
-from ._abi import BODY_DTYPE, CONTACT_DTYPE, HIT_DTYPE, SETTINGS_DTYPE, SHAPE_DTYPE
+from ._abi import BODY_DTYPE, CONTACT_DTYPE, EXT_DTYPE, HIT_DTYPE, MANIFOLD_DTYPE, PAIR_DTYPE, SETTINGS_DTYPE, SHAPE_DTYPE
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtwofold_hip.so")
 
@@ -30,7 +30,8 @@ EXPORTS = ("tfs_abi_version", "tfs_sizeof_body", "tfs_sizeof_contact", "tfs_size
            "tfs_last_step_ms", "tfs_last_step_counts", "tfs_set_profiling", "tfs_read_profile", "tfs_fixed_begin",
            "tfs_fixed_step", "tfs_adjoint_seed", "tfs_adjoint_step", "tfs_adjoint_read", "tfs_check_minmax",
            "tfs_autotune", "tfs_pass_shape", "tfs_set_pass_shape", "tfs_set_cooperative", "tfs_set_body_shapes",
-           "tfs_step_noalloc", "tfs_download_touches", "tfs_set_graph")
+           "tfs_step_noalloc", "tfs_download_touches", "tfs_set_graph", "tfs_stream_contacts", "tfs_upload_contact_ext",
+           "tfs_download_contact_ext")
 
 
 class SolverError(RuntimeError):
@@ -77,6 +78,9 @@ def load_library(path=LIB_PATH):
     lib.tfs_pass_shape.argtypes = [_vp, _vp]
     lib.tfs_set_pass_shape.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
     lib.tfs_set_graph.argtypes = [_vp, ctypes.c_int]
+    lib.tfs_stream_contacts.argtypes = [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32]
+    lib.tfs_upload_contact_ext.argtypes = [_vp, _vp, ctypes.c_uint32]
+    lib.tfs_download_contact_ext.argtypes = [_vp, _vp, ctypes.c_uint32]
     lib.tfs_set_cooperative.argtypes = [_vp, ctypes.c_int]
     lib.tfs_set_body_shapes.argtypes = [_vp, ctypes.c_uint32, _vp]
     lib.tfs_download_touches.argtypes = [_vp, _vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), _vp,
@@ -240,6 +244,12 @@ class Solver:
         self._check(self.lib.tfs_upload_contacts(self.h, _ptr(c), len(c)), "tfs_upload_contacts")
         self.nb, self.nc = len(b), len(c)
 
+    def upload_bodies(self, bodies):
+        """Bodies only: the resident contacts stay (the contact stream's path)."""
+        b = np.ascontiguousarray(bodies, BODY_DTYPE)
+        self._check(self.lib.tfs_upload_bodies(self.h, _ptr(b), len(b)), "tfs_upload_bodies")
+        self.nb = len(b)
+
     def set_worlds(self, body_start):
         """Independent worlds over the resident bodies (tfs_set_worlds); upload contacts after."""
         bs = np.ascontiguousarray(body_start, np.uint32)
@@ -398,6 +408,29 @@ class Solver:
         gm = np.zeros(max(self.nc, 1), np.float32)
         self._check(self.lib.tfs_adjoint_read(self.h, _ptr(gv), _ptr(gw), _ptr(gm)), "tfs_adjoint_read")
         return gv, gw, gm[:self.nc]
+
+    def stream(self, pairs, manifolds):
+        """The contact stream (tfs_stream_contacts): this step's candidate pairs (PAIR_DTYPE, key order) and
+        their manifolds (MANIFOLD_DTYPE) against the resident contacts; the result stays resident."""
+        p = np.ascontiguousarray(pairs, PAIR_DTYPE)
+        m = np.ascontiguousarray(manifolds, MANIFOLD_DTYPE)
+        self._check(self.lib.tfs_stream_contacts(self.h, _ptr(p), len(p), _ptr(m), len(m)), "tfs_stream_contacts")
+        self.nc = self.counts_resident()
+
+    def counts_resident(self):
+        """The number of resident contacts."""
+        n = ctypes.c_uint32()
+        self._check(self.lib.tfs_download_list(self.h, 6, None, 0, ctypes.byref(n)), "tfs_download_list")
+        return n.value
+
+    def upload_ext(self, ext):
+        e = np.ascontiguousarray(ext, EXT_DTYPE)
+        self._check(self.lib.tfs_upload_contact_ext(self.h, _ptr(e), len(e)), "tfs_upload_contact_ext")
+
+    def ext(self):
+        out = np.zeros(self.nc, EXT_DTYPE)
+        self._check(self.lib.tfs_download_contact_ext(self.h, _ptr(out), self.nc), "tfs_download_contact_ext")
+        return out
 
     def touches(self):
         """Touch events of the last step (reportTouches, src/world.cpp:1424-1468): events as
