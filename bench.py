@@ -179,6 +179,8 @@ class SolveSteps:
 
     def __call__(self):
         sim = self.sim
+        if sim.stream and sim.world.worlds == 1:
+            return self.streamed()
         sim.bodies = sim.solver.bodies()
         contacts, woke = sim.world.collide(sim.bodies, sim.dt, sim.contacts)
         t = time.perf_counter()
@@ -196,6 +198,38 @@ class SolveSteps:
         sim.contacts = sim.solver.contacts()
         sim.bodies = sim.solver.bodies()
         self.transfer += time.perf_counter() - t
+        sim.steps += 1
+        return spent
+
+    def streamed(self):
+        """A step with the contacts resident (tfs_stream_contacts): the bodies come back (timed with the
+        transfers), the host finds the candidate pairs and their manifolds (untimed, as the narrowphase),
+        the pairs and manifolds go over and the device updates its contacts (timed with the transfers),
+        then the solve (timed)."""
+        sim = self.sim
+        t = time.perf_counter()
+        poses = sim.solver.poses()
+        self.transfer += time.perf_counter() - t
+        for f in ("p", "q", "com", "v", "w", "awake"):
+            sim.bodies[f] = poses[f]
+        pairs, manifolds, woke = sim.world.pairs(sim.bodies, sim.dt, sim.pinned)
+        t = time.perf_counter()
+        sim.solver.stream(pairs, manifolds)
+        if woke:
+            # the whole bodies, woken where the pairs woke them (the rest of the host copy is stale)
+            woken = sim.bodies["awake"].astype(bool) & ~poses["awake"].astype(bool)
+            full = sim.solver.bodies()
+            full["awake"][woken] = 1
+            full["sleep_time"][woken] = 0.0
+            sim.solver.upload_bodies(full)
+            sim.bodies = full
+        self.sync()
+        self.transfer += time.perf_counter() - t
+        t = time.perf_counter()
+        sim.solver.step_native(sim.dt, sim.world.refresh_fn, sim.world.h)
+        self.sync()
+        spent = time.perf_counter() - t
+        sim.contacts = range(sim.solver.nc)   # only their number is read
         sim.steps += 1
         return spent
 
@@ -553,6 +587,35 @@ def run_pile(args, planar=False):
                         "window": f"{run['steps'] + 1}-{run['steps'] + args.host_refresh_steps}",
                         "refresh": "host callback (twofold_amd/csrc/scene.cpp tfw_refresh)"}
             log(f"host-refresh leg: {host_leg['value']:.4g} iterations/s")
+        streamed = None
+        if args.stream_steps > 0:
+            # the contacts resident on the device (tfs_stream_contacts): only the candidate pairs and their
+            # manifolds go over each step, and the bodies come back; the first streamed step starts from
+            # contacts that are not reusable (they were uploaded), so it and the warm-up are not timed
+            sim.stream = True
+            for _ in range(args.warmup):
+                solve_step()
+            solve_step.transfer = 0.0
+            s_its, s_wall, first = 0, 0.0, run["steps"] + args.warmup + 1
+            for _ in range(args.stream_steps):
+                s_wall += solve_step()
+                s_its += iterations(sim.solver.counts(), substeps)
+            s_transfer = solve_step.transfer
+            c = sim.solver.counts()
+            streamed = {"value": s_its / s_wall, "unit": "iterations/s", "steps": args.stream_steps,
+                        "window": f"{first}-{first + args.stream_steps - 1}",
+                        "ms_per_step": 1e3 * s_wall / args.stream_steps,
+                        "transfer_ms_per_step": 1e3 * s_transfer / args.stream_steps,
+                        "value_with_transfers": s_its / (s_wall + s_transfer), "colors": c["colors"],
+                        "contacts": sim.solver.nc,
+                        "note": "contacts resident on the device, updated each step from the candidate pairs and "
+                                "their manifolds (tfs_stream_contacts: reuse of contacts that still fit, impulse "
+                                "carry); transfers = the bodies' poses back (tfs_download_poses) + the pairs and "
+                                "manifolds over + the stream's device work"}
+            log(f"streamed: {streamed['value']:.4g} iterations/s, {streamed['value_with_transfers']:.4g} with transfers "
+                f"({streamed['transfer_ms_per_step']:.2f} ms of transfers per step)")
+            sim.stream = False
+            sim.contacts = sim.solver.contacts()
         sim.solver.set_profiling(True)
         for _ in range(args.profile_steps):
             solve_step()
@@ -568,7 +631,7 @@ def run_pile(args, planar=False):
         if not args.no_cpu:
             cpu = cpu_reference(scene, bodies0, substeps, args.cpu_seconds)
             if cpu is None:
-                cpu = cpu_port(sim, bodies0, sim.world.collide(bodies0.copy(), sim.dt, sim.contacts)[0], args.cpu_seconds)
+                cpu = cpu_port(sim, bodies0, sim.world.collide(bodies0.copy(), sim.dt, None)[0], args.cpu_seconds)
             log(f"cpu baseline ({cpu['kind']}): {cpu['value']:.3e} iterations/s ({cpu['ms_per_step']:.0f} ms/step)")
         refresh_where = "device" if sim.device_refresh is not None else "host"
     finally:
@@ -602,6 +665,8 @@ def run_pile(args, planar=False):
             "world_steps_per_s": args.steps / wall, "gpu_solve_ms_per_step": solve_ms / args.steps,
             "transfer_ms_per_step": 1e3 * transfers / args.steps,
             "value_with_transfers": its / (wall + transfers), "pass_shape": shape,
+            "contacts": "uploaded from the host each step (tfs_upload_contacts); streamed form below",
+            "streamed": streamed,
             "whole_run": {"steps": f"1-{run['steps']}", "value": run["its"] / run["wall"], "unit": "iterations/s",
                           "ms_per_step": 1e3 * run["wall"] / run["steps"],
                           "note": "every step's solve timed the same way from the first drop step on, the pass-shape "
@@ -677,6 +742,9 @@ def main():
                     help="all: seconds after which the workloads not yet run are skipped (noted in the line)")
     ap.add_argument("--log-every", type=int, default=20, help="settle steps between progress lines")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--stream-steps", type=int, default=20,
+                    help="pile and planar: steps timed after the window with the contacts resident on the device, the "
+                         "candidate pairs and their manifolds sent each step (tfs_stream_contacts)")
     ap.add_argument("--graph", type=int, default=0,
                     help="batched: launch each step's substep loop as one HIP graph (1/0; the other form is timed too)")
     ap.add_argument("--compare-steps", type=int, default=4, help="batched: steps timed with the other launch form")

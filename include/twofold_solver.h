@@ -276,11 +276,11 @@ int tfs_upload_contacts(tfs_ctx* ctx, const tfs_contact* contacts, uint32_t n);
  * one) stay as they were (969-980), and all are merged in key order; they replace the resident
  * contacts. The caller collides every candidate pair (a kept pair's manifold is not read) and wakes
  * bodies as updateContacts does (957-958) before tfs_step. Contacts and their reuse data then never
- * cross PCIe. TFS_E_ARG for a pair out of key order, a body out of range or a manifold range outside
- * `manifolds`.
+ * cross PCIe. *n_contacts (may be NULL) receives the number of resident contacts. TFS_E_ARG for a pair
+ * out of key order, a body out of range or a manifold range outside `manifolds`.
  */
 int tfs_stream_contacts(tfs_ctx* ctx, const tfs_pair* pairs, uint32_t n_pairs, const tfs_manifold* manifolds,
-                        uint32_t n_manifolds);
+                        uint32_t n_manifolds, uint32_t* n_contacts);
 /* The reuse data of the resident contacts (tfs_contact_ext), one per contact in the same order. An upload
  * sets it for the contacts last uploaded (tfs_upload_contacts leaves every contact not reusable). */
 int tfs_upload_contact_ext(tfs_ctx* ctx, const tfs_contact_ext* ext, uint32_t n);
@@ -345,6 +345,20 @@ int tfs_set_body_shapes(tfs_ctx* ctx, uint32_t n, const tfs_shape* shapes);
 #define TFS_NOALLOC_MAX_CONTACTS 8192
 int tfs_step_noalloc(int device, tfs_body* bodies, uint32_t n_bodies, tfs_contact* contacts, uint32_t n_contacts,
                      const tfs_shape* shapes, const tfs_settings* settings, float dt, void* stream);
+
+/* What a host's broadphase and narrowphase read of a body between steps, compact (72 bytes instead of
+ * the 420 of tfs_body): for the contact stream's loop, whose contacts never leave the device. */
+typedef struct tfs_pose {
+    float p[3];
+    float q[4];
+    float com[3];
+    float v[3];
+    float w[3];
+    uint8_t type;
+    uint8_t awake;
+    uint8_t pad_[6];
+} tfs_pose;
+int tfs_download_poses(tfs_ctx* ctx, tfs_pose* out, uint32_t n);
 
 int tfs_download_bodies(tfs_ctx* ctx, tfs_body* out, uint32_t n);
 int tfs_download_contacts(tfs_ctx* ctx, tfs_contact* out, uint32_t n);

@@ -69,3 +69,75 @@ def test_stream_keeps_contacts_resident_and_matches_reference(make, resident):
             bad += parity.compare_touches(rec, got_c, *s.touches())
             assert bad == [], f"{scene.name} step {k}: solve {bad[:5]}"
     assert kept > 0
+
+
+def test_simulation_with_streamed_contacts():
+    """The host world driving the solver with the contacts resident (Simulation(stream=True)): an 8x8x8
+    pile dropped for 150 steps; the contacts the device keeps are as many as the host world's own
+    narrowphase finds, a growing share of them reused (still fitting, measured again) once the pile
+    rests, and the bodies stay finite and inside the walls."""
+    from twofold_amd.world import Simulation
+
+    with Simulation(scenes.pile(side=8, half_room=6.0), device_refresh=True, stream=True) as sim:
+        reused = 0
+        for k in range(150):
+            sim.step()
+            if k >= 100:
+                reused += int(sim.solver.ext()["reusable"].sum())
+        bodies = sim.solver.bodies()
+        contacts = sim.solver.contacts()
+        host, _ = sim.world.collide(bodies.copy(), sim.dt)
+    assert np.isfinite(bodies["com"]).all() and (np.abs(bodies["com"][:, [0, 2]]) < 7.0).all()
+    assert len(contacts) > 500 and abs(len(contacts) - len(host)) <= 0.05 * len(host), (len(contacts), len(host))
+    assert reused > 0
+
+
+def test_streamed_simulation_matches_the_oracle_step_for_step():
+    """The host world driving the GPU with resident contacts against the oracle driven the same way (the
+    same candidate pairs and manifolds from the host world at the GPU's poses, the same refresh callback):
+    a 10x10x10 pile dropped for 200 steps, bodies and contacts equal bit for bit after every step."""
+    import ctypes
+
+    from twofold_amd import _abi
+    from twofold_amd.world import Simulation
+
+    lib = refworld._load_oracle()
+    with Simulation(scenes.pile(side=10, half_room=7.0), stream=True) as sim:
+        hw = sim.world
+        fn = ctypes.cast(hw.refresh_fn, refworld.REFRESH_FN)
+        ext = np.zeros(0, _abi.EXT_DTYPE)
+        contacts = np.zeros(0, _abi.CONTACT_DTYPE)
+        for k in range(200):
+            bodies = sim.solver.bodies() if k else sim.bodies.copy()
+            before = bodies.copy()
+            pairs, manifolds, woke = hw.pairs(bodies, hw.dt)
+            contacts, ext = refworld.oracle_stream(before, contacts, ext, pairs, manifolds, hw.settings)
+            ext = ext.copy()
+            sim.solver.stream(pairs, manifolds)
+            if woke:
+                sim.solver.upload_bodies(bodies)
+            sim.solver.step_native(hw.dt, hw.refresh_fn, hw.h)
+            b = np.ascontiguousarray(bodies.copy())
+            c = np.ascontiguousarray(contacts.copy())
+            n = max(len(c), 1)
+            arrays = [np.zeros(n, np.uint32) for _ in range(6)]
+            colors = np.zeros(n, np.uint8)
+            L = refworld._Lists()
+            for name, a in zip(("active", "fast", "fast_solid", "impact", "by_color", "impact_by_color"), arrays):
+                setattr(L, name, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+            L.colors = colors.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+            ev = np.zeros(n, np.uint32)
+            hits = np.zeros(n, _abi.HIT_DTYPE)
+            L.touch_events = ev.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+            L.hits = hits.ctypes.data
+            L.ext = ext.ctypes.data
+            assert lib.tfo_step(b.ctypes.data, len(b), c.ctypes.data, len(c), hw.settings.ctypes.data, np.float32(hw.dt), fn,
+                                ctypes.c_void_p(hw.h), ctypes.byref(L)) == 0
+            contacts = c
+            gb, gc, ge = sim.solver.bodies(), sim.solver.contacts(), sim.solver.ext()
+            for name, want, got in (("bodies", b, gb), ("contacts", c, gc), ("ext", ext, ge)):
+                fields = [f for f in want.dtype.names if f != "pad_" and
+                          np.ascontiguousarray(want[f]).tobytes() != np.ascontiguousarray(got[f]).tobytes()]
+                assert not fields, f"step {k}: {name} differ in {fields}"
+            if k % 50 == 49:
+                print(f"streamed pile: step {k + 1}, {len(c)} contacts, {L.n_colors} colours", flush=True)

@@ -56,6 +56,10 @@ PAIR_DTYPE = np.dtype([("key", np.uint64), ("body_a", U32), ("body_b", U32), ("f
 MANIFOLD_DTYPE = np.dtype([("normal", F, 3), ("count", np.int32), ("position", F, (4, 3)), ("separation", F, 4)],
                           align=True)
 
+# tfs_pose: a body's pose and velocities (tfs_download_poses)
+POSE_DTYPE = np.dtype([("p", F, 3), ("q", F, 4), ("com", F, 3), ("v", F, 3), ("w", F, 3), ("type", U8), ("awake", U8),
+                       ("pad_", U8, 6)], align=True)
+
 # tfs_hit: a hit event of tfs_download_touches
 HIT_DTYPE = np.dtype([("contact", U32), ("point", U32), ("position", F, 3), ("normal", F, 3), ("speed", F)])
 

@@ -833,6 +833,73 @@ uint32_t tfw_collide(void* h, tfs_body* bodies, uint32_t nb, float dt, const tfs
     return n;
 }
 
+// The contact stream's input (tfs_stream_contacts): the candidate pairs of the current poses (those
+// with a dynamic awake body), in key order, with this world's materials mixed for each (friction the
+// geometric mean, restitution and rolling resistance the larger) and the narrowphase's manifold for
+// each that has one, normal of unit length, points at the surfaces' midpoints. Returns the number of
+// pairs and writes at most `cap` of them and of the manifolds (*n_manifolds: their number). Sleeping
+// dynamic bodies of a pair with a manifold and an awake body are woken in `bodies`; *woke counts them
+// (the caller uploads the bodies again when it is not 0, after streaming: updateContacts wakes them
+// after the contacts are made, src/world.cpp:957-958, 987).
+uint32_t tfw_pairs(void* h, tfs_body* bodies, uint32_t nb, float dt, tfs_pair* pairs, tfs_manifold* manifolds, uint32_t cap,
+                   uint32_t* n_manifolds, uint32_t* woke) {
+    World& w = *(World*)h;
+    if (nb != w.bodies.size()) { return 0; }
+    const std::vector<uint64_t> keys = candidates(w, bodies, dt);
+    std::vector<Manifold> found(keys.size());
+    std::vector<uint8_t> hit(keys.size(), 0);
+#pragma omp parallel for schedule(dynamic, 256) num_threads(w.threads)
+    for (long k = 0; k < (long)keys.size(); ++k) {
+        const uint32_t ia = (uint32_t)(keys[(size_t)k] >> 32), ib = (uint32_t)keys[(size_t)k];
+        if (collide(w, ia, bodies[ia], ib, bodies[ib], dt, found[(size_t)k])) { hit[(size_t)k] = 1; }
+    }
+    uint32_t nm = 0, wakes = 0;
+    for (size_t k = 0; k < keys.size(); ++k) {
+        const uint32_t ia = (uint32_t)(keys[k] >> 32), ib = (uint32_t)keys[k];
+        if (k < cap && pairs) {
+            tfs_pair& p = pairs[k];
+            std::memset(&p, 0, sizeof p);
+            p.key = keys[k];
+            p.body_a = ia;
+            p.body_b = ib;
+            const Shape& sa = w.shapes[ia];
+            const Shape& sb = w.shapes[ib];
+            p.friction = std::sqrt(sa.friction * sb.friction);
+            p.restitution = std::max(sa.restitution, sb.restitution);
+            p.rolling_resistance = std::max(sa.rolling, sb.rolling);
+            p.first = nm;
+            p.count = hit[k] ? 1u : 0u;
+        }
+        if (!hit[k]) { continue; }
+        if (nm < cap && manifolds) {
+            const Manifold& m = found[k];
+            tfs_manifold& x = manifolds[nm];
+            std::memset(&x, 0, sizeof x);
+            st(x.normal, unit(m.n));
+            x.count = m.count;
+            for (int i = 0; i < m.count && i < 4; ++i) {
+                st(x.position[i], m.p[i].pos);
+                x.separation[i] = m.p[i].sep;
+            }
+        }
+        ++nm;
+        tfs_body& a = bodies[ia];
+        tfs_body& b = bodies[ib];
+        for (int s = 0; s < 2; ++s) {
+            tfs_body& x = s ? b : a;
+            const tfs_body& y = s ? a : b;
+            if (x.type == TFS_DYNAMIC && !x.awake && moving(y)) {
+                x.awake = 1;
+                x.sleep_time = 0.0f;
+                ++wakes;
+            }
+        }
+    }
+    if (n_manifolds) { *n_manifolds = nm; }
+    if (woke) { *woke = wakes; }
+    return (uint32_t)keys.size();
+}
+
 // The bodies' shapes for the device refresh (tfs_set_body_shapes), which collides spheres, boxes and
 // half-spaces with this world's narrowphase (narrow.h): the count, or -1 if a body's shape is a
 // cylinder or a prism. Writes at most `cap`.

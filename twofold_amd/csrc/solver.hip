@@ -4137,6 +4137,25 @@ int noallocSetUp(int device) {
 
 }  // namespace
 
+// tfs_pose of each body, for tfs_download_poses.
+__global__ void k_poses(const tfs_body* bodies, uint32_t n, tfs_pose* out) {
+    for (uint32_t i = gid(); i < n; i += gstride()) {
+        const tfs_body& b = bodies[i];
+        tfs_pose x;
+        memset(&x, 0, sizeof x);
+        for (int k = 0; k < 3; ++k) {
+            x.p[k] = b.p[k];
+            x.com[k] = b.com[k];
+            x.v[k] = b.v[k];
+            x.w[k] = b.w[k];
+        }
+        for (int k = 0; k < 4; ++k) { x.q[k] = b.q[k]; }
+        x.type = b.type;
+        x.awake = b.awake;
+        out[i] = x;
+    }
+}
+
 // The solves' max and min (maxq, minq) over given pairs, for tfs_check_minmax.
 __global__ void k_minmax(const float* a, const float* b, uint32_t n, float* out) {
     for (uint32_t i = gid(); i < n; i += gstride()) {
@@ -4399,7 +4418,7 @@ int tfs_download_contact_ext(tfs_ctx* ctx, tfs_contact_ext* out, uint32_t n) {
 }
 
 int tfs_stream_contacts(tfs_ctx* ctx, const tfs_pair* pairs, uint32_t n_pairs, const tfs_manifold* manifolds,
-                        uint32_t n_manifolds) {
+                        uint32_t n_manifolds, uint32_t* n_contacts) {
     if (!ctx || (!pairs && n_pairs) || (!manifolds && n_manifolds)) { return fail(ctx, TFS_E_ARG, "null argument"); }
     if (!ctx->haveBodies) { return fail(ctx, TFS_E_STATE, "upload bodies before streaming contacts"); }
     ctx->fixedOn = false;
@@ -4487,6 +4506,7 @@ int tfs_stream_contacts(tfs_ctx* ctx, const tfs_pair* pairs, uint32_t n_pairs, c
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     ctx->nc = total;
     ctx->haveContacts = true;
+    if (n_contacts) { *n_contacts = total; }
     return 0;
 }
 
@@ -4713,6 +4733,24 @@ int tfs_adjoint_read(tfs_ctx* ctx, float* dl_dv, float* dl_dw, float* dl_dfricti
     if (dl_dfriction && ctx->nc) {
         HIP_TRY(hipMemcpy(dl_dfriction, ctx->muContact, sizeof(float) * ctx->nc, hipMemcpyDeviceToHost));
     }
+    return 0;
+}
+
+int tfs_download_poses(tfs_ctx* ctx, tfs_pose* out, uint32_t n) {
+    if (!ctx || (!out && n)) { return fail(ctx, TFS_E_ARG, "null argument"); }
+    if (n > ctx->nb) { return fail(ctx, TFS_E_ARG, "%u poses asked for, %u bodies resident", n, ctx->nb); }
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (n == 0) { return 0; }
+    // packed on the device into the contact-sized staging array, then one copy
+    const size_t need = sizeof(tfs_pose) * n;
+    if (need > sizeof(tfs_contact) * (size_t)ctx->capC) {
+        int r;
+        if ((r = allocContacts(ctx, (uint32_t)((need + sizeof(tfs_contact) - 1) / sizeof(tfs_contact))))) { return r; }
+    }
+    tfs_pose* staged = (tfs_pose*)ctx->fresh;
+    LAUNCH(k_poses, blocksFor(n), kThreads, ctx->bodies, n, staged);
+    HIP_TRY(hipMemcpyAsync(out, staged, need, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
     return 0;
 }
 

@@ -31,7 +31,7 @@ EXPORTS = ("tfs_abi_version", "tfs_sizeof_body", "tfs_sizeof_contact", "tfs_size
            "tfs_fixed_step", "tfs_adjoint_seed", "tfs_adjoint_step", "tfs_adjoint_read", "tfs_check_minmax",
            "tfs_autotune", "tfs_pass_shape", "tfs_set_pass_shape", "tfs_set_cooperative", "tfs_set_body_shapes",
            "tfs_step_noalloc", "tfs_download_touches", "tfs_set_graph", "tfs_stream_contacts", "tfs_upload_contact_ext",
-           "tfs_download_contact_ext")
+           "tfs_download_contact_ext", "tfs_download_poses")
 
 
 class SolverError(RuntimeError):
@@ -78,8 +78,9 @@ def load_library(path=LIB_PATH):
     lib.tfs_pass_shape.argtypes = [_vp, _vp]
     lib.tfs_set_pass_shape.argtypes = [_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, ctypes.c_int]
     lib.tfs_set_graph.argtypes = [_vp, ctypes.c_int]
-    lib.tfs_stream_contacts.argtypes = [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32]
+    lib.tfs_stream_contacts.argtypes = [_vp, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
     lib.tfs_upload_contact_ext.argtypes = [_vp, _vp, ctypes.c_uint32]
+    lib.tfs_download_poses.argtypes = [_vp, _vp, ctypes.c_uint32]
     lib.tfs_download_contact_ext.argtypes = [_vp, _vp, ctypes.c_uint32]
     lib.tfs_set_cooperative.argtypes = [_vp, ctypes.c_int]
     lib.tfs_set_body_shapes.argtypes = [_vp, ctypes.c_uint32, _vp]
@@ -140,6 +141,37 @@ def device_synchronize():
     rc = hip_runtime().hipDeviceSynchronize()
     if rc != 0:
         raise SolverError(f"hipDeviceSynchronize failed ({rc})")
+
+
+class PinnedArray:
+    """A numpy structured array in pinned host memory (hipHostMalloc), which HIP copies to and from the
+    device by DMA at full speed; `view(n)` is its first n records."""
+
+    def __init__(self, dtype, capacity):
+        self.hip = hip_runtime()
+        self.hip.hipHostMalloc.argtypes = [ctypes.POINTER(_vp), ctypes.c_size_t, ctypes.c_uint]
+        self.hip.hipHostFree.argtypes = [_vp]
+        self.dtype = np.dtype(dtype)
+        self.capacity = max(int(capacity), 1)
+        self.ptr = _vp()
+        rc = self.hip.hipHostMalloc(ctypes.byref(self.ptr), self.capacity * self.dtype.itemsize, 0)
+        if rc != 0:
+            raise SolverError(f"hipHostMalloc failed ({rc})")
+        raw = (ctypes.c_char * (self.capacity * self.dtype.itemsize)).from_address(self.ptr.value)
+        self.array = np.frombuffer(raw, self.dtype)
+
+    def view(self, n):
+        return self.array[:n]
+
+    def close(self):
+        if self.ptr:
+            self.array = None
+            self.hip.hipHostFree(self.ptr)
+            self.ptr = _vp()
+
+    def __del__(self):
+        if not sys.is_finalizing():
+            self.close()
 
 
 class DeviceArray:
@@ -299,6 +331,12 @@ class Solver:
         self._cb = cb
         self._check(self.lib.tfs_step(self.h, ctypes.c_float(dt), cb, None), "tfs_step")
 
+    def poses(self):
+        """The bodies' poses and velocities, compact (POSE_DTYPE, tfs_download_poses)."""
+        out = np.zeros(self.nb, _abi.POSE_DTYPE)
+        self._check(self.lib.tfs_download_poses(self.h, _ptr(out), self.nb), "tfs_download_poses")
+        return out
+
     def bodies(self):
         out = np.zeros(self.nb, BODY_DTYPE)
         self._check(self.lib.tfs_download_bodies(self.h, _ptr(out), self.nb), "tfs_download_bodies")
@@ -414,14 +452,10 @@ class Solver:
         their manifolds (MANIFOLD_DTYPE) against the resident contacts; the result stays resident."""
         p = np.ascontiguousarray(pairs, PAIR_DTYPE)
         m = np.ascontiguousarray(manifolds, MANIFOLD_DTYPE)
-        self._check(self.lib.tfs_stream_contacts(self.h, _ptr(p), len(p), _ptr(m), len(m)), "tfs_stream_contacts")
-        self.nc = self.counts_resident()
-
-    def counts_resident(self):
-        """The number of resident contacts."""
         n = ctypes.c_uint32()
-        self._check(self.lib.tfs_download_list(self.h, 6, None, 0, ctypes.byref(n)), "tfs_download_list")
-        return n.value
+        self._check(self.lib.tfs_stream_contacts(self.h, _ptr(p), len(p), _ptr(m), len(m), ctypes.byref(n)),
+                    "tfs_stream_contacts")
+        self.nc = n.value
 
     def upload_ext(self, ext):
         e = np.ascontiguousarray(ext, EXT_DTYPE)

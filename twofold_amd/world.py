@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 from . import _abi
-from ._abi import BODY_DTYPE, CONTACT_DTYPE, SETTINGS_DTYPE, SHAPE_DTYPE
+from ._abi import BODY_DTYPE, CONTACT_DTYPE, MANIFOLD_DTYPE, PAIR_DTYPE, SETTINGS_DTYPE, SHAPE_DTYPE
 from .solver import REFRESH_FN, Solver, SolverError
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtwofold_scene.so")
@@ -44,6 +44,9 @@ def _load():
         lib.tfw_collide.restype = ctypes.c_uint32
         lib.tfw_collide.argtypes = [_vp, _vp, ctypes.c_uint32, ctypes.c_float, _vp, ctypes.c_uint32, _vp,
                                     ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        lib.tfw_pairs.restype = ctypes.c_uint32
+        lib.tfw_pairs.argtypes = [_vp, _vp, ctypes.c_uint32, ctypes.c_float, _vp, _vp, ctypes.c_uint32,
+                                  ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
         lib.tfw_body_shapes.restype = ctypes.c_int
         lib.tfw_body_shapes.argtypes = [_vp, _vp, ctypes.c_uint32]
         _lib = lib
@@ -126,11 +129,44 @@ class HostWorld:
             cap = n
 
 
+    def pairs(self, bodies, dt, pinned=None):
+        """The contact stream's input at the bodies' poses: candidate pairs (PAIR_DTYPE, key order) and
+        their manifolds (MANIFOLD_DTYPE). May wake bodies in place; returns (pairs, manifolds, woken).
+        `pinned` (a dict this call fills and reuses) keeps the two arrays in pinned host memory, from
+        which the stream's upload goes by DMA."""
+        nm, woke = ctypes.c_uint32(), ctypes.c_uint32()
+        cap = max(4 * len(bodies), 1024)
+        while True:
+            if pinned is not None:
+                if pinned.get("cap", 0) < cap:
+                    from .solver import PinnedArray
+                    for k in ("pairs", "manifolds"):
+                        if k in pinned:
+                            pinned[k].close()
+                    pinned["pairs"] = PinnedArray(PAIR_DTYPE, cap)
+                    pinned["manifolds"] = PinnedArray(MANIFOLD_DTYPE, cap)
+                    pinned["cap"] = cap
+                pairs, mf = pinned["pairs"].view(cap), pinned["manifolds"].view(cap)
+            else:
+                pairs = np.zeros(cap, PAIR_DTYPE)
+                mf = np.zeros(cap, MANIFOLD_DTYPE)
+            n = self.lib.tfw_pairs(self.h, bodies.ctypes.data_as(_vp), len(bodies), np.float32(dt),
+                                   pairs.ctypes.data_as(_vp), mf.ctypes.data_as(_vp), cap, ctypes.byref(nm),
+                                   ctypes.byref(woke))
+            if n <= cap:
+                return pairs[:n], mf[:nm.value], woke.value
+            cap = n
+
+
 class Simulation:
     """A scene (or a list of scenes: independent worlds solved together) stepped with the host
     narrowphase and the GPU solver."""
 
-    def __init__(self, scene, settings=None, device=0, device_refresh=False):
+    def __init__(self, scene, settings=None, device=0, device_refresh=False, stream=False):
+        """`stream`: the contacts stay on the device from step to step; each step sends the candidate
+        pairs and their manifolds (tfs_stream_contacts) instead of the contacts."""
+        self.stream = stream
+        self.pinned = {}
         self.world = HostWorld(scene, settings)
         self.settings = self.world.settings
         self.dt = self.world.dt
@@ -153,8 +189,15 @@ class Simulation:
             self.solver.upload(bodies, contacts)
 
     def collide(self):
-        """The narrowphase for the next step; uploads its contacts (and bodies it woke)."""
+        """The narrowphase for the next step; uploads its contacts (and bodies it woke), or streams its
+        pairs and manifolds onto the resident contacts."""
         self.bodies = self.solver.bodies() if self.steps else self.bodies
+        if self.stream and self.world.worlds == 1:
+            pairs, manifolds, woke = self.world.pairs(self.bodies, self.dt, self.pinned)
+            self.solver.stream(pairs, manifolds)
+            if woke:
+                self.solver.upload_bodies(self.bodies)
+            return
         contacts, woke = self.world.collide(self.bodies, self.dt, self.contacts)
         if woke or not self.steps:
             self.upload(self.bodies, contacts)
@@ -165,7 +208,8 @@ class Simulation:
     def solve(self):
         """The solve block and sleep pass on the GPU, over the uploaded contacts."""
         self.solver.step_native(self.dt, self.world.refresh_fn, self.world.h)
-        self.contacts = self.solver.contacts()
+        if not self.stream:
+            self.contacts = self.solver.contacts()
         self.steps += 1
 
     def step(self):
@@ -173,8 +217,12 @@ class Simulation:
         self.solve()
 
     def close(self):
-        """Releases the solver's device state and the host world."""
+        """Releases the solver's device state, the pinned buffers and the host world."""
         self.solver.close()
+        for k in ("pairs", "manifolds"):
+            if k in self.pinned:
+                self.pinned[k].close()
+        self.pinned = {}
         self.world.close()
 
     def __enter__(self):
