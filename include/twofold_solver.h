@@ -266,14 +266,14 @@ int tfs_upload_contacts(tfs_ctx* ctx, const tfs_contact* contacts, uint32_t n);
  * The contact stream: the solver half of updateContacts (src/world.cpp:880-989) on the device, so that
  * the contacts stay resident from step to step. Before a step, over the resident bodies and the
  * resident contacts of the last step, for each pair (key order): when the pair had contacts and every
- * one still fits (stillFits, src/world.cpp:1002-1009: the bodies moved and turned relative to each
+ * one still fits (stillFits, src/world.cpp:1002-1008: the bodies moved and turned relative to each
  * other less than the reuse distance and angle since the points were collided), the old contacts are
- * kept with their points measured again (measureAgain, 1013-1025); otherwise each manifold becomes a
+ * kept with their points measured again (measureAgain, 1011-1025); otherwise each manifold becomes a
  * contact (its points on both surfaces, anchors lifted apart when far, 927-943), remembers the pair's
  * pose (remember, 995-1000) and takes the impulses of the old contact with the closest normal, point by
  * point (closestNormal, carryImpulses, 543-549, 102-120), and the touched flag. The pair's materials go
  * to all its contacts. The contacts of resting pairs (no body dynamic and awake, or a moving kinematic
- * one) stay as they were (969-980), and all are merged in key order; they replace the resident
+ * one) stay as they were (972-984), and all are merged in key order; they replace the resident
  * contacts. The caller collides every candidate pair (a kept pair's manifold is not read) and wakes
  * bodies as updateContacts does (957-958) before tfs_step. Contacts and their reuse data then never
  * cross PCIe. *n_contacts (may be NULL) receives the number of resident contacts. TFS_E_ARG for a pair

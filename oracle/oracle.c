@@ -1048,7 +1048,7 @@ static V3 lift(const tfs_body* a, const tfs_body* b, V3 n, float sep, float spec
     return sep > speculative ? mul(n, 0.5f * sep) : vec(0, 0, 0);
 }
 
-static int stillFits(const tfs_contact_ext* c, const tfs_body* a, const tfs_body* b, const tfs_settings* s) {  /* 1002-1009 */
+static int stillFits(const tfs_contact_ext* c, const tfs_body* a, const tfs_body* b, const tfs_settings* s) {  /* 1002-1008 */
     if (!c->reusable) { return 0; }
     const float reach = s->contact_reuse_distance * s->length_unit;
     const V3 off = sub(unrotate(q4(a->q), sub(v3(b->com), v3(a->com))), v3(c->base_offset));
@@ -1058,7 +1058,7 @@ static int stillFits(const tfs_contact_ext* c, const tfs_body* a, const tfs_body
 }
 
 static void measureAgain(const tfs_body* a, const tfs_body* b, tfs_contact* c, const tfs_contact_ext* e,
-                         float speculative) {                                                  /* 1013-1025 */
+                         float speculative) {                                                  /* 1011-1025 */
     const V3 n = rotate(q4(a->q), v3(e->local_normal));
     const V3 t1 = perpendicular(n);
     st3(c->normal, n);
@@ -1159,7 +1159,7 @@ uint32_t tfo_stream_contacts(const tfs_body* bodies, const tfs_contact* old, con
             out[i].rolling_resistance = x->rolling_resistance;
         }
     }
-    /* resting pairs stay (969-980), merged in key order (981-983); the two key sets are disjoint */
+    /* resting pairs stay (972-984), merged in key order (985-987); the two key sets are disjoint */
     const uint32_t collided = n;
     tfs_contact* rest = (tfs_contact*)malloc(sizeof(tfs_contact) * (nOld ? nOld : 1));
     tfs_contact_ext* restExt = (tfs_contact_ext*)malloc(sizeof(tfs_contact_ext) * (nOld ? nOld : 1));

@@ -69,3 +69,66 @@ def test_adjoint_checker_sees_a_planted_mistake(case, monkeypatch):
 def test_every_contact_has_a_colour(case):
     assert np.all(case.colors < 32)
     assert case.contacts["count"].min() >= 1
+
+
+def _columns(seed):
+    """Free spheres in vertical columns, touching along y, with vertical velocities only: every
+    contact normal is exactly vertical and no tangential velocity ever arises, so no body spins and
+    the reference's rotations (turned() with dq, rebaseContact) are exactly the identity that the
+    fixed-contact model assumes."""
+    from twofold_amd import scenes
+    s = scenes.Scene(f"columns-{seed}", substeps=4, steps=2)
+    rng = np.random.default_rng(seed)
+    for cx in range(3):
+        for cz in range(2):
+            for k in range(4):
+                b = s.body(scenes.DYNAMIC, pos=(2.0 * cx, 0.5 * k - 0.002 * k, 2.0 * cz),
+                           v=(0.0, float(rng.normal(0.0, 0.15)), 0.0))
+                s.shape(b, scenes.SPHERE, (0.25,), density=1000.0, friction=0.6)
+    return s
+
+
+@needs_oracle
+@pytest.mark.parametrize("seed", [3, 4])
+def test_model_forward_matches_the_oracle(seed):
+    """One step of the float64 model against the C oracle's full step (tfo_step, itself bit-exact with
+    the reference) on a scene where the model's simplifications are exact: no refresh, no impact
+    pass, no restitution, no rest list and no rotation. The second step carries stored impulses, so
+    the warm start is compared too. Agreement to float32 rounding pins the model's forward: contact
+    softness, bias, the clamp, the warm start, the substep order and the position update."""
+    from twofold_amd.world import HostWorld
+    import sys, os
+    sys.path.insert(0, os.path.join(refworld.ROOT, "oracle"))
+    from fixed_contacts import FixedContacts
+    world = HostWorld(_columns(seed))
+    settings, dt = world.settings, float(world.dt)
+    bodies = world.bodies()
+    contacts, _ = world.collide(bodies, world.dt)
+    for step in range(2):
+        assert len(contacts) > 0
+        lib = refworld._load_oracle()
+        b = bodies.copy()
+        for i in range(len(b)):
+            lib.tfo_update_inverse_mass(b[i:i + 1].ctypes.data)
+        colors = np.full(len(contacts), 255, np.uint8)
+        lst = np.arange(len(contacts), dtype=np.uint32)
+        lib.tfo_color(b.ctypes.data, len(b), contacts.ctypes.data, lst.ctypes.data, len(lst), colors.ctypes.data)
+        model = FixedContacts(bodies, contacts, colors, settings, dt)
+        com = model.forward(steps=1)
+        v, w = model.last[0], model.last[1]
+        ob, oc, lists = refworld.oracle_step(bodies, contacts, settings, dt)
+        assert len(lists["fast"]) == 0 and len(lists["impact"]) == 0 and len(lists["rest"]) == 0
+        assert len(lists["active"]) == len(contacts)
+        assert np.all(ob["w"] == 0.0)
+        if step == 1:
+            assert np.any(contacts["points"]["normal_impulse"] > 0.0)
+        scale = max(float(np.abs(ob["v"]).max()), 1.0)
+        np.testing.assert_allclose(v, ob["v"], rtol=0, atol=2e-6 * scale)
+        np.testing.assert_allclose(w, 0.0, atol=0)
+        np.testing.assert_allclose(com, ob["com"], rtol=0, atol=2e-6)
+        lam = model.last[2]
+        ref = oc["points"]["normal_impulse"][:, :1]
+        np.testing.assert_allclose(lam[:, :1], ref, rtol=0, atol=1e-5 * max(float(np.abs(ref).max()), 1.0))
+        bodies = ob
+        contacts, _ = world.collide(bodies, world.dt, oc)
+    world.close()

@@ -1558,7 +1558,7 @@ __global__ void k_stream_match(const tfs_pair* pairs, uint32_t np, const tfs_con
     }
 }
 
-// src/world.cpp:969-980: an old contact whose bodies are neither dynamic and awake nor moving kinematic,
+// src/world.cpp:972-984: an old contact whose bodies are neither dynamic and awake nor moving kinematic,
 // one of them dynamic, stays as it is.
 __global__ void k_stream_resting(const tfs_contact* old, uint32_t nOld, const tfs_body* bodies, uint32_t* flag) {
     for (uint32_t i = gid(); i < nOld; i += gstride()) {

@@ -960,7 +960,7 @@ TF_HD V3 lift(const tfs_body& a, const tfs_body& b, V3 normal, float separation,
     return separation > speculative ? normal * (0.5f * separation) : vec(0.0f, 0.0f, 0.0f);
 }
 
-// stillFits: src/world.cpp:1002-1009. `reach` = contactReuseDistance x lengthUnit, `reuseCos` =
+// stillFits: src/world.cpp:1002-1008. `reach` = contactReuseDistance x lengthUnit, `reuseCos` =
 // cos(contactReuseAngle / 2) as the host's cosf gives it.
 TF_HD bool stillFits(const tfs_contact_ext& c, const tfs_body& a, const tfs_body& b, float reach, float reuseCos) {
     if (!c.reusable) { return false; }
@@ -979,7 +979,7 @@ TF_HD void remember(const tfs_body& a, const tfs_body& b, const tfs_contact& c, 
     e.reusable = 1;
 }
 
-// measureAgain: src/world.cpp:1013-1025. Kept points stay where they were on each body; the normal
+// measureAgain: src/world.cpp:1011-1025. Kept points stay where they were on each body; the normal
 // turns with body A.
 TF_HD void measureAgain(const tfs_body& a, const tfs_body& b, tfs_contact& c, const tfs_contact_ext& e, float speculative) {
     const V3 n = rotate(ld4(a.q), ld3(e.local_normal));
