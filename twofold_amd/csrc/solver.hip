@@ -572,6 +572,7 @@ struct Rows {
     float4* imp;    // normal impulse, tangent impulse
     float* peak;    // largest normal impulse
     float4* ax;     // point axes: [((k * 3 + d) * 5 + q) * cap + r], d = normal, tangent1, tangent2
+    float4* sep;    // each point's separation in the current pass (one lane per point), written by the pass
     uint32_t cap;
 };
 
@@ -873,6 +874,7 @@ struct PassPlan {
     uint32_t n;                 // segments
     uint32_t impactFrom;        // segments from here are impact rows
     uint32_t tailFrom;          // segments from here are solved by block 0 alone (passTail)
+    uint32_t sepRows;           // rows 0..sepRows-1 get their separations in the pass's first phase
     uint32_t begin[kMaxSegments], end[kMaxSegments];
 };
 
@@ -969,20 +971,19 @@ constexpr uint32_t kTailSlots = 512;      // LDS body slots (at most 2 per tail 
 constexpr uint32_t kTailSegRows = 64;     // a tail colour has at most one wave of rows
 constexpr uint32_t kNoSlot = 0xffffffffu;
 
-// What a row reads that the tail does not change: directions, anchors, axes, the bodies' position
-// and rotation changes.
+// What a row reads that the pass does not change: directions, its points' separations in this pass,
+// and the packed axes (MADE: the anchors, to make the axes from).
 template <bool MADE>
 struct RowData {
     float4 n, t1, t2;
-    float4 pa[4], pb[4];
+    float4 sep;
+    float4 pa[MADE ? 4 : 1], pb[MADE ? 4 : 1];
     float4 axv[4][MADE ? 1 : 15];
-    float4 dpA, dqA, dpB, dqB;
 };
 
-// DPQ: the bodies' dp and dq were written earlier in the same launch (the fused substep kernel), by
-// other blocks: they are loaded coherently.
-template <bool MADE, bool DPQ = false>
-__device__ __forceinline__ void loadRowData(const Rows& rows, uint32_t r, uint4 h, const float4* motion, RowData<MADE>& d) {
+// Everything but the separations, which the pass's first phase writes (loadRowSep).
+template <bool MADE>
+__device__ __forceinline__ void loadRowData(const Rows& rows, uint32_t r, uint4 h, RowData<MADE>& d) {
     const int count = (int)(h.z & kCountMask);
     d.n = rows.nrm[r];
     d.t1 = rows.tg1[r];
@@ -991,19 +992,53 @@ __device__ __forceinline__ void loadRowData(const Rows& rows, uint32_t r, uint4 
     for (int k = 0; k < 4; ++k) {
         if (k < count) {
             const uint32_t j = (uint32_t)k * rows.cap + r;
-            d.pa[k] = rows.pa[j];
-            d.pb[k] = rows.pb[j];
-            if (!MADE) {
+            if constexpr (MADE) {
+                d.pa[k] = rows.pa[j];
+                d.pb[k] = rows.pb[j];
+            } else {
 #pragma unroll
                 for (int q = 0; q < 15; ++q) { d.axv[k][q] = rows.ax[(size_t)(k * 15 + q) * rows.cap + r]; }
             }
         }
     }
+}
+
+// A row's separations, written in the same launch by the pass's first phase on any block: loaded
+// coherently, once that phase's barrier has passed.
+__device__ __forceinline__ float4 loadRowSep(const Rows& rows, uint32_t r) { return Access<true>::ld(&rows.sep[r]); }
+
+// The first phase of a pass: every point's current separation (solveLanes, src/lanes.cpp:236-240: its
+// anchors turned by the bodies' rotation this step, plus their travel dp = b.dp - a.dp, along the
+// normal, plus the adjusted separation), the same operations as pointSeparation in the row solve. It
+// depends only on the bodies' dp and dq, which no row of a pass changes, so the rows of every colour
+// take it from here instead of computing it in their chain. Stored coherently for the threads that
+// solve the row, on any block. DPQ: dp and dq were written in the same launch (the fused substep
+// kernel), by other blocks: they are loaded coherently.
+template <bool DPQ>
+__device__ __forceinline__ void rowSeparations(const Rows& rows, uint32_t r, const float4* motion) {
     using X = Access<DPQ>;
-    d.dpA = X::ld(&motion[4 * h.x + 2]);
-    d.dqA = X::ld(&motion[4 * h.x + 3]);
-    d.dpB = X::ld(&motion[4 * h.y + 2]);
-    d.dqB = X::ld(&motion[4 * h.y + 3]);
+    const uint4 h = rows.hdr[r];
+    const int count = (int)(h.z & kCountMask);
+    const float4 n4 = rows.nrm[r];
+    const float4 dpA = X::ld(&motion[4 * h.x + 2]), dqA = X::ld(&motion[4 * h.x + 3]);
+    const float4 dpB = X::ld(&motion[4 * h.y + 2]), dqB = X::ld(&motion[4 * h.y + 3]);
+    const Motion A{ vec(0.0f, 0.0f, 0.0f), vec(0.0f, 0.0f, 0.0f), vec(dpA.x, dpA.y, dpA.z), quat(dqA.x, dqA.y, dqA.z, dqA.w) };
+    const Motion B{ vec(0.0f, 0.0f, 0.0f), vec(0.0f, 0.0f, 0.0f), vec(dpB.x, dpB.y, dpB.z), quat(dqB.x, dqB.y, dqB.z, dqB.w) };
+    const V3 dp = B.dp - A.dp, n = vec(n4.x, n4.y, n4.z);
+    float sp[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (k < count) {
+            const uint32_t j = (uint32_t)k * rows.cap + r;
+            const float4 a = rows.pa[j], b = rows.pb[j];
+            RowPoint p;
+            p.ra = vec(a.x, a.y, a.z);
+            p.adj = a.w;
+            p.rb = vec(b.x, b.y, b.z);
+            sp[k] = pointSeparation(A, B, dp, p, n);
+        }
+    }
+    Access<true>::st(&rows.sep[r], make_float4(sp[0], sp[1], sp[2], sp[3]));
 }
 
 // rowStep over data already loaded (loadRowData): only what rows of the pass write (velocities,
@@ -1015,16 +1050,18 @@ __device__ __forceinline__ void rowSolveLoaded(const Rows& rows, uint32_t r, uin
     const int count = (int)(h.z & kCountMask);
     if (count == 0) { return; }
     const Row row{ h.z, vec(d.n.x, d.n.y, d.n.z), vec(d.t1.x, d.t1.y, d.t1.z), vec(d.t2.x, d.t2.y, d.t2.z), d.n.w, d.t1.w };
+    const float sp[4] = { d.sep.x, d.sep.y, d.sep.z, d.sep.w };
     RowPoint p[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         if (k < count) {
             const uint32_t j = (uint32_t)k * rows.cap + r;
             const float4 im = X::ld(&rows.imp[j]);
-            p[k].ra = vec(d.pa[k].x, d.pa[k].y, d.pa[k].z);
-            p[k].adj = d.pa[k].w;
-            p[k].rb = vec(d.pb[k].x, d.pb[k].y, d.pb[k].z);
-            p[k].rv = d.pb[k].w;
+            if constexpr (MADE) {
+                p[k].ra = vec(d.pa[k].x, d.pa[k].y, d.pa[k].z);
+                p[k].rb = vec(d.pb[k].x, d.pb[k].y, d.pb[k].z);
+            }
+            p[k].sep = sp[k];
             p[k].ni = im.x;
             p[k].ti = vec(im.y, im.z, im.w);
             p[k].nimax = X::ld1(&rows.peak[j]);
@@ -1032,33 +1069,27 @@ __device__ __forceinline__ void rowSolveLoaded(const Rows& rows, uint32_t r, uin
     }
     const float4 va = X::ld(&motion[4 * h.x]), wa = X::ld(&motion[4 * h.x + 1]);
     const float4 vb = X::ld(&motion[4 * h.y]), wb = X::ld(&motion[4 * h.y + 1]);
-    Motion A{ vec(va.x, va.y, va.z), vec(wa.x, wa.y, wa.z), vec(d.dpA.x, d.dpA.y, d.dpA.z),
-              quat(d.dqA.x, d.dqA.y, d.dqA.z, d.dqA.w) };
-    Motion B{ vec(vb.x, vb.y, vb.z), vec(wb.x, wb.y, wb.z), vec(d.dpB.x, d.dpB.y, d.dpB.z),
-              quat(d.dqB.x, d.dqB.y, d.dqB.z, d.dqB.w) };
+    // (dp and dq enter only through the separations)
+    Motion A{ vec(va.x, va.y, va.z), vec(wa.x, wa.y, wa.z), vec(0.0f, 0.0f, 0.0f), quat(0.0f, 0.0f, 0.0f, 1.0f) };
+    Motion B{ vec(vb.x, vb.y, vb.z), vec(wb.x, wb.y, wb.z), vec(0.0f, 0.0f, 0.0f), quat(0.0f, 0.0f, 0.0f, 1.0f) };
     if (TAPE && tape.v) {
         const uint32_t c = tape.cap;
         tape.v[r] = va;
         tape.v[c + r] = wa;
         tape.v[2 * c + r] = vb;
         tape.v[3 * c + r] = wb;
-        const V3 dp = B.dp - A.dp;
-        float sp[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            if (k < count) {
-                tape.imp[(uint32_t)k * c + r] = make_float4(p[k].ni, p[k].ti.x, p[k].ti.y, p[k].ti.z);
-                sp[k] = pointSeparation(A, B, dp, p[k], row.n);
-            }
+            if (k < count) { tape.imp[(uint32_t)k * c + r] = make_float4(p[k].ni, p[k].ti.x, p[k].ti.y, p[k].ti.z); }
         }
-        tape.sep[r] = make_float4(sp[0], sp[1], sp[2], sp[3]);
+        tape.sep[r] = d.sep;
     }
-    if (MADE) {
+    if constexpr (MADE) {
         const RowMadeAxes ax{ loadMass(mass, h.x), loadMass(mass, h.y), row, p };
-        tf::rows<MODE>(row, p, ax, A, B, ctx);
+        tf::rows<MODE, RowMadeAxes, true>(row, p, ax, A, B, ctx);
     } else {
-        const RegAxes<MADE ? 1 : 15> ax{ d.axv };
-        tf::rows<MODE>(row, p, ax, A, B, ctx);
+        const RegAxes<15> ax{ d.axv };
+        tf::rows<MODE, RegAxes<15>, true>(row, p, ax, A, B, ctx);
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -1141,7 +1172,10 @@ __device__ __forceinline__ void passTail(const Rows& rows, const uint4* impactHd
     RowData<MADE> d;
     uint32_t sa = kNoSlot, sb = kNoSlot;
     if (row != kNoSlot) {
-        loadRowData<MADE, FUSED>(rows, row, h, motion, d);
+        // (the separations are final: the pass's first phase ended with a barrier that every block
+        // has passed before any tail begins)
+        loadRowData<MADE>(rows, row, h, d);
+        d.sep = loadRowSep(rows, row);
         sa = tailSlot(slotKey, h.x);
         sb = tailSlot(slotKey, h.y);
         if (sa == kNoSlot || sb == kNoSlot) { atomicOr(&cnt->error, 4u); }
@@ -1173,49 +1207,44 @@ __device__ __forceinline__ void passTail(const Rows& rows, const uint4* impactHd
     }
     __syncthreads();
     const Row rowv{ h.z, vec(d.n.x, d.n.y, d.n.z), vec(d.t1.x, d.t1.y, d.t1.z), vec(d.t2.x, d.t2.y, d.t2.z), d.n.w, d.t1.w };
+    const float sp[4] = { d.sep.x, d.sep.y, d.sep.z, d.sep.w };
     for (uint32_t s = tf; s < plan.n; ++s) {
         if (owner && count > 0 && (s == activeSeg || s == impactSeg)) {
             RowPoint p[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 if (k < count) {
-                    p[k].ra = vec(d.pa[k].x, d.pa[k].y, d.pa[k].z);
-                    p[k].adj = d.pa[k].w;
-                    p[k].rb = vec(d.pb[k].x, d.pb[k].y, d.pb[k].z);
-                    p[k].rv = d.pb[k].w;
+                    if constexpr (MADE) {
+                        p[k].ra = vec(d.pa[k].x, d.pa[k].y, d.pa[k].z);
+                        p[k].rb = vec(d.pb[k].x, d.pb[k].y, d.pb[k].z);
+                    }
+                    p[k].sep = sp[k];
                     p[k].ni = im[k].x;
                     p[k].ti = vec(im[k].y, im[k].z, im[k].w);
                     p[k].nimax = peak[k];
                 }
             }
             const float4 va = slotV[sa], wa = slotW[sa], vb = slotV[sb], wb = slotW[sb];
-            Motion A{ vec(va.x, va.y, va.z), vec(wa.x, wa.y, wa.z), vec(d.dpA.x, d.dpA.y, d.dpA.z),
-                      quat(d.dqA.x, d.dqA.y, d.dqA.z, d.dqA.w) };
-            Motion B{ vec(vb.x, vb.y, vb.z), vec(wb.x, wb.y, wb.z), vec(d.dpB.x, d.dpB.y, d.dpB.z),
-                      quat(d.dqB.x, d.dqB.y, d.dqB.z, d.dqB.w) };
+            Motion A{ vec(va.x, va.y, va.z), vec(wa.x, wa.y, wa.z), vec(0.0f, 0.0f, 0.0f), quat(0.0f, 0.0f, 0.0f, 1.0f) };
+            Motion B{ vec(vb.x, vb.y, vb.z), vec(wb.x, wb.y, wb.z), vec(0.0f, 0.0f, 0.0f), quat(0.0f, 0.0f, 0.0f, 1.0f) };
             if (TAPE && tape.v && s == activeSeg) {
                 const uint32_t c = tape.cap;
                 tape.v[row] = va;
                 tape.v[c + row] = wa;
                 tape.v[2 * c + row] = vb;
                 tape.v[3 * c + row] = wb;
-                const V3 dp = B.dp - A.dp;
-                float sp[4] = { 0.0f, 0.0f, 0.0f, 0.0f };
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    if (k < count) {
-                        tape.imp[(uint32_t)k * c + row] = im[k];
-                        sp[k] = pointSeparation(A, B, dp, p[k], rowv.n);
-                    }
+                    if (k < count) { tape.imp[(uint32_t)k * c + row] = im[k]; }
                 }
-                tape.sep[row] = make_float4(sp[0], sp[1], sp[2], sp[3]);
+                tape.sep[row] = d.sep;
             }
-            if (MADE) {
+            if constexpr (MADE) {
                 const RowMadeAxes ax{ loadMass(mass, h.x), loadMass(mass, h.y), rowv, p };
-                tf::rows<MODE>(rowv, p, ax, A, B, ctx);
+                tf::rows<MODE, RowMadeAxes, true>(rowv, p, ax, A, B, ctx);
             } else {
-                const RegAxes<MADE ? 1 : 15> ax{ d.axv };
-                tf::rows<MODE>(rowv, p, ax, A, B, ctx);
+                const RegAxes<15> ax{ d.axv };
+                tf::rows<MODE, RegAxes<15>, true>(rowv, p, ax, A, B, ctx);
             }
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -1254,7 +1283,8 @@ __device__ __forceinline__ void passTail(const Rows& rows, const uint4* impactHd
     }
 }
 
-// Headers, directions, points, axes and the bodies' dp and dq do not change during a pass, so each
+// First, with plan.sepRows, every active row's separations (rowSeparations) over all threads, and a
+// grid barrier. Headers, directions, axes and separations do not change during the colours, so each
 // thread loads them for its first row of the next segment while it waits at the barrier; after the
 // barrier only the velocities and impulses that earlier segments wrote are loaded.
 // Segments from plan.tailFrom on are the tail (passTail).
@@ -1267,12 +1297,21 @@ __device__ __forceinline__ void passRun(const Rows& rows, const uint4* impactHdr
     const uint32_t g = passSlot(), stride = gstride();
     auto header = [&](uint32_t s, uint32_t i) { return s >= plan.impactFrom ? impactHdr[i] : rows.hdr[i]; };
     const uint32_t grid = plan.tailFrom;
+    if (plan.sepRows > 0) {
+        for (uint32_t r = gid(); r < plan.sepRows; r += stride) { rowSeparations<FUSED>(rows, r, motion); }
+        gridArrive(bar, phase);
+    }
     uint4 next = make_uint4(0u, 0u, 0u, 0u);
     RowData<MADE> nd;
-    if (grid > 0 && plan.begin[0] + g < plan.end[0]) {
+    const bool firstRow = grid > 0 && plan.begin[0] + g < plan.end[0];
+    uint32_t firstAt = 0;   // the row whose data the first segment's row reads (an impact row: its active row)
+    if (firstRow) {
         next = header(0, plan.begin[0] + g);
-        loadRowData<MADE, FUSED>(rows, 0 >= plan.impactFrom ? next.w : plan.begin[0] + g, next, motion, nd);
+        firstAt = plan.impactFrom == 0 ? next.w : plan.begin[0] + g;
+        loadRowData<MADE>(rows, firstAt, next, nd);
     }
+    if (plan.sepRows > 0) { gridWait(bar, phase, cnt); }
+    if (firstRow) { nd.sep = loadRowSep(rows, firstAt); }
     for (uint32_t s = 0; s < grid; ++s) {
         const bool impact = s >= plan.impactFrom;
         const uint32_t first = plan.begin[s] + g;
@@ -1283,7 +1322,8 @@ __device__ __forceinline__ void passRun(const Rows& rows, const uint4* impactHdr
             uint4 h = firstHdr;
             if (i != first) {
                 h = header(s, i);
-                loadRowData<MADE, FUSED>(rows, impact ? h.w : i, h, motion, nd);
+                loadRowData<MADE>(rows, impact ? h.w : i, h, nd);
+                nd.sep = loadRowSep(rows, impact ? h.w : i);
             }
             rowSolveLoaded<MODE, MADE, TAPE>(rows, impact ? h.w : i, h, nd, motion, mass, ctx, impact ? Tape{} : tape);
         }
@@ -1291,7 +1331,11 @@ __device__ __forceinline__ void passRun(const Rows& rows, const uint4* impactHdr
             // The next segment's first row: what no row of the pass writes is loaded between arriving
             // at the barrier (which waits for this thread's memory operations) and waiting at it.
             gridArrive(bar, phase);
-            if (more) { loadRowData<MADE, FUSED>(rows, s + 1 >= plan.impactFrom ? next.w : plan.begin[s + 1] + g, next, motion, nd); }
+            if (more) {
+                const uint32_t at = s + 1 >= plan.impactFrom ? next.w : plan.begin[s + 1] + g;
+                loadRowData<MADE>(rows, at, next, nd);
+                nd.sep = loadRowSep(rows, at);
+            }
             gridWait(bar, phase, cnt);
         } else if (grid < plan.n) {
             gridArrive(bar, phase);
@@ -1314,7 +1358,7 @@ __global__ void __launch_bounds__(kPassThreads) k_pass(Rows rows, const uint4* i
 // The barriers a pass launch of `plan` arrives at.
 __host__ __device__ __forceinline__ uint32_t passBarriers(const PassPlan& plan) {
     if (plan.n == 0) { return 0; }
-    return plan.tailFrom == plan.n ? plan.n - 1 : plan.tailFrom;
+    return (plan.tailFrom == plan.n ? plan.n - 1 : plan.tailFrom) + (plan.sepRows > 0 ? 1u : 0u);
 }
 
 // ---- warm start, one body at a time ----------------------------------------------------------------
@@ -3005,7 +3049,7 @@ int allocRows(tfs_ctx* ctx, Rows& r, uint32_t cap) {
     if ((e = grow(ctx, r.hdr, cap)) || (e = grow(ctx, r.nrm, cap)) || (e = grow(ctx, r.tg1, cap)) ||
         (e = grow(ctx, r.tg2, cap)) || (e = grow(ctx, r.pa, 4 * (size_t)cap)) || (e = grow(ctx, r.pb, 4 * (size_t)cap)) ||
         (e = grow(ctx, r.imp, 4 * (size_t)cap)) || (e = grow(ctx, r.peak, 4 * (size_t)cap)) ||
-        (e = grow(ctx, r.ax, 60 * (size_t)cap))) {
+        (e = grow(ctx, r.ax, 60 * (size_t)cap)) || (e = grow(ctx, r.sep, cap))) {
         return e;
     }
     r.cap = cap;
@@ -3194,6 +3238,9 @@ void planPass(const tfs_ctx* ctx, int mode, const DevCounts& hc, bool withActive
     };
     if (withActive) { add(hc.colorStart, hc.colorPoints); }
     plan.impactFrom = plan.n;
+    // the separations of every coloured active row, which the impact rows of the pass read too (an
+    // impact-only launch follows the active launch of the same pass, which computed them)
+    plan.sepRows = withActive && plan.n > 0 ? hc.colorStart[TFS_COLORS] : 0u;
     if (withImpact) { add(hc.impactColorStart, hc.impactColorPoints); }
     grid = 1;
     if (plan.n == 0) { return; }
@@ -4290,7 +4337,7 @@ void tfs_destroy(tfs_ctx* ctx) {
                      ctx->impact, ctx->rowsList, ctx->impactRowsList, ctx->colors, ctx->ep, ctx->rank, ctx->rowOf,
                      ctx->hist, ctx->status, ctx->cnt, ctx->bar, ctx->rows.hdr, ctx->rows.nrm,
                      ctx->rows.tg1, ctx->rows.tg2, ctx->rows.pa, ctx->rows.pb, ctx->rows.imp, ctx->rows.peak,
-                     ctx->rows.ax, ctx->impactHdr, ctx->bdeg, ctx->bstart, ctx->bcursor, ctx->bep,
+                     ctx->rows.ax, ctx->rows.sep, ctx->impactHdr, ctx->bdeg, ctx->bstart, ctx->bcursor, ctx->bep,
                      ctx->edges, ctx->bodyWorld, ctx->refreshing, ctx->worldStart, ctx->restTracked,
                      ctx->restDeg, ctx->restStart, ctx->restCursor, ctx->restEp, ctx->restRank, ctx->restLevel,
                      ctx->restCount, ctx->restWord, ctx->restEntries[0], ctx->restEntries[1], ctx->restLevelStart[0],

@@ -231,6 +231,7 @@ struct RowPoint {
     float ni;        // normal impulse
     V3 ti;           // tangent impulse
     float nimax;     // largest normal impulse this step
+    float sep;       // the current separation, when the pass computed it beforehand (PRESEP)
 };
 
 struct Row {
@@ -338,8 +339,9 @@ TF_HD FrictionTrace frictionPoint(RowPoint& p, const Axis& x1, const Axis& x2, c
 }
 
 // The 3D contact solve: solveLanes for one lane (src/lanes.cpp:220-315), which is also the 3D part
-// of solveContacts (src/solver.cpp:370-417).
-template <int N, bool BIAS, class AX>
+// of solveContacts (src/solver.cpp:370-417). PRESEP: each point's separation (pointSeparation of the
+// same bodies' dp and dq, which do not change during a pass) was computed before the pass, in p[i].sep.
+template <int N, bool BIAS, class AX, bool PRESEP = false>
 TF_HD void solveRowN(const Row& r, RowPoint* p, const AX& ax, Motion& a, Motion& b, const Ctx& c, int count) {
     const bool moveA = r.flags & kMoveA, moveB = r.flags & kMoveB;
     const Soft soft = (moveA && moveB) ? c.contactSoft : c.staticSoft;
@@ -347,7 +349,8 @@ TF_HD void solveRowN(const Row& r, RowPoint* p, const AX& ax, Motion& a, Motion&
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         if (i >= count) { continue; }
-        normalPoint<BIAS>(p[i], pointSeparation(a, b, dp, p[i], r.n), ax(i, 0), r.n, a, b, c, soft, moveA, moveB);
+        const float sep = PRESEP ? p[i].sep : pointSeparation(a, b, dp, p[i], r.n);
+        normalPoint<BIAS>(p[i], sep, ax(i, 0), r.n, a, b, c, soft, moveA, moveB);
     }
 #pragma unroll
     for (int j = 0; j < N; ++j) {
@@ -482,7 +485,7 @@ enum RowMode { kWarm = 0, kBias = 1, kRelax = 2, kRestitute = 3 };
 // One row in the given mode: its first (flags & kCountMask) points, at most 4. In relax mode the
 // friction half runs from the last point down, so skipping the empty slots 3..count gives the order
 // count-1..0 that a solve over exactly `count` points has.
-template <int MODE, class AX>
+template <int MODE, class AX, bool PRESEP = false>
 TF_HD void rows(const Row& r, RowPoint* p, const AX& ax, Motion& a, Motion& b, const Ctx& c) {
     const int count = (int)(r.flags & kCountMask);
     if (MODE == kWarm) {
@@ -490,7 +493,7 @@ TF_HD void rows(const Row& r, RowPoint* p, const AX& ax, Motion& a, Motion& b, c
     } else if (MODE == kRestitute) {
         restituteRowN<4>(r, p, ax, a, b, c, count);
     } else {
-        solveRowN<4, MODE == kBias>(r, p, ax, a, b, c, count);
+        solveRowN<4, MODE == kBias, AX, PRESEP>(r, p, ax, a, b, c, count);
     }
 }
 
