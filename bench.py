@@ -40,6 +40,14 @@ HBM_PEAK_GBS = 8000.0
 K_PASS_TRAFFIC_BYTES = 102.4e6
 K_PASS_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, mean of the last 32 k_pass dispatches, scaled by a "
                          "256 MiB copy (profiles/k_pass_pile32k_traffic.md)")
+# The same for the batched worlds' passes (mean of the 16 batched k_pass dispatches of 2 timed steps) and
+# for the adjoint's reverse passes (mean of the last 32 k_pass_adj dispatches), measured the same way.
+BATCHED_TRAFFIC_BYTES = 2.815e9
+BATCHED_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, mean of the batched k_pass dispatches of 2 timed "
+                          "steps, scaled by a 256 MiB copy (profiles/k_pass_batched4096_kernel_stats.md)")
+ADJOINT_TRAFFIC_BYTES = 868.6e6
+ADJOINT_TRAFFIC_SOURCE = ("rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, mean of the last 32 k_pass_adj dispatches, scaled by "
+                          "a 256 MiB copy (profiles/k_pass_adj_adjoint1024_kernel_stats.md)")
 
 
 def log(*a):
@@ -384,7 +392,8 @@ def _run_batched(args, worlds, sim, t0):
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None, "traffic_source": None,
+            "traffic": BATCHED_TRAFFIC_BYTES if args.worlds == 4096 else None,
+            "traffic_source": BATCHED_TRAFFIC_SOURCE if args.worlds == 4096 else None,
             "kernel": "k_pass<bias|relax> (one grid-barrier launch per solve pass over all worlds' rows)",
             "launches": launches, "profiled_steps": args.batch_profile_steps,
             "bytes_per_launch": algo / max(launches, 1), "ms_per_launch": ms / max(launches, 1),
@@ -481,7 +490,8 @@ def run_adjoint(args):
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None, "traffic_source": None,
+            "traffic": ADJOINT_TRAFFIC_BYTES if args.worlds == 1024 else None,
+            "traffic_source": ADJOINT_TRAFFIC_SOURCE if args.worlds == 1024 else None,
             "kernel": "k_pass_adj<bias|relax> (one grid-barrier launch per reverse pass over all worlds' rows; bytes "
                       "per row 440 + 348n)",
             "launches": launches, "profiled_steps": args.steps,
