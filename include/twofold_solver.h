@@ -308,16 +308,19 @@ int tfs_set_worlds(tfs_ctx* ctx, uint32_t n_worlds, const uint32_t* body_start);
 int tfs_step(tfs_ctx* ctx, float dt, tfs_refresh_fn refresh, void* user);
 
 /*
- * The device refresh (no counterpart in the reference, whose refresh collides on the host: the
- * narrowphase half of refreshContacts, src/world.cpp:1076-1122). With one shape per resident body
- * set here, every refresh of tfs_step collides its pairs on the device, with the host world's
- * sphere, box and half-space narrowphase (twofold_amd/csrc/narrow.h, the same source the host world
- * runs, so the manifolds are the ones its refresh callback would give, bit for bit); the refresh
- * callback is then not called and may be NULL. This takes the host round trip and the transfer of
- * the pairs out of each refresh. n = 0 turns it off. TFS_E_ARG for an unknown kind or a size that is
- * not finite and positive; TFS_E_STATE from tfs_step when n is not the number of resident bodies;
- * TFS_E_KERNEL from tfs_step when a pair is two boxes that both stay extruded along the world z
- * (static, or locked to a plane with normal z), which the host world collides as a 2D problem.
+ * The device refresh (the reference's refresh collides on the host: the narrowphase half of
+ * refreshContacts, src/world.cpp:1076-1122). With one shape per resident body set here, at the body's
+ * origin, every refresh of tfs_step collides its pairs on the device with the reference's own
+ * narrowphase for spheres, boxes and half-spaces, restated operation for operation
+ * (twofold_amd/csrc/collide3d.h: src/collide3d.cpp collide, collideHalfSpace, collideSpheres,
+ * collideSpherePoly, the two-box collideConvex with its GJK gap test, reduceManifold), so the manifolds
+ * are the reference's bit for bit; the refresh callback is then not called and may be NULL. This takes
+ * the host round trip and the transfer of the pairs out of each refresh. Box half-extents and sphere
+ * radii are the sizes the reference keeps after building the shape (buildShape, src/shape.cpp:149).
+ * n = 0 turns it off. TFS_E_ARG for an unknown kind or a size that is not finite and positive;
+ * TFS_E_STATE from tfs_step when n is not the number of resident bodies; TFS_E_KERNEL from tfs_step
+ * when a pair is one the reference collides in 2D (every dynamic body of the pair locked to one plane,
+ * usePlanar, src/world.cpp:737-750).
  */
 int tfs_set_body_shapes(tfs_ctx* ctx, uint32_t n, const tfs_shape* shapes);
 

@@ -678,4 +678,77 @@ uint32_t tfr_cap_refresh(void* h, uint32_t k, int* substep, float* remaining, ui
 size_t tfr_sizeof_body(void) { return sizeof(tfs_body); }
 size_t tfr_sizeof_contact(void) { return sizeof(tfs_contact); }
 
+// Each body's shape as tfs_shape (include/twofold_solver.h) for the device refresh: the reference's
+// sizes after buildShape, for bodies with exactly one shape at the body's origin. Kinds: 0 sphere,
+// 1 box, 4 half-space; 255 for anything else (another geometry, several shapes, a shape off the origin).
+uint32_t tfr_body_shapes(void* h, tfs_shape* out, uint32_t cap) {
+    const World::Impl& w = *((Handle*)h)->world.impl_;
+    const uint32_t n = (uint32_t)w.bodies.size();
+    for (uint32_t i = 0; i < n && i < cap; ++i) {
+        tfs_shape& o = out[i];
+        std::memset(&o, 0, sizeof o);
+        o.kind = 255;
+        const Body& b = w.bodies[i];
+        if (b.shapes.size() != 1) { continue; }
+        const Shape& s = w.shapes[b.shapes[0]];
+        const bool atOrigin = s.local.p.x == 0.0f && s.local.p.y == 0.0f && s.local.p.z == 0.0f && s.local.q.x == 0.0f &&
+                              s.local.q.y == 0.0f && s.local.q.z == 0.0f && s.local.q.w == 1.0f;
+        if (!atOrigin) { continue; }
+        if (s.kind == ShapeKind::Sphere) {
+            o.kind = 0;
+            o.radius = s.radius;
+        } else if (const Box* bx = std::get_if<Box>(&s.geometry)) {
+            o.kind = 1;
+            put3(o.half, bx->halfExtents);
+        } else if (s.kind == ShapeKind::HalfSpace) {
+            o.kind = 4;
+        }
+    }
+    return n;
+}
+
+// The reference's own narrowphase on one pair of shapes, for checking a restatement of it: kind 0 sphere
+// (dims[0] = radius), 1 box (dims = half-extents), 4 half-space; pa, pb the shapes' world transforms
+// (p[3], q[4]); motion = v, w and centre of A, then of B (18 floats), over dt, fast as Motion::fast.
+// The shapes go through buildShape (src/shape.cpp:149), so their sizes are the ones the reference keeps;
+// solid[0..3] / solid[4..7] receive them (half-extents, radius). out = normal[3], count, then per point
+// position[3], separation, predicted. Returns 0, or -1 for another kind.
+int tfr_collide(int ka, const float* da, const float* pa, int kb, const float* db, const float* pb, float margin,
+                float linearSlop, const float* motion, float dt, int fast, float* solid, float* out) {
+    Shape s[2];
+    const int kinds[2] = { ka, kb };
+    const float* dims[2] = { da, db };
+    for (int i = 0; i < 2; ++i) {
+        if (kinds[i] == 0) {
+            s[i].geometry = Sphere{ dims[i][0] };
+        } else if (kinds[i] == 1) {
+            s[i].geometry = Box{ Vec3{ dims[i][0], dims[i][1], dims[i][2] } };
+        } else if (kinds[i] == 4) {
+            s[i].geometry = HalfSpace{};
+        } else {
+            return -1;
+        }
+        buildShape(s[i]);
+        float* o = solid + 4 * i;
+        o[0] = o[1] = o[2] = 0.0f;
+        o[3] = s[i].radius;
+        if (const Box* b = std::get_if<Box>(&s[i].geometry)) { put3(o, b->halfExtents); }
+    }
+    const Transform xa{ vec3(pa), quat4(pa + 3) }, xb{ vec3(pb), quat4(pb + 3) };
+    Tolerances tol;
+    tol.linearSlop = linearSlop;
+    Motion mo{ vec3(motion), vec3(motion + 3), vec3(motion + 6), vec3(motion + 9), vec3(motion + 12),
+               vec3(motion + 15), dt, fast != 0 };
+    Manifold m;
+    collide(s[0], xa, s[1], xb, margin, tol, mo, m);
+    put3(out, m.normal);
+    out[3] = (float)m.count;
+    for (int i = 0; i < 4; ++i) {
+        put3(out + 4 + 5 * i, m.points[i].position);
+        out[4 + 5 * i + 3] = m.points[i].separation;
+        out[4 + 5 * i + 4] = m.points[i].predicted;
+    }
+    return 0;
+}
+
 }

@@ -75,6 +75,8 @@ def _load_ref():
         lib.tfr_cap_ext.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_uint32]
         lib.tfr_hits.restype = ctypes.c_uint32
         lib.tfr_hits.argtypes = [_vp, _vp, ctypes.c_uint32]
+        lib.tfr_body_shapes.restype = ctypes.c_uint32
+        lib.tfr_body_shapes.argtypes = [_vp, _vp, ctypes.c_uint32]
         lib.tfr_cap_refresh.restype = ctypes.c_uint32
         lib.tfr_cap_refresh.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int), _fp, _vp, _vp,
                                         ctypes.c_uint32]
@@ -152,6 +154,14 @@ class RefWorld:
         n = self.lib.tfr_hits(self.h, None, 0)
         out = np.zeros(n, _abi.HIT_DTYPE)
         self.lib.tfr_hits(self.h, _ptr(out), n)
+        return out
+
+    def body_shapes(self):
+        """Each body's one shape as the device refresh takes it (SHAPE_DTYPE), with the reference's sizes;
+        kind 255 where a body has another geometry or more than one shape."""
+        n = self.lib.tfr_num_bodies(self.h)
+        out = np.zeros(n, _abi.SHAPE_DTYPE)
+        self.lib.tfr_body_shapes(self.h, _ptr(out), n)
         return out
 
     def bodies(self):

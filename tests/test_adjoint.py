@@ -13,7 +13,7 @@ needs_oracle = pytest.mark.skipif(not __import__("os").path.exists(refworld.ORAC
 
 @pytest.fixture(scope="module")
 def case():
-    return Case([0x51, 0x52], steps=4)
+    return Case([0x53, 0x54], steps=4)
 
 
 def directional(case, name, u, h):

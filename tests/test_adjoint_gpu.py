@@ -8,9 +8,10 @@ within 1e-4 of the displacement's norm. Gradients over 2 steps: within 1e-4 rela
 parameter kind (float32 rounding of the reverse sweep itself). Over the workload's 32 steps the loss is
 piecewise smooth (every clamp is a kink) and float32 rounding moves a few marginal clamps to their other
 side: the float64 model's own gradient moves by 1e-4 when v0 is perturbed by 1e-6 relative over 16
-steps. There the directional derivatives are held to 1e-3 of central differences of the float64 loss,
-and the norm-wise comparison with the float64 gradient to 1e-3 for velocities and friction and 5e-3
-for spins, whose gradient is the smallest and most sensitive."""
+steps. There the norm-wise comparison with the float64 gradient is held to 1e-3 for velocities and
+friction and 5e-3 for spins, whose gradient is the smallest and most sensitive, and so are directional
+derivatives against central differences of the float64 loss, along directions with a real projection on
+the gradient."""
 import numpy as np
 import pytest
 
@@ -106,24 +107,36 @@ def test_gradients_match_model(case, gpu_run, model_run):
     assert errs["v"] <= 1e-3 and errs["friction"] <= 1e-3 and errs["w"] <= 5e-3, errs
 
 
-def test_gradients_match_central_differences(case, gpu_run):
-    """Directional derivatives of the float64 loss (central, h = 1e-5) against the HIP gradient."""
+def _direction(rng, g, mask=None):
+    """The unit direction of g plus half a seeded random unit direction: the derivative along it is at
+    least |g| / 2, so float32 rounding of the gradient (held norm-wise to the tolerances above) is not
+    magnified by cancellation, as along a random direction, which among hundreds of components is
+    nearly orthogonal to any given gradient."""
+    r = rng.standard_normal(g.shape)
+    if mask is not None:
+        r[~mask] = 0.0
+        g = np.where(mask[:, None], g, 0.0) if g.ndim == 2 else np.where(mask, g, 0.0)
+    return 0.5 * r / np.linalg.norm(r) + g / np.linalg.norm(g)
+
+
+def test_gradients_match_central_differences(case, gpu_run, model_run):
+    """Directional derivatives of the float64 loss (central, h = 1e-5) against the HIP gradient, to the
+    norm-wise tolerances of the comparison with the model: 1e-3 for velocities and friction, 5e-3 for
+    spins."""
     m = case.model
     rng = np.random.default_rng(11)
-    u = rng.standard_normal(m.v0.shape)
-    u[~m.moving] = 0.0
     h = 1e-5
+    u = _direction(rng, model_run["gv"], m.moving)
     fd = (m.loss(case.target, v0=m.v0 + h * u, steps=case.steps)
           - m.loss(case.target, v0=m.v0 - h * u, steps=case.steps)) / (2 * h)
     an = float(np.sum(gpu_run["gv"].astype(np.float64) * u))
     assert abs(fd - an) <= 1e-3 * abs(fd), (fd, an)
-    uw0 = rng.standard_normal(m.w0.shape)
-    uw0[~m.moving] = 0.0
+    uw0 = _direction(rng, model_run["gw"], m.moving)
     fd = (m.loss(case.target, w0=m.w0 + h * uw0, steps=case.steps)
           - m.loss(case.target, w0=m.w0 - h * uw0, steps=case.steps)) / (2 * h)
     an = float(np.sum(gpu_run["gw"].astype(np.float64) * uw0))
-    assert abs(fd - an) <= 1e-3 * abs(fd), (fd, an)
-    uw = rng.standard_normal(len(case.scenes))
+    assert abs(fd - an) <= 5e-3 * abs(fd), (fd, an)
+    uw = _direction(rng, case.per_world(model_run["gmu"]))
     du = uw[case.world_of_contact]
     fd = (m.loss(case.target, mu=m.mu0 + h * du, steps=case.steps)
           - m.loss(case.target, mu=m.mu0 - h * du, steps=case.steps)) / (2 * h)

@@ -1,7 +1,6 @@
-// narrow.h - the host world's narrowphase for spheres, boxes and half-spaces, one source for the
-// host world (scene.cpp, g++) and the device refresh (solver.hip, hipcc). Both builds keep every
-// multiply and add separate and use IEEE division and square root, and nothing here depends on the
-// compiler beyond that, so a pair collided on either side gives the same manifold bit for bit.
+// narrow.h - the host world's own narrowphase for spheres, boxes and half-spaces (scene.cpp), for the
+// pairs that the reference's narrowphase restated in collide3d.h does not take: those the reference
+// collides in 2D (every dynamic body locked to one plane) and those with a cylinder or a prism.
 //
 // The manifolds follow the reference's conventions (normal from shape A to shape B, shapes ordered
 // by index, points at the midpoint between the two surfaces, a speculative margin grown by the
@@ -82,6 +81,7 @@ struct Manifold {
     V n;
     int count = 0;
     Point p[4];
+    bool reference = false;  // made by the reference's narrowphase (collide3d.h), not by this file
 };
 
 struct Pose {

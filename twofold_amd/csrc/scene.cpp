@@ -7,13 +7,16 @@
 //
 // It stands in for the reference's broadphase and narrowphase (src/world.cpp:752-993,
 // src/collide3d.cpp), which sit outside the solve path, so that the solver can be run on the GPU
-// box on the piles it is measured on. Its manifolds follow the reference's conventions - normal
-// from shape A to shape B, shapes ordered by index, points at the midpoint between the two
-// surfaces, a speculative margin grown by the pair's travel (src/world.cpp:484-486, 509-511),
-// anchors lifted onto each surface for separated points (src/world.cpp:536-540), impulses carried
-// to the nearest new point (src/world.cpp:102-120), friction and restitution mixed as
-// src/world.cpp:39-40 - but not its bits: box-box uses separating axes and face clipping of its own,
-// and pairs of z-extruded shapes a 2D version of the same.
+// box on the piles it is measured on. Pairs of spheres, boxes and half-spaces go through the
+// reference's own narrowphase, restated in collide3d.h (the source the device refresh runs too), and
+// their contacts take the reference's geometry (src/world.cpp:927-943, 1111-1121): they are the
+// reference's manifolds bit for bit. The other pairs (cylinders, prisms, and pairs the reference
+// collides in 2D) follow its conventions - normal from shape A to shape B, shapes ordered by index,
+// points at the midpoint between the two surfaces, a speculative margin grown by the pair's travel
+// (src/world.cpp:484-486, 509-511), anchors lifted onto each surface for separated points
+// (src/world.cpp:536-540) - but not its bits: narrow.h, and a 2D version of it for z-extruded shapes.
+// Impulses carry to the nearest new point (src/world.cpp:102-120), and friction and restitution mix as
+// src/world.cpp:39-40.
 #include <omp.h>
 
 #include <algorithm>
@@ -25,6 +28,7 @@
 #include <vector>
 
 #include "../../include/twofold_solver.h"
+#include "collide3d.h"
 #include "narrow.h"
 
 namespace {
@@ -421,11 +425,31 @@ float travelOf(const tfs_body& b, float dt) {
     return (len(ld(b.v)) + len(ld(b.w)) * b.extent) * dt;
 }
 
-// One manifold for the pair (a < b), or none.
+bool basicKind(int kind) { return kind == kSphere || kind == kBox || kind == kHalfSpace; }
+tfc::Solid solidOf(const Shape& s) { return tfc::Solid{ s.kind, tf::vec(s.half.x, s.half.y, s.half.z), s.radius }; }
+
+// One manifold for the pair (a < b), or none. Spheres, boxes and half-spaces go through the reference's
+// narrowphase (collide3d.h, the source the device refresh runs), except pairs the reference collides in
+// 2D, which take the host world's own.
 bool collide(const World& w, uint32_t ia, const tfs_body& a, uint32_t ib, const tfs_body& b, float dt, Manifold& m) {
-    const float margin = pairMargin(ld(a.v), ld(a.w), a.extent, ld(b.v), ld(b.w), b.extent, dt, w.settings.speculative);
     const Shape& sa = w.shapes[ia];
     const Shape& sb = w.shapes[ib];
+    if (basicKind(sa.kind) && basicKind(sb.kind)) {
+        tfc::Manifold r;
+        const int got = tfc::pairManifold(a, tf::ld3(a.v), tf::ld3(a.w), solidOf(sa), b, tf::ld3(b.v), tf::ld3(b.w),
+                                          solidOf(sb), dt, w.settings.speculative, 0.005f * w.settings.length_unit, r);
+        if (got >= 0) {
+            m.reference = true;
+            m.n = vec3(r.normal.x, r.normal.y, r.normal.z);
+            m.count = got > 0 ? r.count : 0;
+            for (int i = 0; i < m.count; ++i) {
+                m.p[i].pos = vec3(r.points[i].position.x, r.points[i].position.y, r.points[i].position.z);
+                m.p[i].sep = r.points[i].separation;
+            }
+            return got > 0;
+        }
+    }
+    const float margin = pairMargin(ld(a.v), ld(a.w), a.extent, ld(b.v), ld(b.w), b.extent, dt, w.settings.speculative);
     const Pose pa = poseOf(a), pb = poseOf(b);
     const bool bothInPlane = inPlane(a) && inPlane(b);
     // spheres, boxes and half-spaces as the device refresh collides them (narrow.h)
@@ -474,7 +498,17 @@ void toContact(const World& w, uint32_t ia, const tfs_body& a, uint32_t ib, cons
     c.friction = std::sqrt(sa.friction * sb.friction);
     c.restitution = std::max(sa.restitution, sb.restitution);
     c.rolling_resistance = std::max(sa.rolling, sb.rolling);
-    contactGeometry(a, b, m, w.settings.speculative, c);
+    if (!m.reference) {
+        contactGeometry(a, b, m, w.settings.speculative, c);
+        return;
+    }
+    tf::V3 pos[4];
+    float sep[4];
+    for (int i = 0; i < 4; ++i) {
+        pos[i] = tf::vec(m.p[i].pos.x, m.p[i].pos.y, m.p[i].pos.z);
+        sep[i] = m.p[i].sep;
+    }
+    tfc::fillContact(a, b, tf::vec(m.n.x, m.n.y, m.n.z), m.count, pos, sep, w.settings.speculative, c);
 }
 
 // carryImpulses: src/world.cpp:102-120.
@@ -654,6 +688,32 @@ void massProperties(tfs_body& b, const Shape& s) {
 }  // namespace
 
 extern "C" {
+
+// The restated reference narrowphase (collide3d.h, host build) on one pair, in tfr_collide's layout:
+// kind 0 sphere (dims[0] = radius), 1 box (half-extents), 4 half-space; pa, pb = p[3], q[4];
+// motion = v, w, centre of A then of B. out = normal[3], count, per point position[3], separation,
+// predicted. Returns 0, or -1 for another kind.
+int tfw_collide3d(int ka, const float* da, const float* pa, int kb, const float* db, const float* pb, float margin,
+                  float linearSlop, const float* motion, float dt, int fast, float* out) {
+    const tfc::Solid a{ ka, tf::vec(da[0], ka == tfc::kBox ? da[1] : 0.0f, ka == tfc::kBox ? da[2] : 0.0f),
+                        ka == tfc::kSphere ? da[0] : 0.0f };
+    const tfc::Solid b{ kb, tf::vec(db[0], kb == tfc::kBox ? db[1] : 0.0f, kb == tfc::kBox ? db[2] : 0.0f),
+                        kb == tfc::kSphere ? db[0] : 0.0f };
+    const tfc::Xf xa{ tf::ld3(pa), tf::ld4(pa + 3) }, xb{ tf::ld3(pb), tf::ld4(pb + 3) };
+    const tfc::Motion mo{ tf::ld3(motion), tf::ld3(motion + 3), tf::ld3(motion + 6), tf::ld3(motion + 9),
+                          tf::ld3(motion + 12), tf::ld3(motion + 15), dt, fast != 0 };
+    tfc::Manifold m;
+    if (!tfc::collide(a, xa, b, xb, margin, linearSlop, mo, m)) { return -1; }
+    tf::st3(out, m.normal);
+    out[3] = (float)m.count;
+    for (int i = 0; i < 4; ++i) {
+        const tfc::ManifoldPoint p = i < m.count ? m.points[i] : tfc::ManifoldPoint{ tf::vec(0, 0, 0), 0.0f, 0.0f };
+        tf::st3(out + 4 + 5 * i, p.position);
+        out[4 + 5 * i + 3] = p.separation;
+        out[4 + 5 * i + 4] = p.predicted;
+    }
+    return 0;
+}
 
 // Half the OpenMP threads by default (TWOFOLD_HOST_THREADS overrides): a process held to a CPU quota
 // that its spinning workers use up is stopped for the rest of the quota period, which stalls a solve
@@ -875,7 +935,7 @@ uint32_t tfw_pairs(void* h, tfs_body* bodies, uint32_t nb, float dt, tfs_pair* p
             const Manifold& m = found[k];
             tfs_manifold& x = manifolds[nm];
             std::memset(&x, 0, sizeof x);
-            st(x.normal, unit(m.n));
+            st(x.normal, m.reference ? m.n : unit(m.n));
             x.count = m.count;
             for (int i = 0; i < m.count && i < 4; ++i) {
                 st(x.position[i], m.p[i].pos);

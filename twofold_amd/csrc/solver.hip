@@ -30,7 +30,7 @@
 #include <vector>
 
 #include "stages.h"
-#include "narrow.h"
+#include "collide3d.h"
 
 using namespace tf;
 
@@ -2564,13 +2564,15 @@ __global__ void k_gather_again(const uint32_t* again, uint32_t n, const tfs_cont
     }
 }
 
-// The narrowphase half of a refresh on the device: each listed pair collided again at its bodies'
-// current poses and velocities, with the host world's narrowphase (narrow.h), into fresh[k] as the
-// refresh callback would write it. A pair narrow.h does not collide raises error bit 16.
+// The narrowphase half of a refresh on the device (src/world.cpp:1076-1122 for one shape per body): each
+// listed pair collided again at its bodies' current poses and velocities over the time left in the step
+// with the reference's narrowphase (collide3d.h: spheres, boxes, half-spaces), into fresh[k] as the
+// refresh callback would write it. A pair collide3d.h does not collide (another kind of shape, or a pair
+// the reference collides in 2D) raises error bit 16.
 __global__ void __launch_bounds__(kThreads) k_refresh_collide(const uint32_t* again, uint32_t n, const tfs_contact* contacts,
                                                               const tfs_body* bodies, const float4* motion,
                                                               const tfs_shape* shapes, float remaining, float speculative,
-                                                              tfs_contact* fresh, DevCounts* cnt) {
+                                                              float linearSlop, tfs_contact* fresh, DevCounts* cnt) {
     for (uint32_t k = gid(); k < n; k += gstride()) {
         const tfs_contact& c = contacts[again[k]];
         const uint32_t ia = c.body_a, ib = c.body_b;
@@ -2578,20 +2580,23 @@ __global__ void __launch_bounds__(kThreads) k_refresh_collide(const uint32_t* ag
         const tfs_body& b = bodies[ib];
         const tfs_shape sa = shapes[ia], sb = shapes[ib];
         const float4 va = motion[4 * ia], wa = motion[4 * ia + 1], vb = motion[4 * ib], wb = motion[4 * ib + 1];
-        const float margin = tfn::pairMargin(tfn::vec3(va.x, va.y, va.z), tfn::vec3(wa.x, wa.y, wa.z), a.extent,
-                                             tfn::vec3(vb.x, vb.y, vb.z), tfn::vec3(wb.x, wb.y, wb.z), b.extent, remaining,
-                                             speculative);
-        const tfn::Pose pa = tfn::poseOf(a.com, a.q), pb = tfn::poseOf(b.com, b.q);
-        const bool bothInPlane = tfn::inPlane(a) && tfn::inPlane(b);
-        tfn::Manifold m;
-        const int got = tfn::collideBasic((int)sa.kind, tfn::ld(sa.half), sa.radius, pa, (int)sb.kind, tfn::ld(sb.half),
-                                          sb.radius, pb, bothInPlane, margin, m);
+        const tfc::Solid solidA{ (int)sa.kind, ld3(sa.half), sa.radius }, solidB{ (int)sb.kind, ld3(sb.half), sb.radius };
+        tfc::Manifold m;
+        const int got = tfc::pairManifold(a, vec(va.x, va.y, va.z), vec(wa.x, wa.y, wa.z), solidA, b, vec(vb.x, vb.y, vb.z),
+                                          vec(wb.x, wb.y, wb.z), solidB, remaining, speculative, linearSlop, m);
         tfs_contact& f = fresh[k];
         f.count = 0;
         if (got < 0) {
             atomicOr(&cnt->error, 16u);
         } else if (got > 0) {
-            tfn::contactGeometry(a, b, m, speculative, f);
+            V3 pos[4];
+            float sep[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                pos[i] = m.points[i].position;
+                sep[i] = m.points[i].separation;
+            }
+            tfc::fillContact(a, b, m.normal, m.count, pos, sep, speculative, f);
         }
     }
 }
@@ -3533,7 +3538,7 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
             HIP_TRY(hipEventRecord(ctx->evPool[ev], ctx->stream));
         }
         LAUNCH(k_refresh_collide, blocksFor(nAgain), kThreads, again, nAgain, ctx->contacts, ctx->bodies, ctx->motion,
-               ctx->shapes, remaining, ctx->settings.speculative, ctx->fresh, ctx->cnt);
+               ctx->shapes, remaining, ctx->settings.speculative, 0.005f * ctx->settings.length_unit, ctx->fresh, ctx->cnt);
         if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
         return refreshApply(ctx, again, nAgain, laned, impactLaned, nRest);
     }

@@ -403,8 +403,8 @@ class Solver:
 
     def set_body_shapes(self, shapes):
         """One shape per resident body (SHAPE_DTYPE): every refresh then collides its pairs on the
-        device, with the host world's sphere/box/half-space narrowphase, and no callback is called.
-        None or an empty array turns it off."""
+        device with the reference's sphere/box/half-space narrowphase (collide3d.h), and no callback is
+        called. None or an empty array turns it off."""
         if shapes is None or len(shapes) == 0:
             self._check(self.lib.tfs_set_body_shapes(self.h, 0, None), "tfs_set_body_shapes")
             return
