@@ -308,6 +308,18 @@ int tfs_set_worlds(tfs_ctx* ctx, uint32_t n_worlds, const uint32_t* body_start);
 int tfs_step(tfs_ctx* ctx, float dt, tfs_refresh_fn refresh, void* user);
 
 /*
+ * The sleep pass apart from the solve block. tfs_step runs updateSleep (src/world.cpp:1174-1247) right
+ * after the solve block by default. The reference runs it later in World::step, after the continuous
+ * collision pass, the sensors and the kinematic targets (src/world.cpp:1696-1725), which can move and
+ * slow bodies; a binding that keeps that order turns the step's sleep pass off (on = 0) and calls
+ * tfs_sleep where the reference calls updateSleep (1724), after uploading the bodies those passes
+ * changed. tfs_sleep is the same pass over the resident bodies and contacts, with the same results as
+ * inside tfs_step; dt is the step's time.
+ */
+int tfs_set_step_sleep(tfs_ctx* ctx, int on);
+int tfs_sleep(tfs_ctx* ctx, float dt);
+
+/*
  * The device refresh (the reference's refresh collides on the host: the narrowphase half of
  * refreshContacts, src/world.cpp:1076-1122). With one shape per resident body set here, at the body's
  * origin, every refresh of tfs_step collides its pairs on the device with the reference's own

@@ -31,6 +31,29 @@
 #include "twofold/world.h"
 #undef private
 
+#ifdef TFB_BOUND
+// The bound build (oracle/ref_bound.cpp): world.cpp's Clock (src/world.cpp:33) becomes this clock,
+// whose now() also moves the binding through the step (the end of the solve block, the sleep pass),
+// and the rest of the solve block's free functions are hooked too.
+namespace std {
+namespace chrono {
+struct twofold_step_clock {
+    using duration = steady_clock::duration;
+    using rep = steady_clock::rep;
+    using period = steady_clock::period;
+    using time_point = steady_clock::time_point;
+    static constexpr bool is_steady = true;
+    static time_point now();
+};
+}  // namespace chrono
+}  // namespace std
+#define steady_clock twofold_step_clock
+#define warmStart tfcap_warmStart
+#define solveBatches tfcap_solveBatches
+#define restituteBatches tfcap_restituteBatches
+#define makeContactAxes tfcap_makeContactAxes
+#endif
+
 #define prepareContacts tfcap_prepareContacts
 #define colorContacts tfcap_colorContacts
 #define batchContacts tfcap_batchContacts
@@ -48,6 +71,13 @@
 #undef warmStartBatches
 #undef applyRestitution
 #undef limitSpeed
+#ifdef TFB_BOUND
+#undef steady_clock
+#undef warmStart
+#undef solveBatches
+#undef restituteBatches
+#undef makeContactAxes
+#endif
 
 #include "../include/twofold_solver.h"
 
@@ -63,7 +93,22 @@ void prepareContact(const Body& a, const Body& b, Contact& c, PlanarAxes* planar
 void warmStartBatches(SolverData& s, const ContactBatches& batches);
 void applyRestitution(SolverData& s, const std::vector<uint32_t>& active, const SolverContext& ctx);
 void limitSpeed(Body& b, float maxSpeed, float maxSpin);
+#ifdef TFB_BOUND
+void warmStart(SolverData& s, const std::vector<uint32_t>& active);
+void solveBatches(SolverData& s, const ContactBatches& batches, const SolverContext& ctx, bool useBias);
+void restituteBatches(SolverData& s, const ContactBatches& batches, const SolverContext& ctx);
+void makeContactAxes(const Body& a, const Body& b, const Contact& c, PointAxes* axes);
+#endif
 }
+
+#ifdef TFB_BOUND
+// The binding's part of the step (oracle/ref_bound.cpp): the solve block on the GPU.
+bool tfbBound();
+void tfbSolveBlock();
+#else
+bool tfbBound() { return false; }
+void tfbSolveBlock() {}
+#endif
 
 using namespace twofold;
 
@@ -217,6 +262,10 @@ float g_dt = 0.0f;
 namespace twofold {
 
 void tfcap_prepareContacts(SolverData& s, const std::vector<uint32_t>& active) {
+    if (tfbBound()) {
+        tfbSolveBlock();
+        return;
+    }
     if (g_cap) {
         Capture& c = *g_cap;
         c.solved = true;
@@ -273,12 +322,14 @@ void tfcap_prepareContacts(SolverData& s, const std::vector<uint32_t>& active) {
 }
 
 void tfcap_colorContacts(SolverData& s, const std::vector<uint32_t>& list, ContactColors& colors) {
+    if (tfbBound()) { return; }
     colorContacts(s, list, colors);
     if (g_cap) { g_cap->colors = colors.of; }
 }
 
 void tfcap_batchContacts(SolverData& s, const std::vector<uint32_t>& list, const ContactColors& colors,
                          const SolverContext& ctx, ContactBatches& out) {
+    if (tfbBound()) { return; }
     batchContacts(s, list, colors, ctx, out);
     if (g_counts) {
         uint32_t points = 0;
@@ -307,6 +358,7 @@ void tfcap_batchContacts(SolverData& s, const std::vector<uint32_t>& list, const
 }
 
 void tfcap_warmStartBatches(SolverData& s, const ContactBatches& batches) {
+    if (tfbBound()) { return; }
     warmStartBatches(s, batches);
     ++g_substep;
 }
@@ -326,6 +378,7 @@ void tfcap_prepareContact(const Body& a, const Body& b, Contact& c, PlanarAxes* 
 }
 
 void tfcap_prepareLanes(SolverData& s, const SolverContext& ctx, ContactBatches& batches) {
+    if (tfbBound()) { return; }
     prepareLanes(s, ctx, batches);
     if (!g_cap) { return; }
     if (g_lanesCalls % 2 == 0) {
@@ -356,6 +409,7 @@ void tfcap_prepareLanes(SolverData& s, const SolverContext& ctx, ContactBatches&
 }
 
 void tfcap_applyRestitution(SolverData& s, const std::vector<uint32_t>& active, const SolverContext& ctx) {
+    if (tfbBound()) { return; }
     applyRestitution(s, active, ctx);
     g_final = true;
     g_limitCount = 0;
@@ -373,6 +427,22 @@ void tfcap_limitSpeed(Body& b, float maxSpeed, float maxSpin) {
     c.postContacts.clear();
     for (const Contact& x : g_impl->contacts) { c.postContacts.push_back(toContact(x)); }
 }
+
+#ifdef TFB_BOUND
+// The rest of the solve block in the bound build: done on the GPU by tfbSolveBlock.
+void tfcap_warmStart(SolverData& s, const std::vector<uint32_t>& active) {
+    if (!tfbBound()) { warmStart(s, active); }
+}
+void tfcap_solveBatches(SolverData& s, const ContactBatches& batches, const SolverContext& ctx, bool useBias) {
+    if (!tfbBound()) { solveBatches(s, batches, ctx, useBias); }
+}
+void tfcap_restituteBatches(SolverData& s, const ContactBatches& batches, const SolverContext& ctx) {
+    if (!tfbBound()) { restituteBatches(s, batches, ctx); }
+}
+void tfcap_makeContactAxes(const Body& a, const Body& b, const Contact& c, PointAxes* axes) {
+    if (!tfbBound()) { makeContactAxes(a, b, c, axes); }
+}
+#endif
 
 }
 

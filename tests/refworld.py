@@ -14,6 +14,9 @@ from twofold_amd._abi import BODY_DTYPE, CONTACT_DTYPE
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libtwofold_ref.so")
+# the same engine with the drop-in binding compiled in (oracle/ref_bound.cpp): solve block and sleep pass
+# on the GPU through the C ABI
+BOUND_SO = os.path.join(ROOT, "oracle", "_ref", "libtwofold_bound.so")
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 
 _fp = ctypes.POINTER(ctypes.c_float)
@@ -28,17 +31,16 @@ def _f(v):
     return np.ascontiguousarray(v, np.float32).ctypes.data_as(_fp)
 
 
-_ref = None
+_refs = {}
 
 
 def ref_available():
     return os.path.exists(REF_SO)
 
 
-def _load_ref():
-    global _ref
-    if _ref is None:
-        lib = ctypes.CDLL(REF_SO)
+def _load_ref(path=REF_SO):
+    if path not in _refs:
+        lib = ctypes.CDLL(path)
         lib.tfr_create.restype = _vp
         lib.tfr_create.argtypes = [_vp]
         lib.tfr_destroy.argtypes = [_vp]
@@ -81,8 +83,10 @@ def _load_ref():
         lib.tfr_cap_refresh.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int), _fp, _vp, _vp,
                                         ctypes.c_uint32]
         _abi.check_sizes(lib, "tfr")
-        _ref = lib
-    return _ref
+        if path == BOUND_SO:
+            lib.tfb_error.restype = ctypes.c_char_p
+        _refs[path] = lib
+    return _refs[path]
 
 
 class StepRecord:
@@ -96,8 +100,8 @@ class StepRecord:
 class RefWorld:
     """The reference engine, one world, built from a scene recipe."""
 
-    def __init__(self, scene, settings=None, continuous=True):
-        self.lib = _load_ref()
+    def __init__(self, scene, settings=None, continuous=True, lib_path=REF_SO):
+        self.lib = _load_ref(lib_path)
         s = settings if settings is not None else _abi.default_settings(substeps=scene.substeps,
                                                                         **getattr(scene, "settings", {}))
         self.settings = s

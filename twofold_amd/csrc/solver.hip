@@ -2828,6 +2828,8 @@ struct tfs_ctx {
     // hook-and-jump rounds of the island pass launched before the host first checks (1..kIslandAhead;
     // TWOFOLD_ISLAND_AHEAD lowers it, so that tests reach the host-driven rounds)
     int islandAhead = kIslandAhead;
+    // tfs_step runs the sleep pass after the solve block (off: the caller runs it, tfs_sleep)
+    bool stepSleep = true;
     // Measurement: HIP events around each row launch, summed per family (0 warm start, 1 solve,
     // 2 restitution), with the algorithmic bytes of the rows each launch covers.
     bool profiling = false;
@@ -3714,6 +3716,73 @@ void tuneRecord(tfs_ctx* ctx, float ms) {
     ctx->tuning = false;
 }
 
+// updateSleep (src/world.cpp:1174-1247) over the resident bodies and contacts: sleep timers and the
+// kinematic wake, then hook-and-jump rounds until one links nothing - four rounds, then two that run
+// only if the round before them linked something - then the island times and the sleep, which run only
+// if the last round linked nothing: no host check in between. `it` receives the rounds launched; the
+// last round's flag is copied to the host for finishSleep.
+int launchSleep(tfs_ctx* ctx, float dt, int& it) {
+    const uint32_t nb = ctx->nb, nc = ctx->nc;
+    const tfs_settings& s = ctx->settings;
+    it = 0;
+    LAUNCH(k_sleep_time, blocksFor(nb), kThreads, ctx->bodies, nb, dt, s.sleep_threshold * s.length_unit, ctx->parent,
+           ctx->islandTime);
+    LAUNCH(k_sleep_kinematic, blocksFor(nc), kThreads, ctx->contacts, nc, ctx->bodies, ctx->edges);
+    const bool worlds = ctx->nWorlds > 1;
+    const uint32_t* bw = worlds ? ctx->bodyWorld : nullptr;
+    const uint32_t* ws = worlds ? ctx->worldStart : nullptr;
+    const int ahead = ctx->islandAhead;
+    for (; it < ahead; ++it) {
+        const uint32_t* prev = it >= 4 ? &ctx->cnt->changed[it - 1] : nullptr;
+        LAUNCH(k_sleep_hook, blocksFor(nc), kThreads, ctx->edges, nc, ctx->parent, &ctx->cnt->changed[it], prev);
+        LAUNCH(k_sleep_jump, blocksFor(nb), kThreads, ctx->parent, nb, prev);
+    }
+    const uint32_t* last = &ctx->cnt->changed[ahead - 1];
+    LAUNCH(k_island_time, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime, bw, ws, last);
+    LAUNCH(k_sleep_apply, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime, s.time_to_sleep,
+           ctx->motion, ctx->mass, last);
+    HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->changed[ahead - 1], last, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    return 0;
+}
+
+// The rare step whose islands need more rounds than launchSleep ran (seen once its flag reached the
+// host): two rounds at a time, checking each pair, then the island times and the sleep.
+int finishSleep(tfs_ctx* ctx, int it, bool timed) {
+    const uint32_t nb = ctx->nb, nc = ctx->nc;
+    const tfs_settings& s = ctx->settings;
+    const bool worlds = ctx->nWorlds > 1;
+    const uint32_t* bw = worlds ? ctx->bodyWorld : nullptr;
+    const uint32_t* ws = worlds ? ctx->worldStart : nullptr;
+    const int ahead = ctx->islandAhead;
+    if (ctx->hostCnt->changed[ahead - 1]) {
+        for (;;) {
+            if (it + 2 > kIslandRounds) {
+                return fail(ctx, TFS_E_KERNEL, "the island pass did not settle in %d rounds", kIslandRounds);
+            }
+            for (int j = 0; j < 2; ++j, ++it) {
+                LAUNCH(k_sleep_hook, blocksFor(nc), kThreads, ctx->edges, nc, ctx->parent, &ctx->cnt->changed[it], nullptr);
+                LAUNCH(k_sleep_jump, blocksFor(nb), kThreads, ctx->parent, nb, nullptr);
+            }
+            HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->changed[it - 1], &ctx->cnt->changed[it - 1], sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            if (!ctx->hostCnt->changed[it - 1]) { break; }
+        }
+        LAUNCH(k_island_time, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime, bw, ws, nullptr);
+        LAUNCH(k_sleep_apply, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime, s.time_to_sleep,
+               ctx->motion, ctx->mass, nullptr);
+        HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        if (timed) {
+            HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
+            HIP_TRY(hipEventSynchronize(ctx->ev[2]));
+        } else {
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+        }
+    }
+    return 0;
+}
+
 int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     if (ctx->nShapes && ctx->nShapes != ctx->nb) {
         return fail(ctx, TFS_E_STATE, "%u body shapes for %u bodies", ctx->nShapes, ctx->nb);
@@ -3867,30 +3936,9 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
         LAUNCH(k_touch_events, blocksFor(nc), kThreads, ctx->touchList, &ctx->cnt->nTouchEvents, ctx->contacts);
     }
 
-    // sleep and islands
-    LAUNCH(k_sleep_time, blocksFor(nb), kThreads, ctx->bodies, nb, dt, s.sleep_threshold * s.length_unit, ctx->parent,
-           ctx->islandTime);
-    LAUNCH(k_sleep_kinematic, blocksFor(nc), kThreads, ctx->contacts, nc, ctx->bodies, ctx->edges);
-    // Hook-and-jump rounds until one links nothing. Four rounds, then two that run only if the round
-    // before them linked something, then the island times and the sleep, which run only if the last
-    // round linked nothing: no host check in between. In the rare step whose islands need more rounds,
-    // the host sees that at the step's end and goes on two rounds at a time, checking each pair.
-    const bool worlds = ctx->nWorlds > 1;
-    const uint32_t* bw = worlds ? ctx->bodyWorld : nullptr;
-    const uint32_t* ws = worlds ? ctx->worldStart : nullptr;
+    // sleep and islands (unless the caller runs them later, tfs_set_step_sleep)
     int it = 0;
-    const int ahead = ctx->islandAhead;
-    for (; it < ahead; ++it) {
-        const uint32_t* prev = it >= 4 ? &ctx->cnt->changed[it - 1] : nullptr;
-        LAUNCH(k_sleep_hook, blocksFor(nc), kThreads, ctx->edges, nc, ctx->parent, &ctx->cnt->changed[it], prev);
-        LAUNCH(k_sleep_jump, blocksFor(nb), kThreads, ctx->parent, nb, prev);
-    }
-    const uint32_t* last = &ctx->cnt->changed[ahead - 1];
-    LAUNCH(k_island_time, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime, bw, ws, last);
-    LAUNCH(k_sleep_apply, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime, s.time_to_sleep,
-           ctx->motion, ctx->mass, last);
-    HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->changed[ahead - 1], last, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                           ctx->stream));
+    if (ctx->stepSleep && (r = launchSleep(ctx, dt, it))) { return r; }
     HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
     static_assert(offsetof(DevCounts, nHits) == offsetof(DevCounts, nTouchEvents) + sizeof(uint32_t) &&
                       offsetof(DevCounts, touching) == offsetof(DevCounts, nTouchEvents) + 2 * sizeof(uint32_t),
@@ -3906,31 +3954,7 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     ctx->touchCounts[0] = ctx->hostCnt->nTouchEvents;
     ctx->touchCounts[1] = ctx->hostCnt->nHits;
     ctx->touchCounts[2] = ctx->hostCnt->touching;
-    if (ctx->hostCnt->changed[ahead - 1]) {
-        for (;;) {
-            if (it + 2 > kIslandRounds) {
-                return fail(ctx, TFS_E_KERNEL, "the island pass did not settle in %d rounds", kIslandRounds);
-            }
-            for (int j = 0; j < 2; ++j, ++it) {
-                LAUNCH(k_sleep_hook, blocksFor(nc), kThreads, ctx->edges, nc, ctx->parent, &ctx->cnt->changed[it], nullptr);
-                LAUNCH(k_sleep_jump, blocksFor(nb), kThreads, ctx->parent, nb, nullptr);
-            }
-            HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->changed[it - 1], &ctx->cnt->changed[it - 1], sizeof(uint32_t),
-                                   hipMemcpyDeviceToHost, ctx->stream));
-            HIP_TRY(hipStreamSynchronize(ctx->stream));
-            if (!ctx->hostCnt->changed[it - 1]) { break; }
-        }
-        LAUNCH(k_island_time, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime, bw, ws, nullptr);
-        LAUNCH(k_sleep_apply, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->parent, ctx->islandTime, s.time_to_sleep,
-               ctx->motion, ctx->mass, nullptr);
-        HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-        if (timed) {
-            HIP_TRY(hipEventRecord(ctx->ev[2], ctx->stream));
-            HIP_TRY(hipEventSynchronize(ctx->ev[2]));
-        } else {
-            HIP_TRY(hipStreamSynchronize(ctx->stream));
-        }
-    }
+    if (ctx->stepSleep && (r = finishSleep(ctx, it, timed))) { return r; }
     if (ctx->hostCnt->error) { return kernelFail(ctx, ctx->hostCnt->error); }
     if (timed) {
         HIP_TRY(hipEventElapsedTime(&ctx->solveMs, ctx->ev[0], ctx->ev[1]));
@@ -4569,6 +4593,27 @@ int tfs_step(tfs_ctx* ctx, float dt, tfs_refresh_fn refresh, void* user) {
     if (hipSetDevice(ctx->device) != hipSuccess) { return fail(ctx, TFS_E_DEVICE, "hipSetDevice failed"); }
     ctx->fixedOn = false;
     return step(ctx, dt, refresh, user);
+}
+
+int tfs_set_step_sleep(tfs_ctx* ctx, int on) {
+    if (!ctx) { return TFS_E_ARG; }
+    ctx->stepSleep = on != 0;
+    return 0;
+}
+
+int tfs_sleep(tfs_ctx* ctx, float dt) {
+    if (!ctx) { return TFS_E_ARG; }
+    if (!ctx->haveBodies || !ctx->haveContacts) { return fail(ctx, TFS_E_STATE, "upload bodies and contacts first"); }
+    if (!(dt > 0.0f) || !(dt <= 3.4e38f)) { return 0; }
+    if (hipSetDevice(ctx->device) != hipSuccess) { return fail(ctx, TFS_E_DEVICE, "hipSetDevice failed"); }
+    HIP_TRY(hipMemsetAsync(ctx->cnt, 0, sizeof(DevCounts), ctx->stream));
+    int it = 0, r;
+    if ((r = launchSleep(ctx, dt, it))) { return r; }
+    HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->error, &ctx->cnt->error, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if ((r = finishSleep(ctx, it, false))) { return r; }
+    if (ctx->hostCnt->error) { return kernelFail(ctx, ctx->hostCnt->error); }
+    return 0;
 }
 
 int tfs_step_noalloc(int device, tfs_body* bodies, uint32_t n_bodies, tfs_contact* contacts, uint32_t n_contacts,

@@ -31,7 +31,7 @@ EXPORTS = ("tfs_abi_version", "tfs_sizeof_body", "tfs_sizeof_contact", "tfs_size
            "tfs_fixed_step", "tfs_adjoint_seed", "tfs_adjoint_step", "tfs_adjoint_read", "tfs_check_minmax",
            "tfs_autotune", "tfs_pass_shape", "tfs_set_pass_shape", "tfs_set_cooperative", "tfs_set_body_shapes",
            "tfs_step_noalloc", "tfs_download_touches", "tfs_set_graph", "tfs_stream_contacts", "tfs_upload_contact_ext",
-           "tfs_download_contact_ext", "tfs_download_poses")
+           "tfs_download_contact_ext", "tfs_download_poses", "tfs_set_step_sleep", "tfs_sleep")
 
 
 class SolverError(RuntimeError):
@@ -83,6 +83,8 @@ def load_library(path=LIB_PATH):
     lib.tfs_download_poses.argtypes = [_vp, _vp, ctypes.c_uint32]
     lib.tfs_download_contact_ext.argtypes = [_vp, _vp, ctypes.c_uint32]
     lib.tfs_set_cooperative.argtypes = [_vp, ctypes.c_int]
+    lib.tfs_set_step_sleep.argtypes = [_vp, ctypes.c_int]
+    lib.tfs_sleep.argtypes = [_vp, ctypes.c_float]
     lib.tfs_set_body_shapes.argtypes = [_vp, ctypes.c_uint32, _vp]
     lib.tfs_download_touches.argtypes = [_vp, _vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), _vp,
                                          ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
@@ -400,6 +402,14 @@ class Solver:
     def set_cooperative(self, on):
         """Launch the grid-barrier passes cooperatively (True) or as plain kernels (False, the default)."""
         self._check(self.lib.tfs_set_cooperative(self.h, int(bool(on))), "tfs_set_cooperative")
+
+    def set_step_sleep(self, on):
+        """Whether step() runs the sleep pass after the solve block (True, the default)."""
+        self._check(self.lib.tfs_set_step_sleep(self.h, int(bool(on))), "tfs_set_step_sleep")
+
+    def sleep(self, dt):
+        """The sleep pass alone (updateSleep) over the resident bodies and contacts."""
+        self._check(self.lib.tfs_sleep(self.h, np.float32(dt)), "tfs_sleep")
 
     def set_body_shapes(self, shapes):
         """One shape per resident body (SHAPE_DTYPE): every refresh then collides its pairs on the
