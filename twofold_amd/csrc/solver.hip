@@ -2119,6 +2119,8 @@ struct RestPack {
     float4* p;
     float* peak;
     float4* ax;
+    float4* pre;     // a planar contact's PlanarRowPre for the current pass: [q * cap + k], q = scale, bias,
+                     // impulse scale, one float4 lane per point
     uint32_t* pos;
     uint32_t cap;
 };
@@ -2403,6 +2405,11 @@ __global__ void __launch_bounds__(ONE_BLOCK ? kRestBlockThreads : kPassThreads)
             const Mass mb = c.planar ? loadPlanarMass(mass, ib) : loadMass(mass, ib);
             if (MODE == kRestitute) {
                 restituteRest(c, va, vb, ma, mb, a, b, ctx, ax);
+            } else if (c.planar) {
+                const size_t cap = rp.cap;
+                const float4 q0 = rp.pre[k], q1 = rp.pre[cap + k], q2 = rp.pre[2 * cap + k];
+                const PlanarRowPre pre{ { q0.x, q0.y, q0.z, q0.w }, { q1.x, q1.y, q1.z, q1.w }, { q2.x, q2.y, q2.z, q2.w } };
+                solvePlanar<MODE == kBias, true>(c, va, vb, ma, mb, a, b, ctx, ax, &pre);
             } else {
                 solveRest<MODE == kBias>(c, va, vb, ma, mb, a, b, ctx, ax);
             }
@@ -2423,6 +2430,205 @@ __global__ void __launch_bounds__(ONE_BLOCK ? kRestBlockThreads : kPassThreads)
                 gridSync(bar, phase, cnt);
             }
         }
+    }
+}
+
+// The narrow levels of the active rest list in one work-group, with each level's contact data loaded
+// while the level before it solves. A level's planar contacts read only what the pass does not
+// change (directions, friction, rolling terms, their points' impulses and planar axes, the biases and
+// factors of k_rest_pre, the bodies' planar masses) besides their bodies' velocities, so a thread
+// loads its next level's contact into registers (a `WalkRow`) before solving the current one, and a
+// level's dependent chain is its velocities' round trip, the solve and the stores. Contacts that are
+// not planar, and the second and later contacts of a thread in a level wider than the work-group, take
+// the general path (loadRest, solveRest) as k_rest_levels does. Pack positions are list positions.
+constexpr uint32_t kWalkMaxLevels = 6144;  // level starts kept in LDS (24 KiB)
+constexpr int kWalkThreads = 256;          // one wave per SIMD: room for two levels' rows in registers
+
+struct WalkRow {
+    uint4 e;
+    float4 n, t1, plane, roll;
+    float rollRes;
+    float4 imp[4];
+    float peak[4];
+    float4 ax0[4];
+    float2 ax1[4];
+    float4 pre[3];
+    float4 massA, massB;   // the mass records' planar parts (invMass, pim in .z, .w; pii in the next .x)
+    float piiA, piiB;
+};
+
+__device__ __forceinline__ void loadWalkRow(const uint4* entries, const RestPack& rp, const float4* mass, uint32_t k,
+                                            WalkRow& w) {
+    const size_t cap = rp.cap;
+    w.e = entries[k];
+    w.n = rp.h[k];
+    w.t1 = rp.h[cap + k];
+    w.rollRes = rp.h[2 * cap + k].w;
+    w.plane = rp.h[3 * cap + k];
+    w.roll = rp.h[4 * cap + k];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        w.imp[m] = rp.p[(size_t)(m * 3 + 2) * cap + k];
+        w.peak[m] = rp.peak[(size_t)m * cap + k];
+        w.ax0[m] = rp.ax[(size_t)(2 * m) * cap + k];
+        const float4 x1 = rp.ax[(size_t)(2 * m + 1) * cap + k];
+        w.ax1[m] = make_float2(x1.x, x1.y);
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) { w.pre[q] = rp.pre[(size_t)q * cap + k]; }
+    w.massA = mass[6 * w.e.y + 4];
+    w.massB = mass[6 * w.e.z + 4];
+    w.piiA = mass[6 * w.e.y + 5].x;
+    w.piiB = mass[6 * w.e.z + 5].x;
+}
+
+// One rest contact by the general path (k_rest_levels' body), for what the walk does not prefetch.
+template <int MODE>
+__device__ __forceinline__ void restGeneral(const uint4 e, uint32_t k, const RestPack& rp, float4* motion,
+                                            const float4* mass, const Ctx& ctx) {
+    tfs_contact c;
+    PlanarAxes ax[4];
+    loadRest(rp, k, c, ax);
+    const uint32_t ia = e.y, ib = e.z;
+    const V3 none = vec(0.0f, 0.0f, 0.0f);
+    const BodyView va{ (uint8_t)((e.w & 4u) ? TFS_STATIC : TFS_DYNAMIC), 1, 0, none };
+    const BodyView vb{ (uint8_t)((e.w & 8u) ? TFS_STATIC : TFS_DYNAMIC), 1, 0, none };
+    Motion a = loadMotion(motion, ia), b = loadMotion(motion, ib);
+    const Mass ma = c.planar ? loadPlanarMass(mass, ia) : loadMass(mass, ia);
+    const Mass mb = c.planar ? loadPlanarMass(mass, ib) : loadMass(mass, ib);
+    if (c.planar) {
+        const size_t cap = rp.cap;
+        const float4 q0 = rp.pre[k], q1 = rp.pre[cap + k], q2 = rp.pre[2 * cap + k];
+        const PlanarRowPre pre{ { q0.x, q0.y, q0.z, q0.w }, { q1.x, q1.y, q1.z, q1.w }, { q2.x, q2.y, q2.z, q2.w } };
+        solvePlanar<MODE == kBias, true>(c, va, vb, ma, mb, a, b, ctx, ax, &pre);
+    } else {
+        solveRest<MODE == kBias>(c, va, vb, ma, mb, a, b, ctx, ax);
+    }
+    storeRest(c, rp, k);
+    if (e.w & 1u) {
+        motion[4 * ia] = make_float4(a.v.x, a.v.y, a.v.z, 0.0f);
+        motion[4 * ia + 1] = make_float4(a.w.x, a.w.y, a.w.z, 0.0f);
+    }
+    if (e.w & 2u) {
+        motion[4 * ib] = make_float4(b.v.x, b.v.y, b.v.z, 0.0f);
+        motion[4 * ib + 1] = make_float4(b.w.x, b.w.y, b.w.z, 0.0f);
+    }
+}
+
+// One level of the walk for thread g: its contact `cur` (from the level before), the next level's row
+// loaded into `next` meanwhile, then the level's other contacts of this thread by the general path.
+template <int MODE>
+__device__ __forceinline__ void walkLevel(uint32_t lv, uint32_t levels, const uint32_t* starts, uint32_t g,
+                                          const uint4* entries, const RestPack& rp, float4* motion, const float4* mass,
+                                          const Ctx& ctx, const WalkRow& cur, bool have, WalkRow& next, bool& haveNext) {
+    const uint32_t lo = starts[lv], hi = starts[lv + 1];
+    const bool planar = have && (__float_as_uint(cur.roll.w) & 8u);
+    // this level's velocities first, then the next level's row: loads complete in the order they
+    // are issued, so the solve waits only for its own
+    const uint32_t ia = cur.e.y, ib = cur.e.z;
+    float4 v0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), w0 = v0, v1 = v0, w1 = v0;
+    if (planar) {
+        v0 = motion[4 * ia];
+        w0 = motion[4 * ia + 1];
+        v1 = motion[4 * ib];
+        w1 = motion[4 * ib + 1];
+    }
+    haveNext = false;
+    if (lv + 1 < levels) {
+        haveNext = hi + g < starts[lv + 2];
+        if (haveNext) { loadWalkRow(entries, rp, mass, hi + g, next); }
+    }
+    if (have) {
+        const uint32_t k = lo + g;
+        if (planar) {
+            const uint32_t meta = __float_as_uint(cur.roll.w);
+            tfs_contact c;
+            c.count = (int32_t)(meta & 7u);
+            c.planar = 1;
+            c.normal[0] = cur.n.x; c.normal[1] = cur.n.y; c.normal[2] = cur.n.z; c.friction = cur.n.w;
+            c.tangent1[0] = cur.t1.x; c.tangent1[1] = cur.t1.y; c.tangent1[2] = cur.t1.z;
+            c.rolling_resistance = cur.rollRes;
+            c.plane[0] = cur.plane.x; c.plane[1] = cur.plane.y; c.plane[2] = cur.plane.z; c.rolling_radius = cur.plane.w;
+            c.rolling_impulse[0] = cur.roll.x; c.rolling_impulse[1] = cur.roll.y; c.rolling_impulse[2] = cur.roll.z;
+            PlanarAxes ax[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                tfs_point& q = c.points[m];
+                q.normal_impulse = cur.imp[m].x;
+                q.tangent_impulse[0] = cur.imp[m].y; q.tangent_impulse[1] = cur.imp[m].z; q.tangent_impulse[2] = cur.imp[m].w;
+                q.max_normal_impulse = cur.peak[m];
+                ax[m] = PlanarAxes{ cur.ax0[m].x, cur.ax0[m].y, cur.ax0[m].z, cur.ax0[m].w, cur.ax1[m].x, cur.ax1[m].y };
+            }
+            const PlanarRowPre pre{ { cur.pre[0].x, cur.pre[0].y, cur.pre[0].z, cur.pre[0].w },
+                                    { cur.pre[1].x, cur.pre[1].y, cur.pre[1].z, cur.pre[1].w },
+                                    { cur.pre[2].x, cur.pre[2].y, cur.pre[2].z, cur.pre[2].w } };
+            Mass ma{}, mb{};
+            ma.invMass = cur.massA.z; ma.pim = cur.massA.w; ma.pii = cur.piiA;
+            mb.invMass = cur.massB.z; mb.pim = cur.massB.w; mb.pii = cur.piiB;
+            const V3 none = vec(0.0f, 0.0f, 0.0f);
+            const BodyView va{ (uint8_t)((cur.e.w & 4u) ? TFS_STATIC : TFS_DYNAMIC), 1, 0, none };
+            const BodyView vb{ (uint8_t)((cur.e.w & 8u) ? TFS_STATIC : TFS_DYNAMIC), 1, 0, none };
+            const Q4 unit = quat(0.0f, 0.0f, 0.0f, 1.0f);
+            Motion a{ vec(v0.x, v0.y, v0.z), vec(w0.x, w0.y, w0.z), none, unit };
+            Motion b{ vec(v1.x, v1.y, v1.z), vec(w1.x, w1.y, w1.z), none, unit };
+            solvePlanar<MODE == kBias, true>(c, va, vb, ma, mb, a, b, ctx, ax, &pre);
+            storeRest(c, rp, k);
+            if (cur.e.w & 1u) {
+                motion[4 * ia] = make_float4(a.v.x, a.v.y, a.v.z, 0.0f);
+                motion[4 * ia + 1] = make_float4(a.w.x, a.w.y, a.w.z, 0.0f);
+            }
+            if (cur.e.w & 2u) {
+                motion[4 * ib] = make_float4(b.v.x, b.v.y, b.v.z, 0.0f);
+                motion[4 * ib + 1] = make_float4(b.w.x, b.w.y, b.w.z, 0.0f);
+            }
+        } else {
+            restGeneral<MODE>(cur.e, k, rp, motion, mass, ctx);
+        }
+    }
+    for (uint32_t i = lo + g + blockDim.x; i < hi; i += blockDim.x) { restGeneral<MODE>(entries[i], i, rp, motion, mass, ctx); }
+    __syncthreads();
+}
+
+// The levels two at a time, the rows of alternate levels in two register sets (no copy between levels).
+template <int MODE>
+__global__ void __launch_bounds__(kWalkThreads)
+    k_rest_walk(const uint4* entries, const uint32_t* levelStart, uint32_t levels, RestPack rp, float4* motion,
+                const float4* mass, Ctx ctx) {
+    __shared__ uint32_t starts[kWalkMaxLevels + 1];
+    for (uint32_t i = threadIdx.x; i <= levels; i += blockDim.x) { starts[i] = levelStart[i]; }
+    __syncthreads();
+    const uint32_t g = threadIdx.x;
+    WalkRow rowA, rowB;
+    bool haveA = starts[0] + g < starts[1], haveB = false;
+    if (haveA) { loadWalkRow(entries, rp, mass, starts[0] + g, rowA); }
+    for (uint32_t lv = 0; lv < levels; lv += 2) {
+        walkLevel<MODE>(lv, levels, starts, g, entries, rp, motion, mass, ctx, rowA, haveA, rowB, haveB);
+        if (lv + 1 < levels) {
+            walkLevel<MODE>(lv + 1, levels, starts, g, entries, rp, motion, mass, ctx, rowB, haveB, rowA, haveA);
+        }
+    }
+}
+
+// planarPre of every planar contact of the active rest list before a velocity pass walks its levels:
+// the pass does not change dp and dq, so each point's bias and impulse factors leave the levels'
+// dependent chain; the impact rest list, in the same pass, reads the same pack positions.
+template <bool BIAS>
+__global__ void __launch_bounds__(kThreads) k_rest_pre(const uint4* __restrict__ entries, uint32_t n, RestPack rp,
+                                                       const float4* __restrict__ motion, Ctx ctx) {
+    const size_t cap = rp.cap;
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        const uint32_t meta = __float_as_uint(rp.h[4 * cap + k].w);
+        if (!(meta & 8u)) { continue; }
+        const uint4 e = entries[k];
+        tfs_contact c;
+        PlanarAxes ax[4];
+        loadRest(rp, k, c, ax);
+        const Motion a = loadMotion(motion, e.y), b = loadMotion(motion, e.z);
+        PlanarRowPre q;
+        planarPre(c, !(e.w & 4u), !(e.w & 8u), a, b, ctx, ax, BIAS, q);
+        rp.pre[k] = make_float4(q.scale[0], q.scale[1], q.scale[2], q.scale[3]);
+        rp.pre[cap + k] = make_float4(q.bias[0], q.bias[1], q.bias[2], q.bias[3]);
+        rp.pre[2 * cap + k] = make_float4(q.impulseScale[0], q.impulseScale[1], q.impulseScale[2], q.impulseScale[3]);
     }
 }
 
@@ -2889,6 +3095,7 @@ struct tfs_ctx {
     float4* restP = nullptr;
     float* restPeak = nullptr;
     float4* restAx = nullptr;
+    float4* restPre = nullptr;      // per pass: the planar rest contacts' normal-row biases and factors
     uint32_t* restPos = nullptr;
     uint8_t* colors = nullptr;
     uint32_t* ep = nullptr;
@@ -3082,6 +3289,7 @@ int allocContacts(tfs_ctx* ctx, uint32_t cap) {
         (r = grow(ctx, ctx->restLevelStart[0], (size_t)cap + 1)) ||
         (r = grow(ctx, ctx->restLevelStart[1], (size_t)cap + 1)) || (r = grow(ctx, ctx->restH, 5 * (size_t)cap)) ||
         (r = grow(ctx, ctx->restP, 12 * (size_t)cap)) || (r = grow(ctx, ctx->restPeak, 4 * (size_t)cap)) || (r = grow(ctx, ctx->restAx, 8 * (size_t)cap)) ||
+        (r = grow(ctx, ctx->restPre, 3 * (size_t)cap)) ||
         (r = grow(ctx, ctx->restPos, cap)) || (r = grow(ctx, ctx->wrestEp, 2 * (size_t)cap)) ||
         (r = grow(ctx, ctx->touchFlag, cap)) || (r = grow(ctx, ctx->hitFlag, cap)) || (r = grow(ctx, ctx->touchList, cap)) ||
         (r = grow(ctx, ctx->hitList, cap)) || (r = grow(ctx, ctx->hitData, cap)) || (r = grow(ctx, ctx->hits, cap)) ||
@@ -3394,7 +3602,7 @@ int buildRestLevels(tfs_ctx* ctx, int which, const uint32_t* list, uint32_t n) {
 // A rest pass: the list's leading wide levels (more contacts than a rest work-group has threads) in
 // one launch over many blocks with the grid barrier, the levels from the first narrow one on in one work-group.
 RestPack restPack(const tfs_ctx* ctx) {
-    return RestPack{ ctx->restH, ctx->restP, ctx->restPeak, ctx->restAx, ctx->restPos, ctx->capC };
+    return RestPack{ ctx->restH, ctx->restP, ctx->restPeak, ctx->restAx, ctx->restPre, ctx->restPos, ctx->capC };
 }
 
 // The start of a substep: velocities integrated and the warm start of every row, coloured and rest,
@@ -3449,6 +3657,11 @@ int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32
         ctx->pending.push_back({ 5, ev, ctx->restBytes[which] });
         HIP_TRY(hipEventRecord(ctx->evPool[ev], ctx->stream));
     }
+    if (which == 0 && mode == kBias) {
+        LAUNCH(k_rest_pre<true>, blocksFor(n), kThreads, entries, n, rp, ctx->motion, c);
+    } else if (which == 0 && mode == kRelax) {
+        LAUNCH(k_rest_pre<false>, blocksFor(n), kThreads, entries, n, rp, ctx->motion, c);
+    }
     if (wide > 0) {
         const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>((n + 63) / 64, ctx->maxPassBlocks));
         void* args[] = { (void*)&entries, (void*)&starts, (void*)&wide, (void*)&rp, (void*)&viaPos, &ctx->motion,
@@ -3459,7 +3672,15 @@ int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32
         if (int e = launchGrid(ctx, fn, grid, args)) { return e; }
         advanceBarrier(ctx->barrierBase, grid, wide - 1);
     }
-    if (wide < levels) {
+    if (wide < levels && which == 0 && mode != kRestitute && levels - wide <= kWalkMaxLevels) {
+        const uint32_t* from = starts + wide;
+        const uint32_t left = levels - wide;
+        if (mode == kBias) {
+            LAUNCH(k_rest_walk<kBias>, 1, kWalkThreads, entries, from, left, rp, ctx->motion, mass, c);
+        } else {
+            LAUNCH(k_rest_walk<kRelax>, 1, kWalkThreads, entries, from, left, rp, ctx->motion, mass, c);
+        }
+    } else if (wide < levels) {
         const uint32_t* from = starts + wide;
         const uint32_t left = levels - wide;
         switch (mode) {
@@ -4370,7 +4591,7 @@ void tfs_destroy(tfs_ctx* ctx) {
                      ctx->edges, ctx->bodyWorld, ctx->refreshing, ctx->worldStart, ctx->restTracked,
                      ctx->restDeg, ctx->restStart, ctx->restCursor, ctx->restEp, ctx->restRank, ctx->restLevel,
                      ctx->restCount, ctx->restWord, ctx->restEntries[0], ctx->restEntries[1], ctx->restLevelStart[0],
-                     ctx->restLevelStart[1], ctx->restH, ctx->restP, ctx->restPeak, ctx->restAx, ctx->restPos, ctx->wrestDeg,
+                     ctx->restLevelStart[1], ctx->restH, ctx->restP, ctx->restPeak, ctx->restAx, ctx->restPre, ctx->restPos, ctx->wrestDeg,
                      ctx->wrestStart, ctx->wrestCursor, ctx->wrestEp, ctx->touchFlag, ctx->hitFlag, ctx->touchList,
                      ctx->hitList, ctx->hitData, ctx->hits, ctx->ext, ctx->contactsNext, ctx->extNext, ctx->pairs,
                      ctx->pairLo, ctx->pairHi, ctx->pairKeep, ctx->pairStart, ctx->manifolds, ctx->ckBody, ctx->ckRow, ctx->tapeBuf, ctx->adj.bar, ctx->adj.dpBar,

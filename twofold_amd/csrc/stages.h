@@ -707,25 +707,28 @@ TF_HD void warmRestSide(const tfs_contact& c, bool sideB, bool moves, const Mass
     }
 }
 
-// solvePlanar: src/solver.cpp:137-209.
-template <bool useBias>
-TF_HD void solvePlanar(tfs_contact& c, const BodyView& va, const BodyView& vb, const Mass& ma, const Mass& mb,
-                       Motion& a, Motion& b, const Ctx& ctx, const PlanarAxes* axes) {
-    const bool moveA = !anchoredView(va), moveB = !anchoredView(vb);
+// What a planar contact's normal rows take from the bodies' travel this pass (dp, dq), which a velocity
+// pass does not change (src/solver.cpp:143-166): per point the bias of its separation, and the factors
+// of its impulse, scale = -normalMass * massScale (the first product of the impulse's expression,
+// evaluated left to right) and impulseScale.
+struct PlanarRowPre {
+    float scale[4], bias[4], impulseScale[4];
+};
+
+TF_HD void planarPre(const tfs_contact& c, bool moveA, bool moveB, const Motion& a, const Motion& b, const Ctx& ctx,
+                     const PlanarAxes* axes, bool useBias, PlanarRowPre& out) {
     const Soft soft = (!moveA || !moveB) ? ctx.staticSoft : ctx.contactSoft;
-    const V3 normal = ld3(c.normal), tangent1 = ld3(c.tangent1), plane = ld3(c.plane);
+    const V3 normal = ld3(c.normal), plane = ld3(c.plane);
     const float shift = dot(b.dp - a.dp, normal);
     float tax, tay, tbx, tby;
     planarTurn(a, plane, tax, tay);
     planarTurn(b, plane, tbx, tby);
-    PlanarState st(a, b, ma, mb, plane);
-    bool changed = false;
-    TF_POINTS(i, c.count) {
-        tfs_point& p = c.points[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
         const PlanarAxes& x = axes[i];
         const float reachA = tax * x.tangentA + tay * x.normalA;
         const float reachB = tbx * x.tangentB + tby * x.normalB;
-        const float sep = shift + reachB - reachA + p.adjusted_separation;
+        const float sep = shift + reachB - reachA + c.points[i].adjusted_separation;
         float bias = 0.0f, massScale = 1.0f, impulseScale = 0.0f;
         if (sep > 0.0f) {
             bias = sep * ctx.invH;
@@ -734,8 +737,29 @@ TF_HD void solvePlanar(tfs_contact& c, const BodyView& va, const BodyView& vb, c
             massScale = soft.massScale;
             impulseScale = soft.impulseScale;
         }
+        out.scale[i] = -x.normalMass * massScale;
+        out.bias[i] = bias;
+        out.impulseScale[i] = impulseScale;
+    }
+}
+
+// solvePlanar: src/solver.cpp:137-209. PRE: the normal rows' biases and factors come from planarPre of
+// the same pass (the same operations, done before the pass).
+template <bool useBias, bool PRE = false>
+TF_HD void solvePlanar(tfs_contact& c, const BodyView& va, const BodyView& vb, const Mass& ma, const Mass& mb,
+                       Motion& a, Motion& b, const Ctx& ctx, const PlanarAxes* axes, const PlanarRowPre* pre = nullptr) {
+    const bool moveA = !anchoredView(va), moveB = !anchoredView(vb);
+    const V3 normal = ld3(c.normal), tangent1 = ld3(c.tangent1), plane = ld3(c.plane);
+    PlanarRowPre local;
+    if (!PRE) { planarPre(c, moveA, moveB, a, b, ctx, axes, useBias, local); }
+    const PlanarRowPre& q = PRE ? *pre : local;
+    PlanarState st(a, b, ma, mb, plane);
+    bool changed = false;
+    TF_POINTS(i, c.count) {
+        tfs_point& p = c.points[i];
+        const PlanarAxes& x = axes[i];
         const float vn = st.speed(normal, x.normalA, x.normalB);
-        float impulse = -x.normalMass * massScale * (vn + bias) - impulseScale * p.normal_impulse;
+        float impulse = q.scale[i] * (vn + q.bias[i]) - q.impulseScale[i] * p.normal_impulse;
         const float total = maxq(p.normal_impulse + impulse, 0.0f);
         impulse = total - p.normal_impulse;
         p.normal_impulse = total;
