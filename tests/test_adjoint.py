@@ -132,3 +132,21 @@ def test_model_forward_matches_the_oracle(seed):
         bodies = ob
         contacts, _ = world.collide(bodies, world.dt, oc)
     world.close()
+
+
+def test_full_size_sample_answers_are_the_models():
+    """The kept answers of the float64 model for the sampled full-size adjoint worlds
+    (tests/golden/adjoint-throw1024-sample.npz, which the GPU test reads instead of running the model
+    for 45 s a world) are still what the model gives: one of the compared worlds, recomputed."""
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "adjoint-throw1024-sample.npz")
+    fx = np.load(path)
+    k = int(fx["sample"][1])
+    case = Case([int(fx["seed0"]) + k], steps=int(fx["steps"]))
+    m = case.model
+    assert int(fx[f"w{k}_n_contacts"]) == len(case.contacts)
+    np.testing.assert_array_equal(m.forward(steps=int(fx["steps"])), fx[f"w{k}_com"])
+    gv, gw, gmu = m.adjoint(case.target, steps=int(fx["steps"]))
+    np.testing.assert_array_equal(gv, fx[f"w{k}_gv"])
+    np.testing.assert_array_equal(gw, fx[f"w{k}_gw"])
+    assert np.sum(gmu) == fx[f"w{k}_gmu_sum"]
