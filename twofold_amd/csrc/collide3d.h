@@ -105,55 +105,89 @@ TF_HD int edgeV1(int e) { return (int)((0x767576352431ull >> (4 * e)) & 15u); }
 TF_HD int topoFace(int k, int s) { return (int)((0x452013u >> (4 * (2 * k + s))) & 15u); }
 TF_HD int topoEdge(int i, int s1, int s2) { return (int)((0x6472a931b580ull >> (4 * (4 * i + 2 * s1 + s2))) & 15u); }
 
+// A box's vertices follow from its half-extents and their index: makeBox's vertex i is
+// ((i + 1) & 2 ? x : -x, i & 2 ? y : -y, i < 4 ? -z : z), and otherFrame carries it into the pair's frame
+// as r * v + t. Only the face normals and offsets are kept, and a normal or offset at an index known
+// only at run time is picked by selects: no box is ever held in memory indexed at run time, which on
+// the device would put it in scratch memory.
 struct BoxPoly {
-    V3 v[8];
-    V3 n[6];
-    float d[6];
+    V3 h;
+    M3 r;
+    V3 t;
+    bool moved;
+    V3 faceN[6];
+    float faceD[6];
 };
+
+TF_HD V3 localVertex(V3 h, int i) { return vec(((i + 1) & 2) ? h.x : -h.x, (i & 2) ? h.y : -h.y, i < 4 ? -h.z : h.z); }
+TF_HD V3 vertexOf(const BoxPoly& p, int i) {
+    const V3 v = localVertex(p.h, i);
+    return p.moved ? tf::operator*(p.r, v) + p.t : v;
+}
+// Picked component by component by a chain of selects between floats: a loop over the array, an index
+// into it or a select between whole vectors is turned back into an access at a run-time index.
+TF_HD float pick6(int f, float a0, float a1, float a2, float a3, float a4, float a5) {
+    return f == 0 ? a0 : f == 1 ? a1 : f == 2 ? a2 : f == 3 ? a3 : f == 4 ? a4 : a5;
+}
+TF_HD V3 normalOf(const BoxPoly& p, int f) {
+    const V3* n = p.faceN;
+    return vec(pick6(f, n[0].x, n[1].x, n[2].x, n[3].x, n[4].x, n[5].x),
+               pick6(f, n[0].y, n[1].y, n[2].y, n[3].y, n[4].y, n[5].y),
+               pick6(f, n[0].z, n[1].z, n[2].z, n[3].z, n[4].z, n[5].z));
+}
+TF_HD float offsetOf(const BoxPoly& p, int f) {
+    const float* d = p.faceD;
+    return pick6(f, d[0], d[1], d[2], d[3], d[4], d[5]);
+}
 
 // makePolyhedron's vertices, normals (Newell's method about the centroid) and offsets for makeBox(h).
 TF_HD void makeBoxPoly(V3 h, BoxPoly& p) {
-    const float px[4] = { -h.x, h.x, h.x, -h.x };
-    const float py[4] = { -h.y, -h.y, h.y, h.y };
-    const float zMin = -h.z, zMax = h.z;
-    for (int i = 0; i < 4; ++i) {
-        p.v[i] = vec(px[i], py[i], zMin);
-        p.v[4 + i] = vec(px[i], py[i], zMax);
-    }
+    p.h = h;
+    p.r = M3{ { vec(1.0f, 0.0f, 0.0f), vec(0.0f, 1.0f, 0.0f), vec(0.0f, 0.0f, 1.0f) } };
+    p.t = vec(0.0f, 0.0f, 0.0f);
+    p.moved = false;
     V3 c = vec(0.0f, 0.0f, 0.0f);
-    for (int i = 0; i < 8; ++i) { c = c + p.v[i]; }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) { c = c + localVertex(h, i); }
     c = vdiv(c, 8.0f);
+#pragma unroll
     for (int f = 0; f < 6; ++f) {
         V3 n = vec(0.0f, 0.0f, 0.0f);
+#pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const V3 a = p.v[faceVertex(f, i)] - c;
-            const V3 b = p.v[faceVertex(f, (i + 1) & 3)] - c;
+            const V3 a = localVertex(h, faceVertex(f, i)) - c;
+            const V3 b = localVertex(h, faceVertex(f, (i + 1) & 3)) - c;
             n.x += (a.y - b.y) * (a.z + b.z);
             n.y += (a.z - b.z) * (a.x + b.x);
             n.z += (a.x - b.x) * (a.y + b.y);
         }
-        p.n[f] = normalize(n);
-        p.d[f] = dot(p.n[f], p.v[faceVertex(f, 0)]);
+        p.faceN[f] = normalize(n);
+        p.faceD[f] = dot(p.faceN[f], localVertex(h, faceVertex(f, 0)));
     }
 }
 
 // otherFrame (src/collide3d.cpp:47-75): the box's vertices, normals and offsets in the frame of the
-// pair's first shape (the edge arcs and bounds it also makes are not read for two boxes).
+// pair's first shape (the edge arcs and bounds it also makes are not read for two boxes). p is a box
+// in its own frame.
 TF_HD void otherFrame(const BoxPoly& p, const Xf& xf, BoxPoly& o) {
-    const M3 r = tf::toMat3(xf.q);
-    for (int i = 0; i < 8; ++i) { o.v[i] = tf::operator*(r, p.v[i]) + xf.p; }
+    o.h = p.h;
+    o.r = tf::toMat3(xf.q);
+    o.t = xf.p;
+    o.moved = true;
+#pragma unroll
     for (int f = 0; f < 6; ++f) {
-        o.n[f] = tf::operator*(r, p.n[f]);
-        o.d[f] = dot(o.n[f], o.v[faceVertex(f, 0)]);
+        o.faceN[f] = tf::operator*(o.r, p.faceN[f]);
+        o.faceD[f] = dot(o.faceN[f], vertexOf(o, faceVertex(f, 0)));
     }
 }
 
 // supportFace (src/collide3d.cpp:77-88).
 TF_HD int supportFace(const BoxPoly& p, V3 n) {
     int best = 0;
-    float most = dot(p.n[0], n);
+    float most = dot(p.faceN[0], n);
+#pragma unroll
     for (int i = 1; i < 6; ++i) {
-        const float d = dot(p.n[i], n);
+        const float d = dot(p.faceN[i], n);
         if (d > most) {
             most = d;
             best = i;
@@ -276,8 +310,9 @@ TF_HD void collideSphereBox(const BoxPoly& p, const Xf& xp, V3 center, float r, 
     int best = -1;
     float bestSep = -FLT_MAX;
     bool outside = false;
+#pragma unroll
     for (int i = 0; i < 6; ++i) {
-        const float s = dot(p.n[i], c) - p.d[i];
+        const float s = dot(p.faceN[i], c) - p.faceD[i];
         if (s > r + margin) { return; }
         if (s > 0.0f) { outside = true; }
         if (s > bestSep) {
@@ -288,7 +323,7 @@ TF_HD void collideSphereBox(const BoxPoly& p, const Xf& xp, V3 center, float r, 
     if (best < 0) { return; }
 
     if (!outside) {
-        const V3 n = p.n[best];
+        const V3 n = normalOf(p, best);
         m.normal = rotate(xp.q, n);
         m.points[0] = ManifoldPoint{ apply(xp, c - n * (0.5f * (bestSep + r))), bestSep - r, 0.0f };
         m.count = 1;
@@ -297,15 +332,17 @@ TF_HD void collideSphereBox(const BoxPoly& p, const Xf& xp, V3 center, float r, 
 
     V3 closest = vec(0.0f, 0.0f, 0.0f);
     float closestSq = FLT_MAX;
+#pragma unroll
     for (int i = 0; i < 6; ++i) {
-        const V3 fn = p.n[i];
-        const float s = dot(fn, c) - p.d[i];
+        const V3 fn = p.faceN[i];
+        const float s = dot(fn, c) - p.faceD[i];
         if (s <= 0.0f) { continue; }
         const V3 proj = c - fn * s;
         bool inside = true;
+#pragma unroll
         for (int k = 0; k < 4; ++k) {
-            const V3 a = p.v[faceVertex(i, k)];
-            const V3 b = p.v[faceVertex(i, (k + 1) & 3)];
+            const V3 a = vertexOf(p, faceVertex(i, k));
+            const V3 b = vertexOf(p, faceVertex(i, (k + 1) & 3));
             if (dot(cross(b - a, fn), proj - a) > 0.0f) {
                 inside = false;
                 const V3 ab = b - a;
@@ -350,8 +387,9 @@ TF_HD void collideHalfSpace(const Xf& xh, int otherKind, const BoxPoly& box, flo
     int np = 0;
     const V3 localN = invRotate(xo.q, n);
     const float localOffset = offset - dot(n, xo.p);
+#pragma unroll
     for (int i = 0; i < 8; ++i) {
-        const V3 v = box.v[i];
+        const V3 v = vertexOf(box, i);
         const float s = dot(localN, v) - localOffset;
         if (s > margin) { continue; }
         const V3 w = apply(xo, v);
@@ -508,14 +546,15 @@ TF_HD void clipPolygon(V3* poly, int& count, V3 n, V3 origin) {
 // face refFace of ref, in the pair's frame, placed in the world by xf.
 TF_HD void faceContact(const BoxPoly& ref, int refFace, const BoxPoly& inc, bool flip, const Xf& xf, float margin,
                        float band, const Motion& motion, Manifold& m) {
-    const V3 refN = ref.n[refFace];
-    const float refD = ref.d[refFace];
+    const V3 refN = normalOf(ref, refFace);
+    const float refD = offsetOf(ref, refFace);
     const V3 worldN = rotate(xf.q, refN);
 
     int incFace = 0;
     float minDot = FLT_MAX;
+#pragma unroll
     for (int i = 0; i < 6; ++i) {
-        const float d = dot(inc.n[i], refN);
+        const float d = dot(inc.faceN[i], refN);
         if (d < minDot) {
             minDot = d;
             incFace = i;
@@ -524,10 +563,10 @@ TF_HD void faceContact(const BoxPoly& ref, int refFace, const BoxPoly& inc, bool
 
     V3 poly[kMaxPoints];
     int count = 4;
-    for (int k = 0; k < 4; ++k) { poly[k] = inc.v[faceVertex(incFace, k)]; }
+    for (int k = 0; k < 4; ++k) { poly[k] = vertexOf(inc, faceVertex(incFace, k)); }
     for (int k = 0; k < 4; ++k) {
-        const V3 a = ref.v[faceVertex(refFace, k)];
-        const V3 b = ref.v[faceVertex(refFace, (k + 1) & 3)];
+        const V3 a = vertexOf(ref, faceVertex(refFace, k));
+        const V3 b = vertexOf(ref, faceVertex(refFace, (k + 1) & 3));
         clipPolygon(poly, count, cross(b - a, refN), a);
     }
 
@@ -541,9 +580,11 @@ TF_HD void faceContact(const BoxPoly& ref, int refFace, const BoxPoly& inc, bool
         pts[np++] = ManifoldPoint{ w - worldN * (0.5f * s), s, predict(motion, w, worldN, s) };
     }
     if (np == 0) {
-        V3 deepest = inc.v[0];
+        V3 deepest = vertexOf(inc, 0);
+#pragma unroll
         for (int k = 1; k < 8; ++k) {
-            if (dot(refN, inc.v[k]) < dot(refN, deepest)) { deepest = inc.v[k]; }
+            const V3 v = vertexOf(inc, k);
+            if (dot(refN, v) < dot(refN, deepest)) { deepest = v; }
         }
         const float s = dot(refN, deepest) - refD;
         if (s <= margin) {
@@ -595,22 +636,24 @@ TF_HD void edgeContact(const BoxPoly& a, const BoxPoly& b, const EdgeQuery& eq, 
     const V3 n = eq.axis;
     const V3 worldN = rotate(xf.q, n);
     const int fa = supportFace(a, n);
-    const float cosA = dot(a.n[fa], n);
+    const V3 faN = normalOf(a, fa);
+    const float faD = offsetOf(a, fa);
+    const float cosA = dot(faN, n);
     ManifoldPoint pts[kMaxPoints];
     int np = 0;
     if (cosA > 0.0f) {
         const int fi = supportFace(b, neg(n));
         V3 poly[kMaxPoints];
         int count = 4;
-        for (int k = 0; k < 4; ++k) { poly[k] = b.v[faceVertex(fi, k)]; }
+        for (int k = 0; k < 4; ++k) { poly[k] = vertexOf(b, faceVertex(fi, k)); }
         for (int k = 0; k < 4; ++k) {
-            const V3 p = a.v[faceVertex(fa, k)];
-            const V3 q = a.v[faceVertex(fa, (k + 1) & 3)];
+            const V3 p = vertexOf(a, faceVertex(fa, k));
+            const V3 q = vertexOf(a, faceVertex(fa, (k + 1) & 3));
             clipPolygon(poly, count, cross(q - p, n), p);
         }
         for (int k = 0; k < count; ++k) {
             const V3 p = poly[k];
-            const float s = maxf((dot(a.n[fa], p) - a.d[fa]) / cosA, eq.separation);
+            const float s = maxf((dot(faN, p) - faD) / cosA, eq.separation);
             if (s > margin) { continue; }
             const V3 w = apply(xf, p);
             pts[np++] = ManifoldPoint{ w - worldN * (0.5f * s), s, predict(motion, w, worldN, s) };
@@ -621,7 +664,8 @@ TF_HD void edgeContact(const BoxPoly& a, const BoxPoly& b, const EdgeQuery& eq, 
         return;
     }
     V3 c1, c2;
-    closestOnSegments(a.v[edgeV0(eq.a)], a.v[edgeV1(eq.a)], b.v[edgeV0(eq.b)], b.v[edgeV1(eq.b)], c1, c2);
+    closestOnSegments(vertexOf(a, edgeV0(eq.a)), vertexOf(a, edgeV1(eq.a)), vertexOf(b, edgeV0(eq.b)),
+                      vertexOf(b, edgeV1(eq.b)), c1, c2);
     m.normal = worldN;
     m.points[0] = ManifoldPoint{ apply(xf, (c1 + c2) * 0.5f), eq.separation, 0.0f };
     m.count = 1;
@@ -667,15 +711,16 @@ struct Simplex {
 TF_HD V3 support(const BoxPoly& c, const Xf& xf, V3 d) {
     const V3 local = invRotate(xf.q, d);
     int best = 0;
-    float most = dot(c.v[0], local);
+    float most = dot(vertexOf(c, 0), local);
+#pragma unroll
     for (int i = 1; i < 8; ++i) {
-        const float s = dot(c.v[i], local);
+        const float s = dot(vertexOf(c, i), local);
         if (s > most) {
             most = s;
             best = i;
         }
     }
-    return apply(xf, c.v[best]);
+    return apply(xf, vertexOf(c, best));
 }
 
 // segmentWeights (src/distance.cpp:59-65).
@@ -805,8 +850,8 @@ TF_HD V3 closestOf(const Simplex& s) {
 TF_HD int searchBoxes(const BoxPoly& a, const Xf& xa, const BoxPoly& b, const Xf& xb, float limit, V3& v) {
     const bool bounded = limit >= 0.0f;
     Simplex s;
-    s.v[0].a = apply(xa, a.v[0]);
-    s.v[0].b = apply(xb, b.v[0]);
+    s.v[0].a = apply(xa, vertexOf(a, 0));
+    s.v[0].b = apply(xb, vertexOf(b, 0));
     s.v[0].w = s.v[0].b - s.v[0].a;
     s.u[0] = 1.0;
     s.u[1] = s.u[2] = s.u[3] = 0.0;

@@ -41,6 +41,8 @@ constexpr int kScanSlots = 16;         // scans in one step
 constexpr int kProfileFamilies = 8;
 constexpr int kKeys = 5 * TFS_COLORS + 1; // row-order buckets: colour x point count, then the rest
 constexpr uint32_t kSpinLimit = 1u << 26;
+constexpr int kCollideThreads = 64;    // the device narrowphase: one wave per work-group
+constexpr uint32_t kOrderMax = 16384;  // longest fast list put in collision order
 constexpr int kIslandRounds = 64;      // hook-and-jump rounds of the island pass before it gives up
 constexpr int kIslandAhead = 6;        // rounds launched before the host checks the first time (at most)
 // The per-body kernels of a substep run a wave per block: a 32k-body pile is then 513 blocks over the
@@ -2775,11 +2777,20 @@ __global__ void k_gather_again(const uint32_t* again, uint32_t n, const tfs_cont
 // with the reference's narrowphase (collide3d.h: spheres, boxes, half-spaces), into fresh[k] as the
 // refresh callback would write it. A pair collide3d.h does not collide (another kind of shape, or a pair
 // the reference collides in 2D) raises error bit 16.
-__global__ void __launch_bounds__(kThreads) k_refresh_collide(const uint32_t* again, uint32_t n, const tfs_contact* contacts,
-                                                              const tfs_body* bodies, const float4* motion,
-                                                              const tfs_shape* shapes, float remaining, float speculative,
-                                                              float linearSlop, tfs_contact* fresh, DevCounts* cnt) {
-    for (uint32_t k = gid(); k < n; k += gstride()) {
+//
+// A wave takes as long as the costliest path any of its lanes takes (two boxes: the closed-form
+// queries, the gap test in double, clipping), so the pairs go in `order`: the list's positions with
+// the pairs of two boxes first, then sphere-box, half-space-box and the rest (k_refresh_order), one
+// wave to a work-group so the waves spread over the CUs. Without an order (a list longer than
+// kOrderMax) the pairs go in list order. At one wave per SIMD the compiler keeps the clipping's and the
+// gap test's small arrays (indexed at run time) in LDS rather than in scratch memory.
+__global__ void __launch_bounds__(kCollideThreads) __attribute__((amdgpu_waves_per_eu(1, 1))) k_refresh_collide(const uint32_t* again, const uint32_t* order, uint32_t n,
+                                                                     const tfs_contact* contacts, const tfs_body* bodies,
+                                                                     const float4* motion, const tfs_shape* shapes,
+                                                                     float remaining, float speculative, float linearSlop,
+                                                                     tfs_contact* fresh, DevCounts* cnt) {
+    for (uint32_t j = gid(); j < n; j += gstride()) {
+        const uint32_t k = order ? order[j] : j;
         const tfs_contact& c = contacts[again[k]];
         const uint32_t ia = c.body_a, ib = c.body_b;
         const tfs_body& a = bodies[ia];
@@ -2804,6 +2815,53 @@ __global__ void __launch_bounds__(kThreads) k_refresh_collide(const uint32_t* ag
             }
             tfc::fillContact(a, b, m.normal, m.count, pos, sep, speculative, f);
         }
+    }
+}
+
+// The cost class of a pair for the collision order: 0 two boxes, 1 a sphere and a box, 2 a half-space and
+// a box, 3 anything else (two spheres, a half-space and a sphere, kinds the device does not collide).
+__device__ __forceinline__ uint32_t collideClass(uint32_t ka, uint32_t kb) {
+    const bool boxA = ka == (uint32_t)tfc::kBox, boxB = kb == (uint32_t)tfc::kBox;
+    if (boxA && boxB) { return 0; }
+    const bool sphere = ka == (uint32_t)tfc::kSphere || kb == (uint32_t)tfc::kSphere;
+    const bool half = ka == (uint32_t)tfc::kHalfSpace || kb == (uint32_t)tfc::kHalfSpace;
+    if ((boxA || boxB) && sphere) { return 1; }
+    if ((boxA || boxB) && half) { return 2; }
+    return 3;
+}
+
+// The collision order of the two fast lists (work-group 0: the fast list, 1: the fast solid list), by
+// cost class, costliest first: a counting sort in LDS. Within a class the order is whatever the atomics
+// give; each pair's manifold goes to its own list position, so the results do not depend on it.
+// Lists longer than kOrderMax are left in list order (the host passes no order for them).
+__global__ void __launch_bounds__(1024) k_refresh_order(const uint32_t* fast, const uint32_t* fastSolid, const DevCounts* cnt,
+                                                        const tfs_contact* contacts, const tfs_shape* shapes,
+                                                        uint32_t* fastOrder, uint32_t* fastSolidOrder) {
+    constexpr uint32_t kClasses = 4;
+    __shared__ uint32_t count[kClasses];
+    const uint32_t* list = blockIdx.x ? fastSolid : fast;
+    uint32_t* out = blockIdx.x ? fastSolidOrder : fastOrder;
+    const uint32_t n = blockIdx.x ? cnt->nFastSolid : cnt->nFast;
+    if (n > kOrderMax) { return; }
+    if (threadIdx.x < kClasses) { count[threadIdx.x] = 0; }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        const tfs_contact& c = contacts[list[k]];
+        atomicAdd(&count[collideClass(shapes[c.body_a].kind, shapes[c.body_b].kind)], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t at = 0;
+        for (uint32_t q = 0; q < kClasses; ++q) {
+            const uint32_t m = count[q];
+            count[q] = at;
+            at += m;
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        const tfs_contact& c = contacts[list[k]];
+        out[atomicAdd(&count[collideClass(shapes[c.body_a].kind, shapes[c.body_b].kind)], 1u)] = k;
     }
 }
 
@@ -3074,6 +3132,8 @@ struct tfs_ctx {
     uint32_t* active = nullptr;
     uint32_t* fast = nullptr;
     uint32_t* fastSolid = nullptr;
+    uint32_t* fastOrder = nullptr;        // the fast lists' positions by collision cost (k_refresh_order)
+    uint32_t* fastSolidOrder = nullptr;
     uint32_t* impactFlag = nullptr;
     uint32_t* impactPos = nullptr;
     uint32_t* impact = nullptr;
@@ -3278,6 +3338,8 @@ int allocContacts(tfs_ctx* ctx, uint32_t cap) {
     if ((r = grow(ctx, ctx->contacts, cap)) || (r = grow(ctx, ctx->fresh, cap)) || (r = grow(ctx, ctx->fActive, cap)) ||
         (r = grow(ctx, ctx->fFast, cap)) || (r = grow(ctx, ctx->fFastSolid, cap)) || (r = grow(ctx, ctx->active, cap)) ||
         (r = grow(ctx, ctx->fast, cap)) || (r = grow(ctx, ctx->fastSolid, cap)) ||
+        (r = grow(ctx, ctx->fastOrder, std::min(cap, kOrderMax))) ||
+        (r = grow(ctx, ctx->fastSolidOrder, std::min(cap, kOrderMax))) ||
         (r = grow(ctx, ctx->impactFlag, cap)) || (r = grow(ctx, ctx->impactPos, cap)) ||
         (r = grow(ctx, ctx->impact, cap)) || (r = grow(ctx, ctx->rowsList, cap)) ||
         (r = grow(ctx, ctx->impactRowsList, cap)) || (r = grow(ctx, ctx->colors, cap)) ||
@@ -3760,8 +3822,10 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
             ctx->pending.push_back({ 4, ev, (double)nAgain });
             HIP_TRY(hipEventRecord(ctx->evPool[ev], ctx->stream));
         }
-        LAUNCH(k_refresh_collide, blocksFor(nAgain), kThreads, again, nAgain, ctx->contacts, ctx->bodies, ctx->motion,
-               ctx->shapes, remaining, ctx->settings.speculative, 0.005f * ctx->settings.length_unit, ctx->fresh, ctx->cnt);
+        const uint32_t* order = nAgain > kOrderMax ? nullptr : half ? ctx->fastOrder : ctx->fastSolidOrder;
+        LAUNCH(k_refresh_collide, blocksFor(nAgain, kCollideThreads), kCollideThreads, again, order, nAgain, ctx->contacts,
+               ctx->bodies, ctx->motion, ctx->shapes, remaining, ctx->settings.speculative,
+               0.005f * ctx->settings.length_unit, ctx->fresh, ctx->cnt);
         if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
         return refreshApply(ctx, again, nAgain, laned, impactLaned, nRest);
     }
@@ -4072,6 +4136,10 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     bool hostRefresh = false;
     for (int i = 1; i < n; ++i) { hostRefresh = hostRefresh || (!ctx->nShapes && (i == n / 2 ? hc.nFast : hc.nFastSolid) > 0); }
     const bool graph = ctx->useGraph && timed && !ctx->profiling && !ctx->tuning && !ctx->coopLaunch && !hostRefresh;
+    if (n > 1 && ctx->nShapes && (hc.nFast > 0 || hc.nFastSolid > 0)) {
+        LAUNCH(k_refresh_order, 2, 1024, ctx->fast, ctx->fastSolid, ctx->cnt, ctx->contacts, ctx->shapes, ctx->fastOrder,
+               ctx->fastSolidOrder);
+    }
     auto loop = [&]() -> int {
         for (int i = 0; i < n; ++i) {
             const bool half = i == n / 2;
