@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define TFS_ABI_VERSION 2
+#define TFS_ABI_VERSION 3
 
 enum {
     TFS_OK = 0,
@@ -151,6 +151,8 @@ typedef struct tfs_settings {
     float hit_threshold;        /* approach speed above which a touching point is a hit */
     float contact_reuse_distance;  /* a pair moved less than this relative to itself keeps its points */
     float contact_reuse_angle;     /* ... and turned less than this (radians) */
+    float joint_hertz;             /* WorldSettings::jointHertz (world.h:237) */
+    float joint_damping_ratio;     /* WorldSettings::jointDampingRatio (world.h:238) */
 } tfs_settings;
 
 /* What a contact keeps for its reuse in the next step (src/internal.h:463-503: localNormal,
@@ -236,6 +238,54 @@ typedef struct tfs_shape {
     float radius;
 } tfs_shape;
 
+/* A joint as the solve block reads and writes it: src/internal.h:508-548 (Joint), in the reference's
+ * own terms (anchors, axes and reference directions in each body's frame, the rest turn, limits, motor,
+ * spring, friction, break limits, the impulses carried from step to step, the rows' force and torque
+ * per unit impulse, and the force and torque of the last step). body_a may be TFS_WORLD_BODY (joined to
+ * the world: a static body at the origin). The device solves the point, fixed, distance and slider
+ * joints and the hinge without limits; a limited hinge and the swing-twist joint read angles through
+ * atan2 and asin (src/joint.cpp:212-217, 239-287), whose float bits are the host C library's, and are
+ * refused (TFS_E_ARG). */
+#define TFS_JOINT_POINT 0
+#define TFS_JOINT_FIXED 1
+#define TFS_JOINT_HINGE 2
+#define TFS_JOINT_SLIDER 3
+#define TFS_JOINT_DISTANCE 4
+#define TFS_JOINT_SWING_TWIST 5
+#define TFS_WORLD_BODY 0xffffffffu
+
+typedef struct tfs_joint {
+    uint32_t body_a;
+    uint32_t body_b;
+    uint8_t type;               /* TFS_JOINT_* */
+    uint8_t alive;
+    uint8_t broken;
+    uint8_t limited;
+    float local_a[3];
+    float local_b[3];
+    float axis_a[3];
+    float axis_b[3];
+    float ref_a[3];
+    float ref_b[3];
+    float rest_turn[4];
+    float length;
+    float lower, upper;
+    float motor_speed, max_motor;
+    float hertz, damping_ratio;
+    float max_friction;
+    float frame_a[4], frame_b[4];
+    float swing_y, swing_z;
+    float break_force, break_torque;
+    float linear_impulse[3];    /* in/out: carried from step to step */
+    float angular_impulse[3];
+    float axial_impulse, motor_impulse, lower_impulse, upper_impulse, friction_impulse, swing_impulse;
+    float friction_turn[3];
+    float row_force[2][3];      /* out */
+    float row_torque[2][3];
+    float force[3];             /* out: over the last step it was active */
+    float torque[3];
+} tfs_joint;
+
 typedef struct tfs_ctx tfs_ctx;
 
 /* Version of this ABI (TFS_ABI_VERSION) and struct sizes, for bindings to check their layouts. */
@@ -243,6 +293,7 @@ int tfs_abi_version(void);
 size_t tfs_sizeof_body(void);
 size_t tfs_sizeof_contact(void);
 size_t tfs_sizeof_settings(void);
+size_t tfs_sizeof_joint(void);
 
 /* The reference's default settings (include/twofold/world.h:216-241; src/internal.h:348-351). */
 void tfs_default_settings(tfs_settings* out);
@@ -335,6 +386,18 @@ int tfs_sleep(tfs_ctx* ctx, float dt);
  * usePlanar, src/world.cpp:737-750).
  */
 int tfs_set_body_shapes(tfs_ctx* ctx, uint32_t n, const tfs_shape* shapes);
+
+/* The joints (src/internal.h:508-548), in the reference's order, and the world body's velocities
+ * (world_motion: v then w, NULL for zero): the reference keeps one static world body whose velocities
+ * the joint pushes touch (src/joint.cpp:140-150). Each tfs_step then does the joints' part of the solve
+ * block (src/world.cpp:1623-1632, 1647, 1670-1678, 1685-1691): the active joints (alive, not broken, a
+ * dynamic awake end) scaled to the kept speed, warm started and solved before the contacts in every
+ * pass, broken past their limits, settled, and the bodies' inverse masses made again; the sleep pass
+ * joins their bodies' islands (1210-1223). The joint and world-body arrays stay resident (upload once,
+ * download when needed). TFS_E_ARG for a body index out of range, a limited hinge or a swing-twist
+ * joint. n = 0 removes them. Replaces World::Impl::joints and worldBody for the solve block. */
+int tfs_upload_joints(tfs_ctx* ctx, const tfs_joint* joints, uint32_t n, const float* world_motion);
+int tfs_download_joints(tfs_ctx* ctx, tfs_joint* out, uint32_t n, float* world_motion);
 
 /*
  * One step that allocates nothing. No counterpart in the reference: it is for callers that cannot

@@ -10,7 +10,8 @@
 // colorContacts 1651, batchContacts 1656-1657 (colours, row order, impact list), warmStartBatches
 // 1672 (substep counter), prepareContact 1126 (the fresh manifold of a refreshed fast contact),
 // prepareLanes 1666 (end of a refresh), applyRestitution 1683 + limitSpeed 1157 (end of the solve
-// block, after the final limitSpeeds at 1684).
+// block, after the final limitSpeeds at 1684), settleJoints 1687 (with active joints, the end of the
+// solve block is after it and the inverse masses of 1688-1690).
 
 #include <algorithm>
 #include <cfloat>
@@ -62,6 +63,7 @@ struct twofold_step_clock {
 #define warmStartBatches tfcap_warmStartBatches
 #define applyRestitution tfcap_applyRestitution
 #define limitSpeed tfcap_limitSpeed
+#define settleJoints tfcap_settleJoints
 #include "world.cpp"
 #undef prepareContacts
 #undef colorContacts
@@ -71,6 +73,7 @@ struct twofold_step_clock {
 #undef warmStartBatches
 #undef applyRestitution
 #undef limitSpeed
+#undef settleJoints
 #ifdef TFB_BOUND
 #undef steady_clock
 #undef warmStart
@@ -93,6 +96,7 @@ void prepareContact(const Body& a, const Body& b, Contact& c, PlanarAxes* planar
 void warmStartBatches(SolverData& s, const ContactBatches& batches);
 void applyRestitution(SolverData& s, const std::vector<uint32_t>& active, const SolverContext& ctx);
 void limitSpeed(Body& b, float maxSpeed, float maxSpin);
+void settleJoints(SolverData& s, const std::vector<uint32_t>& active, int iterations);
 #ifdef TFB_BOUND
 void warmStart(SolverData& s, const std::vector<uint32_t>& active);
 void solveBatches(SolverData& s, const ContactBatches& batches, const SolverContext& ctx, bool useBias);
@@ -196,6 +200,54 @@ tfs_contact toContact(const Contact& c) {
     return o;
 }
 
+tfs_joint toJoint(const Joint& j) {
+    tfs_joint o;
+    std::memset(&o, 0, sizeof o);
+    o.body_a = j.bodyA;
+    o.body_b = j.bodyB;
+    o.type = (uint8_t)j.type;
+    o.alive = j.alive ? 1 : 0;
+    o.broken = j.broken ? 1 : 0;
+    o.limited = j.limited ? 1 : 0;
+    put3(o.local_a, j.localA);
+    put3(o.local_b, j.localB);
+    put3(o.axis_a, j.axisA);
+    put3(o.axis_b, j.axisB);
+    put3(o.ref_a, j.refA);
+    put3(o.ref_b, j.refB);
+    put4(o.rest_turn, j.restTurn);
+    o.length = j.length;
+    o.lower = j.lower;
+    o.upper = j.upper;
+    o.motor_speed = j.motorSpeed;
+    o.max_motor = j.maxMotor;
+    o.hertz = j.hertz;
+    o.damping_ratio = j.dampingRatio;
+    o.max_friction = j.maxFriction;
+    put4(o.frame_a, j.frameA);
+    put4(o.frame_b, j.frameB);
+    o.swing_y = j.swingY;
+    o.swing_z = j.swingZ;
+    o.break_force = j.breakForce;
+    o.break_torque = j.breakTorque;
+    put3(o.linear_impulse, j.linearImpulse);
+    put3(o.angular_impulse, j.angularImpulse);
+    o.axial_impulse = j.axialImpulse;
+    o.motor_impulse = j.motorImpulse;
+    o.lower_impulse = j.lowerImpulse;
+    o.upper_impulse = j.upperImpulse;
+    o.friction_impulse = j.frictionImpulse;
+    o.swing_impulse = j.swingImpulse;
+    put3(o.friction_turn, j.frictionTurn);
+    for (int k = 0; k < 2; ++k) {
+        put3(o.row_force[k], j.rowForce[k]);
+        put3(o.row_torque[k], j.rowTorque[k]);
+    }
+    put3(o.force, j.force);
+    put3(o.torque, j.torque);
+    return o;
+}
+
 tfs_contact_ext toExt(const Contact& c) {
     tfs_contact_ext o;
     std::memset(&o, 0, sizeof o);
@@ -235,6 +287,9 @@ struct Capture {
     std::vector<tfs_contact_ext> oldExt, preExt;
     std::vector<tfs_pair> pairs;
     std::vector<tfs_manifold> manifolds;
+    // the joints and the world body's velocities (v, w) at the solve block's start and end
+    std::vector<tfs_joint> preJoints, postJoints;
+    float preWorld[6] = {}, postWorld[6] = {};
 };
 
 struct Handle {
@@ -261,6 +316,13 @@ float g_dt = 0.0f;
 
 namespace twofold {
 
+void recordJoints(std::vector<tfs_joint>& out, float* world) {
+    out.clear();
+    for (const Joint& j : g_impl->joints) { out.push_back(toJoint(j)); }
+    put3(world, g_impl->worldBody.v);
+    put3(world + 3, g_impl->worldBody.w);
+}
+
 void tfcap_prepareContacts(SolverData& s, const std::vector<uint32_t>& active) {
     if (tfbBound()) {
         tfbSolveBlock();
@@ -274,6 +336,7 @@ void tfcap_prepareContacts(SolverData& s, const std::vector<uint32_t>& active) {
         c.preContacts.clear();
         for (const Contact& x : s.contacts) { c.preContacts.push_back(toContact(x)); }
         c.active = active;
+        recordJoints(c.preJoints, c.preWorld);
         c.fast = g_impl->fastContacts;
         c.fastSolid = g_impl->fastSolidContacts;
         c.preExt.clear();
@@ -426,6 +489,21 @@ void tfcap_limitSpeed(Body& b, float maxSpeed, float maxSpin) {
     for (const Body& x : g_impl->bodies) { c.postBodies.push_back(toBody(x)); }
     c.postContacts.clear();
     for (const Contact& x : g_impl->contacts) { c.postContacts.push_back(toContact(x)); }
+    recordJoints(c.postJoints, c.postWorld);
+}
+
+// With active joints the solve block ends after settleJoints and the inverse masses made again
+// (src/world.cpp:1687-1690); they are made here too (the same values again) to record that end.
+void tfcap_settleJoints(SolverData& s, const std::vector<uint32_t>& active, int iterations) {
+    settleJoints(s, active, iterations);
+    if (!g_cap || tfbBound()) { return; }
+    Capture& c = *g_cap;
+    c.postBodies.clear();
+    for (Body& b : g_impl->bodies) {
+        if (b.alive && b.type == MotionType::Dynamic && b.awake) { updateInverseMass(b); }
+        c.postBodies.push_back(toBody(b));
+    }
+    recordJoints(c.postJoints, c.postWorld);
 }
 
 #ifdef TFB_BOUND
@@ -479,6 +557,8 @@ void* tfr_create(const tfs_settings* s) {
         ws.hitThreshold = s->hit_threshold;
         ws.contactReuseDistance = s->contact_reuse_distance;
         ws.contactReuseAngle = s->contact_reuse_angle;
+        ws.jointHertz = s->joint_hertz;
+        ws.jointDampingRatio = s->joint_damping_ratio;
     }
     return new Handle(ws);
 }
@@ -505,6 +585,8 @@ void tfr_default_settings(tfs_settings* o) {
     o->hit_threshold = s.hitThreshold;
     o->contact_reuse_distance = s.contactReuseDistance;
     o->contact_reuse_angle = s.contactReuseAngle;
+    o->joint_hertz = s.jointHertz;
+    o->joint_damping_ratio = s.jointDampingRatio;
 }
 
 // Makes `h` a copy of `from` (World's copy is a full snapshot, src/world.cpp:1732-1737).
@@ -707,6 +789,47 @@ uint32_t tfr_cap_ext(void* h, int which, tfs_contact_ext* out, uint32_t cap) {
 
 // Bodies before the step.
 uint32_t tfr_cap_old_bodies(void* h, tfs_body* out, uint32_t cap) { return copyOut(((Handle*)h)->cap.oldBodies, out, cap); }
+
+// The recorded joints (which: 0 at the solve block's start, 1 at its end) and the world body's v, w.
+uint32_t tfr_cap_joints(void* h, int which, tfs_joint* out, uint32_t cap, float* world) {
+    const Capture& c = ((Handle*)h)->cap;
+    if (world) { std::memcpy(world, which ? c.postWorld : c.preWorld, sizeof c.preWorld); }
+    return copyOut(which ? c.postJoints : c.preJoints, out, cap);
+}
+
+size_t tfr_sizeof_joint(void) { return sizeof(tfs_joint); }
+
+// A joint as World::createJoint makes it (src/world.cpp:2221-2300): kind TFS_JOINT_*, body_a -1 for the
+// world, and p[] = anchor A (3), anchor B (3, the distance joint's), axis (3), limited, lower, upper,
+// motor speed, max motor force or torque, friction, length, hertz, damping ratio, break force, break
+// torque, collide connected. Returns the joint's index, or -1.
+int tfr_add_joint(void* h, int kind, int bodyA, int bodyB, const float* p) {
+    World& w = ((Handle*)h)->world;
+    JointDesc d;
+    if (bodyA >= 0) { d.bodyA = BodyId{ (uint32_t)bodyA, w.impl_->bodies[bodyA].generation }; }
+    d.bodyB = BodyId{ (uint32_t)bodyB, w.impl_->bodies[bodyB].generation };
+    const Vec3 anchor{ p[0], p[1], p[2] }, anchorB{ p[3], p[4], p[5] }, axis{ p[6], p[7], p[8] };
+    const bool limited = p[9] != 0.0f;
+    switch (kind) {
+        case TFS_JOINT_POINT: d.kind = PointJoint{ anchor }; break;
+        case TFS_JOINT_FIXED: d.kind = FixedJoint{ anchor }; break;
+        case TFS_JOINT_HINGE: d.kind = HingeJoint{ anchor, axis, limited, p[10], p[11], p[12], p[13], p[14] }; break;
+        case TFS_JOINT_SLIDER: d.kind = SliderJoint{ anchor, axis, limited, p[10], p[11], p[12], p[13], p[14] }; break;
+        case TFS_JOINT_DISTANCE: d.kind = DistanceJoint{ anchor, anchorB, p[15], p[16], p[17], limited, p[10], p[11] }; break;
+        default: return -1;
+    }
+    d.breakForce = p[18];
+    d.breakTorque = p[19];
+    d.collideConnected = p[20] != 0.0f;
+    const JointId id = w.createJoint(d);
+    return id.index == UINT32_MAX ? -1 : (int)id.index;
+}
+
+uint32_t tfr_joints(void* h, tfs_joint* out, uint32_t cap) {
+    std::vector<tfs_joint> v;
+    for (const Joint& j : ((Handle*)h)->world.impl_->joints) { v.push_back(toJoint(j)); }
+    return copyOut(v, out, cap);
+}
 
 // The step's candidate pairs and their manifolds.
 uint32_t tfr_cap_pairs(void* h, tfs_pair* out, uint32_t cap) { return copyOut(((Handle*)h)->cap.pairs, out, cap); }

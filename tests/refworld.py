@@ -82,6 +82,12 @@ def _load_ref(path=REF_SO):
         lib.tfr_cap_refresh.restype = ctypes.c_uint32
         lib.tfr_cap_refresh.argtypes = [_vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int), _fp, _vp, _vp,
                                         ctypes.c_uint32]
+        lib.tfr_add_joint.restype = ctypes.c_int
+        lib.tfr_add_joint.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _fp]
+        lib.tfr_cap_joints.restype = ctypes.c_uint32
+        lib.tfr_cap_joints.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_uint32, _fp]
+        lib.tfr_joints.restype = ctypes.c_uint32
+        lib.tfr_joints.argtypes = [_vp, _vp, ctypes.c_uint32]
         _abi.check_sizes(lib, "tfr")
         if path == BOUND_SO:
             lib.tfb_error.restype = ctypes.c_char_p
@@ -94,7 +100,8 @@ class StepRecord:
 
     __slots__ = ("pre_bodies", "pre_contacts", "post_bodies", "post_contacts", "step_bodies", "lists",
                  "colors", "ctx", "refreshes", "dt", "after_contacts", "hits", "touching", "old_bodies",
-                 "old_contacts", "old_ext", "pre_ext", "pairs", "manifolds")
+                 "old_contacts", "old_ext", "pre_ext", "pairs", "manifolds", "pre_joints", "post_joints",
+                 "pre_world", "post_world")
 
 
 class RefWorld:
@@ -121,6 +128,9 @@ class RefWorld:
                                        sh["rolling"])
             if r < 0:
                 raise ValueError("shape rejected")
+        for j in getattr(scene, "joints", []):
+            if self.lib.tfr_add_joint(self.h, j["kind"], j["a"], j["b"], _f(j["p"])) < 0:
+                raise ValueError("joint rejected")
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -174,6 +184,12 @@ class RefWorld:
         self.lib.tfr_bodies(self.h, _ptr(out), n)
         return out
 
+    def joints(self):
+        n = self.lib.tfr_joints(self.h, None, 0)
+        out = np.zeros(n, _abi.JOINT_DTYPE)
+        self.lib.tfr_joints(self.h, _ptr(out), n)
+        return out
+
     def contacts(self):
         n = self.lib.tfr_num_contacts(self.h)
         out = np.zeros(n, CONTACT_DTYPE)
@@ -207,6 +223,11 @@ class RefWorld:
         r.old_ext = grab(lib.tfr_cap_ext, _abi.EXT_DTYPE, 2)
         r.pre_ext = grab(lib.tfr_cap_ext, _abi.EXT_DTYPE, 0)
         r.pairs = grab(lib.tfr_cap_pairs, _abi.PAIR_DTYPE)
+        nj = lib.tfr_cap_joints(h, 0, None, 0, None)
+        r.pre_joints, r.post_joints = np.zeros(nj, _abi.JOINT_DTYPE), np.zeros(nj, _abi.JOINT_DTYPE)
+        r.pre_world, r.post_world = np.zeros(6, np.float32), np.zeros(6, np.float32)
+        lib.tfr_cap_joints(h, 0, _ptr(r.pre_joints), nj, r.pre_world.ctypes.data_as(_fp))
+        lib.tfr_cap_joints(h, 1, _ptr(r.post_joints), nj, r.post_world.ctypes.data_as(_fp))
         r.manifolds = grab(lib.tfr_cap_manifolds, _abi.MANIFOLD_DTYPE)
         # after the whole step: the contacts' touched flags as reportTouches left them, its hits and count
         r.after_contacts = self.contacts()

@@ -67,13 +67,31 @@ HIT_DTYPE = np.dtype([("contact", U32), ("point", U32), ("position", F, 3), ("no
 SHAPE_DTYPE = np.dtype([("kind", np.uint32), ("half", np.float32, 3), ("radius", np.float32)])
 SHAPE_SPHERE, SHAPE_BOX, SHAPE_HALF_SPACE = 0, 1, 4
 
+# tfs_joint: a joint as the solve block reads and writes it (src/internal.h:508-548)
+JOINT_DTYPE = np.dtype(
+    [
+        ("body_a", U32), ("body_b", U32), ("type", U8), ("alive", U8), ("broken", U8), ("limited", U8),
+        ("local_a", F, 3), ("local_b", F, 3), ("axis_a", F, 3), ("axis_b", F, 3), ("ref_a", F, 3), ("ref_b", F, 3),
+        ("rest_turn", F, 4), ("length", F), ("lower", F), ("upper", F), ("motor_speed", F), ("max_motor", F),
+        ("hertz", F), ("damping_ratio", F), ("max_friction", F), ("frame_a", F, 4), ("frame_b", F, 4),
+        ("swing_y", F), ("swing_z", F), ("break_force", F), ("break_torque", F),
+        ("linear_impulse", F, 3), ("angular_impulse", F, 3), ("axial_impulse", F), ("motor_impulse", F),
+        ("lower_impulse", F), ("upper_impulse", F), ("friction_impulse", F), ("swing_impulse", F),
+        ("friction_turn", F, 3), ("row_force", F, (2, 3)), ("row_torque", F, (2, 3)), ("force", F, 3),
+        ("torque", F, 3),
+    ],
+    align=True,
+)
+JOINT_POINT, JOINT_FIXED, JOINT_HINGE, JOINT_SLIDER, JOINT_DISTANCE, JOINT_SWING_TWIST = range(6)
+WORLD_BODY = 0xFFFFFFFF
+
 SETTINGS_DTYPE = np.dtype(
     [
         ("gravity", F, 3), ("length_unit", F), ("substeps", np.int32),
         ("contact_hertz", F), ("contact_damping_ratio", F), ("max_pushout_speed", F),
         ("max_linear_speed", F), ("max_angular_speed", F), ("restitution_threshold", F),
         ("sleep_threshold", F), ("time_to_sleep", F), ("speculative", F), ("hit_threshold", F),
-        ("contact_reuse_distance", F), ("contact_reuse_angle", F),
+        ("contact_reuse_distance", F), ("contact_reuse_angle", F), ("joint_hertz", F), ("joint_damping_ratio", F),
     ],
     align=True,
 )
@@ -104,6 +122,8 @@ def default_settings(**overrides):
     s["hit_threshold"] = 1.0
     s["contact_reuse_distance"] = 0.0005
     s["contact_reuse_angle"] = 0.035
+    s["joint_hertz"] = 60.0
+    s["joint_damping_ratio"] = 2.0
     for k, v in overrides.items():
         s[k] = v
     return s
@@ -113,7 +133,9 @@ def check_sizes(lib, prefix="tfs"):
     """Compare the numpy layouts with the C sizes a library reports."""
     import ctypes
 
-    for name, dt in (("body", BODY_DTYPE), ("contact", CONTACT_DTYPE)):
+    for name, dt in (("body", BODY_DTYPE), ("contact", CONTACT_DTYPE), ("joint", JOINT_DTYPE)):
+        if not hasattr(lib, f"{prefix}_sizeof_{name}"):
+            continue
         fn = getattr(lib, f"{prefix}_sizeof_{name}")
         fn.restype = ctypes.c_size_t
         size = fn()

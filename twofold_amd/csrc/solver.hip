@@ -31,6 +31,7 @@
 
 #include "stages.h"
 #include "collide3d.h"
+#include "joints.h"
 
 using namespace tf;
 
@@ -64,6 +65,7 @@ struct DevCounts {
     uint32_t changed[kIslandRounds];
     uint32_t nTouchEvents, nHits, touching;   // reportTouches of the step
     uint32_t streamPairs, streamRest;         // contacts the stream's pairs and resting contacts leave
+    uint32_t nJointsActive, nJointsBegun;     // active joints now (after breaking), and at the step's start
     uint32_t colorStart[TFS_COLORS + 1];
     uint32_t impactColorStart[TFS_COLORS + 1];
     uint32_t colorPoints[TFS_COLORS + 1];
@@ -2288,12 +2290,13 @@ __global__ void __launch_bounds__(kThreads)
                     const uint32_t* __restrict__ start, const uint32_t* __restrict__ ep, Rows rows,
                     const uint32_t* __restrict__ restDeg, const uint32_t* __restrict__ restStart,
                     const uint32_t* __restrict__ restEp, const uint4* __restrict__ entries, RestPack rp,
-                    const float4* __restrict__ mass) {
+                    const float4* __restrict__ mass, int parts) {
+    // parts: 1 integrateVelocities, 2 the warm start, 3 both (with joints, their warm start goes between)
     for (uint32_t x = gid(); x < nb; x += gstride()) {
         tfs_body& body = bodies[x];
-        const bool integrate = body.alive && body.type == TFS_DYNAMIC && body.awake;
-        const uint32_t d = deg[x];
-        const uint32_t dr = REST ? restDeg[x] : 0u;
+        const bool integrate = (parts & 1) && body.alive && body.type == TFS_DYNAMIC && body.awake;
+        const uint32_t d = (parts & 2) ? deg[x] : 0u;
+        const uint32_t dr = REST && (parts & 2) ? restDeg[x] : 0u;
         if (!integrate && d == 0 && dr == 0) { continue; }
         Motion m = loadMotion(motion, x);
         if (integrate) {
@@ -2999,6 +3002,184 @@ __global__ void k_sleep_jump(uint32_t* parent, uint32_t n, const uint32_t* prev)
     for (uint32_t i = gid(); i < n; i += gstride()) { parent[i] = rootOf(parent, i); }
 }
 
+// The joints' part of the island pass (src/world.cpp:1210-1223), after the contacts' edges: a joint that
+// is alive and not broken joins two awake dynamic bodies, or keeps a dynamic body awake beside a moving
+// kinematic one. The world body is static.
+__global__ void k_sleep_joints(const tfs_joint* joints, uint32_t nj, tfs_body* bodies, uint2* edges) {
+    for (uint32_t k = gid(); k < nj; k += gstride()) {
+        const tfs_joint& j = joints[k];
+        edges[k] = make_uint2(0u, 0u);
+        if (!j.alive || j.broken) { continue; }
+        const bool worldA = j.body_a == TFS_WORLD_BODY;
+        tfs_body& b = bodies[j.body_b];
+        const bool da = !worldA && awakeDynamic(bodies[j.body_a]), db = awakeDynamic(b);
+        if (da && db) {
+            edges[k] = make_uint2(j.body_a, j.body_b);
+            continue;
+        }
+        if (da && b.type == TFS_KINEMATIC && movingAwake(b, ld3(b.v), ld3(b.w))) {
+            bodies[j.body_a].sleep_time = 0.0f;
+        } else if (db && !worldA && bodies[j.body_a].type == TFS_KINEMATIC) {
+            tfs_body& a = bodies[j.body_a];
+            if (movingAwake(a, ld3(a.v), ld3(a.w))) { b.sleep_time = 0.0f; }
+        }
+    }
+}
+
+// ---- joints: src/world.cpp:1623-1632, 1647, 1670-1678, 1685-1691; src/joint.cpp ------------------------
+//
+// The reference solves its joints one at a time, in order, before the contacts of every pass, so each
+// kernel here is one thread walking the active joints; a joint's two ends are loaded, solved and stored
+// before the next joint reads them.
+
+__device__ tfj::End loadEnd(uint32_t i, const tfs_body* bodies, const float4* motion, const float4* world) {
+    tfj::End e;
+    if (i == TFS_WORLD_BODY) {
+        e.v = vec(world[0].x, world[0].y, world[0].z);
+        e.w = vec(world[1].x, world[1].y, world[1].z);
+        e.com = e.lc = e.p = vec(0.0f, 0.0f, 0.0f);
+        e.q = quat(0.0f, 0.0f, 0.0f, 1.0f);
+        e.iMW = e.iIW = tf::mzero();
+        e.invMass = 0.0f;
+        e.locks = 0;
+        e.type = TFS_STATIC;
+        e.awake = 1;
+        return e;
+    }
+    const tfs_body& b = bodies[i];
+    const float4 v = motion[4 * i], w = motion[4 * i + 1];
+    e.v = vec(v.x, v.y, v.z);
+    e.w = vec(w.x, w.y, w.z);
+    e.com = ld3(b.com);
+    e.lc = ld3(b.local_center);
+    e.p = ld3(b.p);
+    e.q = ld4(b.q);
+    e.iMW = tf::ld9(b.inv_mass_world);
+    e.iIW = tf::ld9(b.inv_inertia_world);
+    e.invMass = b.inv_mass;
+    e.locks = b.locks;
+    e.type = b.type;
+    e.awake = b.awake;
+    return e;
+}
+__device__ void storeEndVelocity(uint32_t i, const tfj::End& e, float4* motion, float4* world) {
+    float4* m = i == TFS_WORLD_BODY ? world : motion + 4 * (size_t)i;
+    m[0] = make_float4(e.v.x, e.v.y, e.v.z, 0.0f);
+    m[1] = make_float4(e.w.x, e.w.y, e.w.z, 0.0f);
+}
+__device__ void storeEndPose(uint32_t i, const tfj::End& e, tfs_body* bodies) {
+    if (i == TFS_WORLD_BODY) { return; }
+    tfs_body& b = bodies[i];
+    st3(b.com, e.com);
+    st4(b.q, e.q);
+    st3(b.p, e.p);
+}
+__device__ __forceinline__ bool jointEndAwake(uint32_t i, const tfs_body* bodies) {
+    return i != TFS_WORLD_BODY && bodies[i].type == TFS_DYNAMIC && bodies[i].awake;
+}
+
+// The step's active joints (alive, not broken, with an awake dynamic end), their force and torque cleared.
+__global__ void k_joints_begin(tfs_joint* joints, uint32_t nj, const tfs_body* bodies, uint32_t* active, DevCounts* cnt) {
+    if (gid() != 0) { return; }
+    uint32_t n = 0;
+    for (uint32_t k = 0; k < nj; ++k) {
+        tfs_joint& j = joints[k];
+        if (!j.alive || j.broken) { continue; }
+        if (!jointEndAwake(j.body_a, bodies) && !jointEndAwake(j.body_b, bodies)) { continue; }
+        active[n++] = k;
+        st3(j.force, vec(0.0f, 0.0f, 0.0f));
+        st3(j.torque, vec(0.0f, 0.0f, 0.0f));
+    }
+    cnt->nJointsActive = n;
+    cnt->nJointsBegun = n;
+}
+
+// A substep's joint start, after integrateVelocities: scaleToKeptSpeed, then warmStartJoints.
+__global__ void k_joints_warm(tfs_joint* joints, const uint32_t* active, const DevCounts* cnt, tfs_body* bodies,
+                              float4* motion, float4* world) {
+    if (gid() != 0) { return; }
+    const uint32_t n = cnt->nJointsActive;
+    auto kept = [&](uint32_t i) { return i == TFS_WORLD_BODY ? 1.0f : bodies[i].kept; };
+    for (uint32_t k = 0; k < n; ++k) {
+        tfs_joint& j = joints[active[k]];
+        const float f = minf(kept(j.body_a), kept(j.body_b));
+        if (f >= 1.0f) { continue; }
+        tfj::scaleImpulses(j, f);
+    }
+    for (uint32_t k = 0; k < n; ++k) {
+        const tfs_joint& j = joints[active[k]];
+        if (j.body_a != TFS_WORLD_BODY) { bodies[j.body_a].kept = 1.0f; }
+        if (j.body_b != TFS_WORLD_BODY) { bodies[j.body_b].kept = 1.0f; }
+    }
+    for (uint32_t k = 0; k < n; ++k) {
+        const tfs_joint& j = joints[active[k]];
+        tfj::End a = loadEnd(j.body_a, bodies, motion, world), b = loadEnd(j.body_b, bodies, motion, world);
+        tfj::warmStartJoint(a, b, j);
+        storeEndVelocity(j.body_a, a, motion, world);
+        storeEndVelocity(j.body_b, b, motion, world);
+    }
+}
+
+// solveJoints (src/joint.cpp:584-590): biased or relaxed, the relax pass adding each joint's reaction.
+__global__ void k_joints_solve(tfs_joint* joints, const uint32_t* active, const DevCounts* cnt, const tfs_body* bodies,
+                               float4* motion, float4* world, tfj::JCtx jc, int useBias) {
+    if (gid() != 0) { return; }
+    const uint32_t n = cnt->nJointsActive;
+    for (uint32_t k = 0; k < n; ++k) {
+        tfs_joint& j = joints[active[k]];
+        tfj::End a = loadEnd(j.body_a, bodies, motion, world), b = loadEnd(j.body_b, bodies, motion, world);
+        tfj::solveJoint(a, b, j, jc, useBias != 0);
+        if (!useBias) { tfj::addReaction(j); }
+        storeEndVelocity(j.body_a, a, motion, world);
+        storeEndVelocity(j.body_b, b, motion, world);
+    }
+}
+
+// After the final speed limits: breakJoints (src/world.cpp:1545-1563), then settleJoints over the joints
+// left (src/joint.cpp:544-553, kSettleIterations = 2 rounds, src/world.cpp:16).
+__global__ void k_joints_end(tfs_joint* joints, uint32_t* active, DevCounts* cnt, tfs_body* bodies,
+                             const float4* motion, const float4* world, float dt) {
+    if (gid() != 0) { return; }
+    const uint32_t n = cnt->nJointsActive;
+    uint32_t kept = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t ji = active[k];
+        tfs_joint& j = joints[ji];
+        const V3 force = tfj::vdiv(ld3(j.force), dt), torque = tfj::vdiv(ld3(j.torque), dt);
+        st3(j.force, force);
+        st3(j.torque, torque);
+        if (length(force) > j.break_force || length(torque) > j.break_torque) {
+            j.broken = 1;
+            st3(j.linear_impulse, vec(0.0f, 0.0f, 0.0f));
+            st3(j.angular_impulse, vec(0.0f, 0.0f, 0.0f));
+            st3(j.friction_turn, vec(0.0f, 0.0f, 0.0f));
+            j.axial_impulse = j.motor_impulse = j.lower_impulse = j.upper_impulse = 0.0f;
+            j.friction_impulse = j.swing_impulse = 0.0f;
+            continue;
+        }
+        active[kept++] = ji;
+    }
+    cnt->nJointsActive = kept;
+    for (int round = 0; round < 2; ++round) {
+        for (uint32_t k = 0; k < kept; ++k) {
+            const tfs_joint& j = joints[active[k]];
+            tfj::End a = loadEnd(j.body_a, bodies, motion, world), b = loadEnd(j.body_b, bodies, motion, world);
+            tfj::settleJoint(a, b, j);
+            storeEndPose(j.body_a, a, bodies);
+            storeEndPose(j.body_b, b, bodies);
+        }
+    }
+}
+
+// Then, when the step had active joints, every awake dynamic body's inverse mass (src/world.cpp:1688-1690).
+__global__ void k_joints_mass(tfs_body* bodies, uint32_t nb, const DevCounts* cnt) {
+    if (cnt->nJointsBegun == 0) { return; }
+    for (uint32_t i = gid(); i < nb; i += gstride()) {
+        tfs_body& b = bodies[i];
+        if (b.alive && b.type == TFS_DYNAMIC && b.awake) { updateInverseMass(b); }
+    }
+}
+
 // Island ids are the lowest body index of the island; with several worlds, local to the body's world.
 // Given `unsettled`, both run only if that round linked nothing (*unsettled == 0).
 __global__ void k_island_time(tfs_body* bodies, uint32_t n, const uint32_t* parent, uint32_t* islandTime,
@@ -3139,7 +3320,12 @@ struct tfs_ctx {
     uint32_t* impact = nullptr;
     uint32_t* rowsList = nullptr;
     uint32_t* impactRowsList = nullptr;
-    uint2* edges = nullptr;
+    uint2* edges = nullptr;               // sleep islands: one edge per contact, then one per joint
+    // joints (tfs_upload_joints): the records, the active list of the step, the world body's v and w
+    tfs_joint* joints = nullptr;
+    uint32_t* jointActive = nullptr;
+    float4* worldMotion = nullptr;
+    uint32_t nJoints = 0, capJ = 0;
     // the rest lists in levels (active rest, impact rest): scratch, and per list its order and level starts
     uint8_t* restTracked = nullptr;
     uint32_t *restDeg = nullptr, *restStart = nullptr, *restCursor = nullptr, *restEp = nullptr, *restRank = nullptr,
@@ -3344,7 +3530,7 @@ int allocContacts(tfs_ctx* ctx, uint32_t cap) {
         (r = grow(ctx, ctx->impact, cap)) || (r = grow(ctx, ctx->rowsList, cap)) ||
         (r = grow(ctx, ctx->impactRowsList, cap)) || (r = grow(ctx, ctx->colors, cap)) ||
         (r = grow(ctx, ctx->ep, 2 * (size_t)cap)) || (r = grow(ctx, ctx->rank, 2 * (size_t)cap)) ||
-        (r = grow(ctx, ctx->rowOf, cap)) || (r = grow(ctx, ctx->edges, cap)) ||
+        (r = grow(ctx, ctx->rowOf, cap)) || (r = grow(ctx, ctx->edges, (size_t)cap + ctx->capJ)) ||
         (r = grow(ctx, ctx->restEp, 2 * (size_t)cap)) || (r = grow(ctx, ctx->restRank, 2 * (size_t)cap)) ||
         (r = grow(ctx, ctx->restLevel, cap)) || (r = grow(ctx, ctx->restCount, cap)) ||
         (r = grow(ctx, ctx->restEntries[0], cap)) || (r = grow(ctx, ctx->restEntries[1], cap)) ||
@@ -3669,9 +3855,10 @@ RestPack restPack(const tfs_ctx* ctx) {
 
 // The start of a substep: velocities integrated and the warm start of every row, coloured and rest,
 // one launch over the bodies (k_substep_start). Profile family 0 times it, with the warm start's bytes.
-int substepStart(tfs_ctx* ctx, const Ctx& c, uint32_t laned, uint32_t points, bool rest) {
+int substepStart(tfs_ctx* ctx, const Ctx& c, uint32_t laned, uint32_t points, bool rest, int parts) {
     size_t ev = 0;
-    if (ctx->profiling) {
+    const bool timed = ctx->profiling && (parts & 2);
+    if (timed) {
         ev = ctx->pending.size() * 2;
         while (ctx->evPool.size() < ev + 2) {
             hipEvent_t x;
@@ -3685,12 +3872,12 @@ int substepStart(tfs_ctx* ctx, const Ctx& c, uint32_t laned, uint32_t points, bo
     if (rest) {
         LAUNCH(k_substep_start<true>, blocksFor(nb, kBodyThreads), kBodyThreads, ctx->bodies, nb, ctx->motion, c, ctx->bdeg, ctx->bstart,
                ctx->bep, ctx->rows, ctx->wrestDeg, ctx->wrestStart, ctx->wrestEp, ctx->restEntries[0], restPack(ctx),
-               ctx->mass);
+               ctx->mass, parts);
     } else {
         LAUNCH(k_substep_start<false>, blocksFor(nb, kBodyThreads), kBodyThreads, ctx->bodies, nb, ctx->motion, c, ctx->bdeg, ctx->bstart,
-               ctx->bep, ctx->rows, nullptr, nullptr, nullptr, nullptr, RestPack{}, ctx->mass);
+               ctx->bep, ctx->rows, nullptr, nullptr, nullptr, nullptr, RestPack{}, ctx->mass, parts);
     }
-    if (ctx->profiling) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
+    if (timed) { HIP_TRY(hipEventRecord(ctx->evPool[ev + 1], ctx->stream)); }
     return 0;
 }
 
@@ -4013,13 +4200,17 @@ int launchSleep(tfs_ctx* ctx, float dt, int& it) {
     LAUNCH(k_sleep_time, blocksFor(nb), kThreads, ctx->bodies, nb, dt, s.sleep_threshold * s.length_unit, ctx->parent,
            ctx->islandTime);
     LAUNCH(k_sleep_kinematic, blocksFor(nc), kThreads, ctx->contacts, nc, ctx->bodies, ctx->edges);
+    if (ctx->nJoints) {
+        LAUNCH(k_sleep_joints, blocksFor(ctx->nJoints), kThreads, ctx->joints, ctx->nJoints, ctx->bodies, ctx->edges + nc);
+    }
+    const uint32_t ne = nc + ctx->nJoints;
     const bool worlds = ctx->nWorlds > 1;
     const uint32_t* bw = worlds ? ctx->bodyWorld : nullptr;
     const uint32_t* ws = worlds ? ctx->worldStart : nullptr;
     const int ahead = ctx->islandAhead;
     for (; it < ahead; ++it) {
         const uint32_t* prev = it >= 4 ? &ctx->cnt->changed[it - 1] : nullptr;
-        LAUNCH(k_sleep_hook, blocksFor(nc), kThreads, ctx->edges, nc, ctx->parent, &ctx->cnt->changed[it], prev);
+        LAUNCH(k_sleep_hook, blocksFor(ne), kThreads, ctx->edges, ne, ctx->parent, &ctx->cnt->changed[it], prev);
         LAUNCH(k_sleep_jump, blocksFor(nb), kThreads, ctx->parent, nb, prev);
     }
     const uint32_t* last = &ctx->cnt->changed[ahead - 1];
@@ -4034,7 +4225,7 @@ int launchSleep(tfs_ctx* ctx, float dt, int& it) {
 // The rare step whose islands need more rounds than launchSleep ran (seen once its flag reached the
 // host): two rounds at a time, checking each pair, then the island times and the sleep.
 int finishSleep(tfs_ctx* ctx, int it, bool timed) {
-    const uint32_t nb = ctx->nb, nc = ctx->nc;
+    const uint32_t nb = ctx->nb, ne = ctx->nc + ctx->nJoints;
     const tfs_settings& s = ctx->settings;
     const bool worlds = ctx->nWorlds > 1;
     const uint32_t* bw = worlds ? ctx->bodyWorld : nullptr;
@@ -4046,7 +4237,7 @@ int finishSleep(tfs_ctx* ctx, int it, bool timed) {
                 return fail(ctx, TFS_E_KERNEL, "the island pass did not settle in %d rounds", kIslandRounds);
             }
             for (int j = 0; j < 2; ++j, ++it) {
-                LAUNCH(k_sleep_hook, blocksFor(nc), kThreads, ctx->edges, nc, ctx->parent, &ctx->cnt->changed[it], nullptr);
+                LAUNCH(k_sleep_hook, blocksFor(ne), kThreads, ctx->edges, ne, ctx->parent, &ctx->cnt->changed[it], nullptr);
                 LAUNCH(k_sleep_jump, blocksFor(nb), kThreads, ctx->parent, nb, nullptr);
             }
             HIP_TRY(hipMemcpyAsync(&ctx->hostCnt->changed[it - 1], &ctx->cnt->changed[it - 1], sizeof(uint32_t),
@@ -4132,10 +4323,15 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
 
     // substeps: src/world.cpp:1658-1681. A substep with no rest list may run as one fused launch; with
     // no host callback to wait for, the whole loop may go out as one HIP graph.
-    const bool fuse = ctx->fused && ctx->usePass && hc.nActive == laned && hc.nImpact == impactLaned;
+    // With joints each substep's start is split around their warm start, and their solves go before the
+    // contacts' in every pass: no fused substep and no graph.
+    const bool joints = ctx->nJoints > 0;
+    tfj::JCtx jc{ c.h, c.invH, makeSoft(minf(s.joint_hertz, 0.5f * c.invH), s.joint_damping_ratio, c.h) };
+    if (joints) { LAUNCH(k_joints_begin, 1, 64, ctx->joints, ctx->nJoints, ctx->bodies, ctx->jointActive, ctx->cnt); }
+    const bool fuse = !joints && ctx->fused && ctx->usePass && hc.nActive == laned && hc.nImpact == impactLaned;
     bool hostRefresh = false;
     for (int i = 1; i < n; ++i) { hostRefresh = hostRefresh || (!ctx->nShapes && (i == n / 2 ? hc.nFast : hc.nFastSolid) > 0); }
-    const bool graph = ctx->useGraph && timed && !ctx->profiling && !ctx->tuning && !ctx->coopLaunch && !hostRefresh;
+    const bool graph = !joints && ctx->useGraph && timed && !ctx->profiling && !ctx->tuning && !ctx->coopLaunch && !hostRefresh;
     if (n > 1 && ctx->nShapes && (hc.nFast > 0 || hc.nFastSolid > 0)) {
         LAUNCH(k_refresh_order, 2, 1024, ctx->fast, ctx->fastSolid, ctx->cnt, ctx->contacts, ctx->shapes, ctx->fastOrder,
                ctx->fastSolidOrder);
@@ -4152,11 +4348,22 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
                 if ((e = substepFused(ctx, c, hc))) { return e; }
                 continue;
             }
-            if ((e = substepStart(ctx, c, laned, hc.pointsActive, hc.nActive > laned)) ||
-                (e = solvePass(ctx, kBias, hc, laned, impactLaned, c))) {
+            if (joints) {
+                if ((e = substepStart(ctx, c, laned, hc.pointsActive, hc.nActive > laned, 1))) { return e; }
+                LAUNCH(k_joints_warm, 1, 64, ctx->joints, ctx->jointActive, ctx->cnt, ctx->bodies, ctx->motion,
+                       ctx->worldMotion);
+                if ((e = substepStart(ctx, c, laned, hc.pointsActive, hc.nActive > laned, 2))) { return e; }
+                LAUNCH(k_joints_solve, 1, 64, ctx->joints, ctx->jointActive, ctx->cnt, ctx->bodies, ctx->motion,
+                       ctx->worldMotion, jc, 1);
+            } else if ((e = substepStart(ctx, c, laned, hc.pointsActive, hc.nActive > laned, 3))) {
                 return e;
             }
+            if ((e = solvePass(ctx, kBias, hc, laned, impactLaned, c))) { return e; }
             LAUNCH(k_integrate_positions, blocksFor(nb, kBodyThreads), kBodyThreads, ctx->bodies, nb, ctx->motion, c);
+            if (joints) {
+                LAUNCH(k_joints_solve, 1, 64, ctx->joints, ctx->jointActive, ctx->cnt, ctx->bodies, ctx->motion,
+                       ctx->worldMotion, jc, 0);
+            }
             if ((e = solvePass(ctx, kRelax, hc, laned, impactLaned, c))) { return e; }
         }
         return 0;
@@ -4205,6 +4412,10 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
         return r;
     }
     LAUNCH(k_limit_speeds, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c);
+    if (joints) {
+        LAUNCH(k_joints_end, 1, 64, ctx->joints, ctx->jointActive, ctx->cnt, ctx->bodies, ctx->motion, ctx->worldMotion, dt);
+        LAUNCH(k_joints_mass, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->cnt);
+    }
     LAUNCH(k_unpack, blocksFor(laned), kThreads, laned, ctx->rows, ctx->contacts);
     if (hc.nActive > laned) {
         LAUNCH(k_rest_unpack, blocksFor(hc.nActive - laned), kThreads, ctx->restEntries[0], hc.nActive - laned,
@@ -4308,7 +4519,7 @@ int fixedStep(tfs_ctx* ctx, bool record) {
                             : runRows(ctx, mode, nullptr, hc.colorStart, hc.colorPoints, c, t);
     };
     for (int i = 0; i < n; ++i) {
-        if ((r = substepStart(ctx, c, laned, hc.pointsActive, false)) || (r = pass(kBias, 2 * i))) { return r; }
+        if ((r = substepStart(ctx, c, laned, hc.pointsActive, false, 3)) || (r = pass(kBias, 2 * i))) { return r; }
         LAUNCH(k_fx_positions, blocksFor(nb), kThreads, ctx->bodies, nb, ctx->motion, c.h);
         if ((r = pass(kRelax, 2 * i + 1))) { return r; }
     }
@@ -4561,6 +4772,7 @@ int tfs_check_minmax(int device, const float* a, const float* b, uint32_t n, flo
 size_t tfs_sizeof_body(void) { return sizeof(tfs_body); }
 size_t tfs_sizeof_contact(void) { return sizeof(tfs_contact); }
 size_t tfs_sizeof_settings(void) { return sizeof(tfs_settings); }
+size_t tfs_sizeof_joint(void) { return sizeof(tfs_joint); }
 
 void tfs_default_settings(tfs_settings* s) {
     if (!s) { return; }
@@ -4580,6 +4792,8 @@ void tfs_default_settings(tfs_settings* s) {
     s->hit_threshold = 1.0f;
     s->contact_reuse_distance = 0.0005f;
     s->contact_reuse_angle = 0.035f;
+    s->joint_hertz = 60.0f;
+    s->joint_damping_ratio = 2.0f;
 }
 
 int tfs_create(int device, uint32_t maxBodies, uint32_t maxContacts, tfs_ctx** out) {
@@ -4663,7 +4877,8 @@ void tfs_destroy(tfs_ctx* ctx) {
                      ctx->wrestStart, ctx->wrestCursor, ctx->wrestEp, ctx->touchFlag, ctx->hitFlag, ctx->touchList,
                      ctx->hitList, ctx->hitData, ctx->hits, ctx->ext, ctx->contactsNext, ctx->extNext, ctx->pairs,
                      ctx->pairLo, ctx->pairHi, ctx->pairKeep, ctx->pairStart, ctx->manifolds, ctx->ckBody, ctx->ckRow, ctx->tapeBuf, ctx->adj.bar, ctx->adj.dpBar,
-                     ctx->adj.comBar, ctx->adj.impBar, ctx->adj.sepBar, ctx->adj.muBar, ctx->dlcom, ctx->muContact };
+                     ctx->adj.comBar, ctx->adj.impBar, ctx->adj.sepBar, ctx->adj.muBar, ctx->dlcom, ctx->muContact,
+                     ctx->fastOrder, ctx->fastSolidOrder, ctx->joints, ctx->jointActive, ctx->worldMotion };
     for (void* p : ptrs) {
         if (p) { (void)hipFree(p); }
     }
@@ -5025,6 +5240,56 @@ int tfs_set_body_shapes(tfs_ctx* ctx, uint32_t n, const tfs_shape* shapes) {
     }
     if (n) { HIP_TRY(hipMemcpy(ctx->shapes, shapes, sizeof(tfs_shape) * n, hipMemcpyHostToDevice)); }
     ctx->nShapes = n;
+    return 0;
+}
+
+int tfs_upload_joints(tfs_ctx* ctx, const tfs_joint* joints, uint32_t n, const float* world_motion) {
+    if (!ctx || (n && !joints)) { return TFS_E_ARG; }
+    if (ctx->arena) { return fail(ctx, TFS_E_STATE, "a context that may not allocate takes no joints"); }
+    for (uint32_t k = 0; k < n; ++k) {
+        const tfs_joint& j = joints[k];
+        if (j.body_b >= ctx->nb || (j.body_a != TFS_WORLD_BODY && (j.body_a >= ctx->nb || j.body_a == j.body_b))) {
+            return fail(ctx, TFS_E_ARG, "joint %u: bodies %u and %u out of range for %u bodies", k, j.body_a, j.body_b, ctx->nb);
+        }
+        if (j.alive && !tfj::supported(j)) {
+            return fail(ctx, TFS_E_ARG, "joint %u: a %s reads angles through atan2; the device does not solve it", k,
+                        j.type == TFS_JOINT_HINGE ? "limited hinge" : "swing-twist joint");
+        }
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) { return fail(ctx, TFS_E_DEVICE, "hipSetDevice failed"); }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    int r;
+    if (!ctx->worldMotion && (r = grow(ctx, ctx->worldMotion, 2))) { return r; }
+    if (n > ctx->capJ) {
+        if ((r = grow(ctx, ctx->joints, n)) || (r = grow(ctx, ctx->jointActive, n)) ||
+            (r = grow(ctx, ctx->edges, (size_t)ctx->capC + n))) {
+            return r;
+        }
+        ctx->capJ = n;
+    }
+    if (n) { HIP_TRY(hipMemcpy(ctx->joints, joints, sizeof(tfs_joint) * n, hipMemcpyHostToDevice)); }
+    float4 wm[2] = { make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f) };
+    if (world_motion) {
+        wm[0] = make_float4(world_motion[0], world_motion[1], world_motion[2], 0.0f);
+        wm[1] = make_float4(world_motion[3], world_motion[4], world_motion[5], 0.0f);
+    }
+    HIP_TRY(hipMemcpy(ctx->worldMotion, wm, sizeof wm, hipMemcpyHostToDevice));
+    ctx->nJoints = n;
+    return 0;
+}
+
+int tfs_download_joints(tfs_ctx* ctx, tfs_joint* out, uint32_t n, float* world_motion) {
+    if (!ctx || (n && !out)) { return TFS_E_ARG; }
+    if (n > ctx->nJoints) { return fail(ctx, TFS_E_ARG, "%u joints asked for, %u resident", n, ctx->nJoints); }
+    if (hipSetDevice(ctx->device) != hipSuccess) { return fail(ctx, TFS_E_DEVICE, "hipSetDevice failed"); }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (n) { HIP_TRY(hipMemcpy(out, ctx->joints, sizeof(tfs_joint) * n, hipMemcpyDeviceToHost)); }
+    if (world_motion) {
+        float4 wm[2] = { make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f) };
+        if (ctx->worldMotion) { HIP_TRY(hipMemcpy(wm, ctx->worldMotion, sizeof wm, hipMemcpyDeviceToHost)); }
+        const float v[6] = { wm[0].x, wm[0].y, wm[0].z, wm[1].x, wm[1].y, wm[1].z };
+        std::memcpy(world_motion, v, sizeof v);
+    }
     return 0;
 }
 

@@ -97,6 +97,7 @@ class Scene:
         self.steps = steps
         self.bodies = []
         self.shapes = []
+        self.joints = []
         # WorldSettings that differ from the defaults (tfs_settings field names)
         self.settings = dict(settings or {})
 
@@ -125,10 +126,106 @@ class Scene:
             restitution=f32(restitution), rolling=f32(rolling)))
         return len(self.shapes) - 1
 
+    def joint(self, kind, a, b, anchor=(0, 0, 0), anchor_b=(0, 0, 0), axis=(1, 0, 0), limited=False, lower=0.0,
+              upper=0.0, motor_speed=0.0, max_motor=0.0, friction=0.0, length=-1.0, hertz=0.0, damping_ratio=0.0,
+              break_force=math.inf, break_torque=math.inf, collide_connected=False):
+        """A joint as World::createJoint takes it (include/twofold/world.h:135-210): kind JOINT_*, body a
+        (None: the world), body b, world-space anchors and axis at the bodies' creation poses; the
+        hinge's and slider's limits, motor and friction, the distance joint's length (negative: the
+        distance the anchors start at), spring and limits (its min and max distance as lower, upper)."""
+        p = np.array([*anchor, *anchor_b, *axis, 1.0 if limited else 0.0, lower, upper, motor_speed, max_motor,
+                      friction, length, hertz, damping_ratio, break_force, break_torque,
+                      1.0 if collide_connected else 0.0], f32)
+        self.joints.append(dict(kind=kind, a=-1 if a is None else a, b=b, p=p))
+        return len(self.joints) - 1
+
     def ground(self, friction=0.6, restitution=0.0, rolling=0.0):
         g = self.body(STATIC, rot=UP)
         self.shape(g, HALF_SPACE, friction=friction, restitution=restitution, rolling=rolling)
         return g
+
+
+JOINT_POINT, JOINT_FIXED, JOINT_HINGE, JOINT_SLIDER, JOINT_DISTANCE = range(5)
+
+
+def chain(links=10, steps=240):
+    """A chain of boxes (half-extents 0.3 x 0.1 x 0.1) held end to end by point joints, its first link
+    held to the world at (0, 8, 0), laid out level along x so it swings down and folds onto a pile of
+    spheres and boxes on a half-space; a second chain hangs from a kinematic body moving along z."""
+    s = Scene("chain", substeps=4, steps=steps)
+    s.ground(0.6)
+    prev = None
+    for k in range(links):
+        x = f32(0.3) + f32(0.6) * f32(k)
+        b = s.body(pos=(x, 8.0, 0.0))
+        s.shape(b, BOX, (0.3, 0.1, 0.1), density=500.0)
+        anchor = (x - f32(0.3), 8.0, 0.0)
+        s.joint(JOINT_POINT, prev, b, anchor=anchor)
+        prev = b
+    for i in range(3):
+        for j in range(3):
+            b = s.body(pos=(1.0 + 1.1 * i, 0.6 + 0.0 * j, -1.1 + 1.1 * j))
+            if (i + j) % 2:
+                s.shape(b, SPHERE, (0.5,), density=300.0)
+            else:
+                s.shape(b, BOX, (0.5, 0.5, 0.5), density=300.0)
+    mover = s.body(KINEMATIC, pos=(-3.0, 6.0, -2.0), v=(0.0, 0.0, 1.0))
+    s.shape(mover, BOX, (0.2, 0.2, 0.2))
+    prev = mover
+    for k in range(4):
+        y = f32(5.6) - f32(0.8) * f32(k)
+        b = s.body(pos=(-3.0, y, -2.0))
+        s.shape(b, SPHERE, (0.2,), density=800.0)
+        s.joint(JOINT_POINT, prev, b, anchor=(-3.0, y + f32(0.4), -2.0))
+        prev = b
+    return s
+
+
+def joint_mix(steps=240):
+    """Every joint the device solves, each beside boxes and spheres that strike it: a bridge of planks
+    joined by hinges (unlimited, one with friction, one with a motor) whose ends are held to the world by
+    point joints; two boxes welded by a fixed joint tumbling onto it; a box on a rigid rope (distance
+    joint) from the world and a sphere on a sprung distance joint with limits; a slider along x with
+    limits, motor and friction carrying a box; and a fixed joint that breaks under a falling block."""
+    s = Scene("joint-mix", substeps=4, steps=steps)
+    s.ground(0.6)
+    planks = []
+    for k in range(6):
+        b = s.body(pos=(f32(-2.5) + f32(k), 3.0, 0.0))
+        s.shape(b, BOX, (0.45, 0.05, 0.5), density=400.0)
+        planks.append(b)
+    s.joint(JOINT_POINT, None, planks[0], anchor=(-3.0, 3.0, 0.0))
+    for k in range(5):
+        x = f32(-2.0) + f32(k)
+        s.joint(JOINT_HINGE, planks[k], planks[k + 1], anchor=(x, 3.0, 0.0), axis=(0.0, 0.0, 1.0),
+                friction=0.5 if k == 1 else 0.0, motor_speed=0.5 if k == 3 else 0.0, max_motor=2.0 if k == 3 else 0.0)
+    s.joint(JOINT_POINT, None, planks[-1], anchor=(3.0, 3.0, 0.0))
+    a = s.body(pos=(-0.5, 5.0, 0.0), w=(0.0, 0.0, 2.0))
+    s.shape(a, BOX, (0.25, 0.25, 0.25), density=600.0)
+    b = s.body(pos=(0.1, 5.0, 0.0))
+    s.shape(b, SPHERE, (0.25,), density=600.0)
+    s.joint(JOINT_FIXED, a, b, anchor=(-0.2, 5.0, 0.0))
+    rope = s.body(pos=(5.0, 4.0, 0.0), v=(0.0, 0.0, 2.0))
+    s.shape(rope, BOX, (0.3, 0.3, 0.3), density=500.0)
+    s.joint(JOINT_DISTANCE, None, rope, anchor=(5.0, 7.0, 0.0), anchor_b=(5.0, 4.3, 0.0))
+    bob = s.body(pos=(7.0, 4.0, 0.0), v=(1.0, 0.0, 0.0))
+    s.shape(bob, SPHERE, (0.3,), density=500.0)
+    s.joint(JOINT_DISTANCE, None, bob, anchor=(7.0, 6.0, 0.0), anchor_b=(7.0, 4.0, 0.0), length=1.5, hertz=3.0,
+            damping_ratio=0.3, limited=True, lower=1.0, upper=2.5)
+    base = s.body(STATIC, pos=(0.0, 0.25, 4.0))
+    s.shape(base, BOX, (3.0, 0.25, 0.5))
+    cart = s.body(pos=(0.0, 0.8, 4.0))
+    s.shape(cart, BOX, (0.3, 0.3, 0.3), density=400.0)
+    s.joint(JOINT_SLIDER, base, cart, anchor=(0.0, 0.8, 4.0), axis=(1.0, 0.0, 0.0), limited=True, lower=-1.5,
+            upper=1.0, motor_speed=1.0, max_motor=20.0, friction=1.0)
+    post = s.body(pos=(-5.0, 0.5, 3.0))
+    s.shape(post, BOX, (0.5, 0.5, 0.5), density=200.0)
+    top = s.body(pos=(-5.0, 1.5, 3.0))
+    s.shape(top, BOX, (0.5, 0.5, 0.5), density=200.0)
+    s.joint(JOINT_FIXED, post, top, anchor=(-5.0, 1.0, 3.0), break_force=400.0, break_torque=300.0)
+    block = s.body(pos=(-5.0, 5.0, 3.0), v=(0.0, -6.0, 0.0))
+    s.shape(block, BOX, (0.6, 0.6, 0.6), density=2000.0)
+    return s
 
 
 def pyramid(rows=20, density=1000.0, friction=0.6, spacing=1.02, steps=600):

@@ -15,12 +15,13 @@ import weakref
 import numpy as np
 
 from . import _abi
-from ._abi import BODY_DTYPE, CONTACT_DTYPE, EXT_DTYPE, HIT_DTYPE, MANIFOLD_DTYPE, PAIR_DTYPE, SETTINGS_DTYPE, SHAPE_DTYPE
+from ._abi import (BODY_DTYPE, CONTACT_DTYPE, EXT_DTYPE, HIT_DTYPE, JOINT_DTYPE, MANIFOLD_DTYPE, PAIR_DTYPE, SETTINGS_DTYPE,
+                   SHAPE_DTYPE)
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtwofold_hip.so")
 
 _vp = ctypes.c_void_p
-ABI_VERSION = 2  # TFS_ABI_VERSION of include/twofold_solver.h this binding mirrors
+ABI_VERSION = 3  # TFS_ABI_VERSION of include/twofold_solver.h this binding mirrors
 REFRESH_FN = ctypes.CFUNCTYPE(ctypes.c_int, _vp, ctypes.c_int, ctypes.c_float, ctypes.c_uint32,
                               ctypes.POINTER(ctypes.c_uint32), _vp, _vp, _vp)
 
@@ -31,7 +32,8 @@ EXPORTS = ("tfs_abi_version", "tfs_sizeof_body", "tfs_sizeof_contact", "tfs_size
            "tfs_fixed_step", "tfs_adjoint_seed", "tfs_adjoint_step", "tfs_adjoint_read", "tfs_check_minmax",
            "tfs_autotune", "tfs_pass_shape", "tfs_set_pass_shape", "tfs_set_cooperative", "tfs_set_body_shapes",
            "tfs_step_noalloc", "tfs_download_touches", "tfs_set_graph", "tfs_stream_contacts", "tfs_upload_contact_ext",
-           "tfs_download_contact_ext", "tfs_download_poses", "tfs_set_step_sleep", "tfs_sleep")
+           "tfs_download_contact_ext", "tfs_download_poses", "tfs_set_step_sleep", "tfs_sleep", "tfs_upload_joints",
+           "tfs_download_joints", "tfs_sizeof_joint")
 
 
 class SolverError(RuntimeError):
@@ -86,6 +88,8 @@ def load_library(path=LIB_PATH):
     lib.tfs_set_step_sleep.argtypes = [_vp, ctypes.c_int]
     lib.tfs_sleep.argtypes = [_vp, ctypes.c_float]
     lib.tfs_set_body_shapes.argtypes = [_vp, ctypes.c_uint32, _vp]
+    lib.tfs_upload_joints.argtypes = [_vp, _vp, ctypes.c_uint32, _vp]
+    lib.tfs_download_joints.argtypes = [_vp, _vp, ctypes.c_uint32, _vp]
     lib.tfs_download_touches.argtypes = [_vp, _vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), _vp,
                                          ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]
     lib.tfs_step_noalloc.argtypes = [ctypes.c_int, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32, _vp, _vp, ctypes.c_float,
@@ -420,6 +424,25 @@ class Solver:
             return
         shapes = np.ascontiguousarray(shapes, SHAPE_DTYPE)
         self._check(self.lib.tfs_set_body_shapes(self.h, len(shapes), shapes.ctypes.data_as(_vp)), "tfs_set_body_shapes")
+
+    def upload_joints(self, joints, world_motion=None):
+        """The joints (JOINT_DTYPE, the reference's order) and the world body's velocities (v, w; None for
+        zero); every step then solves them with the contacts (tfs_upload_joints). Upload the bodies
+        first."""
+        joints = np.ascontiguousarray(joints if joints is not None else np.zeros(0, JOINT_DTYPE), JOINT_DTYPE)
+        wm = None if world_motion is None else np.ascontiguousarray(world_motion, np.float32)
+        self._check(self.lib.tfs_upload_joints(self.h, joints.ctypes.data_as(_vp), len(joints),
+                                               None if wm is None else wm.ctypes.data_as(_vp)), "tfs_upload_joints")
+        self.nj = len(joints)
+
+    def joints(self):
+        """The resident joints and the world body's velocities (tfs_download_joints)."""
+        n = getattr(self, "nj", 0)
+        out = np.zeros(n, JOINT_DTYPE)
+        wm = np.zeros(6, np.float32)
+        self._check(self.lib.tfs_download_joints(self.h, out.ctypes.data_as(_vp), n, wm.ctypes.data_as(_vp)),
+                    "tfs_download_joints")
+        return out, wm
 
     def set_pass_shape(self, blocks, tail_rows, made_axes, fused=False):
         self._check(self.lib.tfs_set_pass_shape(self.h, blocks, tail_rows, int(bool(made_axes)), int(bool(fused))),
