@@ -15,9 +15,13 @@
 // bodies and contacts, runs tfs_step, writes the results back, empties the active and fast lists and
 // sets the bodies aside, so that the rest of the solve block's code (member functions over the bodies
 // and the lists) has nothing to do; the clock read that times the solve block (1692) puts the bodies
-// back. The third clock read after it (the sleep pass's start, 1723) uploads the bodies as the
-// continuous pass and the sensors left them, runs tfs_sleep, writes the results back and sets bodies
-// and contacts aside so that updateSleep has nothing to do; the next clock read (1725) puts them back.
+// back. Joints go up with the bodies, and come back into World::Impl::joints and the world body; while
+// the rest of the solve block runs (breakJoints over the step's active list) the world holds stand-ins,
+// and the real joints come back with the bodies, with the step's broken joints listed. The third clock
+// read after it (the sleep pass's start, 1723) uploads the bodies as the
+// continuous pass and the sensors left them, runs tfs_sleep, writes the results back and sets bodies,
+// contacts and joints aside so that updateSleep has nothing to do; the next clock read (1725) puts them
+// back.
 #define TFB_BOUND 1
 #include "ref_capture.cpp"
 
@@ -33,6 +37,8 @@ struct BoundState {
     std::string error;
     std::vector<Body> bodies;
     std::vector<Contact> contacts;
+    std::vector<Joint> joints;
+    std::vector<uint32_t> broken;  // the joints the step broke, in index order
 };
 BoundState g_b;
 
@@ -105,6 +111,26 @@ void fromExt(Contact& c, const tfs_contact_ext& o) {
     c.reusable = o.reusable != 0;
 }
 
+// What the solve block changes of a joint.
+void fromJoint(Joint& j, const tfs_joint& o) {
+    j.linearImpulse = v3(o.linear_impulse);
+    j.angularImpulse = v3(o.angular_impulse);
+    j.axialImpulse = o.axial_impulse;
+    j.motorImpulse = o.motor_impulse;
+    j.lowerImpulse = o.lower_impulse;
+    j.upperImpulse = o.upper_impulse;
+    j.frictionImpulse = o.friction_impulse;
+    j.swingImpulse = o.swing_impulse;
+    j.frictionTurn = v3(o.friction_turn);
+    for (int k = 0; k < 2; ++k) {
+        j.rowForce[k] = v3(o.row_force[k]);
+        j.rowTorque[k] = v3(o.row_torque[k]);
+    }
+    j.force = v3(o.force);
+    j.torque = v3(o.torque);
+    j.broken = o.broken != 0;
+}
+
 tfs_settings settingsOf(const World::Impl& w) {
     tfs_settings o;
     std::memset(&o, 0, sizeof o);
@@ -124,6 +150,8 @@ tfs_settings settingsOf(const World::Impl& w) {
     o.hit_threshold = s.hitThreshold;
     o.contact_reuse_distance = s.contactReuseDistance;
     o.contact_reuse_angle = s.contactReuseAngle;
+    o.joint_hertz = s.jointHertz;
+    o.joint_damping_ratio = s.jointDampingRatio;
     return o;
 }
 
@@ -208,12 +236,35 @@ bool upload(World::Impl& w) {
         contacts[i] = toContact(w.contacts[i]);
         ext[i] = toExt(w.contacts[i]);
     }
+    const uint32_t nj = (uint32_t)w.joints.size();
+    std::vector<tfs_joint> joints(nj);
+    for (uint32_t i = 0; i < nj; ++i) { joints[i] = toJoint(w.joints[i]); }
+    float world[6];
+    put3(world, w.worldBody.v);
+    put3(world + 3, w.worldBody.w);
     const tfs_settings s = settingsOf(w);
     return check(tfs_set_settings(g_b.ctx, &s), "tfs_set_settings") &&
            check(tfs_set_step_sleep(g_b.ctx, 0), "tfs_set_step_sleep") &&
            check(tfs_upload_bodies(g_b.ctx, bodies.data(), nb), "tfs_upload_bodies") &&
            check(tfs_upload_contacts(g_b.ctx, contacts.data(), nc), "tfs_upload_contacts") &&
-           check(tfs_upload_contact_ext(g_b.ctx, ext.data(), nc), "tfs_upload_contact_ext");
+           check(tfs_upload_contact_ext(g_b.ctx, ext.data(), nc), "tfs_upload_contact_ext") &&
+           check(tfs_upload_joints(g_b.ctx, joints.data(), nj, world), "tfs_upload_joints");
+}
+
+// The joints and the world body's velocities after the solve block, and which joints it broke.
+bool downloadJoints(World::Impl& w) {
+    const uint32_t nj = (uint32_t)w.joints.size();
+    std::vector<tfs_joint> joints(nj);
+    float world[6];
+    if (!check(tfs_download_joints(g_b.ctx, joints.data(), nj, world), "tfs_download_joints")) { return false; }
+    g_b.broken.clear();
+    for (uint32_t i = 0; i < nj; ++i) {
+        if (joints[i].broken && !w.joints[i].broken) { g_b.broken.push_back(i); }
+        fromJoint(w.joints[i], joints[i]);
+    }
+    w.worldBody.v = v3(world);
+    w.worldBody.w = v3(world + 3);
+    return true;
 }
 
 bool downloadBodies(World::Impl& w) {
@@ -227,9 +278,12 @@ void tick() {
     World::Impl* w = g_impl;
     if (!w) { return; }
     switch (g_b.phase) {
-        case 1:  // the solve block's time is read (src/world.cpp:1692): its bodies come back
+        case 1:  // the solve block's time is read (src/world.cpp:1692): its bodies and joints come back
             w->bodies.swap(g_b.bodies);
             g_b.bodies.clear();
+            w->joints.swap(g_b.joints);
+            g_b.joints.clear();
+            w->brokenJoints = g_b.broken;
             g_b.phase = 2;
             g_b.ticks = 0;
             break;
@@ -238,13 +292,16 @@ void tick() {
             if (g_b.error.empty() && upload(*w) && check(tfs_sleep(g_b.ctx, g_dt), "tfs_sleep")) { downloadBodies(*w); }
             g_b.bodies.swap(w->bodies);
             g_b.contacts.swap(w->contacts);
+            g_b.joints.swap(w->joints);
             g_b.phase = 3;
             break;
-        case 3:  // the sleep pass's time (1725): bodies and contacts come back
+        case 3:  // the sleep pass's time (1725): bodies, contacts and joints come back
             w->bodies.swap(g_b.bodies);
             w->contacts.swap(g_b.contacts);
+            w->joints.swap(g_b.joints);
             g_b.bodies.clear();
             g_b.contacts.clear();
+            g_b.joints.clear();
             g_b.phase = 0;
             break;
         default:
@@ -279,11 +336,14 @@ void tfbSolveBlock() {
                 fromExt(w.contacts[i], ext[i]);
             }
         }
+        downloadJoints(w);
     }
     w.activeContacts.clear();
     w.fastContacts.clear();
     w.fastSolidContacts.clear();
     g_b.bodies.swap(w.bodies);
+    g_b.joints.swap(w.joints);
+    w.joints.assign(g_b.joints.size(), Joint{});
     g_b.phase = 1;
 }
 

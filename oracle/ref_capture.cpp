@@ -53,6 +53,9 @@ struct twofold_step_clock {
 #define solveBatches tfcap_solveBatches
 #define restituteBatches tfcap_restituteBatches
 #define makeContactAxes tfcap_makeContactAxes
+#define scaleToKeptSpeed tfcap_scaleToKeptSpeed
+#define warmStartJoints tfcap_warmStartJoints
+#define solveJoints tfcap_solveJoints
 #endif
 
 #define prepareContacts tfcap_prepareContacts
@@ -80,6 +83,9 @@ struct twofold_step_clock {
 #undef solveBatches
 #undef restituteBatches
 #undef makeContactAxes
+#undef scaleToKeptSpeed
+#undef warmStartJoints
+#undef solveJoints
 #endif
 
 #include "../include/twofold_solver.h"
@@ -102,6 +108,9 @@ void warmStart(SolverData& s, const std::vector<uint32_t>& active);
 void solveBatches(SolverData& s, const ContactBatches& batches, const SolverContext& ctx, bool useBias);
 void restituteBatches(SolverData& s, const ContactBatches& batches, const SolverContext& ctx);
 void makeContactAxes(const Body& a, const Body& b, const Contact& c, PointAxes* axes);
+void scaleToKeptSpeed(SolverData& s, const std::vector<uint32_t>& active);
+void warmStartJoints(SolverData& s, const std::vector<uint32_t>& active);
+void solveJoints(SolverData& s, const std::vector<uint32_t>& active, const SolverContext& ctx, bool useBias);
 #endif
 }
 
@@ -495,6 +504,7 @@ void tfcap_limitSpeed(Body& b, float maxSpeed, float maxSpin) {
 // With active joints the solve block ends after settleJoints and the inverse masses made again
 // (src/world.cpp:1687-1690); they are made here too (the same values again) to record that end.
 void tfcap_settleJoints(SolverData& s, const std::vector<uint32_t>& active, int iterations) {
+    if (tfbBound()) { return; }
     settleJoints(s, active, iterations);
     if (!g_cap || tfbBound()) { return; }
     Capture& c = *g_cap;
@@ -519,6 +529,15 @@ void tfcap_restituteBatches(SolverData& s, const ContactBatches& batches, const 
 }
 void tfcap_makeContactAxes(const Body& a, const Body& b, const Contact& c, PointAxes* axes) {
     if (!tfbBound()) { makeContactAxes(a, b, c, axes); }
+}
+void tfcap_scaleToKeptSpeed(SolverData& s, const std::vector<uint32_t>& active) {
+    if (!tfbBound()) { scaleToKeptSpeed(s, active); }
+}
+void tfcap_warmStartJoints(SolverData& s, const std::vector<uint32_t>& active) {
+    if (!tfbBound()) { warmStartJoints(s, active); }
+}
+void tfcap_solveJoints(SolverData& s, const std::vector<uint32_t>& active, const SolverContext& ctx, bool useBias) {
+    if (!tfbBound()) { solveJoints(s, active, ctx, useBias); }
 }
 #endif
 

@@ -3,8 +3,8 @@ World::step runs its solve block and its sleep pass on the GPU through the C ABI
 sleep pass off, then tfs_sleep where the reference calls updateSleep, after the continuous pass and the
 sensors) and its mid-step refresh through the refresh callback with the reference's own narrowphase,
 stepped side by side with the unmodified reference from the same scene. After every step the two
-worlds' bodies and contacts (every field the ABI carries), and the step's hit events, are equal bit
-for bit: the binding changes nothing a user of World can observe."""
+worlds' bodies, contacts and joints (every field the ABI carries), and the step's hit events, are equal
+bit for bit: the binding changes nothing a user of World can observe."""
 import os
 
 import numpy as np
@@ -24,6 +24,8 @@ SCENES = {
     "edges": lambda: scenes.edges(),
     "planar-mix": lambda: scenes.planar_mix(),
     "rolling": lambda: scenes.rolling(),
+    "chain": lambda: scenes.chain(),
+    "joint-mix": lambda: scenes.joint_mix(),
 }
 
 
@@ -47,10 +49,14 @@ def test_bound_world_steps_as_the_reference(name):
             if pc.tobytes() != bc.tobytes():
                 bad = [f for f in pc.dtype.names if pc[f].tobytes() != bc[f].tobytes()]
                 pytest.fail(f"{name} step {k}: contact fields {bad} differ ({len(pc)} / {len(bc)} contacts)")
+            pj, bj = plain.joints(), bound.joints()
+            if pj.tobytes() != bj.tobytes():
+                bad = [f for f in pj.dtype.names if pj[f].tobytes() != bj[f].tobytes()]
+                pytest.fail(f"{name} step {k}: joint fields {bad} differ")
             ph, bh = plain.hits(), bound.hits()
             assert ph.tobytes() == bh.tobytes(), f"{name} step {k}: hit events differ"
             asleep = int(np.sum((pb["type"] == 2) & (pb["awake"] == 0)))
-        print(f"{name}: {scene.steps} steps through World::step, bodies and contacts equal every step; "
+        print(f"{name}: {scene.steps} steps through World::step, bodies, contacts and joints equal every step; "
               f"{asleep} bodies asleep at the end", flush=True)
     finally:
         del plain, bound
