@@ -691,8 +691,9 @@ def run_pile(args, planar=False):
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic[0] if traffic else None, "traffic_source": traffic[1] if traffic else None,
-            "kernel": ("k_rest_levels<bias|relax> (one solve pass over a rest list: the leading wide levels in a "
-                       "grid-barrier launch, the rest in one work-group; bytes per planar contact 248 + 68n)" if planar
+            "kernel": ("k_rest_walk<bias|relax> (one solve pass over the active rest list: k_rest_pre, the leading "
+                       "wide levels in a grid-barrier k_rest_levels, the narrow rest walked by k_rest_walk in one "
+                       "work-group; bytes per planar contact 248 + 68n)" if planar
                        else "k_pass<bias|relax> (one grid-barrier launch per solve pass)"),
             "launches": launches, "profiled_steps": args.profile_steps,
             "bytes_per_launch": algo / max(launches, 1), "ms_per_launch": ms / max(launches, 1),

@@ -481,9 +481,9 @@ int tfs_last_step_ms(tfs_ctx* ctx, float* solve_ms, float* sleep_ms);
  * time each refresh's narrowphase: through the callback, with the host clock, 3 the wait for the
  * contacts collided again to reach host memory and 4 the callback itself; on the device
  * (tfs_set_body_shapes), 4 the event time of the collision kernel and 3 nothing. Family 4's "bytes"
- * sum counts the pairs collided again. Family 5 is the biased and relax passes over the contacts outside the colours (the rest
- * lists, active and impact), event-timed per pass over one list, with bytes per contact of 248 + 68n
- * (planar) or 224 + 296n (3D). Family 6 is the reverse sweep's passes (tfs_adjoint_step), event-timed per
+ * sum counts the pairs collided again. Family 5 is the biased and relax passes over the contacts outside the colours in the
+ * active rest list (not the impact rest list's repeats), event-timed per pass, with bytes per contact of
+ * 248 + 68n (planar) or 224 + 296n (3D). Family 6 is the reverse sweep's passes (tfs_adjoint_step), event-timed per
  * pass, with 440 + 348n bytes per row (the solve's row, its record, the velocity and impulse adjoints).
  * Family 7 is the fused substeps (tfs_set_pass_shape), event-timed per launch, with the warm start's and
  * both solve passes' bytes.

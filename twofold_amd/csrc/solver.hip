@@ -3895,7 +3895,9 @@ int runRest(tfs_ctx* ctx, int mode, const uint32_t* list, uint32_t begin, uint32
     const RestPack rp = restPack(ctx);
     const bool viaPos = which == 1;
     size_t ev = 0;
-    const bool timed = ctx->profiling && (mode == kBias || mode == kRelax);
+    // Profile family 5 times the passes over the active rest list (the impact rest list's are a few
+    // contacts each and would only dilute the per-pass figure).
+    const bool timed = ctx->profiling && which == 0 && (mode == kBias || mode == kRelax);
     if (timed) {
         ev = ctx->pending.size() * 2;
         while (ctx->evPool.size() < ev + 2) {
