@@ -83,3 +83,20 @@ def test_unsupported_joints_are_refused():
         s.upload_joints(j)
     finally:
         s.close()
+
+
+def test_joints_naming_bodies_no_longer_resident_are_refused():
+    """Joints uploaded for more bodies than are resident after a later body upload: the step refuses
+    them (TFS_E_STATE) before any kernel reads a body out of range."""
+    scene = scenes.chain()
+    w = refworld.RefWorld(scene)
+    bodies, joints = w.bodies(), w.joints()
+    s = Solver(len(bodies), 64)
+    try:
+        s.upload(bodies, w.contacts())
+        s.upload_joints(joints)
+        s.upload(bodies[:4], w.contacts()[:0])
+        with pytest.raises(RuntimeError, match="upload them again"):
+            s.step(scene.dt, None)
+    finally:
+        s.close()

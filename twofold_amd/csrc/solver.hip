@@ -3326,6 +3326,7 @@ struct tfs_ctx {
     uint32_t* jointActive = nullptr;
     float4* worldMotion = nullptr;
     uint32_t nJoints = 0, capJ = 0;
+    uint32_t jointBodies = 0;             // 1 + the highest body index a resident joint names
     // the rest lists in levels (active rest, impact rest): scratch, and per list its order and level starts
     uint8_t* restTracked = nullptr;
     uint32_t *restDeg = nullptr, *restStart = nullptr, *restCursor = nullptr, *restEp = nullptr, *restRank = nullptr,
@@ -4265,6 +4266,10 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
     if (ctx->nShapes && ctx->nShapes != ctx->nb) {
         return fail(ctx, TFS_E_STATE, "%u body shapes for %u bodies", ctx->nShapes, ctx->nb);
     }
+    if (ctx->nJoints && ctx->jointBodies > ctx->nb) {
+        return fail(ctx, TFS_E_STATE, "the resident joints name body %u of %u: upload them again", ctx->jointBodies - 1,
+                    ctx->nb);
+    }
     if (ctx->tuning) { applyShape(ctx, ctx->tuneShapes[tuneCandidate(ctx)]); }
     const uint32_t nb = ctx->nb, nc = ctx->nc;
     const tfs_settings& s = ctx->settings;
@@ -5111,6 +5116,10 @@ int tfs_sleep(tfs_ctx* ctx, float dt) {
     if (!ctx) { return TFS_E_ARG; }
     if (!ctx->haveBodies || !ctx->haveContacts) { return fail(ctx, TFS_E_STATE, "upload bodies and contacts first"); }
     if (!(dt > 0.0f) || !(dt <= 3.4e38f)) { return 0; }
+    if (ctx->nJoints && ctx->jointBodies > ctx->nb) {
+        return fail(ctx, TFS_E_STATE, "the resident joints name body %u of %u: upload them again", ctx->jointBodies - 1,
+                    ctx->nb);
+    }
     if (hipSetDevice(ctx->device) != hipSuccess) { return fail(ctx, TFS_E_DEVICE, "hipSetDevice failed"); }
     HIP_TRY(hipMemsetAsync(ctx->cnt, 0, sizeof(DevCounts), ctx->stream));
     int it = 0, r;
@@ -5248,8 +5257,10 @@ int tfs_set_body_shapes(tfs_ctx* ctx, uint32_t n, const tfs_shape* shapes) {
 int tfs_upload_joints(tfs_ctx* ctx, const tfs_joint* joints, uint32_t n, const float* world_motion) {
     if (!ctx || (n && !joints)) { return TFS_E_ARG; }
     if (ctx->arena) { return fail(ctx, TFS_E_STATE, "a context that may not allocate takes no joints"); }
+    uint32_t named = 0;
     for (uint32_t k = 0; k < n; ++k) {
         const tfs_joint& j = joints[k];
+        named = std::max(named, std::max(j.body_b, j.body_a == TFS_WORLD_BODY ? 0u : j.body_a) + 1);
         if (j.body_b >= ctx->nb || (j.body_a != TFS_WORLD_BODY && (j.body_a >= ctx->nb || j.body_a == j.body_b))) {
             return fail(ctx, TFS_E_ARG, "joint %u: bodies %u and %u out of range for %u bodies", k, j.body_a, j.body_b, ctx->nb);
         }
@@ -5277,6 +5288,7 @@ int tfs_upload_joints(tfs_ctx* ctx, const tfs_joint* joints, uint32_t n, const f
     }
     HIP_TRY(hipMemcpy(ctx->worldMotion, wm, sizeof wm, hipMemcpyHostToDevice));
     ctx->nJoints = n;
+    ctx->jointBodies = named;
     return 0;
 }
 
