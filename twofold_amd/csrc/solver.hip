@@ -440,16 +440,30 @@ __device__ __forceinline__ void storeWord(unsigned long long* p, unsigned long l
 
 // Blocks take chunks of the active list in order from a ticket, so every chunk a contact can wait
 // on belongs to a block that is already running; the first unfinished contact can always proceed.
-__global__ void k_color(const uint32_t* active, const DevCounts* cntIn, tfs_contact* contacts, const uint8_t* moving,
-                        const uint32_t* rank, unsigned long long* word, uint8_t* colors, uint32_t* ticket,
-                        DevCounts* cnt) {
+//
+// A contact waits on the contact before it on each of its moving bodies (the body's endpoint of rank
+// r - 1, from the position-sorted endpoints). When that contact is in the same chunk, the wait is in
+// LDS: each finished contact leaves its bodies' words in its slot and then marks the slot done, at
+// work-group scope, a round trip of about a hundred cycles where a word in global memory, read at agent
+// scope across the XCDs, costs a microsecond or more. Most links of the pile's dependency chain are
+// within a chunk (contacts come in key order, a body's contacts as the first body together).
+constexpr int kColorThreads = 1024;
+
+__global__ void __launch_bounds__(kColorThreads)
+    k_color(const uint32_t* active, const DevCounts* cntIn, tfs_contact* contacts, const uint8_t* moving,
+            const uint32_t* rank, const uint32_t* epStart, const uint32_t* ep, unsigned long long* word,
+            uint8_t* colors, uint32_t* ticket, DevCounts* cnt) {
     __shared__ uint32_t chunk;
+    __shared__ uint32_t slotDone[kColorThreads];
+    __shared__ unsigned long long slotWord[kColorThreads][2];
     const uint32_t n = cntIn->nActive;
     for (;;) {
         if (threadIdx.x == 0) { chunk = atomicAdd(ticket, 1u); }
+        slotDone[threadIdx.x] = 0u;
         __syncthreads();
-        const uint32_t k = chunk * blockDim.x + threadIdx.x;
-        if (chunk * blockDim.x >= n) { break; }
+        const uint32_t base = chunk * blockDim.x;
+        const uint32_t k = base + threadIdx.x;
+        if (base >= n) { break; }
         if (k < n) {
             const uint32_t ci = active[k];
             const uint32_t a = contacts[ci].body_a, b = contacts[ci].body_b;
@@ -457,14 +471,55 @@ __global__ void k_color(const uint32_t* active, const DevCounts* cntIn, tfs_cont
             if (colourable(contacts[ci])) {
                 const bool ma = moving[a], mb = moving[b];
                 const uint32_t ra = rank[k << 1], rb = rank[(k << 1) | 1u];
-                // The work happens inside the loop: a lane that left the loop would wait for the
-                // rest of its wave, which may be waiting for it.
-                bool done = false;
+                // the predecessor's endpoint on each body, when it is in this chunk: its slot and side
+                uint32_t la = 0xffffffffu, lb = 0xffffffffu;
+                if (ma && ra > 0) {
+                    const uint32_t e = ep[epStart[a] + ra - 1];
+                    if ((e >> 1) >= base) { la = (((e >> 1) - base) << 1) | (e & 1u); }
+                }
+                if (mb && rb > 0) {
+                    const uint32_t e = ep[epStart[b] + rb - 1];
+                    if ((e >> 1) >= base) { lb = (((e >> 1) - base) << 1) | (e & 1u); }
+                }
+                // First the links from earlier chunks, in global memory: a wave polls them all before
+                // it looks at its chunk's own links, so that those then go at LDS speed (a wave's loop
+                // turn lasts as long as its slowest lane's load).
+                unsigned long long wa = 0ull, wb = 0ull;
+                bool readyA = !ma || la != 0xffffffffu, readyB = !mb || lb != 0xffffffffu;
                 uint32_t spins = 0;
+                while (!(readyA && readyB)) {
+                    if (!readyA) {
+                        wa = loadWord(&word[a]);
+                        readyA = (uint32_t)(wa >> 32) == ra;
+                    }
+                    if (!readyB) {
+                        wb = loadWord(&word[b]);
+                        readyB = (uint32_t)(wb >> 32) == rb;
+                    }
+                    if (readyA && readyB) { break; }
+                    if (++spins > kSpinLimit) {
+                        atomicOr(&cnt->error, 1u);
+                        readyA = readyB = true;
+                        la = lb = 0xffffffffu;
+                    } else {
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                // Then the links within the chunk, through the slots. The work happens inside the
+                // loop: a lane that left the loop would wait for the rest of its wave, which may be
+                // waiting for it.
+                bool done = false;
                 while (!done) {
-                    const unsigned long long wa = ma ? loadWord(&word[a]) : 0ull;
-                    const unsigned long long wb = mb ? loadWord(&word[b]) : 0ull;
-                    if ((!ma || (uint32_t)(wa >> 32) == ra) && (!mb || (uint32_t)(wb >> 32) == rb)) {
+                    bool okA = true, okB = true;
+                    if (ma && la != 0xffffffffu) {
+                        okA = __hip_atomic_load(&slotDone[la >> 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u;
+                        if (okA) { wa = slotWord[la >> 1][la & 1u]; }
+                    }
+                    if (mb && lb != 0xffffffffu) {
+                        okB = __hip_atomic_load(&slotDone[lb >> 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u;
+                        if (okB) { wb = slotWord[lb >> 1][lb & 1u]; }
+                    }
+                    if (okA && okB) {
                         const uint32_t taken = (uint32_t)wa | (uint32_t)wb;
                         uint32_t add = 0;
                         if (taken != 0xffffffffu) {
@@ -472,11 +527,17 @@ __global__ void k_color(const uint32_t* active, const DevCounts* cntIn, tfs_cont
                             col = (uint8_t)bit;
                             add = 1u << bit;
                         }
-                        if (ma) { storeWord(&word[a], ((unsigned long long)(ra + 1) << 32) | ((uint32_t)wa | add)); }
-                        if (mb) { storeWord(&word[b], ((unsigned long long)(rb + 1) << 32) | ((uint32_t)wb | add)); }
+                        const unsigned long long na = ((unsigned long long)(ra + 1) << 32) | ((uint32_t)wa | add);
+                        const unsigned long long nb = ((unsigned long long)(rb + 1) << 32) | ((uint32_t)wb | add);
+                        if (ma) { storeWord(&word[a], na); }
+                        if (mb) { storeWord(&word[b], nb); }
+                        slotWord[threadIdx.x][0] = na;
+                        slotWord[threadIdx.x][1] = nb;
+                        __hip_atomic_store(&slotDone[threadIdx.x], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
                         done = true;
                     } else if (++spins > kSpinLimit) {
                         atomicOr(&cnt->error, 1u);
+                        __hip_atomic_store(&slotDone[threadIdx.x], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
                         done = true;
                     } else {
                         __builtin_amdgcn_s_sleep(1);
@@ -4110,8 +4171,8 @@ int beginStep(tfs_ctx* ctx, float dt, DevCounts& hc) {
     LAUNCH(k_ep_fill, blocksFor(nc), kThreads, ctx->active, ctx->cnt, ctx->contacts, ctx->moving, ctx->epStart,
            ctx->cursor, ctx->ep);
     LAUNCH(k_ep_rank, blocksFor(nb), kThreads, nb, ctx->deg, ctx->epStart, ctx->ep, ctx->rank);
-    LAUNCH(k_color, blocksFor(nc, kThreads, 2048), kThreads, ctx->active, ctx->cnt, ctx->contacts, ctx->moving,
-           ctx->rank, ctx->word, ctx->colors, &ctx->cnt->ticket, ctx->cnt);
+    LAUNCH(k_color, blocksFor(nc, kColorThreads, 2048), kColorThreads, ctx->active, ctx->cnt, ctx->contacts, ctx->moving,
+           ctx->rank, ctx->epStart, ctx->ep, ctx->word, ctx->colors, &ctx->cnt->ticket, ctx->cnt);
 
     // row order and packing
     if ((r = rowOrder(ctx, ctx->active, &ctx->cnt->nActive, nc, ctx->rowsList, ctx->cnt->colorStart,
