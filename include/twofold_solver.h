@@ -395,7 +395,9 @@ int tfs_set_body_shapes(tfs_ctx* ctx, uint32_t n, const tfs_shape* shapes);
  * pass, broken past their limits, settled, and the bodies' inverse masses made again; the sleep pass
  * joins their bodies' islands (1210-1223). The joint and world-body arrays stay resident (upload once,
  * download when needed). TFS_E_ARG for a body index out of range, a limited hinge or a swing-twist
- * joint. n = 0 removes them. Replaces World::Impl::joints and worldBody for the solve block. */
+ * joint; tfs_step fails with TFS_E_STATE while joints are resident beside several worlds
+ * (tfs_set_worlds: each world has its own world body) or name bodies no longer resident. n = 0 removes
+ * them. Replaces World::Impl::joints and worldBody for the solve block. */
 int tfs_upload_joints(tfs_ctx* ctx, const tfs_joint* joints, uint32_t n, const float* world_motion);
 int tfs_download_joints(tfs_ctx* ctx, tfs_joint* out, uint32_t n, float* world_motion);
 

@@ -100,3 +100,20 @@ def test_joints_naming_bodies_no_longer_resident_are_refused():
             s.step(scene.dt, None)
     finally:
         s.close()
+
+
+def test_joints_beside_several_worlds_are_refused():
+    """Each of the reference's worlds has a world body of its own; joints beside several worlds side by
+    side (tfs_set_worlds) are refused at the step (TFS_E_STATE)."""
+    scene = scenes.chain()
+    w = refworld.RefWorld(scene)
+    bodies, joints = w.bodies(), w.joints()
+    both = np.concatenate([bodies, bodies])
+    s = Solver(len(both), 64)
+    try:
+        s.upload_worlds(both, np.array([0, len(bodies), len(both)], np.uint32), w.contacts()[:0])
+        s.upload_joints(joints)
+        with pytest.raises(RuntimeError, match="one world at a time"):
+            s.step(scene.dt, None)
+    finally:
+        s.close()

@@ -4331,6 +4331,10 @@ int step(tfs_ctx* ctx, float dt, tfs_refresh_fn cb, void* user) {
         return fail(ctx, TFS_E_STATE, "the resident joints name body %u of %u: upload them again", ctx->jointBodies - 1,
                     ctx->nb);
     }
+    if (ctx->nJoints && ctx->nWorlds > 1) {
+        // each of the reference's worlds has a world body of its own; the joints here share one
+        return fail(ctx, TFS_E_STATE, "joints with %u worlds side by side: step one world at a time", ctx->nWorlds);
+    }
     if (ctx->tuning) { applyShape(ctx, ctx->tuneShapes[tuneCandidate(ctx)]); }
     const uint32_t nb = ctx->nb, nc = ctx->nc;
     const tfs_settings& s = ctx->settings;
