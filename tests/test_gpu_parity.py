@@ -120,6 +120,36 @@ def test_gpu_matches_reference_full_pile():
         s.close()
 
 
+@needs_ref
+def test_gpu_matches_reference_full_pile_drop():
+    """The 32,768-body pile's drop at full size, the reference stepping from the start: every 40th of
+    its first 240 steps (while the bodies fall, pile up and settle, with hundreds of fast pairs
+    refreshed mid-step) is replayed on the GPU from the inputs the reference's solve block saw (its
+    own narrowphase and refresh manifolds), bit for bit."""
+    scene = scenes.pile(side=32)
+    w = refworld.RefWorld(scene)
+    s = Solver(len(scene.bodies), 4 * len(scene.bodies))
+    checked = refreshed = 0
+    try:
+        for k in range(240):
+            if k % 40 != 39:
+                w.step(record=False)
+                continue
+            rec = w.step()
+            assert rec is not None
+            case = parity.record_dict(rec)
+            b, c, lists = gpu_step(s, w.settings, case)
+            bad = parity.compare(case, b, c, lists)
+            assert bad == [], f"full pile drop step {k}: {bad[:5]}"
+            checked += 1
+            refreshed += sum(len(r["again"]) for r in case["refreshes"])
+            print(f"full pile drop: step {k}, {len(case['pre_contacts'])} contacts, "
+                  f"{len(case['refreshes'])} refreshes, bit-exact", flush=True)
+    finally:
+        s.close()
+    assert checked == 6 and refreshed > 0
+
+
 def same_bits(a, b):
     """Fields of two structured arrays that differ in any bit."""
     out = []
