@@ -2109,34 +2109,84 @@ __global__ void k_rest_narrow(const uint32_t* levelStart, const uint32_t* levels
 
 // The levels, as the colouring's dataflow: a contact proceeds once its tracked bodies have seen all
 // their earlier contacts; each body's word is (contacts seen << 32) | its highest level so far.
-__global__ void k_rest_level(const uint32_t* list, uint32_t n, const tfs_contact* contacts, RestScratch z,
-                             uint32_t* ticket, uint32_t* levels, DevCounts* cnt) {
+// As k_color: 1,024-contact chunks by ticket, the links from earlier chunks through the bodies' global
+// words (a wave waits for all of its lanes' before the next part), the links within the chunk through
+// LDS slots, each finished contact leaving its bodies' words in its slot.
+__global__ void __launch_bounds__(kColorThreads)
+    k_rest_level(const uint32_t* list, uint32_t n, const tfs_contact* contacts, RestScratch z, uint32_t* ticket,
+                 uint32_t* levels, DevCounts* cnt) {
     __shared__ uint32_t chunk;
+    __shared__ uint32_t slotDone[kColorThreads];
+    __shared__ unsigned long long slotWord[kColorThreads][2];
     for (;;) {
         if (threadIdx.x == 0) { chunk = atomicAdd(ticket, 1u); }
+        slotDone[threadIdx.x] = 0u;
         __syncthreads();
-        const uint32_t k = chunk * blockDim.x + threadIdx.x;
-        if (chunk * blockDim.x >= n) { break; }
+        const uint32_t base = chunk * blockDim.x;
+        const uint32_t k = base + threadIdx.x;
+        if (base >= n) { break; }
         if (k < n) {
             const tfs_contact& c = contacts[list[k]];
             const uint32_t a = c.body_a, b = c.body_b;
             const bool ta = z.tracked[a], tb = z.tracked[b];
             const uint32_t ra = ta ? z.rank[k << 1] : 0u, rb = tb ? z.rank[(k << 1) | 1u] : 0u;
-            bool done = false;
+            uint32_t la = 0xffffffffu, lb = 0xffffffffu;
+            if (ta && ra > 0) {
+                const uint32_t e = z.ep[z.start[a] + ra - 1];
+                if ((e >> 1) >= base) { la = (((e >> 1) - base) << 1) | (e & 1u); }
+            }
+            if (tb && rb > 0) {
+                const uint32_t e = z.ep[z.start[b] + rb - 1];
+                if ((e >> 1) >= base) { lb = (((e >> 1) - base) << 1) | (e & 1u); }
+            }
+            unsigned long long wa = 0ull, wb = 0ull;
+            bool readyA = !ta || la != 0xffffffffu, readyB = !tb || lb != 0xffffffffu;
             uint32_t spins = 0;
+            while (!(readyA && readyB)) {
+                if (!readyA) {
+                    wa = loadWord(&z.word[a]);
+                    readyA = (uint32_t)(wa >> 32) == ra;
+                }
+                if (!readyB) {
+                    wb = loadWord(&z.word[b]);
+                    readyB = (uint32_t)(wb >> 32) == rb;
+                }
+                if (readyA && readyB) { break; }
+                if (++spins > kSpinLimit) {
+                    atomicOr(&cnt->error, 8u);
+                    readyA = readyB = true;
+                    la = lb = 0xffffffffu;
+                } else {
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            bool done = false;
             while (!done) {
-                const unsigned long long wa = ta ? loadWord(&z.word[a]) : 0ull;
-                const unsigned long long wb = tb ? loadWord(&z.word[b]) : 0ull;
-                if ((!ta || (uint32_t)(wa >> 32) == ra) && (!tb || (uint32_t)(wb >> 32) == rb)) {
+                bool okA = true, okB = true;
+                if (ta && la != 0xffffffffu) {
+                    okA = __hip_atomic_load(&slotDone[la >> 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u;
+                    if (okA) { wa = slotWord[la >> 1][la & 1u]; }
+                }
+                if (tb && lb != 0xffffffffu) {
+                    okB = __hip_atomic_load(&slotDone[lb >> 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u;
+                    if (okB) { wb = slotWord[lb >> 1][lb & 1u]; }
+                }
+                if (okA && okB) {
                     const uint32_t lv = 1u + max((uint32_t)wa, (uint32_t)wb);
-                    if (ta) { storeWord(&z.word[a], ((unsigned long long)(ra + 1) << 32) | lv); }
-                    if (tb) { storeWord(&z.word[b], ((unsigned long long)(rb + 1) << 32) | lv); }
+                    const unsigned long long na = ((unsigned long long)(ra + 1) << 32) | lv;
+                    const unsigned long long nb = ((unsigned long long)(rb + 1) << 32) | lv;
+                    if (ta) { storeWord(&z.word[a], na); }
+                    if (tb) { storeWord(&z.word[b], nb); }
+                    slotWord[threadIdx.x][0] = na;
+                    slotWord[threadIdx.x][1] = nb;
+                    __hip_atomic_store(&slotDone[threadIdx.x], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
                     z.level[k] = lv - 1u;
                     atomicAdd(&z.count[lv - 1u], 1u);
                     atomicMax(levels, lv);
                     done = true;
                 } else if (++spins > kSpinLimit) {
                     atomicOr(&cnt->error, 8u);
+                    __hip_atomic_store(&slotDone[threadIdx.x], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
                     z.level[k] = 0u;
                     done = true;
                 } else {
@@ -3896,7 +3946,7 @@ int buildRestLevels(tfs_ctx* ctx, int which, const uint32_t* list, uint32_t n) {
     if ((r = scan(ctx, z.deg, nullptr, nb, z.start, nullptr, &ctx->cnt->scanTotal))) { return r; }
     LAUNCH(k_rest_ep_fill, blocksFor(n), kThreads, list, n, ctx->contacts, z);
     LAUNCH(k_ep_rank, blocksFor(nb), kThreads, nb, z.deg, z.start, z.ep, z.rank);
-    LAUNCH(k_rest_level, blocksFor(n, kThreads, 2048), kThreads, list, n, ctx->contacts, z, &ctx->cnt->restTicket[which],
+    LAUNCH(k_rest_level, blocksFor(n, kColorThreads, 2048), kColorThreads, list, n, ctx->contacts, z, &ctx->cnt->restTicket[which],
            &ctx->cnt->restLevels[which], ctx->cnt);
     // level starts: the exclusive prefix of the level sizes, and n after the last level
     if ((r = scan(ctx, z.count, nullptr, n, ctx->restLevelStart[which], nullptr, ctx->restLevelStart[which] + n))) {
