@@ -720,19 +720,6 @@ __global__ void __launch_bounds__(kThreads) k_pack(const uint32_t* __restrict__ 
     for (uint32_t r = gid(); r < n; r += gstride()) { packRow(r, list[r], contacts, moving, mass, rows, rowOf); }
 }
 
-// The rows of the contacts a refresh collided again, once their new points are in: the other rows
-// were packed while the host collided them.
-__global__ void __launch_bounds__(kThreads) k_pack_again(const uint32_t* __restrict__ again, uint32_t n,
-                                                         const uint8_t* __restrict__ colors,
-                                                         const tfs_contact* __restrict__ contacts,
-                                                         const uint8_t* __restrict__ moving, const float4* __restrict__ mass,
-                                                         Rows rows, uint32_t* __restrict__ rowOf) {
-    for (uint32_t k = gid(); k < n; k += gstride()) {
-        const uint32_t ci = again[k];
-        if (colors[ci] < TFS_COLORS) { packRow(rowOf[ci], ci, contacts, moving, mass, rows, rowOf); }
-    }
-}
-
 // The impact rows' headers: their active row's header with the active row index in place of the
 // contact index.
 __global__ void k_impact_rows(const uint32_t* list, uint32_t n, const uint32_t* rowOf, const uint4* hdr, uint4* impactHdr) {
@@ -2982,15 +2969,16 @@ __global__ void __launch_bounds__(1024) k_refresh_order(const uint32_t* fast, co
 // The solver side of each refreshed fast contact (src/world.cpp:1108-1138).
 // A contact collided again is no longer reusable and its points' surface points are cleared
 // (src/world.cpp:1110, 1117: the points start over), in its reuse data `ext`.
-__global__ void __launch_bounds__(kThreads) k_refresh_apply(const uint32_t* again, uint32_t n, const tfs_contact* fresh, tfs_contact* contacts,
-                                const tfs_body* bodies, const float4* motion, const float4* mass, float matchSq,
-                                tfs_contact_ext* ext) {
-    for (uint32_t k = gid(); k < n; k += gstride()) {
+// Each refreshed contact keeps its row, whose points and axes are then packed again (as k_pack_again,
+// by the same thread: the row reads only its own contact).
+__device__ __forceinline__ void refreshOne(uint32_t k, const uint32_t* again, const tfs_contact* fresh, tfs_contact* contacts,
+                                           const tfs_body* bodies, const float4* motion, const float4* mass, float matchSq,
+                                           tfs_contact_ext* ext) {
         tfs_contact& c = contacts[again[k]];
         const tfs_contact& f = fresh[k];
         if (f.count <= 0) {
             c.count = 0;
-            continue;
+            return;
         }
         {
             tfs_contact_ext& e = ext[again[k]];
@@ -3034,6 +3022,16 @@ __global__ void __launch_bounds__(kThreads) k_refresh_apply(const uint32_t* agai
         }
         carryImpulses(old, x, matchSq);
         c = x;
+}
+
+__global__ void __launch_bounds__(kThreads) k_refresh_apply(const uint32_t* again, uint32_t n, const tfs_contact* fresh, tfs_contact* contacts,
+                                const tfs_body* bodies, const float4* motion, const float4* mass, float matchSq,
+                                tfs_contact_ext* ext, const uint8_t* colors, const uint8_t* moving, Rows rows,
+                                uint32_t* rowOf) {
+    for (uint32_t k = gid(); k < n; k += gstride()) {
+        refreshOne(k, again, fresh, contacts, bodies, motion, mass, matchSq, ext);
+        const uint32_t ci = again[k];
+        if (colors[ci] < TFS_COLORS) { packRow(rowOf[ci], ci, contacts, moving, mass, rows, rowOf); }
     }
 }
 
@@ -4146,7 +4144,7 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
            ctx->pinAgain, ctx->pinContacts, ctx->pinBodies);
     HIP_TRY(hipEventRecord(ctx->ev[3], ctx->stream));
     // Every row but those of the contacts collided again is final now (rebased, new inverse masses):
-    // they are packed while the host collides, and the others again after (k_pack_again), as
+    // they are packed while the host collides, and the others again after (by k_refresh_apply), as
     // prepareLanes packs all rows after a refresh.
     LAUNCH(k_pack, blocksFor(laned), kThreads, ctx->rowsList, laned, ctx->contacts, ctx->moving, ctx->mass, ctx->rows,
            ctx->rowOf);
@@ -4172,11 +4170,10 @@ int refresh(tfs_ctx* ctx, int substep, bool half, uint32_t nAgain, float remaini
 // After the narrowphase of a refresh: the new manifolds into the contacts, their rows packed again.
 int refreshApply(tfs_ctx* ctx, const uint32_t* again, uint32_t nAgain, uint32_t laned, uint32_t impactLaned, uint32_t nRest) {
     const float matchSq = ctx->settings.speculative * ctx->settings.speculative;
+    // A refreshed contact keeps its row; its points and axes are packed again in the same launch.
     LAUNCH(k_refresh_apply, blocksFor(nAgain), kThreads, again, nAgain, ctx->fresh,
-           ctx->contacts, ctx->bodies, ctx->motion, ctx->mass, matchSq, ctx->ext);
-    // A refreshed contact keeps its row; its points and axes are packed again.
-    LAUNCH(k_pack_again, blocksFor(nAgain), kThreads, again, nAgain, ctx->colors, ctx->contacts, ctx->moving, ctx->mass,
-           ctx->rows, ctx->rowOf);
+           ctx->contacts, ctx->bodies, ctx->motion, ctx->mass, matchSq, ctx->ext, ctx->colors, ctx->moving, ctx->rows,
+           ctx->rowOf);
     if (impactLaned > 0) {
         LAUNCH(k_impact_rows, blocksFor(impactLaned), kThreads, ctx->impactRowsList, impactLaned, ctx->rowOf,
                ctx->rows.hdr, ctx->impactHdr);
